@@ -1,0 +1,174 @@
+/*
+ * pmoracle.h -- FP64 CPU restatement of ProtoMol's force-evaluation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in protomol_amd/.  It is linked or loaded only by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py; the product
+ * path never calls it.
+ *
+ * Every routine follows the loop order and arithmetic of the reference file
+ * cited beside it (paths relative to the ProtoMol 2.1 tree, framework/).
+ * Units: Angstrom, kcal/mol, amu; charges pre-scaled by sqrt(332.0636)
+ * (frontend/buildTopology.cpp:148).
+ *
+ * Pinning (what checks each function, see tests/test_oracle*.py):
+ *   orc_cells / orc_delta_list / orc_cutoff   -> compiled reference
+ *       (oracle/_ref), committed golden fixtures, brute-force O(N^2) pair sets
+ *   orc_pme (reciprocal, correction, self, charged) -> compiled reference,
+ *       golden fixtures, direct Ewald summation (numpy), naive DFT for the FFT
+ *   orc_pme_real                               -> compiled reference
+ *   orc_bspline                                -> partition of unity, closed
+ *       form of the cubic B-spline, compiled reference (through orc_pme)
+ *   orc_leapfrog / orc_langevin_half_kick      -> closed form on one particle
+ *       (harmonic oscillator) and the reference formulas
+ *   orc_hessian_pairs                          -> central finite differences
+ *       of orc_cutoff forces; compiled reference Hessian (Hessian.cpp)
+ *   orc_rem_delta                              -> closed form
+ */
+#ifndef PMORACLE_H
+#define PMORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ExclusionClass, topology/ExclusionTable.h:20-25 */
+enum { ORC_EXCL_NONE = 0, ORC_EXCL_MODIFIED = 1, ORC_EXCL_FULL = 2 };
+
+/* pair potentials (forces/LennardJonesForce.h, CoulombForce.h,
+ * CoulombEwaldRealForce.h) */
+enum { ORC_POT_LJ = 0, ORC_POT_COULOMB = 1, ORC_POT_EWALDREAL = 2 };
+
+/* switching functions (topology/CutoffSwitchingFunction.h,
+ * C1SwitchingFunction.h, C2SwitchingFunction.h) */
+enum { ORC_SWF_CUTOFF = 0, ORC_SWF_C1 = 1, ORC_SWF_C2 = 2 };
+
+/* energies[] layout: ScalarStructure indices COULOMB, LENNARDJONES then the
+ * nine atomic virial entries (base/ScalarStructure.h:25-45) */
+enum {
+  ORC_E_COULOMB = 0, ORC_E_LJ = 1,
+  ORC_E_VXX = 2, ORC_E_VXY, ORC_E_VXZ, ORC_E_VYX, ORC_E_VYY, ORC_E_VYZ,
+  ORC_E_VZX, ORC_E_VZY, ORC_E_VZZ,
+  ORC_E_COUNT
+};
+
+/* one OneAtomPair term: potential + switching function + parameters */
+typedef struct {
+  int pot;          /* ORC_POT_* */
+  int swf;          /* ORC_SWF_* */
+  double alpha;     /* Ewald splitting (EwaldReal only) */
+  double switchon;  /* C2 only */
+  double cutoff;    /* switching-function cutoff */
+} orc_term;
+
+/* Cell binning (topology/buildCellLists.cpp:11-38).  Outputs:
+ *   dims[3]   cell-grid dimensions
+ *   realcell[3] real cell size
+ *   cell[n]   linear cell index (x*ny+y)*nz+z of every atom
+ *   head[ncells] first atom of every cell (-1 if empty), next[n] list link.
+ * Returns the number of cells, or -1 if head is too small (ncap). */
+int orc_cells(int n, const double* pos, const double* box, const double* origin,
+              double cellsize, int* dims, double* realcell, int* cell,
+              int* head, int* next, int ncap);
+
+/* Deduplicated, sorted delta list of the periodic enumerator
+ * (topology/CellListEnumerator_periodicBoundaries.h:37-108).  out holds
+ * 3*cap ints.  Returns the count (or -count if cap was too small). */
+int orc_delta_list(const double* box, const double* origin, double cellsize,
+                   double cutoff, int* out, int cap);
+
+/* Nonbonded cutoff system force (forces/NonbondedCutoffSystemForce.h:64-74,
+ * NonbondedCutoffForce.h:86-109, OneAtomPair.h:127-188,
+ * OneAtomPairTwo.h:131-199).  nterms is 1 (OneAtomPair) or 2
+ * (OneAtomPairTwo).  forces (3n) and energies (ORC_E_COUNT) are ACCUMULATED
+ * into.  Optional per-cell statistics (ncells entries each, may be NULL):
+ *   cand[c]  candidate pairs visited with c as the first cell of the pair
+ *   inrc[c]  of those, pairs with r^2 <= rc^2 that are not fully excluded
+ * Returns 0 on success, negative on error. */
+int orc_cutoff(int n, const double* pos, const double* q, const int* type,
+               int ntypes, const double* ljA, const double* ljB,
+               const double* ljA14, const double* ljB14, const int* molecule,
+               int nexcl, const int* ex_a, const int* ex_b, const int* ex_cls,
+               double csf, const double* box, const double* origin,
+               double cellsize, int nterms, const orc_term* terms,
+               double algo_cutoff, int want_virial, double* forces,
+               double* energies, long long* cand, long long* inrc);
+
+/* B-spline weights of order p at fractional offset w (base/BSpline.cpp:64-117). */
+void orc_bspline(int order, double w, double* theta, double* dtheta);
+
+/* Ewald splitting parameter by bisection
+ * (forces/NonbondedPMEwaldSystemForce.h:564-586). */
+double orc_pme_alpha(double rc, double accuracy);
+
+/* PME reciprocal + correction + self + charged-system terms
+ * (forces/NonbondedPMEwaldSystemForce.h:271-349,499-648,879-981;
+ *  base/Grid.h:138-228,276-389,557-665).  flags: bit0 real, bit1 reciprocal,
+ * bit2 correction.  alpha<=0 selects the bisection.  Accumulates into
+ * forces/energies.  parts (may be NULL) receives
+ * {real, reciprocal, intra(correction), pointSelf, chargedSystem, alpha}.
+ * qgrid (may be NULL, nx*ny*nz doubles) receives the spread charge grid. */
+int orc_pme(int n, const double* pos, const double* q, const int* molecule,
+            int nexcl, const int* ex_a, const int* ex_b, const int* ex_cls,
+            double csf, const double* box, const double* origin,
+            double cellsize, int nx, int ny, int nz, int order, double rc,
+            double accuracy, double alpha, int flags, int want_virial,
+            double* forces, double* energies, double* parts, double* qgrid);
+
+/* In-place unnormalised 3D complex FFT, data[nx][ny][nz] interleaved re,im.
+ * sign -1: exp(-i...) ("backward" in base/FFTComplex.cpp:79-82), +1: exp(+i).
+ * Factors 2,3,4,5 natively, other primes by direct DFT. */
+void orc_fft3d(double* data, int nx, int ny, int nz, int sign);
+
+/* Leapfrog kick-drift over nsteps with the force callback replaced by a
+ * caller-provided force array sequence is not possible in C ABI; instead
+ * orc_leapfrog_kickdrift does one velocity+position update
+ * (integrators/LeapfrogIntegrator.cpp:46-104): v += F*h*fac/m; x += v*h with
+ * h = dt*INV_TIMEFACTOR; fac = 0.5 for the half kick, 1 for the full kick. */
+void orc_leapfrog_kickdrift(int n, double dt, double fac, const double* mass,
+                            const double* f, double* v, double* x);
+/* Half kick only (LeapfrogIntegrator::doHalfKick via STSIntegrator). */
+void orc_half_kick(int n, double dt, const double* mass, const double* f,
+                   double* v);
+
+/* Langevin BBK-style half kick (integrators/LangevinLeapfrogIntegrator.cpp:58-79)
+ * with caller-provided Gaussian numbers xi[3n]; gamma in 1/ps. */
+void orc_langevin_half_kick(int n, double dt, double temperature, double gamma,
+                            const double* mass, const double* f,
+                            const double* xi, double* v);
+
+/* Kinetic energy (topology/topologyutilities.cpp:182-191). */
+double orc_kinetic_energy(int n, const double* mass, const double* v);
+
+/* Dense pair Hessian-vector product of the cutoff force set, all pairs under
+ * the minimal image that are not fully excluded
+ * (integrators/Hessian.cpp:456-624, forces/ReducedHessLennardJones.cpp:7-50,
+ * ReducedHessCoulomb.cpp:8-42, C1/C2SwitchingFunction.cpp hessian()).
+ * hv[3n] = H v. */
+int orc_hessian_vec(int n, const double* pos, const double* q, const int* type,
+                    int ntypes, const double* ljA, const double* ljB,
+                    const double* ljA14, const double* ljB14,
+                    const int* molecule, int nexcl, const int* ex_a,
+                    const int* ex_b, const int* ex_cls, double csf,
+                    const double* box, const double* origin, int nterms,
+                    const orc_term* terms, const double* v, double* hv);
+
+/* Replica-exchange acceptance exponent (applications/rem-app/rem.cpp:941-946). */
+double orc_rem_delta(double ti, double tj, double ei, double ej);
+
+/* OpenMP-parallel nonbonded cutoff force over cell packages with per-thread
+ * force buffers (the multi-core CPU baseline; same pairs as orc_cutoff, the
+ * summation order differs).  Returns the thread count used. */
+int orc_cutoff_omp(int nthreads, int n, const double* pos, const double* q,
+                   const int* type, int ntypes, const double* ljA,
+                   const double* ljB, const double* ljA14, const double* ljB14,
+                   const int* molecule, int nexcl, const int* ex_a,
+                   const int* ex_b, const int* ex_cls, double csf,
+                   const double* box, const double* origin, double cellsize,
+                   int nterms, const orc_term* terms, double algo_cutoff,
+                   double* forces, double* energies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,222 @@
+"""ctypes bindings for the parity checkers (TEST INFRASTRUCTURE ONLY).
+
+  Oracle("port")      -> oracle/libpmoracle.so, the C restatement
+  Oracle("reference") -> oracle/_ref/libpmref.so, the ProtoMol sources compiled
+                         here (absent where the reference tree is absent)
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use
+this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PORT_LIB = os.path.join(HERE, "libpmoracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libpmref.so")
+
+POT_LJ, POT_COULOMB, POT_EWALDREAL = 0, 1, 2
+SWF_CUTOFF, SWF_C1, SWF_C2 = 0, 1, 2
+E_COULOMB, E_LJ = 0, 1
+E_COUNT = 11
+
+
+class Term(C.Structure):
+    _fields_ = [("pot", C.c_int), ("swf", C.c_int), ("alpha", C.c_double),
+                ("switchon", C.c_double), ("cutoff", C.c_double)]
+
+
+def term(pot, swf, cutoff, switchon=0.0, alpha=0.0):
+    return Term(pot, swf, alpha, switchon, cutoff)
+
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_L = C.POINTER(C.c_longlong)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_I)
+
+
+def available(kind: str) -> bool:
+    return os.path.exists(PORT_LIB if kind == "port" else REF_LIB)
+
+
+class Oracle:
+    def __init__(self, kind: str = "port"):
+        self.kind = kind
+        path = PORT_LIB if kind == "port" else REF_LIB
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle{' ref' if kind != 'port' else ''})")
+        self.lib = C.CDLL(path)
+        p = "orc_" if kind == "port" else "ref_"
+        self._cells = getattr(self.lib, p + "cells")
+        self._cutoff = getattr(self.lib, p + "cutoff")
+        self._pme = getattr(self.lib, p + "pme")
+        self._bspline = getattr(self.lib, p + "bspline")
+        self._hess = getattr(self.lib, p + "hessian_vec")
+        self._cutoff_omp = getattr(self.lib, "orc_cutoff_omp", None) if kind == "port" else None
+        self._delta = getattr(self.lib, "orc_delta_list", None) if kind == "port" else None
+        self._fft = getattr(self.lib, "orc_fft3d", None) if kind == "port" else None
+        self._alpha = getattr(self.lib, "orc_pme_alpha", None) if kind == "port" else None
+        if self._alpha is not None:
+            self._alpha.restype = C.c_double
+        rem = getattr(self.lib, "orc_rem_delta", None) if kind == "port" else None
+        if rem is not None:
+            rem.restype = C.c_double
+        self._rem = rem
+
+    # ---------------------------------------------------------------- cells
+    def cells(self, sys, cellsize):
+        pos, pp = _d(sys.pos)
+        box, bp = _d(sys.box)
+        org, op = _d(sys.origin)
+        dims = (C.c_int * 3)()
+        real = (C.c_double * 3)()
+        nc = self._cells(sys.n, pp, bp, op, C.c_double(cellsize), dims, real, None, None, None, 0)
+        nc = dims[0] * dims[1] * dims[2]
+        cell = np.zeros(sys.n, np.int32)
+        head = np.zeros(nc, np.int32)
+        nxt = np.zeros(sys.n, np.int32)
+        r = self._cells(sys.n, pp, bp, op, C.c_double(cellsize), dims, real,
+                        cell.ctypes.data_as(_I), head.ctypes.data_as(_I), nxt.ctypes.data_as(_I), nc)
+        assert r == nc, r
+        return dict(dims=tuple(dims), realcell=tuple(real), cell=cell, head=head, next=nxt)
+
+    def delta_list(self, sys, cellsize, cutoff):
+        box, bp = _d(sys.box)
+        org, op = _d(sys.origin)
+        out = np.zeros(3 * 4096, np.int32)
+        n = self._delta(bp, op, C.c_double(cellsize), C.c_double(cutoff), out.ctypes.data_as(_I), 4096)
+        assert n >= 0
+        return out[: 3 * n].reshape(-1, 3)
+
+    # --------------------------------------------------------------- cutoff
+    def _sysargs(self, sys):
+        keep = []
+
+        def d(a):
+            a2, p = _d(a)
+            keep.append(a2)
+            return p
+
+        def i(a):
+            a2, p = _i(a)
+            keep.append(a2)
+            return p
+
+        args = [sys.n, d(sys.pos), d(sys.q), i(sys.type), sys.ntypes, d(sys.ljA), d(sys.ljB),
+                d(sys.ljA14), d(sys.ljB14), i(sys.molecule), len(sys.ex_a), i(sys.ex_a),
+                i(sys.ex_b), i(sys.ex_cls), C.c_double(sys.csf), d(sys.box), d(sys.origin)]
+        return args, keep
+
+    def cutoff(self, sys, terms, algo_cutoff, cellsize, virial=False, counts=False):
+        args, keep = self._sysargs(sys)
+        tarr = (Term * len(terms))(*terms)
+        f = np.zeros((sys.n, 3))
+        e = np.zeros(E_COUNT)
+        nc = None
+        cand = inrc = None
+        if counts:
+            info = self.cells(sys, cellsize)
+            nc = int(np.prod(info["dims"]))
+            cand = np.zeros(nc, np.int64)
+            inrc = np.zeros(nc, np.int64)
+        r = self._cutoff(*args, C.c_double(cellsize), len(terms), tarr, C.c_double(algo_cutoff),
+                         int(virial), f.ctypes.data_as(_D), e.ctypes.data_as(_D),
+                         cand.ctypes.data_as(_L) if counts else None,
+                         inrc.ctypes.data_as(_L) if counts else None)
+        if r != 0:
+            raise ValueError(f"{self.kind} cutoff failed ({r})")
+        out = dict(forces=f, energies=e)
+        if counts:
+            out.update(cand=cand, inrc=inrc)
+        return out
+
+    def cutoff_omp(self, sys, terms, algo_cutoff, cellsize, threads=0):
+        args, keep = self._sysargs(sys)
+        tarr = (Term * len(terms))(*terms)
+        f = np.zeros((sys.n, 3))
+        e = np.zeros(E_COUNT)
+        used = self._cutoff_omp(threads, *args, C.c_double(cellsize), len(terms), tarr,
+                                C.c_double(algo_cutoff), f.ctypes.data_as(_D), e.ctypes.data_as(_D))
+        return dict(forces=f, energies=e, threads=used)
+
+    # ------------------------------------------------------------------ PME
+    def pme(self, sys, grid, order, rc, cellsize, accuracy=1e-6, alpha=-1.0, flags=6,
+            virial=False, want_grid=False):
+        keep = []
+
+        def d(a):
+            a2, p = _d(a)
+            keep.append(a2)
+            return p
+
+        def i(a):
+            a2, p = _i(a)
+            keep.append(a2)
+            return p
+
+        f = np.zeros((sys.n, 3))
+        e = np.zeros(E_COUNT)
+        common = [sys.n, d(sys.pos), d(sys.q), i(sys.molecule), len(sys.ex_a), i(sys.ex_a),
+                  i(sys.ex_b), i(sys.ex_cls), C.c_double(sys.csf), d(sys.box), d(sys.origin),
+                  C.c_double(cellsize), grid[0], grid[1], grid[2], order, C.c_double(rc),
+                  C.c_double(accuracy), C.c_double(alpha), flags, int(virial),
+                  f.ctypes.data_as(_D), e.ctypes.data_as(_D)]
+        out = dict(forces=f, energies=e)
+        if self.kind == "port":
+            parts = np.zeros(6)
+            qg = np.zeros(int(np.prod(grid))) if want_grid else None
+            r = self._pme(*common, parts.ctypes.data_as(_D),
+                          qg.ctypes.data_as(_D) if want_grid else None)
+            out["parts"] = dict(zip(["real", "reciprocal", "intra", "self", "charged", "alpha"], parts))
+            if want_grid:
+                out["qgrid"] = qg.reshape(grid)
+        else:
+            r = self._pme(*common)
+        if r != 0:
+            raise ValueError(f"{self.kind} pme failed ({r})")
+        return out
+
+    def bspline(self, order, w):
+        th = np.zeros(order)
+        dth = np.zeros(order)
+        self._bspline(order, C.c_double(w), th.ctypes.data_as(_D), dth.ctypes.data_as(_D))
+        return th, dth
+
+    def pme_alpha(self, rc, accuracy):
+        return self._alpha(C.c_double(rc), C.c_double(accuracy))
+
+    def fft3d(self, data, sign):
+        a = np.ascontiguousarray(data, dtype=np.complex128).copy()
+        nx, ny, nz = a.shape
+        self._fft(a.ctypes.data_as(_D), nx, ny, nz, sign)
+        return a
+
+    def rem_delta(self, ti, tj, ei, ej):
+        return self._rem(C.c_double(ti), C.c_double(tj), C.c_double(ei), C.c_double(ej))
+
+    # -------------------------------------------------------------- Hessian
+    def hessian_vec(self, sys, terms, v, cellsize=4.0):
+        args, keep = self._sysargs(sys)
+        tarr = (Term * len(terms))(*terms)
+        v2, vp = _d(np.asarray(v).reshape(-1))
+        hv = np.zeros(3 * sys.n)
+        if self.kind == "port":
+            r = self._hess(*args, len(terms), tarr, vp, hv.ctypes.data_as(_D))
+        else:
+            r = self._hess(*args, C.c_double(cellsize), len(terms), tarr, vp, hv.ctypes.data_as(_D))
+        if r != 0:
+            raise ValueError(f"{self.kind} hessian failed ({r})")
+        return hv.reshape(-1, 3)

@@ -1,0 +1,392 @@
+// ref_harness.cpp -- C entry points around the compiled ProtoMol reference.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's own sources (read in place from the reference tree) into
+// oracle/_ref/libpmref.so.  The library instantiates the reference's force
+// templates exactly as factories/registerForceExemplarsCutoff.cpp and
+// registerForceExemplarsFastElectrostatic.cpp register them, builds a
+// Topology<PeriodicBoundaryConditions,CubicCellManager> from plain arrays the
+// way frontend/buildTopology.cpp fills it, and evaluates forces the way
+// applications/mdEnergy-app/mdEnergy.cpp:387-392 does (clear energies, zero
+// forces, evaluate).  Its entry points mirror oracle/pmoracle.h so that the
+// tests can compare the restatement against the reference call for call.
+#include "Report.h"
+// base/Grid.h names report/error/endr unqualified; the reference includes it
+// after a using-directive (forces/NonbondedPMEwaldSystemForce.h:18).
+using namespace ProtoMol::Report;
+#include "Topology.h"
+#include "PeriodicBoundaryConditions.h"
+#include "CubicCellManager.h"
+#include "CellListEnumerator_periodicBoundaries.h"
+#include "NonbondedCutoffSystemForce.h"
+#include "OneAtomPair.h"
+#include "OneAtomPairTwo.h"
+#include "LennardJonesForce.h"
+#include "CoulombForce.h"
+#include "CoulombEwaldRealForce.h"
+#include "C1SwitchingFunction.h"
+#include "C2SwitchingFunction.h"
+#include "CutoffSwitchingFunction.h"
+#include "NonbondedPMEwaldSystemForce.h"
+#include "BSpline.h"
+#include "ScalarStructure.h"
+#include "Vector3DBlock.h"
+#include "ForceGroup.h"
+#include "Hessian.h"
+
+#include <vector>
+#include <map>
+
+using namespace ProtoMol;
+
+namespace {
+
+typedef Topology<PeriodicBoundaryConditions, CubicCellManager> Topo;
+
+enum { POT_LJ = 0, POT_COULOMB = 1, POT_EWALDREAL = 2 };
+enum { SWF_CUTOFF = 0, SWF_C1 = 1, SWF_C2 = 2 };
+
+struct Term {
+  int pot;
+  int swf;
+  double alpha;
+  double switchon;
+  double cutoff;
+};
+
+// Counters fed by the constraint hooks of OneAtomPair(Two)
+// (forces/oneAtomContraints.h): every candidate pair passes PRE_CHECK, every
+// pair that reaches the force computation passes POST_CHECK.
+std::vector<int>* g_cellOf = 0;
+long long* g_cand = 0;
+long long* g_inrc = 0;
+
+struct CountConstraint {
+  enum { PRE_CHECK = 1, POST_CHECK = 1 };
+  static bool check(const GenericTopology*, int i, int) {
+    if (g_cand) g_cand[(*g_cellOf)[i]]++;
+    return true;
+  }
+  static bool check(const GenericTopology*, int) { return true; }
+  static std::string getPrefixId() { return ""; }
+  static std::string getPostfixId() { return "Count"; }
+  static void check(const GenericTopology*, int i, int, const Vector3D&, Real, const Vector3D&) {
+    if (g_inrc) g_inrc[(*g_cellOf)[i]]++;
+  }
+};
+
+Topo* makeTopo(int n, const double* pos, const double* q, const int* type, int ntypes,
+               const double* ljA, const double* ljB, const double* ljA14, const double* ljB14,
+               const int* molecule, int nexcl, const int* ea, const int* eb, const int* ec,
+               double csf, const double* box, const double* origin, double cellsize,
+               Vector3DBlock& positions) {
+  // frontend/buildTopology.cpp fills the same members from PSF/PAR data.
+  Topo* t = new Topo(csf, ExclusionType(ExclusionType::ONE4MODIFIED),
+                     PeriodicBoundaryConditions(Vector3D(box[0], 0, 0), Vector3D(0, box[1], 0),
+                                                Vector3D(0, 0, box[2]),
+                                                Vector3D(origin[0], origin[1], origin[2])),
+                     CubicCellManager(cellsize));
+  int nmol = 0;
+  for (int i = 0; i < n; i++) nmol = std::max(nmol, molecule[i] + 1);
+  t->atomTypes.resize(ntypes);
+  t->atoms.resize(n);
+  t->molecules.resize(nmol);
+  for (int i = 0; i < n; i++) {
+    t->atoms[i].type = type[i];
+    t->atoms[i].scaledCharge = q[i];
+    t->atoms[i].scaledMass = 1.0;
+    t->atoms[i].molecule = molecule[i];
+    t->molecules[molecule[i]].atoms.push_back(i);
+  }
+  t->lennardJonesParameters.resize(ntypes);
+  for (int a = 0; a < ntypes; a++)
+    for (int b = 0; b < ntypes; b++)
+      t->lennardJonesParameters.set(a, b, LennardJonesParameters(ljA[a * ntypes + b], ljB[a * ntypes + b],
+                                                                 ljA14[a * ntypes + b], ljB14[a * ntypes + b]));
+  t->exclusions.resize(n);
+  for (int k = 0; k < nexcl; k++) t->exclusions.add(ea[k], eb[k], (ExclusionClass)ec[k]);
+  t->exclusions.optimize();
+  positions.resize(n);
+  for (int i = 0; i < n; i++) positions[i] = Vector3D(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  return t;
+}
+
+void readEnergies(const ScalarStructure& e, double* energies) {
+  energies[0] += e[ScalarStructure::COULOMB];
+  energies[1] += e[ScalarStructure::LENNARDJONES];
+  const ScalarStructure::Index v[9] = {ScalarStructure::VIRIALXX, ScalarStructure::VIRIALXY,
+                                       ScalarStructure::VIRIALXZ, ScalarStructure::VIRIALYX,
+                                       ScalarStructure::VIRIALYY, ScalarStructure::VIRIALYZ,
+                                       ScalarStructure::VIRIALZX, ScalarStructure::VIRIALZY,
+                                       ScalarStructure::VIRIALZZ};
+  for (int k = 0; k < 9; k++) energies[2 + k] += e[v[k]];
+}
+
+// Switching-function and potential objects from a Term, as
+// registerForceExemplarsCutoff.cpp instantiates them.
+template <class S> S makeSwf(const Term& t);
+template <> C2SwitchingFunction makeSwf<C2SwitchingFunction>(const Term& t) {
+  return C2SwitchingFunction(t.switchon, t.cutoff);
+}
+template <> C1SwitchingFunction makeSwf<C1SwitchingFunction>(const Term& t) {
+  return C1SwitchingFunction(t.cutoff);
+}
+template <> CutoffSwitchingFunction makeSwf<CutoffSwitchingFunction>(const Term& t) {
+  return CutoffSwitchingFunction(t.cutoff);
+}
+template <class P> P makePot(const Term& t);
+template <> LennardJonesForce makePot<LennardJonesForce>(const Term&) { return LennardJonesForce(); }
+template <> CoulombForce makePot<CoulombForce>(const Term&) { return CoulombForce(); }
+template <> CoulombEwaldRealForce makePot<CoulombEwaldRealForce>(const Term& t) {
+  return CoulombEwaldRealForce(t.alpha);
+}
+
+template <class Pair>
+SystemForce* makeCutoff(double algoCutoff, const Pair& p) {
+  return new NonbondedCutoffSystemForce<CubicCellManager, Pair>(algoCutoff, p);
+}
+
+template <class S1, class P1, bool Count>
+SystemForce* make1(const Term* t, double algoCutoff) {
+  if (Count) {
+    typedef OneAtomPair<PeriodicBoundaryConditions, S1, P1, CountConstraint> Pair;
+    return makeCutoff(algoCutoff, Pair(makePot<P1>(t[0]), makeSwf<S1>(t[0])));
+  }
+  typedef OneAtomPair<PeriodicBoundaryConditions, S1, P1> Pair;
+  return makeCutoff(algoCutoff, Pair(makePot<P1>(t[0]), makeSwf<S1>(t[0])));
+}
+
+template <class S1, class P1, class S2, class P2, bool Count>
+SystemForce* make2(const Term* t, double algoCutoff) {
+  if (Count) {
+    typedef OneAtomPairTwo<PeriodicBoundaryConditions, S1, P1, S2, P2, CountConstraint> Pair;
+    return makeCutoff(algoCutoff, Pair(makePot<P1>(t[0]), makeSwf<S1>(t[0]), makePot<P2>(t[1]), makeSwf<S2>(t[1])));
+  }
+  typedef OneAtomPairTwo<PeriodicBoundaryConditions, S1, P1, S2, P2> Pair;
+  return makeCutoff(algoCutoff, Pair(makePot<P1>(t[0]), makeSwf<S1>(t[0]), makePot<P2>(t[1]), makeSwf<S2>(t[1])));
+}
+
+template <class P1, bool Count>
+SystemForce* dispatch1Swf(const Term* t, double c) {
+  switch (t[0].swf) {
+    case SWF_C1: return make1<C1SwitchingFunction, P1, Count>(t, c);
+    case SWF_C2: return make1<C2SwitchingFunction, P1, Count>(t, c);
+    default: return make1<CutoffSwitchingFunction, P1, Count>(t, c);
+  }
+}
+
+template <bool Count>
+SystemForce* dispatch1(const Term* t, double c) {
+  switch (t[0].pot) {
+    case POT_LJ: return dispatch1Swf<LennardJonesForce, Count>(t, c);
+    case POT_COULOMB: return dispatch1Swf<CoulombForce, Count>(t, c);
+    default: return dispatch1Swf<CoulombEwaldRealForce, Count>(t, c);
+  }
+}
+
+template <class S1, class P1, class P2, bool Count>
+SystemForce* dispatch2Swf2(const Term* t, double c) {
+  switch (t[1].swf) {
+    case SWF_C1: return make2<S1, P1, C1SwitchingFunction, P2, Count>(t, c);
+    case SWF_C2: return make2<S1, P1, C2SwitchingFunction, P2, Count>(t, c);
+    default: return make2<S1, P1, CutoffSwitchingFunction, P2, Count>(t, c);
+  }
+}
+
+template <class P1, class P2, bool Count>
+SystemForce* dispatch2Swf1(const Term* t, double c) {
+  switch (t[0].swf) {
+    case SWF_C1: return dispatch2Swf2<C1SwitchingFunction, P1, P2, Count>(t, c);
+    case SWF_C2: return dispatch2Swf2<C2SwitchingFunction, P1, P2, Count>(t, c);
+    default: return dispatch2Swf2<CutoffSwitchingFunction, P1, P2, Count>(t, c);
+  }
+}
+
+template <bool Count>
+SystemForce* dispatch2(const Term* t, double c) {
+  // The registered OneAtomPairTwo ids put LennardJones first
+  // (registerForceExemplarsCutoff.cpp:75-142).
+  if (t[0].pot != POT_LJ) return 0;
+  switch (t[1].pot) {
+    case POT_COULOMB: return dispatch2Swf1<LennardJonesForce, CoulombForce, Count>(t, c);
+    case POT_EWALDREAL: return dispatch2Swf1<LennardJonesForce, CoulombEwaldRealForce, Count>(t, c);
+    default: return 0;
+  }
+}
+
+template <bool R, bool Q, bool C>
+SystemForce* makePME(int nx, int ny, int nz, int order, double rc, double acc, double alpha) {
+  return new NonbondedPMEwaldSystemForce<PeriodicBoundaryConditions, CubicCellManager, R, Q, C,
+                                         BSpline, CutoffSwitchingFunction>(nx, ny, nz, order, rc, acc,
+                                                                           alpha, 3.0);
+}
+
+void cellOfAtoms(const Topo* t, std::vector<int>& cellOf) {
+  const ArrayCellListStructure& cl = t->cellLists;
+  int ny = cl.getDimY(), nz = cl.getDimZ();
+  cellOf.assign(t->atoms.size(), -1);
+  for (ArrayCellListStructure::const_iterator it = cl.begin(); it != cl.end(); ++it) {
+    if (it->second < 0) continue;
+    int c = (it->first.x * ny + it->first.y) * nz + it->first.z;
+    for (int a = it->second; a != -1; a = t->atoms[a].cellListNext) cellOf[a] = c;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_cells(int n, const double* pos, const double* box, const double* origin, double cellsize,
+              int* dims, double* realcell, int* cell, int* head, int* next, int ncap) {
+  std::vector<double> q(n, 0.0), lj(1, 0.0);
+  std::vector<int> type(n, 0), mol(n);
+  for (int i = 0; i < n; i++) mol[i] = i;
+  Vector3DBlock positions;
+  Topo* t = makeTopo(n, pos, &q[0], &type[0], 1, &lj[0], &lj[0], &lj[0], &lj[0], &mol[0], 0, 0, 0, 0,
+                     1.0, box, origin, cellsize, positions);
+  t->uncacheCellList();
+  t->updateCellLists(&positions);
+  const ArrayCellListStructure& cl = t->cellLists;
+  int nx = cl.getDimX(), ny = cl.getDimY(), nz = cl.getDimZ();
+  int nc = nx * ny * nz;
+  if (dims) { dims[0] = nx; dims[1] = ny; dims[2] = nz; }
+  if (realcell) {
+    Vector3D r = t->cellManager.getRealCellSize();
+    realcell[0] = r.x; realcell[1] = r.y; realcell[2] = r.z;
+  }
+  if (head && nc <= ncap) {
+    for (int c = 0; c < nc; c++) head[c] = -1;
+    // iterate the whole array (begin() is trimmed to the first non-empty cell)
+    for (ArrayCellListStructure::const_iterator it = cl.begin(); it != cl.end(); ++it) {
+      int c = (it->first.x * ny + it->first.y) * nz + it->first.z;
+      head[c] = it->second;
+    }
+    for (int i = 0; i < n; i++) if (next) next[i] = t->atoms[i].cellListNext;
+    if (cell) {
+      std::vector<int> co;
+      cellOfAtoms(t, co);
+      for (int i = 0; i < n; i++) cell[i] = co[i];
+    }
+  }
+  delete t;
+  return nc > ncap ? -1 : nc;
+}
+
+int ref_cutoff(int n, const double* pos, const double* q, const int* type, int ntypes,
+               const double* ljA, const double* ljB, const double* ljA14, const double* ljB14,
+               const int* molecule, int nexcl, const int* ea, const int* eb, const int* ec, double csf,
+               const double* box, const double* origin, double cellsize, int nterms, const Term* terms,
+               double algoCutoff, int wantVirial, double* forces, double* energies, long long* cand,
+               long long* inrc) {
+  Vector3DBlock positions;
+  Topo* t = makeTopo(n, pos, q, type, ntypes, ljA, ljB, ljA14, ljB14, molecule, nexcl, ea, eb, ec, csf,
+                     box, origin, cellsize, positions);
+  bool count = (cand != 0 || inrc != 0);
+  SystemForce* f = 0;
+  if (nterms == 1) f = count ? dispatch1<true>(terms, algoCutoff) : dispatch1<false>(terms, algoCutoff);
+  else if (nterms == 2) f = count ? dispatch2<true>(terms, algoCutoff) : dispatch2<false>(terms, algoCutoff);
+  if (!f) { delete t; return -1; }
+  std::vector<int> cellOf;
+  if (count) {
+    t->uncacheCellList();
+    t->updateCellLists(&positions);
+    cellOfAtoms(t, cellOf);
+    int nc = t->cellLists.getDimX() * t->cellLists.getDimY() * t->cellLists.getDimZ();
+    if (cand) for (int c = 0; c < nc; c++) cand[c] = 0;
+    if (inrc) for (int c = 0; c < nc; c++) inrc[c] = 0;
+    g_cellOf = &cellOf;
+    g_cand = cand;
+    g_inrc = inrc;
+  }
+  Vector3DBlock fo;
+  fo.resize(n);
+  fo.zero();
+  ScalarStructure e;
+  e.clear();
+  e.virial(wantVirial != 0);
+  t->uncacheCellList();     // ForceGroup::evaluateSystemForces (ForceGroup.cpp:50)
+  f->evaluate(t, &positions, &fo, &e);
+  g_cellOf = 0; g_cand = 0; g_inrc = 0;
+  for (int i = 0; i < n; i++) {
+    forces[3 * i] += fo[i].x; forces[3 * i + 1] += fo[i].y; forces[3 * i + 2] += fo[i].z;
+  }
+  readEnergies(e, energies);
+  delete f;
+  delete t;
+  return 0;
+}
+
+int ref_pme(int n, const double* pos, const double* q, const int* molecule, int nexcl, const int* ea,
+            const int* eb, const int* ec, double csf, const double* box, const double* origin,
+            double cellsize, int nx, int ny, int nz, int order, double rc, double accuracy, double alpha,
+            int flags, int wantVirial, double* forces, double* energies) {
+  std::vector<double> lj(1, 0.0);
+  std::vector<int> type(n, 0);
+  Vector3DBlock positions;
+  Topo* t = makeTopo(n, pos, q, &type[0], 1, &lj[0], &lj[0], &lj[0], &lj[0], molecule, nexcl, ea, eb, ec,
+                     csf, box, origin, cellsize, positions);
+  SystemForce* f = 0;
+  switch (flags & 7) {
+    case 1: f = makePME<true, false, false>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    case 2: f = makePME<false, true, false>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    case 3: f = makePME<true, true, false>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    case 4: f = makePME<false, false, true>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    case 5: f = makePME<true, false, true>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    case 6: f = makePME<false, true, true>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    case 7: f = makePME<true, true, true>(nx, ny, nz, order, rc, accuracy, alpha); break;
+    default: delete t; return -1;
+  }
+  Vector3DBlock fo;
+  fo.resize(n);
+  fo.zero();
+  ScalarStructure e;
+  e.clear();
+  e.virial(wantVirial != 0);
+  t->uncacheCellList();
+  f->evaluate(t, &positions, &fo, &e);
+  for (int i = 0; i < n; i++) {
+    forces[3 * i] += fo[i].x; forces[3 * i + 1] += fo[i].y; forces[3 * i + 2] += fo[i].z;
+  }
+  readEnergies(e, energies);
+  delete f;
+  delete t;
+  return 0;
+}
+
+void ref_bspline(int order, double w, double* theta, double* dtheta) {
+  BSpline b(order, w);
+  for (int k = 0; k < order; k++) { theta[k] = b.theta[k]; dtheta[k] = b.dTheta[k]; }
+}
+
+// Dense Hessian of LJ + Coulomb forces through Hessian::findForces/evaluate
+// (integrators/Hessian.cpp:88-267, 269-624), contracted with v.
+int ref_hessian_vec(int n, const double* pos, const double* q, const int* type, int ntypes,
+                    const double* ljA, const double* ljB, const double* ljA14, const double* ljB14,
+                    const int* molecule, int nexcl, const int* ea, const int* eb, const int* ec,
+                    double csf, const double* box, const double* origin, double cellsize, int nterms,
+                    const Term* terms, const double* v, double* hv) {
+  Vector3DBlock positions;
+  Topo* t = makeTopo(n, pos, q, type, ntypes, ljA, ljB, ljA14, ljB14, molecule, nexcl, ea, eb, ec, csf,
+                     box, origin, cellsize, positions);
+  ForceGroup* fg = new ForceGroup();
+  for (int k = 0; k < nterms; k++) {
+    SystemForce* f = dispatch1<false>(&terms[k], terms[k].cutoff);
+    if (!f) { delete fg; delete t; return -1; }
+    fg->addSystemForce(f);
+  }
+  unsigned int sz = 3 * n;
+  Hessian h(sz);
+  h.findForces(fg);
+  h.clear();
+  h.evaluate(&positions, t, false);
+  for (unsigned int r = 0; r < sz; r++) {
+    double s = 0.0;
+    for (unsigned int c = 0; c < sz; c++) s += h.hessM[(size_t)r * sz + c] * v[c];
+    hv[r] = s;
+  }
+  delete fg;
+  delete t;
+  return 0;
+}
+
+}  // extern "C"

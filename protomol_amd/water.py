@@ -1,0 +1,171 @@
+"""Synthetic rigid-geometry 3-site water boxes (the benchmark and test inputs).
+
+The molecule, units and topology conventions follow ProtoMol:
+  * charges are scaled by sqrt(332.0636) = 18.2226123264
+    (frontend/buildTopology.cpp:148, base/pmconstants.cpp:72);
+  * Lennard-Jones A/B tables use the CHARMM mixing rule
+    A = (Ri+Rj)^12 sqrt(ei ej), B = 2 (Ri+Rj)^6 sqrt(ei ej) with R = Rmin/2
+    (frontend/buildTopology.cpp:643-664);
+  * the three intramolecular pairs of a TIP3 water are fully excluded
+    (bond + angle under exclude scaled1-4, frontend/buildTopology.cpp:963-1011);
+  * velocities are in A per AKMA time unit so that x += v * dt * INV_TIMEFACTOR
+    (integrators/LeapfrogIntegrator.cpp:96-100).
+Positions are generated from a seed: molecule oxygens on a cubic lattice in a
+box centred on the origin, uniformly random orientations, Maxwell-Boltzmann
+velocities.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+SQRT_COULOMB = 18.2226123264
+BOLTZMANN = 0.001987191
+INV_TIMEFACTOR = 0.02045482828087295
+
+EXCL_NONE, EXCL_MODIFIED, EXCL_FULL = 0, 1, 2
+
+Q_O, Q_H = -0.834, 0.417
+M_O, M_H = 15.9994, 1.008
+R_OH = 0.9572
+THETA_HOH = 104.52
+SIGMA_O, EPS_O = 3.1507, 0.1521
+
+
+@dataclasses.dataclass
+class WaterSystem:
+    """Plain arrays describing one periodic system (AoS positions, Angstrom)."""
+
+    pos: np.ndarray          # (n, 3) float64
+    vel: np.ndarray          # (n, 3) float64
+    q: np.ndarray            # (n,) scaled charges
+    type: np.ndarray         # (n,) int32 atom type
+    mass: np.ndarray         # (n,) amu
+    molecule: np.ndarray     # (n,) int32
+    ex_a: np.ndarray         # exclusion pairs (int32)
+    ex_b: np.ndarray
+    ex_cls: np.ndarray
+    ljA: np.ndarray          # (T, T) float64
+    ljB: np.ndarray
+    ljA14: np.ndarray
+    ljB14: np.ndarray
+    box: np.ndarray          # (3,) box edges (orthogonal)
+    origin: np.ndarray       # (3,) box centre
+    csf: float = 1.0         # coulombScalingFactor
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    @property
+    def ntypes(self) -> int:
+        return int(self.ljA.shape[0])
+
+    def copy(self) -> "WaterSystem":
+        return dataclasses.replace(self, **{f.name: (np.array(getattr(self, f.name), copy=True)
+                                                     if isinstance(getattr(self, f.name), np.ndarray)
+                                                     else getattr(self, f.name))
+                                            for f in dataclasses.fields(self)})
+
+
+def _lj_tables(sigma_eps):
+    """CHARMM mixing: (Ri+Rj)^12 * sqrt(ei*ej) with R = sigma*2^(-5/6)."""
+    t = len(sigma_eps)
+    A = np.zeros((t, t))
+    B = np.zeros((t, t))
+    for i, (si, ei) in enumerate(sigma_eps):
+        for j, (sj, ej) in enumerate(sigma_eps):
+            r = si * 2.0 ** (-5.0 / 6.0) + sj * 2.0 ** (-5.0 / 6.0)
+            e = math.sqrt(ei * ej)
+            A[i, j] = r ** 12 * e
+            B[i, j] = 2.0 * r ** 6 * e
+    return A, B
+
+
+def _random_rotations(rng, m):
+    """Uniform random rotation matrices (Shoemake's quaternion method)."""
+    u1, u2, u3 = rng.random(m), rng.random(m), rng.random(m)
+    a = np.sqrt(1.0 - u1)
+    b = np.sqrt(u1)
+    qw = a * np.sin(2 * np.pi * u2)
+    qx = a * np.cos(2 * np.pi * u2)
+    qy = b * np.sin(2 * np.pi * u3)
+    qz = b * np.cos(2 * np.pi * u3)
+    R = np.empty((m, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (qy * qy + qz * qz)
+    R[:, 0, 1] = 2 * (qx * qy - qz * qw)
+    R[:, 0, 2] = 2 * (qx * qz + qy * qw)
+    R[:, 1, 0] = 2 * (qx * qy + qz * qw)
+    R[:, 1, 1] = 1 - 2 * (qx * qx + qz * qz)
+    R[:, 1, 2] = 2 * (qy * qz - qx * qw)
+    R[:, 2, 0] = 2 * (qx * qz - qy * qw)
+    R[:, 2, 1] = 2 * (qy * qz + qx * qw)
+    R[:, 2, 2] = 1 - 2 * (qx * qx + qy * qy)
+    return R
+
+
+def water_box(n_molecules: int, box: float, seed: int = 0, temperature: float = 300.0,
+              lattice: int | None = None) -> WaterSystem:
+    """Build a periodic cube of `n_molecules` waters with edge `box` (Angstrom).
+
+    Oxygens sit on an `lattice`^3 cubic lattice (default: the smallest cube
+    holding n_molecules sites; a seeded subset is used when it holds more).
+    """
+    rng = np.random.default_rng(seed)
+    side = lattice or int(math.ceil(n_molecules ** (1.0 / 3.0) - 1e-9))
+    while side ** 3 < n_molecules:
+        side += 1
+    nsites = side ** 3
+    sites = np.arange(nsites)
+    if nsites > n_molecules:
+        sites = np.sort(rng.choice(nsites, size=n_molecules, replace=False))
+    a = box / side
+    ix, iy, iz = sites // (side * side), (sites // side) % side, sites % side
+    centres = np.stack([ix, iy, iz], axis=1) * a + 0.5 * a - 0.5 * box
+    half = math.radians(THETA_HOH) / 2.0
+    local = np.array([[0.0, 0.0, 0.0],
+                      [R_OH * math.sin(half), 0.0, R_OH * math.cos(half)],
+                      [-R_OH * math.sin(half), 0.0, R_OH * math.cos(half)]])
+    rot = _random_rotations(rng, n_molecules)
+    pos = centres[:, None, :] + np.einsum("mij,aj->mai", rot, local)
+    pos = pos.reshape(-1, 3)
+    n = 3 * n_molecules
+    q = np.tile([Q_O, Q_H, Q_H], n_molecules) * SQRT_COULOMB
+    typ = np.tile([0, 1, 1], n_molecules).astype(np.int32)
+    mass = np.tile([M_O, M_H, M_H], n_molecules)
+    mol = np.repeat(np.arange(n_molecules, dtype=np.int32), 3)
+    base = 3 * np.arange(n_molecules, dtype=np.int32)
+    ex_a = np.stack([base, base, base + 1], axis=1).reshape(-1).astype(np.int32)
+    ex_b = np.stack([base + 1, base + 2, base + 2], axis=1).reshape(-1).astype(np.int32)
+    ex_c = np.full(ex_a.shape, EXCL_FULL, dtype=np.int32)
+    A, B = _lj_tables([(SIGMA_O, EPS_O), (0.0, 0.0)])
+    vel = rng.standard_normal((n, 3)) * np.sqrt(BOLTZMANN * temperature / mass)[:, None]
+    return WaterSystem(pos=np.ascontiguousarray(pos), vel=vel, q=q, type=typ, mass=mass,
+                       molecule=mol, ex_a=ex_a, ex_b=ex_b, ex_cls=ex_c, ljA=A, ljB=B,
+                       ljA14=A.copy(), ljB14=B.copy(), box=np.array([box, box, box], dtype=float),
+                       origin=np.zeros(3))
+
+
+# Named workloads (molecule count, box edge in Angstrom).
+CONFIGS = {
+    "water648": (216, 18.62),            # 648 atoms, cutoff 8 with C2 from 7, cell 4
+    "water98k": (32768, 99.3),           # 98,304 atoms, PME 96^3, rc 10
+    "water1m": (333333, 215.4),          # 999,999 atoms, PME 240^3, rc 10
+    "water8k": (2700, 43.2),             # 8,100 atoms, PME 48^3, rc 9
+}
+
+
+def named(name: str, seed: int = 0) -> WaterSystem:
+    nmol, box = CONFIGS[name]
+    lattice = {"water1m": 70, "water8k": 14}.get(name)
+    return water_box(nmol, box, seed=seed, lattice=lattice)
+
+
+def replica_sizes(count: int = 64, seed: int = 0, lo: int = 100, hi: int = 1000):
+    """Uneven replica sizes (molecules) and cube edges at 0.0334 molecules/A^3."""
+    rng = np.random.default_rng(seed)
+    n = rng.integers(lo, hi + 1, size=count)
+    edges = (n / 0.0334) ** (1.0 / 3.0)
+    return n, edges
