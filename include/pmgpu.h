@@ -1,0 +1,177 @@
+/*
+ * pmgpu.h -- C ABI of the MI355X (gfx950) implementation of ProtoMol's
+ * force-evaluation hot path: the periodic cell-list cutoff nonbonded system
+ * force and the PME reciprocal/correction force.
+ *
+ * The entry points replace, one for one, what a ProtoMol SystemForce does for
+ * these forces (ProtoMol 2.1 tree, framework/):
+ *   - force construction by id and positional parameters
+ *       ForceFactory::make -> Force::make -> doMake
+ *       (frontend/ForceFactoryDetails.h:60, forces/Force.cpp:15-21,
+ *        forces/NonbondedCutoffForce.h:113-129,
+ *        forces/NonbondedPMEwaldSystemForce.h:1078-1139)
+ *   - evaluation
+ *       SystemForce::evaluate(topo, positions, forces, energies)
+ *       (forces/SystemForce.h:39), driven by ForceGroup::evaluateSystemForces
+ *       (forces/ForceGroup.cpp:43-96)
+ *   - Force::uncache (forces/Force.h, NonbondedPMEwaldSystemForce.h:118-121)
+ * plus the topology data that GenericTopology holds (atoms, types, LJ table,
+ * exclusions, boundary conditions, cell manager; topology/GenericTopology.h,
+ * topology/Topology.h) and an on-device MD driver that replaces
+ * LeapfrogIntegrator::run / LangevinLeapfrogIntegrator
+ * (integrators/LeapfrogIntegrator.cpp:107-121,
+ *  integrators/LangevinLeapfrogIntegrator.cpp:58-79).
+ *
+ * Conventions (identical to ProtoMol): positions/forces are AoS double[3N]
+ * (base/Vector3DBlock.h), Angstrom, kcal/mol, amu; charges are pre-scaled by
+ * sqrt(332.0636); forces and energies are ACCUMULATED into (+=), the caller
+ * zeroes them (StandardIntegrator.cpp:61).  energies[] is the prefix of
+ * ScalarStructure (base/ScalarStructure.h:25-45):
+ *   [0] COULOMB, [1] LENNARDJONES, [2] BOND, [3] ANGLE, [4] DIHEDRAL,
+ *   [5] IMPROPER, [6] OTHER, [7..15] VIRIALXX XY XZ YX YY YZ ZX ZY ZZ.
+ * All functions return PM_OK (0) or a PM_ERR_* code; pm_last_error() gives the
+ * message, worded as the reference's errMsg/report text where one exists.
+ */
+#ifndef PMGPU_H
+#define PMGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_ABI_VERSION 1
+
+enum {
+  PM_OK = 0,
+  PM_ERR_ARGUMENT = 1,     /* invalid parameter (reference: make() returns NULL + errMsg) */
+  PM_ERR_STATE = 2,        /* call out of order (no topology, no boundary, ...) */
+  PM_ERR_DEVICE = 3,       /* HIP runtime failure */
+  PM_ERR_CAPACITY = 4,     /* a device buffer bound was exceeded (message says which) */
+  PM_ERR_UNSUPPORTED = 5   /* force id not implemented on the GPU */
+};
+
+enum { PM_PRECISION_MIXED = 0, PM_PRECISION_DOUBLE = 1 };
+enum { PM_EVAL_VIRIAL = 1 };
+enum { PM_INTEGRATOR_LEAPFROG = 0, PM_INTEGRATOR_LANGEVIN = 1 };
+enum { PM_ENERGY_COUNT = 16 };
+enum {
+  PM_T_BIN = 0, PM_T_LIST = 1, PM_T_PAIR = 2, PM_T_PME = 3, PM_T_CORR = 4,
+  PM_T_BONDED = 5, PM_T_REDUCE = 6, PM_T_INTEGRATE = 7, PM_TIMING_CATEGORIES = 8
+};
+
+typedef struct pm_context pm_context;
+
+/* Version of this ABI (PM_ABI_VERSION). */
+int pm_abi_version(void);
+
+/* Last error message of this context (never NULL). */
+const char* pm_last_error(const pm_context* ctx);
+
+/* Create a context on HIP device `device`.  No reference counterpart: the
+ * reference keeps this state in the Force objects and the topology. */
+int pm_create(int device, pm_context** out);
+void pm_destroy(pm_context* ctx);
+
+/* Arithmetic mode of every kernel: FP32 pair/reciprocal arithmetic with
+ * 64-bit fixed-point force accumulation (MIXED), or FP64 throughout (DOUBLE). */
+int pm_set_precision(pm_context* ctx, int precision);
+
+/* Topology arrays as frontend/buildTopology.cpp builds them: scaled charges
+ * (:148), atom types, LJ A/B/A14/B14 tables ntypes x ntypes row-major
+ * (:643-664, LennardJonesParameterTable), molecule of each atom (:733-905),
+ * exclusion pairs and their ExclusionClass 0 NONE / 1 MODIFIED / 2 FULL
+ * (:963-1011, ExclusionTable::add), masses, coulombScalingFactor. */
+int pm_set_topology(pm_context* ctx, int n, const double* scaled_charge, const int* type,
+                    int ntypes, const double* ljA, const double* ljB, const double* ljA14,
+                    const double* ljB14, const int* molecule, const double* mass, int nexcl,
+                    const int* ex_a, const int* ex_b, const int* ex_class,
+                    double coulomb_scaling_factor);
+
+/* Bonded terms of the topology (frontend/buildTopology.cpp fills
+ * GenericTopology::bonds and ::angles from PSF/PAR): bond atoms, spring
+ * constant (kcal/mol/A^2), rest length (A); angle atoms (a2 is the vertex),
+ * force constant (kcal/mol/rad^2), rest angle (rad), Urey-Bradley constant
+ * and rest length.  Used by the "Bond" and "Angle" forces
+ * (forces/BondSystemForce.h:111-145, forces/AngleSystemForce.h:113-200). */
+int pm_set_bonded(pm_context* ctx, int nbonds, const int* bond_a, const int* bond_b,
+                  const double* bond_k, const double* bond_r0, int nangles, const int* angle_a,
+                  const int* angle_b, const int* angle_c, const double* angle_k,
+                  const double* angle_theta0, const double* angle_kub, const double* angle_rub);
+
+/* Periodic boundary conditions and cubic cell manager: cellBasisVector1-3,
+ * cellOrigin and cellsize (topology/PeriodicBoundaryConditions.cpp:34-77,
+ * topology/CubicCellManager.cpp:25-42).  Only orthogonal boxes (the reference
+ * PME requires them, NonbondedPMEwaldSystemForce.h:542-543). */
+int pm_set_boundary(pm_context* ctx, const double* e1, const double* e2, const double* e3,
+                    const double* origin, double cellsize);
+
+/* Create a force from its ProtoMol id and positional parameter values, e.g.
+ *   "LennardJones CoulombEwaldReal -algorithm NonbondedCutoff
+ *    -switchingFunction C2 -switchingFunction Cutoff"
+ *      values = {switchon, cutoff (C2), alpha, cutoff (Cutoff), algorithm cutoff}
+ *   "Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline"
+ *      values = {nx, ny, nz, cutoff, order, accuracy, alpha}   (alias "CoulombPME")
+ *   "Bond", "Angle"   (no values; the topology's bonded terms)
+ * Replaces ForceFactory::make + doMake (forces/NonbondedCutoffForce.h:123-129,
+ * forces/NonbondedPMEwaldSystemForce.h:1100-1139); the same checks, the same
+ * messages.  *handle identifies the force in later calls. */
+int pm_force_create(pm_context* ctx, const char* force_id, const double* values, int nvalues,
+                    int* handle);
+
+/* Force::uncache: drop cached precomputation (PME alpha/tables, cell plans). */
+int pm_force_uncache(pm_context* ctx, int handle);
+
+/* SystemForce::evaluate for a group of forces sharing one cell-list build
+ * (ForceGroup::evaluateSystemForces).  Host buffers; positions double[3n],
+ * forces double[3n] += , energies double[PM_ENERGY_COUNT] += .
+ * flags: PM_EVAL_VIRIAL to accumulate the atomic virial. */
+int pm_evaluate(pm_context* ctx, const int* handles, int nhandles, const double* positions,
+                double* forces, double* energies, int flags);
+
+/* Cell binning diagnostics of topology/buildCellLists.cpp:11-38 and the
+ * enumerator CellListEnumerator_periodicBoundaries.h: cell grid dims[3],
+ * cell of every atom, per-cell head/next linked lists in the reference's
+ * order, and per first-cell candidate / in-cutoff (r^2 <= pair_cutoff^2, not
+ * FULL-excluded) pair counts over the enumerator's cell pairs built from
+ * algorithm_cutoff.  Any output pointer may be NULL; head/cand/inrc need
+ * dims[0]*dims[1]*dims[2] entries (query with head == NULL first). */
+int pm_cell_diagnostics(pm_context* ctx, const double* positions, double algorithm_cutoff,
+                        double pair_cutoff, int* dims, int* cell_of_atom, int* head, int* next,
+                        long long* cand, long long* inrc);
+
+/* On-device MD driver (replaces LeapfrogIntegrator::run and the Langevin
+ * half kick).  dt in fs, temperature in K, gamma in 1/ps, seed for the
+ * Langevin noise.  Positions/velocities stay in HBM between calls. */
+int pm_md_setup(pm_context* ctx, const int* handles, int nhandles, int integrator, double dt,
+                double temperature, double gamma, unsigned long long seed);
+int pm_md_set_state(pm_context* ctx, const double* x, const double* v);
+/* LeapfrogIntegrator::run(nsteps): half kick + drift, forces, (kick + drift,
+ * forces) x (nsteps-1), half kick. */
+int pm_md_run(pm_context* ctx, int nsteps);
+/* Copy back x, v (double[3n], may be NULL) and the energies of the last force
+ * evaluation (double[PM_ENERGY_COUNT] overwritten, may be NULL). */
+int pm_md_get_state(pm_context* ctx, double* x, double* v, double* energies);
+
+/* Kernel statistics of the last evaluation (for roofline reporting):
+ * stats[0] pair-list entries (i-cluster x j-atom), stats[1] pairs evaluated
+ * (lanes issued), stats[2] clusters, stats[3] padded slots. */
+int pm_last_stats(pm_context* ctx, long long* stats, int nstats);
+
+/* Kernel timing with HIP events recorded on the context's stream around the
+ * kernel groups selected by mask (bit k = category k):
+ *   0 binning/slots, 1 pair lists, 2 pair kernel, 3 PME reciprocal
+ *   (spread, FFTs, influence function, gather), 4 PME correction,
+ *   5 bond/angle, 6 energy reduction, 7 position/velocity update.
+ * No reference counterpart (the reference's TimerStatistic wraps whole force
+ * evaluations, base/TimerStatistic.h:22-32).  pm_set_timing resets the
+ * totals; pm_get_timing returns total milliseconds and launch counts. */
+int pm_set_timing(pm_context* ctx, unsigned mask);
+int pm_get_timing(pm_context* ctx, double* ms, long long* count, int ncategories);
+
+/* Wait for all work queued by this context. */
+int pm_synchronize(pm_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

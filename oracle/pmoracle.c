@@ -1310,6 +1310,83 @@ int orc_hessian_vec(int n, const double* pos, const double* q, const int* type,
   return 0;
 }
 
+/* ------------------------------------------------------------- bonded */
+static double nrm(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+void orc_bonded(int n, const double* pos, const double* box, const double* origin, int nb,
+                const int* ba, const int* bb, const double* bk, const double* br0, int na,
+                const int* aa, const int* ab, const int* ac, const double* ak, const double* at0,
+                const double* akub, const double* arub, int which, int want_virial,
+                double* forces, double* energies) {
+  orc_pbc p;
+  int k, c;
+  (void)n;
+  pbc_set(&p, box, origin);
+  if (which & 1) {
+    /* BondSystemForce::calcBond (forces/BondSystemForce.h:111-145) */
+    for (k = 0; k < nb; k++) {
+      int a1 = ba[k], a2 = bb[k];
+      double r12[3], r2, r, dpotdr, f1[3];
+      pbc_min_diff(&p, pos + 3 * a2, pos + 3 * a1, r12, &r2);
+      r = nrm(r12);
+      dpotdr = 2.0 * bk[k] * (r - br0[k]);
+      for (c = 0; c < 3; c++) f1[c] = r12[c] * (-dpotdr / r);
+      for (c = 0; c < 3; c++) { forces[3 * a1 + c] += f1[c]; forces[3 * a2 + c] -= f1[c]; }
+      energies[ORC_E_BOND] += bk[k] * (r - br0[k]) * (r - br0[k]);
+      if (want_virial) add_virial(energies, f1, r12);
+    }
+  }
+  if (which & 2) {
+    /* AngleSystemForce::calcAngle (forces/AngleSystemForce.h:113-200) */
+    for (k = 0; k < na; k++) {
+      int a1 = aa[k], a2 = ab[k], a3 = ac[k];
+      double r12[3], r32[3], r13[3], t2, d12, d32, d13, cr[3], theta, st, ct, dpot;
+      double f1[3], f2[3], f3[3], eh, eu;
+      pbc_min_diff(&p, pos + 3 * a2, pos + 3 * a1, r12, &t2);
+      pbc_min_diff(&p, pos + 3 * a2, pos + 3 * a3, r32, &t2);
+      pbc_min_diff(&p, pos + 3 * a3, pos + 3 * a1, r13, &t2);
+      d12 = nrm(r12); d32 = nrm(r32); d13 = nrm(r13);
+      cr[0] = r12[1] * r32[2] - r12[2] * r32[1];
+      cr[1] = r12[2] * r32[0] - r12[0] * r32[2];
+      cr[2] = r12[0] * r32[1] - r12[1] * r32[0];
+      theta = atan2(nrm(cr), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
+      st = sin(theta); ct = cos(theta);
+      dpot = 2.0 * ak[k] * (theta - at0[k]);
+      for (c = 0; c < 3; c++) {
+        double dr12 = r12[c] / d12, dr32 = r32[c] / d32, dr13 = r13[c] / d13;
+        double dth1 = (dr12 * ct - dr32) / (st * d12);
+        double dth3 = (dr32 * ct - dr12) / (st * d32);
+        double ub1 = dr13 * (2.0 * akub[k] * (d13 - arub[k]));
+        f1[c] = -dth1 * dpot - ub1;
+        f3[c] = -dth3 * dpot - (-ub1);
+        f2[c] = -f1[c] - f3[c];
+      }
+      for (c = 0; c < 3; c++) {
+        forces[3 * a1 + c] += f1[c];
+        forces[3 * a2 + c] += f2[c];
+        forces[3 * a3 + c] += f3[c];
+      }
+      eh = ak[k] * (theta - at0[k]) * (theta - at0[k]);
+      eu = akub[k] * (d13 - arub[k]) * (d13 - arub[k]);
+      energies[ORC_E_ANGLE] += eh + eu;
+      if (want_virial) {
+        double xy = f1[0] * r12[1] + f3[0] * r32[1];
+        double xz = f1[0] * r12[2] + f3[0] * r32[2];
+        double yz = f1[1] * r12[2] + f3[1] * r32[2];
+        energies[ORC_E_VXX] += f1[0] * r12[0] + f3[0] * r32[0];
+        energies[ORC_E_VXY] += xy;
+        energies[ORC_E_VXZ] += xz;
+        energies[ORC_E_VYX] += xy;
+        energies[ORC_E_VYY] += f1[1] * r12[1] + f3[1] * r32[1];
+        energies[ORC_E_VYZ] += yz;
+        energies[ORC_E_VZX] += xz;
+        energies[ORC_E_VZY] += yz;
+        energies[ORC_E_VZZ] += f1[2] * r12[2] + f3[2] * r32[2];
+      }
+    }
+  }
+}
+
 double orc_rem_delta(double ti, double tj, double ei, double ej) {
   /* rem.cpp:941-946 */
   return (1 / ORC_BOLTZMANN / ti - 1 / ORC_BOLTZMANN / tj) * (ej - ei);

@@ -43,11 +43,12 @@ enum { ORC_POT_LJ = 0, ORC_POT_COULOMB = 1, ORC_POT_EWALDREAL = 2 };
  * C1SwitchingFunction.h, C2SwitchingFunction.h) */
 enum { ORC_SWF_CUTOFF = 0, ORC_SWF_C1 = 1, ORC_SWF_C2 = 2 };
 
-/* energies[] layout: ScalarStructure indices COULOMB, LENNARDJONES then the
- * nine atomic virial entries (base/ScalarStructure.h:25-45) */
+/* energies[] layout: the ScalarStructure prefix COULOMB, LENNARDJONES, BOND,
+ * ANGLE, DIHEDRAL, IMPROPER, OTHER, then the nine atomic virial entries
+ * (base/ScalarStructure.h:25-45) */
 enum {
-  ORC_E_COULOMB = 0, ORC_E_LJ = 1,
-  ORC_E_VXX = 2, ORC_E_VXY, ORC_E_VXZ, ORC_E_VYX, ORC_E_VYY, ORC_E_VYZ,
+  ORC_E_COULOMB = 0, ORC_E_LJ = 1, ORC_E_BOND = 2, ORC_E_ANGLE = 3,
+  ORC_E_VXX = 7, ORC_E_VXY, ORC_E_VXZ, ORC_E_VYX, ORC_E_VYY, ORC_E_VYZ,
   ORC_E_VZX, ORC_E_VZY, ORC_E_VZZ,
   ORC_E_COUNT
 };
@@ -152,6 +153,14 @@ int orc_hessian_vec(int n, const double* pos, const double* q, const int* type,
                     const int* ex_b, const int* ex_cls, double csf,
                     const double* box, const double* origin, int nterms,
                     const orc_term* terms, const double* v, double* hv);
+
+/* Bond and angle system forces (forces/BondSystemForce.h:111-145,
+ * forces/AngleSystemForce.h:113-200).  which: bit0 bonds, bit1 angles. */
+void orc_bonded(int n, const double* pos, const double* box, const double* origin, int nb,
+                const int* ba, const int* bb, const double* bk, const double* br0, int na,
+                const int* aa, const int* ab, const int* ac, const double* ak, const double* at0,
+                const double* akub, const double* arub, int which, int want_virial,
+                double* forces, double* energies);
 
 /* Replica-exchange acceptance exponent (applications/rem-app/rem.cpp:941-946). */
 double orc_rem_delta(double ti, double tj, double ei, double ej);

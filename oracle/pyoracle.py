@@ -20,8 +20,9 @@ REF_LIB = os.path.join(HERE, "_ref", "libpmref.so")
 
 POT_LJ, POT_COULOMB, POT_EWALDREAL = 0, 1, 2
 SWF_CUTOFF, SWF_C1, SWF_C2 = 0, 1, 2
-E_COULOMB, E_LJ = 0, 1
-E_COUNT = 11
+E_COULOMB, E_LJ, E_BOND, E_ANGLE = 0, 1, 2, 3
+E_VIRIAL = 7
+E_COUNT = 16
 
 
 class Term(C.Structure):
@@ -65,6 +66,7 @@ class Oracle:
         self._pme = getattr(self.lib, p + "pme")
         self._bspline = getattr(self.lib, p + "bspline")
         self._hess = getattr(self.lib, p + "hessian_vec")
+        self._bonded = getattr(self.lib, p + "bonded")
         self._cutoff_omp = getattr(self.lib, "orc_cutoff_omp", None) if kind == "port" else None
         self._delta = getattr(self.lib, "orc_delta_list", None) if kind == "port" else None
         self._fft = getattr(self.lib, "orc_fft3d", None) if kind == "port" else None
@@ -206,6 +208,28 @@ class Oracle:
 
     def rem_delta(self, ti, tj, ei, ej):
         return self._rem(C.c_double(ti), C.c_double(tj), C.c_double(ei), C.c_double(ej))
+
+    # --------------------------------------------------------------- bonded
+    def bonded(self, sys, which=3, virial=False):
+        keep = []
+
+        def d(a):
+            a2, p = _d(a)
+            keep.append(a2)
+            return p
+
+        def i(a):
+            a2, p = _i(a)
+            keep.append(a2)
+            return p
+
+        f = np.zeros((sys.n, 3))
+        e = np.zeros(E_COUNT)
+        r = self._bonded(sys.n, d(sys.pos), d(sys.box), d(sys.origin), len(sys.bond_a), i(sys.bond_a),
+                         i(sys.bond_b), d(sys.bond_k), d(sys.bond_r0), len(sys.angle_a), i(sys.angle_a),
+                         i(sys.angle_b), i(sys.angle_c), d(sys.angle_k), d(sys.angle_t0), d(sys.angle_kub),
+                         d(sys.angle_rub), which, int(virial), f.ctypes.data_as(_D), e.ctypes.data_as(_D))
+        return dict(forces=f, energies=e)
 
     # -------------------------------------------------------------- Hessian
     def hessian_vec(self, sys, terms, v, cellsize=4.0):

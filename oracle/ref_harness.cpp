@@ -33,6 +33,8 @@ using namespace ProtoMol::Report;
 #include "Vector3DBlock.h"
 #include "ForceGroup.h"
 #include "Hessian.h"
+#include "BondSystemForce.h"
+#include "AngleSystemForce.h"
 
 #include <vector>
 #include <map>
@@ -114,12 +116,14 @@ Topo* makeTopo(int n, const double* pos, const double* q, const int* type, int n
 void readEnergies(const ScalarStructure& e, double* energies) {
   energies[0] += e[ScalarStructure::COULOMB];
   energies[1] += e[ScalarStructure::LENNARDJONES];
+  energies[2] += e[ScalarStructure::BOND];
+  energies[3] += e[ScalarStructure::ANGLE];
   const ScalarStructure::Index v[9] = {ScalarStructure::VIRIALXX, ScalarStructure::VIRIALXY,
                                        ScalarStructure::VIRIALXZ, ScalarStructure::VIRIALYX,
                                        ScalarStructure::VIRIALYY, ScalarStructure::VIRIALYZ,
                                        ScalarStructure::VIRIALZX, ScalarStructure::VIRIALZY,
                                        ScalarStructure::VIRIALZZ};
-  for (int k = 0; k < 9; k++) energies[2 + k] += e[v[k]];
+  for (int k = 0; k < 9; k++) energies[7 + k] += e[v[k]];
 }
 
 // Switching-function and potential objects from a Term, as
@@ -349,6 +353,41 @@ int ref_pme(int n, const double* pos, const double* q, const int* molecule, int 
   }
   readEnergies(e, energies);
   delete f;
+  delete t;
+  return 0;
+}
+
+// BondSystemForce / AngleSystemForce over topo->bonds / topo->angles
+// (forces/BondSystemForce.h:99-145, forces/AngleSystemForce.h:98-200).
+int ref_bonded(int n, const double* pos, const double* box, const double* origin, int nb, const int* ba,
+               const int* bb, const double* bk, const double* br0, int na, const int* aa, const int* ab,
+               const int* ac, const double* ak, const double* at0, const double* akub, const double* arub,
+               int which, int wantVirial, double* forces, double* energies) {
+  std::vector<double> q(n, 0.0), lj(1, 0.0);
+  std::vector<int> type(n, 0), mol(n, 0);
+  Vector3DBlock positions;
+  Topo* t = makeTopo(n, pos, &q[0], &type[0], 1, &lj[0], &lj[0], &lj[0], &lj[0], &mol[0], 0, 0, 0, 0, 1.0,
+                     box, origin, 5.0, positions);
+  for (int k = 0; k < nb; k++) t->bonds.push_back(Bond(ba[k], bb[k], bk[k], br0[k]));
+  for (int k = 0; k < na; k++) {
+    Angle a;
+    a.atom1 = aa[k]; a.atom2 = ab[k]; a.atom3 = ac[k];
+    a.forceConstant = ak[k]; a.restAngle = at0[k];
+    a.ureyBradleyConstant = akub[k]; a.ureyBradleyRestLength = arub[k];
+    t->angles.push_back(a);
+  }
+  Vector3DBlock fo;
+  fo.resize(n);
+  fo.zero();
+  ScalarStructure e;
+  e.clear();
+  e.virial(wantVirial != 0);
+  if (which & 1) { BondSystemForce<PeriodicBoundaryConditions> f; f.evaluate(t, &positions, &fo, &e); }
+  if (which & 2) { AngleSystemForce<PeriodicBoundaryConditions> f; f.evaluate(t, &positions, &fo, &e); }
+  for (int i = 0; i < n; i++) {
+    forces[3 * i] += fo[i].x; forces[3 * i + 1] += fo[i].y; forces[3 * i + 2] += fo[i].z;
+  }
+  readEnergies(e, energies);
   delete t;
   return 0;
 }

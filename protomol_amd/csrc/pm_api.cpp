@@ -1,0 +1,1473 @@
+// pm_api.cpp -- host side of the C ABI declared in include/pmgpu.h.
+//
+// Owns the device buffers, restates the reference's host-side set-up
+// arithmetic (periodic box, cell grid, enumerator delta list, Ewald alpha,
+// PME tables) and queues the gfx950 kernels on one HIP stream per context.
+#include "pmgpu.h"
+#include "pm_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+const double kEpsilon = 1.0e-14;                 // base/pmconstants.cpp:45
+const int kMaxInt2 = 2147483647 / 2;             // base/pmconstants.cpp:41
+const double kPi = 3.14159265358979323846;
+const double kInvTimeFactor = 0.02045482828087295;
+
+struct Term {
+  int pot;      // 0 LJ, 1 Coulomb, 2 CoulombEwaldReal
+  int swf;      // PM_SWF_C1, PM_SWF_C2, PM_SWF_CUTOFF
+  double alpha = 0, switchon = 0, cutoff = 0;
+};
+
+struct ForceDesc {
+  bool pme = false;
+  int bonded = 0;   // 1 "Bond", 2 "Angle" (forces/BondSystemForce.h, AngleSystemForce.h)
+  // nonbonded cutoff
+  std::vector<Term> terms;
+  double algo_cutoff = 0;
+  // PME
+  int nx = 0, ny = 0, nz = 0, order = 4;
+  double rc = 0, accuracy = 1e-6, alpha_in = -1;
+  bool real = false, recip = false, corr = false;
+  // cached PME state (NonbondedPMEwaldSystemForce::initialize, :499-648)
+  bool cached = false;
+  double alpha = 0;
+  std::string id;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct pm_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int precision = PM_PRECISION_MIXED;
+
+  // topology
+  int n = 0;
+  int ntypes = 0;
+  std::vector<double> q, mass, ljA, ljB, ljA14, ljB14;
+  std::vector<int> type, molecule;
+  std::vector<int> ex_a, ex_b, ex_c;       // as given
+  std::vector<int> tab_a, tab_b, tab_c;    // ExclusionTable::getTable order
+  int max_delta = 0;
+  double csf = 1.0;
+  bool have_topo = false;
+
+  // boundary
+  PmBox box{};
+  double cellsize = 0;
+  bool have_box = false;
+  double realcell[3] = {0, 0, 0};
+  int ncells = 0;
+
+  std::vector<ForceDesc> forces;
+
+  // bonded terms
+  int nbonds = 0, nangles = 0;
+  int *d_b_a = nullptr, *d_b_b = nullptr, *d_b_row = nullptr, *d_b_idx = nullptr;
+  double *d_b_k = nullptr, *d_b_r0 = nullptr;
+  int *d_a_a = nullptr, *d_a_b = nullptr, *d_a_c = nullptr, *d_a_row = nullptr, *d_a_idx = nullptr;
+  double *d_a_k = nullptr, *d_a_t0 = nullptr, *d_a_kub = nullptr, *d_a_rub = nullptr;
+
+  // device topology
+  double *d_q = nullptr, *d_mass = nullptr;
+  int *d_type = nullptr, *d_mol = nullptr;
+  int *d_ex_row = nullptr, *d_ex_partner = nullptr, *d_ex_cls = nullptr;      // lookup CSR
+  int *d_cx_row = nullptr, *d_cx_partner = nullptr, *d_cx_cls = nullptr;      // correction CSR
+  int *d_tab_a = nullptr, *d_tab_b = nullptr, *d_tab_c = nullptr;
+  float4* d_lj_f = nullptr;
+  double4* d_lj_d = nullptr;
+
+  // state
+  double *d_pos = nullptr, *d_vel = nullptr;
+  long long* d_force = nullptr;
+  double* d_energy = nullptr;     // [PM_NE]
+  double* d_scratch3 = nullptr;   // [3n] double conversions
+
+  // cells and slots
+  int *d_cell_of = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_slot = nullptr,
+      *d_cell_cursor = nullptr, *d_cell_atoms = nullptr, *d_cluster_cell = nullptr, *d_counts = nullptr;
+  float* d_key = nullptr;
+  int4* d_slot_pos = nullptr;
+  longlong4* d_slot_pos64 = nullptr;
+  int *d_slot_meta = nullptr, *d_atom_of_slot = nullptr;
+  int ncl_max = 0, nslot_max = 0;
+  int cells_alloc = 0;
+
+  // stencils keyed by algorithm cutoff
+  std::map<double, std::pair<int4*, int>> stencils;
+  std::map<double, std::pair<int4*, int>> half_stencils;
+
+  // lists
+  int *d_jlist = nullptr, *d_jcount = nullptr, *d_overflow = nullptr;
+  long long jlist_alloc = 0;
+  int lcap = 0;
+  unsigned long long* d_stats = nullptr;
+
+  // energy partial records
+  double* d_epart = nullptr;
+  long long epart_alloc = 0;
+
+  // PME buffers (one grid size at a time)
+  int pme_nx = 0, pme_ny = 0, pme_nz = 0, pme_prec = -1;
+  unsigned long long* d_grid_fix = nullptr;
+  void* d_grid_r = nullptr;
+  void* d_grid_c = nullptr;
+  double *d_bmod = nullptr, *d_expt = nullptr, *d_tw = nullptr;
+  PmFftPlan plans[3];
+  double pme_alpha_tables = -1;
+
+  // MD
+  std::vector<int> md_handles;
+  int md_integrator = PM_INTEGRATOR_LEAPFROG;
+  double md_dt = 1.0, md_temp = 300.0, md_gamma = 0.0;
+  unsigned long long md_seed = 0, md_counter = 0;
+  bool md_forces_valid = false;
+
+  long long last_stats[4] = {0, 0, 0, 0};
+
+  // kernel timing (HIP events on the context's stream), per category
+  unsigned timing_mask = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+  std::vector<hipEvent_t> ev_pool;
+  double t_ms[PM_TIMING_CATEGORIES] = {0};
+  long long t_count[PM_TIMING_CATEGORIES] = {0};
+};
+
+namespace {
+
+int fail(pm_context* c, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHECK(ctx, call)                                                                      \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(ctx, PM_ERR_DEVICE, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_),       \
+                  __FILE__, __LINE__, #call);                                                    \
+  } while (0)
+
+template <typename T>
+int dalloc(pm_context* c, T** p, size_t count) {
+  if (*p) { hipFree(*p); *p = nullptr; }
+  if (count == 0) count = 1;
+  HIPCHECK(c, hipMalloc((void**)p, count * sizeof(T)));
+  return PM_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+template <typename T>
+int upload(pm_context* c, T** p, const std::vector<T>& v) {
+  int r = dalloc(c, p, v.size());
+  if (r) return r;
+  if (!v.empty()) HIPCHECK(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PM_OK;
+}
+
+std::string fmtnum(double v) {
+  std::ostringstream os;
+  os << v;
+  return os.str();
+}
+
+// ---------------------------------------------------------------- timing
+hipEvent_t take_event(pm_context* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+void harvest_timing(pm_context* c) {
+  for (auto& p : c->ev_open) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      c->t_ms[p.first] += ms;
+      c->t_count[p.first] += 1;
+    }
+    c->ev_pool.push_back(p.second.first);
+    c->ev_pool.push_back(p.second.second);
+  }
+  c->ev_open.clear();
+}
+
+struct TimedScope {
+  pm_context* c;
+  int cat;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedScope(pm_context* ctx, int category) : c(ctx), cat(category) {
+    if (c->timing_mask & (1u << cat)) {
+      if (c->ev_open.size() > 8192) {
+        hipStreamSynchronize(c->stream);
+        harvest_timing(c);
+      }
+      a = take_event(c);
+      b = take_event(c);
+      hipEventRecord(a, c->stream);
+    }
+  }
+  ~TimedScope() {
+    if (a) {
+      hipEventRecord(b, c->stream);
+      c->ev_open.push_back(std::make_pair(cat, std::make_pair(a, b)));
+    }
+  }
+};
+
+// ----------------------------------------------------------- cell helpers
+// ArrayCellListStructure::basisCell (topology/ArrayCellListStructure.h:117-122)
+int basis(int d, int n) {
+  int maxn = kMaxInt2 - kMaxInt2 % n;
+  int n1 = -n / 2;
+  int maxn1 = maxn - n1;
+  return (d + maxn1) % n + n1;
+}
+
+// CellListEnumerator<PBC,Cubic>::initialize delta list
+// (topology/CellListEnumerator_periodicBoundaries.h:37-108): every cell delta
+// whose minimum gap is below the cutoff, mapped to its basis cell,
+// deduplicated, zero excluded, sorted lexicographically.
+std::vector<int4> delta_list(const pm_context* c, double cutoff) {
+  const double cutoff2 = cutoff * cutoff;
+  const double* rc = c->realcell;
+  const int* n = c->box.n;
+  int nx = (int)(cutoff / rc[0] + 1.0 + kEpsilon);
+  int ny = (int)(cutoff / rc[1] + 1.0 + kEpsilon);
+  int nz = (int)(cutoff / rc[2] + 1.0 + kEpsilon);
+  int n0 = std::min(nx, n[0]), n1 = std::min(ny, n[1]), n2 = std::min(nz, n[2]);
+  double xx = rc[0] * rc[0], yy = rc[1] * rc[1], zz = rc[2] * rc[2];
+  std::set<std::tuple<int, int, int>> seen;
+  seen.insert(std::make_tuple(0, 0, 0));
+  std::vector<std::tuple<int, int, int>> out;
+  for (int k = -n0; k <= n0; k++) {
+    int x = std::max(std::abs(k) - 1, 0);
+    double d0 = x * x * xx;
+    for (int l = -n1; l <= n1; l++) {
+      int y = std::max(std::abs(l) - 1, 0);
+      double d1 = d0 + y * y * yy;
+      for (int m = -n2; m <= n2; m++) {
+        int z = std::max(std::abs(m) - 1, 0);
+        if (d1 + z * z * zz < cutoff2) {
+          auto b = std::make_tuple(basis(k, n[0]), basis(l, n[1]), basis(m, n[2]));
+          if (!seen.count(b)) { seen.insert(b); out.push_back(b); }
+        }
+      }
+    }
+  }
+  std::sort(out.begin(), out.end());
+  std::vector<int4> r;
+  for (auto& t : out) r.push_back(make_int4(std::get<0>(t), std::get<1>(t), std::get<2>(t), 0));
+  return r;
+}
+
+// full stencil: self, the deltas and their negations (basis form, deduplicated)
+std::vector<int4> full_stencil(const pm_context* c, double cutoff) {
+  std::vector<int4> h = delta_list(c, cutoff);
+  std::set<std::tuple<int, int, int>> seen;
+  std::vector<int4> r;
+  auto add = [&](int a, int b, int d) {
+    auto t = std::make_tuple(basis(a, c->box.n[0]), basis(b, c->box.n[1]), basis(d, c->box.n[2]));
+    if (seen.count(t)) return;
+    seen.insert(t);
+    r.push_back(make_int4(std::get<0>(t), std::get<1>(t), std::get<2>(t), 0));
+  };
+  add(0, 0, 0);
+  for (auto& d : h) { add(d.x, d.y, d.z); add(-d.x, -d.y, -d.z); }
+  return r;
+}
+
+int get_stencil(pm_context* c, double cutoff, bool half, int4** dptr, int* count) {
+  auto& m = half ? c->half_stencils : c->stencils;
+  auto it = m.find(cutoff);
+  if (it == m.end()) {
+    std::vector<int4> v = half ? delta_list(c, cutoff) : full_stencil(c, cutoff);
+    int4* d = nullptr;
+    int r = dalloc(c, &d, v.size());
+    if (r) return r;
+    if (!v.empty()) HIPCHECK(c, hipMemcpy(d, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice));
+    it = m.emplace(cutoff, std::make_pair(d, (int)v.size())).first;
+  }
+  *dptr = it->second.first;
+  *count = it->second.second;
+  return PM_OK;
+}
+
+void free_stencils(pm_context* c) {
+  for (auto& kv : c->stencils) hipFree(kv.second.first);
+  for (auto& kv : c->half_stencils) hipFree(kv.second.first);
+  c->stencils.clear();
+  c->half_stencils.clear();
+}
+
+// ------------------------------------------------------------ allocation
+int ensure_cells(pm_context* c) {
+  if (c->cells_alloc == c->ncells && c->d_cell_count && c->d_slot_pos) return PM_OK;
+  int n = c->n, nc = c->ncells;
+  int r;
+  if ((r = dalloc(c, &c->d_cell_of, n))) return r;
+  if ((r = dalloc(c, &c->d_key, n))) return r;
+  if ((r = dalloc(c, &c->d_cell_count, nc))) return r;
+  if ((r = dalloc(c, &c->d_cell_start, nc))) return r;
+  if ((r = dalloc(c, &c->d_cell_slot, nc))) return r;
+  if ((r = dalloc(c, &c->d_cell_cursor, nc))) return r;
+  if ((r = dalloc(c, &c->d_cell_atoms, n))) return r;
+  if ((r = dalloc(c, &c->d_counts, 4))) return r;
+  long long slots = (long long)n + (long long)(PM_M - 1) * nc;
+  c->ncl_max = (int)((slots + PM_M - 1) / PM_M);
+  c->nslot_max = c->ncl_max * PM_M;
+  if ((r = dalloc(c, &c->d_cluster_cell, c->ncl_max))) return r;
+  if ((r = dalloc(c, &c->d_slot_pos, (size_t)c->nslot_max + 1))) return r;
+  if ((r = dalloc(c, &c->d_slot_pos64, (size_t)c->nslot_max + 1))) return r;
+  if ((r = dalloc(c, &c->d_slot_meta, (size_t)c->nslot_max + 1))) return r;
+  if ((r = dalloc(c, &c->d_atom_of_slot, (size_t)c->nslot_max + 1))) return r;
+  HIPCHECK(c, hipMemset(c->d_counts, 0, 4 * sizeof(int)));
+  if (!c->d_overflow) {
+    if ((r = dalloc(c, &c->d_overflow, 1))) return r;
+    HIPCHECK(c, hipMemset(c->d_overflow, 0, sizeof(int)));
+  }
+  if (!c->d_stats) {
+    if ((r = dalloc(c, &c->d_stats, 2))) return r;
+    HIPCHECK(c, hipMemset(c->d_stats, 0, 2 * sizeof(unsigned long long)));
+  }
+  c->cells_alloc = nc;
+  c->jlist_alloc = 0;
+  return PM_OK;
+}
+
+// list capacity per cluster: atoms within the list cutoff of a cell-sized box
+int list_capacity(const pm_context* c, double cut) {
+  double V = c->box.L[0] * c->box.L[1] * c->box.L[2];
+  double rho = c->n / V;
+  double a = std::max(c->realcell[0], std::max(c->realcell[1], c->realcell[2]));
+  double ext = a + 2.0 * cut;
+  double est = rho * ext * ext * ext;
+  long long cap = (long long)(1.5 * est) + 256;
+  cap = std::min(cap, (long long)c->n + 64);
+  cap = (cap + 63) / 64 * 64;
+  return (int)std::max(cap, 64LL);
+}
+
+int ensure_lists(pm_context* c, int lcap) {
+  long long need = (long long)c->ncl_max * lcap;
+  if (need <= c->jlist_alloc && c->d_jlist) { c->lcap = lcap; return PM_OK; }
+  int r;
+  if ((r = dalloc(c, &c->d_jlist, (size_t)need))) return r;
+  if ((r = dalloc(c, &c->d_jcount, c->ncl_max))) return r;
+  c->jlist_alloc = need;
+  c->lcap = lcap;
+  return PM_OK;
+}
+
+int ensure_epart(pm_context* c, long long records) {
+  if (records <= c->epart_alloc && c->d_epart) return PM_OK;
+  int r = dalloc(c, &c->d_epart, (size_t)records * PM_NE);
+  if (r) return r;
+  c->epart_alloc = records;
+  return PM_OK;
+}
+
+// ---------------------------------------------------------------- PME set-up
+// erfc bisection of NonbondedPMEwaldSystemForce::initialize (:564-586)
+double pme_alpha(double rc, double accuracy) {
+  double alpha = 1.0;
+  while (std::erfc(alpha * rc) / rc >= accuracy) alpha *= 2.0;
+  double low = 0.0, high = alpha;
+  for (unsigned int i = 0; i < 100; ++i) {
+    alpha = 0.5 * (low + high);
+    if (std::erfc(alpha * rc) / rc >= accuracy) low = alpha; else high = alpha;
+  }
+  return alpha;
+}
+
+// BSpline::set at w = 0 (base/BSpline.cpp:64-117)
+void bspline_host(int order, double w, double* th) {
+  std::vector<double> t(order), d(order);
+  if (order == 2) { th[1] = w; th[0] = 1 - w; return; }
+  t[order - 1] = 0.0; t[1] = w; t[0] = 1.0 - w;
+  for (int k = 2; k < order - 1; k++) {
+    double div = 1.0 / k;
+    t[k] = div * w * t[k - 1];
+    for (int j = 1; j < k; j++) t[k - j] = div * ((w + j) * t[k - j - 1] + (k + 1 - j - w) * t[k - j]);
+    t[0] = div * (1 - w) * t[0];
+  }
+  double div = 1.0 / (order - 1.0);
+  t[order - 1] = div * w * t[order - 2];
+  for (int j = 1; j < order - 1; j++)
+    t[order - 1 - j] = div * ((w + j) * t[order - j - 2] + (order - j - w) * t[order - 1 - j]);
+  t[0] = div * (1 - w) * t[0];
+  for (int k = 0; k < order; k++) th[k] = t[k];
+}
+
+// Grid::dftmod (base/Grid.h:650-665)
+void dftmod(int order, int n, const double* theta, double* mod) {
+  for (int i = 0; i < n; i++) {
+    double sc = 0.0, ss = 0.0;
+    for (int j = 0; j < order; j++) {
+      double x = kPi * 2.0 * i * j / (double)n;
+      sc += theta[j] * std::cos(x);
+      ss += theta[j] * std::sin(x);
+    }
+    mod[i] = 1.0 / (sc * sc + ss * ss);
+  }
+}
+
+bool fft_plan(int n, PmFftPlan& p) {
+  p.n = n;
+  p.nst = 0;
+  int m = n;
+  while (m % 4 == 0 && m != 8) { p.rad[p.nst++] = 4; m /= 4; if (p.nst >= PM_FFT_MAXST) return false; }
+  while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; if (p.nst >= PM_FFT_MAXST) return false; }
+  while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; if (p.nst >= PM_FFT_MAXST) return false; }
+  while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; if (p.nst >= PM_FFT_MAXST) return false; }
+  return m == 1;
+}
+
+int ensure_pme(pm_context* c, ForceDesc& f) {
+  if (!f.cached) {
+    f.alpha = f.alpha_in <= 0.0 ? pme_alpha(f.rc, f.accuracy) : f.alpha_in;
+    f.cached = true;
+  }
+  const int nx = f.nx, ny = f.ny, nz = f.nz, nzc = nz / 2 + 1;
+  const bool same_grid = c->pme_nx == nx && c->pme_ny == ny && c->pme_nz == nz && c->pme_prec == c->precision;
+  if (!f.recip) return PM_OK;
+  if (!same_grid) {
+    if (!fft_plan(nx, c->plans[0]) || !fft_plan(ny, c->plans[1]) || !fft_plan(nz, c->plans[2]))
+      return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: the GPU FFT handles lengths with factors 2, 3 and 5 only", nx, ny, nz);
+    size_t ng = (size_t)nx * ny * nz, ngc = (size_t)nx * ny * nzc;
+    size_t er = c->precision == PM_PRECISION_DOUBLE ? sizeof(double) : sizeof(float);
+    int r;
+    if ((r = dalloc(c, &c->d_grid_fix, ng))) return r;
+    HIPCHECK(c, hipMemset(c->d_grid_fix, 0, ng * sizeof(unsigned long long)));
+    if (c->d_grid_r) hipFree(c->d_grid_r);
+    if (c->d_grid_c) hipFree(c->d_grid_c);
+    HIPCHECK(c, hipMalloc(&c->d_grid_r, ng * er));
+    HIPCHECK(c, hipMalloc(&c->d_grid_c, ngc * er * 2));
+    // twiddles exp(-2 pi i m / n) for the three lengths
+    std::vector<double> tw;
+    int off = 0;
+    for (int a = 0; a < 3; a++) {
+      int len = c->plans[a].n;
+      c->plans[a].tw = off;
+      for (int m = 0; m < len; m++) {
+        double ang = -2.0 * kPi * (double)m / len;
+        tw.push_back(std::cos(ang));
+        tw.push_back(std::sin(ang));
+      }
+      off += len;
+    }
+    if ((r = upload(c, &c->d_tw, tw))) return r;
+    c->pme_nx = nx; c->pme_ny = ny; c->pme_nz = nz; c->pme_prec = c->precision;
+    c->pme_alpha_tables = -1;
+  }
+  if (c->pme_alpha_tables != f.alpha) {
+    // Grid::initialize (base/Grid.h:557-648)
+    const double lx = c->box.L[0], ly = c->box.L[1], lz = c->box.L[2];
+    const double fac = kPi * kPi / (f.alpha * f.alpha);
+    std::vector<double> bm(nx + ny + nz), ex(nx + ny + nzc);
+    std::vector<double> th(f.order);
+    bspline_host(f.order, 0.0, th.data());
+    dftmod(f.order, nx, th.data(), bm.data());
+    dftmod(f.order, ny, th.data(), bm.data() + nx);
+    dftmod(f.order, nz, th.data(), bm.data() + nx + ny);
+    const double wr = 1.0 / lx, lr = 1.0 / ly, hr = 1.0 / lz;
+    for (int i = 0; i < nx; i++) { int i0 = i <= nx / 2 ? i : i - nx; double t = i0 * wr; ex[i] = std::exp(-fac * (t * t)); }
+    for (int j = 0; j < ny; j++) { int j0 = j <= ny / 2 ? j : j - ny; double t = j0 * lr; ex[nx + j] = std::exp(-fac * (t * t)); }
+    for (int k = 0; k < nzc; k++) { double t = k * hr; ex[nx + ny + k] = std::exp(-fac * (t * t)); }
+    int r;
+    if ((r = upload(c, &c->d_bmod, bm))) return r;
+    if ((r = upload(c, &c->d_expt, ex))) return r;
+    c->pme_alpha_tables = f.alpha;
+  }
+  return PM_OK;
+}
+
+// ------------------------------------------------------------ force parsing
+std::vector<std::string> tokens(const std::string& s) {
+  std::istringstream is(s);
+  std::vector<std::string> t;
+  std::string w;
+  while (is >> w) t.push_back(w);
+  return t;
+}
+
+bool ieq(const std::string& a, const std::string& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); i++)
+    if (tolower((unsigned char)a[i]) != tolower((unsigned char)b[i])) return false;
+  return true;
+}
+
+int parse_force(pm_context* c, const char* id_c, const double* v, int nv, ForceDesc& f) {
+  std::string id(id_c ? id_c : "");
+  std::vector<std::string> t = tokens(id);
+  if (t.size() == 1 && (ieq(t[0], "Bond") || ieq(t[0], "Angle"))) {
+    if (nv != 0) return fail(c, PM_ERR_ARGUMENT, "force '%s' takes no parameters", id.c_str());
+    f.bonded = ieq(t[0], "Bond") ? 1 : 2;
+    f.id = id;
+    return PM_OK;
+  }
+  if (t.size() == 1 && ieq(t[0], "CoulombPME")) {
+    // alias of the full PME force (registerForceExemplarsFastElectrostatic.cpp:50)
+    t = tokens("Coulomb -algorithm PMEwald -real -reciprocal -correction -interpolation BSpline");
+  }
+  std::vector<int> pots;
+  std::vector<int> swfs;
+  std::string algo;
+  for (size_t i = 0; i < t.size(); i++) {
+    const std::string& w = t[i];
+    if (ieq(w, "LennardJones")) pots.push_back(0);
+    else if (ieq(w, "Coulomb")) pots.push_back(1);
+    else if (ieq(w, "CoulombEwaldReal")) pots.push_back(2);
+    else if (ieq(w, "-algorithm") && i + 1 < t.size()) algo = t[++i];
+    else if (ieq(w, "-switchingFunction") && i + 1 < t.size()) {
+      const std::string& s = t[++i];
+      if (ieq(s, "C1")) swfs.push_back(PM_SWF_C1);
+      else if (ieq(s, "C2")) swfs.push_back(PM_SWF_C2);
+      else if (ieq(s, "Cutoff")) swfs.push_back(PM_SWF_CUTOFF);
+      else return fail(c, PM_ERR_UNSUPPORTED, "Force '%s': switching function %s is not implemented on the GPU", id.c_str(), s.c_str());
+    } else if (ieq(w, "-real")) f.real = true;
+    else if (ieq(w, "-reciprocal")) f.recip = true;
+    else if (ieq(w, "-correction")) f.corr = true;
+    else if (ieq(w, "-interpolation") && i + 1 < t.size()) {
+      if (!ieq(t[++i], "BSpline"))
+        return fail(c, PM_ERR_UNSUPPORTED, "Force '%s': only BSpline interpolation is implemented on the GPU", id.c_str());
+    } else {
+      return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+    }
+  }
+  f.id = id;
+  if (ieq(algo, "PMEwald")) {
+    // NonbondedPMEwaldSystemForce::doMake (:1100-1139), periodic: 7 values
+    if (pots.size() != 1 || pots[0] != 1 || !swfs.empty())
+      return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+    if (!f.real && !f.recip && !f.corr)
+      return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+    if (f.real && !f.recip && f.corr)
+      ;  // registered (true,false,true)
+    if (nv != 7) return fail(c, PM_ERR_ARGUMENT, "force PMEwald : expected 7 parameter values, got %d.", nv);
+    f.pme = true;
+    f.nx = (int)v[0]; f.ny = (int)v[1]; f.nz = (int)v[2];
+    f.rc = v[3]; f.order = (int)v[4]; f.accuracy = v[5]; f.alpha_in = v[6];
+    std::string err;
+    const std::string kw = "PMEwald";
+    if (f.order < 2 || (f.order % 2) != 0) err += kw + " order (=" + fmtnum(v[4]) + ") > 1 and even.";
+    if (f.nx < f.order || f.ny < f.order || f.nz < f.order)
+      err += kw + " force: " + fmtnum(v[4]) + " <= nx (=" + fmtnum(v[0]) + "), " + fmtnum(v[4]) + " <= ny (=" +
+             fmtnum(v[1]) + "), " + fmtnum(v[4]) + " <= nz (=" + fmtnum(v[2]) + ").";
+    if (f.rc <= 0.0) err += kw + " cutoff (=" + fmtnum(v[3]) + ") > 0.";
+    if (f.accuracy <= 0.0) err += kw + " accuracy (=" + fmtnum(v[5]) + ") > 0.";
+    if (!err.empty()) return fail(c, PM_ERR_ARGUMENT, " force %s :%s", kw.c_str(), err.c_str());
+    if (f.order != 4 && f.order != 6 && f.order != 8)
+      return fail(c, PM_ERR_UNSUPPORTED, "PME order %d: the GPU kernels implement orders 4, 6 and 8", f.order);
+    return PM_OK;
+  }
+  if (!ieq(algo, "NonbondedCutoff"))
+    return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+  if (pots.empty() || pots.size() > 2 || swfs.size() != pots.size())
+    return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+  if (pots.size() == 2 && (pots[0] != 0 || pots[1] == 0))
+    return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+  // positional values: per term potential params then switching params, then
+  // the algorithm cutoff (OneAtomPair(Two)::getParameters, NonbondedCutoffForce.h:113-117)
+  int k = 0;
+  for (size_t i = 0; i < pots.size(); i++) {
+    Term tm;
+    tm.pot = pots[i];
+    tm.swf = swfs[i];
+    if (tm.pot == 2) {
+      if (k >= nv) return fail(c, PM_ERR_ARGUMENT, "force '%s': missing -alpha", id.c_str());
+      tm.alpha = v[k++];
+      if (!(tm.alpha > 0)) return fail(c, PM_ERR_ARGUMENT, "CoulombEwaldReal -alpha (=%s) > 0.", fmtnum(tm.alpha).c_str());
+    }
+    if (tm.swf == PM_SWF_C2) {
+      if (k + 1 >= nv) return fail(c, PM_ERR_ARGUMENT, "force '%s': missing C2 parameters", id.c_str());
+      tm.switchon = v[k++];
+      tm.cutoff = v[k++];
+      // C2SwitchingFunction::make (topology/C2SwitchingFunction.cpp:32-41)
+      if (tm.switchon < 0.0 || tm.cutoff <= 0.0 || tm.switchon >= tm.cutoff)
+        return fail(c, PM_ERR_ARGUMENT, "C2 switching function: 0 <= switchon (=%s) < cutoff (=%s).",
+                    fmtnum(tm.switchon).c_str(), fmtnum(tm.cutoff).c_str());
+    } else {
+      if (k >= nv) return fail(c, PM_ERR_ARGUMENT, "force '%s': missing switching cutoff", id.c_str());
+      tm.cutoff = v[k++];
+      if (!(tm.cutoff > 0))
+        return fail(c, PM_ERR_ARGUMENT, "%s swf cutoff (=%s) > 0.", tm.swf == PM_SWF_C1 ? "C1" : "cutoff",
+                    fmtnum(tm.cutoff).c_str());
+    }
+    f.terms.push_back(tm);
+  }
+  if (k != nv - 1) return fail(c, PM_ERR_ARGUMENT, "force '%s': expected %d parameter values, got %d", id.c_str(), k + 1, nv);
+  f.algo_cutoff = v[k];
+  if (!(f.algo_cutoff > 0)) return fail(c, PM_ERR_ARGUMENT, "algorithm cutoff (=%s) > 0.", fmtnum(f.algo_cutoff).c_str());
+  return PM_OK;
+}
+
+void fill_swf(const Term& t, double& cut2, double& on2, double& sw1, double& sw2, double& sw3, double& c15_1,
+              double& c05_3, double& c15_3) {
+  const double c = t.cutoff, s = t.switchon;
+  cut2 = c * c;
+  on2 = s * s;
+  const double d = c * c - s * s;
+  const double d3 = d * d * d;
+  sw1 = 1.0 / d3;
+  sw2 = c * c - 3.0 * s * s;
+  sw3 = 4.0 / d3;
+  c15_1 = 1.5 / c;
+  c05_3 = 0.5 / (c * c * c);
+  c15_3 = 1.5 / (c * c * c);
+}
+
+// pair parameters for one kernel pass over the given terms (LJ first)
+PmPairParams pair_params(const std::vector<Term>& terms, double csf) {
+  PmPairParams p{};
+  p.lj_swf = PM_SWF_NONE;
+  p.coul_kind = PM_COUL_NONE;
+  p.coul_swf = PM_SWF_CUTOFF;
+  p.csf = csf;
+  p.cut2 = 0;
+  bool modify = false;
+  for (const Term& t : terms) {
+    double cut2, on2, sw1, sw2, sw3, a1, a2, a3;
+    fill_swf(t, cut2, on2, sw1, sw2, sw3, a1, a2, a3);
+    p.cut2 = std::max(p.cut2, cut2);
+    if (t.swf != PM_SWF_CUTOFF) modify = true;
+    if (t.pot == 0) {
+      p.lj_swf = t.swf;
+      p.lj_cut2 = cut2; p.lj_on2 = on2; p.lj_sw1 = sw1; p.lj_sw2 = sw2; p.lj_sw3 = sw3;
+      p.lj_c15_1 = a1; p.lj_c05_3 = a2; p.lj_c15_3 = a3;
+    } else {
+      p.coul_kind = t.pot == 1 ? PM_COUL_PLAIN : PM_COUL_EWALD;
+      p.coul_swf = t.swf;
+      p.c_cut2 = cut2; p.c_on2 = on2; p.c_sw1 = sw1; p.c_sw2 = sw2; p.c_sw3 = sw3;
+      p.c_c15_1 = a1; p.c_c05_3 = a2; p.c_c15_3 = a3;
+      if (t.pot == 2) { p.alpha = t.alpha; p.two_alpha_pi = 2.0 * t.alpha / std::sqrt(kPi); }
+    }
+  }
+  p.modify = modify ? 1 : 0;
+  return p;
+}
+
+struct PairPass {
+  std::vector<Term> terms;
+  double algo_cutoff;
+};
+
+// Plan the pair passes of a force group: one pass per force, except that a
+// single-term LennardJones force and a single-term Coulomb force with the same
+// algorithm cutoff and the same switching cutoff share one pass (the sum of
+// two OneAtomPair evaluations over the same pair set).
+std::vector<PairPass> plan_passes(pm_context* c, const std::vector<int>& handles) {
+  std::vector<PairPass> passes;
+  std::vector<bool> used(handles.size(), false);
+  for (size_t i = 0; i < handles.size(); i++) {
+    ForceDesc& f = c->forces[handles[i]];
+    if (f.pme || f.bonded || used[i]) continue;
+    PairPass p{f.terms, f.algo_cutoff};
+    used[i] = true;
+    if (f.terms.size() == 1) {
+      for (size_t j = i + 1; j < handles.size(); j++) {
+        ForceDesc& g = c->forces[handles[j]];
+        if (used[j] || g.pme || g.bonded || g.terms.size() != 1) continue;
+        const Term& a = f.terms[0];
+        const Term& b = g.terms[0];
+        bool lj_coul = (a.pot == 0) != (b.pot == 0);
+        if (lj_coul && g.algo_cutoff == f.algo_cutoff && a.cutoff == b.cutoff) {
+          p.terms = a.pot == 0 ? std::vector<Term>{a, b} : std::vector<Term>{b, a};
+          used[j] = true;
+          break;
+        }
+      }
+    }
+    passes.push_back(p);
+  }
+  // PME real-space terms run as a CoulombEwaldReal / Cutoff pair pass
+  for (size_t i = 0; i < handles.size(); i++) {
+    ForceDesc& f = c->forces[handles[i]];
+    if (!f.pme || !f.real) continue;
+    Term t;
+    t.pot = 2; t.swf = PM_SWF_CUTOFF; t.alpha = f.alpha; t.cutoff = f.rc;
+    passes.push_back(PairPass{{t}, f.rc});
+  }
+  return passes;
+}
+
+// ---------------------------------------------------------- evaluation core
+// Queues one force-group evaluation at the positions in c->d_pos: forces into
+// c->d_force (fixed point, zeroed first), energies into c->d_energy.
+int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const bool vir = (flags & PM_EVAL_VIRIAL) != 0;
+  int r;
+  for (int h : handles) {
+    ForceDesc& f = c->forces[h];
+    if (f.pme) {
+      if ((r = ensure_pme(c, f))) return r;
+      if (f.real) {
+        // realTerm lattice images |n| <= 2 rc (PeriodicBoundaryConditions.cpp:80-111)
+        double lmin = std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
+        if (lmin <= 2.0 * f.rc)
+          return fail(c, PM_ERR_UNSUPPORTED, "PME -real with box edge %g <= 2*cutoff: lattice images beyond the minimal image are not implemented on the GPU", lmin);
+      }
+    }
+  }
+  std::vector<PairPass> passes = plan_passes(c, handles);
+  bool any_pme_recip = false;
+  for (int h : handles) if (c->forces[h].pme && c->forces[h].recip) any_pme_recip = true;
+
+  // binning and slot layout (shared by every pass and by PME)
+  if ((r = ensure_cells(c))) return r;
+  if (!c->d_force || !c->d_pos || !c->d_q || !c->d_type || !c->d_stats || !c->d_overflow || !c->d_energy)
+    return fail(c, PM_ERR_STATE, "internal: device buffer missing before launch");
+  const int nc = c->ncells;
+  {
+  TimedScope ts(c, PM_T_BIN);
+  pm_launch_zero_ll(s, c->d_force, 3LL * n);
+  pm_launch_zero_i(s, c->d_cell_count, nc);
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_key);
+  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_cell_cursor, c->d_counts,
+                       c->d_cluster_cell, c->ncl_max);
+  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_cursor, c->d_cell_atoms);
+  pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
+                       c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
+                       c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
+                       c->d_atom_of_slot, c->ntypes, c->nslot_max);
+  }
+
+  // energy records: pair passes, PME x-pass blocks, correction blocks
+  long long rec = (long long)passes.size() * c->ncl_max;
+  std::vector<std::pair<int, long long>> pme_rec;   // (handle, offset)
+  for (int h : handles) {
+    ForceDesc& f = c->forces[h];
+    if (!f.pme) continue;
+    long long off = rec;
+    if (f.recip) rec += pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision);
+    if (f.corr) rec += ((long long)c->tab_a.size() + 255) / 256;
+    pme_rec.push_back(std::make_pair(h, off));
+  }
+  bool do_bond = false, do_angle = false;
+  for (int h : handles) {
+    if (c->forces[h].bonded == 1) do_bond = true;
+    if (c->forces[h].bonded == 2) do_angle = true;
+  }
+  if (do_bond && c->nbonds == 0) do_bond = false;
+  if (do_angle && c->nangles == 0) do_angle = false;
+  const long long bonded_off = rec;
+  if (do_bond) rec += (c->nbonds + 255) / 256;
+  if (do_angle) rec += (c->nangles + 255) / 256;
+  if ((r = ensure_epart(c, std::max(rec, 1LL)))) return r;
+
+  double lmin_half = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
+  double cdiag = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
+                           c->realcell[2] * c->realcell[2]);
+  pm_launch_zero_ll(s, reinterpret_cast<long long*>(c->d_stats), 2);
+  for (size_t pi = 0; pi < passes.size(); pi++) {
+    PairPass& ps = passes[pi];
+    PmPairParams pp = pair_params(ps.terms, c->csf);
+    int4* st;
+    int nst;
+    if ((r = get_stencil(c, ps.algo_cutoff, false, &st, &nst))) return r;
+    if (nst > 1300)
+      return fail(c, PM_ERR_UNSUPPORTED, "cell stencil of %d cells (cellsize %g, cutoff %g): use a cellsize of at least cutoff/4", nst, c->cellsize, ps.algo_cutoff);
+    double pcut = std::sqrt(pp.cut2);
+    if ((r = ensure_lists(c, list_capacity(c, pcut)))) return r;
+    PmNbLaunch a{};
+    a.slot_pos = c->d_slot_pos;
+    a.slot_pos64 = c->d_slot_pos64;
+    a.slot_meta = c->d_slot_meta;
+    a.cluster_cell = c->d_cluster_cell;
+    a.counts = c->d_counts;
+    a.ncl_max = c->ncl_max;
+    a.dummy_slot = c->nslot_max;
+    a.cell_slot = c->d_cell_slot;
+    a.cell_count = c->d_cell_count;
+    a.stencil = st;
+    a.nstencil = nst;
+    a.jlist = c->d_jlist;
+    a.jcount = c->d_jcount;
+    a.lcap = c->lcap;
+    a.overflow = c->d_overflow;
+    a.bbox_filter = lmin_half > pcut + cdiag + 1e-3 ? 1 : 0;
+    a.max_delta = c->max_delta;
+    a.molecule = c->d_mol;
+    a.ex_row = c->d_ex_row;
+    a.ex_partner = c->d_ex_partner;
+    a.ex_cls = c->d_ex_cls;
+    a.ntypes1 = c->ntypes + 1;
+    a.lj_f = c->d_lj_f;
+    a.lj_d = c->d_lj_d;
+    a.force_fix = c->d_force;
+    a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
+    a.want_virial = vir ? 1 : 0;
+    a.stats = c->d_stats;
+    {
+      TimedScope ts(c, PM_T_LIST);
+      pm_launch_list(s, c->box, a, pp);
+    }
+    {
+      TimedScope ts(c, PM_T_PAIR);
+      pm_launch_pair(s, c->box, a, pp, c->precision);
+    }
+  }
+  double const_coul = 0.0, const_vir = 0.0;
+  for (auto& hr : pme_rec) {
+    ForceDesc& f = c->forces[hr.first];
+    long long off = hr.second;
+    if (f.recip) {
+      PmPmeLaunch a{};
+      a.n = n;
+      a.nx = f.nx; a.ny = f.ny; a.nz = f.nz; a.nzc = f.nz / 2 + 1;
+      a.order = f.order;
+      for (int k = 0; k < 3; k++) {
+        a.hr[k] = (k == 0 ? f.nx : k == 1 ? f.ny : f.nz) / c->box.L[k];
+        a.Lr[k] = 1.0 / c->box.L[k];
+      }
+      a.q = c->d_q;
+      a.slot_pos = c->d_slot_pos;
+      a.atom_of_slot = c->d_atom_of_slot;
+      a.counts = c->d_counts;
+      a.nslot_max = c->nslot_max;
+      a.grid_fix = c->d_grid_fix;
+      a.grid_r = c->d_grid_r;
+      a.grid_c = c->d_grid_c;
+      a.bmod = c->d_bmod;
+      a.expf_tab = c->d_expt;
+      a.inv_pi_v = 1.0 / (kPi * (c->box.L[0] * c->box.L[1] * c->box.L[2]));
+      a.fac = kPi * kPi / (f.alpha * f.alpha);
+      a.tw = c->d_tw;
+      a.force_fix = c->d_force;
+      a.epart = c->d_epart + off * PM_NE;
+      a.want_virial = vir ? 1 : 0;
+      TimedScope ts(c, PM_T_PME);
+      if (!pm_launch_pme_plans(s, a, c->plans, c->d_slot_pos64, c->precision))
+        return fail(c, PM_ERR_UNSUPPORTED, "PME order %d not implemented on the GPU", f.order);
+      off += pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision);
+    }
+    if (f.corr) {
+      int ntab = (int)c->tab_a.size();
+      int nblk = (ntab + 255) / 256;
+      TimedScope ts(c, PM_T_CORR);
+      pm_launch_correction(s, c->box, n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls, f.alpha,
+                           c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
+                           c->d_epart + off * PM_NE, nblk);
+      // point self energy and charged-system term (:604-628), cached per topology
+      double q2 = 0.0, qs = 0.0;
+      for (int i = 0; i < n; i++) q2 += c->q[i] * c->q[i];
+      for (int i = 0; i < n; i++) qs += c->q[i];
+      const_coul += -q2 * f.alpha / std::sqrt(kPi);
+      if (std::fabs(qs * 0.00268283) > 1.0e-5) {
+        double V = c->box.L[0] * c->box.L[1] * c->box.L[2];
+        double ch = -kPi / (2.0 * V * f.alpha * f.alpha) * qs * qs;
+        const_coul += ch;
+        if (vir) const_vir += ch;
+      }
+    }
+  }
+  if (do_bond || do_angle) {
+    PmBondedLaunch b{};
+    b.n = n;
+    b.pos = c->d_pos;
+    b.b_row = c->d_b_row; b.b_idx = c->d_b_idx; b.a_row = c->d_a_row; b.a_idx = c->d_a_idx;
+    b.nb = c->nbonds; b.na = c->nangles;
+    b.b_a = c->d_b_a; b.b_b = c->d_b_b; b.b_k = c->d_b_k; b.b_r0 = c->d_b_r0;
+    b.a_a = c->d_a_a; b.a_b = c->d_a_b; b.a_c = c->d_a_c; b.a_k = c->d_a_k; b.a_t0 = c->d_a_t0;
+    b.a_kub = c->d_a_kub; b.a_rub = c->d_a_rub;
+    b.do_bond = do_bond ? 1 : 0;
+    b.do_angle = do_angle ? 1 : 0;
+    b.force_fix = c->d_force;
+    b.epart = c->d_epart + bonded_off * PM_NE;
+    b.want_virial = vir ? 1 : 0;
+    int rb, ra;
+    TimedScope ts(c, PM_T_BONDED);
+    pm_launch_bonded(s, c->box, b, &rb, &ra);
+  }
+  {
+    TimedScope ts(c, PM_T_REDUCE);
+    pm_launch_reduce_energy(s, c->d_epart, (int)rec, const_coul, const_vir, c->d_energy, 1);
+  }
+  (void)any_pme_recip;
+  HIPCHECK(c, hipGetLastError());
+  return PM_OK;
+}
+
+int check_overflow(pm_context* c) {
+  if (!c->d_overflow) return PM_OK;
+  int ov = 0;
+  HIPCHECK(c, hipMemcpyAsync(&ov, c->d_overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long st[2] = {0, 0};
+  if (c->d_stats) HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  int counts[4] = {0, 0, 0, 0};
+  if (c->d_counts) HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->last_stats[0] = (long long)st[0];
+  c->last_stats[1] = (long long)st[1];
+  c->last_stats[2] = counts[1];
+  c->last_stats[3] = counts[0];
+  if (ov) {
+    HIPCHECK(c, hipMemset(c->d_overflow, 0, sizeof(int)));
+    return fail(c, PM_ERR_CAPACITY, "pair list capacity (%d entries per cluster) exceeded: atoms are packed far denser than the box average", c->lcap);
+  }
+  return PM_OK;
+}
+
+void free_all(pm_context* c) {
+  dfree(c->d_q); dfree(c->d_mass); dfree(c->d_type); dfree(c->d_mol);
+  dfree(c->d_ex_row); dfree(c->d_ex_partner); dfree(c->d_ex_cls);
+  dfree(c->d_cx_row); dfree(c->d_cx_partner); dfree(c->d_cx_cls);
+  dfree(c->d_tab_a); dfree(c->d_tab_b); dfree(c->d_tab_c);
+  dfree(c->d_lj_f); dfree(c->d_lj_d);
+  dfree(c->d_pos); dfree(c->d_vel); dfree(c->d_force); dfree(c->d_energy); dfree(c->d_scratch3);
+  dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
+  dfree(c->d_cell_cursor); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
+  dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
+  dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_overflow); dfree(c->d_stats); dfree(c->d_epart);
+  dfree(c->d_grid_fix);
+  if (c->d_grid_r) hipFree(c->d_grid_r);
+  if (c->d_grid_c) hipFree(c->d_grid_c);
+  c->d_grid_r = c->d_grid_c = nullptr;
+  dfree(c->d_bmod); dfree(c->d_expt); dfree(c->d_tw);
+  dfree(c->d_b_a); dfree(c->d_b_b); dfree(c->d_b_row); dfree(c->d_b_idx); dfree(c->d_b_k); dfree(c->d_b_r0);
+  dfree(c->d_a_a); dfree(c->d_a_b); dfree(c->d_a_c); dfree(c->d_a_row); dfree(c->d_a_idx);
+  dfree(c->d_a_k); dfree(c->d_a_t0); dfree(c->d_a_kub); dfree(c->d_a_rub);
+  c->nbonds = c->nangles = 0;
+  free_stencils(c);
+  c->cells_alloc = 0;
+  c->jlist_alloc = 0;
+  c->epart_alloc = 0;
+  c->pme_nx = c->pme_ny = c->pme_nz = 0;
+  c->pme_prec = -1;
+}
+
+int set_device(pm_context* c) {
+  HIPCHECK(c, hipSetDevice(c->device));
+  return PM_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int pm_abi_version(void) { return PM_ABI_VERSION; }
+
+const char* pm_last_error(const pm_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pm_create(int device, pm_context** out) {
+  if (!out) return PM_ERR_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PM_ERR_DEVICE;
+  if (device < 0 || device >= count) return PM_ERR_ARGUMENT;
+  pm_context* c = new pm_context();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return PM_ERR_DEVICE;
+  }
+  *out = c;
+  return PM_OK;
+}
+
+void pm_destroy(pm_context* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  harvest_timing(ctx);
+  for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+  free_all(ctx);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int pm_set_precision(pm_context* c, int precision) {
+  if (!c) return PM_ERR_ARGUMENT;
+  if (precision != PM_PRECISION_MIXED && precision != PM_PRECISION_DOUBLE)
+    return fail(c, PM_ERR_ARGUMENT, "precision must be PM_PRECISION_MIXED (0) or PM_PRECISION_DOUBLE (1)");
+  c->precision = precision;
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_set_topology(pm_context* c, int n, const double* q, const int* type, int ntypes, const double* ljA,
+                    const double* ljB, const double* ljA14, const double* ljB14, const int* molecule,
+                    const double* mass, int nexcl, const int* ex_a, const int* ex_b, const int* ex_class,
+                    double csf) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (n <= 0 || !q || !type || ntypes <= 0 || !ljA || !ljB || !ljA14 || !ljB14 || !molecule)
+    return fail(c, PM_ERR_ARGUMENT, "pm_set_topology: n > 0 and non-null arrays required");
+  if (n >= (1 << 23) - 1) return fail(c, PM_ERR_CAPACITY, "pm_set_topology: at most %d atoms", (1 << 23) - 2);
+  if (ntypes > 31) return fail(c, PM_ERR_UNSUPPORTED, "pm_set_topology: at most 31 atom types (got %d)", ntypes);
+  if (nexcl > 0 && (!ex_a || !ex_b || !ex_class)) return fail(c, PM_ERR_ARGUMENT, "pm_set_topology: exclusion arrays missing");
+  for (int i = 0; i < n; i++)
+    if (type[i] < 0 || type[i] >= ntypes) return fail(c, PM_ERR_ARGUMENT, "atom %d has type %d outside [0, %d)", i, type[i], ntypes);
+  for (int k = 0; k < nexcl; k++)
+    if (ex_a[k] < 0 || ex_a[k] >= n || ex_b[k] < 0 || ex_b[k] >= n || ex_class[k] < 0 || ex_class[k] > 2)
+      return fail(c, PM_ERR_ARGUMENT, "exclusion %d (%d, %d, class %d) out of range", k, ex_a[k], ex_b[k], ex_class[k]);
+  free_all(c);
+  c->n = n;
+  c->ntypes = ntypes;
+  c->q.assign(q, q + n);
+  c->type.assign(type, type + n);
+  c->molecule.assign(molecule, molecule + n);
+  c->mass.assign(n, 1.0);
+  if (mass) c->mass.assign(mass, mass + n);
+  c->ljA.assign(ljA, ljA + ntypes * ntypes);
+  c->ljB.assign(ljB, ljB + ntypes * ntypes);
+  c->ljA14.assign(ljA14, ljA14 + ntypes * ntypes);
+  c->ljB14.assign(ljB14, ljB14 + ntypes * ntypes);
+  c->csf = csf;
+  c->ex_a.assign(ex_a, ex_a + nexcl);
+  c->ex_b.assign(ex_b, ex_b + nexcl);
+  c->ex_c.assign(ex_class, ex_class + nexcl);
+  // lookup CSR by lower atom in add order (ExclusionTable::add/check, last add wins)
+  std::vector<int> row(n + 1, 0), partner(nexcl), cls(nexcl);
+  c->max_delta = 0;
+  for (int k = 0; k < nexcl; k++) row[std::min(ex_a[k], ex_b[k]) + 1]++;
+  for (int i = 0; i < n; i++) row[i + 1] += row[i];
+  {
+    std::vector<int> fill(row.begin(), row.end() - 1);
+    for (int k = 0; k < nexcl; k++) {
+      int a = std::min(ex_a[k], ex_b[k]), b = std::max(ex_a[k], ex_b[k]);
+      partner[fill[a]] = b;
+      cls[fill[a]] = ex_class[k];
+      fill[a]++;
+      c->max_delta = std::max(c->max_delta, b - a);
+    }
+  }
+  // the table (ExclusionTable::myTable): distinct (a, b, class), class != NONE, a != b
+  c->tab_a.clear(); c->tab_b.clear(); c->tab_c.clear();
+  {
+    std::set<std::tuple<int, int, int>> seen;
+    for (int k = 0; k < nexcl; k++) {
+      int a = std::min(ex_a[k], ex_b[k]), b = std::max(ex_a[k], ex_b[k]);
+      if (ex_class[k] == PM_EXCL_NONE || a == b) continue;
+      auto t = std::make_tuple(a, b, ex_class[k]);
+      if (seen.count(t)) continue;
+      seen.insert(t);
+      c->tab_a.push_back(a); c->tab_b.push_back(b); c->tab_c.push_back(ex_class[k]);
+    }
+  }
+  // correction CSR per atom: partner*2 + (1 if this atom is the pair's second atom)
+  int ntab = (int)c->tab_a.size();
+  std::vector<int> crow(n + 1, 0), cpart(2 * ntab), ccls(2 * ntab);
+  for (int t = 0; t < ntab; t++) { crow[c->tab_a[t] + 1]++; crow[c->tab_b[t] + 1]++; }
+  for (int i = 0; i < n; i++) crow[i + 1] += crow[i];
+  {
+    std::vector<int> fill(crow.begin(), crow.end() - 1);
+    for (int t = 0; t < ntab; t++) {
+      int a = c->tab_a[t], b = c->tab_b[t];
+      cpart[fill[a]] = 2 * b; ccls[fill[a]] = c->tab_c[t]; fill[a]++;
+      cpart[fill[b]] = 2 * a + 1; ccls[fill[b]] = c->tab_c[t]; fill[b]++;
+    }
+  }
+  // LJ tables with one extra all-zero type for padding slots
+  int t1 = ntypes + 1;
+  std::vector<float4> ljf(t1 * t1, make_float4(0, 0, 0, 0));
+  std::vector<double4> ljd(t1 * t1, make_double4(0, 0, 0, 0));
+  for (int a = 0; a < ntypes; a++)
+    for (int b = 0; b < ntypes; b++) {
+      int k = a * ntypes + b;
+      ljf[a * t1 + b] = make_float4((float)ljA[k], (float)ljB[k], (float)ljA14[k], (float)ljB14[k]);
+      ljd[a * t1 + b] = make_double4(ljA[k], ljB[k], ljA14[k], ljB14[k]);
+    }
+  std::vector<double> mass_v = c->mass;
+  if ((r = upload(c, &c->d_q, c->q))) return r;
+  if ((r = upload(c, &c->d_mass, mass_v))) return r;
+  if ((r = upload(c, &c->d_type, c->type))) return r;
+  if ((r = upload(c, &c->d_mol, c->molecule))) return r;
+  if ((r = upload(c, &c->d_ex_row, row))) return r;
+  if ((r = upload(c, &c->d_ex_partner, partner))) return r;
+  if ((r = upload(c, &c->d_ex_cls, cls))) return r;
+  if ((r = upload(c, &c->d_cx_row, crow))) return r;
+  if ((r = upload(c, &c->d_cx_partner, cpart))) return r;
+  if ((r = upload(c, &c->d_cx_cls, ccls))) return r;
+  if ((r = upload(c, &c->d_tab_a, c->tab_a))) return r;
+  if ((r = upload(c, &c->d_tab_b, c->tab_b))) return r;
+  if ((r = upload(c, &c->d_tab_c, c->tab_c))) return r;
+  if ((r = upload(c, &c->d_lj_f, ljf))) return r;
+  if ((r = upload(c, &c->d_lj_d, ljd))) return r;
+  if ((r = dalloc(c, &c->d_pos, 3 * (size_t)n))) return r;
+  if ((r = dalloc(c, &c->d_vel, 3 * (size_t)n))) return r;
+  HIPCHECK(c, hipMemset(c->d_vel, 0, 3 * (size_t)n * sizeof(double)));
+  if ((r = dalloc(c, &c->d_force, 3 * (size_t)n))) return r;
+  if ((r = dalloc(c, &c->d_scratch3, 3 * (size_t)n))) return r;
+  if ((r = dalloc(c, &c->d_energy, PM_NE))) return r;
+  HIPCHECK(c, hipMemset(c->d_energy, 0, PM_NE * sizeof(double)));
+  for (auto& f : c->forces) f.cached = false;
+  c->have_topo = true;
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_set_bonded(pm_context* c, int nb, const int* ba, const int* bb, const double* bk, const double* br0,
+                  int na, const int* aa, const int* ab, const int* ac, const double* ak, const double* at0,
+                  const double* akub, const double* arub) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if (nb < 0 || na < 0 || (nb > 0 && (!ba || !bb || !bk || !br0)) ||
+      (na > 0 && (!aa || !ab || !ac || !ak || !at0 || !akub || !arub)))
+    return fail(c, PM_ERR_ARGUMENT, "pm_set_bonded: arrays missing");
+  const int n = c->n;
+  for (int k = 0; k < nb; k++)
+    if (ba[k] < 0 || ba[k] >= n || bb[k] < 0 || bb[k] >= n || ba[k] == bb[k])
+      return fail(c, PM_ERR_ARGUMENT, "bond %d (%d, %d) out of range", k, ba[k], bb[k]);
+  for (int k = 0; k < na; k++)
+    if (aa[k] < 0 || aa[k] >= n || ab[k] < 0 || ab[k] >= n || ac[k] < 0 || ac[k] >= n ||
+        aa[k] == ab[k] || ab[k] == ac[k] || aa[k] == ac[k])
+      return fail(c, PM_ERR_ARGUMENT, "angle %d (%d, %d, %d) out of range", k, aa[k], ab[k], ac[k]);
+  // per-atom CSR of the terms each atom belongs to
+  std::vector<int> brow(n + 1, 0), bidx(2 * (size_t)nb), arow(n + 1, 0), aidx(3 * (size_t)na);
+  for (int k = 0; k < nb; k++) { brow[ba[k] + 1]++; brow[bb[k] + 1]++; }
+  for (int k = 0; k < na; k++) { arow[aa[k] + 1]++; arow[ab[k] + 1]++; arow[ac[k] + 1]++; }
+  for (int i = 0; i < n; i++) { brow[i + 1] += brow[i]; arow[i + 1] += arow[i]; }
+  {
+    std::vector<int> fb(brow.begin(), brow.end() - 1), fa(arow.begin(), arow.end() - 1);
+    for (int k = 0; k < nb; k++) { bidx[fb[ba[k]]++] = k; bidx[fb[bb[k]]++] = k; }
+    for (int k = 0; k < na; k++) { aidx[fa[aa[k]]++] = k; aidx[fa[ab[k]]++] = k; aidx[fa[ac[k]]++] = k; }
+  }
+  auto vi = [](const int* p, int m) { return std::vector<int>(p, p + m); };
+  auto vd = [](const double* p, int m) { return std::vector<double>(p, p + m); };
+  if ((r = upload(c, &c->d_b_row, brow))) return r;
+  if ((r = upload(c, &c->d_b_idx, bidx))) return r;
+  if ((r = upload(c, &c->d_a_row, arow))) return r;
+  if ((r = upload(c, &c->d_a_idx, aidx))) return r;
+  if ((r = upload(c, &c->d_b_a, vi(ba, nb)))) return r;
+  if ((r = upload(c, &c->d_b_b, vi(bb, nb)))) return r;
+  if ((r = upload(c, &c->d_b_k, vd(bk, nb)))) return r;
+  if ((r = upload(c, &c->d_b_r0, vd(br0, nb)))) return r;
+  if ((r = upload(c, &c->d_a_a, vi(aa, na)))) return r;
+  if ((r = upload(c, &c->d_a_b, vi(ab, na)))) return r;
+  if ((r = upload(c, &c->d_a_c, vi(ac, na)))) return r;
+  if ((r = upload(c, &c->d_a_k, vd(ak, na)))) return r;
+  if ((r = upload(c, &c->d_a_t0, vd(at0, na)))) return r;
+  if ((r = upload(c, &c->d_a_kub, vd(akub, na)))) return r;
+  if ((r = upload(c, &c->d_a_rub, vd(arub, na)))) return r;
+  c->nbonds = nb;
+  c->nangles = na;
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_set_boundary(pm_context* c, const double* e1, const double* e2, const double* e3, const double* origin,
+                    double cellsize) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!e1 || !e2 || !e3 || !origin) return fail(c, PM_ERR_ARGUMENT, "pm_set_boundary: null vector");
+  if (e1[1] != 0.0 || e1[2] != 0.0 || e2[0] != 0.0 || e2[2] != 0.0 || e3[0] != 0.0 || e3[1] != 0.0)
+    return fail(c, PM_ERR_UNSUPPORTED, "[NonbondedFullEwaldSystemForce::initialize] Not orthogonal, aborting.");
+  const double lx = e1[0], ly = e2[1], lz = e3[2];
+  // PeriodicBoundaryConditions::set (topology/PeriodicBoundaryConditions.cpp:34-77)
+  double V = std::fabs(lx * ly * lz);
+  if (!(V >= kEpsilon)) return fail(c, PM_ERR_ARGUMENT, "[PeriodicBoundaryConditions::set] No volume, aborting.");
+  if (!std::isfinite(V)) return fail(c, PM_ERR_ARGUMENT, "[PeriodicBoundaryConditions::set] Infinite volume, aborting.");
+  if (!(cellsize > 0)) return fail(c, PM_ERR_ARGUMENT, "Cubic cellmanager: cutoff > 0 (%s).", fmtnum(cellsize).c_str());
+  if (lx < 0 || ly < 0 || lz < 0) return fail(c, PM_ERR_UNSUPPORTED, "cell basis vectors must point along +x, +y, +z");
+  PmBox& b = c->box;
+  const double L[3] = {lx, ly, lz};
+  const double a1x = ly * lz - 0.0 * 0.0, a2y = lz * lx - 0.0 * 0.0, a3z = lx * ly - 0.0 * 0.0;
+  b.er[0] = a1x / (lx * a1x + 0.0 * 0.0 + 0.0 * 0.0);
+  b.er[1] = a2y / (0.0 * 0.0 + ly * a2y + 0.0 * 0.0);
+  b.er[2] = a3z / (0.0 * 0.0 + 0.0 * 0.0 + lz * a3z);
+  double mn[3], mx[3];
+  for (int k = 0; k < 3; k++) {
+    b.L[k] = L[k];
+    b.origin[k] = origin[k];
+    double a = origin[k] - L[k] * 0.5, bb = origin[k] + L[k] * 0.5;
+    mn[k] = std::min(a, bb);
+    mx[k] = std::max(a, bb);
+    b.h[k] = mx[k] - mn[k];
+    b.h2[k] = b.h[k] * 0.5;
+    b.delta[k] = origin[k] - mn[k];
+    // CubicCellManager::initialize and ArrayCellListStructure::initialize
+    double d = mx[k] - mn[k];
+    double f = std::floor(d / cellsize);
+    c->realcell[k] = d / std::max(1.0, f);
+    b.rreal[k] = 1.0 / c->realcell[k];
+    b.n[k] = std::max(1, (int)std::floor(d / c->realcell[k] + kEpsilon));
+    b.to_fix[k] = 4294967296.0 / L[k];
+    b.fix_to_A[k] = L[k] / 4294967296.0;
+  }
+  double dx = (lx * 0.5) * (lx * 0.5), dy = (ly * 0.5) * (ly * 0.5), dz = (lz * 0.5) * (lz * 0.5);
+  b.d = std::min(dx, std::min(dy, dz));
+  long long ncl = (long long)b.n[0] * b.n[1] * b.n[2];
+  if (ncl > 64LL * 1024 * 1024)
+    return fail(c, PM_ERR_CAPACITY, "Your systems is expanding such that the the ratio simulation box / cell size is to big.");
+  c->ncells = (int)ncl;
+  c->cellsize = cellsize;
+  free_stencils(c);
+  c->cells_alloc = 0;
+  c->pme_alpha_tables = -1;
+  c->have_box = true;
+  for (auto& f : c->forces) f.cached = false;
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_force_create(pm_context* c, const char* id, const double* values, int nvalues, int* handle) {
+  if (!c || !handle) return PM_ERR_ARGUMENT;
+  if (nvalues < 0 || (nvalues > 0 && !values)) return fail(c, PM_ERR_ARGUMENT, "pm_force_create: values missing");
+  ForceDesc f;
+  int r = parse_force(c, id, values, nvalues, f);
+  if (r) return r;
+  c->forces.push_back(f);
+  *handle = (int)c->forces.size() - 1;
+  return PM_OK;
+}
+
+int pm_force_uncache(pm_context* c, int handle) {
+  if (!c) return PM_ERR_ARGUMENT;
+  if (handle < 0 || handle >= (int)c->forces.size()) return fail(c, PM_ERR_ARGUMENT, "unknown force handle %d", handle);
+  c->forces[handle].cached = false;
+  c->pme_alpha_tables = -1;
+  return PM_OK;
+}
+
+static int check_ready(pm_context* c, const int* handles, int nh) {
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if (!c->have_box) return fail(c, PM_ERR_STATE, "no boundary conditions: call pm_set_boundary first");
+  if (nh <= 0 || !handles) return fail(c, PM_ERR_ARGUMENT, "empty force group");
+  for (int i = 0; i < nh; i++)
+    if (handles[i] < 0 || handles[i] >= (int)c->forces.size()) return fail(c, PM_ERR_ARGUMENT, "unknown force handle %d", handles[i]);
+  return PM_OK;
+}
+
+int pm_evaluate(pm_context* c, const int* handles, int nh, const double* positions, double* forces,
+                double* energies, int flags) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if ((r = check_ready(c, handles, nh))) return r;
+  if (!positions || !forces || !energies) return fail(c, PM_ERR_ARGUMENT, "pm_evaluate: null buffer");
+  const int n = c->n;
+  HIPCHECK(c, hipMemcpyAsync(c->d_pos, positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  std::vector<int> hv(handles, handles + nh);
+  if ((r = evaluate_group(c, hv, flags))) return r;
+  pm_launch_fix_to_double(c->stream, 3 * n, c->d_force, c->d_scratch3);
+  std::vector<double> f(3 * (size_t)n), e(PM_NE);
+  HIPCHECK(c, hipMemcpyAsync(f.data(), c->d_scratch3, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(e.data(), c->d_energy, PM_NE * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if ((r = check_overflow(c))) return r;
+  for (size_t k = 0; k < f.size(); k++) forces[k] += f[k];
+  for (int k = 0; k < PM_NE; k++) energies[k] += e[k];
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm_cutoff, double pair_cutoff,
+                        int* dims, int* cell_of_atom, int* head, int* next, long long* cand, long long* inrc) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo || !c->have_box) return fail(c, PM_ERR_STATE, "topology and boundary required");
+  if (dims) { dims[0] = c->box.n[0]; dims[1] = c->box.n[1]; dims[2] = c->box.n[2]; }
+  if (!positions) return PM_OK;
+  const int n = c->n, nc = c->ncells;
+  if ((r = ensure_cells(c))) return r;
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync(c->d_pos, positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  pm_launch_zero_i(s, c->d_cell_count, nc);
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_key);
+  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_cell_cursor, c->d_counts,
+                       c->d_cluster_cell, c->ncl_max);
+  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_cursor, c->d_cell_atoms);
+  std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);
+  HIPCHECK(c, hipMemcpyAsync(cell.data(), c->d_cell_of, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(cnt.data(), c->d_cell_count, nc * sizeof(int), hipMemcpyDeviceToHost, s));
+  long long* d_cand = nullptr;
+  long long* d_inrc = nullptr;
+  if (cand || inrc) {
+    int4* half;
+    int nhalf;
+    if ((r = get_stencil(c, algorithm_cutoff, true, &half, &nhalf))) return r;
+    if ((r = dalloc(c, &d_cand, nc))) return r;
+    if ((r = dalloc(c, &d_inrc, nc))) { dfree(d_cand); return r; }
+    pm_launch_diag(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_start, c->d_cell_atoms, half, nhalf, c->d_mol,
+                   c->d_ex_row, c->d_ex_partner, c->d_ex_cls, pair_cutoff * pair_cutoff, nc, c->d_cell_count,
+                   d_cand, d_inrc);
+    std::vector<long long> hc(nc), hi(nc);
+    HIPCHECK(c, hipMemcpyAsync(hc.data(), d_cand, nc * sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(c, hipMemcpyAsync(hi.data(), d_inrc, nc * sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(c, hipStreamSynchronize(s));
+    if (cand) std::copy(hc.begin(), hc.end(), cand);
+    if (inrc) std::copy(hi.begin(), hi.end(), inrc);
+    dfree(d_cand);
+    dfree(d_inrc);
+  }
+  HIPCHECK(c, hipStreamSynchronize(s));
+  if (cell_of_atom) std::copy(cell.begin(), cell.end(), cell_of_atom);
+  if (head || next) {
+    // linked lists in the reference's order: atoms prepended from N-1 down to 0
+    // (buildCellLists.cpp:19-36), i.e. ascending index within each cell
+    std::vector<int> hd(nc, -1), nx(n, -1);
+    for (int i = n - 1; i >= 0; i--) { nx[i] = hd[cell[i]]; hd[cell[i]] = i; }
+    if (head) std::copy(hd.begin(), hd.end(), head);
+    if (next) std::copy(nx.begin(), nx.end(), next);
+  }
+  return PM_OK;
+}
+
+int pm_md_setup(pm_context* c, const int* handles, int nh, int integrator, double dt, double temperature,
+                double gamma, unsigned long long seed) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = check_ready(c, handles, nh))) return r;
+  if (integrator != PM_INTEGRATOR_LEAPFROG && integrator != PM_INTEGRATOR_LANGEVIN)
+    return fail(c, PM_ERR_ARGUMENT, "unknown integrator %d", integrator);
+  if (!(dt > 0)) return fail(c, PM_ERR_ARGUMENT, "timestep must be > 0");
+  if (integrator == PM_INTEGRATOR_LANGEVIN && !(gamma > 0))
+    return fail(c, PM_ERR_ARGUMENT, "Langevin gamma must be > 0");
+  c->md_handles.assign(handles, handles + nh);
+  c->md_integrator = integrator;
+  c->md_dt = dt;
+  c->md_temp = temperature;
+  c->md_gamma = gamma;
+  c->md_seed = seed;
+  c->md_counter = 0;
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_md_set_state(pm_context* c, const double* x, const double* v) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology");
+  const size_t bytes = 3 * (size_t)c->n * sizeof(double);
+  if (x) HIPCHECK(c, hipMemcpy(c->d_pos, x, bytes, hipMemcpyHostToDevice));
+  if (v) HIPCHECK(c, hipMemcpy(c->d_vel, v, bytes, hipMemcpyHostToDevice));
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+int pm_md_run(pm_context* c, int nsteps) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (c->md_handles.empty()) return fail(c, PM_ERR_STATE, "pm_md_setup not called");
+  if (nsteps < 1) return PM_OK;
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const double h = c->md_dt * kInvTimeFactor;
+  if (!c->md_forces_valid) {
+    // initializeForces (StandardIntegrator.cpp:46-59)
+    if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+    c->md_forces_valid = true;
+  }
+  if (c->md_integrator == PM_INTEGRATOR_LEAPFROG) {
+    // LeapfrogIntegrator::run (integrators/LeapfrogIntegrator.cpp:107-121)
+    {
+      TimedScope ts(c, PM_T_INTEGRATE);
+      pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+    }
+    if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+    for (int i = 1; i < nsteps; i++) {
+      {
+        TimedScope ts(c, PM_T_INTEGRATE);
+        pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+      }
+      if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+    }
+    pm_launch_kick(s, n, 2, 0.5 * c->md_dt * kInvTimeFactor, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+  } else {
+    // StandardIntegrator::run with the Langevin half kick
+    // (StandardIntegrator.cpp:24-35, LangevinLeapfrogIntegrator.cpp:50-79)
+    const double g = c->md_gamma / (1000 * kInvTimeFactor);
+    for (int i = 0; i < nsteps; i++) {
+      pm_launch_langevin_kick(s, n, h, c->md_temp, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->md_counter++);
+      pm_launch_drift(s, 3 * n, h, c->d_vel, c->d_pos);
+      if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+      pm_launch_langevin_kick(s, n, h, c->md_temp, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->md_counter++);
+    }
+  }
+  HIPCHECK(c, hipGetLastError());
+  return PM_OK;
+}
+
+int pm_md_get_state(pm_context* c, double* x, double* v, double* energies) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  const size_t bytes = 3 * (size_t)c->n * sizeof(double);
+  if (x) HIPCHECK(c, hipMemcpyAsync(x, c->d_pos, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (v) HIPCHECK(c, hipMemcpyAsync(v, c->d_vel, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (energies) HIPCHECK(c, hipMemcpyAsync(energies, c->d_energy, PM_NE * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return check_overflow(c);
+}
+
+int pm_last_stats(pm_context* c, long long* stats, int nstats) {
+  if (!c || !stats) return PM_ERR_ARGUMENT;
+  for (int k = 0; k < nstats && k < 4; k++) stats[k] = c->last_stats[k];
+  return PM_OK;
+}
+
+int pm_set_timing(pm_context* c, unsigned mask) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  harvest_timing(c);
+  c->timing_mask = mask;
+  for (int k = 0; k < PM_TIMING_CATEGORIES; k++) { c->t_ms[k] = 0; c->t_count[k] = 0; }
+  return PM_OK;
+}
+
+int pm_get_timing(pm_context* c, double* ms, long long* count, int ncat) {
+  if (!c || !ms || !count) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  harvest_timing(c);
+  for (int k = 0; k < ncat && k < PM_TIMING_CATEGORIES; k++) { ms[k] = c->t_ms[k]; count[k] = c->t_count[k]; }
+  return PM_OK;
+}
+
+int pm_synchronize(pm_context* c) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return PM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+// pm_internal.h -- structures shared by the host side and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Fixed-point scales.  Forces: 2^32 units per kcal/mol/A in int64 (range
+// +-2^31 kcal/mol/A, resolution 2.3e-10).  PME charge grid: 2^40 units per
+// scaled charge (range +-2^23, resolution 9e-13).
+#define PM_FORCE_FIX 4294967296.0
+#define PM_FORCE_UNFIX (1.0 / 4294967296.0)
+#define PM_GRID_FIX 1099511627776.0
+#define PM_GRID_UNFIX (1.0 / 1099511627776.0)
+
+// ExclusionClass, topology/ExclusionTable.h:20-25
+#define PM_EXCL_NONE 0
+#define PM_EXCL_MODIFIED 1
+#define PM_EXCL_FULL 2
+
+// Pair-kernel potential variants.  LJ switching: 0 none (no LJ term),
+// 1 C1, 2 C2, 3 Cutoff.  Coulomb kind: 0 none, 1 Coulomb, 2 CoulombEwaldReal.
+// Coulomb switching: 1 C1, 2 C2, 3 Cutoff.
+#define PM_SWF_NONE 0
+#define PM_SWF_C1 1
+#define PM_SWF_C2 2
+#define PM_SWF_CUTOFF 3
+#define PM_COUL_NONE 0
+#define PM_COUL_PLAIN 1
+#define PM_COUL_EWALD 2
+
+// Atoms per i-cluster and j-slots per wave (lane = jslot * PM_M + ii).
+#define PM_M 8
+#define PM_JW (64 / PM_M)
+
+// Bounds that keep degenerate inputs from turning into unbounded loops.
+#define PM_SORT_MAX 512
+#define PM_CAND_FACTOR 16
+#define PM_DIAG_MAX 4096
+
+// Layout of an energy record: the prefix of ScalarStructure
+// (base/ScalarStructure.h:25-45) up to the atomic virial.
+#define PM_E_COUL 0
+#define PM_E_LJ 1
+#define PM_E_BOND 2
+#define PM_E_ANGLE 3
+#define PM_E_VIR 7
+#define PM_NE 16
+
+// Box description used by binning, the pair kernels and PME.
+struct PmBox {
+  double L[3];        // edge lengths (e1.x, e2.y, e3.z)
+  double er[3];       // myE1r.x, myE2r.y, myE3r.z (PeriodicBoundaryConditions.cpp:59-64)
+  double origin[3];
+  double h[3];        // myH = max - min
+  double h2[3];       // myH2 = myH * 0.5
+  double d;           // myD = min((L/2)^2)
+  double delta[3];    // origin - min (buildCellLists.cpp:15)
+  double rreal[3];    // 1 / real cell size (CubicCellManager.cpp:33-35)
+  int n[3];           // cell grid dimensions
+  double to_fix[3];   // 2^32 / L
+  double fix_to_A[3]; // L / 2^32
+};
+
+// Switched pair potentials of one (possibly fused) nonbonded force.
+struct PmPairParams {
+  int lj_swf, coul_kind, coul_swf;
+  int modify;          // apply switching (any term's MODIFY flag)
+  double cut2;         // max of the terms' switching cutoffs squared
+  // LJ switching
+  double lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3, lj_c15_3;
+  // Coulomb switching
+  double c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3;
+  double alpha, two_alpha_pi;
+  double csf;
+};
+
+struct PmNbLaunch {
+  // slots
+  const int4* slot_pos;      // X, Y, Z fixed point, w = charge (float bits)
+  const longlong4* slot_pos64; // 64-bit fixed point + charge (double bits), FP64 mode
+  const int* slot_meta;      // type | orig << 8
+  const int* cluster_cell;
+  const int* counts;         // [0] nslot, [1] nclusters
+  int ncl_max;
+  int dummy_slot;
+  // cells
+  const int* cell_slot;
+  const int* cell_count;
+  const int4* stencil;
+  int nstencil;
+  // lists
+  int* jlist;
+  int* jcount;
+  int lcap;
+  int* overflow;
+  int bbox_filter;
+  int max_delta;             // largest |a-b| over exclusion pairs
+  // topology
+  const int* molecule;
+  const int* ex_row;
+  const int* ex_partner;
+  const int* ex_cls;
+  int ntypes1;               // types + 1 dummy
+  const float4* lj_f;
+  const double4* lj_d;
+  // outputs
+  long long* force_fix;      // [3n] fixed point, accumulated (+=) for every real atom
+  double* epart;             // [ncl_max * PM_NE]
+  int want_virial;
+  unsigned long long* stats; // [2] entries, evaluated lanes (may be null)
+};
+
+// ---------------------------------------------------------------- launchers
+// Binning (reference-exact) and slot layout.
+void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
+                   int* cell_count, float* key);
+void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
+                          int* cell_slot, int* cell_cursor, int* counts, int* cluster_cell,
+                          int ncl_max);
+void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, int* cell_cursor, int* cell_atoms);
+void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
+                          const double* q, const int* type, const int* cell_count,
+                          const int* cell_start, const int* cell_slot, int* cell_atoms,
+                          const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
+                          int* atom_of_slot, int dummy_type, int dummy_slot);
+void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
+void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p,
+                    int precision);
+void pm_launch_diag(hipStream_t s, const PmBox& box, int n, const double* pos, const int* cell_of,
+                    const int* cell_start, const int* cell_atoms, const int4* half_stencil,
+                    int nhalf, const int* molecule, const int* ex_row, const int* ex_partner,
+                    const int* ex_cls, double pair_cut2, int ncells, const int* cell_count,
+                    long long* cand, long long* inrc);
+
+// PME
+struct PmPmeLaunch {
+  int n;
+  int nx, ny, nz, nzc;       // nzc = nz/2 + 1
+  int order;
+  double hr[3];              // n / L per axis
+  const double* q;           // scaled charges (atom order)
+  const int4* slot_pos;
+  const int* atom_of_slot;
+  const int* counts;
+  int nslot_max;
+  unsigned long long* grid_fix;  // [nx*ny*nz] int64 charge grid (cleared by the z pass)
+  void* grid_r;              // real grid (float or double) [nx*ny*nz]
+  void* grid_c;              // complex half grid [nx*ny*nzc] (float2 / double2)
+  const double* bmod;        // [nx + ny + nz] B(m) = 1/|b(m)|^2 (Grid::dftmod)
+  const double* expf_tab;    // [nx + ny + nzc] exp(-pi^2 (m/L)^2 / alpha^2)
+  double inv_pi_v;           // 1/(pi V)
+  double fac;                // pi^2/alpha^2
+  double Lr[3];              // 1/L
+  const double* tw;          // twiddles exp(-2 pi i m/n), interleaved re, im
+  long long* force_fix;
+  double* epart;             // [x-pass blocks * PM_NE]
+  int want_virial;
+};
+
+#define PM_FFT_MAXST 10
+struct PmFftPlan {
+  int n;
+  int nst;
+  int rad[PM_FFT_MAXST];
+  int tw;  // offset of this length's twiddles in the table (complex entries)
+};
+bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
+                         const longlong4* pos64, int precision);
+int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision);
+
+// Correction (excluded pairs), reductions, conversions and integrators
+void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
+                          const int* row, const int* partner, const int* cls, double alpha, double csf,
+                          int want_virial, long long* force_fix, int ntab, const int* tab_a,
+                          const int* tab_b, const int* tab_c, double* epart, int nblk);
+void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, double constant_coul,
+                             double constant_vir_diag, double* out, int overwrite);
+void pm_launch_zero_ll(hipStream_t s, long long* p, long long count);
+void pm_launch_zero_i(hipStream_t s, int* p, long long count);
+void pm_launch_zero_d(hipStream_t s, double* p, long long count);
+void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* out);
+void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out);
+void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
+                    double* v, double* x);
+void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double temperature,
+                             double gamma_akma, const double* mass, const long long* f, double* v,
+                             unsigned long long seed, unsigned long long counter);
+void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x);
+
+// Bonded terms (Bond / Angle system forces), per-atom gather formulation
+struct PmBondedLaunch {
+  int n;
+  const double* pos;
+  // per-atom CSR of the bonds and angles an atom belongs to
+  const int* b_row; const int* b_idx;     // bond index
+  const int* a_row; const int* a_idx;     // angle index
+  int nb, na;
+  const int* b_a; const int* b_b; const double* b_k; const double* b_r0;
+  const int* a_a; const int* a_b; const int* a_c; const double* a_k; const double* a_t0;
+  const double* a_kub; const double* a_rub;
+  int do_bond, do_angle;
+  long long* force_fix;
+  double* epart;        // records: bond blocks then angle blocks
+  int want_virial;
+};
+void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, int* nrec_bond, int* nrec_angle);

@@ -1,0 +1,265 @@
+// pm_md.hip -- FP64 kernels around the force kernels: the PME excluded-pair
+// correction, energy reductions, fixed-point conversions and the leapfrog /
+// Langevin position-velocity updates.
+#include "pm_internal.h"
+
+#include <math.h>
+
+namespace {
+
+// PeriodicBoundaryConditions::minimalDifference (topology/PeriodicBoundaryConditions.h:76-117)
+__device__ inline void min_diff(const PmBox& b, const double* c1, const double* c2, double* d, double& s) {
+#pragma clang fp contract(off)
+  double x = c2[0] - c1[0], y = c2[1] - c1[1], z = c2[2] - c1[2];
+  s = x * x + y * y + z * z;
+  if (s > b.d) {
+    if (x < -b.h2[0]) x += b.h[0]; else if (x > b.h2[0]) x -= b.h[0];
+    if (y < -b.h2[1]) y += b.h[1]; else if (y > b.h2[1]) y -= b.h[1];
+    if (z < -b.h2[2]) z += b.h[2]; else if (z > b.h2[2]) z -= b.h[2];
+    s = x * x + y * y + z * z;
+    if (s > b.d) {
+      x -= b.L[0] * __builtin_rint(b.er[0] * x);
+      y -= b.L[1] * __builtin_rint(b.er[1] * y);
+      z -= b.L[2] * __builtin_rint(b.er[2] * z);
+      s = x * x + y * y + z * z;
+    }
+  }
+  d[0] = x; d[1] = y; d[2] = z;
+}
+
+// NonbondedPMEwaldSystemForce::correctionTerm (forces/NonbondedPMEwaldSystemForce.h:942-981)
+// for one exclusion pair a1 < a2: fij (applied -= to a1, += to a2) and energy.
+__device__ inline void corr_pair(const PmBox& b, const double* pos, const double* q, int a1, int a2,
+                                 int cls, double alpha, double csf, double* rij, double* fij, double& e) {
+  double r2;
+  min_diff(b, pos + 3 * a1, pos + 3 * a2, rij, r2);
+  double qq = q[a1] * q[a2];
+  if (cls == PM_EXCL_MODIFIED) qq *= 1 - csf;
+  const double r = sqrt(r2);
+  const double rr = 1 / r;
+  const double ee = erf(alpha * r) * rr;
+  e = -qq * ee;
+  const double tapi = 2.0 * alpha / sqrt(3.14159265358979323846);
+  const double s = (qq * (tapi * exp(-alpha * alpha * r2) - ee) * rr * rr);
+  fij[0] = rij[0] * s; fij[1] = rij[1] * s; fij[2] = rij[2] * s;
+}
+
+// per-atom: sum over the atom's exclusion partners (CSR, partner encoded as
+// 2*atom + (1 if this atom is the pair's second atom))
+__global__ void k_corr_force(PmBox box, int n, const double* __restrict__ pos, const double* __restrict__ q,
+                             const int* __restrict__ row, const int* __restrict__ partner,
+                             const int* __restrict__ cls, double alpha, double csf,
+                             long long* __restrict__ force_fix) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[3] = {0, 0, 0};
+  for (int u = row[i]; u < row[i + 1]; u++) {
+    const int pe = partner[u];
+    const int other = pe >> 1;
+    const bool second = pe & 1;
+    const int a1 = second ? other : i, a2 = second ? i : other;
+    double rij[3], fij[3], e;
+    corr_pair(box, pos, q, a1, a2, cls[u], alpha, csf, rij, fij, e);
+    if (second) { f[0] += fij[0]; f[1] += fij[1]; f[2] += fij[2]; }
+    else { f[0] -= fij[0]; f[1] -= fij[1]; f[2] -= fij[2]; }
+  }
+  force_fix[3 * i] += __double2ll_rn(f[0] * PM_FORCE_FIX);
+  force_fix[3 * i + 1] += __double2ll_rn(f[1] * PM_FORCE_FIX);
+  force_fix[3 * i + 2] += __double2ll_rn(f[2] * PM_FORCE_FIX);
+}
+
+// per table entry: intra-molecular energy and virial, block partial sums
+__global__ __launch_bounds__(256) void k_corr_energy(PmBox box, const double* __restrict__ pos,
+                                                     const double* __restrict__ q, int ntab,
+                                                     const int* __restrict__ ta, const int* __restrict__ tb,
+                                                     const int* __restrict__ tc, double alpha, double csf,
+                                                     int want_virial, double* __restrict__ epart) {
+  __shared__ double s_red[4][7];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (t < ntab) {
+    double rij[3], fij[3], e;
+    corr_pair(box, pos, q, ta[t], tb[t], tc[t], alpha, csf, rij, fij, e);
+    v[0] = e;
+    if (want_virial) {
+      // ScalarStructure::addVirial (base/ScalarStructure.cpp:41-63)
+      v[1] = fij[0] * rij[0]; v[2] = fij[0] * rij[1]; v[3] = fij[0] * rij[2];
+      v[4] = fij[1] * rij[1]; v[5] = fij[1] * rij[2]; v[6] = fij[2] * rij[2];
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int u = 0; u < 7; u++) {
+    double x = v[u];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) s_red[wv][u] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s[7];
+    for (int u = 0; u < 7; u++) s[u] = s_red[0][u] + s_red[1][u] + s_red[2][u] + s_red[3][u];
+    double* ep = epart + (long long)blockIdx.x * PM_NE;
+    ep[PM_E_COUL] = s[0];
+    for (int u = 1; u < PM_E_VIR; u++) ep[u] = 0.0;
+    ep[PM_E_VIR + 0] = s[1]; ep[PM_E_VIR + 1] = s[2]; ep[PM_E_VIR + 2] = s[3];
+    ep[PM_E_VIR + 3] = s[2]; ep[PM_E_VIR + 4] = s[4]; ep[PM_E_VIR + 5] = s[5];
+    ep[PM_E_VIR + 6] = s[3]; ep[PM_E_VIR + 7] = s[5]; ep[PM_E_VIR + 8] = s[6];
+  }
+}
+
+// deterministic sum of nparts energy records plus constant terms
+__global__ __launch_bounds__(256) void k_reduce_energy(const double* __restrict__ parts, int nparts,
+                                                       double const_coul, double const_vir_diag,
+                                                       double* __restrict__ out, int overwrite) {
+  __shared__ double s[256];
+  for (int c = 0; c < PM_NE; c++) {
+    double acc = 0;
+    for (int p = threadIdx.x; p < nparts; p += 256) acc += parts[(long long)p * PM_NE + c];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double v = s[0];
+      if (c == PM_E_COUL) v += const_coul;
+      if (c == PM_E_VIR + 0 || c == PM_E_VIR + 4 || c == PM_E_VIR + 8) v += const_vir_diag;
+      out[c] = overwrite ? v : out[c] + v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_ll(long long* p, long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+__global__ void k_zero_i(int* p, long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+__global__ void k_zero_d(double* p, long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0.0;
+}
+__global__ void k_fix_to_double(int n3, const long long* __restrict__ f, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) out[i] = (double)f[i] * PM_FORCE_UNFIX;
+}
+__global__ void k_add_fix(int n3, const long long* __restrict__ f, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) out[i] += (double)f[i] * PM_FORCE_UNFIX;
+}
+
+// LeapfrogIntegrator::doHalfKickdoDrift / doKickdoDrift
+// (integrators/LeapfrogIntegrator.cpp:46-104) and StandardIntegrator::doHalfKick
+// (StandardIntegrator.cpp:83-96).  mode 0: v += F*h*0.5/m then x += v*h;
+// mode 1: v += F*h/m then x += v*h; mode 2: v += F*hh/m (hh = 0.5*dt*INV_TIMEFACTOR).
+__global__ void k_kick_drift(int n, int mode, double h, const double* __restrict__ mass,
+                             const long long* __restrict__ f, double* __restrict__ v, double* __restrict__ x) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double F = (double)f[3 * i + k] * PM_FORCE_UNFIX;
+    double vv = v[3 * i + k];
+    if (mode == 0) vv += F * h * 0.5 / m;
+    else vv += F * h / m;
+    v[3 * i + k] = vv;
+    if (mode != 2) x[3 * i + k] += vv * h;
+  }
+}
+
+// counter-based normal deviates (SplitMix64 hash + Box-Muller)
+__device__ inline unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double gauss(unsigned long long seed, unsigned long long counter, unsigned long long idx) {
+  unsigned long long h1 = mix64(seed ^ mix64(counter * 0x100000001B3ull + idx * 2));
+  unsigned long long h2 = mix64(h1 ^ 0x5851F42D4C957F2Dull);
+  double u1 = ((h1 >> 11) + 1) * (1.0 / 9007199254740993.0);   // (0, 1]
+  double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);         // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// LangevinLeapfrogIntegrator::doHalfKick (integrators/LangevinLeapfrogIntegrator.cpp:58-79)
+__global__ void k_langevin_kick(int n, double dt, double temperature, double g, const double* __restrict__ mass,
+                                const long long* __restrict__ f, double* __restrict__ v,
+                                unsigned long long seed, unsigned long long counter) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double fdt = (1.0 - exp(-0.5 * g * dt)) / g;
+  const double vdt = exp(-0.5 * g * dt);
+  const double ndt = sqrt((1.0 - exp(-g * dt)) / (2.0 * g));
+  const double fc = 2 * 0.001987191 * temperature * g;
+  const double m = mass[i];
+  const double s = sqrt(fc / m);
+  for (int k = 0; k < 3; k++) {
+    const double F = (double)f[3 * i + k] * PM_FORCE_UNFIX;
+    const double xi = gauss(seed, counter, 3ull * i + k);
+    v[3 * i + k] = v[3 * i + k] * vdt + F * fdt / m + xi * s * ndt;
+  }
+}
+
+__global__ void k_drift(int n3, double h, const double* __restrict__ v, double* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) x[i] += h * v[i];   // Vector3DBlock::intoWeightedAdd
+}
+
+inline dim3 grid_for(long long n, int b) { return dim3((unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1)); }
+
+}  // namespace
+
+void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
+                          const int* row, const int* partner, const int* cls, double alpha, double csf,
+                          int want_virial, long long* force_fix, int ntab, const int* tab_a,
+                          const int* tab_b, const int* tab_c, double* epart, int nblk) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_corr_force, grid_for(n, 256), dim3(256), 0, s, box, n, pos, q, row, partner, cls, alpha,
+                       csf, force_fix);
+  if (nblk > 0)
+    hipLaunchKernelGGL(k_corr_energy, dim3(nblk), dim3(256), 0, s, box, pos, q, ntab, tab_a, tab_b, tab_c, alpha,
+                       csf, want_virial, epart);
+}
+
+void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, double constant_coul,
+                             double constant_vir_diag, double* out, int overwrite) {
+  hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, s, parts, nparts, constant_coul, constant_vir_diag,
+                     out, overwrite);
+}
+
+void pm_launch_zero_ll(hipStream_t s, long long* p, long long count) {
+  if (count > 0) hipLaunchKernelGGL(k_zero_ll, grid_for(count, 256), dim3(256), 0, s, p, count);
+}
+void pm_launch_zero_i(hipStream_t s, int* p, long long count) {
+  if (count > 0) hipLaunchKernelGGL(k_zero_i, grid_for(count, 256), dim3(256), 0, s, p, count);
+}
+void pm_launch_zero_d(hipStream_t s, double* p, long long count) {
+  if (count > 0) hipLaunchKernelGGL(k_zero_d, grid_for(count, 256), dim3(256), 0, s, p, count);
+}
+void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* out) {
+  if (n3 > 0) hipLaunchKernelGGL(k_fix_to_double, grid_for(n3, 256), dim3(256), 0, s, n3, f, out);
+}
+void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out) {
+  if (n3 > 0) hipLaunchKernelGGL(k_add_fix, grid_for(n3, 256), dim3(256), 0, s, n3, f, out);
+}
+void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
+                    double* v, double* x) {
+  if (n > 0) hipLaunchKernelGGL(k_kick_drift, grid_for(n, 256), dim3(256), 0, s, n, mode, h, mass, f, v, x);
+}
+void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double temperature, double gamma_akma,
+                             const double* mass, const long long* f, double* v, unsigned long long seed,
+                             unsigned long long counter) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_langevin_kick, grid_for(n, 256), dim3(256), 0, s, n, dt_akma, temperature, gamma_akma,
+                       mass, f, v, seed, counter);
+}
+void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x) {
+  if (n3 > 0) hipLaunchKernelGGL(k_drift, grid_for(n3, 256), dim3(256), 0, s, n3, h, v, x);
+}

@@ -1,0 +1,746 @@
+// pm_nonbonded.hip -- cell binning, slot layout, neighbour lists and the
+// cutoff pair kernels (gfx950, wave64).
+//
+// Binning reproduces topology/buildCellLists.cpp:11-38 bit for bit (FP64,
+// no contraction); the pair kernels evaluate OneAtomPair / OneAtomPairTwo
+// (forces/OneAtomPair.h:127-188, forces/OneAtomPairTwo.h:131-199) for every
+// pair within the switching cutoff, each pair from both sides, so that every
+// atom's force is the sum of one lane group in a fixed order (no atomics,
+// deterministic run to run).
+#include "pm_internal.h"
+
+#include <math.h>
+#include <type_traits>
+
+namespace {
+
+// ------------------------------------------------------------------ binning
+// PeriodicBoundaryConditions::minimalPosition (topology/PeriodicBoundaryConditions.h:120-160)
+__device__ inline void min_pos(const PmBox& b, double cx, double cy, double cz, double& ox,
+                               double& oy, double& oz) {
+#pragma clang fp contract(off)
+  double x = cx - b.origin[0], y = cy - b.origin[1], z = cz - b.origin[2];
+  if (x * x + y * y + z * z <= b.d) { ox = x; oy = y; oz = z; return; }
+  if (x < -b.h2[0]) x += b.h[0]; else if (x > b.h2[0]) x -= b.h[0];
+  if (y < -b.h2[1]) y += b.h[1]; else if (y > b.h2[1]) y -= b.h[1];
+  if (z < -b.h2[2]) z += b.h[2]; else if (z > b.h2[2]) z -= b.h[2];
+  if (x * x + y * y + z * z <= b.d) { ox = x; oy = y; oz = z; return; }
+  double tx = b.L[0] * __builtin_rint(b.er[0] * x);
+  double ty = b.L[1] * __builtin_rint(b.er[1] * y);
+  double tz = b.L[2] * __builtin_rint(b.er[2] * z);
+  ox = x - tx; oy = y - ty; oz = z - tz;
+}
+
+// CubicCellManager::findCell (topology/CubicCellManager.h:61-64); coordinates
+// outside [0, n) (the reference's x == +L/2 edge, or non-finite input) are
+// clamped into the grid so that no index leaves its buffer.
+__device__ inline int cell_index(const PmBox& b, double px, double py, double pz, double* mpz) {
+#pragma clang fp contract(off)
+  double mx, my, mz;
+  min_pos(b, px, py, pz, mx, my, mz);
+  double v[3] = {b.delta[0] + mx, b.delta[1] + my, b.delta[2] + mz};
+  int c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double f = __builtin_floor(v[k] * b.rreal[k]);
+    int ck;
+    if (!(f >= 0.0)) ck = 0;
+    else if (f >= (double)b.n[k]) ck = b.n[k] - 1;
+    else ck = (int)f;
+    c[k] = ck;
+  }
+  if (mpz) *mpz = mz;
+  return (c[0] * b.n[1] + c[1]) * b.n[2] + c[2];
+}
+
+__global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
+                      int* __restrict__ cell_count, float* __restrict__ key) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mz;
+  int c = cell_index(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &mz);
+  cell_of[i] = c;
+  key[i] = (float)mz;
+  atomicAdd(&cell_count[c], 1);
+}
+
+// Exclusive scans of the cell counts (atoms and M-padded slots) in one
+// workgroup; also assigns every cluster its cell.
+__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
+                                                     int* __restrict__ start, int* __restrict__ slot,
+                                                     int* __restrict__ cursor, int* __restrict__ counts,
+                                                     int* __restrict__ cluster_cell, int ncl_max) {
+  __shared__ int s_a[1024], s_s[1024];
+  int t = threadIdx.x;
+  int per = (ncells + 1023) / 1024;
+  int lo = t * per, hi = min(ncells, lo + per);
+  int sa = 0, ss = 0;
+  for (int c = lo; c < hi; c++) { sa += cnt[c]; ss += (cnt[c] + PM_M - 1) / PM_M * PM_M; }
+  s_a[t] = sa; s_s[t] = ss;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int va = t >= off ? s_a[t - off] : 0;
+    int vs = t >= off ? s_s[t - off] : 0;
+    __syncthreads();
+    s_a[t] += va; s_s[t] += vs;
+    __syncthreads();
+  }
+  int ra = s_a[t] - sa, rs = s_s[t] - ss;
+  for (int c = lo; c < hi; c++) {
+    int k = cnt[c];
+    int ps = (k + PM_M - 1) / PM_M * PM_M;
+    start[c] = ra; cursor[c] = ra; slot[c] = rs;
+    for (int u = rs / PM_M; u < (rs + ps) / PM_M && u < ncl_max; u++) cluster_cell[u] = c;
+    ra += k; rs += ps;
+  }
+  if (t == 1023) { counts[0] = s_s[1023]; counts[1] = s_s[1023] / PM_M; counts[2] = s_a[1023]; }
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ cell_of, int* __restrict__ cursor,
+                          int* __restrict__ cell_atoms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int p = atomicAdd(&cursor[cell_of[i]], 1);
+  cell_atoms[p] = i;
+}
+
+__device__ inline unsigned int to_fix(double x, double s) {
+  // periodic 32-bit fixed point: round(x * 2^32 / L) mod 2^32
+  long long v = __double2ll_rn(x * s);
+  return (unsigned int)v;
+}
+
+__device__ inline unsigned long long to_fix64(double x, double s) {
+  // round(x * 2^64 / L) mod 2^64 from round(x * 2^32 / L) split in two
+  double t = x * s;                       // units of 2^-32 box
+  double hi = __builtin_floor(t);
+  double lo = (t - hi) * 4294967296.0;    // fraction in units of 2^-64 box
+  unsigned long long h = (unsigned long long)(long long)hi;
+  unsigned long long l = (unsigned long long)__double2ll_rn(lo);
+  return (h << 32) + l;
+}
+
+// One thread per cell: order the cell's atoms by (z, index) and write the
+// padded slot arrays.
+__global__ void k_fill_slots(PmBox box, int ncells, const double* __restrict__ pos,
+                             const double* __restrict__ q, const int* __restrict__ type,
+                             const int* __restrict__ cell_count, const int* __restrict__ cell_start,
+                             const int* __restrict__ cell_slot, int* __restrict__ cell_atoms,
+                             const float* __restrict__ key, int4* __restrict__ slot_pos,
+                             longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
+                             int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) {
+    // the shared padding slot of the pair lists
+    slot_pos[dummy_slot] = make_int4(0, 0, 0, 0);
+    if (slot_pos64) slot_pos64[dummy_slot] = make_longlong4(0, 0, 0, 0);
+    slot_meta[dummy_slot] = dummy_type | (0x7fffff << 8);
+    atom_of_slot[dummy_slot] = -1;
+  }
+  if (c >= ncells) return;
+  int k = cell_count[c];
+  if (k == 0) return;
+  int* a = cell_atoms + cell_start[c];
+  // insertion sort by (key, index); a cell holding more than PM_SORT_MAX atoms
+  // (collapsed or non-finite coordinates) keeps its arrival order so that the
+  // thread's work stays bounded
+  const int ks = k <= PM_SORT_MAX ? k : 0;
+  for (int u = 1; u < ks; u++) {
+    int v = a[u];
+    float kv = key[v];
+    int w = u - 1;
+    while (w >= 0) {
+      int aw = a[w];
+      float kw = key[aw];
+      if (kw > kv || (kw == kv && aw > v)) { a[w + 1] = aw; w--; }
+      else break;
+    }
+    a[w + 1] = v;
+  }
+  int s0 = cell_slot[c];
+  int4 first = make_int4(0, 0, 0, 0);
+  for (int u = 0; u < k; u++) {
+    int at = a[u];
+    int4 p;
+    p.x = (int)to_fix(pos[3 * at], box.to_fix[0]);
+    p.y = (int)to_fix(pos[3 * at + 1], box.to_fix[1]);
+    p.z = (int)to_fix(pos[3 * at + 2], box.to_fix[2]);
+    p.w = __float_as_int((float)q[at]);
+    if (u == 0) first = p;
+    slot_pos[s0 + u] = p;
+    if (slot_pos64) {
+      // 64-bit periodic fixed point for the FP64 kernels
+      longlong4 p8;
+      p8.x = (long long)to_fix64(pos[3 * at], box.to_fix[0]);
+      p8.y = (long long)to_fix64(pos[3 * at + 1], box.to_fix[1]);
+      p8.z = (long long)to_fix64(pos[3 * at + 2], box.to_fix[2]);
+      p8.w = __double_as_longlong(q[at]);
+      slot_pos64[s0 + u] = p8;
+    }
+    slot_meta[s0 + u] = type[at] | (at << 8);
+    atom_of_slot[s0 + u] = at;
+  }
+  int ps = (k + PM_M - 1) / PM_M * PM_M;
+  for (int u = k; u < ps; u++) {
+    int4 p = first;
+    p.w = 0;
+    slot_pos[s0 + u] = p;
+    if (slot_pos64) {
+      longlong4 p8 = slot_pos64[s0];
+      p8.w = 0;
+      slot_pos64[s0 + u] = p8;
+    }
+    slot_meta[s0 + u] = dummy_type | (0x7fffff << 8);
+    atom_of_slot[s0 + u] = -1;
+  }
+}
+
+// ------------------------------------------------------------ wave helpers
+__device__ inline int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+__device__ inline float wave_min_m(float v) {
+#pragma unroll
+  for (int off = 1; off < PM_M; off <<= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline float wave_max_m(float v) {
+#pragma unroll
+  for (int off = 1; off < PM_M; off <<= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline int wave_imin_m(int v) {
+#pragma unroll
+  for (int off = 1; off < PM_M; off <<= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline int wave_imax_m(int v) {
+#pragma unroll
+  for (int off = 1; off < PM_M; off <<= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ inline int wrap_cell(int c, int d, int n) {
+  int v = (c + d) % n;
+  return v < 0 ? v + n : v;
+}
+
+// ------------------------------------------------------------- pair lists
+// One wave per i-cluster: every atom of the cells in the full stencil whose
+// distance to the cluster's bounding box is below the list cutoff is appended
+// to the cluster's list (bit 31 marks a possible exclusion or self pair).
+__global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) {
+  extern __shared__ int s_dyn[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nst = a.nstencil;
+  int* s_start = s_dyn + wave * (3 * nst + 1);
+  int* s_cnt = s_start + nst;
+  int* s_pref = s_cnt + nst;     // nst + 1 entries
+  const int cl = blockIdx.x * 4 + wave;
+  const int ncl = a.counts[1];
+  const bool active = cl < ncl;  // every wave reaches every barrier below
+  const int cell = active ? a.cluster_cell[cl] : 0;
+  const int cz = cell % box.n[2], cy = (cell / box.n[2]) % box.n[1], cx = cell / (box.n[1] * box.n[2]);
+
+  // i-cluster: anchor, bounding box and index range
+  const int4 anchor = active ? a.slot_pos[cl * PM_M] : make_int4(0, 0, 0, 0);
+  const float fx = (float)box.fix_to_A[0], fy = (float)box.fix_to_A[1], fz = (float)box.fix_to_A[2];
+  float lox = 1e30f, loy = 1e30f, loz = 1e30f, hix = -1e30f, hiy = -1e30f, hiz = -1e30f;
+  int omin = 0x7fffffff, omax = -1;
+  if (active && lane < PM_M) {
+    int4 p = a.slot_pos[cl * PM_M + lane];
+    int m = a.slot_meta[cl * PM_M + lane];
+    int o = (m >> 8) & 0x7fffff;
+    if (o != 0x7fffff) {
+      float dx = (float)(int)(p.x - anchor.x) * fx;
+      float dy = (float)(int)(p.y - anchor.y) * fy;
+      float dz = (float)(int)(p.z - anchor.z) * fz;
+      lox = hix = dx; loy = hiy = dy; loz = hiz = dz;
+      omin = omax = o;
+    }
+  }
+  lox = wave_min_m(lox); loy = wave_min_m(loy); loz = wave_min_m(loz);
+  hix = wave_max_m(hix); hiy = wave_max_m(hiy); hiz = wave_max_m(hiz);
+  omin = wave_imin_m(omin); omax = wave_imax_m(omax);
+  lox = __shfl(lox, 0, 64); loy = __shfl(loy, 0, 64); loz = __shfl(loz, 0, 64);
+  hix = __shfl(hix, 0, 64); hiy = __shfl(hiy, 0, 64); hiz = __shfl(hiz, 0, 64);
+  omin = __shfl(omin, 0, 64); omax = __shfl(omax, 0, 64);
+  const int flo = omin - a.max_delta, fhi = omax + a.max_delta;
+
+  // stencil cells and their prefix counts
+  if (active) {
+    for (int s = lane; s < nst; s += 64) {
+      int4 d = a.stencil[s];
+      int nb = (wrap_cell(cx, d.x, box.n[0]) * box.n[1] + wrap_cell(cy, d.y, box.n[1])) * box.n[2] +
+               wrap_cell(cz, d.z, box.n[2]);
+      s_start[s] = a.cell_slot[nb];
+      s_cnt[s] = a.cell_count[nb];
+    }
+  }
+  __syncthreads();
+  int carry = 0;
+  for (int s0 = 0; s0 < nst; s0 += 64) {
+    int s = s0 + lane;
+    int v = (active && s < nst) ? s_cnt[s] : 0;
+    int inc = wave_incl_scan(v, lane);
+    if (active && s < nst) s_pref[s] = carry + inc - v;
+    carry += __shfl(inc, 63, 64);
+  }
+  if (active && lane == 0) s_pref[nst] = carry;
+  __syncthreads();
+  if (!active) return;
+  const int total = carry;
+
+  int* out = a.jlist + (long long)cl * a.lcap;
+  int count = 0;
+  // candidates beyond PM_CAND_FACTOR * lcap mean a degenerate cell layout:
+  // stop scanning and report it as a list overflow
+  const int scan = min(total, PM_CAND_FACTOR * a.lcap);
+  if (scan < total && lane == 0) a.overflow[0] = 1;
+  for (int k0 = 0; k0 < scan; k0 += 64) {
+    int k = k0 + lane;
+    bool pass = false;
+    int entry = 0;
+    if (k < scan) {
+      int lo = 0, hi = nst;  // largest s with s_pref[s] <= k
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= k) lo = mid; else hi = mid;
+      }
+      int sj = s_start[lo] + (k - s_pref[lo]);
+      int4 p = a.slot_pos[sj];
+      pass = true;
+      if (a.bbox_filter) {
+        float dx = (float)(int)(p.x - anchor.x) * fx;
+        float dy = (float)(int)(p.y - anchor.y) * fy;
+        float dz = (float)(int)(p.z - anchor.z) * fz;
+        float gx = fmaxf(0.f, fmaxf(lox - dx, dx - hix));
+        float gy = fmaxf(0.f, fmaxf(loy - dy, dy - hiy));
+        float gz = fmaxf(0.f, fmaxf(loz - dz, dz - hiz));
+        pass = gx * gx + gy * gy + gz * gz < cut2_list;
+      }
+      int o = (a.slot_meta[sj] >> 8) & 0x7fffff;
+      bool flag = (o >= flo) && (o <= fhi);
+      entry = sj | (flag ? (int)0x80000000 : 0);
+    }
+    unsigned long long bal = __ballot(pass);
+    int before = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+    int idx = count + before;
+    if (pass && idx < a.lcap) out[idx] = entry;
+    count += __popcll(bal);
+  }
+  if (lane == 0) {
+    if (count > a.lcap) { a.overflow[0] = 1; count = a.lcap; }
+    a.jcount[cl] = count;
+    if (a.stats) {
+      atomicAdd(&a.stats[0], (unsigned long long)count);
+      atomicAdd(&a.stats[1], (unsigned long long)((count + PM_JW - 1) / PM_JW * 64));
+    }
+  }
+}
+
+// ------------------------------------------------------------ pair kernel
+template <typename T> struct Prec;
+template <> struct Prec<float> {
+  static __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+  static __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ inline float expneg(float x) {  // exp(-x), x >= 0
+    return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f);
+  }
+  static __device__ inline float sqrt_(float x) { return __builtin_sqrtf(x); }
+};
+template <> struct Prec<double> {
+  static __device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
+  static __device__ inline double rcp(double x) { return 1.0 / x; }
+  static __device__ inline double expneg(double x) { return exp(-x); }
+  static __device__ inline double sqrt_(double x) { return sqrt(x); }
+};
+
+// C1 / C2 / Cutoff switching functions (topology/C1SwitchingFunction.h:64-75,
+// C2SwitchingFunction.h:64-82, CutoffSwitchingFunction.h:64-68).
+template <typename T, int SWF>
+__device__ inline void switching(T r2, T dist, T cut2, T on2, T sw1, T sw2, T sw3, T c15_1, T c05_3,
+                                 T c15_3, T& v, T& d) {
+  if constexpr (SWF == PM_SWF_C2) {
+    T c2 = cut2 - r2;
+    T c4 = c2 * (sw2 + T(2) * r2);
+    bool in = r2 >= on2;
+    v = in ? sw1 * (c2 * c4) : T(1);
+    d = in ? sw3 * (c2 * c2 - c4) : T(0);
+    if (r2 > cut2) { v = T(0); d = T(0); }
+  } else if constexpr (SWF == PM_SWF_C1) {
+    v = T(1) - dist * (c15_1 - r2 * c05_3);
+    d = -c15_1 * Prec<T>::rcp(dist) + dist * c15_3;
+    if (r2 > cut2) { v = T(0); d = T(0); }
+  } else {
+    v = r2 > cut2 ? T(0) : T(1);
+    d = T(0);
+  }
+}
+
+__device__ inline int excl_class(const PmNbLaunch& a, int oi, int oj) {
+  if (oi == oj) return PM_EXCL_FULL;
+  if (a.molecule[oi] != a.molecule[oj]) return PM_EXCL_NONE;   // OneAtomPair.h:149-153
+  int lo = min(oi, oj), hi = max(oi, oj);
+  int r = PM_EXCL_NONE;
+  for (int u = a.ex_row[lo]; u < a.ex_row[lo + 1]; u++)
+    if (a.ex_partner[u] == hi) r = a.ex_cls[u];   // last add wins (ExclusionTable::add)
+  return r;
+}
+
+template <typename T, int LJSWF, int CK, int CSWF>
+__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) {
+  extern __shared__ unsigned char s_lj_raw[];
+  typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
+  LJ4* s_lj = reinterpret_cast<LJ4*>(s_lj_raw);
+  const int nt1 = a.ntypes1;
+  if constexpr (LJSWF != PM_SWF_NONE) {
+    const LJ4* src;
+    if constexpr (sizeof(T) == 4) src = a.lj_f; else src = a.lj_d;
+    for (int t = threadIdx.x; t < nt1 * nt1; t += blockDim.x) s_lj[t] = src[t];
+    __syncthreads();
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int cl = blockIdx.x * 4 + wave;
+  const int ncl = a.counts[1];
+  if (cl >= ncl) {
+    if (cl < a.ncl_max && lane < PM_NE) a.epart[(long long)cl * PM_NE + lane] = 0.0;
+    return;
+  }
+  const int ii = lane % PM_M, js = lane / PM_M;
+  const int si = cl * PM_M + ii;
+  const int mi = a.slot_meta[si];
+  const int ti = mi & 0xff, oi = (mi >> 8) & 0x7fffff;
+  const bool ireal = oi != 0x7fffff;
+  // positions: 32-bit periodic fixed point (mixed) or 64-bit (FP64); the
+  // wrapped integer difference is the minimal-image vector
+  typedef typename std::conditional<sizeof(T) == 4, int4, longlong4>::type P4;
+  P4 pi;
+  if constexpr (sizeof(T) == 4) pi = a.slot_pos[si]; else pi = a.slot_pos64[si];
+  T qi;
+  if constexpr (sizeof(T) == 4) qi = __int_as_float(pi.w); else qi = __longlong_as_double(pi.w);
+  const T sx = sizeof(T) == 4 ? (T)box.fix_to_A[0] : (T)(box.fix_to_A[0] * 2.3283064365386963e-10);
+  const T sy = sizeof(T) == 4 ? (T)box.fix_to_A[1] : (T)(box.fix_to_A[1] * 2.3283064365386963e-10);
+  const T sz = sizeof(T) == 4 ? (T)box.fix_to_A[2] : (T)(box.fix_to_A[2] * 2.3283064365386963e-10);
+
+  // parameters in working precision
+  const T cut2 = (T)pp.cut2;
+  const T lj_cut2 = (T)pp.lj_cut2, lj_on2 = (T)pp.lj_on2, lj_sw1 = (T)pp.lj_sw1, lj_sw2 = (T)pp.lj_sw2,
+          lj_sw3 = (T)pp.lj_sw3, lj_c15_1 = (T)pp.lj_c15_1, lj_c05_3 = (T)pp.lj_c05_3,
+          lj_c15_3 = (T)pp.lj_c15_3;
+  const T c_cut2 = (T)pp.c_cut2, c_on2 = (T)pp.c_on2, c_sw1 = (T)pp.c_sw1, c_sw2 = (T)pp.c_sw2,
+          c_sw3 = (T)pp.c_sw3, c_c15_1 = (T)pp.c_c15_1, c_c05_3 = (T)pp.c_c05_3, c_c15_3 = (T)pp.c_c15_3;
+  const T alpha = (T)pp.alpha, tapi = (T)pp.two_alpha_pi, csf = (T)pp.csf;
+
+  double fxa = 0, fya = 0, fza = 0;
+  T elj = 0, ec = 0;
+  T vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+  const int* lst = a.jlist + (long long)cl * a.lcap;
+  const int cnt = a.jcount[cl];
+  const bool vir = a.want_virial != 0;
+
+  for (int base = 0; base < cnt; base += PM_JW) {
+    const int k = base + js;
+    const bool valid = k < cnt;
+    const int e = valid ? lst[k] : a.dummy_slot;
+    const bool flagged = e < 0;
+    const int sj = e & 0x7fffffff;
+    P4 pj;
+    if constexpr (sizeof(T) == 4) pj = a.slot_pos[sj]; else pj = a.slot_pos64[sj];
+    const int mj = a.slot_meta[sj];
+    T dx, dy, dz;
+    if constexpr (sizeof(T) == 4) {
+      dx = (T)(int)(pj.x - pi.x) * sx;
+      dy = (T)(int)(pj.y - pi.y) * sy;
+      dz = (T)(int)(pj.z - pi.z) * sz;
+    } else {
+      dx = (T)(long long)((unsigned long long)pj.x - (unsigned long long)pi.x) * sx;
+      dy = (T)(long long)((unsigned long long)pj.y - (unsigned long long)pi.y) * sy;
+      dz = (T)(long long)((unsigned long long)pj.z - (unsigned long long)pi.z) * sz;
+    }
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    bool mask = valid && ireal && (r2 <= cut2);
+    int excl = PM_EXCL_NONE;
+    if (__ballot(flagged && mask) != 0ull) {
+      if (flagged && mask) {
+        excl = excl_class(a, oi, (mj >> 8) & 0x7fffff);
+        if (excl == PM_EXCL_FULL) mask = false;
+      }
+    }
+    T qj;
+    if constexpr (sizeof(T) == 4) qj = __int_as_float(pj.w); else qj = __longlong_as_double(pj.w);
+    T rr2, rinv;
+    if constexpr (sizeof(T) == 4) { rinv = Prec<T>::rsq(r2); rr2 = rinv * rinv; }
+    else { rr2 = T(1) / r2; rinv = sqrt(rr2); }
+    T F = 0, E1 = 0, E2 = 0;
+    if constexpr (LJSWF != PM_SWF_NONE) {
+      const int tj = mj & 0xff;
+      const LJ4 ab = s_lj[ti * nt1 + tj];
+      const T A = excl == PM_EXCL_MODIFIED ? (T)ab.z : (T)ab.x;
+      const T B = excl == PM_EXCL_MODIFIED ? (T)ab.w : (T)ab.y;
+      const T r6 = rr2 * rr2 * rr2;
+      const T r12 = r6 * r6;
+      const T r6B = B * r6;
+      const T r12A = A * r12;
+      T e1 = r12A - r6B;
+      T f1 = T(12) * r12A * rr2 - T(6) * r6B * rr2;
+      if (pp.modify) {
+        T v, d;
+        switching<T, LJSWF == PM_SWF_CUTOFF ? PM_SWF_CUTOFF : LJSWF>(
+            r2, r2 * rinv, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3, lj_c15_3, v, d);
+        f1 = f1 * v - e1 * d;
+        e1 = e1 * v;
+      }
+      F += f1;
+      E1 = e1;
+    }
+    if constexpr (CK != PM_COUL_NONE) {
+      T e2, f2;
+      if constexpr (CK == PM_COUL_PLAIN) {
+        e2 = qi * qj * rinv;
+        if (excl == PM_EXCL_MODIFIED) e2 *= csf;
+        f2 = e2 * rr2;
+      } else {
+        T qq = qi * qj;
+        if (excl == PM_EXCL_MODIFIED) qq *= csf;
+        const T r = r2 * rinv;
+        const T ar = alpha * r;
+        const T t = Prec<T>::rcp(T(1) + T(0.3275911) * ar);
+        const T poly = t * (T(0.254829592) + t * (T(-0.284496736) + t * (T(1.421413741) +
+                                                 t * (T(-1.453152027) + t * T(1.061405429)))));
+        const T ex = qq * Prec<T>::expneg(ar * ar);
+        e2 = poly * ex * rinv;
+        f2 = (e2 + tapi * ex) * rinv * rinv;
+      }
+      if (pp.modify) {
+        T v, d;
+        switching<T, CSWF>(r2, r2 * rinv, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3,
+                           c_c15_3, v, d);
+        f2 = f2 * v - e2 * d;
+        e2 = e2 * v;
+      }
+      F += f2;
+      E2 = e2;
+    }
+    F = mask ? F : T(0);
+    E1 = mask ? E1 : T(0);
+    E2 = mask ? E2 : T(0);
+    const T fx_ = F * dx, fy_ = F * dy, fz_ = F * dz;
+    fxa -= (double)fx_;
+    fya -= (double)fy_;
+    fza -= (double)fz_;
+    elj += E1;
+    ec += E2;
+    if (vir) {
+      vxx += fx_ * dx; vxy += fx_ * dy; vxz += fx_ * dz;
+      vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
+    }
+  }
+  // reduce the i-force over the j-slots (lanes ii, ii+M, ...)
+#pragma unroll
+  for (int off = PM_M; off < 64; off <<= 1) {
+    fxa += __shfl_xor(fxa, off, 64);
+    fya += __shfl_xor(fya, off, 64);
+    fza += __shfl_xor(fza, off, 64);
+  }
+  if (js == 0 && ireal) {
+    a.force_fix[3 * oi] += __double2ll_rn(fxa * PM_FORCE_FIX);
+    a.force_fix[3 * oi + 1] += __double2ll_rn(fya * PM_FORCE_FIX);
+    a.force_fix[3 * oi + 2] += __double2ll_rn(fza * PM_FORCE_FIX);
+  }
+  // energies and virial (each pair was seen from both sides: factor 1/2)
+  double r[8] = {ireal ? (double)ec : 0.0, ireal ? (double)elj : 0.0,
+                 ireal ? (double)vxx : 0.0, ireal ? (double)vxy : 0.0, ireal ? (double)vxz : 0.0,
+                 ireal ? (double)vyy : 0.0, ireal ? (double)vyz : 0.0, ireal ? (double)vzz : 0.0};
+  const int nr = vir ? 8 : 2;
+  for (int u = 0; u < nr; u++) {
+    double v = r[u];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+    r[u] = v * 0.5;
+  }
+  if (lane == 0) {
+    double* ep = a.epart + (long long)cl * PM_NE;
+    ep[PM_E_COUL] = r[0];
+    ep[PM_E_LJ] = r[1];
+    for (int u = 2; u < PM_E_VIR; u++) ep[u] = 0.0;
+    if (vir) {
+      // fij . diff with fij = F*d; the reference adds virial of +fij (ScalarStructure::addVirial)
+      ep[PM_E_VIR + 0] = r[2]; ep[PM_E_VIR + 1] = r[3]; ep[PM_E_VIR + 2] = r[4];
+      ep[PM_E_VIR + 3] = r[3]; ep[PM_E_VIR + 4] = r[5]; ep[PM_E_VIR + 5] = r[6];
+      ep[PM_E_VIR + 6] = r[4]; ep[PM_E_VIR + 7] = r[6]; ep[PM_E_VIR + 8] = r[7];
+    } else {
+      for (int u = 0; u < 9; u++) ep[PM_E_VIR + u] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------- binning diagnostics
+// PeriodicBoundaryConditions::minimalDifference (.h:76-117), orthogonal branch.
+__device__ inline double min_diff_r2(const PmBox& b, const double* c1, const double* c2) {
+#pragma clang fp contract(off)
+  double x = c2[0] - c1[0], y = c2[1] - c1[1], z = c2[2] - c1[2];
+  double s = x * x + y * y + z * z;
+  if (s > b.d) {
+    if (x < -b.h2[0]) x += b.h[0]; else if (x > b.h2[0]) x -= b.h[0];
+    if (y < -b.h2[1]) y += b.h[1]; else if (y > b.h2[1]) y -= b.h[1];
+    if (z < -b.h2[2]) z += b.h[2]; else if (z > b.h2[2]) z -= b.h[2];
+    s = x * x + y * y + z * z;
+    if (s > b.d) {
+      x -= b.L[0] * __builtin_rint(b.er[0] * x);
+      y -= b.L[1] * __builtin_rint(b.er[1] * y);
+      z -= b.L[2] * __builtin_rint(b.er[2] * z);
+      s = x * x + y * y + z * z;
+    }
+  }
+  return s;
+}
+
+__device__ inline bool diag_pair_in(const PmBox& b, const double* pos, int i, int j, double cut2,
+                                    const int* molecule, const int* ex_row, const int* ex_partner,
+                                    const int* ex_cls) {
+  double r2 = min_diff_r2(b, pos + 3 * i, pos + 3 * j);
+  if (r2 > cut2) return false;
+  if (molecule[i] == molecule[j]) {
+    int lo = min(i, j), hi = max(i, j), r = PM_EXCL_NONE;
+    for (int u = ex_row[lo]; u < ex_row[lo + 1]; u++)
+      if (ex_partner[u] == hi) r = ex_cls[u];
+    if (r == PM_EXCL_FULL) return false;
+  }
+  return true;
+}
+
+// One thread per cell: candidate and in-cutoff pairs of the cell pairs whose
+// first cell it is, in the enumeration of
+// topology/CellListEnumerator_periodicBoundaries.h:143-179.
+__global__ void k_diag(PmBox box, const double* __restrict__ pos, const int* __restrict__ cell_start,
+                       const int* __restrict__ cell_count, const int* __restrict__ cell_atoms,
+                       const int4* __restrict__ half, int nhalf, const int* __restrict__ molecule,
+                       const int* __restrict__ ex_row, const int* __restrict__ ex_partner,
+                       const int* __restrict__ ex_cls, double cut2, int ncells,
+                       long long* __restrict__ cand, long long* __restrict__ inrc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  long long nc = 0, ni = 0;
+  int k = cell_count[c];
+  if (k > PM_DIAG_MAX) {
+    // collapsed or non-finite coordinates: report instead of an unbounded loop
+    cand[c] = -1;
+    inrc[c] = -1;
+    return;
+  }
+  if (k > 0) {
+    const int* A = cell_atoms + cell_start[c];
+    for (int u = 0; u < k; u++)
+      for (int w = u + 1; w < k; w++) {
+        nc++;
+        ni += diag_pair_in(box, pos, A[u], A[w], cut2, molecule, ex_row, ex_partner, ex_cls);
+      }
+    int cz = c % box.n[2], cy = (c / box.n[2]) % box.n[1], cx = c / (box.n[1] * box.n[2]);
+    for (int t = 0; t < nhalf; t++) {
+      int4 d = half[t];
+      int nb = (wrap_cell(cx, d.x, box.n[0]) * box.n[1] + wrap_cell(cy, d.y, box.n[1])) * box.n[2] +
+               wrap_cell(cz, d.z, box.n[2]);
+      if (nb < c) continue;
+      int kb = cell_count[nb];
+      if (kb == 0 || kb > PM_DIAG_MAX) continue;
+      const int* B = cell_atoms + cell_start[nb];
+      for (int u = 0; u < k; u++)
+        for (int w = 0; w < kb; w++) {
+          nc++;
+          ni += diag_pair_in(box, pos, A[u], B[w], cut2, molecule, ex_row, ex_partner, ex_cls);
+        }
+    }
+  }
+  cand[c] = nc;
+  inrc[c] = ni;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- launchers
+void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
+                   int* cell_count, float* key) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, cell_of, cell_count, key);
+}
+
+void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
+                          int* cell_slot, int* cell_cursor, int* counts, int* cluster_cell, int ncl_max) {
+  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
+                     cell_cursor, counts, cluster_cell, ncl_max);
+}
+
+void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, int* cell_cursor, int* cell_atoms) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell_of, cell_cursor, cell_atoms);
+}
+
+void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
+                          const double* q, const int* type, const int* cell_count,
+                          const int* cell_start, const int* cell_slot, int* cell_atoms,
+                          const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
+                          int* atom_of_slot, int dummy_type, int dummy_slot) {
+  int nb = (ncells + 127) / 128;
+  hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(128), 0, s, box, ncells, pos, q, type,
+                     cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta,
+                     atom_of_slot, dummy_type, dummy_slot);
+}
+
+void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
+  double cut = sqrt(p.cut2);
+  float c = (float)(cut * (1.0 + 1e-5) + 1e-4);
+  size_t lds = 4 * (3 * a.nstencil + 1) * sizeof(int);
+  hipLaunchKernelGGL(k_list, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
+}
+
+template <typename T, int LJ, int CK, int CS>
+static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
+  size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
+  hipLaunchKernelGGL((k_pair<T, LJ, CK, CS>), dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, p);
+}
+
+template <typename T>
+static void launch_pair_p(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
+  const int lj = p.lj_swf, ck = p.coul_kind, cs = p.coul_swf;
+#define PM_CASE(L, C, S) \
+  if (lj == L && ck == C && (C == PM_COUL_NONE || cs == S)) { launch_pair_t<T, L, C, S>(s, box, a, p); return; }
+  // single-term forces (registerForceExemplarsCutoff.cpp:39-63)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C1)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C2)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_NONE, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C1, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_CUTOFF, PM_COUL_NONE, PM_SWF_CUTOFF)
+  // OneAtomPairTwo ids (:75-96) and fused LJ + Coulomb force pairs
+  PM_CASE(PM_SWF_C1, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C1)
+  PM_CASE(PM_SWF_C2, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_CUTOFF, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C2)
+  PM_CASE(PM_SWF_C1, PM_COUL_PLAIN, PM_SWF_C1)
+#undef PM_CASE
+}
+
+void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int precision) {
+  if (precision == 1) launch_pair_p<double>(s, box, a, p);
+  else launch_pair_p<float>(s, box, a, p);
+}
+
+void pm_launch_diag(hipStream_t s, const PmBox& box, int n, const double* pos, const int* cell_of,
+                    const int* cell_start, const int* cell_atoms, const int4* half_stencil, int nhalf,
+                    const int* molecule, const int* ex_row, const int* ex_partner, const int* ex_cls,
+                    double pair_cut2, int ncells, const int* cell_count, long long* cand, long long* inrc) {
+  (void)n; (void)cell_of;
+  hipLaunchKernelGGL(k_diag, dim3((ncells + 63) / 64), dim3(64), 0, s, box, pos, cell_start, cell_count,
+                     cell_atoms, half_stencil, nhalf, molecule, ex_row, ex_partner, ex_cls, pair_cut2, ncells,
+                     cand, inrc);
+}

@@ -1,0 +1,511 @@
+// pm_pme.hip -- smooth particle-mesh Ewald reciprocal term on gfx950.
+//
+// Follows NonbondedPMEwaldSystemForce::reciprocalTerm
+// (forces/NonbondedPMEwaldSystemForce.h:879-932) with Grid<BSpline>
+// (base/Grid.h):
+//   spread  = Grid::anterpolateCharge (:138-190), deterministic: every
+//             stencil contribution is added as a 64-bit fixed-point integer;
+//   forward = fftBack, exp(-i) (base/FFTComplex.cpp:79-82), as R2C along z
+//             and C2C along y and x (Hermitian symmetry of a real grid);
+//   scalarSum (:276-389), fused with the x transforms (forward, multiply by
+//             theta(m), inverse) while the line sits in LDS;
+//   inverse = fftForward, exp(+i), C2C along y, C2R along z;
+//   gather  = Grid::interpolateForce (:193-228), one thread per atom.
+// The 1D transforms are Stockham autosort mixed-radix (2, 3, 4, 5) passes
+// over lines staged in LDS.
+#include "pm_internal.h"
+
+#include <math.h>
+#include <type_traits>
+
+namespace {
+
+template <typename T> struct Cx;
+template <> struct Cx<float> { typedef float2 t; };
+template <> struct Cx<double> { typedef double2 t; };
+
+template <typename T2>
+__device__ inline T2 cmul(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename T2> __device__ inline T2 cadd(T2 a, T2 b) { T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T2> __device__ inline T2 csub(T2 a, T2 b) { T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// multiply by sign * i
+template <typename T2> __device__ inline T2 mul_si(T2 a, int sign) {
+  T2 r;
+  if (sign < 0) { r.x = a.y; r.y = -a.x; } else { r.x = -a.y; r.y = a.x; }
+  return r;
+}
+
+// R-point DFTs, out_k = sum_r v_r exp(sign 2 pi i r k / R)
+template <typename T, typename T2>
+__device__ inline void dft(T2* v, int R, int sign) {
+  if (R == 2) {
+    T2 a = v[0], b = v[1];
+    v[0] = cadd(a, b); v[1] = csub(a, b);
+  } else if (R == 4) {
+    T2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    T2 c = cadd(v[1], v[3]), d = mul_si(csub(v[1], v[3]), sign);
+    v[0] = cadd(a, c); v[1] = cadd(b, d); v[2] = csub(a, c); v[3] = csub(b, d);
+  } else if (R == 3) {
+    const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+    T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    T2 t1; t1.x = v[0].x - (T)0.5 * s.x; t1.y = v[0].y - (T)0.5 * s.y;
+    T2 t2 = mul_si(d, sign); t2.x *= h; t2.y *= h;
+    v[0] = cadd(v[0], s); v[1] = cadd(t1, t2); v[2] = csub(t1, t2);
+  } else {  // R == 5
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    T2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    T2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    T2 x0 = v[0];
+    T2 p1, p2, q1, q2;
+    p1.x = x0.x + c1 * a1.x + c2 * a2.x; p1.y = x0.y + c1 * a1.y + c2 * a2.y;
+    p2.x = x0.x + c2 * a1.x + c1 * a2.x; p2.y = x0.y + c2 * a1.y + c1 * a2.y;
+    T2 m1; m1.x = s1 * b1.x + s2 * b2.x; m1.y = s1 * b1.y + s2 * b2.y;
+    T2 m2; m2.x = s2 * b1.x - s1 * b2.x; m2.y = s2 * b1.y - s1 * b2.y;
+    q1 = mul_si(m1, sign); q2 = mul_si(m2, sign);
+    v[0].x = x0.x + a1.x + a2.x; v[0].y = x0.y + a1.y + a2.y;
+    v[1] = cadd(p1, q1); v[4] = csub(p1, q1);
+    v[2] = cadd(p2, q2); v[3] = csub(p2, q2);
+  }
+}
+
+// Stockham passes over `nl` lines of length p.n held in LDS (line l at
+// buf[l * n]); all threads of the block take part.  Returns the buffer that
+// holds the result.
+template <typename T, typename T2>
+__device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const double* tw, int sign) {
+  const int n = p.n;
+  T2* in = b0;
+  T2* out = b1;
+  int Ns = 1;
+  for (int st = 0; st < p.nst; st++) {
+    const int R = p.rad[st];
+    const int nR = n / R;
+    const int tws = n / (Ns * R);
+    for (int t = threadIdx.x; t < nl * nR; t += blockDim.x) {
+      const int l = t / nR;
+      const int j = t - l * nR;
+      const int k = j % Ns;
+      T2 v[5];
+      const T2* src = in + l * n + j;
+      for (int r = 0; r < R; r++) v[r] = src[r * nR];
+      if (Ns > 1) {
+        for (int r = 1; r < R; r++) {
+          int m = (r * k * tws) % n;
+          T2 w;
+          w.x = (T)tw[2 * (p.tw + m)];
+          w.y = (T)tw[2 * (p.tw + m) + 1];
+          if (sign > 0) w.y = -w.y;
+          v[r] = cmul(v[r], w);
+        }
+      }
+      dft<T, T2>(v, R, sign);
+      const int idxD = (j / Ns) * Ns * R + k;
+      T2* dst = out + l * n + idxD;
+      for (int r = 0; r < R; r++) dst[r * Ns] = v[r];
+    }
+    __syncthreads();
+    T2* tmp = in; in = out; out = tmp;
+    Ns *= R;
+  }
+  return in;
+}
+
+// ---------------------------------------------------------- B-spline (P)
+// BSpline::set (base/BSpline.cpp:64-117) for an even order P >= 4.
+template <typename T, int P>
+__device__ inline void bspline(T w, T* th, T* dth) {
+  th[P - 1] = T(0);
+  th[1] = w;
+  th[0] = T(1) - w;
+#pragma unroll
+  for (int k = 2; k < P - 1; k++) {
+    const T div = T(1) / T(k);
+    th[k] = div * w * th[k - 1];
+#pragma unroll
+    for (int j = 1; j < k; j++)
+      th[k - j] = div * ((w + T(j)) * th[k - j - 1] + (T(k + 1 - j) - w) * th[k - j]);
+    th[0] = div * (T(1) - w) * th[0];
+  }
+  dth[0] = -th[0];
+#pragma unroll
+  for (int j = 1; j <= P - 1; j++) dth[j] = th[j - 1] - th[j];
+  const T div = T(1) / T(P - 1);
+  th[P - 1] = div * w * th[P - 2];
+#pragma unroll
+  for (int j = 1; j < P - 1; j++)
+    th[P - 1 - j] = div * ((w + T(j)) * th[P - j - 2] + (T(P - j) - w) * th[P - 1 - j]);
+  th[0] = div * (T(1) - w) * th[0];
+}
+
+// grid coordinate of a fixed-point position: integer cell and fraction
+template <typename T>
+__device__ inline void grid_coord(unsigned int X, int n, int& ix, T& w) {
+  unsigned long long u = (unsigned long long)X * (unsigned long long)n;
+  ix = (int)(u >> 32);
+  w = (T)(unsigned int)(u & 0xffffffffull) * (T)2.3283064365386963e-10;  // 2^-32
+}
+template <typename T>
+__device__ inline void grid_coord64(unsigned long long X, int n, int& ix, T& w) {
+  unsigned long long hi = __umul64hi(X, (unsigned long long)n);
+  unsigned long long lo = X * (unsigned long long)n;
+  ix = (int)hi;
+  w = (T)lo * (T)5.421010862427522e-20;  // 2^-64
+}
+
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_spread(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.counts[0]) return;
+  const int at = a.atom_of_slot[s];
+  if (at < 0) return;
+  int ix, iy, iz;
+  T wx, wy, wz;
+  if constexpr (sizeof(T) == 4) {
+    const int4 p = a.slot_pos[s];
+    grid_coord<T>((unsigned)p.x, a.nx, ix, wx);
+    grid_coord<T>((unsigned)p.y, a.ny, iy, wy);
+    grid_coord<T>((unsigned)p.z, a.nz, iz, wz);
+  } else {
+    const longlong4 p = pos64[s];
+    grid_coord64<T>((unsigned long long)p.x, a.nx, ix, wx);
+    grid_coord64<T>((unsigned long long)p.y, a.ny, iy, wy);
+    grid_coord64<T>((unsigned long long)p.z, a.nz, iz, wz);
+  }
+  T tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+  bspline<T, P>(wx, tx, dx);
+  bspline<T, P>(wy, ty, dy);
+  bspline<T, P>(wz, tz, dz);
+  const int off = -(P - 1) / 2;
+  const T q = (T)a.q[at];
+  const T scale = (T)PM_GRID_FIX;
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int gx = (ix + off + i + a.nx) % a.nx;
+    const T qa = q * tx[i];
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      const int gy = (iy + off + j + a.ny) % a.ny;
+      const T qab = qa * ty[j];
+      unsigned long long* row = a.grid_fix + ((long long)gx * a.ny + gy) * a.nz;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int gz = (iz + off + k + a.nz) % a.nz;
+        long long v;
+        if constexpr (sizeof(T) == 4) v = __float2ll_rn(qab * tz[k] * scale);
+        else v = __double2ll_rn(qab * tz[k] * scale);
+        atomicAdd(row + gz, (unsigned long long)v);
+      }
+    }
+  }
+}
+
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.counts[0]) return;
+  const int at = a.atom_of_slot[s];
+  if (at < 0) return;
+  int ix, iy, iz;
+  T wx, wy, wz;
+  if constexpr (sizeof(T) == 4) {
+    const int4 p = a.slot_pos[s];
+    grid_coord<T>((unsigned)p.x, a.nx, ix, wx);
+    grid_coord<T>((unsigned)p.y, a.ny, iy, wy);
+    grid_coord<T>((unsigned)p.z, a.nz, iz, wz);
+  } else {
+    const longlong4 p = pos64[s];
+    grid_coord64<T>((unsigned long long)p.x, a.nx, ix, wx);
+    grid_coord64<T>((unsigned long long)p.y, a.ny, iy, wy);
+    grid_coord64<T>((unsigned long long)p.z, a.nz, iz, wz);
+  }
+  T tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+  bspline<T, P>(wx, tx, dx);
+  bspline<T, P>(wy, ty, dy);
+  bspline<T, P>(wz, tz, dz);
+  const int off = -(P - 1) / 2;
+  const T* g = reinterpret_cast<const T*>(a.grid_r);
+  T fx = 0, fy = 0, fz = 0;
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int gx = (ix + off + i + a.nx) % a.nx;
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      const int gy = (iy + off + j + a.ny) % a.ny;
+      const T xij = dx[i] * ty[j];
+      const T yij = tx[i] * dy[j];
+      const T zij = tx[i] * ty[j];
+      const T* row = g + ((long long)gx * a.ny + gy) * a.nz;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int gz = (iz + off + k + a.nz) % a.nz;
+        const T term = row[gz];
+        fx -= term * xij * tz[k];
+        fy -= term * yij * tz[k];
+        fz -= term * zij * dz[k];
+      }
+    }
+  }
+  const double q = a.q[at];
+  a.force_fix[3 * at] += __double2ll_rn((double)fx * a.hr[0] * q * PM_FORCE_FIX);
+  a.force_fix[3 * at + 1] += __double2ll_rn((double)fy * a.hr[1] * q * PM_FORCE_FIX);
+  a.force_fix[3 * at + 2] += __double2ll_rn((double)fz * a.hr[2] * q * PM_FORCE_FIX);
+}
+
+// ------------------------------------------------------------- FFT passes
+// z forward: fixed-point grid -> real -> complex half lines; clears the
+// fixed-point grid for the next spread.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, int TL) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * p.n;
+  const int n = p.n, nl_tot = a.nx * a.ny;
+  const int line0 = blockIdx.x * TL;
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int l = e / n, k = e - l * n;
+    const int line = line0 + l;
+    T2 v; v.x = 0; v.y = 0;
+    if (line < nl_tot) {
+      unsigned long long* gp = a.grid_fix + (long long)line * n + k;
+      v.x = (T)((double)(long long)(*gp) * PM_GRID_UNFIX);
+      *gp = 0ull;
+    }
+    b0[e] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, -1);
+  T2* gc = reinterpret_cast<T2*>(a.grid_c);
+  for (int e = threadIdx.x; e < TL * a.nzc; e += blockDim.x) {
+    const int l = e / a.nzc, k = e - l * a.nzc;
+    const int line = line0 + l;
+    if (line < nl_tot) gc[(long long)line * a.nzc + k] = r[l * n + k];
+  }
+}
+
+// y pass (either direction): lines (x, kz) with TL consecutive kz per block
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int TL, int sign) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * p.n;
+  const int n = p.n;  // ny
+  const int ntile = (a.nzc + TL - 1) / TL;
+  const int x = blockIdx.x / ntile;
+  const int kz0 = (blockIdx.x - x * ntile) * TL;
+  T2* gc = reinterpret_cast<T2*>(a.grid_c);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int y = e / TL, l = e - y * TL;
+    T2 v; v.x = 0; v.y = 0;
+    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
+    b0[l * n + y] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, sign);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int y = e / TL, l = e - y * TL;
+    if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r[l * n + y];
+  }
+}
+
+// x pass with the influence function: forward transform, multiply by
+// theta(m) (Grid::scalarSum, base/Grid.h:276-389), inverse transform.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, int TL) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * p.n;
+  __shared__ double s_red[8][8];
+  const int n = p.n;  // nx
+  const int ntile = (a.nzc + TL - 1) / TL;
+  const int y = blockIdx.x / ntile;
+  const int kz0 = (blockIdx.x - y * ntile) * TL;
+  T2* gc = reinterpret_cast<T2*>(a.grid_c);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int x = e / TL, l = e - x * TL;
+    T2 v; v.x = 0; v.y = 0;
+    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
+    b0[l * n + x] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, -1);
+  // convolution
+  double en = 0, vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+  const int j0 = y <= a.ny / 2 ? y : y - a.ny;
+  const double mj = j0 * a.Lr[1];
+  const double* bmx = a.bmod;
+  const double* bmy = a.bmod + a.nx;
+  const double* bmz = a.bmod + a.nx + a.ny;
+  const double* ex = a.expf_tab;
+  const double* ey = a.expf_tab + a.nx;
+  const double* ez = a.expf_tab + a.nx + a.ny;
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int l = e / n, kx = e - l * n;
+    const int kz = kz0 + l;
+    if (kz >= a.nzc) continue;
+    T2 v = r[e];
+    T theta = 0;
+    if (kx != 0 || y != 0 || kz != 0) {
+      const int i0 = kx <= a.nx / 2 ? kx : kx - a.nx;
+      const double mi = i0 * a.Lr[0];
+      const double mk = kz * a.Lr[2];
+      const double m2 = mi * mi + mj * mj + mk * mk;
+      const double th = bmx[kx] * bmy[y] * bmz[kz] * (ex[kx] * a.inv_pi_v) * ey[y] * ez[kz] / m2;
+      theta = (T)th;
+      const double w = (kz == 0 || ((a.nz & 1) == 0 && kz == a.nz / 2)) ? 1.0 : 2.0;
+      const double q2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+      const double eg = w * q2 * th;
+      en += eg;
+      if (a.want_virial) {
+        const double vv = 2.0 * (1.0 / m2 + a.fac);
+        vxx += eg * (1.0 - vv * mi * mi);
+        vxy -= eg * (vv * mi * mj);
+        vxz -= eg * (vv * mi * mk);
+        vyy += eg * (1.0 - vv * mj * mj);
+        vyz -= eg * (vv * mj * mk);
+        vzz += eg * (1.0 - vv * mk * mk);
+      }
+    }
+    v.x *= theta;
+    v.y *= theta;
+    r[e] = v;
+  }
+  // block reduction of the energy/virial partials (fixed order)
+  double part[7] = {en, vxx, vxy, vxz, vyy, vyz, vzz};
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int u = 0; u < 7; u++) {
+    double t = part[u];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (lane == 0) s_red[wv][u] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[7];
+    for (int u = 0; u < 7; u++) {
+      double t = 0;
+      for (int w2 = 0; w2 < (int)(blockDim.x >> 6); w2++) t += s_red[w2][u];
+      tot[u] = t;
+    }
+    double* ep = a.epart + (long long)blockIdx.x * PM_NE;
+    ep[PM_E_COUL] = tot[0] * 0.5;
+    for (int u = 1; u < PM_E_VIR; u++) ep[u] = 0.0;
+    ep[PM_E_VIR + 0] = tot[1] * 0.5; ep[PM_E_VIR + 1] = tot[2] * 0.5; ep[PM_E_VIR + 2] = tot[3] * 0.5;
+    ep[PM_E_VIR + 3] = tot[2] * 0.5; ep[PM_E_VIR + 4] = tot[4] * 0.5; ep[PM_E_VIR + 5] = tot[5] * 0.5;
+    ep[PM_E_VIR + 6] = tot[3] * 0.5; ep[PM_E_VIR + 7] = tot[5] * 0.5; ep[PM_E_VIR + 8] = tot[6] * 0.5;
+  }
+  // (the conv loop wrote r[] before the barrier above)
+  T2* other = (r == b0) ? b1 : b0;
+  T2* r2 = fft_lines<T, T2>(r, other, TL, p, a.tw, +1);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int x = e / TL, l = e - x * TL;
+    if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r2[l * n + x];
+  }
+}
+
+// z inverse: complex half lines -> full Hermitian lines -> real grid
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_z_inv(PmPmeLaunch a, PmFftPlan p, int TL) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * p.n;
+  const int n = p.n, nl_tot = a.nx * a.ny, nzc = a.nzc;
+  const int line0 = blockIdx.x * TL;
+  const T2* gc = reinterpret_cast<const T2*>(a.grid_c);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int l = e / n, k = e - l * n;
+    const int line = line0 + l;
+    T2 v; v.x = 0; v.y = 0;
+    if (line < nl_tot) {
+      if (k < nzc) v = gc[(long long)line * nzc + k];
+      else { v = gc[(long long)line * nzc + (n - k)]; v.y = -v.y; }
+    }
+    b0[e] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, +1);
+  T* g = reinterpret_cast<T*>(a.grid_r);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int l = e / n, k = e - l * n;
+    const int line = line0 + l;
+    if (line < nl_tot) g[(long long)line * n + k] = r[e].x;
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- launcher
+static int lines_per_block(int n, size_t elem) {
+  int tl = (int)(32768 / ((size_t)n * elem * 2));
+  if (tl > 16) tl = 16;
+  if (tl < 1) tl = 1;
+  return tl;
+}
+
+template <typename T, int P>
+static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
+                         const longlong4* pos64) {
+  typedef typename Cx<T>::t T2;
+  const int nthreads = 256;
+  hipLaunchKernelGGL((k_spread<T, P>), dim3((a.nslot_max + 255) / 256), dim3(nthreads), 0, s, a, pos64);
+  {
+    const int tl = lines_per_block(a.nz, sizeof(T2));
+    const int nb = (a.nx * a.ny + tl - 1) / tl;
+    hipLaunchKernelGGL(k_fft_z_fwd<T>, dim3(nb), dim3(nthreads), 2 * tl * a.nz * sizeof(T2), s, a, plans[2], tl);
+  }
+  {
+    const int tl = lines_per_block(a.ny, sizeof(T2));
+    const int nb = a.nx * ((a.nzc + tl - 1) / tl);
+    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), 2 * tl * a.ny * sizeof(T2), s, a, plans[1], tl, -1);
+  }
+  {
+    const int tl = lines_per_block(a.nx, sizeof(T2));
+    const int nb = a.ny * ((a.nzc + tl - 1) / tl);
+    hipLaunchKernelGGL(k_fft_x_conv<T>, dim3(nb), dim3(nthreads), 2 * tl * a.nx * sizeof(T2), s, a, plans[0], tl);
+  }
+  {
+    const int tl = lines_per_block(a.ny, sizeof(T2));
+    const int nb = a.nx * ((a.nzc + tl - 1) / tl);
+    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), 2 * tl * a.ny * sizeof(T2), s, a, plans[1], tl, +1);
+  }
+  {
+    const int tl = lines_per_block(a.nz, sizeof(T2));
+    const int nb = (a.nx * a.ny + tl - 1) / tl;
+    hipLaunchKernelGGL(k_fft_z_inv<T>, dim3(nb), dim3(nthreads), 2 * tl * a.nz * sizeof(T2), s, a, plans[2], tl);
+  }
+  hipLaunchKernelGGL((k_gather<T, P>), dim3((a.nslot_max + 255) / 256), dim3(nthreads), 0, s, a, pos64);
+}
+
+int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision) {
+  (void)nz;
+  const size_t e = precision == 1 ? sizeof(double2) : sizeof(float2);
+  const int nzc = nz / 2 + 1;
+  const int tl = lines_per_block(nx, e);
+  (void)ny;
+  return ny * ((nzc + tl - 1) / tl);
+}
+
+template <typename T>
+static bool launch_pme_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64) {
+  switch (a.order) {
+    case 4: launch_pme_t<T, 4>(s, a, plans, pos64); return true;
+    case 6: launch_pme_t<T, 6>(s, a, plans, pos64); return true;
+    case 8: launch_pme_t<T, 8>(s, a, plans, pos64); return true;
+    default: return false;
+  }
+}
+
+bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
+                         int precision) {
+  if (precision == 1) return launch_pme_p<double>(s, a, plans, pos64);
+  return launch_pme_p<float>(s, a, plans, pos64);
+}

@@ -1,0 +1,226 @@
+"""ctypes binding of the C ABI in include/pmgpu.h (libpmgpu.so).
+
+The library is required: importing this module on a machine without the
+built library, or creating a Context without a HIP device, raises.  There is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpmgpu.so")
+
+PM_OK, PM_ERR_ARGUMENT, PM_ERR_STATE, PM_ERR_DEVICE, PM_ERR_CAPACITY, PM_ERR_UNSUPPORTED = range(6)
+PRECISION = {"mixed": 0, "double": 1}
+PM_EVAL_VIRIAL = 1
+INTEGRATOR = {"leapfrog": 0, "langevin": 1}
+ENERGY_COUNT = 16
+COULOMB, LENNARDJONES, BOND, ANGLE = 0, 1, 2, 3
+VIRIAL = 7
+
+# exported symbols, kept in sync with include/pmgpu.h (tested)
+EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set_precision",
+           "pm_set_topology", "pm_set_bonded", "pm_set_boundary", "pm_force_create", "pm_force_uncache",
+           "pm_evaluate", "pm_cell_diagnostics", "pm_md_setup", "pm_md_set_state", "pm_md_run",
+           "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_synchronize"]
+TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
+
+
+class PmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[pm error {code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run `python -m protomol_amd.build`")
+    lib = C.CDLL(path)
+    D = C.POINTER(C.c_double)
+    I = C.POINTER(C.c_int)
+    L = C.POINTER(C.c_longlong)
+    P = C.c_void_p
+    lib.pm_abi_version.restype = C.c_int
+    lib.pm_last_error.restype = C.c_char_p
+    lib.pm_last_error.argtypes = [P]
+    lib.pm_create.argtypes = [C.c_int, C.POINTER(P)]
+    lib.pm_destroy.argtypes = [P]
+    lib.pm_destroy.restype = None
+    lib.pm_set_precision.argtypes = [P, C.c_int]
+    lib.pm_set_topology.argtypes = [P, C.c_int, D, I, C.c_int, D, D, D, D, I, D, C.c_int, I, I, I, C.c_double]
+    lib.pm_set_bonded.argtypes = [P, C.c_int, I, I, D, D, C.c_int, I, I, I, D, D, D, D]
+    lib.pm_set_boundary.argtypes = [P, D, D, D, D, C.c_double]
+    lib.pm_force_create.argtypes = [P, C.c_char_p, D, C.c_int, I]
+    lib.pm_force_uncache.argtypes = [P, C.c_int]
+    lib.pm_evaluate.argtypes = [P, I, C.c_int, D, D, D, C.c_int]
+    lib.pm_cell_diagnostics.argtypes = [P, D, C.c_double, C.c_double, I, I, I, I, L, L]
+    lib.pm_md_setup.argtypes = [P, I, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_ulonglong]
+    lib.pm_md_set_state.argtypes = [P, D, D]
+    lib.pm_md_run.argtypes = [P, C.c_int]
+    lib.pm_md_get_state.argtypes = [P, D, D, D]
+    lib.pm_last_stats.argtypes = [P, L, C.c_int]
+    lib.pm_set_timing.argtypes = [P, C.c_uint]
+    lib.pm_get_timing.argtypes = [P, D, L, C.c_int]
+    lib.pm_synchronize.argtypes = [P]
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _lp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_longlong))
+
+
+class Context:
+    """One HIP device context: topology, boundary, forces, MD state."""
+
+    def __init__(self, device: int = 0, precision: str = "mixed"):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.pm_create(device, C.byref(h))
+        if rc != PM_OK:
+            raise PmError(rc, f"pm_create(device={device}) failed (no HIP device?)")
+        self.h = h
+        self.n = 0
+        self._keep = []
+        self.set_precision(precision)
+
+    def close(self):
+        if self.h:
+            self.lib.pm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != PM_OK:
+            raise PmError(rc, self.lib.pm_last_error(self.h).decode())
+
+    def set_precision(self, precision: str):
+        self._check(self.lib.pm_set_precision(self.h, PRECISION[precision]))
+        self.precision = precision
+
+    def set_topology(self, s):
+        q = np.ascontiguousarray(s.q, np.float64)
+        t = np.ascontiguousarray(s.type, np.int32)
+        A, B = np.ascontiguousarray(s.ljA, np.float64), np.ascontiguousarray(s.ljB, np.float64)
+        A14, B14 = np.ascontiguousarray(s.ljA14, np.float64), np.ascontiguousarray(s.ljB14, np.float64)
+        mol = np.ascontiguousarray(s.molecule, np.int32)
+        mass = np.ascontiguousarray(s.mass, np.float64)
+        ea, eb, ec = (np.ascontiguousarray(x, np.int32) for x in (s.ex_a, s.ex_b, s.ex_cls))
+        self._check(self.lib.pm_set_topology(self.h, len(q), _dp(q), _ip(t), A.shape[0], _dp(A), _dp(B),
+                                             _dp(A14), _dp(B14), _ip(mol), _dp(mass), len(ea), _ip(ea),
+                                             _ip(eb), _ip(ec), float(s.csf)))
+        self.n = len(q)
+        if getattr(s, "bond_a", None) is not None and (len(s.bond_a) or len(s.angle_a)):
+            self.set_bonded(s)
+
+    def set_bonded(self, s):
+        ba, bb = (np.ascontiguousarray(x, np.int32) for x in (s.bond_a, s.bond_b))
+        bk, br = (np.ascontiguousarray(x, np.float64) for x in (s.bond_k, s.bond_r0))
+        aa, ab, ac = (np.ascontiguousarray(x, np.int32) for x in (s.angle_a, s.angle_b, s.angle_c))
+        ak, at, au, ar = (np.ascontiguousarray(x, np.float64) for x in (s.angle_k, s.angle_t0, s.angle_kub, s.angle_rub))
+        self._keep = [ba, bb, bk, br, aa, ab, ac, ak, at, au, ar]
+        self._check(self.lib.pm_set_bonded(self.h, len(ba), _ip(ba), _ip(bb), _dp(bk), _dp(br), len(aa),
+                                           _ip(aa), _ip(ab), _ip(ac), _dp(ak), _dp(at), _dp(au), _dp(ar)))
+
+    def set_boundary(self, box, origin, cellsize):
+        e1 = np.array([box[0], 0, 0], np.float64)
+        e2 = np.array([0, box[1], 0], np.float64)
+        e3 = np.array([0, 0, box[2]], np.float64)
+        o = np.ascontiguousarray(origin, np.float64)
+        self._check(self.lib.pm_set_boundary(self.h, _dp(e1), _dp(e2), _dp(e3), _dp(o), float(cellsize)))
+
+    def force(self, force_id: str, values) -> int:
+        v = np.ascontiguousarray(values, np.float64)
+        h = C.c_int()
+        self._check(self.lib.pm_force_create(self.h, force_id.encode(), _dp(v), len(v), C.byref(h)))
+        return h.value
+
+    def uncache(self, handle: int):
+        self._check(self.lib.pm_force_uncache(self.h, handle))
+
+    def evaluate(self, handles, positions, virial=False, forces=None, energies=None):
+        hs = np.ascontiguousarray(handles, np.int32)
+        x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        f = np.zeros(3 * self.n) if forces is None else forces.reshape(-1)
+        e = np.zeros(ENERGY_COUNT) if energies is None else energies
+        self._check(self.lib.pm_evaluate(self.h, _ip(hs), len(hs), _dp(x), _dp(f), _dp(e),
+                                         PM_EVAL_VIRIAL if virial else 0))
+        return f.reshape(-1, 3), e
+
+    def cell_diagnostics(self, positions, algorithm_cutoff, pair_cutoff):
+        dims = np.zeros(3, np.int32)
+        self._check(self.lib.pm_cell_diagnostics(self.h, None, 0.0, 0.0, _ip(dims), None, None, None, None, None))
+        nc = int(np.prod(dims))
+        x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        cell = np.zeros(self.n, np.int32)
+        head = np.zeros(nc, np.int32)
+        nxt = np.zeros(self.n, np.int32)
+        cand = np.zeros(nc, np.int64)
+        inrc = np.zeros(nc, np.int64)
+        self._check(self.lib.pm_cell_diagnostics(self.h, _dp(x), float(algorithm_cutoff), float(pair_cutoff),
+                                                 _ip(dims), _ip(cell), _ip(head), _ip(nxt), _lp(cand), _lp(inrc)))
+        return dict(dims=tuple(int(d) for d in dims), cell=cell, head=head, next=nxt, cand=cand, inrc=inrc)
+
+    def md_setup(self, handles, integrator="leapfrog", dt=1.0, temperature=300.0, gamma=0.0, seed=0):
+        hs = np.ascontiguousarray(handles, np.int32)
+        self._check(self.lib.pm_md_setup(self.h, _ip(hs), len(hs), INTEGRATOR[integrator], float(dt),
+                                         float(temperature), float(gamma), int(seed)))
+
+    def md_set_state(self, x, v):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        self._check(self.lib.pm_md_set_state(self.h, _dp(x), _dp(v)))
+
+    def md_run(self, nsteps: int):
+        self._check(self.lib.pm_md_run(self.h, int(nsteps)))
+
+    def md_get_state(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        e = np.zeros(ENERGY_COUNT)
+        self._check(self.lib.pm_md_get_state(self.h, _dp(x), _dp(v), _dp(e)))
+        return x, v, e
+
+    def stats(self):
+        s = np.zeros(4, np.int64)
+        self._check(self.lib.pm_last_stats(self.h, _lp(s), 4))
+        return dict(list_entries=int(s[0]), lanes=int(s[1]), clusters=int(s[2]), slots=int(s[3]))
+
+    def set_timing(self, categories=()):
+        mask = 0
+        for c in categories:
+            mask |= 1 << TIMING.index(c)
+        self._check(self.lib.pm_set_timing(self.h, mask))
+
+    def timing(self):
+        ms = np.zeros(len(TIMING))
+        cnt = np.zeros(len(TIMING), np.int64)
+        self._check(self.lib.pm_get_timing(self.h, _dp(ms), _lp(cnt), len(TIMING)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(TIMING)}
+
+    def synchronize(self):
+        self._check(self.lib.pm_synchronize(self.h))

@@ -32,6 +32,8 @@ M_O, M_H = 15.9994, 1.008
 R_OH = 0.9572
 THETA_HOH = 104.52
 SIGMA_O, EPS_O = 3.1507, 0.1521
+# CHARMM TIP3 flexible terms (OT-HT bond, HT-OT-HT angle, no Urey-Bradley)
+K_BOND, K_ANGLE = 450.0, 55.0
 
 
 @dataclasses.dataclass
@@ -54,6 +56,18 @@ class WaterSystem:
     box: np.ndarray          # (3,) box edges (orthogonal)
     origin: np.ndarray       # (3,) box centre
     csf: float = 1.0         # coulombScalingFactor
+    # bonded terms (BondSystemForce / AngleSystemForce inputs)
+    bond_a: np.ndarray = None
+    bond_b: np.ndarray = None
+    bond_k: np.ndarray = None
+    bond_r0: np.ndarray = None
+    angle_a: np.ndarray = None
+    angle_b: np.ndarray = None
+    angle_c: np.ndarray = None
+    angle_k: np.ndarray = None
+    angle_t0: np.ndarray = None
+    angle_kub: np.ndarray = None
+    angle_rub: np.ndarray = None
 
     @property
     def n(self) -> int:
@@ -107,11 +121,12 @@ def _random_rotations(rng, m):
 
 
 def water_box(n_molecules: int, box: float, seed: int = 0, temperature: float = 300.0,
-              lattice: int | None = None) -> WaterSystem:
+              lattice: int | None = None, jitter: float = 0.0) -> WaterSystem:
     """Build a periodic cube of `n_molecules` waters with edge `box` (Angstrom).
 
     Oxygens sit on an `lattice`^3 cubic lattice (default: the smallest cube
-    holding n_molecules sites; a seeded subset is used when it holds more).
+    holding n_molecules sites; a seeded subset is used when it holds more),
+    displaced by a seeded uniform offset in [-jitter, jitter] per axis.
     """
     rng = np.random.default_rng(seed)
     side = lattice or int(math.ceil(n_molecules ** (1.0 / 3.0) - 1e-9))
@@ -124,6 +139,8 @@ def water_box(n_molecules: int, box: float, seed: int = 0, temperature: float = 
     a = box / side
     ix, iy, iz = sites // (side * side), (sites // side) % side, sites % side
     centres = np.stack([ix, iy, iz], axis=1) * a + 0.5 * a - 0.5 * box
+    if jitter > 0.0:
+        centres = centres + rng.uniform(-jitter, jitter, size=centres.shape)
     half = math.radians(THETA_HOH) / 2.0
     local = np.array([[0.0, 0.0, 0.0],
                       [R_OH * math.sin(half), 0.0, R_OH * math.cos(half)],
@@ -142,10 +159,18 @@ def water_box(n_molecules: int, box: float, seed: int = 0, temperature: float = 
     ex_c = np.full(ex_a.shape, EXCL_FULL, dtype=np.int32)
     A, B = _lj_tables([(SIGMA_O, EPS_O), (0.0, 0.0)])
     vel = rng.standard_normal((n, 3)) * np.sqrt(BOLTZMANN * temperature / mass)[:, None]
+    nb = 2 * n_molecules
+    bond_a = np.stack([base, base], axis=1).reshape(-1).astype(np.int32)
+    bond_b = np.stack([base + 1, base + 2], axis=1).reshape(-1).astype(np.int32)
     return WaterSystem(pos=np.ascontiguousarray(pos), vel=vel, q=q, type=typ, mass=mass,
                        molecule=mol, ex_a=ex_a, ex_b=ex_b, ex_cls=ex_c, ljA=A, ljB=B,
                        ljA14=A.copy(), ljB14=B.copy(), box=np.array([box, box, box], dtype=float),
-                       origin=np.zeros(3))
+                       origin=np.zeros(3),
+                       bond_a=bond_a, bond_b=bond_b, bond_k=np.full(nb, K_BOND), bond_r0=np.full(nb, R_OH),
+                       angle_a=(base + 1).astype(np.int32), angle_b=base.astype(np.int32),
+                       angle_c=(base + 2).astype(np.int32), angle_k=np.full(n_molecules, K_ANGLE),
+                       angle_t0=np.full(n_molecules, math.radians(THETA_HOH)),
+                       angle_kub=np.zeros(n_molecules), angle_rub=np.zeros(n_molecules))
 
 
 # Named workloads (molecule count, box edge in Angstrom).
@@ -157,10 +182,10 @@ CONFIGS = {
 }
 
 
-def named(name: str, seed: int = 0) -> WaterSystem:
+def named(name: str, seed: int = 0, jitter: float = 0.0) -> WaterSystem:
     nmol, box = CONFIGS[name]
     lattice = {"water1m": 70, "water8k": 14}.get(name)
-    return water_box(nmol, box, seed=seed, lattice=lattice)
+    return water_box(nmol, box, seed=seed, lattice=lattice, jitter=jitter)
 
 
 def replica_sizes(count: int = 64, seed: int = 0, lo: int = 100, hi: int = 1000):

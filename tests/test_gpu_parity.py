@@ -1,0 +1,206 @@
+"""GPU parity: the HIP kernels through the C ABI against the FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from protomol_amd import water
+from tests.cases import ALPHA_RC8, ALPHA_RC10, c1_terms, gpu_forces_c1, rms_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def setup_ctx(make, s, cellsize, precision="mixed"):
+    c = make(precision)
+    c.set_topology(s)
+    c.set_boundary(s.box, s.origin, cellsize)
+    return c
+
+
+def test_cell_binning_bitexact(gpu_ctx_factory, port):
+    say("case: binning water648 cell 4 cutoff 8")
+    s = water.named("water648", seed=1)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0)
+    g = c.cell_diagnostics(s.pos, 8.0, 8.0)
+    o = port.cells(s, 4.0)
+    assert g["dims"] == o["dims"]
+    np.testing.assert_array_equal(g["cell"], o["cell"])
+    np.testing.assert_array_equal(g["head"], o["head"])
+    np.testing.assert_array_equal(g["next"], o["next"])
+    lj, _ = c1_terms()
+    oc = port.cutoff(s, lj, 8.0, 4.0, counts=True)
+    np.testing.assert_array_equal(g["cand"], oc["cand"])
+    np.testing.assert_array_equal(g["inrc"], oc["inrc"])
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 2e-6), ("double", 1e-11)])
+def test_cutoff_c1(gpu_ctx_factory, port, precision, tol):
+    say(f"case: water648 LJ+Coulomb C2 {precision}")
+    s = water.named("water648", seed=1)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    f, e = c.evaluate(gpu_forces_c1(c), s.pos, virial=True)
+    lj, co = c1_terms()
+    a = port.cutoff(s, lj, 8.0, 4.0, virial=True)
+    b = port.cutoff(s, co, 8.0, 4.0, virial=True)
+    fr = a["forces"] + b["forces"]
+    er = a["energies"] + b["energies"]
+    say("rms rel force", rms_rel(f, fr), "E", e[:2], er[:2])
+    assert rms_rel(f, fr) < tol
+    np.testing.assert_allclose(e[:2], er[:2], rtol=tol * 5, atol=0)
+    np.testing.assert_allclose(e[7:], er[7:], rtol=tol * 50, atol=tol * 50 * np.abs(er[7:]).max())
+
+
+VARIANTS = [
+    ("LennardJones -algorithm NonbondedCutoff -switchingFunction C1", [8.0, 8.0], [po.term(po.POT_LJ, po.SWF_C1, 8.0)]),
+    ("LennardJones -algorithm NonbondedCutoff -switchingFunction Cutoff", [8.0, 8.0], [po.term(po.POT_LJ, po.SWF_CUTOFF, 8.0)]),
+    ("Coulomb -algorithm NonbondedCutoff -switchingFunction C1", [8.0, 8.0], [po.term(po.POT_COULOMB, po.SWF_C1, 8.0)]),
+    ("Coulomb -algorithm NonbondedCutoff -switchingFunction Cutoff", [8.0, 8.0], [po.term(po.POT_COULOMB, po.SWF_CUTOFF, 8.0)]),
+    ("CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction Cutoff", [ALPHA_RC8, 8.0, 8.0],
+     [po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 8.0, alpha=ALPHA_RC8)]),
+    ("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction Cutoff",
+     [7.0, 8.0, ALPHA_RC8, 8.0, 8.0],
+     [po.term(po.POT_LJ, po.SWF_C2, 8.0, 7.0), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 8.0, alpha=ALPHA_RC8)]),
+    ("LennardJones Coulomb -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction C1",
+     [7.0, 8.0, 8.0, 8.0],
+     [po.term(po.POT_LJ, po.SWF_C2, 8.0, 7.0), po.term(po.POT_COULOMB, po.SWF_C1, 8.0)]),
+]
+
+
+@pytest.mark.parametrize("fid,vals,terms", VARIANTS, ids=[v[0].split(" -")[0] + "/" + v[0].split()[-1] for v in VARIANTS])
+@pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-11)])
+def test_cutoff_variants(gpu_ctx_factory, port, fid, vals, terms, precision, tol):
+    say(f"case: variant {fid} {precision}")
+    s = water.named("water648", seed=2, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    f, e = c.evaluate([c.force(fid, vals)], s.pos)
+    o = port.cutoff(s, terms, vals[-1], 4.0)
+    say("rms rel force", rms_rel(f, o["forces"]))
+    assert rms_rel(f, o["forces"]) < tol
+    np.testing.assert_allclose(e[:2], o["energies"][:2], rtol=tol * 10, atol=1e-9)
+
+
+@pytest.mark.parametrize("grid,order", [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8)])
+@pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-10)])
+def test_pme_small(gpu_ctx_factory, port, grid, order, precision, tol):
+    say(f"case: pme water648 grid {grid} order {order} {precision}")
+    s = water.named("water648", seed=3, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    h = c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                [grid[0], grid[1], grid[2], 8.0, order, 1e-6, -1.0])
+    f, e = c.evaluate([h], s.pos, virial=True)
+    o = port.pme(s, grid, order, 8.0, 4.0, flags=6, virial=True)
+    say("rms rel force", rms_rel(f, o["forces"]), "E", e[0], o["energies"][0])
+    assert rms_rel(f, o["forces"]) < tol
+    # the total is a small difference of large terms: scale the tolerance by them
+    scale = sum(abs(o["parts"][k]) for k in ("reciprocal", "intra", "self"))
+    assert abs(e[0] - o["energies"][0]) < tol * 0.2 * scale
+    np.testing.assert_allclose(e[7:], o["energies"][7:], rtol=tol * 100, atol=tol * 100 * np.abs(o["energies"][7:]).max())
+
+
+def test_pme_full_alias(gpu_ctx_factory, port):
+    say("case: CoulombPME alias water648 grid 24")
+    s = water.named("water648", seed=4, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, "double")
+    h = c.force("CoulombPME", [24, 24, 24, 8.0, 4, 1e-6, -1.0])
+    f, e = c.evaluate([h], s.pos)
+    o = port.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=7)
+    assert rms_rel(f, o["forces"]) < 1e-10
+    np.testing.assert_allclose(e[0], o["energies"][0], rtol=1e-10)
+
+
+def test_water98k_pme_mixed_accuracy(gpu_ctx_factory, port):
+    """98,304-atom water, LJ-C2 + Ewald-real rc 10 plus PME 96^3: RMS relative
+    force error of the mixed-precision path against the FP64 oracle."""
+    say("case: water98k full force field mixed")
+    s = water.named("water98k", seed=1, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 5.0, "mixed")
+    h1 = c.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction Cutoff",
+                 [9.0, 10.0, ALPHA_RC10, 10.0, 10.0])
+    h2 = c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                 [96, 96, 96, 10.0, 4, 1e-6, -1.0])
+    f, e = c.evaluate([h1, h2], s.pos)
+    t = [po.term(po.POT_LJ, po.SWF_C2, 10.0, 9.0), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 10.0, alpha=ALPHA_RC10)]
+    a = port.cutoff(s, t, 10.0, 5.0)
+    b = port.pme(s, (96, 96, 96), 4, 10.0, 5.0, flags=6)
+    fr = a["forces"] + b["forces"]
+    er = a["energies"] + b["energies"]
+    err = rms_rel(f, fr)
+    # E_coulomb = real + reciprocal + intra + self: relative to the sum of magnitudes
+    scale = abs(a["energies"][0]) + sum(abs(b["parts"][k]) for k in ("reciprocal", "intra", "self"))
+    say("rms rel force", err, "E", e[:2], er[:2], "dE_coul/scale", (e[0] - er[0]) / scale,
+        "rel E_lj", (e[1] - er[1]) / er[1])
+    assert err < 5e-6
+    assert abs(e[0] - er[0]) < 1e-7 * scale
+    assert abs(e[1] - er[1]) < 5e-6 * abs(er[1])
+
+
+def test_determinism(gpu_ctx_factory):
+    say("case: determinism water648 mixed")
+    s = water.named("water648", seed=5)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, "mixed")
+    hs = gpu_forces_c1(c) + [c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                                     [24, 24, 24, 8.0, 4, 1e-6, -1.0])]
+    f1, e1 = c.evaluate(hs, s.pos)
+    f2, e2 = c.evaluate(hs, s.pos)
+    np.testing.assert_array_equal(f1, f2)
+    np.testing.assert_array_equal(e1, e2)
+
+
+def test_leapfrog_double(gpu_ctx_factory, port):
+    say("case: leapfrog water648 double 5 steps")
+    s = water.named("water648", seed=6, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, "double")
+    hs = gpu_forces_c1(c)
+    c.md_setup(hs, "leapfrog", dt=0.5)
+    c.md_set_state(s.pos, s.vel)
+    c.md_run(5)
+    xg, vg, eg = c.md_get_state()
+    # oracle: LeapfrogIntegrator::run(5)
+    x = s.pos.copy()
+    v = s.vel.copy()
+    lj, co = c1_terms()
+
+    def forces(xx):
+        s2 = s.copy()
+        s2.pos = xx
+        return port.cutoff(s2, lj, 8.0, 4.0)["forces"] + port.cutoff(s2, co, 8.0, 4.0)["forces"]
+
+    import ctypes as C
+    lib = port.lib
+    D = C.POINTER(C.c_double)
+
+    def kd(fac, f):
+        lib.orc_leapfrog_kickdrift(s.n, C.c_double(0.5), C.c_double(fac), s.mass.ctypes.data_as(D),
+                                   f.ctypes.data_as(D), v.ctypes.data_as(D), x.ctypes.data_as(D))
+
+    f = forces(x)
+    kd(0.5, f)
+    f = forces(x)
+    for _ in range(4):
+        kd(1.0, f)
+        f = forces(x)
+    lib.orc_half_kick(s.n, C.c_double(0.5), s.mass.ctypes.data_as(D), f.ctypes.data_as(D), v.ctypes.data_as(D))
+    say("max |dx|", np.abs(xg - x).max(), "max |dv|", np.abs(vg - v).max())
+    np.testing.assert_allclose(xg, x, atol=1e-9, rtol=0)
+    np.testing.assert_allclose(vg, v, atol=1e-9 * np.abs(v).max(), rtol=0)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_bonded(gpu_ctx_factory, port, precision):
+    say(f"case: bond+angle water648 {precision}")
+    s = water.named("water648", seed=7, jitter=0.4)
+    rng = np.random.default_rng(7)
+    s.pos = s.pos + rng.uniform(-0.1, 0.1, size=s.pos.shape)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    f, e = c.evaluate([c.force("Bond", []), c.force("Angle", [])], s.pos, virial=True)
+    o = port.bonded(s, 3, virial=True)
+    say("rms rel force", rms_rel(f, o["forces"]), "E", e[2:4], o["energies"][2:4])
+    assert rms_rel(f, o["forces"]) < 1e-11
+    np.testing.assert_allclose(e[2:4], o["energies"][2:4], rtol=1e-11)
+    np.testing.assert_allclose(e[7:], o["energies"][7:], rtol=1e-9, atol=1e-9 * np.abs(o["energies"][7:]).max())
