@@ -125,6 +125,7 @@ struct pm_context {
   // energy partial records
   double* d_epart = nullptr;
   long long epart_alloc = 0;
+  double* d_ered = nullptr;       // reduction stage-1 partials
 
   // PME buffers (one grid size at a time)
   int pme_nx = 0, pme_ny = 0, pme_nz = 0, pme_prec = -1;
@@ -728,7 +729,7 @@ std::vector<PairPass> plan_passes(pm_context* c, const std::vector<int>& handles
 // ---------------------------------------------------------- evaluation core
 // Queues one force-group evaluation at the positions in c->d_pos: forces into
 // c->d_force (fixed point, zeroed first), energies into c->d_energy.
-int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags) {
+int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bool want_energy = true) {
   hipStream_t s = c->stream;
   const int n = c->n;
   const bool vir = (flags & PM_EVAL_VIRIAL) != 0;
@@ -794,7 +795,6 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags) {
   double lmin_half = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
   double cdiag = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
                            c->realcell[2] * c->realcell[2]);
-  pm_launch_zero_ll(s, reinterpret_cast<long long*>(c->d_stats), 2);
   for (size_t pi = 0; pi < passes.size(); pi++) {
     PairPass& ps = passes[pi];
     PmPairParams pp = pair_params(ps.terms, c->csf);
@@ -833,7 +833,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags) {
     a.force_fix = c->d_force;
     a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
     a.want_virial = vir ? 1 : 0;
-    a.stats = c->d_stats;
+    a.want_energy = want_energy ? 1 : 0;
     {
       TimedScope ts(c, PM_T_LIST);
       pm_launch_list(s, c->box, a, pp);
@@ -872,6 +872,34 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags) {
       a.force_fix = c->d_force;
       a.epart = c->d_epart + off * PM_NE;
       a.want_virial = vir ? 1 : 0;
+      // spread bricks: 2x2x2 cells per brick (fewer where the tile would not fit 48 KB of LDS)
+      a.cell_slot = c->d_cell_slot;
+      a.cell_count = c->d_cell_count;
+      {
+        const int ng[3] = {f.nx, f.ny, f.nz};
+        int br = 2;
+        for (;;) {
+          size_t pts = 1;
+          for (int d = 0; d < 3; d++) {
+            a.ncell[d] = c->box.n[d];
+            a.brick[d] = std::min(br, c->box.n[d]);
+            a.nbrick[d] = (c->box.n[d] + a.brick[d] - 1) / a.brick[d];
+            a.cell_du[d] = c->realcell[d] * ng[d] / c->box.L[d];
+            double u = (c->box.origin[d] - 0.5 * c->box.L[d]) / c->box.L[d];
+            u -= std::floor(u);
+            a.cell_lo_u[d] = u * ng[d];
+            int t = (int)std::ceil(a.brick[d] * a.cell_du[d]) + f.order + 3;
+            a.tile[d] = std::min(t, ng[d]);
+            pts *= (size_t)a.tile[d];
+          }
+          if (pts * 8 <= 48 * 1024 || br == 1) break;
+          br = 1;
+        }
+        if (a.brick[0] * a.brick[1] * a.brick[2] > 64)
+          return fail(c, PM_ERR_UNSUPPORTED, "internal: spread brick of more than 64 cells");
+        if ((size_t)a.tile[0] * a.tile[1] * a.tile[2] * 8 > 64 * 1024)
+          return fail(c, PM_ERR_UNSUPPORTED, "PME grid too fine for the cell size: spread tile exceeds 64 KB of LDS (use a smaller cellsize)");
+      }
       TimedScope ts(c, PM_T_PME);
       if (!pm_launch_pme_plans(s, a, c->plans, c->d_slot_pos64, c->precision))
         return fail(c, PM_ERR_UNSUPPORTED, "PME order %d not implemented on the GPU", f.order);
@@ -915,9 +943,10 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags) {
     TimedScope ts(c, PM_T_BONDED);
     pm_launch_bonded(s, c->box, b, &rb, &ra);
   }
-  {
+  if (want_energy) {
+    if (!c->d_ered && (r = dalloc(c, &c->d_ered, (size_t)pm_reduce_scratch_records((int)rec) * PM_NE))) return r;
     TimedScope ts(c, PM_T_REDUCE);
-    pm_launch_reduce_energy(s, c->d_epart, (int)rec, const_coul, const_vir, c->d_energy, 1);
+    pm_launch_reduce_energy(s, c->d_epart, (int)rec, const_coul, const_vir, c->d_energy, 1, c->d_ered);
   }
   (void)any_pme_recip;
   HIPCHECK(c, hipGetLastError());
@@ -929,7 +958,10 @@ int check_overflow(pm_context* c) {
   int ov = 0;
   HIPCHECK(c, hipMemcpyAsync(&ov, c->d_overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   unsigned long long st[2] = {0, 0};
-  if (c->d_stats) HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  if (c->d_stats && c->d_jcount && c->d_counts) {
+    pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats);
+    HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  }
   int counts[4] = {0, 0, 0, 0};
   if (c->d_counts) HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -954,7 +986,7 @@ void free_all(pm_context* c) {
   dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
   dfree(c->d_cell_cursor); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
-  dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_overflow); dfree(c->d_stats); dfree(c->d_epart);
+  dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_overflow); dfree(c->d_stats); dfree(c->d_epart); dfree(c->d_ered);
   dfree(c->d_grid_fix);
   if (c->d_grid_r) hipFree(c->d_grid_r);
   if (c->d_grid_c) hipFree(c->d_grid_c);
@@ -1390,7 +1422,7 @@ int pm_md_run(pm_context* c, int nsteps) {
   const double h = c->md_dt * kInvTimeFactor;
   if (!c->md_forces_valid) {
     // initializeForces (StandardIntegrator.cpp:46-59)
-    if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+    if ((r = evaluate_group(c, c->md_handles, 0, false))) return r;
     c->md_forces_valid = true;
   }
   if (c->md_integrator == PM_INTEGRATOR_LEAPFROG) {
@@ -1399,13 +1431,13 @@ int pm_md_run(pm_context* c, int nsteps) {
       TimedScope ts(c, PM_T_INTEGRATE);
       pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
     }
-    if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+    if ((r = evaluate_group(c, c->md_handles, 0, nsteps == 1))) return r;
     for (int i = 1; i < nsteps; i++) {
       {
         TimedScope ts(c, PM_T_INTEGRATE);
         pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
       }
-      if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+      if ((r = evaluate_group(c, c->md_handles, 0, i == nsteps - 1))) return r;
     }
     pm_launch_kick(s, n, 2, 0.5 * c->md_dt * kInvTimeFactor, c->d_mass, c->d_force, c->d_vel, c->d_pos);
   } else {
@@ -1415,7 +1447,7 @@ int pm_md_run(pm_context* c, int nsteps) {
     for (int i = 0; i < nsteps; i++) {
       pm_launch_langevin_kick(s, n, h, c->md_temp, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->md_counter++);
       pm_launch_drift(s, 3 * n, h, c->d_vel, c->d_pos);
-      if ((r = evaluate_group(c, c->md_handles, 0))) return r;
+      if ((r = evaluate_group(c, c->md_handles, 0, i == nsteps - 1))) return r;
       pm_launch_langevin_kick(s, n, h, c->md_temp, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->md_counter++);
     }
   }

@@ -107,7 +107,7 @@ struct PmNbLaunch {
   long long* force_fix;      // [3n] fixed point, accumulated (+=) for every real atom
   double* epart;             // [ncl_max * PM_NE]
   int want_virial;
-  unsigned long long* stats; // [2] entries, evaluated lanes (may be null)
+  int want_energy;           // write the per-cluster energy records
 };
 
 // ---------------------------------------------------------------- launchers
@@ -123,6 +123,7 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
                           int* atom_of_slot, int dummy_type, int dummy_slot);
+void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
 void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p,
                     int precision);
@@ -155,6 +156,15 @@ struct PmPmeLaunch {
   long long* force_fix;
   double* epart;             // [x-pass blocks * PM_NE]
   int want_virial;
+  // spread by cell bricks staged in LDS tiles
+  const int* cell_slot;
+  const int* cell_count;
+  int ncell[3];              // cell grid
+  int brick[3];              // cells per brick
+  int nbrick[3];             // bricks per axis
+  int tile[3];               // LDS tile extent (grid points) per axis
+  double cell_lo_u[3];       // grid coordinate of cell 0's lower face: frac((origin - L/2)/L) * n
+  double cell_du[3];         // grid points per cell
 };
 
 #define PM_FFT_MAXST 10
@@ -174,7 +184,8 @@ void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* 
                           int want_virial, long long* force_fix, int ntab, const int* tab_a,
                           const int* tab_b, const int* tab_c, double* epart, int nblk);
 void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, double constant_coul,
-                             double constant_vir_diag, double* out, int overwrite);
+                             double constant_vir_diag, double* out, int overwrite, double* scratch);
+int pm_reduce_scratch_records(int nparts);
 void pm_launch_zero_ll(hipStream_t s, long long* p, long long count);
 void pm_launch_zero_i(hipStream_t s, int* p, long long count);
 void pm_launch_zero_d(hipStream_t s, double* p, long long count);

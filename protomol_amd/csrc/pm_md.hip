@@ -107,28 +107,36 @@ __global__ __launch_bounds__(256) void k_corr_energy(PmBox box, const double* __
   }
 }
 
-// deterministic sum of nparts energy records plus constant terms
-__global__ __launch_bounds__(256) void k_reduce_energy(const double* __restrict__ parts, int nparts,
-                                                       double const_coul, double const_vir_diag,
-                                                       double* __restrict__ out, int overwrite) {
+// Deterministic two-stage sum of nparts energy records: stage 1 sums fixed
+// contiguous chunks (one block each), stage 2 sums the chunk totals in order
+// and adds the constant terms.
+#define PM_RED_BLOCKS 256
+__global__ __launch_bounds__(256) void k_reduce_energy1(const double* __restrict__ parts, int nparts, int chunk,
+                                                        double* __restrict__ partial) {
   __shared__ double s[256];
-  for (int c = 0; c < PM_NE; c++) {
-    double acc = 0;
-    for (int p = threadIdx.x; p < nparts; p += 256) acc += parts[(long long)p * PM_NE + c];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      double v = s[0];
-      if (c == PM_E_COUL) v += const_coul;
-      if (c == PM_E_VIR + 0 || c == PM_E_VIR + 4 || c == PM_E_VIR + 8) v += const_vir_diag;
-      out[c] = overwrite ? v : out[c] + v;
-    }
-    __syncthreads();
+  const int c = threadIdx.x % PM_NE;           // component
+  const int r0 = threadIdx.x / PM_NE;          // 16 records in flight per component
+  const int lo = blockIdx.x * chunk, hi = min(nparts, lo + chunk);
+  double acc = 0;
+  for (int p = lo + r0; p < hi; p += 256 / PM_NE) acc += parts[(long long)p * PM_NE + c];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < PM_NE) {
+    double t = 0;
+    for (int u = 0; u < 256 / PM_NE; u++) t += s[u * PM_NE + threadIdx.x];
+    partial[blockIdx.x * PM_NE + threadIdx.x] = t;
   }
+}
+
+__global__ void k_reduce_energy2(const double* __restrict__ partial, int nblocks, double const_coul,
+                                 double const_vir_diag, double* __restrict__ out, int overwrite) {
+  const int c = threadIdx.x;
+  if (c >= PM_NE) return;
+  double v = 0;
+  for (int b = 0; b < nblocks; b++) v += partial[b * PM_NE + c];
+  if (c == PM_E_COUL) v += const_coul;
+  if (c == PM_E_VIR + 0 || c == PM_E_VIR + 4 || c == PM_E_VIR + 8) v += const_vir_diag;
+  out[c] = overwrite ? v : out[c] + v;
 }
 
 __global__ void k_zero_ll(long long* p, long long count) {
@@ -228,10 +236,19 @@ void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* 
                        csf, want_virial, epart);
 }
 
+int pm_reduce_scratch_records(int nparts) {
+  (void)nparts;
+  return PM_RED_BLOCKS;
+}
+
 void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, double constant_coul,
-                             double constant_vir_diag, double* out, int overwrite) {
-  hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, s, parts, nparts, constant_coul, constant_vir_diag,
-                     out, overwrite);
+                             double constant_vir_diag, double* out, int overwrite, double* scratch) {
+  int nb = nparts < PM_RED_BLOCKS ? (nparts > 0 ? nparts : 1) : PM_RED_BLOCKS;
+  int chunk = (nparts + nb - 1) / nb;
+  if (chunk < 1) chunk = 1;
+  hipLaunchKernelGGL(k_reduce_energy1, dim3(nb), dim3(256), 0, s, parts, nparts, chunk, scratch);
+  hipLaunchKernelGGL(k_reduce_energy2, dim3(1), dim3(64), 0, s, scratch, nb, constant_coul, constant_vir_diag, out,
+                     overwrite);
 }
 
 void pm_launch_zero_ll(hipStream_t s, long long* p, long long count) {

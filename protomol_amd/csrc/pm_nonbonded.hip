@@ -120,17 +120,20 @@ __device__ inline unsigned long long to_fix64(double x, double s) {
   return (h << 32) + l;
 }
 
-// One thread per cell: order the cell's atoms by (z, index) and write the
-// padded slot arrays.
-__global__ void k_fill_slots(PmBox box, int ncells, const double* __restrict__ pos,
+// One wave per cell: every atom's slot is its rank in (z, index) order within
+// the cell (ties impossible: indices are distinct), so the layout does not
+// depend on the arrival order of k_scatter.  Cells holding more than
+// PM_SORT_MAX atoms (collapsed or non-finite coordinates) keep arrival order.
+__global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const double* __restrict__ pos,
                              const double* __restrict__ q, const int* __restrict__ type,
                              const int* __restrict__ cell_count, const int* __restrict__ cell_start,
-                             const int* __restrict__ cell_slot, int* __restrict__ cell_atoms,
+                             const int* __restrict__ cell_slot, const int* __restrict__ cell_atoms,
                              const float* __restrict__ key, int4* __restrict__ slot_pos,
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c == 0) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c == 0 && lane == 0) {
     // the shared padding slot of the pair lists
     slot_pos[dummy_slot] = make_int4(0, 0, 0, 0);
     if (slot_pos64) slot_pos64[dummy_slot] = make_longlong4(0, 0, 0, 0);
@@ -138,55 +141,57 @@ __global__ void k_fill_slots(PmBox box, int ncells, const double* __restrict__ p
     atom_of_slot[dummy_slot] = -1;
   }
   if (c >= ncells) return;
-  int k = cell_count[c];
+  const int k = cell_count[c];
   if (k == 0) return;
-  int* a = cell_atoms + cell_start[c];
-  // insertion sort by (key, index); a cell holding more than PM_SORT_MAX atoms
-  // (collapsed or non-finite coordinates) keeps its arrival order so that the
-  // thread's work stays bounded
-  const int ks = k <= PM_SORT_MAX ? k : 0;
-  for (int u = 1; u < ks; u++) {
-    int v = a[u];
-    float kv = key[v];
-    int w = u - 1;
-    while (w >= 0) {
-      int aw = a[w];
-      float kw = key[aw];
-      if (kw > kv || (kw == kv && aw > v)) { a[w + 1] = aw; w--; }
-      else break;
+  const int* a = cell_atoms + cell_start[c];
+  const int s0 = cell_slot[c];
+  const bool sort = k <= PM_SORT_MAX;
+  for (int i = lane; i < k; i += 64) {
+    const int ai = a[i];
+    const float ki = key[ai];
+    int rank = i;
+    if (sort) {
+      rank = 0;
+      for (int j = 0; j < k; j++) {
+        const int aj = a[j];
+        const float kj = key[aj];
+        rank += (kj < ki || (kj == ki && aj < ai)) ? 1 : 0;
+      }
     }
-    a[w + 1] = v;
-  }
-  int s0 = cell_slot[c];
-  int4 first = make_int4(0, 0, 0, 0);
-  for (int u = 0; u < k; u++) {
-    int at = a[u];
     int4 p;
-    p.x = (int)to_fix(pos[3 * at], box.to_fix[0]);
-    p.y = (int)to_fix(pos[3 * at + 1], box.to_fix[1]);
-    p.z = (int)to_fix(pos[3 * at + 2], box.to_fix[2]);
-    p.w = __float_as_int((float)q[at]);
-    if (u == 0) first = p;
-    slot_pos[s0 + u] = p;
+    p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
+    p.y = (int)to_fix(pos[3 * ai + 1], box.to_fix[1]);
+    p.z = (int)to_fix(pos[3 * ai + 2], box.to_fix[2]);
+    p.w = __float_as_int((float)q[ai]);
+    slot_pos[s0 + rank] = p;
     if (slot_pos64) {
       // 64-bit periodic fixed point for the FP64 kernels
       longlong4 p8;
-      p8.x = (long long)to_fix64(pos[3 * at], box.to_fix[0]);
-      p8.y = (long long)to_fix64(pos[3 * at + 1], box.to_fix[1]);
-      p8.z = (long long)to_fix64(pos[3 * at + 2], box.to_fix[2]);
-      p8.w = __double_as_longlong(q[at]);
-      slot_pos64[s0 + u] = p8;
+      p8.x = (long long)to_fix64(pos[3 * ai], box.to_fix[0]);
+      p8.y = (long long)to_fix64(pos[3 * ai + 1], box.to_fix[1]);
+      p8.z = (long long)to_fix64(pos[3 * ai + 2], box.to_fix[2]);
+      p8.w = __double_as_longlong(q[ai]);
+      slot_pos64[s0 + rank] = p8;
     }
-    slot_meta[s0 + u] = type[at] | (at << 8);
-    atom_of_slot[s0 + u] = at;
+    slot_meta[s0 + rank] = type[ai] | (ai << 8);
+    atom_of_slot[s0 + rank] = ai;
   }
-  int ps = (k + PM_M - 1) / PM_M * PM_M;
-  for (int u = k; u < ps; u++) {
-    int4 p = first;
+  // padding slots: never used as j atoms, masked as i atoms, skipped by PME
+  const int ps = (k + PM_M - 1) / PM_M * PM_M;
+  const int u = k + lane;
+  if (u < ps) {
+    const int a0 = a[0];
+    int4 p;
+    p.x = (int)to_fix(pos[3 * a0], box.to_fix[0]);
+    p.y = (int)to_fix(pos[3 * a0 + 1], box.to_fix[1]);
+    p.z = (int)to_fix(pos[3 * a0 + 2], box.to_fix[2]);
     p.w = 0;
     slot_pos[s0 + u] = p;
     if (slot_pos64) {
-      longlong4 p8 = slot_pos64[s0];
+      longlong4 p8;
+      p8.x = (long long)to_fix64(pos[3 * a0], box.to_fix[0]);
+      p8.y = (long long)to_fix64(pos[3 * a0 + 1], box.to_fix[1]);
+      p8.z = (long long)to_fix64(pos[3 * a0 + 2], box.to_fix[2]);
       p8.w = 0;
       slot_pos64[s0 + u] = p8;
     }
@@ -338,11 +343,28 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
   if (lane == 0) {
     if (count > a.lcap) { a.overflow[0] = 1; count = a.lcap; }
     a.jcount[cl] = count;
-    if (a.stats) {
-      atomicAdd(&a.stats[0], (unsigned long long)count);
-      atomicAdd(&a.stats[1], (unsigned long long)((count + PM_JW - 1) / PM_JW * 64));
-    }
   }
+}
+
+// list statistics: total entries and issued lanes over all clusters
+__global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcount, const int* __restrict__ counts,
+                                                    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s0[256], s1[256];
+  const int ncl = counts[1];
+  unsigned long long e = 0, l = 0;
+  for (int c = threadIdx.x; c < ncl; c += 256) {
+    const int v = jcount[c];
+    e += v;
+    l += (unsigned long long)((v + PM_JW - 1) / PM_JW) * 64;
+  }
+  s0[threadIdx.x] = e;
+  s1[threadIdx.x] = l;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) { s0[threadIdx.x] += s0[threadIdx.x + off]; s1[threadIdx.x] += s1[threadIdx.x + off]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = s0[0]; out[1] = s1[0]; }
 }
 
 // ------------------------------------------------------------ pair kernel
@@ -554,6 +576,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     a.force_fix[3 * oi + 1] += __double2ll_rn(fya * PM_FORCE_FIX);
     a.force_fix[3 * oi + 2] += __double2ll_rn(fza * PM_FORCE_FIX);
   }
+  if (!a.want_energy) return;
   // energies and virial (each pair was seen from both sides: factor 1/2)
   double r[8] = {ireal ? (double)ec : 0.0, ireal ? (double)elj : 0.0,
                  ireal ? (double)vxx : 0.0, ireal ? (double)vxy : 0.0, ireal ? (double)vxz : 0.0,
@@ -687,10 +710,14 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
                           int* atom_of_slot, int dummy_type, int dummy_slot) {
-  int nb = (ncells + 127) / 128;
-  hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(128), 0, s, box, ncells, pos, q, type,
+  int nb = (ncells + 3) / 4;
+  hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, box, ncells, pos, q, type,
                      cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta,
                      atom_of_slot, dummy_type, dummy_slot);
+}
+
+void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out) {
+  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out);
 }
 
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {

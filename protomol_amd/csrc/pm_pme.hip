@@ -41,79 +41,111 @@ template <typename T2> __device__ inline T2 mul_si(T2 a, int sign) {
 }
 
 // R-point DFTs, out_k = sum_r v_r exp(sign 2 pi i r k / R)
-template <typename T, typename T2>
-__device__ inline void dft(T2* v, int R, int sign) {
-  if (R == 2) {
+template <typename T, typename T2, int R> struct Dft;
+template <typename T, typename T2> struct Dft<T, T2, 2> {
+  static __device__ inline void run(T2* v, int) {
     T2 a = v[0], b = v[1];
     v[0] = cadd(a, b); v[1] = csub(a, b);
-  } else if (R == 4) {
+  }
+};
+template <typename T, typename T2> struct Dft<T, T2, 4> {
+  static __device__ inline void run(T2* v, int sign) {
     T2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
     T2 c = cadd(v[1], v[3]), d = mul_si(csub(v[1], v[3]), sign);
     v[0] = cadd(a, c); v[1] = cadd(b, d); v[2] = csub(a, c); v[3] = csub(b, d);
-  } else if (R == 3) {
+  }
+};
+template <typename T, typename T2> struct Dft<T, T2, 3> {
+  static __device__ inline void run(T2* v, int sign) {
     const T h = (T)0.86602540378443864676;  // sqrt(3)/2
     T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
     T2 t1; t1.x = v[0].x - (T)0.5 * s.x; t1.y = v[0].y - (T)0.5 * s.y;
     T2 t2 = mul_si(d, sign); t2.x *= h; t2.y *= h;
     v[0] = cadd(v[0], s); v[1] = cadd(t1, t2); v[2] = csub(t1, t2);
-  } else {  // R == 5
+  }
+};
+template <typename T, typename T2> struct Dft<T, T2, 5> {
+  static __device__ inline void run(T2* v, int sign) {
     const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
     const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
     T2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
     T2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
     T2 x0 = v[0];
-    T2 p1, p2, q1, q2;
+    T2 p1, p2;
     p1.x = x0.x + c1 * a1.x + c2 * a2.x; p1.y = x0.y + c1 * a1.y + c2 * a2.y;
     p2.x = x0.x + c2 * a1.x + c1 * a2.x; p2.y = x0.y + c2 * a1.y + c1 * a2.y;
     T2 m1; m1.x = s1 * b1.x + s2 * b2.x; m1.y = s1 * b1.y + s2 * b2.y;
     T2 m2; m2.x = s2 * b1.x - s1 * b2.x; m2.y = s2 * b1.y - s1 * b2.y;
-    q1 = mul_si(m1, sign); q2 = mul_si(m2, sign);
+    T2 q1 = mul_si(m1, sign), q2 = mul_si(m2, sign);
     v[0].x = x0.x + a1.x + a2.x; v[0].y = x0.y + a1.y + a2.y;
     v[1] = cadd(p1, q1); v[4] = csub(p1, q1);
     v[2] = cadd(p2, q2); v[3] = csub(p2, q2);
   }
+};
+
+// one Stockham pass of radix R over nl lines of length n held in LDS
+template <typename T, typename T2, int R>
+__device__ inline void fft_stage(const T2* __restrict__ in, T2* __restrict__ out, int nl, int n, int Ns,
+                                 const T2* __restrict__ tw, int sign) {
+  const int nR = n / R;
+  const int tws = n / (Ns * R);
+  for (int t = threadIdx.x; t < nl * nR; t += blockDim.x) {
+    const int l = t / nR;
+    const int j = t - l * nR;
+    const int k = j % Ns;
+    T2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = in[l * n + j + r * nR];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        T2 w = tw[r * k * tws];          // r*k*tws < R*Ns*tws = n
+        if (sign > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    Dft<T, T2, R>::run(v, sign);
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) out[l * n + idxD + r * Ns] = v[r];
+  }
 }
 
 // Stockham passes over `nl` lines of length p.n held in LDS (line l at
-// buf[l * n]); all threads of the block take part.  Returns the buffer that
-// holds the result.
+// buf[l * n]); all threads of the block take part.  tw points at the
+// n twiddles exp(-2 pi i m / n) already staged in LDS.  Returns the buffer
+// that holds the result.
 template <typename T, typename T2>
-__device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const double* tw, int sign) {
+__device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const T2* tw, int sign) {
   const int n = p.n;
   T2* in = b0;
   T2* out = b1;
   int Ns = 1;
   for (int st = 0; st < p.nst; st++) {
     const int R = p.rad[st];
-    const int nR = n / R;
-    const int tws = n / (Ns * R);
-    for (int t = threadIdx.x; t < nl * nR; t += blockDim.x) {
-      const int l = t / nR;
-      const int j = t - l * nR;
-      const int k = j % Ns;
-      T2 v[5];
-      const T2* src = in + l * n + j;
-      for (int r = 0; r < R; r++) v[r] = src[r * nR];
-      if (Ns > 1) {
-        for (int r = 1; r < R; r++) {
-          int m = (r * k * tws) % n;
-          T2 w;
-          w.x = (T)tw[2 * (p.tw + m)];
-          w.y = (T)tw[2 * (p.tw + m) + 1];
-          if (sign > 0) w.y = -w.y;
-          v[r] = cmul(v[r], w);
-        }
-      }
-      dft<T, T2>(v, R, sign);
-      const int idxD = (j / Ns) * Ns * R + k;
-      T2* dst = out + l * n + idxD;
-      for (int r = 0; r < R; r++) dst[r * Ns] = v[r];
+    switch (R) {
+      case 2: fft_stage<T, T2, 2>(in, out, nl, n, Ns, tw, sign); break;
+      case 3: fft_stage<T, T2, 3>(in, out, nl, n, Ns, tw, sign); break;
+      case 4: fft_stage<T, T2, 4>(in, out, nl, n, Ns, tw, sign); break;
+      default: fft_stage<T, T2, 5>(in, out, nl, n, Ns, tw, sign); break;
     }
     __syncthreads();
     T2* tmp = in; in = out; out = tmp;
     Ns *= R;
   }
   return in;
+}
+
+// stage the twiddles of one transform length into LDS (followed by a barrier
+// in the caller's first __syncthreads)
+template <typename T2>
+__device__ inline void load_twiddles(T2* dst, const PmFftPlan& p, const double* tw) {
+  for (int m = threadIdx.x; m < p.n; m += blockDim.x) {
+    T2 w;
+    w.x = tw[2 * (p.tw + m)];
+    w.y = tw[2 * (p.tw + m) + 1];
+    dst[m] = w;
+  }
 }
 
 // ---------------------------------------------------------- B-spline (P)
@@ -205,6 +237,115 @@ __global__ __launch_bounds__(256) void k_spread(PmPmeLaunch a, const longlong4* 
   }
 }
 
+// Spread by bricks of cells: one block accumulates the atoms of a brick into
+// an LDS tile of 64-bit fixed-point charges (exact integer adds, so the
+// result does not depend on the order), then adds the tile to the global grid
+// with coalesced 64-bit atomics.  Stencil points outside the tile (atoms on a
+// brick face whose rounding lands them one point further) go to the global
+// grid directly.
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_spread_tile(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
+  extern __shared__ unsigned long long s_tile[];
+  __shared__ int s_cs[64], s_cc[64], s_pre[65];
+  const int bz = blockIdx.x % a.nbrick[2];
+  const int by = (blockIdx.x / a.nbrick[2]) % a.nbrick[1];
+  const int bx = blockIdx.x / (a.nbrick[1] * a.nbrick[2]);
+  const int tnx = a.tile[0], tny = a.tile[1], tnz = a.tile[2];
+  const int ntile = tnx * tny * tnz;
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) s_tile[t] = 0ull;
+  // cells of the brick
+  const int cx0 = bx * a.brick[0], cy0 = by * a.brick[1], cz0 = bz * a.brick[2];
+  const int ex = min(a.brick[0], a.ncell[0] - cx0), ey = min(a.brick[1], a.ncell[1] - cy0),
+            ez = min(a.brick[2], a.ncell[2] - cz0);
+  const int ncb = ex * ey * ez;
+  if (threadIdx.x < ncb) {
+    const int i = threadIdx.x / (ey * ez), j = (threadIdx.x / ez) % ey, k = threadIdx.x % ez;
+    const int c = ((cx0 + i) * a.ncell[1] + (cy0 + j)) * a.ncell[2] + (cz0 + k);
+    s_cs[threadIdx.x] = a.cell_slot[c];
+    s_cc[threadIdx.x] = a.cell_count[c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int u = 0; u < ncb; u++) { s_pre[u] = acc; acc += s_cc[u]; }
+    s_pre[ncb] = acc;
+  }
+  // tile origin: grid point of the brick's lower corner, one point of margin
+  // below the stencil start
+  const int off = -(P - 1) / 2;
+  int t0[3];
+  const int nn[3] = {a.nx, a.ny, a.nz};
+  const int c0[3] = {cx0, cy0, cz0};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (a.tile[d] >= nn[d]) { t0[d] = 0; continue; }
+    double u = a.cell_lo_u[d] + c0[d] * a.cell_du[d];
+    int g = (int)floor(u) + off - 1;
+    g %= nn[d];
+    if (g < 0) g += nn[d];
+    t0[d] = g;
+  }
+  __syncthreads();
+  const int natoms = s_pre[ncb];
+  const T scale = (T)PM_GRID_FIX;
+  for (int t = threadIdx.x; t < natoms; t += blockDim.x) {
+    int cb = 0;
+    while (cb + 1 < ncb && s_pre[cb + 1] <= t) cb++;
+    const int sl = s_cs[cb] + (t - s_pre[cb]);
+    const int at = a.atom_of_slot[sl];
+    int ix, iy, iz;
+    T wx, wy, wz;
+    if constexpr (sizeof(T) == 4) {
+      const int4 p = a.slot_pos[sl];
+      grid_coord<T>((unsigned)p.x, a.nx, ix, wx);
+      grid_coord<T>((unsigned)p.y, a.ny, iy, wy);
+      grid_coord<T>((unsigned)p.z, a.nz, iz, wz);
+    } else {
+      const longlong4 p = pos64[sl];
+      grid_coord64<T>((unsigned long long)p.x, a.nx, ix, wx);
+      grid_coord64<T>((unsigned long long)p.y, a.ny, iy, wy);
+      grid_coord64<T>((unsigned long long)p.z, a.nz, iz, wz);
+    }
+    T tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+    bspline<T, P>(wx, tx, dx);
+    bspline<T, P>(wy, ty, dy);
+    bspline<T, P>(wz, tz, dz);
+    const T q = (T)a.q[at];
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+      const int gx = (ix + off + i + a.nx) % a.nx;
+      int lx = gx - t0[0]; if (lx < 0) lx += a.nx;
+      const T qa = q * tx[i];
+#pragma unroll
+      for (int j = 0; j < P; j++) {
+        const int gy = (iy + off + j + a.ny) % a.ny;
+        int ly = gy - t0[1]; if (ly < 0) ly += a.ny;
+        const T qab = qa * ty[j];
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          const int gz = (iz + off + k + a.nz) % a.nz;
+          int lz = gz - t0[2]; if (lz < 0) lz += a.nz;
+          long long v;
+          if constexpr (sizeof(T) == 4) v = __float2ll_rn(qab * tz[k] * scale);
+          else v = __double2ll_rn(qab * tz[k] * scale);
+          if (lx < tnx && ly < tny && lz < tnz)
+            atomicAdd(&s_tile[(lx * tny + ly) * tnz + lz], (unsigned long long)v);
+          else
+            atomicAdd(a.grid_fix + ((long long)gx * a.ny + gy) * a.nz + gz, (unsigned long long)v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+    const unsigned long long v = s_tile[t];
+    if (v == 0ull) continue;
+    const int lz = t % tnz, ly = (t / tnz) % tny, lx = t / (tny * tnz);
+    const int gx = (t0[0] + lx) % a.nx, gy = (t0[1] + ly) % a.ny, gz = (t0[2] + lz) % a.nz;
+    atomicAdd(a.grid_fix + ((long long)gx * a.ny + gy) * a.nz + gz, v);
+  }
+}
+
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,6 +407,8 @@ __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, i
   extern __shared__ unsigned char s_raw[];
   T2* b0 = reinterpret_cast<T2*>(s_raw);
   T2* b1 = b0 + TL * p.n;
+  T2* tws = b1 + TL * p.n;
+  load_twiddles(tws, p, a.tw);
   const int n = p.n, nl_tot = a.nx * a.ny;
   const int line0 = blockIdx.x * TL;
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
@@ -280,7 +423,7 @@ __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, i
     b0[e] = v;
   }
   __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, -1);
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1);
   T2* gc = reinterpret_cast<T2*>(a.grid_c);
   for (int e = threadIdx.x; e < TL * a.nzc; e += blockDim.x) {
     const int l = e / a.nzc, k = e - l * a.nzc;
@@ -296,6 +439,8 @@ __global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int T
   extern __shared__ unsigned char s_raw[];
   T2* b0 = reinterpret_cast<T2*>(s_raw);
   T2* b1 = b0 + TL * p.n;
+  T2* tws = b1 + TL * p.n;
+  load_twiddles(tws, p, a.tw);
   const int n = p.n;  // ny
   const int ntile = (a.nzc + TL - 1) / TL;
   const int x = blockIdx.x / ntile;
@@ -308,7 +453,7 @@ __global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int T
     b0[l * n + y] = v;
   }
   __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, sign);
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, sign);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int y = e / TL, l = e - y * TL;
     if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r[l * n + y];
@@ -323,6 +468,8 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
   extern __shared__ unsigned char s_raw[];
   T2* b0 = reinterpret_cast<T2*>(s_raw);
   T2* b1 = b0 + TL * p.n;
+  T2* tws = b1 + TL * p.n;
+  load_twiddles(tws, p, a.tw);
   __shared__ double s_red[8][8];
   const int n = p.n;  // nx
   const int ntile = (a.nzc + TL - 1) / TL;
@@ -336,7 +483,7 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
     b0[l * n + x] = v;
   }
   __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, -1);
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1);
   // convolution
   double en = 0, vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
   const int j0 = y <= a.ny / 2 ? y : y - a.ny;
@@ -404,7 +551,7 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
   }
   // (the conv loop wrote r[] before the barrier above)
   T2* other = (r == b0) ? b1 : b0;
-  T2* r2 = fft_lines<T, T2>(r, other, TL, p, a.tw, +1);
+  T2* r2 = fft_lines<T, T2>(r, other, TL, p, tws, +1);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int x = e / TL, l = e - x * TL;
     if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r2[l * n + x];
@@ -418,6 +565,8 @@ __global__ __launch_bounds__(256) void k_fft_z_inv(PmPmeLaunch a, PmFftPlan p, i
   extern __shared__ unsigned char s_raw[];
   T2* b0 = reinterpret_cast<T2*>(s_raw);
   T2* b1 = b0 + TL * p.n;
+  T2* tws = b1 + TL * p.n;
+  load_twiddles(tws, p, a.tw);
   const int n = p.n, nl_tot = a.nx * a.ny, nzc = a.nzc;
   const int line0 = blockIdx.x * TL;
   const T2* gc = reinterpret_cast<const T2*>(a.grid_c);
@@ -432,7 +581,7 @@ __global__ __launch_bounds__(256) void k_fft_z_inv(PmPmeLaunch a, PmFftPlan p, i
     b0[e] = v;
   }
   __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, a.tw, +1);
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, +1);
   T* g = reinterpret_cast<T*>(a.grid_r);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int l = e / n, k = e - l * n;
@@ -456,31 +605,35 @@ static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* p
                          const longlong4* pos64) {
   typedef typename Cx<T>::t T2;
   const int nthreads = 256;
-  hipLaunchKernelGGL((k_spread<T, P>), dim3((a.nslot_max + 255) / 256), dim3(nthreads), 0, s, a, pos64);
+  {
+    const int nbk = a.nbrick[0] * a.nbrick[1] * a.nbrick[2];
+    const size_t lds = (size_t)a.tile[0] * a.tile[1] * a.tile[2] * sizeof(unsigned long long);
+    hipLaunchKernelGGL((k_spread_tile<T, P>), dim3(nbk), dim3(nthreads), lds, s, a, pos64);
+  }
   {
     const int tl = lines_per_block(a.nz, sizeof(T2));
     const int nb = (a.nx * a.ny + tl - 1) / tl;
-    hipLaunchKernelGGL(k_fft_z_fwd<T>, dim3(nb), dim3(nthreads), 2 * tl * a.nz * sizeof(T2), s, a, plans[2], tl);
+    hipLaunchKernelGGL(k_fft_z_fwd<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nz * sizeof(T2), s, a, plans[2], tl);
   }
   {
     const int tl = lines_per_block(a.ny, sizeof(T2));
     const int nb = a.nx * ((a.nzc + tl - 1) / tl);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), 2 * tl * a.ny * sizeof(T2), s, a, plans[1], tl, -1);
+    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.ny * sizeof(T2), s, a, plans[1], tl, -1);
   }
   {
     const int tl = lines_per_block(a.nx, sizeof(T2));
     const int nb = a.ny * ((a.nzc + tl - 1) / tl);
-    hipLaunchKernelGGL(k_fft_x_conv<T>, dim3(nb), dim3(nthreads), 2 * tl * a.nx * sizeof(T2), s, a, plans[0], tl);
+    hipLaunchKernelGGL(k_fft_x_conv<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nx * sizeof(T2), s, a, plans[0], tl);
   }
   {
     const int tl = lines_per_block(a.ny, sizeof(T2));
     const int nb = a.nx * ((a.nzc + tl - 1) / tl);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), 2 * tl * a.ny * sizeof(T2), s, a, plans[1], tl, +1);
+    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.ny * sizeof(T2), s, a, plans[1], tl, +1);
   }
   {
     const int tl = lines_per_block(a.nz, sizeof(T2));
     const int nb = (a.nx * a.ny + tl - 1) / tl;
-    hipLaunchKernelGGL(k_fft_z_inv<T>, dim3(nb), dim3(nthreads), 2 * tl * a.nz * sizeof(T2), s, a, plans[2], tl);
+    hipLaunchKernelGGL(k_fft_z_inv<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nz * sizeof(T2), s, a, plans[2], tl);
   }
   hipLaunchKernelGGL((k_gather<T, P>), dim3((a.nslot_max + 255) / 256), dim3(nthreads), 0, s, a, pos64);
 }
