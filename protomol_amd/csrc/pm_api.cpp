@@ -602,6 +602,18 @@ int parse_force(pm_context* c, const char* id_c, const double* v, int nv, ForceD
     return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
   if (pots.size() == 2 && (pots[0] != 0 || pots[1] == 0))
     return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+  // only the combinations the periodic factory registers
+  // (factories/registerForceExemplarsCutoff.cpp:39,40,45,57,62,63,67,76,88,90,91,131)
+  {
+    bool ok;
+    if (pots.size() == 1)
+      ok = pots[0] != 2 || swfs[0] == PM_SWF_CUTOFF;   // CoulombEwaldReal only with Cutoff (:57)
+    else if (pots[1] == 2)
+      ok = swfs[1] == PM_SWF_CUTOFF;                    // LJ {C1,C2,Cutoff} + EwaldReal/Cutoff (:76,90,131)
+    else
+      ok = swfs[0] == PM_SWF_C2 && (swfs[1] == PM_SWF_C1 || swfs[1] == PM_SWF_CUTOFF);  // (:88, :91)
+    if (!ok) return fail(c, PM_ERR_UNSUPPORTED, "Could not find any match for '%s' in ForceFactory", id.c_str());
+  }
   // positional values: per term potential params then switching params, then
   // the algorithm cutoff (OneAtomPair(Two)::getParameters, NonbondedCutoffForce.h:113-117)
   int k = 0;
