@@ -159,7 +159,8 @@ int pm_last_stats(pm_context* ctx, long long* stats, int nstats);
 
 /* Kernel timing with HIP events recorded on the context's stream around the
  * kernel groups selected by mask (bit k = category k):
- *   0 binning/slots, 1 pair lists, 2 pair kernel, 3 PME reciprocal
+ *   0 binning/slots (and folding per-slot pair forces into atoms),
+ *   1 pair lists, 2 pair kernel, 3 PME reciprocal
  *   (spread, FFTs, influence function, gather), 4 PME correction,
  *   5 bond/angle, 6 energy reduction, 7 position/velocity update.
  * No reference counterpart (the reference's TimerStatistic wraps whole force

@@ -278,8 +278,9 @@ typedef struct {
 
 /* ExclusionTable::add/check (topology/ExclusionTable.cpp:112-134,
  * ExclusionTable.h:130-146): pairs are stored with a1 < a2, the last add of a
- * pair wins for check(), every distinct (a1,a2,class) with class != NONE and
- * a1 != a2 is kept in the table in the order of the adds. */
+ * pair wins for check(); the table keeps, in the order of the adds, the first
+ * add with class != NONE and a1 != a2 of every pair (mySet orders
+ * ExclusionPair by (a1, a2) only, ExclusionTable.cpp:20-28). */
 static void excl_build(orc_excl* e, int n, int nexcl, const int* ea,
                        const int* eb, const int* ec) {
   int i, t;
@@ -304,7 +305,7 @@ static void excl_build(orc_excl* e, int n, int nexcl, const int* ea,
     if (a > b) { int s = a; a = b; b = s; }
     /* the row keeps every add in order: check() takes the last match */
     for (u = e->start[a]; u < fill[a]; u++)
-      if (e->partner[u] == b && e->cls[u] == c) { seen = 1; break; }
+      if (e->partner[u] == b && e->cls[u] != ORC_EXCL_NONE) { seen = 1; break; }
     e->partner[fill[a]] = b;
     e->cls[fill[a]] = c;
     fill[a]++;

@@ -91,6 +91,8 @@ struct pm_context {
   double *d_q = nullptr, *d_mass = nullptr;
   int *d_type = nullptr, *d_mol = nullptr;
   int *d_ex_row = nullptr, *d_ex_partner = nullptr, *d_ex_cls = nullptr;      // lookup CSR
+  int *d_xs_row = nullptr, *d_xs_part = nullptr;   // resolved exclusions, both directions
+  int xs_maxdeg = 0;
   int *d_cx_row = nullptr, *d_cx_partner = nullptr, *d_cx_cls = nullptr;      // correction CSR
   int *d_tab_a = nullptr, *d_tab_b = nullptr, *d_tab_c = nullptr;
   float4* d_lj_f = nullptr;
@@ -104,11 +106,12 @@ struct pm_context {
 
   // cells and slots
   int *d_cell_of = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_slot = nullptr,
-      *d_cell_cursor = nullptr, *d_cell_atoms = nullptr, *d_cluster_cell = nullptr, *d_counts = nullptr;
+      *d_cell_rank = nullptr, *d_cell_atoms = nullptr, *d_cluster_cell = nullptr, *d_counts = nullptr;
   float* d_key = nullptr;
   int4* d_slot_pos = nullptr;
   longlong4* d_slot_pos64 = nullptr;
   int *d_slot_meta = nullptr, *d_atom_of_slot = nullptr;
+  long long* d_force_slot = nullptr;   // [3 * (nslot_max + 1)] pair-kernel forces per slot
   int ncl_max = 0, nslot_max = 0;
   int cells_alloc = 0;
 
@@ -118,6 +121,7 @@ struct pm_context {
 
   // lists
   int *d_jlist = nullptr, *d_jcount = nullptr, *d_overflow = nullptr;
+  unsigned short* d_jmask = nullptr;   // per list entry: FULL / MODIFIED masks of flagged entries
   long long jlist_alloc = 0;
   int lcap = 0;
   unsigned long long* d_stats = nullptr;
@@ -344,7 +348,7 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_cell_count, nc))) return r;
   if ((r = dalloc(c, &c->d_cell_start, nc))) return r;
   if ((r = dalloc(c, &c->d_cell_slot, nc))) return r;
-  if ((r = dalloc(c, &c->d_cell_cursor, nc))) return r;
+  if ((r = dalloc(c, &c->d_cell_rank, n))) return r;
   if ((r = dalloc(c, &c->d_cell_atoms, n))) return r;
   if ((r = dalloc(c, &c->d_counts, 4))) return r;
   long long slots = (long long)n + (long long)(PM_M - 1) * nc;
@@ -355,6 +359,7 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_slot_pos64, (size_t)c->nslot_max + 1))) return r;
   if ((r = dalloc(c, &c->d_slot_meta, (size_t)c->nslot_max + 1))) return r;
   if ((r = dalloc(c, &c->d_atom_of_slot, (size_t)c->nslot_max + 1))) return r;
+  if ((r = dalloc(c, &c->d_force_slot, 3 * ((size_t)c->nslot_max + 1)))) return r;
   HIPCHECK(c, hipMemset(c->d_counts, 0, 4 * sizeof(int)));
   if (!c->d_overflow) {
     if ((r = dalloc(c, &c->d_overflow, 1))) return r;
@@ -387,6 +392,7 @@ int ensure_lists(pm_context* c, int lcap) {
   if (need <= c->jlist_alloc && c->d_jlist) { c->lcap = lcap; return PM_OK; }
   int r;
   if ((r = dalloc(c, &c->d_jlist, (size_t)need))) return r;
+  if ((r = dalloc(c, &c->d_jmask, (size_t)need))) return r;
   if ((r = dalloc(c, &c->d_jcount, c->ncl_max))) return r;
   c->jlist_alloc = need;
   c->lcap = lcap;
@@ -769,12 +775,12 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   const int nc = c->ncells;
   {
   TimedScope ts(c, PM_T_BIN);
-  pm_launch_zero_ll(s, c->d_force, 3LL * n);
-  pm_launch_zero_i(s, c->d_cell_count, nc);
-  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_key);
-  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_cell_cursor, c->d_counts,
+  pm_launch_zero_force_cells(s, c->d_force, 3LL * n, c->d_force_slot, passes.empty() ? 0 : 3LL * c->nslot_max,
+                             c->d_cell_count, nc);
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key);
+  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
                        c->d_cluster_cell, c->ncl_max);
-  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_cursor, c->d_cell_atoms);
+  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms);
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
                        c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
@@ -835,6 +841,10 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.overflow = c->d_overflow;
     a.bbox_filter = lmin_half > pcut + cdiag + 1e-3 ? 1 : 0;
     a.max_delta = c->max_delta;
+    a.xs_row = c->d_xs_row;
+    a.xs_part = c->d_xs_part;
+    a.xs_tcap = PM_M * c->xs_maxdeg;
+    a.jmask = c->d_jmask;
     a.molecule = c->d_mol;
     a.ex_row = c->d_ex_row;
     a.ex_partner = c->d_ex_partner;
@@ -843,6 +853,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.lj_f = c->d_lj_f;
     a.lj_d = c->d_lj_d;
     a.force_fix = c->d_force;
+    a.force_slot = c->d_force_slot;
     a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
     a.want_virial = vir ? 1 : 0;
     a.want_energy = want_energy ? 1 : 0;
@@ -854,6 +865,10 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       TimedScope ts(c, PM_T_PAIR);
       pm_launch_pair(s, c->box, a, pp, c->precision);
     }
+  }
+  if (!passes.empty()) {
+    TimedScope ts(c, PM_T_BIN);
+    pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_force_slot, c->d_force);
   }
   double const_coul = 0.0, const_vir = 0.0;
   for (auto& hr : pme_rec) {
@@ -923,7 +938,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       TimedScope ts(c, PM_T_CORR);
       pm_launch_correction(s, c->box, n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls, f.alpha,
                            c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
-                           c->d_epart + off * PM_NE, nblk);
+                           c->d_epart + off * PM_NE, want_energy ? nblk : 0);
       // point self energy and charged-system term (:604-628), cached per topology
       double q2 = 0.0, qs = 0.0;
       for (int i = 0; i < n; i++) q2 += c->q[i] * c->q[i];
@@ -951,6 +966,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     b.force_fix = c->d_force;
     b.epart = c->d_epart + bonded_off * PM_NE;
     b.want_virial = vir ? 1 : 0;
+    b.want_energy = want_energy ? 1 : 0;
     int rb, ra;
     TimedScope ts(c, PM_T_BONDED);
     pm_launch_bonded(s, c->box, b, &rb, &ra);
@@ -969,18 +985,7 @@ int check_overflow(pm_context* c) {
   if (!c->d_overflow) return PM_OK;
   int ov = 0;
   HIPCHECK(c, hipMemcpyAsync(&ov, c->d_overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  unsigned long long st[2] = {0, 0};
-  if (c->d_stats && c->d_jcount && c->d_counts) {
-    pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats);
-    HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
-  }
-  int counts[4] = {0, 0, 0, 0};
-  if (c->d_counts) HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  c->last_stats[0] = (long long)st[0];
-  c->last_stats[1] = (long long)st[1];
-  c->last_stats[2] = counts[1];
-  c->last_stats[3] = counts[0];
   if (ov) {
     HIPCHECK(c, hipMemset(c->d_overflow, 0, sizeof(int)));
     return fail(c, PM_ERR_CAPACITY, "pair list capacity (%d entries per cluster) exceeded: atoms are packed far denser than the box average", c->lcap);
@@ -988,15 +993,33 @@ int check_overflow(pm_context* c) {
   return PM_OK;
 }
 
+// list statistics of the last evaluation (on demand: one reduction + copy)
+int list_stats(pm_context* c) {
+  unsigned long long st[2] = {0, 0};
+  int counts[4] = {0, 0, 0, 0};
+  if (c->d_stats && c->d_jcount && c->d_counts) {
+    pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats);
+    HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  c->last_stats[0] = (long long)st[0];
+  c->last_stats[1] = (long long)st[1];
+  c->last_stats[2] = counts[1];
+  c->last_stats[3] = counts[0];
+  return PM_OK;
+}
+
 void free_all(pm_context* c) {
   dfree(c->d_q); dfree(c->d_mass); dfree(c->d_type); dfree(c->d_mol);
   dfree(c->d_ex_row); dfree(c->d_ex_partner); dfree(c->d_ex_cls);
+  dfree(c->d_xs_row); dfree(c->d_xs_part); dfree(c->d_jmask);
   dfree(c->d_cx_row); dfree(c->d_cx_partner); dfree(c->d_cx_cls);
   dfree(c->d_tab_a); dfree(c->d_tab_b); dfree(c->d_tab_c);
   dfree(c->d_lj_f); dfree(c->d_lj_d);
   dfree(c->d_pos); dfree(c->d_vel); dfree(c->d_force); dfree(c->d_energy); dfree(c->d_scratch3);
   dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
-  dfree(c->d_cell_cursor); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
+  dfree(c->d_cell_rank); dfree(c->d_force_slot); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
   dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_overflow); dfree(c->d_stats); dfree(c->d_epart); dfree(c->d_ered);
   dfree(c->d_grid_fix);
@@ -1114,14 +1137,48 @@ int pm_set_topology(pm_context* c, int n, const double* q, const int* type, int 
       c->max_delta = std::max(c->max_delta, b - a);
     }
   }
-  // the table (ExclusionTable::myTable): distinct (a, b, class), class != NONE, a != b
+  // resolved exclusion of every pair as OneAtomPair sees it: the class of the
+  // last add (ExclusionTable::add stores it, NONE included), consulted only
+  // for atoms of one molecule (OneAtomPair.h:149-153).  Stored for both atoms,
+  // partner | class << 24, sorted by partner.
+  {
+    std::map<std::pair<int, int>, int> last;
+    for (int k = 0; k < nexcl; k++) {
+      int a = std::min(ex_a[k], ex_b[k]), b = std::max(ex_a[k], ex_b[k]);
+      if (a != b) last[std::make_pair(a, b)] = ex_class[k];
+    }
+    std::vector<std::vector<int>> adj(n);
+    for (auto& kv : last) {
+      int a = kv.first.first, b = kv.first.second;
+      if (kv.second == PM_EXCL_NONE || molecule[a] != molecule[b]) continue;
+      adj[a].push_back(b | (kv.second << 24));
+      adj[b].push_back(a | (kv.second << 24));
+    }
+    std::vector<int> xrow(n + 1, 0), xpart;
+    c->xs_maxdeg = 0;
+    for (int i = 0; i < n; i++) {
+      std::sort(adj[i].begin(), adj[i].end(), [](int u, int v) { return (u & 0xffffff) < (v & 0xffffff); });
+      xrow[i + 1] = xrow[i] + (int)adj[i].size();
+      c->xs_maxdeg = std::max(c->xs_maxdeg, (int)adj[i].size());
+      xpart.insert(xpart.end(), adj[i].begin(), adj[i].end());
+    }
+    if (c->xs_maxdeg > 256) {
+      free_all(c);
+      return fail(c, PM_ERR_UNSUPPORTED, "pm_set_topology: an atom has %d exclusion partners (at most 256)", c->xs_maxdeg);
+    }
+    if ((r = upload(c, &c->d_xs_row, xrow))) return r;
+    if ((r = upload(c, &c->d_xs_part, xpart))) return r;
+  }
+  // the table (ExclusionTable::myTable): the first add with class != NONE and
+  // a != b of every pair, in add order (mySet orders pairs by (a, b) only,
+  // topology/ExclusionTable.cpp:20-28,112-121)
   c->tab_a.clear(); c->tab_b.clear(); c->tab_c.clear();
   {
-    std::set<std::tuple<int, int, int>> seen;
+    std::set<std::pair<int, int>> seen;
     for (int k = 0; k < nexcl; k++) {
       int a = std::min(ex_a[k], ex_b[k]), b = std::max(ex_a[k], ex_b[k]);
       if (ex_class[k] == PM_EXCL_NONE || a == b) continue;
-      auto t = std::make_tuple(a, b, ex_class[k]);
+      auto t = std::make_pair(a, b);
       if (seen.count(t)) continue;
       seen.insert(t);
       c->tab_a.push_back(a); c->tab_b.push_back(b); c->tab_c.push_back(ex_class[k]);
@@ -1350,10 +1407,10 @@ int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm
   hipStream_t s = c->stream;
   HIPCHECK(c, hipMemcpyAsync(c->d_pos, positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
   pm_launch_zero_i(s, c->d_cell_count, nc);
-  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_key);
-  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_cell_cursor, c->d_counts,
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key);
+  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
                        c->d_cluster_cell, c->ncl_max);
-  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_cursor, c->d_cell_atoms);
+  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms);
   std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);
   HIPCHECK(c, hipMemcpyAsync(cell.data(), c->d_cell_of, n * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(c, hipMemcpyAsync(cnt.data(), c->d_cell_count, nc * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1481,6 +1538,9 @@ int pm_md_get_state(pm_context* c, double* x, double* v, double* energies) {
 
 int pm_last_stats(pm_context* c, long long* stats, int nstats) {
   if (!c || !stats) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if ((r = list_stats(c))) return r;
   for (int k = 0; k < nstats && k < 4; k++) stats[k] = c->last_stats[k];
   return PM_OK;
 }

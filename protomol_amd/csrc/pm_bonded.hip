@@ -155,6 +155,6 @@ void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, 
   *nrec_angle = nab;
   if (a.n > 0 && (a.do_bond || a.do_angle))
     hipLaunchKernelGGL(k_bonded_force, dim3((a.n + 255) / 256), dim3(256), 0, s, box, a);
-  if (nbb + nab > 0)
+  if (nbb + nab > 0 && a.want_energy)
     hipLaunchKernelGGL(k_bonded_energy, dim3(nbb + nab), dim3(256), 0, s, box, a, nbb);
 }

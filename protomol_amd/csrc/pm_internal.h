@@ -7,8 +7,8 @@
 // Fixed-point scales.  Forces: 2^32 units per kcal/mol/A in int64 (range
 // +-2^31 kcal/mol/A, resolution 2.3e-10).  PME charge grid: 2^40 units per
 // scaled charge (range +-2^23, resolution 9e-13).
-#define PM_FORCE_FIX 4294967296.0
-#define PM_FORCE_UNFIX (1.0 / 4294967296.0)
+#define PM_FORCE_FIX 1099511627776.0
+#define PM_FORCE_UNFIX (1.0 / 1099511627776.0)
 #define PM_GRID_FIX 1099511627776.0
 #define PM_GRID_UNFIX (1.0 / 1099511627776.0)
 
@@ -95,6 +95,11 @@ struct PmNbLaunch {
   int* overflow;
   int bbox_filter;
   int max_delta;             // largest |a-b| over exclusion pairs
+  // exclusions: resolved classes of both atoms of every pair, partner | class << 24
+  const int* xs_row;
+  const int* xs_part;
+  int xs_tcap;               // LDS entries per cluster: PM_M * largest partner count
+  unsigned short* jmask;     // per list entry (flagged entries): FULL bit ii, MODIFIED bit 8 + ii
   // topology
   const int* molecule;
   const int* ex_row;
@@ -105,6 +110,7 @@ struct PmNbLaunch {
   const double4* lj_d;
   // outputs
   long long* force_fix;      // [3n] fixed point, accumulated (+=) for every real atom
+  long long* force_slot;     // [3 * slots] fixed point per slot (pair kernel), folded into force_fix
   double* epart;             // [ncl_max * PM_NE]
   int want_virial;
   int want_energy;           // write the per-cluster energy records
@@ -113,16 +119,19 @@ struct PmNbLaunch {
 // ---------------------------------------------------------------- launchers
 // Binning (reference-exact) and slot layout.
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
-                   int* cell_count, float* key);
+                   int* cell_count, int* rank, float* key);
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
-                          int* cell_slot, int* cell_cursor, int* counts, int* cluster_cell,
-                          int ncl_max);
-void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, int* cell_cursor, int* cell_atoms);
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max);
+void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
+                       int* cell_atoms);
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
                           const double* q, const int* type, const int* cell_count,
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
                           int* atom_of_slot, int dummy_type, int dummy_slot);
+// force_fix[atom] += force_slot[slot] for every real slot
+void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
+                          const long long* force_slot, long long* force_fix);
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
 void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p,
@@ -189,6 +198,8 @@ int pm_reduce_scratch_records(int nparts);
 void pm_launch_zero_ll(hipStream_t s, long long* p, long long count);
 void pm_launch_zero_i(hipStream_t s, int* p, long long count);
 void pm_launch_zero_d(hipStream_t s, double* p, long long count);
+void pm_launch_zero_force_cells(hipStream_t s, long long* f, long long nf, long long* fs, long long nfs, int* cc,
+                                int nc);
 void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
@@ -213,5 +224,6 @@ struct PmBondedLaunch {
   long long* force_fix;
   double* epart;        // records: bond blocks then angle blocks
   int want_virial;
+  int want_energy;      // write the energy records (skipped on steps that report none)
 };
 void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, int* nrec_bond, int* nrec_angle);

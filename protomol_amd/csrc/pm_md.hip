@@ -128,15 +128,31 @@ __global__ __launch_bounds__(256) void k_reduce_energy1(const double* __restrict
   }
 }
 
-__global__ void k_reduce_energy2(const double* __restrict__ partial, int nblocks, double const_coul,
-                                 double const_vir_diag, double* __restrict__ out, int overwrite) {
-  const int c = threadIdx.x;
-  if (c >= PM_NE) return;
-  double v = 0;
-  for (int b = 0; b < nblocks; b++) v += partial[b * PM_NE + c];
-  if (c == PM_E_COUL) v += const_coul;
-  if (c == PM_E_VIR + 0 || c == PM_E_VIR + 4 || c == PM_E_VIR + 8) v += const_vir_diag;
-  out[c] = overwrite ? v : out[c] + v;
+// stage 2: 16 record groups per component summed with independent loads,
+// then combined in a fixed order (nblocks <= PM_RED_BLOCKS)
+__global__ __launch_bounds__(256) void k_reduce_energy2(const double* __restrict__ partial, int nblocks,
+                                                        double const_coul, double const_vir_diag,
+                                                        double* __restrict__ out, int overwrite) {
+  __shared__ double s[256];
+  const int c = threadIdx.x % PM_NE, g = threadIdx.x / PM_NE;
+  double v[PM_RED_BLOCKS / 16];
+#pragma unroll
+  for (int k = 0; k < PM_RED_BLOCKS / 16; k++) {
+    const int b = g + 16 * k;
+    v[k] = b < nblocks ? partial[b * PM_NE + c] : 0.0;
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < PM_RED_BLOCKS / 16; k++) acc += v[k];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < PM_NE) {
+    double t = 0.0;
+    for (int u = 0; u < 16; u++) t += s[u * PM_NE + threadIdx.x];
+    if (c == PM_E_COUL) t += const_coul;
+    if (c == PM_E_VIR + 0 || c == PM_E_VIR + 4 || c == PM_E_VIR + 8) t += const_vir_diag;
+    out[c] = overwrite ? t : out[c] + t;
+  }
 }
 
 __global__ void k_zero_ll(long long* p, long long count) {
@@ -146,6 +162,13 @@ __global__ void k_zero_ll(long long* p, long long count) {
 __global__ void k_zero_i(int* p, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
+}
+// clears the fixed-point forces and the cell counts of one evaluation in one launch
+__global__ void k_zero_force_cells(long long* f, long long nf, long long* fs, long long nfs, int* cc, int nc) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nf) f[i] = 0;
+  if (i < nfs) fs[i] = 0;
+  if (i < nc) cc[i] = 0;
 }
 __global__ void k_zero_d(double* p, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,7 +270,7 @@ void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, dou
   int chunk = (nparts + nb - 1) / nb;
   if (chunk < 1) chunk = 1;
   hipLaunchKernelGGL(k_reduce_energy1, dim3(nb), dim3(256), 0, s, parts, nparts, chunk, scratch);
-  hipLaunchKernelGGL(k_reduce_energy2, dim3(1), dim3(64), 0, s, scratch, nb, constant_coul, constant_vir_diag, out,
+  hipLaunchKernelGGL(k_reduce_energy2, dim3(1), dim3(256), 0, s, scratch, nb, constant_coul, constant_vir_diag, out,
                      overwrite);
 }
 
@@ -256,6 +279,12 @@ void pm_launch_zero_ll(hipStream_t s, long long* p, long long count) {
 }
 void pm_launch_zero_i(hipStream_t s, int* p, long long count) {
   if (count > 0) hipLaunchKernelGGL(k_zero_i, grid_for(count, 256), dim3(256), 0, s, p, count);
+}
+void pm_launch_zero_force_cells(hipStream_t s, long long* f, long long nf, long long* fs, long long nfs, int* cc,
+                                int nc) {
+  long long m = nf > nc ? nf : nc;
+  if (nfs > m) m = nfs;
+  if (m > 0) hipLaunchKernelGGL(k_zero_force_cells, grid_for(m, 256), dim3(256), 0, s, f, nf, fs, nfs, cc, nc);
 }
 void pm_launch_zero_d(hipStream_t s, double* p, long long count) {
   if (count > 0) hipLaunchKernelGGL(k_zero_d, grid_for(count, 256), dim3(256), 0, s, p, count);

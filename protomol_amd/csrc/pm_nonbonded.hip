@@ -3,10 +3,9 @@
 //
 // Binning reproduces topology/buildCellLists.cpp:11-38 bit for bit (FP64,
 // no contraction); the pair kernels evaluate OneAtomPair / OneAtomPairTwo
-// (forces/OneAtomPair.h:127-188, forces/OneAtomPairTwo.h:131-199) for every
-// pair within the switching cutoff, each pair from both sides, so that every
-// atom's force is the sum of one lane group in a fixed order (no atomics,
-// deterministic run to run).
+// (forces/OneAtomPair.h:127-188, forces/OneAtomPairTwo.h:131-199) once for
+// every unordered pair within the switching cutoff and add both atoms' forces
+// as 64-bit fixed-point integers (order-independent, deterministic run to run).
 #include "pm_internal.h"
 
 #include <math.h>
@@ -54,54 +53,88 @@ __device__ inline int cell_index(const PmBox& b, double px, double py, double pz
 }
 
 __global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
-                      int* __restrict__ cell_count, float* __restrict__ key) {
+                      int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double mz;
   int c = cell_index(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &mz);
   cell_of[i] = c;
   key[i] = (float)mz;
-  atomicAdd(&cell_count[c], 1);
+  rank[i] = atomicAdd(&cell_count[c], 1);   // arrival rank; k_fill_slots orders by (z, index)
 }
 
 // Exclusive scans of the cell counts (atoms and M-padded slots) in one
-// workgroup; also assigns every cluster its cell.
+// workgroup, 8192 cells per pass staged in LDS; also assigns every cluster
+// its cell.
 __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
                                                      int* __restrict__ start, int* __restrict__ slot,
-                                                     int* __restrict__ cursor, int* __restrict__ counts,
-                                                     int* __restrict__ cluster_cell, int ncl_max) {
-  __shared__ int s_a[1024], s_s[1024];
-  int t = threadIdx.x;
-  int per = (ncells + 1023) / 1024;
-  int lo = t * per, hi = min(ncells, lo + per);
-  int sa = 0, ss = 0;
-  for (int c = lo; c < hi; c++) { sa += cnt[c]; ss += (cnt[c] + PM_M - 1) / PM_M * PM_M; }
-  s_a[t] = sa; s_s[t] = ss;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int va = t >= off ? s_a[t - off] : 0;
-    int vs = t >= off ? s_s[t - off] : 0;
+                                                     int* __restrict__ counts, int* __restrict__ cluster_cell,
+                                                     int ncl_max) {
+  __shared__ int s_c[8192];
+  __shared__ int s_wa[16], s_ws[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int carry_a = 0, carry_s = 0;
+  for (int base = 0; base < ncells; base += 8192) {
+    const int m = min(8192, ncells - base);
+    {
+      // eight independent loads in flight per thread (clamped, then masked)
+      int tmp[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) tmp[k] = cnt[base + min(t + 1024 * k, m - 1)];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (t + 1024 * k < m) s_c[t + 1024 * k] = tmp[k];
+    }
     __syncthreads();
-    s_a[t] += va; s_s[t] += vs;
+    int v[8];
+    int sa = 0, ss = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int idx = 8 * t + k;
+      v[k] = idx < m ? s_c[idx] : 0;
+      sa += v[k];
+      ss += (v[k] + PM_M - 1) / PM_M * PM_M;
+    }
+    // inclusive wave scans, then the 16 wave totals
+    int ia = sa, is = ss;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int ta = __shfl_up(ia, off, 64), ts = __shfl_up(is, off, 64);
+      if (lane >= off) { ia += ta; is += ts; }
+    }
+    if (lane == 63) { s_wa[w] = ia; s_ws[w] = is; }
+    __syncthreads();
+    int wa = 0, ws = 0, ta = 0, ts = 0;
+    for (int u = 0; u < 16; u++) {
+      if (u < w) { wa += s_wa[u]; ws += s_ws[u]; }
+      ta += s_wa[u]; ts += s_ws[u];
+    }
+    int ra = carry_a + wa + ia - sa, rs = carry_s + ws + is - ss;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int idx = 8 * t + k;
+      if (idx < m) {
+        const int c = base + idx;
+        const int ps = (v[k] + PM_M - 1) / PM_M * PM_M;
+        start[c] = ra;
+        slot[c] = rs;
+        for (int u = rs / PM_M; u < (rs + ps) / PM_M && u < ncl_max; u++) cluster_cell[u] = c;
+        ra += v[k];
+        rs += ps;
+      }
+    }
+    carry_a += ta;
+    carry_s += ts;
     __syncthreads();
   }
-  int ra = s_a[t] - sa, rs = s_s[t] - ss;
-  for (int c = lo; c < hi; c++) {
-    int k = cnt[c];
-    int ps = (k + PM_M - 1) / PM_M * PM_M;
-    start[c] = ra; cursor[c] = ra; slot[c] = rs;
-    for (int u = rs / PM_M; u < (rs + ps) / PM_M && u < ncl_max; u++) cluster_cell[u] = c;
-    ra += k; rs += ps;
-  }
-  if (t == 1023) { counts[0] = s_s[1023]; counts[1] = s_s[1023] / PM_M; counts[2] = s_a[1023]; }
+  if (t == 0) { counts[0] = carry_s; counts[1] = carry_s / PM_M; counts[2] = carry_a; }
 }
 
-__global__ void k_scatter(int n, const int* __restrict__ cell_of, int* __restrict__ cursor,
-                          int* __restrict__ cell_atoms) {
+__global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
+                          const int* __restrict__ cell_start, int* __restrict__ cell_atoms) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int p = atomicAdd(&cursor[cell_of[i]], 1);
-  cell_atoms[p] = i;
+  cell_atoms[cell_start[cell_of[i]] + rank[i]] = i;
 }
 
 __device__ inline unsigned int to_fix(double x, double s) {
@@ -237,27 +270,31 @@ __device__ inline int wrap_cell(int c, int d, int n) {
 }
 
 // ------------------------------------------------------------- pair lists
-// One wave per i-cluster: every atom of the cells in the full stencil whose
-// distance to the cluster's bounding box is below the list cutoff is appended
-// to the cluster's list (bit 31 marks a possible exclusion or self pair).
+// One wave per i-cluster: every atom of the stencil cells that follow the
+// cluster's cell (and of its own cell from the cluster on) whose distance to
+// the cluster's bounding box is below the list cutoff is appended to the
+// cluster's list.  Every unordered pair within the cutoff is thus listed
+// exactly once.  Bit 31 marks an entry that some cluster atom excludes or
+// modifies; its masks (FULL bit ii, MODIFIED bit 8 + ii) go to jmask.
 __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) {
   extern __shared__ int s_dyn[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nst = a.nstencil;
-  int* s_start = s_dyn + wave * (3 * nst + 1);
+  int* s_start = s_dyn + wave * (3 * nst + 1 + a.xs_tcap);
   int* s_cnt = s_start + nst;
   int* s_pref = s_cnt + nst;     // nst + 1 entries
+  int* s_tri = s_pref + nst + 1; // xs_tcap entries: partner | ii << 24 | class << 28
   const int cl = blockIdx.x * 4 + wave;
   const int ncl = a.counts[1];
   const bool active = cl < ncl;  // every wave reaches every barrier below
   const int cell = active ? a.cluster_cell[cl] : 0;
   const int cz = cell % box.n[2], cy = (cell / box.n[2]) % box.n[1], cx = cell / (box.n[1] * box.n[2]);
 
-  // i-cluster: anchor, bounding box and index range
+  // i-cluster: anchor, bounding box and exclusion partners
   const int4 anchor = active ? a.slot_pos[cl * PM_M] : make_int4(0, 0, 0, 0);
   const float fx = (float)box.fix_to_A[0], fy = (float)box.fix_to_A[1], fz = (float)box.fix_to_A[2];
   float lox = 1e30f, loy = 1e30f, loz = 1e30f, hix = -1e30f, hiy = -1e30f, hiz = -1e30f;
-  int omin = 0x7fffffff, omax = -1;
+  int xbeg = 0, deg = 0;
   if (active && lane < PM_M) {
     int4 p = a.slot_pos[cl * PM_M + lane];
     int m = a.slot_meta[cl * PM_M + lane];
@@ -267,25 +304,41 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
       float dy = (float)(int)(p.y - anchor.y) * fy;
       float dz = (float)(int)(p.z - anchor.z) * fz;
       lox = hix = dx; loy = hiy = dy; loz = hiz = dz;
-      omin = omax = o;
+      xbeg = a.xs_row[o];
+      deg = a.xs_row[o + 1] - xbeg;
     }
   }
   lox = wave_min_m(lox); loy = wave_min_m(loy); loz = wave_min_m(loz);
   hix = wave_max_m(hix); hiy = wave_max_m(hiy); hiz = wave_max_m(hiz);
-  omin = wave_imin_m(omin); omax = wave_imax_m(omax);
   lox = __shfl(lox, 0, 64); loy = __shfl(loy, 0, 64); loz = __shfl(loz, 0, 64);
   hix = __shfl(hix, 0, 64); hiy = __shfl(hiy, 0, 64); hiz = __shfl(hiz, 0, 64);
-  omin = __shfl(omin, 0, 64); omax = __shfl(omax, 0, 64);
-  const int flo = omin - a.max_delta, fhi = omax + a.max_delta;
+  // partners of the cluster's atoms into LDS (deg <= xs_tcap / PM_M per atom)
+  const int tinc = wave_incl_scan(deg, lane);
+  const int ntri = __shfl(tinc, 63, 64);
+  int pmin = 0x7fffffff, pmax = -1;
+  for (int u = 0; u < deg; u++) {
+    const int w = a.xs_part[xbeg + u];
+    const int o = w & 0xffffff;
+    s_tri[tinc - deg + u] = o | (lane << 24) | ((w >> 24) << 28);
+    pmin = min(pmin, o);
+    pmax = max(pmax, o);
+  }
+  pmin = __shfl(wave_imin_m(pmin), 0, 64);
+  pmax = __shfl(wave_imax_m(pmax), 0, 64);
 
-  // stencil cells and their prefix counts
+  // stencil cells and their prefix counts: half list, so only cells after
+  // this one in linear order, and this cell's atoms from this cluster on
+  // (the pair kernel keeps a pair only when the j slot follows the i slot)
   if (active) {
     for (int s = lane; s < nst; s += 64) {
       int4 d = a.stencil[s];
       int nb = (wrap_cell(cx, d.x, box.n[0]) * box.n[1] + wrap_cell(cy, d.y, box.n[1])) * box.n[2] +
                wrap_cell(cz, d.z, box.n[2]);
-      s_start[s] = a.cell_slot[nb];
-      s_cnt[s] = a.cell_count[nb];
+      int st = a.cell_slot[nb], ct = a.cell_count[nb];
+      if (nb < cell) ct = 0;
+      else if (nb == cell) { ct = st + ct - cl * PM_M; st = cl * PM_M; }
+      s_start[s] = st;
+      s_cnt[s] = ct;
     }
   }
   __syncthreads();
@@ -303,6 +356,7 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
   const int total = carry;
 
   int* out = a.jlist + (long long)cl * a.lcap;
+  unsigned short* outm = a.jmask + (long long)cl * a.lcap;
   int count = 0;
   // candidates beyond PM_CAND_FACTOR * lcap mean a degenerate cell layout:
   // stop scanning and report it as a list overflow
@@ -311,14 +365,14 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
   for (int k0 = 0; k0 < scan; k0 += 64) {
     int k = k0 + lane;
     bool pass = false;
-    int entry = 0;
+    int sj = 0, o = 0;
     if (k < scan) {
       int lo = 0, hi = nst;  // largest s with s_pref[s] <= k
       while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if (s_pref[mid] <= k) lo = mid; else hi = mid;
       }
-      int sj = s_start[lo] + (k - s_pref[lo]);
+      sj = s_start[lo] + (k - s_pref[lo]);
       int4 p = a.slot_pos[sj];
       pass = true;
       if (a.bbox_filter) {
@@ -330,14 +384,26 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
         float gz = fmaxf(0.f, fmaxf(loz - dz, dz - hiz));
         pass = gx * gx + gy * gy + gz * gz < cut2_list;
       }
-      int o = (a.slot_meta[sj] >> 8) & 0x7fffff;
-      bool flag = (o >= flo) && (o <= fhi);
-      entry = sj | (flag ? (int)0x80000000 : 0);
+      o = (a.slot_meta[sj] >> 8) & 0x7fffff;
+    }
+    // exclusion masks of the entry against the cluster's atoms
+    int xm = 0;
+    const bool pre = pass && o >= pmin && o <= pmax;
+    if (__ballot(pre) != 0ull) {
+      if (pre) {
+        for (int t = 0; t < ntri; t++) {
+          const int w = s_tri[t];
+          if ((w & 0xffffff) == o) xm |= ((w >> 28) == PM_EXCL_FULL ? 1 : 256) << ((w >> 24) & 7);
+        }
+      }
     }
     unsigned long long bal = __ballot(pass);
     int before = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
     int idx = count + before;
-    if (pass && idx < a.lcap) out[idx] = entry;
+    if (pass && idx < a.lcap) {
+      out[idx] = sj | (xm != 0 ? (int)0x80000000 : 0);
+      if (xm != 0) outm[idx] = (unsigned short)xm;
+    }
     count += __popcll(bal);
   }
   if (lane == 0) {
@@ -352,10 +418,18 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
   __shared__ unsigned long long s0[256], s1[256];
   const int ncl = counts[1];
   unsigned long long e = 0, l = 0;
-  for (int c = threadIdx.x; c < ncl; c += 256) {
-    const int v = jcount[c];
-    e += v;
-    l += (unsigned long long)((v + PM_JW - 1) / PM_JW) * 64;
+  for (int c0 = 0; c0 < ncl; c0 += 256 * 8) {
+    int v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int c = c0 + 256 * k + threadIdx.x;
+      v[k] = c < ncl ? jcount[c] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      e += v[k];
+      l += (unsigned long long)((v[k] + PM_JW - 1) / PM_JW) * 64;
+    }
   }
   s0[threadIdx.x] = e;
   s1[threadIdx.x] = l;
@@ -365,6 +439,18 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
     __syncthreads();
   }
   if (threadIdx.x == 0) { out[0] = s0[0]; out[1] = s1[0]; }
+}
+
+// slot forces of the pair kernels into the per-atom fixed-point forces
+__global__ void k_fold_slots(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= counts[0]) return;
+  const int at = atom_of_slot[sl];
+  if (at < 0) return;
+  force_fix[3LL * at] += force_slot[3LL * sl];
+  force_fix[3LL * at + 1] += force_slot[3LL * sl + 1];
+  force_fix[3LL * at + 2] += force_slot[3LL * sl + 2];
 }
 
 // ------------------------------------------------------------ pair kernel
@@ -406,16 +492,52 @@ __device__ inline void switching(T r2, T dist, T cut2, T on2, T sw1, T sw2, T sw
   }
 }
 
-__device__ inline int excl_class(const PmNbLaunch& a, int oi, int oj) {
-  if (oi == oj) return PM_EXCL_FULL;
-  if (a.molecule[oi] != a.molecule[oj]) return PM_EXCL_NONE;   // OneAtomPair.h:149-153
-  int lo = min(oi, oj), hi = max(oi, oj);
-  int r = PM_EXCL_NONE;
-  for (int u = a.ex_row[lo]; u < a.ex_row[lo + 1]; u++)
-    if (a.ex_partner[u] == hi) r = a.ex_cls[u];   // last add wins (ExclusionTable::add)
-  return r;
+
+// Sum over the 8 lanes of a row (lanes 8r .. 8r+7) with DPP moves: quad
+// xor 1, quad xor 2, half-row mirror.  Every lane of the row ends with the
+// same value, added in the same order on every run.
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+  // old = 0 (the identity of the sum): the move folds into v_add_f32_dpp
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ inline double dpp_f(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ inline T row8_sum(T v) {
+  v += dpp_f<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);   // row_half_mirror
+  return v;
 }
 
+// 64-bit fixed point (2^40 per kcal/mol/A) of one pair-force component:
+// integer part of v * 2^8 and a 32-bit fraction, so no FP64 is needed in the
+// FP32 kernel (|v| < 2^23 kcal/mol/A).
+__device__ inline long long force_fix_of(float v) {
+  const float w = v * 256.0f;
+  const float fl = __builtin_floorf(w);
+  const int hi = (int)fl;
+  const float fr = __builtin_fminf(w - fl, 0.99999994f);
+  const unsigned int lo = (unsigned int)(fr * 4294967296.0f);
+  return (long long)(((unsigned long long)(unsigned int)hi << 32) | lo);
+}
+__device__ inline long long force_fix_of(double v) { return __double2ll_rn(v * PM_FORCE_FIX); }
+
+// Half-list pair kernel: one wave per i-cluster, lane = jslot * 8 + ii, so a
+// wave evaluates 8 list entries against the 8 cluster atoms per iteration.
+// A pair is kept when the j slot follows the i slot (every unordered pair is
+// listed once, by the cluster of its lower slot).  The i force accumulates
+// per lane in FP64 and is reduced over the j slots at the end; the j force of
+// each entry is summed over its row of 8 lanes with DPP and added with one
+// 64-bit integer atomic per component.  Forces accumulate per slot, so the 24
+// atomics of one instruction fall on a few contiguous cache lines; integer
+// addition is associative, so the result does not depend on their order.
 template <typename T, int LJSWF, int CK, int CSWF>
 __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) {
   extern __shared__ unsigned char s_lj_raw[];
@@ -466,6 +588,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
   const int* lst = a.jlist + (long long)cl * a.lcap;
   const int cnt = a.jcount[cl];
   const bool vir = a.want_virial != 0;
+  long long* __restrict__ ff = a.force_slot;
 
   for (int base = 0; base < cnt; base += PM_JW) {
     const int k = base + js;
@@ -476,6 +599,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     P4 pj;
     if constexpr (sizeof(T) == 4) pj = a.slot_pos[sj]; else pj = a.slot_pos64[sj];
     const int mj = a.slot_meta[sj];
+    const int oj = (mj >> 8) & 0x7fffff;
     T dx, dy, dz;
     if constexpr (sizeof(T) == 4) {
       dx = (T)(int)(pj.x - pi.x) * sx;
@@ -487,12 +611,13 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
       dz = (T)(long long)((unsigned long long)pj.z - (unsigned long long)pi.z) * sz;
     }
     const T r2 = dx * dx + dy * dy + dz * dz;
-    bool mask = valid && ireal && (r2 <= cut2);
+    bool mask = valid && ireal && sj > si && (r2 <= cut2);
     int excl = PM_EXCL_NONE;
     if (__ballot(flagged && mask) != 0ull) {
       if (flagged && mask) {
-        excl = excl_class(a, oi, (mj >> 8) & 0x7fffff);
-        if (excl == PM_EXCL_FULL) mask = false;
+        const int xm = a.jmask[(long long)cl * a.lcap + k];
+        if ((xm >> ii) & 1) { excl = PM_EXCL_FULL; mask = false; }
+        else if ((xm >> (8 + ii)) & 1) excl = PM_EXCL_MODIFIED;
       }
     }
     T qj;
@@ -553,6 +678,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     F = mask ? F : T(0);
     E1 = mask ? E1 : T(0);
     E2 = mask ? E2 : T(0);
+    // fij = F * (xj - xi): -fij on i, +fij on j (OneAtomPair.h:178-181)
     const T fx_ = F * dx, fy_ = F * dy, fz_ = F * dz;
     fxa -= (double)fx_;
     fya -= (double)fy_;
@@ -563,21 +689,26 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
       vxx += fx_ * dx; vxy += fx_ * dy; vxz += fx_ * dz;
       vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
     }
+    // j force of this row: lanes ii = 0, 1, 2 add its x, y, z component
+    const T jx = row8_sum(fx_), jy = row8_sum(fy_), jz = row8_sum(fz_);
+    const T jv = ii == 0 ? jx : (ii == 1 ? jy : jz);
+    if (ii < 3 && jv != T(0))
+      atomicAdd((unsigned long long*)(ff + 3LL * sj + ii), (unsigned long long)force_fix_of(jv));
   }
-  // reduce the i-force over the j-slots (lanes ii, ii+M, ...)
+  // reduce the i force over the j slots (lanes ii, ii+8, ...); lanes
+  // js = 0, 1, 2 then add its x, y, z component
 #pragma unroll
   for (int off = PM_M; off < 64; off <<= 1) {
     fxa += __shfl_xor(fxa, off, 64);
     fya += __shfl_xor(fya, off, 64);
     fza += __shfl_xor(fza, off, 64);
   }
-  if (js == 0 && ireal) {
-    a.force_fix[3 * oi] += __double2ll_rn(fxa * PM_FORCE_FIX);
-    a.force_fix[3 * oi + 1] += __double2ll_rn(fya * PM_FORCE_FIX);
-    a.force_fix[3 * oi + 2] += __double2ll_rn(fza * PM_FORCE_FIX);
+  if (js < 3 && ireal) {
+    const double iv = js == 0 ? fxa : (js == 1 ? fya : fza);
+    atomicAdd((unsigned long long*)(ff + 3LL * si + js), (unsigned long long)__double2ll_rn(iv * PM_FORCE_FIX));
   }
   if (!a.want_energy) return;
-  // energies and virial (each pair was seen from both sides: factor 1/2)
+  // energies and virial of the cluster's pairs (each pair counted once)
   double r[8] = {ireal ? (double)ec : 0.0, ireal ? (double)elj : 0.0,
                  ireal ? (double)vxx : 0.0, ireal ? (double)vxy : 0.0, ireal ? (double)vxz : 0.0,
                  ireal ? (double)vyy : 0.0, ireal ? (double)vyz : 0.0, ireal ? (double)vzz : 0.0};
@@ -586,7 +717,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     double v = r[u];
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-    r[u] = v * 0.5;
+    r[u] = v;
   }
   if (lane == 0) {
     double* ep = a.epart + (long long)cl * PM_NE;
@@ -689,20 +820,22 @@ __global__ void k_diag(PmBox box, const double* __restrict__ pos, const int* __r
 
 // ----------------------------------------------------------------- launchers
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
-                   int* cell_count, float* key) {
+                   int* cell_count, int* rank, float* key) {
   if (n > 0)
-    hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, cell_of, cell_count, key);
+    hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, cell_of, cell_count, rank, key);
 }
 
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
-                          int* cell_slot, int* cell_cursor, int* counts, int* cluster_cell, int ncl_max) {
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max) {
   hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
-                     cell_cursor, counts, cluster_cell, ncl_max);
+                     counts, cluster_cell, ncl_max);
 }
 
-void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, int* cell_cursor, int* cell_atoms) {
+void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
+                       int* cell_atoms) {
   if (n > 0)
-    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell_of, cell_cursor, cell_atoms);
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell_of, rank, cell_start,
+                       cell_atoms);
 }
 
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
@@ -716,6 +849,13 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
                      atom_of_slot, dummy_type, dummy_slot);
 }
 
+void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
+                          const long long* force_slot, long long* force_fix) {
+  if (nslot_max > 0)
+    hipLaunchKernelGGL(k_fold_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, counts, atom_of_slot,
+                       force_slot, force_fix);
+}
+
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out) {
   hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out);
 }
@@ -723,7 +863,7 @@ void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, u
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   double cut = sqrt(p.cut2);
   float c = (float)(cut * (1.0 + 1e-5) + 1e-4);
-  size_t lds = 4 * (3 * a.nstencil + 1) * sizeof(int);
+  size_t lds = 4 * (3 * a.nstencil + 1 + a.xs_tcap) * sizeof(int);
   hipLaunchKernelGGL(k_list, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
 }
 

@@ -6,7 +6,7 @@ import pytest
 
 from oracle import pyoracle as po
 from protomol_amd import water
-from tests.cases import ALPHA_RC8, ALPHA_RC10, c1_terms, gpu_forces_c1, rms_rel
+from tests.cases import ALPHA_RC8, ALPHA_RC10, c1_terms, exclusion_mix, gpu_forces_c1, rms_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -83,6 +83,27 @@ def test_cutoff_variants(gpu_ctx_factory, port, fid, vals, terms, precision, tol
     say("rms rel force", rms_rel(f, o["forces"]))
     assert rms_rel(f, o["forces"]) < tol
     np.testing.assert_allclose(e[:2], o["energies"][:2], rtol=tol * 10, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-11)])
+def test_exclusion_rules(gpu_ctx_factory, port, precision, tol):
+    """MODIFIED (A14/B14, csf) and NONE overrides, duplicate and cross-molecule
+    exclusion pairs, through the two-term cutoff kernels and the PME correction."""
+    say(f"case: exclusion rules water648 {precision}")
+    s = exclusion_mix(water.named("water648", seed=8, jitter=0.4))
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    for fid, vals, terms in VARIANTS[5:]:
+        f, e = c.evaluate([c.force(fid, vals)], s.pos, virial=True)
+        o = port.cutoff(s, terms, vals[-1], 4.0, virial=True)
+        say(fid, "rms rel force", rms_rel(f, o["forces"]))
+        assert rms_rel(f, o["forces"]) < tol
+        np.testing.assert_allclose(e[:2], o["energies"][:2], rtol=tol * 10, atol=1e-9)
+    h = c.force("Coulomb -algorithm PMEwald -correction -interpolation BSpline", [24, 24, 24, 8.0, 4, 1e-6, -1.0])
+    f, e = c.evaluate([h], s.pos, virial=True)
+    o = port.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=4, virial=True)
+    say("correction rms rel force", rms_rel(f, o["forces"]))
+    assert rms_rel(f, o["forces"]) < 1e-11
+    np.testing.assert_allclose(e[0], o["energies"][0], rtol=1e-11)
 
 
 @pytest.mark.parametrize("grid,order", [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8)])

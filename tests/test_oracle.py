@@ -15,6 +15,7 @@ import pytest
 
 from oracle import pyoracle as po
 from protomol_amd import water
+from tests.cases import exclusion_mix
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden", "water648_reference.npz")
@@ -114,6 +115,21 @@ def test_reference_small_box_semantics(port, ref):
             b = ref.cutoff(s, terms, 8.0, cellsize, counts=True)
             np.testing.assert_array_equal(a["forces"], b["forces"])
             np.testing.assert_array_equal(a["cand"], b["cand"])
+
+
+def test_reference_exclusion_rules(port, ref):
+    """MODIFIED / NONE overrides, duplicates and cross-molecule pairs."""
+    s = exclusion_mix(water.named("water648", seed=24, jitter=0.3))
+    for name in ("lj_c2", "coul_c1", "lj_ewr"):
+        a = port.cutoff(s, CASES[name], 8.0, 4.0, virial=True, counts=True)
+        b = ref.cutoff(s, CASES[name], 8.0, 4.0, virial=True, counts=True)
+        np.testing.assert_array_equal(a["forces"], b["forces"])
+        np.testing.assert_array_equal(a["energies"], b["energies"])
+        np.testing.assert_array_equal(a["inrc"], b["inrc"])
+    a = port.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=4, virial=True)
+    b = ref.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=4, virial=True)
+    np.testing.assert_allclose(a["forces"], b["forces"], rtol=0, atol=1e-12 * np.abs(b["forces"]).max())
+    np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-12, atol=1e-9)
 
 
 def test_reference_pme(port, ref):
