@@ -169,6 +169,15 @@ int pm_last_stats(pm_context* ctx, long long* stats, int nstats);
 int pm_set_timing(pm_context* ctx, unsigned mask);
 int pm_get_timing(pm_context* ctx, double* ms, long long* count, int ncategories);
 
+/* Layout and kernel options, chosen per context (no reference counterpart:
+ * they change speed, never results beyond the mixed-precision tolerance):
+ *   PM_OPT_CLUSTER_SIZE  atoms per i-cluster of the pair kernels: 8, 16, or
+ *                        0 for the default (8).
+ * pm_get_option returns the value in effect (after the automatic choice). */
+enum { PM_OPT_CLUSTER_SIZE = 0 };
+int pm_set_option(pm_context* ctx, int option, int value);
+int pm_get_option(pm_context* ctx, int option, int* value);
+
 /* Wait for all work queued by this context. */
 int pm_synchronize(pm_context* ctx);
 

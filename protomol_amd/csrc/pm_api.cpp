@@ -114,6 +114,9 @@ struct pm_context {
   long long* d_force_slot = nullptr;   // [3 * (nslot_max + 1)] pair-kernel forces per slot
   int ncl_max = 0, nslot_max = 0;
   int cells_alloc = 0;
+  int m = 8;              // atoms per cluster of the current layout
+  int cells_m = 0;        // cluster size the slot buffers were sized for
+  int m_override = 0;     // 8 or 16 to force the cluster size (0: from the occupancy)
 
   // stencils keyed by algorithm cutoff
   std::map<double, std::pair<int4*, int>> stencils;
@@ -121,7 +124,7 @@ struct pm_context {
 
   // lists
   int *d_jlist = nullptr, *d_jcount = nullptr, *d_overflow = nullptr;
-  unsigned short* d_jmask = nullptr;   // per list entry: FULL / MODIFIED masks of flagged entries
+  unsigned int* d_jmask = nullptr;     // per list entry: FULL / MODIFIED masks of flagged entries
   long long jlist_alloc = 0;
   int lcap = 0;
   unsigned long long* d_stats = nullptr;
@@ -339,8 +342,17 @@ void free_stencils(pm_context* c) {
 }
 
 // ------------------------------------------------------------ allocation
+// atoms per cluster: 8 unless set (16 halves the list builds and the j-force
+// atomics per lane but widens the clusters' bounding boxes: on the 98k water
+// box, 5 A cells, it issued 29 % more pair lanes and ran 25 % slower)
+int choose_m(const pm_context* c) {
+  if (c->m_override == 8 || c->m_override == 16) return c->m_override;
+  return 8;
+}
+
 int ensure_cells(pm_context* c) {
-  if (c->cells_alloc == c->ncells && c->d_cell_count && c->d_slot_pos) return PM_OK;
+  c->m = choose_m(c);
+  if (c->cells_alloc == c->ncells && c->cells_m == c->m && c->d_cell_count && c->d_slot_pos) return PM_OK;
   int n = c->n, nc = c->ncells;
   int r;
   if ((r = dalloc(c, &c->d_cell_of, n))) return r;
@@ -351,9 +363,10 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_cell_rank, n))) return r;
   if ((r = dalloc(c, &c->d_cell_atoms, n))) return r;
   if ((r = dalloc(c, &c->d_counts, 4))) return r;
-  long long slots = (long long)n + (long long)(PM_M - 1) * nc;
-  c->ncl_max = (int)((slots + PM_M - 1) / PM_M);
-  c->nslot_max = c->ncl_max * PM_M;
+  const int m = c->m;
+  long long slots = (long long)n + (long long)(m - 1) * nc;
+  c->ncl_max = (int)((slots + m - 1) / m);
+  c->nslot_max = c->ncl_max * m;
   if ((r = dalloc(c, &c->d_cluster_cell, c->ncl_max))) return r;
   if ((r = dalloc(c, &c->d_slot_pos, (size_t)c->nslot_max + 1))) return r;
   if ((r = dalloc(c, &c->d_slot_pos64, (size_t)c->nslot_max + 1))) return r;
@@ -370,6 +383,7 @@ int ensure_cells(pm_context* c) {
     HIPCHECK(c, hipMemset(c->d_stats, 0, 2 * sizeof(unsigned long long)));
   }
   c->cells_alloc = nc;
+  c->cells_m = m;
   c->jlist_alloc = 0;
   return PM_OK;
 }
@@ -779,12 +793,12 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
                              c->d_cell_count, nc);
   pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key);
   pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
-                       c->d_cluster_cell, c->ncl_max);
+                       c->d_cluster_cell, c->ncl_max, c->m);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms);
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
                        c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
-                       c->d_atom_of_slot, c->ntypes, c->nslot_max);
+                       c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m);
   }
 
   // energy records: pair passes, PME x-pass blocks, correction blocks
@@ -843,7 +857,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.max_delta = c->max_delta;
     a.xs_row = c->d_xs_row;
     a.xs_part = c->d_xs_part;
-    a.xs_tcap = PM_M * c->xs_maxdeg;
+    a.xs_tcap = c->m * c->xs_maxdeg;
+    a.m = c->m;
     a.jmask = c->d_jmask;
     a.molecule = c->d_mol;
     a.ex_row = c->d_ex_row;
@@ -998,7 +1013,7 @@ int list_stats(pm_context* c) {
   unsigned long long st[2] = {0, 0};
   int counts[4] = {0, 0, 0, 0};
   if (c->d_stats && c->d_jcount && c->d_counts) {
-    pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats);
+    pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats, c->m);
     HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -1326,6 +1341,10 @@ int pm_set_boundary(pm_context* c, const double* e1, const double* e2, const dou
     b.n[k] = std::max(1, (int)std::floor(d / c->realcell[k] + kEpsilon));
     b.to_fix[k] = 4294967296.0 / L[k];
     b.fix_to_A[k] = L[k] / 4294967296.0;
+    double u = mn[k] / L[k];
+    u -= std::floor(u);
+    b.cell0_fix[k] = u * 4294967296.0;
+    b.real_fix[k] = c->realcell[k] * 4294967296.0 / L[k];
   }
   double dx = (lx * 0.5) * (lx * 0.5), dy = (ly * 0.5) * (ly * 0.5), dz = (lz * 0.5) * (lz * 0.5);
   b.d = std::min(dx, std::min(dy, dz));
@@ -1409,7 +1428,7 @@ int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm
   pm_launch_zero_i(s, c->d_cell_count, nc);
   pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key);
   pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
-                       c->d_cluster_cell, c->ncl_max);
+                       c->d_cluster_cell, c->ncl_max, c->m);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms);
   std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);
   HIPCHECK(c, hipMemcpyAsync(cell.data(), c->d_cell_of, n * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1564,6 +1583,27 @@ int pm_get_timing(pm_context* c, double* ms, long long* count, int ncat) {
   harvest_timing(c);
   for (int k = 0; k < ncat && k < PM_TIMING_CATEGORIES; k++) { ms[k] = c->t_ms[k]; count[k] = c->t_count[k]; }
   return PM_OK;
+}
+
+int pm_set_option(pm_context* c, int option, int value) {
+  if (!c) return PM_ERR_ARGUMENT;
+  if (option == PM_OPT_CLUSTER_SIZE) {
+    if (value != 0 && value != 8 && value != 16)
+      return fail(c, PM_ERR_ARGUMENT, "cluster size must be 0 (automatic), 8 or 16 (got %d)", value);
+    c->m_override = value;
+    c->md_forces_valid = false;
+    return PM_OK;
+  }
+  return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);
+}
+
+int pm_get_option(pm_context* c, int option, int* value) {
+  if (!c || !value) return PM_ERR_ARGUMENT;
+  if (option == PM_OPT_CLUSTER_SIZE) {
+    *value = c->have_topo && c->have_box ? choose_m(c) : c->m_override;
+    return PM_OK;
+  }
+  return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);
 }
 
 int pm_synchronize(pm_context* c) {

@@ -28,9 +28,10 @@
 #define PM_COUL_PLAIN 1
 #define PM_COUL_EWALD 2
 
-// Atoms per i-cluster and j-slots per wave (lane = jslot * PM_M + ii).
-#define PM_M 8
-#define PM_JW (64 / PM_M)
+// Atoms per i-cluster: 8 or 16 (chosen per box from the cell occupancy);
+// a wave evaluates 64 / M list entries against the cluster per iteration
+// (lane = jslot * M + ii).
+#define PM_MMAX 16
 
 // Bounds that keep degenerate inputs from turning into unbounded loops.
 #define PM_SORT_MAX 512
@@ -59,6 +60,8 @@ struct PmBox {
   int n[3];           // cell grid dimensions
   double to_fix[3];   // 2^32 / L
   double fix_to_A[3]; // L / 2^32
+  double cell0_fix[3];  // fixed point of the cell grid's lower corner (min), in [0, 2^32)
+  double real_fix[3];   // real cell size in fixed-point units
 };
 
 // Switched pair potentials of one (possibly fused) nonbonded force.
@@ -98,8 +101,9 @@ struct PmNbLaunch {
   // exclusions: resolved classes of both atoms of every pair, partner | class << 24
   const int* xs_row;
   const int* xs_part;
-  int xs_tcap;               // LDS entries per cluster: PM_M * largest partner count
-  unsigned short* jmask;     // per list entry (flagged entries): FULL bit ii, MODIFIED bit 8 + ii
+  int xs_tcap;               // LDS entries per cluster: M * largest partner count
+  unsigned int* jmask;       // per list entry (flagged entries): FULL bit ii, MODIFIED bit 16 + ii
+  int m;                     // atoms per cluster (8 or 16)
   // topology
   const int* molecule;
   const int* ex_row;
@@ -121,18 +125,18 @@ struct PmNbLaunch {
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
                    int* cell_count, int* rank, float* key);
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
-                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max);
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m);
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
                        int* cell_atoms);
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
                           const double* q, const int* type, const int* cell_count,
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
-                          int* atom_of_slot, int dummy_type, int dummy_slot);
+                          int* atom_of_slot, int dummy_type, int dummy_slot, int m);
 // force_fix[atom] += force_slot[slot] for every real slot
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
                           const long long* force_slot, long long* force_fix);
-void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out);
+void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
 void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p,
                     int precision);

@@ -69,7 +69,7 @@ __global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __r
 __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
                                                      int* __restrict__ start, int* __restrict__ slot,
                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
-                                                     int ncl_max) {
+                                                     int ncl_max, int m_cl) {
   __shared__ int s_c[8192];
   __shared__ int s_wa[16], s_ws[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __re
       const int idx = 8 * t + k;
       v[k] = idx < m ? s_c[idx] : 0;
       sa += v[k];
-      ss += (v[k] + PM_M - 1) / PM_M * PM_M;
+      ss += (v[k] + m_cl - 1) / m_cl * m_cl;
     }
     // inclusive wave scans, then the 16 wave totals
     int ia = sa, is = ss;
@@ -115,10 +115,10 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __re
       const int idx = 8 * t + k;
       if (idx < m) {
         const int c = base + idx;
-        const int ps = (v[k] + PM_M - 1) / PM_M * PM_M;
+        const int ps = (v[k] + m_cl - 1) / m_cl * m_cl;
         start[c] = ra;
         slot[c] = rs;
-        for (int u = rs / PM_M; u < (rs + ps) / PM_M && u < ncl_max; u++) cluster_cell[u] = c;
+        for (int u = rs / m_cl; u < (rs + ps) / m_cl && u < ncl_max; u++) cluster_cell[u] = c;
         ra += v[k];
         rs += ps;
       }
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __re
     carry_s += ts;
     __syncthreads();
   }
-  if (t == 0) { counts[0] = carry_s; counts[1] = carry_s / PM_M; counts[2] = carry_a; }
+  if (t == 0) { counts[0] = carry_s; counts[1] = carry_s / m_cl; counts[2] = carry_a; }
 }
 
 __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
                              const int* __restrict__ cell_slot, const int* __restrict__ cell_atoms,
                              const float* __restrict__ key, int4* __restrict__ slot_pos,
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
-                             int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot) {
+                             int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c == 0 && lane == 0) {
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
     atom_of_slot[s0 + rank] = ai;
   }
   // padding slots: never used as j atoms, masked as i atoms, skipped by PME
-  const int ps = (k + PM_M - 1) / PM_M * PM_M;
+  const int ps = (k + m_cl - 1) / m_cl * m_cl;
   const int u = k + lane;
   if (u < ps) {
     const int a0 = a[0];
@@ -243,24 +243,28 @@ __device__ inline int wave_incl_scan(int v, int lane) {
   return v;
 }
 
+template <int M>
 __device__ inline float wave_min_m(float v) {
 #pragma unroll
-  for (int off = 1; off < PM_M; off <<= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  for (int off = 1; off < M; off <<= 1) v = fminf(v, __shfl_xor(v, off, 64));
   return v;
 }
+template <int M>
 __device__ inline float wave_max_m(float v) {
 #pragma unroll
-  for (int off = 1; off < PM_M; off <<= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  for (int off = 1; off < M; off <<= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
   return v;
 }
+template <int M>
 __device__ inline int wave_imin_m(int v) {
 #pragma unroll
-  for (int off = 1; off < PM_M; off <<= 1) v = min(v, __shfl_xor(v, off, 64));
+  for (int off = 1; off < M; off <<= 1) v = min(v, __shfl_xor(v, off, 64));
   return v;
 }
+template <int M>
 __device__ inline int wave_imax_m(int v) {
 #pragma unroll
-  for (int off = 1; off < PM_M; off <<= 1) v = max(v, __shfl_xor(v, off, 64));
+  for (int off = 1; off < M; off <<= 1) v = max(v, __shfl_xor(v, off, 64));
   return v;
 }
 
@@ -275,15 +279,17 @@ __device__ inline int wrap_cell(int c, int d, int n) {
 // the cluster's bounding box is below the list cutoff is appended to the
 // cluster's list.  Every unordered pair within the cutoff is thus listed
 // exactly once.  Bit 31 marks an entry that some cluster atom excludes or
-// modifies; its masks (FULL bit ii, MODIFIED bit 8 + ii) go to jmask.
+// modifies; its masks (FULL bit ii, MODIFIED bit 16 + ii) go to jmask.
+template <int M>
 __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) {
   extern __shared__ int s_dyn[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nst = a.nstencil;
-  int* s_start = s_dyn + wave * (3 * nst + 1 + a.xs_tcap);
+  int* s_start = s_dyn + wave * (3 * nst + 1 + 2 * a.xs_tcap);
   int* s_cnt = s_start + nst;
   int* s_pref = s_cnt + nst;     // nst + 1 entries
   int* s_tri = s_pref + nst + 1; // xs_tcap entries: partner | ii << 24 | class << 28
+  int* s_srt = s_tri + a.xs_tcap; // the same, sorted by partner
   const int cl = blockIdx.x * 4 + wave;
   const int ncl = a.counts[1];
   const bool active = cl < ncl;  // every wave reaches every barrier below
@@ -291,13 +297,13 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
   const int cz = cell % box.n[2], cy = (cell / box.n[2]) % box.n[1], cx = cell / (box.n[1] * box.n[2]);
 
   // i-cluster: anchor, bounding box and exclusion partners
-  const int4 anchor = active ? a.slot_pos[cl * PM_M] : make_int4(0, 0, 0, 0);
+  const int4 anchor = active ? a.slot_pos[cl * M] : make_int4(0, 0, 0, 0);
   const float fx = (float)box.fix_to_A[0], fy = (float)box.fix_to_A[1], fz = (float)box.fix_to_A[2];
   float lox = 1e30f, loy = 1e30f, loz = 1e30f, hix = -1e30f, hiy = -1e30f, hiz = -1e30f;
   int xbeg = 0, deg = 0;
-  if (active && lane < PM_M) {
-    int4 p = a.slot_pos[cl * PM_M + lane];
-    int m = a.slot_meta[cl * PM_M + lane];
+  if (active && lane < M) {
+    int4 p = a.slot_pos[cl * M + lane];
+    int m = a.slot_meta[cl * M + lane];
     int o = (m >> 8) & 0x7fffff;
     if (o != 0x7fffff) {
       float dx = (float)(int)(p.x - anchor.x) * fx;
@@ -308,11 +314,11 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
       deg = a.xs_row[o + 1] - xbeg;
     }
   }
-  lox = wave_min_m(lox); loy = wave_min_m(loy); loz = wave_min_m(loz);
-  hix = wave_max_m(hix); hiy = wave_max_m(hiy); hiz = wave_max_m(hiz);
+  lox = wave_min_m<M>(lox); loy = wave_min_m<M>(loy); loz = wave_min_m<M>(loz);
+  hix = wave_max_m<M>(hix); hiy = wave_max_m<M>(hiy); hiz = wave_max_m<M>(hiz);
   lox = __shfl(lox, 0, 64); loy = __shfl(loy, 0, 64); loz = __shfl(loz, 0, 64);
   hix = __shfl(hix, 0, 64); hiy = __shfl(hiy, 0, 64); hiz = __shfl(hiz, 0, 64);
-  // partners of the cluster's atoms into LDS (deg <= xs_tcap / PM_M per atom)
+  // partners of the cluster's atoms into LDS (deg <= xs_tcap / M per atom)
   const int tinc = wave_incl_scan(deg, lane);
   const int ntri = __shfl(tinc, 63, 64);
   int pmin = 0x7fffffff, pmax = -1;
@@ -323,86 +329,132 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
     pmin = min(pmin, o);
     pmax = max(pmax, o);
   }
-  pmin = __shfl(wave_imin_m(pmin), 0, 64);
-  pmax = __shfl(wave_imax_m(pmax), 0, 64);
+  pmin = __shfl(wave_imin_m<M>(pmin), 0, 64);
+  pmax = __shfl(wave_imax_m<M>(pmax), 0, 64);
 
-  // stencil cells and their prefix counts: half list, so only cells after
-  // this one in linear order, and this cell's atoms from this cluster on
-  // (the pair kernel keeps a pair only when the j slot follows the i slot)
-  if (active) {
-    for (int s = lane; s < nst; s += 64) {
-      int4 d = a.stencil[s];
-      int nb = (wrap_cell(cx, d.x, box.n[0]) * box.n[1] + wrap_cell(cy, d.y, box.n[1])) * box.n[2] +
-               wrap_cell(cz, d.z, box.n[2]);
-      int st = a.cell_slot[nb], ct = a.cell_count[nb];
+  // stencil cells: half list, so only cells after this one in linear order,
+  // and this cell's atoms from this cluster on (the pair kernel keeps a pair
+  // only when the j slot follows the i slot); cells whose box lies beyond the
+  // list cutoff from the cluster's bounding box are dropped; the non-empty
+  // ones are compacted in stencil order
+  int nact = 0;
+  for (int s0 = 0; s0 < nst; s0 += 64) {
+    const int s = s0 + lane;
+    int st = 0, ct = 0;
+    if (active && s < nst) {
+      const int4 d = a.stencil[s];
+      const int wx = wrap_cell(cx, d.x, box.n[0]), wy = wrap_cell(cy, d.y, box.n[1]), wz = wrap_cell(cz, d.z, box.n[2]);
+      const int nb = (wx * box.n[1] + wy) * box.n[2] + wz;
+      st = a.cell_slot[nb];
+      ct = a.cell_count[nb];
       if (nb < cell) ct = 0;
-      else if (nb == cell) { ct = st + ct - cl * PM_M; st = cl * PM_M; }
-      s_start[s] = st;
-      s_cnt[s] = ct;
+      else if (nb == cell) { ct = st + ct - cl * M; st = cl * M; }
+      if (ct > 0 && a.bbox_filter) {
+        // gap between the cell's box (padded by 1e-3 A) and the cluster's bounding box
+        const unsigned int bx = (unsigned int)(long long)(box.cell0_fix[0] + wx * box.real_fix[0]);
+        const unsigned int by = (unsigned int)(long long)(box.cell0_fix[1] + wy * box.real_fix[1]);
+        const unsigned int bz = (unsigned int)(long long)(box.cell0_fix[2] + wz * box.real_fix[2]);
+        const float clx = (float)(int)(bx - (unsigned int)anchor.x) * fx - 1e-3f;
+        const float cly = (float)(int)(by - (unsigned int)anchor.y) * fy - 1e-3f;
+        const float clz = (float)(int)(bz - (unsigned int)anchor.z) * fz - 1e-3f;
+        const float chx = clx + (float)(box.real_fix[0] * box.fix_to_A[0]) + 2e-3f;
+        const float chy = cly + (float)(box.real_fix[1] * box.fix_to_A[1]) + 2e-3f;
+        const float chz = clz + (float)(box.real_fix[2] * box.fix_to_A[2]) + 2e-3f;
+        const float gx = fmaxf(0.f, fmaxf(clx - hix, lox - chx));
+        const float gy = fmaxf(0.f, fmaxf(cly - hiy, loy - chy));
+        const float gz = fmaxf(0.f, fmaxf(clz - hiz, loz - chz));
+        if (gx * gx + gy * gy + gz * gz >= cut2_list) ct = 0;
+      }
     }
+    const unsigned long long bal = __ballot(ct > 0);
+    const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+    if (ct > 0) { s_start[nact + before] = st; s_cnt[nact + before] = ct; }
+    nact += __popcll(bal);
   }
   __syncthreads();
   int carry = 0;
-  for (int s0 = 0; s0 < nst; s0 += 64) {
-    int s = s0 + lane;
-    int v = (active && s < nst) ? s_cnt[s] : 0;
-    int inc = wave_incl_scan(v, lane);
-    if (active && s < nst) s_pref[s] = carry + inc - v;
+  for (int s0 = 0; s0 < nact; s0 += 64) {
+    const int s = s0 + lane;
+    const int v = (active && s < nact) ? s_cnt[s] : 0;
+    const int inc = wave_incl_scan(v, lane);
+    if (active && s < nact) s_pref[s] = carry + inc - v;
     carry += __shfl(inc, 63, 64);
   }
-  if (active && lane == 0) s_pref[nst] = carry;
+  if (active && lane == 0) s_pref[nact] = carry;
+  // exclusion partners sorted by partner index (rank sort, ties by position)
+  for (int t = lane; t < ntri; t += 64) {
+    const int w = s_tri[t], key = w & 0xffffff;
+    int rank = 0;
+    for (int u = 0; u < ntri; u++) {
+      const int ku = s_tri[u] & 0xffffff;
+      rank += (ku < key || (ku == key && u < t)) ? 1 : 0;
+    }
+    s_srt[rank] = w;
+  }
   __syncthreads();
   if (!active) return;
   const int total = carry;
+  int nsteps = 0;
+  while ((1 << nsteps) < ntri + 1) nsteps++;
 
   int* out = a.jlist + (long long)cl * a.lcap;
-  unsigned short* outm = a.jmask + (long long)cl * a.lcap;
+  unsigned int* outm = a.jmask + (long long)cl * a.lcap;
   int count = 0;
   // candidates beyond PM_CAND_FACTOR * lcap mean a degenerate cell layout:
   // stop scanning and report it as a list overflow
   const int scan = min(total, PM_CAND_FACTOR * a.lcap);
   if (scan < total && lane == 0) a.overflow[0] = 1;
+  int scur = 0;   // stencil entry holding candidate k0 (uniform)
   for (int k0 = 0; k0 < scan; k0 += 64) {
-    int k = k0 + lane;
+    const int k = k0 + lane;
+    // owner entry of every lane: walk the boundaries inside this batch
+    while (scur + 1 < nact && s_pref[scur + 1] <= k0) scur++;
+    int s = scur;
+    for (int t = scur + 1; t < nact && s_pref[t] <= k0 + 63; t++)
+      if (k >= s_pref[t]) s = t;
     bool pass = false;
     int sj = 0, o = 0;
     if (k < scan) {
-      int lo = 0, hi = nst;  // largest s with s_pref[s] <= k
-      while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= k) lo = mid; else hi = mid;
-      }
-      sj = s_start[lo] + (k - s_pref[lo]);
-      int4 p = a.slot_pos[sj];
+      sj = s_start[s] + (k - s_pref[s]);
+      const int4 p = a.slot_pos[sj];
       pass = true;
       if (a.bbox_filter) {
-        float dx = (float)(int)(p.x - anchor.x) * fx;
-        float dy = (float)(int)(p.y - anchor.y) * fy;
-        float dz = (float)(int)(p.z - anchor.z) * fz;
-        float gx = fmaxf(0.f, fmaxf(lox - dx, dx - hix));
-        float gy = fmaxf(0.f, fmaxf(loy - dy, dy - hiy));
-        float gz = fmaxf(0.f, fmaxf(loz - dz, dz - hiz));
+        const float dx = (float)(int)(p.x - anchor.x) * fx;
+        const float dy = (float)(int)(p.y - anchor.y) * fy;
+        const float dz = (float)(int)(p.z - anchor.z) * fz;
+        const float gx = fmaxf(0.f, fmaxf(lox - dx, dx - hix));
+        const float gy = fmaxf(0.f, fmaxf(loy - dy, dy - hiy));
+        const float gz = fmaxf(0.f, fmaxf(loz - dz, dz - hiz));
         pass = gx * gx + gy * gy + gz * gz < cut2_list;
       }
       o = (a.slot_meta[sj] >> 8) & 0x7fffff;
     }
-    // exclusion masks of the entry against the cluster's atoms
+    // exclusion masks of the entry against the cluster's atoms: binary search
+    // of the sorted partners, then every entry with this partner
     int xm = 0;
     const bool pre = pass && o >= pmin && o <= pmax;
     if (__ballot(pre) != 0ull) {
+      int lo = 0, hi = ntri;
+      for (int it = 0; it < nsteps; it++) {
+        const int mid = (lo + hi) >> 1;
+        if (lo < hi) {
+          if ((s_srt[mid] & 0xffffff) < o) lo = mid + 1; else hi = mid;
+        }
+      }
       if (pre) {
-        for (int t = 0; t < ntri; t++) {
-          const int w = s_tri[t];
-          if ((w & 0xffffff) == o) xm |= ((w >> 28) == PM_EXCL_FULL ? 1 : 256) << ((w >> 24) & 7);
+        for (int t = lo; t < ntri; t++) {
+          const int w = s_srt[t];
+          if ((w & 0xffffff) != o) break;
+          xm |= ((w >> 28) == PM_EXCL_FULL ? 1 : 65536) << ((w >> 24) & 15);
         }
       }
     }
-    unsigned long long bal = __ballot(pass);
-    int before = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-    int idx = count + before;
+    const unsigned long long bal = __ballot(pass);
+    const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+    const int idx = count + before;
     if (pass && idx < a.lcap) {
       out[idx] = sj | (xm != 0 ? (int)0x80000000 : 0);
-      if (xm != 0) outm[idx] = (unsigned short)xm;
+      if (xm != 0) outm[idx] = (unsigned int)xm;
     }
     count += __popcll(bal);
   }
@@ -414,7 +466,7 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
 
 // list statistics: total entries and issued lanes over all clusters
 __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcount, const int* __restrict__ counts,
-                                                    unsigned long long* __restrict__ out) {
+                                                    unsigned long long* __restrict__ out, int jw) {
   __shared__ unsigned long long s0[256], s1[256];
   const int ncl = counts[1];
   unsigned long long e = 0, l = 0;
@@ -428,7 +480,7 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       e += v[k];
-      l += (unsigned long long)((v[k] + PM_JW - 1) / PM_JW) * 64;
+      l += (unsigned long long)((v[k] + jw - 1) / jw) * 64;
     }
   }
   s0[threadIdx.x] = e;
@@ -492,10 +544,25 @@ __device__ inline void switching(T r2, T dist, T cut2, T on2, T sw1, T sw2, T sw
   }
 }
 
+// force = force * S - energy * dS, energy = energy * S; the Cutoff function
+// (S = 0 or 1, dS = 0) reduces to a select with the same result
+template <typename T, int SWF>
+__device__ inline void apply_switch(T r2, T dist, T cut2, T on2, T sw1, T sw2, T sw3, T c15_1, T c05_3, T c15_3,
+                                    T& f, T& e) {
+  if constexpr (SWF == PM_SWF_CUTOFF) {
+    if (r2 > cut2) { f = T(0); e = T(0); }
+  } else {
+    T v, d;
+    switching<T, SWF>(r2, dist, cut2, on2, sw1, sw2, sw3, c15_1, c05_3, c15_3, v, d);
+    f = f * v - e * d;
+    e = e * v;
+  }
+}
 
-// Sum over the 8 lanes of a row (lanes 8r .. 8r+7) with DPP moves: quad
-// xor 1, quad xor 2, half-row mirror.  Every lane of the row ends with the
-// same value, added in the same order on every run.
+
+// Sum over the M lanes of a row (lanes M*r .. M*r+M-1) with DPP moves: quad
+// xor 1, quad xor 2, half-row mirror (and row mirror for M = 16).  Every lane
+// of the row ends with the same value, added in the same order on every run.
 template <int CTRL>
 __device__ inline float dpp_f(float v) {
   // old = 0 (the identity of the sum): the move folds into v_add_f32_dpp
@@ -508,11 +575,12 @@ __device__ inline double dpp_f(double v) {
   const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-template <typename T>
-__device__ inline T row8_sum(T v) {
+template <int M, typename T>
+__device__ inline T row_sum(T v) {
   v += dpp_f<0xB1>(v);    // quad_perm [1,0,3,2]
   v += dpp_f<0x4E>(v);    // quad_perm [2,3,0,1]
-  v += dpp_f<0x141>(v);   // row_half_mirror
+  v += dpp_f<0x141>(v);   // row_half_mirror: sums of 8 lanes
+  if constexpr (M == 16) v += dpp_f<0x140>(v);   // row_mirror: sums of 16 lanes
   return v;
 }
 
@@ -529,16 +597,16 @@ __device__ inline long long force_fix_of(float v) {
 }
 __device__ inline long long force_fix_of(double v) { return __double2ll_rn(v * PM_FORCE_FIX); }
 
-// Half-list pair kernel: one wave per i-cluster, lane = jslot * 8 + ii, so a
-// wave evaluates 8 list entries against the 8 cluster atoms per iteration.
+// Half-list pair kernel: one wave per i-cluster of M atoms, lane = jslot * M +
+// ii, so a wave evaluates 64 / M list entries against the cluster per iteration.
 // A pair is kept when the j slot follows the i slot (every unordered pair is
 // listed once, by the cluster of its lower slot).  The i force accumulates
 // per lane in FP64 and is reduced over the j slots at the end; the j force of
-// each entry is summed over its row of 8 lanes with DPP and added with one
-// 64-bit integer atomic per component.  Forces accumulate per slot, so the 24
+// each entry is summed over its row of M lanes with DPP and added with one
+// 64-bit integer atomic per component.  Forces accumulate per slot, so the
 // atomics of one instruction fall on a few contiguous cache lines; integer
 // addition is associative, so the result does not depend on their order.
-template <typename T, int LJSWF, int CK, int CSWF>
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
 __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) {
   extern __shared__ unsigned char s_lj_raw[];
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
@@ -554,19 +622,21 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
   const int cl = blockIdx.x * 4 + wave;
   const int ncl = a.counts[1];
   if (cl >= ncl) {
-    if (cl < a.ncl_max && lane < PM_NE) a.epart[(long long)cl * PM_NE + lane] = 0.0;
+    if (WE && cl < a.ncl_max && lane < PM_NE) a.epart[(long long)cl * PM_NE + lane] = 0.0;
     return;
   }
-  const int ii = lane % PM_M, js = lane / PM_M;
-  const int si = cl * PM_M + ii;
+  constexpr int JW = 64 / M;
+  const int ii = lane % M, js = lane / M;
+  const int si = cl * M + ii;
   const int mi = a.slot_meta[si];
   const int ti = mi & 0xff, oi = (mi >> 8) & 0x7fffff;
   const bool ireal = oi != 0x7fffff;
   // positions: 32-bit periodic fixed point (mixed) or 64-bit (FP64); the
   // wrapped integer difference is the minimal-image vector
   typedef typename std::conditional<sizeof(T) == 4, int4, longlong4>::type P4;
-  P4 pi;
-  if constexpr (sizeof(T) == 4) pi = a.slot_pos[si]; else pi = a.slot_pos64[si];
+  const P4* __restrict__ spos;
+  if constexpr (sizeof(T) == 4) spos = a.slot_pos; else spos = a.slot_pos64;
+  const P4 pi = spos[si];
   T qi;
   if constexpr (sizeof(T) == 4) qi = __int_as_float(pi.w); else qi = __longlong_as_double(pi.w);
   const T sx = sizeof(T) == 4 ? (T)box.fix_to_A[0] : (T)(box.fix_to_A[0] * 2.3283064365386963e-10);
@@ -581,25 +651,39 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
   const T c_cut2 = (T)pp.c_cut2, c_on2 = (T)pp.c_on2, c_sw1 = (T)pp.c_sw1, c_sw2 = (T)pp.c_sw2,
           c_sw3 = (T)pp.c_sw3, c_c15_1 = (T)pp.c_c15_1, c_c05_3 = (T)pp.c_c05_3, c_c15_3 = (T)pp.c_c15_3;
   const T alpha = (T)pp.alpha, tapi = (T)pp.two_alpha_pi, csf = (T)pp.csf;
+  const bool modify = pp.modify != 0;
 
   double fxa = 0, fya = 0, fza = 0;
   T elj = 0, ec = 0;
   T vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
-  const int* lst = a.jlist + (long long)cl * a.lcap;
+  const int* __restrict__ lst = a.jlist + (long long)cl * a.lcap;
   const int cnt = a.jcount[cl];
-  const bool vir = a.want_virial != 0;
+  const bool vir = WE && a.want_virial != 0;
   long long* __restrict__ ff = a.force_slot;
 
-  for (int base = 0; base < cnt; base += PM_JW) {
+  // software pipeline: the entry two iterations ahead and the j atom of the
+  // next iteration are in flight while an iteration computes
+  int e_cur = js < cnt ? lst[js] : a.dummy_slot;
+  int e_nxt = js + JW < cnt ? lst[js + JW] : a.dummy_slot;
+  P4 pj_cur = spos[e_cur & 0x7fffffff];
+  int mj_cur = a.slot_meta[e_cur & 0x7fffffff];
+
+  for (int base = 0; base < cnt; base += JW) {
     const int k = base + js;
     const bool valid = k < cnt;
-    const int e = valid ? lst[k] : a.dummy_slot;
-    const bool flagged = e < 0;
+    const int e = e_cur;
+    const P4 pj = pj_cur;
+    const int mj = mj_cur;
     const int sj = e & 0x7fffffff;
-    P4 pj;
-    if constexpr (sizeof(T) == 4) pj = a.slot_pos[sj]; else pj = a.slot_pos64[sj];
-    const int mj = a.slot_meta[sj];
-    const int oj = (mj >> 8) & 0x7fffff;
+    const bool flagged = e < 0;
+    {
+      const int k2 = k + 2 * JW;
+      const int e2 = k2 < cnt ? lst[k2] : a.dummy_slot;
+      pj_cur = spos[e_nxt & 0x7fffffff];
+      mj_cur = a.slot_meta[e_nxt & 0x7fffffff];
+      e_cur = e_nxt;
+      e_nxt = e2;
+    }
     T dx, dy, dz;
     if constexpr (sizeof(T) == 4) {
       dx = (T)(int)(pj.x - pi.x) * sx;
@@ -617,7 +701,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
       if (flagged && mask) {
         const int xm = a.jmask[(long long)cl * a.lcap + k];
         if ((xm >> ii) & 1) { excl = PM_EXCL_FULL; mask = false; }
-        else if ((xm >> (8 + ii)) & 1) excl = PM_EXCL_MODIFIED;
+        else if ((xm >> (16 + ii)) & 1) excl = PM_EXCL_MODIFIED;
       }
     }
     T qj;
@@ -625,7 +709,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     T rr2, rinv;
     if constexpr (sizeof(T) == 4) { rinv = Prec<T>::rsq(r2); rr2 = rinv * rinv; }
     else { rr2 = T(1) / r2; rinv = sqrt(rr2); }
-    T F = 0, E1 = 0, E2 = 0;
+    T F, E1 = 0, E2 = 0;
     if constexpr (LJSWF != PM_SWF_NONE) {
       const int tj = mj & 0xff;
       const LJ4 ab = s_lj[ti * nt1 + tj];
@@ -637,14 +721,10 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
       const T r12A = A * r12;
       T e1 = r12A - r6B;
       T f1 = T(12) * r12A * rr2 - T(6) * r6B * rr2;
-      if (pp.modify) {
-        T v, d;
-        switching<T, LJSWF == PM_SWF_CUTOFF ? PM_SWF_CUTOFF : LJSWF>(
-            r2, r2 * rinv, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3, lj_c15_3, v, d);
-        f1 = f1 * v - e1 * d;
-        e1 = e1 * v;
-      }
-      F += f1;
+      if (modify)
+        apply_switch<T, LJSWF>(r2, r2 * rinv, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3,
+                               lj_c15_3, f1, e1);
+      F = f1;
       E1 = e1;
     }
     if constexpr (CK != PM_COUL_NONE) {
@@ -665,32 +745,27 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
         e2 = poly * ex * rinv;
         f2 = (e2 + tapi * ex) * rinv * rinv;
       }
-      if (pp.modify) {
-        T v, d;
-        switching<T, CSWF>(r2, r2 * rinv, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3,
-                           c_c15_3, v, d);
-        f2 = f2 * v - e2 * d;
-        e2 = e2 * v;
-      }
-      F += f2;
+      if (modify)
+        apply_switch<T, CSWF>(r2, r2 * rinv, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3, f2, e2);
+      if constexpr (LJSWF != PM_SWF_NONE) F += f2; else F = f2;
       E2 = e2;
     }
     F = mask ? F : T(0);
-    E1 = mask ? E1 : T(0);
-    E2 = mask ? E2 : T(0);
     // fij = F * (xj - xi): -fij on i, +fij on j (OneAtomPair.h:178-181)
     const T fx_ = F * dx, fy_ = F * dy, fz_ = F * dz;
     fxa -= (double)fx_;
     fya -= (double)fy_;
     fza -= (double)fz_;
-    elj += E1;
-    ec += E2;
-    if (vir) {
-      vxx += fx_ * dx; vxy += fx_ * dy; vxz += fx_ * dz;
-      vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
+    if constexpr (WE) {
+      elj += mask ? E1 : T(0);
+      ec += mask ? E2 : T(0);
+      if (vir) {
+        vxx += fx_ * dx; vxy += fx_ * dy; vxz += fx_ * dz;
+        vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
+      }
     }
     // j force of this row: lanes ii = 0, 1, 2 add its x, y, z component
-    const T jx = row8_sum(fx_), jy = row8_sum(fy_), jz = row8_sum(fz_);
+    const T jx = row_sum<M>(fx_), jy = row_sum<M>(fy_), jz = row_sum<M>(fz_);
     const T jv = ii == 0 ? jx : (ii == 1 ? jy : jz);
     if (ii < 3 && jv != T(0))
       atomicAdd((unsigned long long*)(ff + 3LL * sj + ii), (unsigned long long)force_fix_of(jv));
@@ -698,7 +773,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
   // reduce the i force over the j slots (lanes ii, ii+8, ...); lanes
   // js = 0, 1, 2 then add its x, y, z component
 #pragma unroll
-  for (int off = PM_M; off < 64; off <<= 1) {
+  for (int off = M; off < 64; off <<= 1) {
     fxa += __shfl_xor(fxa, off, 64);
     fya += __shfl_xor(fya, off, 64);
     fza += __shfl_xor(fza, off, 64);
@@ -707,30 +782,31 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     const double iv = js == 0 ? fxa : (js == 1 ? fya : fza);
     atomicAdd((unsigned long long*)(ff + 3LL * si + js), (unsigned long long)__double2ll_rn(iv * PM_FORCE_FIX));
   }
-  if (!a.want_energy) return;
-  // energies and virial of the cluster's pairs (each pair counted once)
-  double r[8] = {ireal ? (double)ec : 0.0, ireal ? (double)elj : 0.0,
-                 ireal ? (double)vxx : 0.0, ireal ? (double)vxy : 0.0, ireal ? (double)vxz : 0.0,
-                 ireal ? (double)vyy : 0.0, ireal ? (double)vyz : 0.0, ireal ? (double)vzz : 0.0};
-  const int nr = vir ? 8 : 2;
-  for (int u = 0; u < nr; u++) {
-    double v = r[u];
+  if constexpr (WE) {
+    // energies and virial of the cluster's pairs (each pair counted once)
+    double r[8] = {ireal ? (double)ec : 0.0, ireal ? (double)elj : 0.0,
+                   ireal ? (double)vxx : 0.0, ireal ? (double)vxy : 0.0, ireal ? (double)vxz : 0.0,
+                   ireal ? (double)vyy : 0.0, ireal ? (double)vyz : 0.0, ireal ? (double)vzz : 0.0};
+    const int nr = vir ? 8 : 2;
+    for (int u = 0; u < nr; u++) {
+      double v = r[u];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-    r[u] = v;
-  }
-  if (lane == 0) {
-    double* ep = a.epart + (long long)cl * PM_NE;
-    ep[PM_E_COUL] = r[0];
-    ep[PM_E_LJ] = r[1];
-    for (int u = 2; u < PM_E_VIR; u++) ep[u] = 0.0;
-    if (vir) {
-      // fij . diff with fij = F*d; the reference adds virial of +fij (ScalarStructure::addVirial)
-      ep[PM_E_VIR + 0] = r[2]; ep[PM_E_VIR + 1] = r[3]; ep[PM_E_VIR + 2] = r[4];
-      ep[PM_E_VIR + 3] = r[3]; ep[PM_E_VIR + 4] = r[5]; ep[PM_E_VIR + 5] = r[6];
-      ep[PM_E_VIR + 6] = r[4]; ep[PM_E_VIR + 7] = r[6]; ep[PM_E_VIR + 8] = r[7];
-    } else {
-      for (int u = 0; u < 9; u++) ep[PM_E_VIR + u] = 0.0;
+      for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+      r[u] = v;
+    }
+    if (lane == 0) {
+      double* ep = a.epart + (long long)cl * PM_NE;
+      ep[PM_E_COUL] = r[0];
+      ep[PM_E_LJ] = r[1];
+      for (int u = 2; u < PM_E_VIR; u++) ep[u] = 0.0;
+      if (vir) {
+        // fij . diff with fij = F*d; the reference adds virial of +fij (ScalarStructure::addVirial)
+        ep[PM_E_VIR + 0] = r[2]; ep[PM_E_VIR + 1] = r[3]; ep[PM_E_VIR + 2] = r[4];
+        ep[PM_E_VIR + 3] = r[3]; ep[PM_E_VIR + 4] = r[5]; ep[PM_E_VIR + 5] = r[6];
+        ep[PM_E_VIR + 6] = r[4]; ep[PM_E_VIR + 7] = r[6]; ep[PM_E_VIR + 8] = r[7];
+      } else {
+        for (int u = 0; u < 9; u++) ep[PM_E_VIR + u] = 0.0;
+      }
     }
   }
 }
@@ -826,9 +902,9 @@ void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, in
 }
 
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
-                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max) {
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m) {
   hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
-                     counts, cluster_cell, ncl_max);
+                     counts, cluster_cell, ncl_max, m);
 }
 
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
@@ -842,11 +918,11 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
                           const double* q, const int* type, const int* cell_count,
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
-                          int* atom_of_slot, int dummy_type, int dummy_slot) {
+                          int* atom_of_slot, int dummy_type, int dummy_slot, int m) {
   int nb = (ncells + 3) / 4;
   hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, box, ncells, pos, q, type,
                      cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta,
-                     atom_of_slot, dummy_type, dummy_slot);
+                     atom_of_slot, dummy_type, dummy_slot, m);
 }
 
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
@@ -856,21 +932,31 @@ void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const
                        force_slot, force_fix);
 }
 
-void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out) {
-  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out);
+void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m) {
+  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out, 64 / m);
 }
 
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   double cut = sqrt(p.cut2);
   float c = (float)(cut * (1.0 + 1e-5) + 1e-4);
-  size_t lds = 4 * (3 * a.nstencil + 1 + a.xs_tcap) * sizeof(int);
-  hipLaunchKernelGGL(k_list, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
+  size_t lds = 4 * (3 * a.nstencil + 1 + 2 * a.xs_tcap) * sizeof(int);
+  if (a.m == 16)
+    hipLaunchKernelGGL(k_list<16>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
+  else
+    hipLaunchKernelGGL(k_list<8>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
 }
 
 template <typename T, int LJ, int CK, int CS>
 static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
-  hipLaunchKernelGGL((k_pair<T, LJ, CK, CS>), dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, p);
+  const dim3 g((a.ncl_max + 3) / 4), b(256);
+  if (a.m == 16) {
+    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 16>), g, b, lds, s, box, a, p);
+    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 16>), g, b, lds, s, box, a, p);
+  } else {
+    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 8>), g, b, lds, s, box, a, p);
+    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 8>), g, b, lds, s, box, a, p);
+  }
 }
 
 template <typename T>

@@ -26,7 +26,9 @@ VIRIAL = 7
 EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set_precision",
            "pm_set_topology", "pm_set_bonded", "pm_set_boundary", "pm_force_create", "pm_force_uncache",
            "pm_evaluate", "pm_cell_diagnostics", "pm_md_setup", "pm_md_set_state", "pm_md_run",
-           "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_synchronize"]
+           "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
+           "pm_synchronize"]
+OPTIONS = {"cluster_size": 0}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 
@@ -72,6 +74,8 @@ def load_library(path: str = LIB_PATH):
     lib.pm_last_stats.argtypes = [P, L, C.c_int]
     lib.pm_set_timing.argtypes = [P, C.c_uint]
     lib.pm_get_timing.argtypes = [P, D, L, C.c_int]
+    lib.pm_set_option.argtypes = [P, C.c_int, C.c_int]
+    lib.pm_get_option.argtypes = [P, C.c_int, I]
     lib.pm_synchronize.argtypes = [P]
     _lib = lib
     return lib
@@ -221,6 +225,14 @@ class Context:
         cnt = np.zeros(len(TIMING), np.int64)
         self._check(self.lib.pm_get_timing(self.h, _dp(ms), _lp(cnt), len(TIMING)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(TIMING)}
+
+    def set_option(self, name: str, value: int):
+        self._check(self.lib.pm_set_option(self.h, OPTIONS[name], int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int()
+        self._check(self.lib.pm_get_option(self.h, OPTIONS[name], C.byref(v)))
+        return v.value
 
     def synchronize(self):
         self._check(self.lib.pm_synchronize(self.h))

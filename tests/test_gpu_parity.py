@@ -39,11 +39,14 @@ def test_cell_binning_bitexact(gpu_ctx_factory, port):
     np.testing.assert_array_equal(g["inrc"], oc["inrc"])
 
 
+@pytest.mark.parametrize("cluster", [8, 16])
 @pytest.mark.parametrize("precision,tol", [("mixed", 2e-6), ("double", 1e-11)])
-def test_cutoff_c1(gpu_ctx_factory, port, precision, tol):
-    say(f"case: water648 LJ+Coulomb C2 {precision}")
+def test_cutoff_c1(gpu_ctx_factory, port, precision, tol, cluster):
+    say(f"case: water648 LJ+Coulomb C2 {precision} cluster {cluster}")
     s = water.named("water648", seed=1)
     c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    c.set_option("cluster_size", cluster)
+    assert c.get_option("cluster_size") == cluster
     f, e = c.evaluate(gpu_forces_c1(c), s.pos, virial=True)
     lj, co = c1_terms()
     a = port.cutoff(s, lj, 8.0, 4.0, virial=True)
@@ -85,13 +88,15 @@ def test_cutoff_variants(gpu_ctx_factory, port, fid, vals, terms, precision, tol
     np.testing.assert_allclose(e[:2], o["energies"][:2], rtol=tol * 10, atol=1e-9)
 
 
+@pytest.mark.parametrize("cluster", [8, 16])
 @pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-11)])
-def test_exclusion_rules(gpu_ctx_factory, port, precision, tol):
+def test_exclusion_rules(gpu_ctx_factory, port, precision, tol, cluster):
     """MODIFIED (A14/B14, csf) and NONE overrides, duplicate and cross-molecule
     exclusion pairs, through the two-term cutoff kernels and the PME correction."""
-    say(f"case: exclusion rules water648 {precision}")
+    say(f"case: exclusion rules water648 {precision} cluster {cluster}")
     s = exclusion_mix(water.named("water648", seed=8, jitter=0.4))
     c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    c.set_option("cluster_size", cluster)
     for fid, vals, terms in VARIANTS[5:]:
         f, e = c.evaluate([c.force(fid, vals)], s.pos, virial=True)
         o = port.cutoff(s, terms, vals[-1], 4.0, virial=True)
