@@ -660,6 +660,9 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
   const int cnt = a.jcount[cl];
   const bool vir = WE && a.want_virial != 0;
   long long* __restrict__ ff = a.force_slot;
+  const int comp = ii < 3 ? ii : ii - 3;   // force component a lane adds (lanes ii < 6)
+  T pend_v = T(0);                        // j-force component of the previous row
+  int pend_sj = 0;
 
   // software pipeline: the entry two iterations ahead and the j atom of the
   // next iteration are in flight while an iteration computes
@@ -764,11 +767,25 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
         vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
       }
     }
-    // j force of this row: lanes ii = 0, 1, 2 add its x, y, z component
+    // j force of this row, added every second iteration: lanes ii = 0, 1, 2
+    // add the x, y, z component of the previous iteration's row, lanes
+    // ii = 3, 4, 5 those of this one (one atomic instruction per two rows)
     const T jx = row_sum<M>(fx_), jy = row_sum<M>(fy_), jz = row_sum<M>(fz_);
-    const T jv = ii == 0 ? jx : (ii == 1 ? jy : jz);
-    if (ii < 3 && jv != T(0))
-      atomicAdd((unsigned long long*)(ff + 3LL * sj + ii), (unsigned long long)force_fix_of(jv));
+    const T jc = comp == 0 ? jx : (comp == 1 ? jy : jz);
+    if ((base / JW) & 1) {
+      const T v = ii < 3 ? pend_v : jc;
+      const int sl = ii < 3 ? pend_sj : sj;
+      if (ii < 6 && v != T(0))
+        atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
+    } else {
+      pend_v = jc;
+      pend_sj = sj;
+    }
+  }
+  if (((cnt + JW - 1) / JW) & 1) {
+    // odd number of iterations: the last row is still pending
+    if (ii < 3 && pend_v != T(0))
+      atomicAdd((unsigned long long*)(ff + 3LL * pend_sj + comp), (unsigned long long)force_fix_of(pend_v));
   }
   // reduce the i force over the j slots (lanes ii, ii+8, ...); lanes
   // js = 0, 1, 2 then add its x, y, z component
