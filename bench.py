@@ -95,6 +95,33 @@ def run_md(s, precision, steps, warmup, timing=()):
     return dict(seconds=t1 - t0, timing=tim, pairs=pairs, energies=e, x=x, v=v, stats=stats)
 
 
+def nve_drift(s, precision, steps=1000, chunk=100):
+    """Total-energy drift of a leapfrog NVE run: E = kinetic (synchronous
+    velocities after the closing half kick) + potential, sampled every `chunk`
+    steps; the least-squares slope in kcal/mol per ns and per degree of freedom."""
+    from protomol_amd import gpu, water
+    ctx = gpu.Context(0, precision)
+    ctx.set_topology(s)
+    ctx.set_boundary(s.box, s.origin, CELL)
+    hs = make_forces(ctx, pme_alpha(RC, 1e-6))
+    ctx.md_setup(hs, "leapfrog", dt=DT_FS)
+    ctx.md_set_state(s.pos, s.vel)
+    t, e = [], []
+    for k in range(steps // chunk + 1):
+        if k:
+            ctx.md_run(chunk)
+        else:
+            ctx.md_run(1)   # energies of the starting configuration (one step in)
+        x, v, en = ctx.md_get_state()
+        ke = 0.5 * float((s.mass[:, None] * v * v).sum())
+        t.append((k * chunk + (0 if k else 1)) * DT_FS * 1e-6)   # ns
+        e.append(ke + float(en[0] + en[1] + en[2] + en[3]))
+    ctx.close()
+    slope = float(np.polyfit(np.array(t), np.array(e), 1)[0])
+    return {"steps": steps, "kcal_mol_per_ns_per_dof": slope / (3 * s.n),
+            "e_start": e[0], "e_end": e[-1], "samples": len(e)}
+
+
 def cpu_baseline(s, alpha):
     """The reference's own CPU path (oracle/_ref, compiled from the ProtoMol
     sources) timed for one force evaluation of the same force set on one host
@@ -136,6 +163,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-drift", action="store_true", help="skip the 1000-step NVE drift runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {args.gpus} is not run "
@@ -198,6 +226,8 @@ def main():
                          "bond": float(mixed["energies"][2]), "angle": float(mixed["energies"][3])},
         "list_lanes_per_pair": mixed["stats"]["lanes"] / max(mixed["pairs"], 1),
     }
+    if not args.no_drift:
+        line["nve_drift"] = {"mixed": nve_drift(s, "mixed"), "fp64": nve_drift(s, "double")}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, alpha)
         line["cpu_baseline"] = cb

@@ -57,6 +57,8 @@ struct DevBuf {
 struct pm_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // PME, correction and bonded work, beside the pair work
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   int precision = PM_PRECISION_MIXED;
 
@@ -236,21 +238,24 @@ void harvest_timing(pm_context* c) {
 struct TimedScope {
   pm_context* c;
   int cat;
+  hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
-  TimedScope(pm_context* ctx, int category) : c(ctx), cat(category) {
+  TimedScope(pm_context* ctx, int category, hipStream_t stream = nullptr)
+      : c(ctx), cat(category), st(stream ? stream : ctx->stream) {
     if (c->timing_mask & (1u << cat)) {
       if (c->ev_open.size() > 8192) {
         hipStreamSynchronize(c->stream);
+        if (c->stream2) hipStreamSynchronize(c->stream2);
         harvest_timing(c);
       }
       a = take_event(c);
       b = take_event(c);
-      hipEventRecord(a, c->stream);
+      hipEventRecord(a, st);
     }
   }
   ~TimedScope() {
     if (a) {
-      hipEventRecord(b, c->stream);
+      hipEventRecord(b, st);
       c->ev_open.push_back(std::make_pair(cat, std::make_pair(a, b)));
     }
   }
@@ -824,9 +829,12 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   if (do_angle) rec += (c->nangles + 255) / 256;
   if ((r = ensure_epart(c, std::max(rec, 1LL)))) return r;
 
+  // host-side set-up of every launch first (stencils, list capacity, PME
+  // bricks), so that a failure leaves no work half queued
   double lmin_half = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
   double cdiag = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
                            c->realcell[2] * c->realcell[2]);
+  std::vector<std::pair<PmNbLaunch, PmPairParams>> pair_launch;
   for (size_t pi = 0; pi < passes.size(); pi++) {
     PairPass& ps = passes[pi];
     PmPairParams pp = pair_params(ps.terms, c->csf);
@@ -837,7 +845,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       return fail(c, PM_ERR_UNSUPPORTED, "cell stencil of %d cells (cellsize %g, cutoff %g): use a cellsize of at least cutoff/4", nst, c->cellsize, ps.algo_cutoff);
     double pcut = std::sqrt(pp.cut2);
     if ((r = ensure_lists(c, list_capacity(c, pcut)))) return r;
-    PmNbLaunch a{};
+    pair_launch.push_back(std::make_pair(PmNbLaunch{}, pp));
+    PmNbLaunch& a = pair_launch.back().first;
     a.slot_pos = c->d_slot_pos;
     a.slot_pos64 = c->d_slot_pos64;
     a.slot_meta = c->d_slot_meta;
@@ -849,8 +858,6 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.cell_count = c->d_cell_count;
     a.stencil = st;
     a.nstencil = nst;
-    a.jlist = c->d_jlist;
-    a.jcount = c->d_jcount;
     a.lcap = c->lcap;
     a.overflow = c->d_overflow;
     a.bbox_filter = lmin_half > pcut + cdiag + 1e-3 ? 1 : 0;
@@ -859,7 +866,6 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.xs_part = c->d_xs_part;
     a.xs_tcap = c->m * c->xs_maxdeg;
     a.m = c->m;
-    a.jmask = c->d_jmask;
     a.molecule = c->d_mol;
     a.ex_row = c->d_ex_row;
     a.ex_partner = c->d_ex_partner;
@@ -872,19 +878,15 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
     a.want_virial = vir ? 1 : 0;
     a.want_energy = want_energy ? 1 : 0;
-    {
-      TimedScope ts(c, PM_T_LIST);
-      pm_launch_list(s, c->box, a, pp);
-    }
-    {
-      TimedScope ts(c, PM_T_PAIR);
-      pm_launch_pair(s, c->box, a, pp, c->precision);
-    }
   }
-  if (!passes.empty()) {
-    TimedScope ts(c, PM_T_BIN);
-    pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_force_slot, c->d_force);
+  // the list buffers may have grown while the passes were set up
+  for (auto& pl : pair_launch) {
+    pl.first.jlist = c->d_jlist;
+    pl.first.jmask = c->d_jmask;
+    pl.first.jcount = c->d_jcount;
+    pl.first.lcap = c->lcap;
   }
+  std::vector<std::pair<PmPmeLaunch, int>> pme_launch;   // (launch, handle)
   double const_coul = 0.0, const_vir = 0.0;
   for (auto& hr : pme_rec) {
     ForceDesc& f = c->forces[hr.first];
@@ -917,44 +919,35 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       // spread bricks: 2x2x2 cells per brick (fewer where the tile would not fit 48 KB of LDS)
       a.cell_slot = c->d_cell_slot;
       a.cell_count = c->d_cell_count;
-      {
-        const int ng[3] = {f.nx, f.ny, f.nz};
-        int br = 2;
-        for (;;) {
-          size_t pts = 1;
-          for (int d = 0; d < 3; d++) {
-            a.ncell[d] = c->box.n[d];
-            a.brick[d] = std::min(br, c->box.n[d]);
-            a.nbrick[d] = (c->box.n[d] + a.brick[d] - 1) / a.brick[d];
-            a.cell_du[d] = c->realcell[d] * ng[d] / c->box.L[d];
-            double u = (c->box.origin[d] - 0.5 * c->box.L[d]) / c->box.L[d];
-            u -= std::floor(u);
-            a.cell_lo_u[d] = u * ng[d];
-            int t = (int)std::ceil(a.brick[d] * a.cell_du[d]) + f.order + 3;
-            a.tile[d] = std::min(t, ng[d]);
-            pts *= (size_t)a.tile[d];
-          }
-          if (pts * 8 <= 48 * 1024 || br == 1) break;
-          br = 1;
+      const int ng[3] = {f.nx, f.ny, f.nz};
+      int br = 2;
+      for (;;) {
+        size_t pts = 1;
+        for (int d = 0; d < 3; d++) {
+          a.ncell[d] = c->box.n[d];
+          a.brick[d] = std::min(br, c->box.n[d]);
+          a.nbrick[d] = (c->box.n[d] + a.brick[d] - 1) / a.brick[d];
+          a.cell_du[d] = c->realcell[d] * ng[d] / c->box.L[d];
+          double u = (c->box.origin[d] - 0.5 * c->box.L[d]) / c->box.L[d];
+          u -= std::floor(u);
+          a.cell_lo_u[d] = u * ng[d];
+          int t = (int)std::ceil(a.brick[d] * a.cell_du[d]) + f.order + 3;
+          a.tile[d] = std::min(t, ng[d]);
+          pts *= (size_t)a.tile[d];
         }
-        if (a.brick[0] * a.brick[1] * a.brick[2] > 64)
-          return fail(c, PM_ERR_UNSUPPORTED, "internal: spread brick of more than 64 cells");
-        if ((size_t)a.tile[0] * a.tile[1] * a.tile[2] * 8 > 64 * 1024)
-          return fail(c, PM_ERR_UNSUPPORTED, "PME grid too fine for the cell size: spread tile exceeds 64 KB of LDS (use a smaller cellsize)");
+        if (pts * 8 <= 48 * 1024 || br == 1) break;
+        br = 1;
       }
-      TimedScope ts(c, PM_T_PME);
-      if (!pm_launch_pme_plans(s, a, c->plans, c->d_slot_pos64, c->precision))
+      if (a.brick[0] * a.brick[1] * a.brick[2] > 64)
+        return fail(c, PM_ERR_UNSUPPORTED, "internal: spread brick of more than 64 cells");
+      if ((size_t)a.tile[0] * a.tile[1] * a.tile[2] * 8 > 64 * 1024)
+        return fail(c, PM_ERR_UNSUPPORTED, "PME grid too fine for the cell size: spread tile exceeds 64 KB of LDS (use a smaller cellsize)");
+      if (f.order != 4 && f.order != 6 && f.order != 8)
         return fail(c, PM_ERR_UNSUPPORTED, "PME order %d not implemented on the GPU", f.order);
-      off += pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision);
+      pme_launch.push_back(std::make_pair(a, hr.first));
     }
     if (f.corr) {
-      int ntab = (int)c->tab_a.size();
-      int nblk = (ntab + 255) / 256;
-      TimedScope ts(c, PM_T_CORR);
-      pm_launch_correction(s, c->box, n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls, f.alpha,
-                           c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
-                           c->d_epart + off * PM_NE, want_energy ? nblk : 0);
-      // point self energy and charged-system term (:604-628), cached per topology
+      // point self energy and charged-system term (:604-628)
       double q2 = 0.0, qs = 0.0;
       for (int i = 0; i < n; i++) q2 += c->q[i] * c->q[i];
       for (int i = 0; i < n; i++) qs += c->q[i];
@@ -966,6 +959,34 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
         if (vir) const_vir += ch;
       }
     }
+  }
+  if (want_energy && !c->d_ered &&
+      (r = dalloc(c, &c->d_ered, (size_t)pm_reduce_scratch_records((int)rec) * PM_NE)))
+    return r;
+
+  // fork: PME, correction and bonded terms on the second stream, the pair
+  // lists and pair kernels on the first; they write different force arrays
+  // (per atom, per slot) and different energy records
+  hipStream_t s2 = c->stream2;
+  const bool side = !pme_rec.empty() || do_bond || do_angle;
+  if (side) {
+    HIPCHECK(c, hipEventRecord(c->ev_fork, s));
+    HIPCHECK(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
+  }
+  for (auto& pl : pme_launch) {
+    TimedScope ts(c, PM_T_PME, s2);
+    pm_launch_pme_plans(s2, pl.first, c->plans, c->d_slot_pos64, c->precision);
+  }
+  for (auto& hr : pme_rec) {
+    ForceDesc& f = c->forces[hr.first];
+    if (!f.corr) continue;
+    long long off = hr.second + (f.recip ? pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision) : 0);
+    int ntab = (int)c->tab_a.size();
+    int nblk = (ntab + 255) / 256;
+    TimedScope ts(c, PM_T_CORR, s2);
+    pm_launch_correction(s2, c->box, n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls, f.alpha,
+                         c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
+                         c->d_epart + off * PM_NE, want_energy ? nblk : 0);
   }
   if (do_bond || do_angle) {
     PmBondedLaunch b{};
@@ -983,11 +1004,28 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     b.want_virial = vir ? 1 : 0;
     b.want_energy = want_energy ? 1 : 0;
     int rb, ra;
-    TimedScope ts(c, PM_T_BONDED);
-    pm_launch_bonded(s, c->box, b, &rb, &ra);
+    TimedScope ts(c, PM_T_BONDED, s2);
+    pm_launch_bonded(s2, c->box, b, &rb, &ra);
+  }
+  if (side) HIPCHECK(c, hipEventRecord(c->ev_join, s2));
+
+  for (auto& pl : pair_launch) {
+    {
+      TimedScope ts(c, PM_T_LIST);
+      pm_launch_list(s, c->box, pl.first, pl.second);
+    }
+    {
+      TimedScope ts(c, PM_T_PAIR);
+      pm_launch_pair(s, c->box, pl.first, pl.second, c->precision);
+    }
+  }
+  // join, then the per-slot pair forces into the per-atom forces
+  if (side) HIPCHECK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+  if (!passes.empty()) {
+    TimedScope ts(c, PM_T_BIN);
+    pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_force_slot, c->d_force);
   }
   if (want_energy) {
-    if (!c->d_ered && (r = dalloc(c, &c->d_ered, (size_t)pm_reduce_scratch_records((int)rec) * PM_NE))) return r;
     TimedScope ts(c, PM_T_REDUCE);
     pm_launch_reduce_energy(s, c->d_epart, (int)rec, const_coul, const_vir, c->d_energy, 1, c->d_ered);
   }
@@ -1080,6 +1118,16 @@ int pm_create(int device, pm_context** out) {
     delete c;
     return PM_ERR_DEVICE;
   }
+  // both streams at the default priority
+  if (hipStreamCreate(&c->stream2) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (c->stream2) hipStreamDestroy(c->stream2);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return PM_ERR_DEVICE;
+  }
   *out = c;
   return PM_OK;
 }
@@ -1088,9 +1136,13 @@ void pm_destroy(pm_context* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  hipStreamSynchronize(ctx->stream2);
   harvest_timing(ctx);
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   free_all(ctx);
+  hipEventDestroy(ctx->ev_fork);
+  hipEventDestroy(ctx->ev_join);
+  hipStreamDestroy(ctx->stream2);
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1580,6 +1632,7 @@ int pm_get_timing(pm_context* c, double* ms, long long* count, int ncat) {
   int r;
   if ((r = set_device(c))) return r;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream2));
   harvest_timing(c);
   for (int k = 0; k < ncat && k < PM_TIMING_CATEGORIES; k++) { ms[k] = c->t_ms[k]; count[k] = c->t_count[k]; }
   return PM_OK;
@@ -1611,6 +1664,7 @@ int pm_synchronize(pm_context* c) {
   int r;
   if ((r = set_device(c))) return r;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream2));
   return PM_OK;
 }
 

@@ -54,3 +54,26 @@ def exclusion_mix(s):
     s.ex_b = np.array(eb, np.int32)
     s.ex_cls = np.array(ec, np.int32)
     return s
+
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _mix64(z):
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def langevin_noise(seed, counter, n3):
+    """The Gaussian deviates of the device Langevin half kick number `counter`
+    (SplitMix64 of (seed, counter, index), Box-Muller), restated in numpy."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(n3, dtype=np.uint64)
+        key = np.uint64(counter) * np.uint64(0x100000001B3) + idx * np.uint64(2)
+        h1 = _mix64(np.uint64(seed) ^ _mix64(key))
+        h2 = _mix64(h1 ^ np.uint64(0x5851F42D4C957F2D))
+    u1 = ((h1 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740993.0)
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)

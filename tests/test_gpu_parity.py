@@ -230,3 +230,93 @@ def test_bonded(gpu_ctx_factory, port, precision):
     assert rms_rel(f, o["forces"]) < 1e-11
     np.testing.assert_allclose(e[2:4], o["energies"][2:4], rtol=1e-11)
     np.testing.assert_allclose(e[7:], o["energies"][7:], rtol=1e-9, atol=1e-9 * np.abs(o["energies"][7:]).max())
+
+
+def test_langevin_double(gpu_ctx_factory, port):
+    """Three Langevin leapfrog steps (StandardIntegrator::run with the BBK half
+    kick) against the oracle fed with the same counter-based deviates."""
+    say("case: langevin water648 double 3 steps")
+    from tests.cases import langevin_noise
+    import ctypes as C
+    s = water.named("water648", seed=9, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, "double")
+    hs = gpu_forces_c1(c)
+    dt, temp, gamma, seed = 0.5, 300.0, 5.0, 77
+    c.md_setup(hs, "langevin", dt=dt, temperature=temp, gamma=gamma, seed=seed)
+    c.md_set_state(s.pos, s.vel)
+    c.md_run(3)
+    xg, vg, _ = c.md_get_state()
+    lj, co = c1_terms()
+
+    def forces(xx):
+        s2 = s.copy()
+        s2.pos = xx
+        return port.cutoff(s2, lj, 8.0, 4.0)["forces"] + port.cutoff(s2, co, 8.0, 4.0)["forces"]
+
+    D = C.POINTER(C.c_double)
+    x = s.pos.copy()
+    v = s.vel.copy()
+    h = dt * water.INV_TIMEFACTOR
+
+    def kick(f, counter):
+        xi = np.ascontiguousarray(langevin_noise(seed, counter, 3 * s.n))
+        port.lib.orc_langevin_half_kick(s.n, C.c_double(dt), C.c_double(temp), C.c_double(gamma),
+                                        s.mass.ctypes.data_as(D), f.ctypes.data_as(D), xi.ctypes.data_as(D),
+                                        v.ctypes.data_as(D))
+
+    f = forces(x)
+    counter = 0
+    for _ in range(3):
+        kick(f, counter); counter += 1
+        x += h * v
+        f = forces(x)
+        kick(f, counter); counter += 1
+    say("max |dx|", np.abs(xg - x).max(), "max |dv|", np.abs(vg - v).max())
+    np.testing.assert_allclose(xg, x, atol=1e-10, rtol=0)
+    np.testing.assert_allclose(vg, v, atol=1e-9 * np.abs(v).max(), rtol=0)
+
+
+def test_langevin_temperature(gpu_ctx_factory):
+    """Langevin dynamics holds the kinetic temperature of 648 atoms at the bath
+    temperature (mean over 1500 steps after 1500 of equilibration, 2 %)."""
+    say("case: langevin temperature water648 mixed")
+    s = water.named("water648", seed=10, jitter=0.3)
+    s.vel = s.vel * 0.5          # start cold: the thermostat must heat the system
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, "mixed")
+    hs = gpu_forces_c1(c) + [c.force("Bond", []), c.force("Angle", [])]
+    c.md_setup(hs, "langevin", dt=1.0, temperature=300.0, gamma=20.0, seed=5)
+    c.md_set_state(s.pos, s.vel)
+    c.md_run(1500)
+    temps = []
+    for _ in range(30):
+        c.md_run(50)
+        _, v, _ = c.md_get_state()
+        ke = 0.5 * (s.mass[:, None] * v * v).sum()
+        temps.append(2.0 * ke / (3 * s.n * water.BOLTZMANN))
+    t = float(np.mean(temps))
+    say("mean kinetic temperature", t)
+    assert abs(t - 300.0) < 6.0
+
+
+def test_water1m_pme_mixed_accuracy(gpu_ctx_factory, port):
+    """999,999-atom water (215.4 A cube), LJ-C2 + Ewald-real rc 10 plus PME
+    240^3: RMS relative force error of the mixed path against the FP64 oracle,
+    and the in-cutoff pair count against the reference-order enumeration."""
+    say("case: water1m full force field mixed")
+    s = water.named("water1m", seed=1, jitter=0.3)
+    c = setup_ctx(gpu_ctx_factory, s, 5.0, "mixed")
+    h1 = c.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction Cutoff",
+                 [9.0, 10.0, ALPHA_RC10, 10.0, 10.0])
+    h2 = c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                 [240, 240, 240, 10.0, 4, 1e-6, -1.0])
+    f, e = c.evaluate([h1, h2], s.pos)
+    t = [po.term(po.POT_LJ, po.SWF_C2, 10.0, 9.0), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 10.0, alpha=ALPHA_RC10)]
+    a = port.cutoff_omp(s, t, 10.0, 5.0, threads=16)
+    b = port.pme(s, (240, 240, 240), 4, 10.0, 5.0, flags=6)
+    fr = a["forces"] + b["forces"]
+    err = rms_rel(f, fr)
+    say("rms rel force", err, "E", e[:2], (a["energies"] + b["energies"])[:2])
+    assert err < 5e-6
+    d = c.cell_diagnostics(s.pos, 10.0, 10.0)
+    say("pairs within the cutoff", int(d["inrc"].sum()))
+    assert 2.0e8 < int(d["inrc"].sum()) < 2.2e8
