@@ -173,8 +173,12 @@ int pm_get_timing(pm_context* ctx, double* ms, long long* count, int ncategories
  * they change speed, never results beyond the mixed-precision tolerance):
  *   PM_OPT_CLUSTER_SIZE  atoms per i-cluster of the pair kernels: 8, 16, or
  *                        0 for the default (8).
+ *   PM_OPT_GRAPH         1 (default): pm_md_run replays captured HIP graphs
+ *                        of up to 8 steps; 0: every kernel launched directly.
+ *                        Steps are captured only while no kernel timing is
+ *                        requested (pm_set_timing mask 0).
  * pm_get_option returns the value in effect (after the automatic choice). */
-enum { PM_OPT_CLUSTER_SIZE = 0 };
+enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

@@ -151,6 +151,14 @@ struct pm_context {
   double md_dt = 1.0, md_temp = 300.0, md_gamma = 0.0;
   unsigned long long md_seed = 0, md_counter = 0;
   bool md_forces_valid = false;
+  PmMdDevice* d_md = nullptr;        // noise counter and bath temperature read by the kernels
+
+  // captured MD steps (hipGraph), replayed while nothing they reference changes
+  int graph_option = 1;              // PM_OPT_GRAPH
+  unsigned long long gen = 1;        // bumped by every allocation and every setter
+  hipGraphExec_t md_exec = nullptr;
+  int md_exec_steps = 0, md_exec_kind = -1;
+  unsigned long long md_exec_gen = 0;
 
   long long last_stats[4] = {0, 0, 0, 0};
 
@@ -163,6 +171,8 @@ struct pm_context {
 };
 
 namespace {
+
+void drop_graph(pm_context* c);
 
 int fail(pm_context* c, int code, const char* fmt, ...) {
   char buf[1024];
@@ -184,6 +194,7 @@ int fail(pm_context* c, int code, const char* fmt, ...) {
 
 template <typename T>
 int dalloc(pm_context* c, T** p, size_t count) {
+  c->gen++;   // device pointers change: captured MD steps are stale
   if (*p) { hipFree(*p); *p = nullptr; }
   if (count == 0) count = 1;
   HIPCHECK(c, hipMalloc((void**)p, count * sizeof(T)));
@@ -1139,7 +1150,9 @@ void pm_destroy(pm_context* ctx) {
   hipStreamSynchronize(ctx->stream2);
   harvest_timing(ctx);
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+  drop_graph(ctx);
   free_all(ctx);
+  dfree(ctx->d_md);
   hipEventDestroy(ctx->ev_fork);
   hipEventDestroy(ctx->ev_join);
   hipStreamDestroy(ctx->stream2);
@@ -1153,6 +1166,7 @@ int pm_set_precision(pm_context* c, int precision) {
     return fail(c, PM_ERR_ARGUMENT, "precision must be PM_PRECISION_MIXED (0) or PM_PRECISION_DOUBLE (1)");
   c->precision = precision;
   c->md_forces_valid = false;
+  c->gen++;
   return PM_OK;
 }
 
@@ -1300,6 +1314,7 @@ int pm_set_topology(pm_context* c, int n, const double* q, const int* type, int 
   for (auto& f : c->forces) f.cached = false;
   c->have_topo = true;
   c->md_forces_valid = false;
+  c->gen++;
   return PM_OK;
 }
 
@@ -1351,6 +1366,7 @@ int pm_set_bonded(pm_context* c, int nb, const int* ba, const int* bb, const dou
   c->nbonds = nb;
   c->nangles = na;
   c->md_forces_valid = false;
+  c->gen++;
   return PM_OK;
 }
 
@@ -1411,6 +1427,7 @@ int pm_set_boundary(pm_context* c, const double* e1, const double* e2, const dou
   c->have_box = true;
   for (auto& f : c->forces) f.cached = false;
   c->md_forces_valid = false;
+  c->gen++;
   return PM_OK;
 }
 
@@ -1462,6 +1479,7 @@ int pm_evaluate(pm_context* c, const int* handles, int nh, const double* positio
   for (size_t k = 0; k < f.size(); k++) forces[k] += f[k];
   for (int k = 0; k < PM_NE; k++) energies[k] += e[k];
   c->md_forces_valid = false;
+  c->gen++;
   return PM_OK;
 }
 
@@ -1536,6 +1554,10 @@ int pm_md_setup(pm_context* c, const int* handles, int nh, int integrator, doubl
   c->md_seed = seed;
   c->md_counter = 0;
   c->md_forces_valid = false;
+  c->gen++;
+  if (!c->d_md && (r = dalloc(c, &c->d_md, 1))) return r;
+  PmMdDevice md{0ull, temperature};
+  HIPCHECK(c, hipMemcpy(c->d_md, &md, sizeof(md), hipMemcpyHostToDevice));
   return PM_OK;
 }
 
@@ -1550,6 +1572,95 @@ int pm_md_set_state(pm_context* c, const double* x, const double* v) {
   c->md_forces_valid = false;
   return PM_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// One MD step without its closing half kick, as queued on the context's
+// streams: kind 0 = a middle leapfrog step (LeapfrogIntegrator::doKickdoDrift,
+// then the forces), kind 1 = a whole Langevin step (StandardIntegrator::run:
+// half kick, drift, forces, half kick).
+int md_unit(pm_context* c, int kind, bool energy) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const double h = c->md_dt * kInvTimeFactor;
+  int r;
+  if (kind == 0) {
+    {
+      TimedScope ts(c, PM_T_INTEGRATE);
+      pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+    }
+    return evaluate_group(c, c->md_handles, 0, energy);
+  }
+  const double g = c->md_gamma / (1000 * kInvTimeFactor);
+  {
+    TimedScope ts(c, PM_T_INTEGRATE);
+    pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 0);
+    pm_launch_drift(s, 3 * n, h, c->d_vel, c->d_pos);
+  }
+  if ((r = evaluate_group(c, c->md_handles, 0, energy))) return r;
+  TimedScope ts(c, PM_T_INTEGRATE);
+  pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 1);
+  pm_launch_md_advance(s, c->d_md);
+  c->md_counter += 2;
+  return PM_OK;
+}
+
+void drop_graph(pm_context* c) {
+  if (c->md_exec) hipGraphExecDestroy(c->md_exec);
+  c->md_exec = nullptr;
+  c->md_exec_steps = 0;
+  c->md_exec_kind = -1;
+}
+
+// `count` units of `kind` without energies: replays of a captured graph of up
+// to 8 units where allowed (no per-kernel timing requested), the rest queued
+// directly.  The graph holds no memset or copy nodes: only kernels and the
+// fork/join of the second stream.
+int md_units(pm_context* c, int kind, int count) {
+  int r;
+  const int G = std::min(count, 8);
+  const bool use_graph = c->graph_option != 0 && c->timing_mask == 0 && count >= 2;
+  int done = 0;
+  if (use_graph) {
+    if (!c->md_exec || c->md_exec_gen != c->gen || c->md_exec_kind != kind || c->md_exec_steps != G) {
+      drop_graph(c);
+      // one direct unit first, so that every buffer and table exists before capture
+      if ((r = md_unit(c, kind, false))) return r;
+      done = 1;
+      const unsigned long long gen0 = c->gen;
+      hipGraph_t g = nullptr;
+      HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = PM_OK;
+      for (int u = 0; u < G && rc == PM_OK; u++) rc = md_unit(c, kind, false);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc != PM_OK) { if (g) hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return fail(c, PM_ERR_DEVICE, "HIP error %s ending the capture of an MD step", hipGetErrorString(e));
+      // the capture queued nothing: the units it recorded run when the graph is launched
+      c->md_counter -= (kind == 1) ? 2ull * G : 0ull;
+      e = hipGraphInstantiate(&c->md_exec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) { c->md_exec = nullptr; return fail(c, PM_ERR_DEVICE, "HIP error %s instantiating the MD step graph", hipGetErrorString(e)); }
+      if (c->gen != gen0) { drop_graph(c); return fail(c, PM_ERR_STATE, "internal: buffers changed while an MD step was captured"); }
+      c->md_exec_gen = c->gen;
+      c->md_exec_kind = kind;
+      c->md_exec_steps = G;
+    }
+    while (count - done >= G) {
+      HIPCHECK(c, hipGraphLaunch(c->md_exec, c->stream));
+      if (kind == 1) c->md_counter += 2ull * G;
+      done += G;
+    }
+  }
+  for (; done < count; done++)
+    if ((r = md_unit(c, kind, false))) return r;
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int pm_md_run(pm_context* c, int nsteps) {
   if (!c) return PM_ERR_ARGUMENT;
@@ -1566,30 +1677,22 @@ int pm_md_run(pm_context* c, int nsteps) {
     c->md_forces_valid = true;
   }
   if (c->md_integrator == PM_INTEGRATOR_LEAPFROG) {
-    // LeapfrogIntegrator::run (integrators/LeapfrogIntegrator.cpp:107-121)
+    // LeapfrogIntegrator::run (integrators/LeapfrogIntegrator.cpp:107-121):
+    // half kick + drift, forces, (kick + drift, forces) x (n - 1), half kick;
+    // energies of the last force evaluation only
     {
       TimedScope ts(c, PM_T_INTEGRATE);
       pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
     }
     if ((r = evaluate_group(c, c->md_handles, 0, nsteps == 1))) return r;
-    for (int i = 1; i < nsteps; i++) {
-      {
-        TimedScope ts(c, PM_T_INTEGRATE);
-        pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
-      }
-      if ((r = evaluate_group(c, c->md_handles, 0, i == nsteps - 1))) return r;
-    }
+    if (nsteps >= 3 && (r = md_units(c, 0, nsteps - 2))) return r;
+    if (nsteps >= 2 && (r = md_unit(c, 0, true))) return r;
     pm_launch_kick(s, n, 2, 0.5 * c->md_dt * kInvTimeFactor, c->d_mass, c->d_force, c->d_vel, c->d_pos);
   } else {
     // StandardIntegrator::run with the Langevin half kick
     // (StandardIntegrator.cpp:24-35, LangevinLeapfrogIntegrator.cpp:50-79)
-    const double g = c->md_gamma / (1000 * kInvTimeFactor);
-    for (int i = 0; i < nsteps; i++) {
-      pm_launch_langevin_kick(s, n, h, c->md_temp, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->md_counter++);
-      pm_launch_drift(s, 3 * n, h, c->d_vel, c->d_pos);
-      if ((r = evaluate_group(c, c->md_handles, 0, i == nsteps - 1))) return r;
-      pm_launch_langevin_kick(s, n, h, c->md_temp, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->md_counter++);
-    }
+    if (nsteps >= 2 && (r = md_units(c, 1, nsteps - 1))) return r;
+    if ((r = md_unit(c, 1, true))) return r;
   }
   HIPCHECK(c, hipGetLastError());
   return PM_OK;
@@ -1640,11 +1743,18 @@ int pm_get_timing(pm_context* c, double* ms, long long* count, int ncat) {
 
 int pm_set_option(pm_context* c, int option, int value) {
   if (!c) return PM_ERR_ARGUMENT;
+  c->gen++;
   if (option == PM_OPT_CLUSTER_SIZE) {
     if (value != 0 && value != 8 && value != 16)
       return fail(c, PM_ERR_ARGUMENT, "cluster size must be 0 (automatic), 8 or 16 (got %d)", value);
     c->m_override = value;
     c->md_forces_valid = false;
+    c->gen++;
+    return PM_OK;
+  }
+  if (option == PM_OPT_GRAPH) {
+    if (value != 0 && value != 1) return fail(c, PM_ERR_ARGUMENT, "graph option must be 0 or 1 (got %d)", value);
+    c->graph_option = value;
     return PM_OK;
   }
   return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);
@@ -1654,6 +1764,10 @@ int pm_get_option(pm_context* c, int option, int* value) {
   if (!c || !value) return PM_ERR_ARGUMENT;
   if (option == PM_OPT_CLUSTER_SIZE) {
     *value = c->have_topo && c->have_box ? choose_m(c) : c->m_override;
+    return PM_OK;
+  }
+  if (option == PM_OPT_GRAPH) {
+    *value = c->graph_option;
     return PM_OK;
   }
   return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);

@@ -208,9 +208,15 @@ void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* 
 void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
                     double* v, double* x);
-void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double temperature,
-                             double gamma_akma, const double* mass, const long long* f, double* v,
-                             unsigned long long seed, unsigned long long counter);
+// MD state the integrator kernels read from device memory
+struct PmMdDevice {
+  unsigned long long counter;   // Langevin noise counter (2 per step)
+  double temperature;           // Langevin bath temperature (K)
+};
+void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_akma, const double* mass,
+                             const long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
+                             int offset);
+void pm_launch_md_advance(hipStream_t s, PmMdDevice* md);
 void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x);
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation

@@ -220,11 +220,16 @@ __device__ inline double gauss(unsigned long long seed, unsigned long long count
 }
 
 // LangevinLeapfrogIntegrator::doHalfKick (integrators/LangevinLeapfrogIntegrator.cpp:58-79)
-__global__ void k_langevin_kick(int n, double dt, double temperature, double g, const double* __restrict__ mass,
+// The noise counter and the bath temperature come from device memory
+// (PmMdDevice), so a captured step replays with fresh deviates and a bath
+// temperature can be changed between replays.
+__global__ void k_langevin_kick(int n, double dt, double g, const double* __restrict__ mass,
                                 const long long* __restrict__ f, double* __restrict__ v,
-                                unsigned long long seed, unsigned long long counter) {
+                                unsigned long long seed, const PmMdDevice* __restrict__ md, int offset) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const unsigned long long counter = md->counter + (unsigned long long)offset;
+  const double temperature = md->temperature;
   const double fdt = (1.0 - exp(-0.5 * g * dt)) / g;
   const double vdt = exp(-0.5 * g * dt);
   const double ndt = sqrt((1.0 - exp(-g * dt)) / (2.0 * g));
@@ -236,6 +241,11 @@ __global__ void k_langevin_kick(int n, double dt, double temperature, double g, 
     const double xi = gauss(seed, counter, 3ull * i + k);
     v[3 * i + k] = v[3 * i + k] * vdt + F * fdt / m + xi * s * ndt;
   }
+}
+
+// advances the noise counter by the two half kicks of a step
+__global__ void k_md_advance(PmMdDevice* md) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) md->counter += 2;
 }
 
 __global__ void k_drift(int n3, double h, const double* __restrict__ v, double* __restrict__ x) {
@@ -299,12 +309,15 @@ void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass
                     double* v, double* x) {
   if (n > 0) hipLaunchKernelGGL(k_kick_drift, grid_for(n, 256), dim3(256), 0, s, n, mode, h, mass, f, v, x);
 }
-void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double temperature, double gamma_akma,
-                             const double* mass, const long long* f, double* v, unsigned long long seed,
-                             unsigned long long counter) {
+void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_akma, const double* mass,
+                             const long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
+                             int offset) {
   if (n > 0)
-    hipLaunchKernelGGL(k_langevin_kick, grid_for(n, 256), dim3(256), 0, s, n, dt_akma, temperature, gamma_akma,
-                       mass, f, v, seed, counter);
+    hipLaunchKernelGGL(k_langevin_kick, grid_for(n, 256), dim3(256), 0, s, n, dt_akma, gamma_akma, mass, f, v,
+                       seed, md, offset);
+}
+void pm_launch_md_advance(hipStream_t s, PmMdDevice* md) {
+  hipLaunchKernelGGL(k_md_advance, dim3(1), dim3(64), 0, s, md);
 }
 void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x) {
   if (n3 > 0) hipLaunchKernelGGL(k_drift, grid_for(n3, 256), dim3(256), 0, s, n3, h, v, x);

@@ -28,7 +28,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_evaluate", "pm_cell_diagnostics", "pm_md_setup", "pm_md_set_state", "pm_md_run",
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
            "pm_synchronize"]
-OPTIONS = {"cluster_size": 0}
+OPTIONS = {"cluster_size": 0, "graph": 1}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 
