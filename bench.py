@@ -122,6 +122,59 @@ def nve_drift(s, precision, steps=1000, chunk=100):
             "e_start": e[0], "e_end": e[-1], "samples": len(e)}
 
 
+def replica_bench(count=64, steps=100, segments=2, seed=0):
+    """Batched replicas: `count` water boxes of 100-1000 molecules at 0.0334
+    molecules/A^3, LJ + Coulomb C2 7 -> 8 A (no PME), bonds and angles, Langevin
+    at a geometric ladder 300 -> 450 K, an exchange attempt every `steps` steps.
+    Replica-steps/s of the batch (one captured graph for all replicas) against
+    the same steps launched replica after replica."""
+    from protomol_amd import gpu, water
+    systems = water.replica_boxes(count, seed=seed)
+    temps = water.replica_temperatures(count)
+    ctxs = []
+    for k, (s, t) in enumerate(zip(systems, temps)):
+        c = gpu.Context(0, "mixed")
+        c.set_topology(s)
+        c.set_boundary(s.box, s.origin, 4.0)
+        hs = [c.force("LennardJones -algorithm NonbondedCutoff -switchingFunction C2", [7.0, 8.0, 8.0]),
+              c.force("Coulomb -algorithm NonbondedCutoff -switchingFunction C2", [7.0, 8.0, 8.0]),
+              c.force("Bond", []), c.force("Angle", [])]
+        c.md_setup(hs, "langevin", dt=DT_FS, temperature=float(t), gamma=5.0, seed=k + 1)
+        c.md_set_state(s.pos, s.vel)
+        ctxs.append(c)
+    b = gpu.Batch(ctxs)
+    b.run(10)
+    for c in ctxs:
+        c.synchronize()
+    accepted = 0
+    t0 = time.perf_counter()
+    for seg in range(segments):
+        b.run(steps)
+        acc, _ = b.exchange(seg, 12345)
+        accepted += int((acc == 1).sum())
+    for c in ctxs:
+        c.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    for c in ctxs:
+        c.set_option("graph", 0)
+    t0 = time.perf_counter()
+    for seg in range(segments):
+        for c in ctxs:
+            c.md_run(steps)
+    for c in ctxs:
+        c.synchronize()
+    t_seq = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    rs = count * steps * segments
+    return {"replicas": count, "atoms": int(sum(s.n for s in systems)),
+            "atoms_min": int(min(s.n for s in systems)), "atoms_max": int(max(s.n for s in systems)),
+            "steps": steps * segments, "exchange_attempts": segments, "accepted_swaps": accepted,
+            "replica_steps_per_s": rs / t_batch, "sequential_replica_steps_per_s": rs / t_seq,
+            "speedup_vs_sequential": t_seq / t_batch}
+
+
 def cpu_baseline(s, alpha):
     """The reference's own CPU path (oracle/_ref, compiled from the ProtoMol
     sources) timed for one force evaluation of the same force set on one host
@@ -164,6 +217,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-drift", action="store_true", help="skip the 1000-step NVE drift runs")
+    ap.add_argument("--no-replicas", action="store_true", help="skip the batched replica-exchange measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {args.gpus} is not run "
@@ -226,6 +280,8 @@ def main():
                          "bond": float(mixed["energies"][2]), "angle": float(mixed["energies"][3])},
         "list_lanes_per_pair": mixed["stats"]["lanes"] / max(mixed["pairs"], 1),
     }
+    if not args.no_replicas:
+        line["replica_exchange"] = replica_bench()
     if not args.no_drift:
         line["nve_drift"] = {"mixed": nve_drift(s, "mixed"), "fp64": nve_drift(s, "double")}
     if not args.no_cpu_baseline:

@@ -185,6 +185,30 @@ int pm_get_option(pm_context* ctx, int option, int* value);
 /* Wait for all work queued by this context. */
 int pm_synchronize(pm_context* ctx);
 
+/* Batches of independent systems (replicas), each in its own context set up
+ * for Langevin dynamics (pm_md_setup with PM_INTEGRATOR_LANGEVIN), advanced
+ * together: their steps are captured into one HIP graph (every replica on its
+ * own streams) and replayed.  Replica exchange follows
+ * applications/rem-app/rem.cpp: neighbours (k, k+1) of the temperature ladder
+ * with k of the attempt's parity (getNextAddress :949-957) are tested with
+ * delta = (1/(kB Tk) - 1/(kB Tk+1)) (E(k+1) - E(k)) (calcDelta :941-946) and
+ * accepted when delta <= 0 or u <= exp(-delta) (canDoEvenSwitch :841-880);
+ * an accepted pair swaps bath temperatures and the velocities are scaled by
+ * sqrt(T_new / T_old).  E is the potential energy of the last step.  No
+ * reference counterpart for the batch itself: the reference runs one replica
+ * per MPI process. */
+typedef struct pm_batch pm_batch;
+int pm_batch_create(pm_context* const* contexts, int nreplicas, pm_batch** out);
+void pm_batch_destroy(pm_batch* batch);
+const char* pm_batch_last_error(const pm_batch* batch);
+/* nsteps Langevin steps of every replica (energies of the last one). */
+int pm_batch_run(pm_batch* batch, int nsteps);
+/* One exchange attempt; u from (seed, attempt, k).  accepted[k] and deltas[k]
+ * (nreplicas entries each, may be NULL) are written for the tested pairs. */
+int pm_batch_exchange(pm_batch* batch, int attempt, unsigned long long seed, int* accepted, double* deltas);
+/* ladder[k]: the replica now at ladder temperature temps[k]. */
+int pm_batch_ladder(pm_batch* batch, int* ladder, double* temps);
+
 #ifdef __cplusplus
 }
 #endif

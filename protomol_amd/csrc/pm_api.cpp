@@ -979,7 +979,10 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   // lists and pair kernels on the first; they write different force arrays
   // (per atom, per slot) and different energy records
   hipStream_t s2 = c->stream2;
-  const bool side = !pme_rec.empty() || do_bond || do_angle;
+  // the second stream pays off beside the PME kernels; bonded terms alone
+  // stay on the first
+  const bool side = !pme_rec.empty();
+  if (!side) s2 = s;
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_fork, s));
     HIPCHECK(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
@@ -1779,6 +1782,227 @@ int pm_synchronize(pm_context* c) {
   if ((r = set_device(c))) return r;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream2));
+  return PM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+// Many independent systems (replicas), each in its own context, advanced
+// together: their MD steps are captured into one HIP graph (every replica on
+// its own streams, joined on the batch's root stream) and replayed; replica
+// exchange swaps bath temperatures between neighbours of a temperature ladder.
+struct pm_batch {
+  std::vector<pm_context*> ctx;
+  hipStream_t root = nullptr;
+  hipEvent_t ev_root = nullptr;
+  std::vector<hipEvent_t> ev_done;
+  hipGraphExec_t exec = nullptr;
+  int exec_steps = 0;
+  std::vector<unsigned long long> exec_gen;
+  std::vector<int> ladder;    // ladder[k]: replica holding temperature k of the ladder
+  std::vector<double> temps;  // the ladder temperatures, as given at creation
+  std::string err;
+};
+
+namespace {
+
+int bfail(pm_batch* b, int code, const char* msg, const pm_context* c = nullptr) {
+  b->err = msg;
+  if (c && !c->err.empty()) b->err += std::string(": ") + c->err;
+  return code;
+}
+
+void batch_drop_graph(pm_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  b->exec_steps = 0;
+}
+
+// Langevin steps without energies for every replica: each replica's steps
+// form one chain on its own stream, forked from and joined to the batch's
+// root stream once per call
+int batch_units(pm_batch* b, int steps) {
+  if (hipEventRecord(b->ev_root, b->root) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "event record failed");
+  for (size_t k = 0; k < b->ctx.size(); k++) {
+    pm_context* c = b->ctx[k];
+    if (hipStreamWaitEvent(c->stream, b->ev_root, 0) != hipSuccess)
+      return bfail(b, PM_ERR_DEVICE, "stream wait failed");
+    for (int u = 0; u < steps; u++) {
+      int r = md_unit(c, 1, false);
+      if (r) return bfail(b, r, "replica step failed", c);
+    }
+    if (hipEventRecord(b->ev_done[k], c->stream) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "event record failed");
+    if (hipStreamWaitEvent(b->root, b->ev_done[k], 0) != hipSuccess)
+      return bfail(b, PM_ERR_DEVICE, "stream wait failed");
+  }
+  return PM_OK;
+}
+
+// uniform deviate in [0, 1) of exchange attempt `attempt`, ladder pair `k`
+double rem_uniform(unsigned long long seed, unsigned long long attempt, unsigned long long k) {
+  auto mix = [](unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  unsigned long long h = mix(seed ^ mix(attempt * 0x100000001B3ull + 2 * k + 1));
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_batch_create(pm_context* const* contexts, int nrep, pm_batch** out) {
+  if (!contexts || nrep <= 0 || !out) return PM_ERR_ARGUMENT;
+  *out = nullptr;
+  for (int k = 0; k < nrep; k++) {
+    pm_context* c = contexts[k];
+    if (!c) return PM_ERR_ARGUMENT;
+    if (c->md_handles.empty() || c->md_integrator != PM_INTEGRATOR_LANGEVIN) return PM_ERR_STATE;
+    if (c->device != contexts[0]->device) return PM_ERR_ARGUMENT;
+  }
+  pm_batch* b = new pm_batch();
+  b->ctx.assign(contexts, contexts + nrep);
+  if (hipSetDevice(contexts[0]->device) != hipSuccess || hipStreamCreate(&b->root) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_root, hipEventDisableTiming) != hipSuccess) {
+    delete b;
+    return PM_ERR_DEVICE;
+  }
+  b->ev_done.resize(nrep, nullptr);
+  for (int k = 0; k < nrep; k++) hipEventCreateWithFlags(&b->ev_done[k], hipEventDisableTiming);
+  for (int k = 0; k < nrep; k++) {
+    b->ladder.push_back(k);
+    b->temps.push_back(contexts[k]->md_temp);
+  }
+  *out = b;
+  return PM_OK;
+}
+
+void pm_batch_destroy(pm_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->ctx[0]->device);
+  hipStreamSynchronize(b->root);
+  for (pm_context* c : b->ctx) hipStreamSynchronize(c->stream);
+  batch_drop_graph(b);
+  for (hipEvent_t e : b->ev_done) hipEventDestroy(e);
+  hipEventDestroy(b->ev_root);
+  hipStreamDestroy(b->root);
+  delete b;
+}
+
+const char* pm_batch_last_error(const pm_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int pm_batch_run(pm_batch* b, int nsteps) {
+  if (!b) return PM_ERR_ARGUMENT;
+  if (nsteps < 1) return PM_OK;
+  if (hipSetDevice(b->ctx[0]->device) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "hipSetDevice failed");
+  int r;
+  for (pm_context* c : b->ctx) {
+    if (c->timing_mask) return bfail(b, PM_ERR_STATE, "kernel timing is on in a replica context");
+    if (!c->md_forces_valid) {
+      if ((r = evaluate_group(c, c->md_handles, 0, false))) return bfail(b, r, "initial forces failed", c);
+      c->md_forces_valid = true;
+    }
+  }
+  // every step but the last from a captured graph of up to 8 steps
+  const int count = nsteps - 1;
+  int done = 0;
+  const int G = std::min(count, 8);
+  if (count >= 2) {
+    bool stale = !b->exec || b->exec_steps != G || b->exec_gen.size() != b->ctx.size();
+    for (size_t k = 0; !stale && k < b->ctx.size(); k++) stale = b->exec_gen[k] != b->ctx[k]->gen;
+    if (stale) {
+      batch_drop_graph(b);
+      if ((r = batch_units(b, 1))) return r;   // every buffer exists before the capture
+      done = 1;
+      std::vector<unsigned long long> gen0;
+      for (pm_context* c : b->ctx) gen0.push_back(c->gen);
+      hipGraph_t g = nullptr;
+      if (hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bfail(b, PM_ERR_DEVICE, "hipStreamBeginCapture failed");
+      int rc = batch_units(b, G);
+      hipError_t e = hipStreamEndCapture(b->root, &g);
+      // the capture queued nothing: its steps run when the graph is launched
+      for (pm_context* c : b->ctx) c->md_counter -= 2ull * G;
+      if (rc) { if (g) hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return bfail(b, PM_ERR_DEVICE, "hipStreamEndCapture failed");
+      e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) { b->exec = nullptr; return bfail(b, PM_ERR_DEVICE, "hipGraphInstantiate failed"); }
+      for (size_t k = 0; k < b->ctx.size(); k++)
+        if (b->ctx[k]->gen != gen0[k]) { batch_drop_graph(b); return bfail(b, PM_ERR_STATE, "internal: buffers changed during capture"); }
+      b->exec_steps = G;
+      b->exec_gen = gen0;
+    }
+    while (count - done >= G) {
+      if (hipGraphLaunch(b->exec, b->root) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "hipGraphLaunch failed");
+      for (pm_context* c : b->ctx) c->md_counter += 2ull * G;
+      done += G;
+    }
+  }
+  if (done < count && (r = batch_units(b, count - done))) return r;
+  // the last step with energies
+  if (hipEventRecord(b->ev_root, b->root) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "event record failed");
+  for (pm_context* c : b->ctx) {
+    if (hipStreamWaitEvent(c->stream, b->ev_root, 0) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "stream wait failed");
+    if ((r = md_unit(c, 1, true))) return bfail(b, r, "replica step failed", c);
+  }
+  return PM_OK;
+}
+
+int pm_batch_exchange(pm_batch* b, int attempt, unsigned long long seed, int* accepted, double* deltas) {
+  if (!b || attempt < 0) return PM_ERR_ARGUMENT;
+  const int R = (int)b->ctx.size();
+  if (hipSetDevice(b->ctx[0]->device) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "hipSetDevice failed");
+  // potential energy of every replica at its last step (ScalarStructure::potentialEnergy)
+  std::vector<double> epot(R, 0.0);
+  for (int k = 0; k < R; k++) {
+    pm_context* c = b->ctx[k];
+    double e[PM_NE];
+    if (hipMemcpyAsync(e, c->d_energy, sizeof(e), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return bfail(b, PM_ERR_DEVICE, "energy copy failed");
+    for (int u = 0; u < PM_E_VIR; u++) epot[k] += e[u];
+  }
+  // neighbour pairs (k, k+1) of the ladder with k of the attempt's parity
+  // (getNextAddress, applications/rem-app/rem.cpp:949-957); the Metropolis
+  // test on calcDelta (:941-946, canDoEvenSwitch :841-880) swaps the two
+  // replicas' bath temperatures, and their velocities follow by
+  // sqrt(T_new / T_old) (doSwitch's rescaling branch, :930-935)
+  const double kb = 0.001987191;   // Constant::BOLTZMANN (base/pmconstants.cpp:74)
+  for (int k = attempt & 1; k + 1 < R; k += 2) {
+    const int ra = b->ladder[k], rb = b->ladder[k + 1];
+    const double ta = b->temps[k], tb = b->temps[k + 1];
+    const double delta = (1 / kb / ta - 1 / kb / tb) * (epot[rb] - epot[ra]);
+    const double w = delta <= 0 ? 1.0 : std::exp(-delta);
+    const bool acc = rem_uniform(seed, (unsigned long long)attempt, (unsigned long long)k) <= w;
+    if (deltas) deltas[k] = delta;
+    if (accepted) accepted[k] = acc ? 1 : 0;
+    if (!acc) continue;
+    std::swap(b->ladder[k], b->ladder[k + 1]);
+    const int repl[2] = {ra, rb};
+    const double tnew[2] = {tb, ta}, told[2] = {ta, tb};
+    for (int u = 0; u < 2; u++) {
+      pm_context* c = b->ctx[repl[u]];
+      c->md_temp = tnew[u];
+      if (hipMemcpyAsync(&c->d_md->temperature, &c->md_temp, sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return bfail(b, PM_ERR_DEVICE, "temperature copy failed");
+      pm_launch_scale(c->stream, 3 * c->n, std::sqrt(tnew[u] / told[u]), c->d_vel);
+      if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "exchange failed");
+    }
+  }
+  return PM_OK;
+}
+
+int pm_batch_ladder(pm_batch* b, int* ladder, double* temps) {
+  if (!b) return PM_ERR_ARGUMENT;
+  for (size_t k = 0; k < b->ctx.size(); k++) {
+    if (ladder) ladder[k] = b->ladder[k];
+    if (temps) temps[k] = b->temps[k];
+  }
   return PM_OK;
 }
 

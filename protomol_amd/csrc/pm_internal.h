@@ -217,6 +217,7 @@ void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_
                              const long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
                              int offset);
 void pm_launch_md_advance(hipStream_t s, PmMdDevice* md);
+void pm_launch_scale(hipStream_t s, int n3, double factor, double* v);
 void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x);
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation

@@ -243,6 +243,12 @@ __global__ void k_langevin_kick(int n, double dt, double g, const double* __rest
   }
 }
 
+// v *= factor (velocities carried to a new bath temperature)
+__global__ void k_scale(int n3, double factor, double* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) v[i] *= factor;
+}
+
 // advances the noise counter by the two half kicks of a step
 __global__ void k_md_advance(PmMdDevice* md) {
   if (threadIdx.x == 0 && blockIdx.x == 0) md->counter += 2;
@@ -315,6 +321,9 @@ void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_
   if (n > 0)
     hipLaunchKernelGGL(k_langevin_kick, grid_for(n, 256), dim3(256), 0, s, n, dt_akma, gamma_akma, mass, f, v,
                        seed, md, offset);
+}
+void pm_launch_scale(hipStream_t s, int n3, double factor, double* v) {
+  if (n3 > 0) hipLaunchKernelGGL(k_scale, grid_for(n3, 256), dim3(256), 0, s, n3, factor, v);
 }
 void pm_launch_md_advance(hipStream_t s, PmMdDevice* md) {
   hipLaunchKernelGGL(k_md_advance, dim3(1), dim3(64), 0, s, md);

@@ -27,7 +27,8 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_set_topology", "pm_set_bonded", "pm_set_boundary", "pm_force_create", "pm_force_uncache",
            "pm_evaluate", "pm_cell_diagnostics", "pm_md_setup", "pm_md_set_state", "pm_md_run",
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
-           "pm_synchronize"]
+           "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
+           "pm_batch_exchange", "pm_batch_ladder"]
 OPTIONS = {"cluster_size": 0, "graph": 1}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
@@ -77,6 +78,14 @@ def load_library(path: str = LIB_PATH):
     lib.pm_set_option.argtypes = [P, C.c_int, C.c_int]
     lib.pm_get_option.argtypes = [P, C.c_int, I]
     lib.pm_synchronize.argtypes = [P]
+    lib.pm_batch_create.argtypes = [C.POINTER(P), C.c_int, C.POINTER(P)]
+    lib.pm_batch_destroy.argtypes = [P]
+    lib.pm_batch_destroy.restype = None
+    lib.pm_batch_last_error.argtypes = [P]
+    lib.pm_batch_last_error.restype = C.c_char_p
+    lib.pm_batch_run.argtypes = [P, C.c_int]
+    lib.pm_batch_exchange.argtypes = [P, C.c_int, C.c_ulonglong, I, D]
+    lib.pm_batch_ladder.argtypes = [P, I, D]
     _lib = lib
     return lib
 
@@ -236,3 +245,49 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.pm_synchronize(self.h))
+
+
+class Batch:
+    """Replicas advanced together (pm_batch_*): contexts set up for Langevin MD."""
+
+    def __init__(self, contexts):
+        self.lib = load_library()
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        h = C.c_void_p()
+        rc = self.lib.pm_batch_create(arr, len(self.contexts), C.byref(h))
+        if rc != PM_OK:
+            raise PmError(rc, "pm_batch_create failed (every context needs pm_md_setup with Langevin)")
+        self.h = h
+
+    def _check(self, rc):
+        if rc != PM_OK:
+            raise PmError(rc, self.lib.pm_batch_last_error(self.h).decode())
+
+    def run(self, nsteps: int):
+        self._check(self.lib.pm_batch_run(self.h, int(nsteps)))
+
+    def exchange(self, attempt: int, seed: int):
+        n = len(self.contexts)
+        acc = np.full(n, -1, np.int32)
+        dl = np.full(n, np.nan)
+        self._check(self.lib.pm_batch_exchange(self.h, int(attempt), int(seed), _ip(acc), _dp(dl)))
+        return acc, dl
+
+    def ladder(self):
+        n = len(self.contexts)
+        lad = np.zeros(n, np.int32)
+        t = np.zeros(n)
+        self._check(self.lib.pm_batch_ladder(self.h, _ip(lad), _dp(t)))
+        return lad, t
+
+    def close(self):
+        if self.h:
+            self.lib.pm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -194,3 +194,17 @@ def replica_sizes(count: int = 64, seed: int = 0, lo: int = 100, hi: int = 1000)
     n = rng.integers(lo, hi + 1, size=count)
     edges = (n / 0.0334) ** (1.0 / 3.0)
     return n, edges
+
+
+def replica_temperatures(count: int = 64, lo: float = 300.0, hi: float = 450.0):
+    """Geometric temperature ladder lo * (hi/lo)^(k/(count-1))."""
+    if count == 1:
+        return np.array([lo])
+    return lo * (hi / lo) ** (np.arange(count) / (count - 1))
+
+
+def replica_boxes(count: int = 64, seed: int = 0, lo: int = 100, hi: int = 1000, jitter: float = 0.2):
+    """The uneven replica workload: `count` water boxes of seeded sizes (lo..hi
+    molecules) at 0.0334 molecules/A^3, each generated from its own seed."""
+    n, edges = replica_sizes(count, seed, lo, hi)
+    return [water_box(int(m), float(L), seed=1000 * seed + k, jitter=jitter) for k, (m, L) in enumerate(zip(n, edges))]

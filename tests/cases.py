@@ -77,3 +77,22 @@ def langevin_noise(seed, counter, n3):
     u1 = ((h1 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740993.0)
     u2 = (h2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)
+
+
+def rem_uniform(seed, attempt, k):
+    """The exchange test's uniform deviate for (seed, attempt, ladder pair k)."""
+    with np.errstate(over="ignore"):
+        h = _mix64(np.uint64(seed) ^ _mix64(np.uint64(attempt) * np.uint64(0x100000001B3) + np.uint64(2 * k + 1)))
+    return float(h >> np.uint64(11)) * (1.0 / 9007199254740992.0)
+
+
+def replica_context(make, s, temperature, seed, precision="mixed"):
+    """A context for one replica of the batched workload: LJ and Coulomb with C2
+    switching 7 -> 8 A (one fused pass), bonds, angles, Langevin at `temperature`."""
+    c = make(precision)
+    c.set_topology(s)
+    c.set_boundary(s.box, s.origin, 4.0)
+    hs = gpu_forces_c1(c) + [c.force("Bond", []), c.force("Angle", [])]
+    c.md_setup(hs, "langevin", dt=1.0, temperature=float(temperature), gamma=5.0, seed=seed)
+    c.md_set_state(s.pos, s.vel)
+    return c

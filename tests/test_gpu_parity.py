@@ -343,3 +343,65 @@ def test_graph_replay_matches_direct(gpu_ctx_factory, integrator):
         c.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_batch_matches_single_replicas(gpu_ctx_factory):
+    """Replicas of uneven size advanced as one batch (captured graph of all
+    replicas) follow the same trajectories, bit for bit, as when each is run alone."""
+    say("case: batch of 5 uneven replicas vs single runs, 23 steps")
+    from tests.cases import replica_context
+    from protomol_amd import gpu
+    systems = water.replica_boxes(5, seed=3, lo=100, hi=300)
+    temps = water.replica_temperatures(5)
+    ctxs = [replica_context(gpu_ctx_factory, s, t, seed=k + 1) for k, (s, t) in enumerate(zip(systems, temps))]
+    b = gpu.Batch(ctxs)
+    b.run(11)
+    b.run(12)
+    got = [c.md_get_state() for c in ctxs]
+    b.close()
+    for k, (s, t) in enumerate(zip(systems, temps)):
+        c = replica_context(gpu_ctx_factory, s, t, seed=k + 1)
+        c.set_option("graph", 0)
+        c.md_run(11)
+        c.md_run(12)
+        ref = c.md_get_state()
+        for a, r in zip(got[k], ref):
+            np.testing.assert_array_equal(a, r)
+
+
+def test_batch_exchange_rule(gpu_ctx_factory, port):
+    """Replica-exchange attempts: deltas from calcDelta (oracle), Metropolis
+    acceptance on the seeded uniform, swapped temperatures and scaled velocities."""
+    say("case: batch exchange rule, 6 replicas")
+    from tests.cases import replica_context, rem_uniform
+    from protomol_amd import gpu
+    systems = water.replica_boxes(6, seed=4, lo=100, hi=160)
+    temps = water.replica_temperatures(6, 300.0, 330.0)
+    ctxs = [replica_context(gpu_ctx_factory, s, t, seed=k + 7) for k, (s, t) in enumerate(zip(systems, temps))]
+    b = gpu.Batch(ctxs)
+    seed = 99
+    accepted_any = False
+    for attempt in range(6):
+        b.run(20)
+        lad0, tl = b.ladder()
+        states = [c.md_get_state() for c in ctxs]
+        epot = [float(st[2][:7].sum()) for st in states]
+        acc, dl = b.exchange(attempt, seed)
+        lad1, _ = b.ladder()
+        exp_lad = list(lad0)
+        for k in range(attempt % 2, len(ctxs) - 1, 2):
+            ra, rb = lad0[k], lad0[k + 1]
+            d = port.rem_delta(tl[k], tl[k + 1], epot[ra], epot[rb])
+            assert dl[k] == pytest.approx(d, rel=1e-12, abs=1e-12)
+            w = 1.0 if d <= 0 else np.exp(-d)
+            want = rem_uniform(seed, attempt, k) <= w
+            assert bool(acc[k]) == want
+            if want:
+                accepted_any = True
+                exp_lad[k], exp_lad[k + 1] = exp_lad[k + 1], exp_lad[k]
+                va = ctxs[ra].md_get_state()[1]
+                np.testing.assert_allclose(va, states[ra][1] * np.sqrt(tl[k + 1] / tl[k]), rtol=1e-14)
+        assert list(lad1) == exp_lad
+    b.close()
+    say("accepted at least one swap:", accepted_any)
+    assert accepted_any
