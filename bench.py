@@ -95,30 +95,36 @@ def run_md(s, precision, steps, warmup, timing=()):
     return dict(seconds=t1 - t0, timing=tim, pairs=pairs, energies=e, x=x, v=v, stats=stats)
 
 
-def nve_drift(s, precision, steps=1000, chunk=100):
-    """Total-energy drift of a leapfrog NVE run: E = kinetic (synchronous
-    velocities after the closing half kick) + potential, sampled every `chunk`
-    steps; the least-squares slope in kcal/mol per ns and per degree of freedom."""
-    from protomol_amd import gpu, water
+def nve_drift(s, precision, steps=1000, chunk=100, equilibrate=2000):
+    """Total-energy drift of a leapfrog NVE run after `equilibrate` Langevin
+    steps at 300 K (the lattice start relaxes violently over the first
+    picosecond): E = kinetic (synchronous velocities after the closing half
+    kick) + potential, sampled every `chunk` steps; the least-squares slope in
+    kcal/mol per ns and per degree of freedom, and the RMS fluctuation."""
+    from protomol_amd import gpu
     ctx = gpu.Context(0, precision)
     ctx.set_topology(s)
     ctx.set_boundary(s.box, s.origin, CELL)
     hs = make_forces(ctx, pme_alpha(RC, 1e-6))
-    ctx.md_setup(hs, "leapfrog", dt=DT_FS)
+    ctx.md_setup(hs, "langevin", dt=DT_FS, temperature=300.0, gamma=5.0, seed=2024)
     ctx.md_set_state(s.pos, s.vel)
+    ctx.md_run(equilibrate)
+    x, v, _ = ctx.md_get_state()
+    ctx.md_setup(hs, "leapfrog", dt=DT_FS)
+    ctx.md_set_state(x, v)
     t, e = [], []
     for k in range(steps // chunk + 1):
-        if k:
-            ctx.md_run(chunk)
-        else:
-            ctx.md_run(1)   # energies of the starting configuration (one step in)
+        ctx.md_run(chunk if k else 1)
         x, v, en = ctx.md_get_state()
         ke = 0.5 * float((s.mass[:, None] * v * v).sum())
         t.append((k * chunk + (0 if k else 1)) * DT_FS * 1e-6)   # ns
         e.append(ke + float(en[0] + en[1] + en[2] + en[3]))
     ctx.close()
-    slope = float(np.polyfit(np.array(t), np.array(e), 1)[0])
-    return {"steps": steps, "kcal_mol_per_ns_per_dof": slope / (3 * s.n),
+    coef = np.polyfit(np.array(t), np.array(e), 1)
+    resid = np.array(e) - np.polyval(coef, np.array(t))
+    return {"steps": steps, "equilibration_steps": equilibrate,
+            "kcal_mol_per_ns_per_dof": float(coef[0]) / (3 * s.n),
+            "rms_fluctuation_kcal_mol": float(np.sqrt((resid ** 2).mean())),
             "e_start": e[0], "e_end": e[-1], "samples": len(e)}
 
 

@@ -87,6 +87,16 @@ int pm_set_topology(pm_context* ctx, int n, const double* scaled_charge, const i
                     const int* ex_a, const int* ex_b, const int* ex_class,
                     double coulomb_scaling_factor);
 
+/* Per-atom Lennard-Jones parameters in place of the type table: sigma (A) and
+ * epsilon (kcal/mol) of every atom, and their 1-4 values (NULL: the same).
+ * Pairs combine them as frontend/buildTopology.cpp:643-664 combines the type
+ * parameters: R = sigma 2^(-5/6), A = (Ri+Rj)^12 sqrt(eps_i eps_j),
+ * B = 2 (Ri+Rj)^6 sqrt(eps_i eps_j).  sigma == NULL returns to the type
+ * table.  Needed for gradients with respect to per-atom sigma and epsilon
+ * (pm_param_gradients); the reference's own route is one atom type per atom. */
+int pm_set_lj_per_atom(pm_context* ctx, const double* sigma, const double* epsilon, const double* sigma14,
+                       const double* epsilon14);
+
 /* Bonded terms of the topology (frontend/buildTopology.cpp fills
  * GenericTopology::bonds and ::angles from PSF/PAR): bond atoms, spring
  * constant (kcal/mol/A^2), rest length (A); angle atoms (a2 is the vertex),
@@ -168,6 +178,48 @@ int pm_last_stats(pm_context* ctx, long long* stats, int nstats);
  * totals; pm_get_timing returns total milliseconds and launch counts. */
 int pm_set_timing(pm_context* ctx, unsigned mask);
 int pm_get_timing(pm_context* ctx, double* ms, long long* count, int ncategories);
+
+/* Hessian-vector product of the energy of a force group, hv = H v (double[3n],
+ * overwritten), at the given positions: the nonbonded cutoff pairs (the
+ * derivative of exactly the forces pm_evaluate computes, switching included),
+ * the PME reciprocal term (B-spline spread of the directional charge
+ * derivative, a second convolution, second-derivative gather) and the PME
+ * correction term.  Bond and Angle forces are rejected.  Computed by an
+ * adjoint kernel that re-evaluates every pair, storing no Hessian.  The
+ * reference assembles a dense Hessian for LJ and Coulomb cutoff forces only
+ * (integrators/Hessian.cpp:456-624, forces/ReducedHessLennardJones.cpp,
+ * ReducedHessCoulomb.cpp). */
+int pm_hessian_vec(pm_context* ctx, const int* handles, int nhandles, const double* positions, const double* v,
+                   double* hv);
+/* Gradients of the energy of a force group with respect to every atom's scaled
+ * charge (dE_dq, double[n]), and, with per-atom LJ parameters set, its sigma and
+ * epsilon (dE_dsigma, dE_depsilon; pairs that are not MODIFIED; 0 where
+ * epsilon is 0).  Any output may be NULL.  No reference counterpart. */
+int pm_param_gradients(pm_context* ctx, const int* handles, int nhandles, const double* positions, double* dE_dq,
+                       double* dE_dsigma, double* dE_depsilon);
+
+/* Replace every atom's scaled charge (double[n], the same order and scaling as
+ * pm_set_topology), keeping types, exclusions and caches.  The reference
+ * changes charges by writing Atom::scaledCharge in the topology
+ * (base/Atom.h:79); any later evaluation sees the new values. */
+int pm_set_charges(pm_context* ctx, const double* scaled_charge);
+
+/* Lanczos iteration with full reorthogonalisation on the Hessian of a force
+ * group (the same terms as pm_hessian_vec), or with mass_weighted on
+ * M^-1/2 H M^-1/2, the matrix the reference diagonalises for normal modes
+ * (integrators/NormalModeDiagonalize.cpp:204-229, Hessian::evaluate with mass
+ * re-weighting; the reference calls LAPACK dsyevr on the dense matrix).  Runs
+ * `steps` Hessian-vector products from v0 (double[3n], normalised here); the
+ * Krylov basis, Gram-Schmidt coefficients and norms stay in device memory.
+ * Outputs: ritz_values[steps] ascending (NaN past an invariant subspace);
+ * ritz_vectors[nvec][3n] for the nvec lowest (may be NULL when nvec is 0);
+ * residuals[steps] (optional), the bound |beta_m s_mi| on |A y - theta y|;
+ * alpha_beta[2 steps] (optional), the tridiagonal diagonal then
+ * off-diagonal.  1 <= steps <= min(PM_LANCZOS_MAX_STEPS, 3n). */
+#define PM_LANCZOS_MAX_STEPS 256
+int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* positions, const double* v0,
+               int steps, int mass_weighted, int nvec, double* ritz_values, double* ritz_vectors,
+               double* residuals, double* alpha_beta);
 
 /* Layout and kernel options, chosen per context (no reference counterpart:
  * they change speed, never results beyond the mixed-precision tolerance):

@@ -99,6 +99,20 @@ struct pm_context {
   int *d_tab_a = nullptr, *d_tab_b = nullptr, *d_tab_c = nullptr;
   float4* d_lj_f = nullptr;
   double4* d_lj_d = nullptr;
+  // per-atom LJ (R = sigma 2^(-5/6), e = sqrt(eps), R14, e14), when set
+  float4* d_atom_lj_f = nullptr;
+  double4* d_atom_lj_d = nullptr;
+  std::vector<double> lj_R, lj_e;
+
+  // adjoint work buffers (Hessian-vector products, parameter gradients,
+  // Lanczos), kept across calls and grown on demand
+  double4* d_adj_sv = nullptr;          // [adj_nslot] direction or charge per slot
+  long long* d_adj_slot = nullptr;      // [3 adj_nslot] per-slot fixed point
+  long long* d_adj_fix = nullptr;       // [3 adj_n] per-atom fixed point
+  double* d_adj_v = nullptr;            // [3 adj_n] direction
+  double* d_adj_out = nullptr;          // [3 adj_n] FP64 parts
+  long long adj_nslot = 0;
+  int adj_n = 0;
 
   // state
   double *d_pos = nullptr, *d_vel = nullptr;
@@ -777,7 +791,20 @@ std::vector<PairPass> plan_passes(pm_context* c, const std::vector<int>& handles
 // ---------------------------------------------------------- evaluation core
 // Queues one force-group evaluation at the positions in c->d_pos: forces into
 // c->d_force (fixed point, zeroed first), energies into c->d_energy.
-int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bool want_energy = true) {
+// Adjoint evaluation request (pm_hessian_vec, pm_param_gradients): mode 1
+// H v, mode 2 parameter gradients; device buffers sized by the caller.
+struct AdjointSpec {
+  int mode = 0;
+  const double* v = nullptr;   // [3n] direction (mode 1)
+  double4* slot_v = nullptr;   // [nslot_max + 1]
+  long long* slot = nullptr;   // [3 (nslot_max + 1)] per-slot fixed point
+  long long* fix = nullptr;    // [3n] per-atom fixed point
+  double* hv = nullptr;        // [3n] mode 1 (PME and correction parts, FP64)
+  double* dq = nullptr;        // [n] mode 2 (PME, correction and self parts, FP64)
+};
+
+int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bool want_energy = true,
+                   const AdjointSpec* adj = nullptr) {
   hipStream_t s = c->stream;
   const int n = c->n;
   const bool vir = (flags & PM_EVAL_VIRIAL) != 0;
@@ -884,6 +911,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.ntypes1 = c->ntypes + 1;
     a.lj_f = c->d_lj_f;
     a.lj_d = c->d_lj_d;
+    a.atom_lj_f = c->d_atom_lj_f;
+    a.atom_lj_d = c->d_atom_lj_d;
     a.force_fix = c->d_force;
     a.force_slot = c->d_force_slot;
     a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
@@ -974,6 +1003,33 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   if (want_energy && !c->d_ered &&
       (r = dalloc(c, &c->d_ered, (size_t)pm_reduce_scratch_records((int)rec) * PM_NE)))
     return r;
+
+  if (adj) {
+    // adjoint kernels, all on the first stream
+    pm_launch_slot_vec(s, c->d_counts, c->d_atom_of_slot, adj->mode == 1 ? adj->v : c->d_scratch3, adj->slot_v,
+                       c->nslot_max);
+    pm_launch_zero_ll(s, adj->slot, 3LL * (c->nslot_max + 1));
+    pm_launch_zero_ll(s, adj->fix, 3LL * n);
+    if (adj->mode == 1) pm_launch_zero_d(s, adj->hv, 3LL * n);
+    else pm_launch_zero_d(s, adj->dq, n);
+    for (auto& pl : pair_launch) {
+      pm_launch_list(s, c->box, pl.first, pl.second);
+      pm_launch_adjoint(s, c->box, pl.first, pl.second, c->precision, adj->mode, adj->slot_v, adj->slot);
+    }
+    if (!pair_launch.empty())
+      pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, adj->slot, adj->fix);
+    for (auto& pl : pme_launch)
+      pm_launch_pme_adjoint(s, pl.first, c->plans, c->d_slot_pos64, c->precision, adj->mode == 1 ? 1 : 0,
+                            adj->slot_v, adj->mode == 1 ? adj->hv : adj->dq);
+    for (auto& hr : pme_rec) {
+      ForceDesc& f = c->forces[hr.first];
+      if (!f.corr) continue;
+      pm_launch_corr_adjoint(s, c->box, n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls, f.alpha,
+                             c->csf, adj->mode == 1 ? 1 : 0, adj->v, adj->mode == 1 ? adj->hv : adj->dq);
+    }
+    HIPCHECK(c, hipGetLastError());
+    return PM_OK;
+  }
 
   // fork: PME, correction and bonded terms on the second stream, the pair
   // lists and pair kernels on the first; they write different force arrays
@@ -1081,6 +1137,10 @@ void free_all(pm_context* c) {
   dfree(c->d_q); dfree(c->d_mass); dfree(c->d_type); dfree(c->d_mol);
   dfree(c->d_ex_row); dfree(c->d_ex_partner); dfree(c->d_ex_cls);
   dfree(c->d_xs_row); dfree(c->d_xs_part); dfree(c->d_jmask);
+  dfree(c->d_atom_lj_f); dfree(c->d_atom_lj_d);
+  c->lj_R.clear(); c->lj_e.clear();
+  dfree(c->d_adj_sv); dfree(c->d_adj_slot); dfree(c->d_adj_fix); dfree(c->d_adj_v); dfree(c->d_adj_out);
+  c->adj_nslot = 0; c->adj_n = 0;
   dfree(c->d_cx_row); dfree(c->d_cx_partner); dfree(c->d_cx_cls);
   dfree(c->d_tab_a); dfree(c->d_tab_b); dfree(c->d_tab_c);
   dfree(c->d_lj_f); dfree(c->d_lj_d);
@@ -1318,6 +1378,44 @@ int pm_set_topology(pm_context* c, int n, const double* q, const int* type, int 
   c->have_topo = true;
   c->md_forces_valid = false;
   c->gen++;
+  return PM_OK;
+}
+
+int pm_set_lj_per_atom(pm_context* c, const double* sigma, const double* epsilon, const double* sigma14,
+                       const double* epsilon14) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  c->gen++;
+  c->md_forces_valid = false;
+  if (!sigma) {
+    dfree(c->d_atom_lj_f);
+    dfree(c->d_atom_lj_d);
+    c->lj_R.clear();
+    c->lj_e.clear();
+    return PM_OK;
+  }
+  if (!epsilon) return fail(c, PM_ERR_ARGUMENT, "pm_set_lj_per_atom: epsilon missing");
+  const int n = c->n;
+  const double rfac = std::pow(2.0, -5.0 / 6.0);   // Rmin/2 = sigma 2^(1/6) / 2
+  std::vector<float4> f(n);
+  std::vector<double4> d(n);
+  c->lj_R.assign(n, 0.0);
+  c->lj_e.assign(n, 0.0);
+  for (int i = 0; i < n; i++) {
+    const double s1 = sigma[i], e1 = epsilon[i];
+    const double s4 = sigma14 ? sigma14[i] : s1, e4 = epsilon14 ? epsilon14[i] : e1;
+    if (!(s1 >= 0) || !(e1 >= 0) || !(s4 >= 0) || !(e4 >= 0))
+      return fail(c, PM_ERR_ARGUMENT, "atom %d: sigma and epsilon must be >= 0", i);
+    const double R1 = s1 * rfac, E1 = std::sqrt(e1), R4 = s4 * rfac, E4 = std::sqrt(e4);
+    c->lj_R[i] = R1;
+    c->lj_e[i] = E1;
+    d[i] = make_double4(R1, E1, R4, E4);
+    f[i] = make_float4((float)R1, (float)E1, (float)R4, (float)E4);
+  }
+  if ((r = upload(c, &c->d_atom_lj_f, f))) return r;
+  if ((r = upload(c, &c->d_atom_lj_d, d))) return r;
   return PM_OK;
 }
 
@@ -1741,6 +1839,266 @@ int pm_get_timing(pm_context* c, double* ms, long long* count, int ncat) {
   HIPCHECK(c, hipStreamSynchronize(c->stream2));
   harvest_timing(c);
   for (int k = 0; k < ncat && k < PM_TIMING_CATEGORIES; k++) { ms[k] = c->t_ms[k]; count[k] = c->t_count[k]; }
+  return PM_OK;
+}
+
+// the adjoint work buffers for the current atom and slot counts
+static int ensure_adjoint(pm_context* c) {
+  const long long ns = (long long)c->nslot_max + 1;
+  int r;
+  if (c->adj_nslot < ns) {
+    if ((r = dalloc(c, &c->d_adj_sv, (size_t)ns)) || (r = dalloc(c, &c->d_adj_slot, 3 * (size_t)ns))) return r;
+    c->adj_nslot = ns;
+  }
+  if (c->adj_n < c->n) {
+    const size_t n3 = 3 * (size_t)c->n;
+    if ((r = dalloc(c, &c->d_adj_fix, n3)) || (r = dalloc(c, &c->d_adj_v, n3)) || (r = dalloc(c, &c->d_adj_out, n3)))
+      return r;
+    c->adj_n = c->n;
+  }
+  return PM_OK;
+}
+
+static int adjoint_check(pm_context* c, const int* handles, int nh) {
+  int r;
+  if ((r = check_ready(c, handles, nh))) return r;
+  for (int k = 0; k < nh; k++)
+    if (c->forces[handles[k]].bonded)
+      return fail(c, PM_ERR_UNSUPPORTED, "Hessian-vector products and parameter gradients cover the nonbonded cutoff and PME forces, not '%s'", c->forces[handles[k]].id.c_str());
+  if (choose_m(c) != 8) return fail(c, PM_ERR_UNSUPPORTED, "the adjoint kernels use clusters of 8 atoms (PM_OPT_CLUSTER_SIZE 16 is set)");
+  if ((r = ensure_cells(c))) return r;
+  return ensure_adjoint(c);
+}
+
+// one adjoint pass over the positions already in d_pos: mode 1 H v with the
+// direction in d_adj_v, mode 2 parameter gradients; results in d_adj_fix
+// (fixed point) and d_adj_out (FP64 parts)
+static int adjoint_pass(pm_context* c, const int* handles, int nh, int mode) {
+  AdjointSpec a;
+  a.mode = mode;
+  a.slot_v = c->d_adj_sv; a.slot = c->d_adj_slot; a.fix = c->d_adj_fix; a.v = c->d_adj_v;
+  if (mode == 1) a.hv = c->d_adj_out; else a.dq = c->d_adj_out;
+  std::vector<int> hs(handles, handles + nh);
+  return evaluate_group(c, hs, 0, false, &a);
+}
+
+static int adjoint_run(pm_context* c, const int* handles, int nh, const double* positions, const double* v,
+                       int mode, std::vector<long long>& fix, std::vector<double>& dbl) {
+  int r;
+  if (!positions) return fail(c, PM_ERR_ARGUMENT, "positions required");
+  if ((r = adjoint_check(c, handles, nh))) return r;
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  const size_t bytes = 3 * (size_t)n * sizeof(double);
+  hipError_t e = hipMemcpyAsync(c->d_pos, positions, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->d_adj_v, v ? v : positions, bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return fail(c, PM_ERR_DEVICE, "HIP error %s copying the inputs", hipGetErrorString(e));
+  c->md_forces_valid = false;
+  if ((r = adjoint_pass(c, handles, nh, mode))) { hipStreamSynchronize(s); return r; }
+  fix.assign(3 * (size_t)n, 0);
+  dbl.assign(mode == 1 ? 3 * (size_t)n : (size_t)n, 0.0);
+  e = hipMemcpyAsync(fix.data(), c->d_adj_fix, fix.size() * sizeof(long long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dbl.data(), c->d_adj_out, dbl.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(c, PM_ERR_DEVICE, "HIP error %s copying the results", hipGetErrorString(e));
+  return check_overflow(c);
+}
+
+int pm_hessian_vec(pm_context* c, const int* handles, int nh, const double* positions, const double* v,
+                   double* hv) {
+  if (!c || !v || !hv) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  std::vector<long long> fix;
+  std::vector<double> dbl;
+  if ((r = adjoint_run(c, handles, nh, positions, v, 1, fix, dbl))) return r;
+  for (size_t k = 0; k < fix.size(); k++) hv[k] = (double)fix[k] * PM_FORCE_UNFIX + dbl[k];
+  return PM_OK;
+}
+
+int pm_param_gradients(pm_context* c, const int* handles, int nh, const double* positions, double* dE_dq,
+                       double* dE_dsigma, double* dE_depsilon) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if ((dE_dsigma || dE_depsilon) && c->lj_R.empty())
+    return fail(c, PM_ERR_STATE, "gradients with respect to sigma and epsilon need per-atom LJ parameters (pm_set_lj_per_atom)");
+  std::vector<long long> fix;
+  std::vector<double> dbl;
+  if ((r = adjoint_run(c, handles, nh, positions, nullptr, 2, fix, dbl))) return r;
+  const double rfac = std::pow(2.0, -5.0 / 6.0);   // dR/dsigma
+  for (int i = 0; i < c->n; i++) {
+    if (dE_dq) dE_dq[i] = (double)fix[3 * i] * PM_FORCE_UNFIX + dbl[i];
+    if (dE_dsigma) dE_dsigma[i] = (double)fix[3 * i + 1] * PM_FORCE_UNFIX * rfac;
+    // de/deps = 1/(2 sqrt(eps)): no finite derivative at eps = 0 (reported 0)
+    if (dE_depsilon) dE_depsilon[i] = c->lj_e[i] > 0 ? (double)fix[3 * i + 2] * PM_FORCE_UNFIX / (2.0 * c->lj_e[i]) : 0.0;
+  }
+  return PM_OK;
+}
+
+int pm_set_charges(pm_context* c, const double* q) {
+  if (!c || !q) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  for (int i = 0; i < c->n; i++)
+    if (!std::isfinite(q[i])) return fail(c, PM_ERR_ARGUMENT, "charge %d is not finite", i);
+  c->q.assign(q, q + c->n);
+  // same size: the copy keeps the device address, so captured MD steps stay valid
+  HIPCHECK(c, hipMemcpy(c->d_q, c->q.data(), c->q.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->md_forces_valid = false;
+  return PM_OK;
+}
+
+// Eigenvalues and eigenvectors of a symmetric m x m matrix a (row major,
+// destroyed) by cyclic Jacobi rotations (Golub & Van Loan, Matrix
+// Computations, 4th ed., Algorithm 8.5.3): z column j holds the eigenvector
+// of d[j].
+static void jacobi_eigen(int m, std::vector<double>& a, std::vector<double>& d, std::vector<double>& z) {
+  z.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; i++) z[(size_t)i * m + i] = 1.0;
+  auto A = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) {
+        tot += A(i, j) * A(i, j);
+        if (i != j) off += A(i, j) * A(i, j);
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < m - 1; p++)
+      for (int q = p + 1; q < m; q++) {
+        const double apq = A(p, q);
+        if (apq == 0.0) continue;
+        const double tau = (A(q, q) - A(p, p)) / (2.0 * apq);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+        for (int k = 0; k < m; k++) {   // A <- A J
+          const double akp = A(k, p), akq = A(k, q);
+          A(k, p) = cs * akp - sn * akq;
+          A(k, q) = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < m; k++) {   // A <- J^T A
+          const double apk = A(p, k), aqk = A(q, k);
+          A(p, k) = cs * apk - sn * aqk;
+          A(q, k) = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < m; k++) {   // Z <- Z J
+          const double zkp = z[(size_t)k * m + p], zkq = z[(size_t)k * m + q];
+          z[(size_t)k * m + p] = cs * zkp - sn * zkq;
+          z[(size_t)k * m + q] = sn * zkp + cs * zkq;
+        }
+      }
+  }
+  d.resize(m);
+  for (int i = 0; i < m; i++) d[i] = A(i, i);
+}
+
+int pm_lanczos(pm_context* c, const int* handles, int nh, const double* positions, const double* v0, int steps,
+               int mass_weighted, int nvec, double* ritz_values, double* ritz_vectors, double* residuals,
+               double* alpha_beta) {
+  if (!c || !v0 || !ritz_values) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!positions) return fail(c, PM_ERR_ARGUMENT, "positions required");
+  const int n = c->n, n3 = 3 * n;
+  if (steps < 1 || steps > PM_LANCZOS_MAX_STEPS || steps > n3)
+    return fail(c, PM_ERR_ARGUMENT, "pm_lanczos: steps must lie in [1, min(%d, 3n)] (got %d)", PM_LANCZOS_MAX_STEPS, steps);
+  if (nvec < 0 || nvec > steps) return fail(c, PM_ERR_ARGUMENT, "pm_lanczos: nvec must lie in [0, steps] (got %d)", nvec);
+  if (nvec > 0 && !ritz_vectors) return fail(c, PM_ERR_ARGUMENT, "pm_lanczos: ritz_vectors required for nvec > 0");
+  double nrm0 = 0.0;
+  for (int k = 0; k < n3; k++) nrm0 += v0[k] * v0[k];
+  if (!(nrm0 > 0.0) || !std::isfinite(nrm0)) return fail(c, PM_ERR_ARGUMENT, "pm_lanczos: the start vector must be finite and nonzero");
+  if (mass_weighted)
+    for (int i = 0; i < n; i++)
+      if (!(c->mass[i] > 0.0)) return fail(c, PM_ERR_ARGUMENT, "pm_lanczos: mass weighting needs positive masses (atom %d)", i);
+  if ((r = adjoint_check(c, handles, nh))) return r;
+  hipStream_t s = c->stream;
+  const int nblk = pm_krylov_dot_blocks(n3);
+  const int m = steps;
+  // device work space: basis V [m + 1][n3], w, 1/sqrt(mass), partial sums,
+  // coefficients [2][m + 1], alpha [m], beta [m], tridiagonal eigenvectors
+  double *V = nullptr, *w = nullptr, *wt = nullptr, *part = nullptr, *coef = nullptr, *ab = nullptr, *S = nullptr,
+         *Y = nullptr;
+  auto cleanup = [&]() { dfree(V); dfree(w); dfree(wt); dfree(part); dfree(coef); dfree(ab); dfree(S); dfree(Y); };
+  if ((r = dalloc(c, &V, (size_t)(m + 1) * n3)) || (r = dalloc(c, &w, (size_t)n3)) ||
+      (r = dalloc(c, &wt, (size_t)n)) || (r = dalloc(c, &part, (size_t)(m + 1) * nblk)) ||
+      (r = dalloc(c, &coef, 2 * (size_t)(m + 1))) || (r = dalloc(c, &ab, 2 * (size_t)m)) ||
+      (r = dalloc(c, &S, (size_t)std::max(1, nvec) * m)) || (r = dalloc(c, &Y, nvec > 0 ? (size_t)nvec * n3 : 1))) {
+    cleanup();
+    return r;
+  }
+  std::vector<double> h(n3);
+  const double inv0 = 1.0 / std::sqrt(nrm0);
+  for (int k = 0; k < n3; k++) h[k] = v0[k] * inv0;
+  std::vector<double> minv(n);
+  for (int i = 0; i < n; i++) minv[i] = 1.0 / std::sqrt(c->mass[i]);
+  const size_t bytes = (size_t)n3 * sizeof(double);
+  hipError_t e = hipMemcpyAsync(c->d_pos, positions, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(V, h.data(), bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(wt, minv.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) { hipStreamSynchronize(s); cleanup(); return fail(c, PM_ERR_DEVICE, "HIP error %s copying the inputs", hipGetErrorString(e)); }
+  c->md_forces_valid = false;
+  const double* mw = mass_weighted ? wt : nullptr;
+  double* alpha = ab;
+  double* beta = ab + m;
+  for (int j = 0; j < m; j++) {
+    double* vj = V + (size_t)j * n3;
+    // w = M^-1/2 H M^-1/2 v_j (M = 1 without mass weighting)
+    pm_launch_scale_copy(s, n3, vj, mw, c->d_adj_v);
+    if ((r = adjoint_pass(c, handles, nh, 1))) { hipStreamSynchronize(s); cleanup(); return r; }
+    pm_launch_fix_combine(s, n3, c->d_adj_fix, c->d_adj_out, mw, w);
+    // two classical Gram-Schmidt passes against v_0..v_j; alpha_j = v_j . w
+    // collects the v_j coefficients of both passes
+    for (int pass = 0; pass < 2; pass++) {
+      double* cf = coef + (size_t)pass * (m + 1);
+      pm_launch_dots(s, n3, j + 1, V, w, part, cf, j, alpha + j, pass == 0, 0);
+      pm_launch_gs_update(s, n3, j + 1, V, cf, w);
+    }
+    // beta_j = |w|, v_{j+1} = w / beta_j
+    pm_launch_dots(s, n3, 1, w, w, part, coef, 0, beta + j, 1, 1);
+    pm_launch_normalise(s, n3, w, beta + j, 1e-300, V + (size_t)(j + 1) * n3);
+  }
+  std::vector<double> abh(2 * (size_t)m);
+  e = hipMemcpyAsync(abh.data(), ab, abh.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { cleanup(); return fail(c, PM_ERR_DEVICE, "HIP error %s in the Lanczos steps", hipGetErrorString(e)); }
+  if ((r = check_overflow(c))) { cleanup(); return r; }
+  // tridiagonal T (truncated where an invariant subspace ended the recurrence)
+  int mt = m;
+  const double bscale = std::fabs(abh[0]) + 1e-300;
+  for (int j = 0; j < m - 1; j++)
+    if (abh[m + j] <= 1e-14 * bscale) { mt = j + 1; break; }
+  std::vector<double> T((size_t)mt * mt, 0.0), d, z;
+  for (int j = 0; j < mt; j++) {
+    T[(size_t)j * mt + j] = abh[j];
+    if (j + 1 < mt) T[(size_t)j * mt + j + 1] = T[(size_t)(j + 1) * mt + j] = abh[m + j];
+  }
+  jacobi_eigen(mt, T, d, z);
+  std::vector<int> ord(mt);
+  for (int i = 0; i < mt; i++) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+  const double blast = abh[m + mt - 1];
+  for (int i = 0; i < m; i++) {
+    ritz_values[i] = i < mt ? d[ord[i]] : std::nan("");
+    if (residuals) residuals[i] = i < mt ? std::fabs(blast * z[(size_t)(mt - 1) * mt + ord[i]]) : std::nan("");
+  }
+  if (alpha_beta) std::copy(abh.begin(), abh.end(), alpha_beta);
+  if (nvec > 0) {
+    const int ny = std::min(nvec, mt);
+    std::vector<double> Sh((size_t)ny * mt);
+    for (int i = 0; i < ny; i++)
+      for (int j = 0; j < mt; j++) Sh[(size_t)i * mt + j] = z[(size_t)j * mt + ord[i]];
+    e = hipMemcpyAsync(S, Sh.data(), Sh.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) pm_launch_ritz(s, n3, mt, V, S, ny, Y);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(ritz_vectors, Y, (size_t)ny * bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (int i = ny; i < nvec; i++)
+      std::fill(ritz_vectors + (size_t)i * n3, ritz_vectors + (size_t)(i + 1) * n3, 0.0);
+    if (e != hipSuccess) { cleanup(); return fail(c, PM_ERR_DEVICE, "HIP error %s assembling the Ritz vectors", hipGetErrorString(e)); }
+  }
+  cleanup();
   return PM_OK;
 }
 

@@ -112,6 +112,10 @@ struct PmNbLaunch {
   int ntypes1;               // types + 1 dummy
   const float4* lj_f;
   const double4* lj_d;
+  // per-atom LJ parameters (when set, instead of the type table): R = Rmin/2,
+  // e = sqrt(eps), R14, e14 per atom; A = (Ri+Rj)^12 ei ej, B = 2 (Ri+Rj)^6 ei ej
+  const float4* atom_lj_f;
+  const double4* atom_lj_d;
   // outputs
   long long* force_fix;      // [3n] fixed point, accumulated (+=) for every real atom
   long long* force_slot;     // [3 * slots] fixed point per slot (pair kernel), folded into force_fix
@@ -138,6 +142,12 @@ void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const
                           const long long* force_slot, long long* force_fix);
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
+// Adjoint pair kernel (clusters of 8): mode 1 H v, mode 2 (dE/dq, dE/dR, dE/de)
+// per slot into `out` (64-bit fixed point, 3 per slot, accumulated).
+void pm_launch_adjoint(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int precision,
+                       int mode, const double4* slot_v, long long* out);
+void pm_launch_slot_vec(hipStream_t s, const int* counts, const int* atom_of_slot, const double* v, double4* sv,
+                        int nslot_max);
 void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p,
                     int precision);
 void pm_launch_diag(hipStream_t s, const PmBox& box, int n, const double* pos, const int* cell_of,
@@ -190,12 +200,31 @@ struct PmFftPlan {
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
                          const longlong4* pos64, int precision);
 int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision);
+// PME adjoint: mode 0 out[n] += dE/dq (reciprocal potential), mode 1
+// out[3n] += (H v) of the reciprocal energy; slot_v = v in slot order
+bool pm_launch_pme_adjoint(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
+                           int precision, int mode, const double4* slot_v, double* out);
 
 // Correction (excluded pairs), reductions, conversions and integrators
 void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
                           const int* row, const int* partner, const int* cls, double alpha, double csf,
                           int want_virial, long long* force_fix, int ntab, const int* tab_a,
                           const int* tab_b, const int* tab_c, double* epart, int nblk);
+void pm_launch_corr_adjoint(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
+                            const int* row, const int* partner, const int* cls, double alpha, double csf, int mode,
+                            const double* v, double* out);
+void pm_launch_add_fix_to(hipStream_t s, int m, const long long* fixed, double* out);
+
+// Lanczos vector kernels (pm_krylov.hip); `partial` holds nv * pm_krylov_dot_blocks(n3) doubles
+int pm_krylov_dot_blocks(int n3);
+void pm_launch_scale_copy(hipStream_t s, int n3, const double* in, const double* w, double* out);
+void pm_launch_fix_combine(hipStream_t s, int n3, const long long* fix, const double* dbl, const double* w,
+                           double* out);
+void pm_launch_dots(hipStream_t s, int n3, int nv, const double* V, const double* x, double* partial, double* out,
+                    int acc_row, double* acc, int acc_set, int take_sqrt);
+void pm_launch_gs_update(hipStream_t s, int n3, int nv, const double* V, const double* c, double* w);
+void pm_launch_normalise(hipStream_t s, int n3, const double* w, const double* nrm, double tiny, double* out);
+void pm_launch_ritz(hipStream_t s, int n3, int nv, const double* V, const double* S, int ny, double* Y);
 void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, double constant_coul,
                              double constant_vir_diag, double* out, int overwrite, double* scratch);
 int pm_reduce_scratch_records(int nparts);

@@ -68,6 +68,57 @@ __global__ void k_corr_force(PmBox box, int n, const double* __restrict__ pos, c
   force_fix[3 * i + 2] += __double2ll_rn(f[2] * PM_FORCE_FIX);
 }
 
+// Adjoint of the correction term (FP64), per atom over its exclusion partners.
+// mode 1: out[3i] += H v with, per pair, t = F dv + 2 F_u (d . dv) d added to
+//   a1 and subtracted from a2 (d = x_a2 - x_a1, F = force/r of corr_pair,
+//   F_u its derivative in u = r^2: the exact derivatives of -qq erf(ar)/r);
+// mode 0: out[i] += dE/dq_i = -q_other erf(a r)/r (x (1 - csf) for MODIFIED)
+//   and the point self term -2 a q_i / sqrt(pi) (NonbondedPMEwaldSystemForce.h:604-613).
+__global__ void k_corr_adjoint(PmBox box, int n, const double* __restrict__ pos, const double* __restrict__ q,
+                               const int* __restrict__ row, const int* __restrict__ partner,
+                               const int* __restrict__ cls, double alpha, double csf, int mode,
+                               const double* __restrict__ v, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rpi = 1.0 / sqrt(3.14159265358979323846);
+  double acc[3] = {0, 0, 0};
+  for (int u = row[i]; u < row[i + 1]; u++) {
+    const int pe = partner[u];
+    const int other = pe >> 1;
+    const bool second = pe & 1;
+    const int a1 = second ? other : i, a2 = second ? i : other;
+    double d[3], r2;
+    min_diff(box, pos + 3 * a1, pos + 3 * a2, d, r2);
+    double sc = 1.0;
+    if (cls[u] == PM_EXCL_MODIFIED) sc = 1 - csf;
+    const double r = sqrt(r2), rr = 1 / r;
+    const double erfv = erf(alpha * r), ex = exp(-alpha * alpha * r2);
+    if (mode == 0) {
+      acc[0] += -sc * q[other] * erfv * rr;
+      continue;
+    }
+    const double qq = q[a1] * q[a2] * sc;
+    const double c = 2.0 * alpha * rpi;
+    // E = -qq erf(ar)/r;  E_r and E_rr in r
+    const double Er = -qq * (c * ex * rr - erfv * rr * rr);
+    const double Err = qq * (2.0 * alpha * alpha * c * ex + 2.0 * c * ex * rr * rr - 2.0 * erfv * rr * rr * rr);
+    const double F = -Er * rr;                         // force/r (fij = F d on a2)
+    const double Fu = -(Err - Er * rr) / (2.0 * r2);   // dF/du
+    const double dv[3] = {v[3 * a2] - v[3 * a1], v[3 * a2 + 1] - v[3 * a1 + 1], v[3 * a2 + 2] - v[3 * a1 + 2]};
+    const double g = 2.0 * Fu * (d[0] * dv[0] + d[1] * dv[1] + d[2] * dv[2]);
+    const double sgn = second ? -1.0 : 1.0;
+    for (int k = 0; k < 3; k++) acc[k] += sgn * (F * dv[k] + g * d[k]);
+  }
+  if (mode == 0) out[i] += acc[0] - 2.0 * alpha * rpi * q[i];
+  else { out[3 * i] += acc[0]; out[3 * i + 1] += acc[1]; out[3 * i + 2] += acc[2]; }
+}
+
+// out[k] = fixed[k] * 2^-40 + out[k]
+__global__ void k_add_fix_to(int m, const long long* __restrict__ fixed, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] += (double)fixed[i] * PM_FORCE_UNFIX;
+}
+
 // per table entry: intra-molecular energy and virial, block partial sums
 __global__ __launch_bounds__(256) void k_corr_energy(PmBox box, const double* __restrict__ pos,
                                                      const double* __restrict__ q, int ntab,
@@ -273,6 +324,18 @@ void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* 
   if (nblk > 0)
     hipLaunchKernelGGL(k_corr_energy, dim3(nblk), dim3(256), 0, s, box, pos, q, ntab, tab_a, tab_b, tab_c, alpha,
                        csf, want_virial, epart);
+}
+
+void pm_launch_corr_adjoint(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
+                            const int* row, const int* partner, const int* cls, double alpha, double csf, int mode,
+                            const double* v, double* out) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_corr_adjoint, grid_for(n, 256), dim3(256), 0, s, box, n, pos, q, row, partner, cls, alpha,
+                       csf, mode, v, out);
+}
+
+void pm_launch_add_fix_to(hipStream_t s, int m, const long long* fixed, double* out) {
+  if (m > 0) hipLaunchKernelGGL(k_add_fix_to, grid_for(m, 256), dim3(256), 0, s, m, fixed, out);
 }
 
 int pm_reduce_scratch_records(int nparts) {

@@ -522,6 +522,32 @@ template <> struct Prec<double> {
   static __device__ inline double sqrt_(double x) { return sqrt(x); }
 };
 
+// LJ A and B of a pair: the type table (LennardJonesParameterTable) or, when
+// per-atom parameters are set, the CHARMM combination of the two atoms'
+// (R, sqrt(eps)) as frontend/buildTopology.cpp:643-664 fills the table; the
+// 1-4 values for MODIFIED pairs.
+template <typename T, typename LJ4>
+__device__ inline void lj_ab(const PmNbLaunch& a, const LJ4* s_lj, int ti, int mj, int nt1, int oi, int excl,
+                             T& A, T& B) {
+  const bool per_atom = sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr;
+  if (per_atom) {
+    const int oj = (mj >> 8) & 0x7fffff;
+    LJ4 pi_, pj_;
+    if constexpr (sizeof(T) == 4) { pi_ = a.atom_lj_f[oi]; pj_ = a.atom_lj_f[oj]; }
+    else { pi_ = a.atom_lj_d[oi]; pj_ = a.atom_lj_d[oj]; }
+    const bool m = excl == PM_EXCL_MODIFIED;
+    const T rs = m ? (T)pi_.z + (T)pj_.z : (T)pi_.x + (T)pj_.x;
+    const T ee = m ? (T)pi_.w * (T)pj_.w : (T)pi_.y * (T)pj_.y;
+    const T r2 = rs * rs, r6 = r2 * r2 * r2;
+    A = r6 * r6 * ee;
+    B = T(2) * r6 * ee;
+  } else {
+    const LJ4 ab = s_lj[ti * nt1 + (mj & 0xff)];
+    A = excl == PM_EXCL_MODIFIED ? (T)ab.z : (T)ab.x;
+    B = excl == PM_EXCL_MODIFIED ? (T)ab.w : (T)ab.y;
+  }
+}
+
 // C1 / C2 / Cutoff switching functions (topology/C1SwitchingFunction.h:64-75,
 // C2SwitchingFunction.h:64-82, CutoffSwitchingFunction.h:64-68).
 template <typename T, int SWF>
@@ -714,10 +740,8 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     else { rr2 = T(1) / r2; rinv = sqrt(rr2); }
     T F, E1 = 0, E2 = 0;
     if constexpr (LJSWF != PM_SWF_NONE) {
-      const int tj = mj & 0xff;
-      const LJ4 ab = s_lj[ti * nt1 + tj];
-      const T A = excl == PM_EXCL_MODIFIED ? (T)ab.z : (T)ab.x;
-      const T B = excl == PM_EXCL_MODIFIED ? (T)ab.w : (T)ab.y;
+      T A, B;
+      lj_ab<T>(a, s_lj, ti, mj, nt1, oi, excl, A, B);
       const T r6 = rr2 * rr2 * rr2;
       const T r12 = r6 * r6;
       const T r6B = B * r6;
@@ -826,6 +850,268 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
       }
     }
   }
+}
+
+// ------------------------------------------------------- adjoint pair kernel
+// Force/r F(u) of one switched term and its derivative F_u = dF/du, from the
+// term's energy E, force/r F0 and F0_u; S, D = 2 dS/du and S_uu from the
+// switching function (OneAtomPair.h:170-175: force = force * S - energy * D).
+template <typename T>
+__device__ inline void switched_fu(T E, T F0, T F0u, T S, T D, T Suu, T& F, T& Fu) {
+  F = F0 * S - E * D;
+  // E_u = -F0/2 for the exact-derivative potentials (LJ, Coulomb)
+  Fu = D * F0 + S * F0u - T(2) * E * Suu;
+}
+
+// MODE 1: Hessian-vector product of the pair energy, H v, with
+//   (H v)_i += F dv + 2 F_u (d . dv) d,  (H v)_j -= the same,
+//   d = x_j - x_i, dv = v_j - v_i, u = |d|^2, F = force/r of the pair as the
+//   force kernel computes it (so that H is minus the derivative of exactly
+//   those forces), F_u = dF/du.
+// MODE 2: parameter gradients per atom: (dE/dq, dE/dR, dE/de) with R the
+//   LJ radius and e = sqrt(eps) of per-atom parameters (dR, de zero with a
+//   type table); MODIFIED pairs contribute to dE/dq only.
+// The pairs come from the same cluster lists as the forces (each listed once);
+// results accumulate per slot as 64-bit fixed point.
+template <typename T, int LJSWF, int CK, int CSWF, int MODE>
+__global__ __launch_bounds__(256) void k_adjoint(PmBox box, PmNbLaunch a, PmPairParams pp,
+                                                 const double4* __restrict__ slot_v, long long* __restrict__ out) {
+  constexpr int M = 8, JW = 8;
+  extern __shared__ unsigned char s_lj_raw[];
+  typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
+  LJ4* s_lj = reinterpret_cast<LJ4*>(s_lj_raw);
+  const int nt1 = a.ntypes1;
+  if constexpr (LJSWF != PM_SWF_NONE) {
+    const LJ4* src;
+    if constexpr (sizeof(T) == 4) src = a.lj_f; else src = a.lj_d;
+    for (int t = threadIdx.x; t < nt1 * nt1; t += blockDim.x) s_lj[t] = src[t];
+    __syncthreads();
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int cl = blockIdx.x * 4 + wave;
+  if (cl >= a.counts[1]) return;
+  const int ii = lane % M, js = lane / M;
+  const int si = cl * M + ii;
+  const int mi = a.slot_meta[si];
+  const int ti = mi & 0xff, oi = (mi >> 8) & 0x7fffff;
+  const bool ireal = oi != 0x7fffff;
+  typedef typename std::conditional<sizeof(T) == 4, int4, longlong4>::type P4;
+  const P4* __restrict__ spos;
+  if constexpr (sizeof(T) == 4) spos = a.slot_pos; else spos = a.slot_pos64;
+  const P4 pi = spos[si];
+  T qi;
+  if constexpr (sizeof(T) == 4) qi = __int_as_float(pi.w); else qi = __longlong_as_double(pi.w);
+  const T sx = sizeof(T) == 4 ? (T)box.fix_to_A[0] : (T)(box.fix_to_A[0] * 2.3283064365386963e-10);
+  const T sy = sizeof(T) == 4 ? (T)box.fix_to_A[1] : (T)(box.fix_to_A[1] * 2.3283064365386963e-10);
+  const T sz = sizeof(T) == 4 ? (T)box.fix_to_A[2] : (T)(box.fix_to_A[2] * 2.3283064365386963e-10);
+  const T cut2 = (T)pp.cut2;
+  const T lj_cut2 = (T)pp.lj_cut2, lj_on2 = (T)pp.lj_on2, lj_sw1 = (T)pp.lj_sw1, lj_sw2 = (T)pp.lj_sw2,
+          lj_sw3 = (T)pp.lj_sw3, lj_c15_1 = (T)pp.lj_c15_1, lj_c05_3 = (T)pp.lj_c05_3,
+          lj_c15_3 = (T)pp.lj_c15_3;
+  const T c_cut2 = (T)pp.c_cut2, c_on2 = (T)pp.c_on2, c_sw1 = (T)pp.c_sw1, c_sw2 = (T)pp.c_sw2,
+          c_sw3 = (T)pp.c_sw3, c_c15_1 = (T)pp.c_c15_1, c_c05_3 = (T)pp.c_c05_3, c_c15_3 = (T)pp.c_c15_3;
+  const T alpha = (T)pp.alpha, tapi = (T)pp.two_alpha_pi, csf = (T)pp.csf;
+  const bool modify = pp.modify != 0;
+  const bool per_atom = sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr;
+  const double4 vid = slot_v[si];
+  const T vix = (T)vid.x, viy = (T)vid.y, viz = (T)vid.z;
+  T Ri = 0, ei = 0;
+  if (MODE == 2 && per_atom && ireal) {
+    if constexpr (sizeof(T) == 4) { const float4 w = a.atom_lj_f[oi]; Ri = w.x; ei = w.y; }
+    else { const double4 w = a.atom_lj_d[oi]; Ri = w.x; ei = w.y; }
+  }
+  double ax = 0, ay = 0, az = 0;
+  const int* __restrict__ lst = a.jlist + (long long)cl * a.lcap;
+  const int cnt = a.jcount[cl];
+  const int comp = ii;   // lanes ii = 0, 1, 2 add the x, y, z component of a row
+
+  for (int base = 0; base < cnt; base += JW) {
+    const int k = base + js;
+    const bool valid = k < cnt;
+    const int e = valid ? lst[k] : a.dummy_slot;
+    const bool flagged = e < 0;
+    const int sj = e & 0x7fffffff;
+    const P4 pj = spos[sj];
+    const int mj = a.slot_meta[sj];
+    const int oj = (mj >> 8) & 0x7fffff;
+    T dx, dy, dz;
+    if constexpr (sizeof(T) == 4) {
+      dx = (T)(int)(pj.x - pi.x) * sx;
+      dy = (T)(int)(pj.y - pi.y) * sy;
+      dz = (T)(int)(pj.z - pi.z) * sz;
+    } else {
+      dx = (T)(long long)((unsigned long long)pj.x - (unsigned long long)pi.x) * sx;
+      dy = (T)(long long)((unsigned long long)pj.y - (unsigned long long)pi.y) * sy;
+      dz = (T)(long long)((unsigned long long)pj.z - (unsigned long long)pi.z) * sz;
+    }
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    bool mask = valid && ireal && sj > si && (r2 <= cut2);
+    int excl = PM_EXCL_NONE;
+    if (__ballot(flagged && mask) != 0ull) {
+      if (flagged && mask) {
+        const int xm = a.jmask[(long long)cl * a.lcap + k];
+        if ((xm >> ii) & 1) { excl = PM_EXCL_FULL; mask = false; }
+        else if ((xm >> (16 + ii)) & 1) excl = PM_EXCL_MODIFIED;
+      }
+    }
+    T qj;
+    if constexpr (sizeof(T) == 4) qj = __int_as_float(pj.w); else qj = __longlong_as_double(pj.w);
+    const T r2s = mask ? r2 : T(1);   // keeps masked lanes finite
+    T rinv, rr2;
+    if constexpr (sizeof(T) == 4) { rinv = Prec<T>::rsq(r2s); rr2 = rinv * rinv; }
+    else { rr2 = T(1) / r2s; rinv = sqrt(rr2); }
+    const T dist = r2s * rinv;
+    T c0 = 0, c1 = 0, c2 = 0;   // this lane's contribution to the i atom (j gets its mirror)
+    if constexpr (MODE == 1) {
+      T F = 0, Fu = 0;
+      if constexpr (LJSWF != PM_SWF_NONE) {
+        T A, B;
+        lj_ab<T>(a, s_lj, ti, mj, nt1, oi, excl, A, B);
+        const T r6 = rr2 * rr2 * rr2;
+        const T E1 = A * r6 * r6 - B * r6;
+        const T F1 = (T(12) * A * r6 * r6 - T(6) * B * r6) * rr2;
+        const T F1u = (T(-84) * A * r6 * r6 + T(24) * B * r6) * rr2 * rr2;
+        if (modify && LJSWF != PM_SWF_CUTOFF) {
+          T S, D;
+          switching<T, LJSWF == PM_SWF_CUTOFF ? PM_SWF_CUTOFF : LJSWF>(
+              r2s, dist, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3, lj_c15_3, S, D);
+          T Suu = T(0);
+          if constexpr (LJSWF == PM_SWF_C2) Suu = r2s >= lj_on2 ? T(2) * lj_sw1 * (lj_sw2 + T(2) * r2s - T(4) * (lj_cut2 - r2s)) : T(0);
+          else if constexpr (LJSWF == PM_SWF_C1) Suu = (lj_c15_1 * rr2 + lj_c15_3) / (T(4) * dist);
+          T Fs, Fsu;
+          switched_fu<T>(E1, F1, F1u, S, D, Suu, Fs, Fsu);
+          F += Fs; Fu += Fsu;
+        } else if (modify && r2s > lj_cut2) {
+        } else {
+          F += F1; Fu += F1u;
+        }
+      }
+      if constexpr (CK != PM_COUL_NONE) {
+        T qq = qi * qj;
+        if (excl == PM_EXCL_MODIFIED) qq *= csf;
+        T E2, F2, F2u;
+        if constexpr (CK == PM_COUL_PLAIN) {
+          E2 = qq * rinv;
+          F2 = E2 * rr2;
+          F2u = T(-1.5) * F2 * rr2;
+        } else {
+          const T ar = alpha * dist;
+          const T t = T(1) / (T(1) + T(0.3275911) * ar);
+          const T P = t * (T(0.254829592) + t * (T(-0.284496736) + t * (T(1.421413741) +
+                                              t * (T(-1.453152027) + t * T(1.061405429)))));
+          const T Pp = T(0.254829592) + t * (T(2) * T(-0.284496736) + t * (T(3) * T(1.421413741) +
+                                              t * (T(4) * T(-1.453152027) + t * T(5) * T(1.061405429))));
+          const T X = exp(-ar * ar);
+          E2 = qq * P * X * rinv;
+          const T h = -T(0.3275911) * alpha * t * t * Pp * rinv - T(2) * alpha * alpha * P - P * rr2;
+          const T E2u = T(0.5) * qq * X * h * rinv;
+          F2 = (E2 + tapi * qq * X) * rr2;
+          F2u = (E2u - alpha * alpha * tapi * qq * X) * rr2 - F2 * rr2;
+        }
+        if (modify && CSWF != PM_SWF_CUTOFF) {
+          T S, D;
+          switching<T, CSWF>(r2s, dist, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3, S, D);
+          T Suu = T(0);
+          if constexpr (CSWF == PM_SWF_C2) Suu = r2s >= c_on2 ? T(2) * c_sw1 * (c_sw2 + T(2) * r2s - T(4) * (c_cut2 - r2s)) : T(0);
+          else if constexpr (CSWF == PM_SWF_C1) Suu = (c_c15_1 * rr2 + c_c15_3) / (T(4) * dist);
+          T Fs, Fsu;
+          switched_fu<T>(E2, F2, F2u, S, D, Suu, Fs, Fsu);
+          F += Fs; Fu += Fsu;
+        } else if (modify && r2s > c_cut2) {
+        } else {
+          F += F2; Fu += F2u;
+        }
+      }
+      F = mask ? F : T(0);
+      Fu = mask ? Fu : T(0);
+      const double4 vjd = slot_v[sj];
+      const T dvx = (T)vjd.x - vix, dvy = (T)vjd.y - viy, dvz = (T)vjd.z - viz;
+      const T g = T(2) * Fu * (dx * dvx + dy * dvy + dz * dvz);
+      c0 = F * dvx + g * dx;
+      c1 = F * dvy + g * dy;
+      c2 = F * dvz + g * dz;
+      ax += (double)c0; ay += (double)c1; az += (double)c2;
+      // j receives -t
+      c0 = -c0; c1 = -c1; c2 = -c2;
+    } else {
+      // dE/dq: the Coulomb energy per unit charge product
+      T eu_i = 0, eu_j = 0, dRs = 0, dee = 0;
+      if constexpr (CK != PM_COUL_NONE) {
+        T eu;
+        if constexpr (CK == PM_COUL_PLAIN) eu = rinv;
+        else {
+          const T ar = alpha * dist;
+          const T t = T(1) / (T(1) + T(0.3275911) * ar);
+          const T P = t * (T(0.254829592) + t * (T(-0.284496736) + t * (T(1.421413741) +
+                                              t * (T(-1.453152027) + t * T(1.061405429)))));
+          eu = P * exp(-ar * ar) * rinv;
+        }
+        if (excl == PM_EXCL_MODIFIED) eu *= csf;
+        if (modify) {
+          if (CSWF == PM_SWF_CUTOFF) { if (r2s > c_cut2) eu = T(0); }
+          else {
+            T S, D;
+            switching<T, CSWF>(r2s, dist, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3, S, D);
+            eu *= S;
+          }
+        }
+        eu_i = qj * eu;
+        eu_j = qi * eu;
+      }
+      if constexpr (LJSWF != PM_SWF_NONE) {
+        if (per_atom && excl != PM_EXCL_MODIFIED) {
+          T Rj, ej;
+          if constexpr (sizeof(T) == 4) { const float4 w = a.atom_lj_f[oj]; Rj = w.x; ej = w.y; }
+          else { const double4 w = a.atom_lj_d[oj]; Rj = w.x; ej = w.y; }
+          const T rs = Ri + Rj, ee = ei * ej;
+          const T rs2 = rs * rs, rs5 = rs2 * rs2 * rs, rs6 = rs5 * rs;
+          const T r6 = rr2 * rr2 * rr2;
+          T S = T(1);
+          if (modify) {
+            if (LJSWF == PM_SWF_CUTOFF) { if (r2s > lj_cut2) S = T(0); }
+            else {
+              T D;
+              switching<T, LJSWF == PM_SWF_CUTOFF ? PM_SWF_CUTOFF : LJSWF>(
+                  r2s, dist, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3, lj_c15_3, S, D);
+            }
+          }
+          dRs = S * T(12) * ee * (rs5 * rs6 * r6 * r6 - rs5 * r6);
+          dee = S * (rs6 * rs6 * r6 * r6 - T(2) * rs6 * r6);
+          c1 = dRs;
+          c2 = dee * ej;
+        }
+      }
+      if (!mask) { eu_i = eu_j = T(0); c1 = c2 = T(0); dee = T(0); dRs = T(0); }
+      ax += (double)eu_i; ay += (double)c1; az += (double)c2;
+      c0 = eu_j;
+      c1 = dRs;
+      c2 = dee * ei;
+    }
+    // j-side: sum over the row, lanes ii = 0, 1, 2 add components 0, 1, 2
+    const T j0 = row_sum<M>(c0), j1 = row_sum<M>(c1), j2 = row_sum<M>(c2);
+    const T jc = comp == 0 ? j0 : (comp == 1 ? j1 : j2);
+    if (ii < 3 && jc != T(0))
+      atomicAdd((unsigned long long*)(out + 3LL * sj + comp), (unsigned long long)force_fix_of(jc));
+  }
+#pragma unroll
+  for (int off = M; off < 64; off <<= 1) {
+    ax += __shfl_xor(ax, off, 64);
+    ay += __shfl_xor(ay, off, 64);
+    az += __shfl_xor(az, off, 64);
+  }
+  if (js < 3 && ireal) {
+    const double iv = js == 0 ? ax : (js == 1 ? ay : az);
+    atomicAdd((unsigned long long*)(out + 3LL * si + js), (unsigned long long)__double2ll_rn(iv * PM_FORCE_FIX));
+  }
+}
+
+// per-slot copy of a per-atom vector (zero for padding slots)
+__global__ void k_slot_vec(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+                           const double* __restrict__ v, double4* __restrict__ sv, int nslot_max) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nslot_max) return;   // the padding slot nslot_max included
+  const int at = s < counts[0] ? atom_of_slot[s] : -1;
+  sv[s] = at >= 0 ? make_double4(v[3 * at], v[3 * at + 1], v[3 * at + 2], 0.0) : make_double4(0, 0, 0, 0);
 }
 
 // ---------------------------------------------------- binning diagnostics
@@ -998,6 +1284,50 @@ static void launch_pair_p(hipStream_t s, const PmBox& box, const PmNbLaunch& a, 
   PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C2)
   PM_CASE(PM_SWF_C1, PM_COUL_PLAIN, PM_SWF_C1)
 #undef PM_CASE
+}
+
+template <typename T, int LJ, int CK, int CS>
+static void launch_adjoint_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int mode,
+                             const double4* sv, long long* out) {
+  size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
+  const dim3 g((a.ncl_max + 3) / 4), b(256);
+  if (mode == 1) hipLaunchKernelGGL((k_adjoint<T, LJ, CK, CS, 1>), g, b, lds, s, box, a, p, sv, out);
+  else hipLaunchKernelGGL((k_adjoint<T, LJ, CK, CS, 2>), g, b, lds, s, box, a, p, sv, out);
+}
+
+template <typename T>
+static void launch_adjoint_p(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int mode,
+                             const double4* sv, long long* out) {
+  const int lj = p.lj_swf, ck = p.coul_kind, cs = p.coul_swf;
+#define PM_CASE(L, C, S) \
+  if (lj == L && ck == C && (C == PM_COUL_NONE || cs == S)) { launch_adjoint_t<T, L, C, S>(s, box, a, p, mode, sv, out); return; }
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C1)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C2)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_NONE, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C1, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_CUTOFF, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C1, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C1)
+  PM_CASE(PM_SWF_C2, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_CUTOFF, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C2)
+  PM_CASE(PM_SWF_C1, PM_COUL_PLAIN, PM_SWF_C1)
+#undef PM_CASE
+}
+
+void pm_launch_adjoint(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int precision,
+                       int mode, const double4* slot_v, long long* out) {
+  if (precision == 1) launch_adjoint_p<double>(s, box, a, p, mode, slot_v, out);
+  else launch_adjoint_p<float>(s, box, a, p, mode, slot_v, out);
+}
+
+void pm_launch_slot_vec(hipStream_t s, const int* counts, const int* atom_of_slot, const double* v, double4* sv,
+                        int nslot_max) {
+  hipLaunchKernelGGL(k_slot_vec, dim3((nslot_max + 1 + 255) / 256), dim3(256), 0, s, counts, atom_of_slot, v, sv,
+                     nslot_max);
 }
 
 void pm_launch_pair(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int precision) {

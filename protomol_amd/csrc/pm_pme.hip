@@ -398,6 +398,171 @@ __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* 
   a.force_fix[3 * at + 2] += __double2ll_rn((double)fz * a.hr[2] * q * PM_FORCE_FIX);
 }
 
+// ---------------------------------------------------------- PME adjoint
+// B-spline weights with their first and second derivatives: the second
+// derivative of the order-P spline is the order P-2 spline differenced twice,
+// d2th[k] = m[k-2] - 2 m[k-1] + m[k] (m: order P-2, zero outside its range).
+template <typename T, int P>
+__device__ inline void bspline2(T w, T* th, T* dth, T* d2th) {
+  T m[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) m[k] = T(0);
+  th[P - 1] = T(0);
+  th[1] = w;
+  th[0] = T(1) - w;
+#pragma unroll
+  for (int k = 2; k < P - 1; k++) {
+    if (k == P - 2) {
+#pragma unroll
+      for (int j = 0; j < P - 2; j++) m[j] = th[j];
+    }
+    const T div = T(1) / T(k);
+    th[k] = div * w * th[k - 1];
+#pragma unroll
+    for (int j = 1; j < k; j++)
+      th[k - j] = div * ((w + T(j)) * th[k - j - 1] + (T(k + 1 - j) - w) * th[k - j]);
+    th[0] = div * (T(1) - w) * th[0];
+  }
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    const T a0 = k >= 2 && k - 2 < P - 2 ? m[k - 2] : T(0);
+    const T a1 = k >= 1 && k - 1 < P - 2 ? m[k - 1] : T(0);
+    const T a2 = k < P - 2 ? m[k] : T(0);
+    d2th[k] = a0 - T(2) * a1 + a2;
+  }
+  dth[0] = -th[0];
+#pragma unroll
+  for (int j = 1; j <= P - 1; j++) dth[j] = th[j - 1] - th[j];
+  const T div = T(1) / T(P - 1);
+  th[P - 1] = div * w * th[P - 2];
+#pragma unroll
+  for (int j = 1; j < P - 1; j++)
+    th[P - 1 - j] = div * ((w + T(j)) * th[P - j - 2] + (T(P - j) - w) * th[P - 1 - j]);
+  th[0] = div * (T(1) - w) * th[0];
+}
+
+template <typename T>
+__device__ inline void slot_grid(const PmPmeLaunch& a, const longlong4* pos64, int s, int& ix, int& iy, int& iz,
+                                 T& wx, T& wy, T& wz) {
+  if constexpr (sizeof(T) == 4) {
+    const int4 p = a.slot_pos[s];
+    grid_coord<T>((unsigned)p.x, a.nx, ix, wx);
+    grid_coord<T>((unsigned)p.y, a.ny, iy, wy);
+    grid_coord<T>((unsigned)p.z, a.nz, iz, wz);
+  } else {
+    const longlong4 p = pos64[s];
+    grid_coord64<T>((unsigned long long)p.x, a.nx, ix, wx);
+    grid_coord64<T>((unsigned long long)p.y, a.ny, iy, wy);
+    grid_coord64<T>((unsigned long long)p.z, a.nz, iz, wz);
+  }
+}
+
+// Directional derivative of the charge grid along v:
+//   dQ(g) = sum_j q_j grad_j W(g - x_j) . v_j   (64-bit fixed point, atomics)
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_spread_dir(PmPmeLaunch a, const longlong4* __restrict__ pos64,
+                                                    const double4* __restrict__ slot_v) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.counts[0]) return;
+  const int at = a.atom_of_slot[s];
+  if (at < 0) return;
+  int ix, iy, iz;
+  T wx, wy, wz;
+  slot_grid<T>(a, pos64, s, ix, iy, iz, wx, wy, wz);
+  T tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+  bspline<T, P>(wx, tx, dx);
+  bspline<T, P>(wy, ty, dy);
+  bspline<T, P>(wz, tz, dz);
+  const double4 v = slot_v[s];
+  const T q = (T)a.q[at];
+  const T vx = (T)(v.x * a.hr[0]) * q, vy = (T)(v.y * a.hr[1]) * q, vz = (T)(v.z * a.hr[2]) * q;
+  const int off = -(P - 1) / 2;
+  const T scale = (T)PM_GRID_FIX;
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int gx = (ix + off + i + a.nx) % a.nx;
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      const int gy = (iy + off + j + a.ny) % a.ny;
+      const T cxy = vx * dx[i] * ty[j] + vy * tx[i] * dy[j];
+      const T cz = vz * tx[i] * ty[j];
+      unsigned long long* row = a.grid_fix + ((long long)gx * a.ny + gy) * a.nz;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int gz = (iz + off + k + a.nz) % a.nz;
+        const T val = cxy * tz[k] + cz * dz[k];
+        long long iv;
+        if constexpr (sizeof(T) == 4) iv = __float2ll_rn(val * scale);
+        else iv = __double2ll_rn(val * scale);
+        atomicAdd(row + gz, (unsigned long long)iv);
+      }
+    }
+  }
+}
+
+// MODE 0: out[at] += phi(x_at) (dE/dq of the reciprocal energy)
+// MODE 1: out[3 at + c] += q (grad grad W . v)_c summed against phi
+// MODE 2: out[3 at + c] += q (grad W)_c summed against dphi (the grid holds it)
+template <typename T, int P, int MODE>
+__global__ __launch_bounds__(256) void k_gather_adj(PmPmeLaunch a, const longlong4* __restrict__ pos64,
+                                                    const double4* __restrict__ slot_v, double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.counts[0]) return;
+  const int at = a.atom_of_slot[s];
+  if (at < 0) return;
+  int ix, iy, iz;
+  T wx, wy, wz;
+  slot_grid<T>(a, pos64, s, ix, iy, iz, wx, wy, wz);
+  T tx[P], ty[P], tz[P], dx[P], dy[P], dz[P], ex[P], ey[P], ez[P];
+  bspline2<T, P>(wx, tx, dx, ex);
+  bspline2<T, P>(wy, ty, dy, ey);
+  bspline2<T, P>(wz, tz, dz, ez);
+  const T h0 = (T)a.hr[0], h1 = (T)a.hr[1], h2 = (T)a.hr[2];
+  const double4 v = slot_v[s];
+  const T vx = (T)v.x, vy = (T)v.y, vz = (T)v.z;
+  const int off = -(P - 1) / 2;
+  const T* g = reinterpret_cast<const T*>(a.grid_r);
+  T r0 = 0, r1 = 0, r2 = 0;
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int gx = (ix + off + i + a.nx) % a.nx;
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      const int gy = (iy + off + j + a.ny) % a.ny;
+      const T* row = g + ((long long)gx * a.ny + gy) * a.nz;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int gz = (iz + off + k + a.nz) % a.nz;
+        const T phi = row[gz];
+        if constexpr (MODE == 0) {
+          r0 += phi * tx[i] * ty[j] * tz[k];
+        } else if constexpr (MODE == 1) {
+          // Hessian of W(g - x) with respect to x, applied to v
+          const T hxx = h0 * h0 * ex[i] * ty[j] * tz[k], hyy = h1 * h1 * tx[i] * ey[j] * tz[k];
+          const T hzz = h2 * h2 * tx[i] * ty[j] * ez[k];
+          const T hxy = h0 * h1 * dx[i] * dy[j] * tz[k], hxz = h0 * h2 * dx[i] * ty[j] * dz[k];
+          const T hyz = h1 * h2 * tx[i] * dy[j] * dz[k];
+          r0 += phi * (hxx * vx + hxy * vy + hxz * vz);
+          r1 += phi * (hxy * vx + hyy * vy + hyz * vz);
+          r2 += phi * (hxz * vx + hyz * vy + hzz * vz);
+        } else {
+          r0 += phi * h0 * dx[i] * ty[j] * tz[k];
+          r1 += phi * h1 * tx[i] * dy[j] * tz[k];
+          r2 += phi * h2 * tx[i] * ty[j] * dz[k];
+        }
+      }
+    }
+  }
+  if constexpr (MODE == 0) {
+    out[at] += (double)r0;
+  } else {
+    const double q = a.q[at];
+    out[3 * at] += q * (double)r0;
+    out[3 * at + 1] += q * (double)r1;
+    out[3 * at + 2] += q * (double)r2;
+  }
+}
+
 // ------------------------------------------------------------- FFT passes
 // z forward: fixed-point grid -> real -> complex half lines; clears the
 // fixed-point grid for the next spread.
@@ -600,16 +765,12 @@ static int lines_per_block(int n, size_t elem) {
   return tl;
 }
 
-template <typename T, int P>
-static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
-                         const longlong4* pos64) {
+// z forward, y forward, x forward + influence function + x inverse, y
+// inverse, z inverse: the fixed-point charge grid to the real potential grid
+template <typename T>
+static void launch_fft_chain(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans) {
   typedef typename Cx<T>::t T2;
   const int nthreads = 256;
-  {
-    const int nbk = a.nbrick[0] * a.nbrick[1] * a.nbrick[2];
-    const size_t lds = (size_t)a.tile[0] * a.tile[1] * a.tile[2] * sizeof(unsigned long long);
-    hipLaunchKernelGGL((k_spread_tile<T, P>), dim3(nbk), dim3(nthreads), lds, s, a, pos64);
-  }
   {
     const int tl = lines_per_block(a.nz, sizeof(T2));
     const int nb = (a.nx * a.ny + tl - 1) / tl;
@@ -635,7 +796,40 @@ static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* p
     const int nb = (a.nx * a.ny + tl - 1) / tl;
     hipLaunchKernelGGL(k_fft_z_inv<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nz * sizeof(T2), s, a, plans[2], tl);
   }
-  hipLaunchKernelGGL((k_gather<T, P>), dim3((a.nslot_max + 255) / 256), dim3(nthreads), 0, s, a, pos64);
+}
+
+template <typename T, int P>
+static void launch_spread_tile(hipStream_t s, const PmPmeLaunch& a, const longlong4* pos64) {
+  const int nbk = a.nbrick[0] * a.nbrick[1] * a.nbrick[2];
+  const size_t lds = (size_t)a.tile[0] * a.tile[1] * a.tile[2] * sizeof(unsigned long long);
+  hipLaunchKernelGGL((k_spread_tile<T, P>), dim3(nbk), dim3(256), lds, s, a, pos64);
+}
+
+template <typename T, int P>
+static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
+                         const longlong4* pos64) {
+  launch_spread_tile<T, P>(s, a, pos64);
+  launch_fft_chain<T>(s, a, plans);
+  hipLaunchKernelGGL((k_gather<T, P>), dim3((a.nslot_max + 255) / 256), dim3(256), 0, s, a, pos64);
+}
+
+// adjoint: mode 0 the reciprocal potential at every atom (dE/dq), mode 1 the
+// reciprocal H v (the gather of grad grad W against phi, then the spread of
+// the directional charge derivative, its potential and the gather of grad W)
+template <typename T, int P>
+static void launch_pme_adj_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
+                             int mode, const double4* slot_v, double* out) {
+  const dim3 g((a.nslot_max + 255) / 256), b(256);
+  launch_spread_tile<T, P>(s, a, pos64);
+  launch_fft_chain<T>(s, a, plans);
+  if (mode == 0) {
+    hipLaunchKernelGGL((k_gather_adj<T, P, 0>), g, b, 0, s, a, pos64, slot_v, out);
+    return;
+  }
+  hipLaunchKernelGGL((k_gather_adj<T, P, 1>), g, b, 0, s, a, pos64, slot_v, out);
+  hipLaunchKernelGGL((k_spread_dir<T, P>), g, b, 0, s, a, pos64, slot_v);
+  launch_fft_chain<T>(s, a, plans);
+  hipLaunchKernelGGL((k_gather_adj<T, P, 2>), g, b, 0, s, a, pos64, slot_v, out);
 }
 
 int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision) {
@@ -655,6 +849,23 @@ static bool launch_pme_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* p
     case 8: launch_pme_t<T, 8>(s, a, plans, pos64); return true;
     default: return false;
   }
+}
+
+template <typename T>
+static bool launch_pme_adj_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
+                             int mode, const double4* sv, double* out) {
+  switch (a.order) {
+    case 4: launch_pme_adj_t<T, 4>(s, a, plans, pos64, mode, sv, out); return true;
+    case 6: launch_pme_adj_t<T, 6>(s, a, plans, pos64, mode, sv, out); return true;
+    case 8: launch_pme_adj_t<T, 8>(s, a, plans, pos64, mode, sv, out); return true;
+    default: return false;
+  }
+}
+
+bool pm_launch_pme_adjoint(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
+                           int precision, int mode, const double4* slot_v, double* out) {
+  if (precision == 1) return launch_pme_adj_p<double>(s, a, plans, pos64, mode, slot_v, out);
+  return launch_pme_adj_p<float>(s, a, plans, pos64, mode, slot_v, out);
 }
 
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,

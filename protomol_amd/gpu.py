@@ -28,7 +28,8 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_evaluate", "pm_cell_diagnostics", "pm_md_setup", "pm_md_set_state", "pm_md_run",
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
-           "pm_batch_exchange", "pm_batch_ladder"]
+           "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
+           "pm_set_charges", "pm_lanczos"]
 OPTIONS = {"cluster_size": 0, "graph": 1}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
@@ -86,6 +87,11 @@ def load_library(path: str = LIB_PATH):
     lib.pm_batch_run.argtypes = [P, C.c_int]
     lib.pm_batch_exchange.argtypes = [P, C.c_int, C.c_ulonglong, I, D]
     lib.pm_batch_ladder.argtypes = [P, I, D]
+    lib.pm_set_lj_per_atom.argtypes = [P, D, D, D, D]
+    lib.pm_hessian_vec.argtypes = [P, I, C.c_int, D, D, D]
+    lib.pm_param_gradients.argtypes = [P, I, C.c_int, D, D, D, D]
+    lib.pm_set_charges.argtypes = [P, D]
+    lib.pm_lanczos.argtypes = [P, I, C.c_int, D, D, C.c_int, C.c_int, C.c_int, D, D, D, D]
     _lib = lib
     return lib
 
@@ -234,6 +240,56 @@ class Context:
         cnt = np.zeros(len(TIMING), np.int64)
         self._check(self.lib.pm_get_timing(self.h, _dp(ms), _lp(cnt), len(TIMING)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(TIMING)}
+
+    def set_lj_per_atom(self, sigma, epsilon, sigma14=None, epsilon14=None):
+        """Per-atom LJ sigma/epsilon (CHARMM combination per pair); None clears."""
+        if sigma is None:
+            self._check(self.lib.pm_set_lj_per_atom(self.h, None, None, None, None))
+            return
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (sigma, epsilon, sigma14, epsilon14)]
+        self._keep_lj = arrs
+        self._check(self.lib.pm_set_lj_per_atom(self.h, *[None if a is None else _dp(a) for a in arrs]))
+
+    def hessian_vec(self, handles, positions, v):
+        hs = np.ascontiguousarray(handles, np.int32)
+        x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        vv = np.ascontiguousarray(v, np.float64).reshape(-1)
+        hv = np.zeros(3 * self.n)
+        self._check(self.lib.pm_hessian_vec(self.h, _ip(hs), len(hs), _dp(x), _dp(vv), _dp(hv)))
+        return hv.reshape(-1, 3)
+
+    def param_gradients(self, handles, positions, lj=True):
+        hs = np.ascontiguousarray(handles, np.int32)
+        x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        dq = np.zeros(self.n)
+        ds = np.zeros(self.n) if lj else None
+        de = np.zeros(self.n) if lj else None
+        self._check(self.lib.pm_param_gradients(self.h, _ip(hs), len(hs), _dp(x), _dp(dq),
+                                                _dp(ds) if lj else None, _dp(de) if lj else None))
+        return dq, ds, de
+
+    def set_charges(self, q):
+        """Replace the scaled charges (double[n])."""
+        qq = np.ascontiguousarray(q, np.float64).reshape(-1)
+        if qq.size != self.n:
+            raise ValueError(f"{qq.size} charges for {self.n} atoms")
+        self._check(self.lib.pm_set_charges(self.h, _dp(qq)))
+
+    def lanczos(self, handles, positions, v0, steps=30, nvec=0, mass_weighted=False):
+        """`steps` Lanczos steps (full reorthogonalisation, on the device) on H or
+        M^-1/2 H M^-1/2.  Returns (ritz_values[steps] ascending,
+        ritz_vectors[nvec, n, 3] of the lowest, residual bounds[steps],
+        alpha[steps], beta[steps])."""
+        hs = np.ascontiguousarray(handles, np.int32)
+        x = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        v = np.ascontiguousarray(v0, np.float64).reshape(-1)
+        vals = np.zeros(steps)
+        vecs = np.zeros((max(nvec, 1), 3 * self.n))
+        res = np.zeros(steps)
+        ab = np.zeros(2 * steps)
+        self._check(self.lib.pm_lanczos(self.h, _ip(hs), len(hs), _dp(x), _dp(v), int(steps), int(bool(mass_weighted)),
+                                        int(nvec), _dp(vals), _dp(vecs) if nvec else None, _dp(res), _dp(ab)))
+        return vals, vecs[:nvec].reshape(nvec, self.n, 3), res, ab[:steps], ab[steps:]
 
     def set_option(self, name: str, value: int):
         self._check(self.lib.pm_set_option(self.h, OPTIONS[name], int(value)))

@@ -208,3 +208,30 @@ def replica_boxes(count: int = 64, seed: int = 0, lo: int = 100, hi: int = 1000,
     molecules) at 0.0334 molecules/A^3, each generated from its own seed."""
     n, edges = replica_sizes(count, seed, lo, hi)
     return [water_box(int(m), float(L), seed=1000 * seed + k, jitter=jitter) for k, (m, L) in enumerate(zip(n, edges))]
+
+
+def perturbed_box(name: str = "water8k", seed: int = 0, dq: float = 0.01, dlj: float = 0.02, jitter: float = 0.3):
+    """A water box with seeded per-atom parameter noise: charges + N(0, dq) e,
+    shifted back to a neutral total; LJ sigma and epsilon x (1 + dlj N(0, 1))
+    per atom (zero parameters stay zero).  Returns the system and the per-atom
+    sigma and epsilon arrays."""
+    s = named(name, seed=seed, jitter=jitter)
+    rng = np.random.default_rng(10_000 + seed)
+    q = s.q / SQRT_COULOMB + rng.normal(0.0, dq, s.n)
+    q -= q.mean()
+    s.q = q * SQRT_COULOMB
+    base_sigma = np.where(s.type == 0, SIGMA_O, 0.0)
+    base_eps = np.where(s.type == 0, EPS_O, 0.0)
+    sigma = base_sigma * (1.0 + dlj * rng.standard_normal(s.n))
+    eps = base_eps * (1.0 + dlj * rng.standard_normal(s.n))
+    return s, sigma, eps
+
+
+def per_atom_lj_tables(sigma, eps):
+    """Type tables with one type per atom that reproduce per-atom CHARMM
+    combination (for checkers on small systems)."""
+    R = np.asarray(sigma) * 2.0 ** (-5.0 / 6.0)
+    e = np.sqrt(np.asarray(eps))
+    rs = R[:, None] + R[None, :]
+    ee = e[:, None] * e[None, :]
+    return rs ** 12 * ee, 2.0 * rs ** 6 * ee

@@ -96,3 +96,29 @@ def replica_context(make, s, temperature, seed, precision="mixed"):
     c.md_setup(hs, "langevin", dt=1.0, temperature=float(temperature), gamma=5.0, seed=seed)
     c.md_set_state(s.pos, s.vel)
     return c
+
+
+def lanczos_reference(matvec, v0, steps):
+    """Lanczos with full reorthogonalisation (two classical Gram-Schmidt
+    passes, alpha_j = the v_j coefficients of both), restated in numpy as the
+    checker of the device iteration.  Returns (alpha, beta, V[steps + 1])."""
+    n = v0.size
+    V = np.zeros((steps + 1, n))
+    V[0] = v0.reshape(-1) / np.linalg.norm(v0)
+    alpha = np.zeros(steps)
+    beta = np.zeros(steps)
+    for j in range(steps):
+        w = np.asarray(matvec(V[j]), float).reshape(-1)
+        for _ in range(2):
+            c = V[:j + 1] @ w
+            w = w - c @ V[:j + 1]
+            alpha[j] += c[j]
+        beta[j] = np.linalg.norm(w)
+        V[j + 1] = w / beta[j] if beta[j] > 1e-300 else 0.0
+    return alpha, beta, V
+
+
+def tridiagonal_eigen(alpha, beta):
+    m = len(alpha)
+    T = np.diag(alpha) + np.diag(beta[:m - 1], 1) + np.diag(beta[:m - 1], -1)
+    return np.linalg.eigh(T)
