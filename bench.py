@@ -70,21 +70,26 @@ def pme_alpha(rc, accuracy):
     return alpha
 
 
-def run_md(s, precision, steps, warmup, timing=()):
+def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1):
     from protomol_amd import gpu
     ctx = gpu.Context(0, precision)
     ctx.set_topology(s)
     ctx.set_boundary(s.box, s.origin, CELL)
+    ctx.set_option("graph", 1 if graph else 0)
+    ctx.set_option("list_skin", skin)
     hs = make_forces(ctx, pme_alpha(RC, 1e-6))
     ctx.md_setup(hs, "leapfrog", dt=DT_FS)
     ctx.md_set_state(s.pos, s.vel)
     ctx.md_run(max(warmup, 1))
     ctx.synchronize()
     ctx.set_timing(timing)
+    builds0 = ctx.stats()["list_builds"]
     t0 = time.perf_counter()
     ctx.md_run(steps)
     ctx.synchronize()
     t1 = time.perf_counter()
+    builds = ctx.stats()["list_builds"] - builds0
+    skin_used = ctx.get_option("list_skin")
     tim = ctx.timing()
     x, v, e = ctx.md_get_state()
     # in-cutoff, non-excluded pairs at the final positions (reference enumerator count)
@@ -92,7 +97,22 @@ def run_md(s, precision, steps, warmup, timing=()):
     pairs = int(diag["inrc"].sum())
     stats = ctx.stats()
     ctx.close()
-    return dict(seconds=t1 - t0, timing=tim, pairs=pairs, energies=e, x=x, v=v, stats=stats)
+    return dict(seconds=t1 - t0, timing=tim, pairs=pairs, energies=e, x=x, v=v, stats=stats, builds=builds,
+                skin=skin_used * 1e-3)
+
+
+def copy_probe(mb):
+    """Ten device-to-device copies of `mb` megabytes: the kernel whose
+    FETCH_SIZE / WRITE_SIZE counters calibrate the HBM-traffic runs."""
+    import torch
+    n = mb * 1000 * 1000 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 10}))
 
 
 def nve_drift(s, precision, steps=1000, chunk=100, equilibrate=2000):
@@ -224,7 +244,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-drift", action="store_true", help="skip the 1000-step NVE drift runs")
     ap.add_argument("--no-replicas", action="store_true", help="skip the batched replica-exchange measurement")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: MD steps replay captured HIP graphs (default); 0: direct launches (for counter runs)")
+    ap.add_argument("--list-skin", type=int, default=-1, metavar="MILLI_A",
+                    help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run 10 device copies of MB megabytes (the calibration kernel of HBM counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {args.gpus} is not run "
               f"(multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
@@ -233,12 +262,17 @@ def main():
     s = build_system()
     alpha = pme_alpha(RC, 1e-6)
 
-    mixed = run_md(s, "mixed", steps, warmup, timing=("pair", "pme"))
-    ms_step = mixed["seconds"] * 1e3 / steps
-    ns_day = (steps / mixed["seconds"]) * DT_FS * 86400 * 1e-6
+    # the throughput pass: no instrumentation (captured graphs unless --graph 0)
+    run = run_md(s, "mixed", steps, warmup, graph=bool(args.graph), skin=args.list_skin)
+    ms_step = run["seconds"] * 1e3 / steps
+    ns_day = (steps / run["seconds"]) * DT_FS * 86400 * 1e-6
+    # the roofline pass: the same steps with HIP events around the pair and PME
+    # kernels on the streams they run on (event timing launches directly)
+    mixed = run_md(s, "mixed", steps, warmup, timing=("pair", "pme"), skin=args.list_skin)
+    ms_step_events = mixed["seconds"] * 1e3 / steps
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(10, steps // 10)
-    fp64 = run_md(s, "double", fp64_steps, max(2, warmup // 10))
+    fp64 = run_md(s, "double", fp64_steps, max(2, warmup // 10), skin=args.list_skin)
     ns_day_fp64 = (fp64_steps / fp64["seconds"]) * DT_FS * 86400 * 1e-6
 
     # roofline of the dominant kernel (the pair kernel), from live HIP events
@@ -261,6 +295,10 @@ def main():
         "steps": steps,
         "warmup": warmup,
         "ms_per_step": ms_step,
+        "ms_per_step_with_kernel_events": ms_step_events,
+        "graph": bool(args.graph),
+        "list_skin_A": run["skin"],
+        "list_builds_per_step": run["builds"] / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -164,7 +164,8 @@ int pm_md_get_state(pm_context* ctx, double* x, double* v, double* energies);
 
 /* Kernel statistics of the last evaluation (for roofline reporting):
  * stats[0] pair-list entries (i-cluster x j-atom), stats[1] pairs evaluated
- * (lanes issued), stats[2] clusters, stats[3] padded slots. */
+ * (lanes issued), stats[2] clusters, stats[3] padded slots, stats[4] pair-list
+ * builds of reusing evaluations since the context was created (PM_OPT_LIST_SKIN). */
 int pm_last_stats(pm_context* ctx, long long* stats, int nstats);
 
 /* Kernel timing with HIP events recorded on the context's stream around the
@@ -229,8 +230,19 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        of up to 8 steps; 0: every kernel launched directly.
  *                        Steps are captured only while no kernel timing is
  *                        requested (pm_set_timing mask 0).
+ *   PM_OPT_LIST_SKIN     pair-list buffer in 1e-3 A: -1 (default) automatic
+ *                        (600), 0 off (cells and lists rebuilt at every
+ *                        evaluation, as the reference rebuilds its cell list,
+ *                        ForceGroup.cpp:50), 1..3000.  Lists then hold the j
+ *                        atoms within the pair cutoff + skin and are rebuilt
+ *                        on the device when an atom has moved more than
+ *                        skin / 2 since the last build; every evaluation still
+ *                        computes exactly the pairs within the cutoff.  Used
+ *                        for one pair pass whose algorithm cutoff covers its
+ *                        pair cutoff in a box wider than 2 (cutoff + skin +
+ *                        cell diagonal); otherwise lists are rebuilt each time.
  * pm_get_option returns the value in effect (after the automatic choice). */
-enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1 };
+enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

@@ -23,7 +23,9 @@ from . import gpu
 
 
 def _np(t):
-    return np.ascontiguousarray(t.detach().to("cpu", torch.float64).numpy())
+    # a copy: numpy() of a CPU float64 tensor shares its storage, and a caller
+    # may change the tensor in place after the call
+    return np.array(t.detach().to("cpu", torch.float64).numpy(), dtype=np.float64, copy=True, order="C")
 
 
 class ForceGroup:
@@ -100,7 +102,22 @@ class _Gradient(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gg):
         pos, q, sigma, eps = ctx.saved_tensors
-        group = ctx.group
+        return _HessianVec.apply(pos, q, sigma, eps, ctx.group, gg), None, None, None, None, None
+
+
+class _HessianVec(torch.autograd.Function):
+    """H u as a function of u (linear, H symmetric: its backward is H again),
+    so that torch's double-backward constructions (functional.hvp) see it."""
+
+    @staticmethod
+    def forward(ctx, pos, q, sigma, eps, group, u):
+        ctx.group = group
+        ctx.save_for_backward(pos, q, sigma, eps)
         group.sync(q, sigma, eps)
-        hv = group.ctx.hessian_vec(group.handles, _np(pos), _np(gg))
-        return torch.from_numpy(hv).to(gg.device, gg.dtype), None, None, None, None, None
+        hv = group.ctx.hessian_vec(group.handles, _np(pos), _np(u))
+        return torch.from_numpy(hv).to(u.device, u.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        pos, q, sigma, eps = ctx.saved_tensors
+        return None, None, None, None, None, _HessianVec.apply(pos, q, sigma, eps, ctx.group, g)

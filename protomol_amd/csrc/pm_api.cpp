@@ -6,6 +6,9 @@
 #include "pmgpu.h"
 #include "pm_internal.h"
 
+// automatic pair-list skin (A) when PM_OPT_LIST_SKIN is -1
+#define PM_LIST_SKIN_AUTO 0.6
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -169,12 +172,20 @@ struct pm_context {
 
   // captured MD steps (hipGraph), replayed while nothing they reference changes
   int graph_option = 1;              // PM_OPT_GRAPH
+
+  // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
+  // cutoff + skin and are rebuilt, with the cell layout, when an atom has
+  // moved more than skin / 2 since the build (checked on the device)
+  int skin_milli = -1;               // -1 automatic, 0 off, else 1e-3 A
+  int* d_rebuild = nullptr;          // device flag: 1 = rebuild in this evaluation
+  int4* d_xbuild = nullptr;          // [n] fixed-point position at the last build, w = 1 when valid
+  bool list_stale = true;            // x_build must be invalidated before the next reuse
   unsigned long long gen = 1;        // bumped by every allocation and every setter
   hipGraphExec_t md_exec = nullptr;
   int md_exec_steps = 0, md_exec_kind = -1;
   unsigned long long md_exec_gen = 0;
 
-  long long last_stats[4] = {0, 0, 0, 0};
+  long long last_stats[5] = {0, 0, 0, 0, 0};
 
   // kernel timing (HIP events on the context's stream), per category
   unsigned timing_mask = 0;
@@ -380,6 +391,12 @@ int choose_m(const pm_context* c) {
   return 8;
 }
 
+// list skin in A: the option, or the automatic choice
+double list_skin(const pm_context* c) {
+  if (c->skin_milli >= 0) return 1e-3 * c->skin_milli;
+  return PM_LIST_SKIN_AUTO;
+}
+
 int ensure_cells(pm_context* c) {
   c->m = choose_m(c);
   if (c->cells_alloc == c->ncells && c->cells_m == c->m && c->d_cell_count && c->d_slot_pos) return PM_OK;
@@ -403,14 +420,18 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_slot_meta, (size_t)c->nslot_max + 1))) return r;
   if ((r = dalloc(c, &c->d_atom_of_slot, (size_t)c->nslot_max + 1))) return r;
   if ((r = dalloc(c, &c->d_force_slot, 3 * ((size_t)c->nslot_max + 1)))) return r;
+  if ((r = dalloc(c, &c->d_xbuild, (size_t)n))) return r;
+  if (!c->d_rebuild && (r = dalloc(c, &c->d_rebuild, 1))) return r;
+  HIPCHECK(c, hipMemset(c->d_rebuild, 0, sizeof(int)));
+  c->list_stale = true;
   HIPCHECK(c, hipMemset(c->d_counts, 0, 4 * sizeof(int)));
   if (!c->d_overflow) {
     if ((r = dalloc(c, &c->d_overflow, 1))) return r;
     HIPCHECK(c, hipMemset(c->d_overflow, 0, sizeof(int)));
   }
   if (!c->d_stats) {
-    if ((r = dalloc(c, &c->d_stats, 2))) return r;
-    HIPCHECK(c, hipMemset(c->d_stats, 0, 2 * sizeof(unsigned long long)));
+    if ((r = dalloc(c, &c->d_stats, 3))) return r;
+    HIPCHECK(c, hipMemset(c->d_stats, 0, 3 * sizeof(unsigned long long)));
   }
   c->cells_alloc = nc;
   c->cells_m = m;
@@ -827,21 +848,49 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
 
   // binning and slot layout (shared by every pass and by PME)
   if ((r = ensure_cells(c))) return r;
-  if (!c->d_force || !c->d_pos || !c->d_q || !c->d_type || !c->d_stats || !c->d_overflow || !c->d_energy)
+  if (!c->d_force || !c->d_pos || !c->d_q || !c->d_type || !c->d_stats || !c->d_overflow || !c->d_energy ||
+      !c->d_rebuild || !c->d_xbuild)
     return fail(c, PM_ERR_STATE, "internal: device buffer missing before launch");
   const int nc = c->ncells;
+  // list reuse: one pair pass whose algorithm cutoff covers its pair cutoff
+  // (the reference's cell pairs then hold every pair within the cutoff, so
+  // the pair set does not depend on the cells), in a box where the skin
+  // keeps the minimal image unique; never for adjoint passes
+  const double skin = list_skin(c);
+  bool reuse = false;
+  if (skin > 0.0 && passes.size() == 1 && !adj) {
+    const double pc = std::sqrt(pair_params(passes[0].terms, c->csf).cut2);
+    const double lmin_h = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
+    const double cd = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
+                                c->realcell[2] * c->realcell[2]);
+    reuse = passes[0].algo_cutoff >= pc && lmin_h > pc + skin + cd + 1e-3;
+  }
+  const int* rb = reuse ? c->d_rebuild : nullptr;
   {
   TimedScope ts(c, PM_T_BIN);
-  pm_launch_zero_force_cells(s, c->d_force, 3LL * n, c->d_force_slot, passes.empty() ? 0 : 3LL * c->nslot_max,
+  if (reuse) {
+    if (c->list_stale) {
+      pm_launch_invalidate_build(s, n, c->d_xbuild);
+      c->list_stale = false;
+    }
+    const float half = (float)(0.5 * skin);
+    pm_launch_check_disp(s, c->box, n, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
+  }
+  pm_launch_zero_force_cells(s, rb, c->d_force, 3LL * n, c->d_force_slot, passes.empty() ? 0 : 3LL * c->nslot_max,
                              c->d_cell_count, nc);
-  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key);
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key, rb);
   pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
-                       c->d_cluster_cell, c->ncl_max, c->m);
-  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms);
+                       c->d_cluster_cell, c->ncl_max, c->m, rb);
+  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, rb);
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
                        c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
-                       c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m);
+                       c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m, rb, reuse ? c->d_xbuild : nullptr);
+  if (reuse)
+    pm_launch_refresh_slots(s, c->box, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_pos, c->d_q,
+                            c->d_slot_pos, c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr);
+  else
+    c->list_stale = true;   // a layout built without the skin is not reusable
   }
 
   // energy records: pair passes, PME x-pass blocks, correction blocks
@@ -878,11 +927,12 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     PmPairParams pp = pair_params(ps.terms, c->csf);
     int4* st;
     int nst;
-    if ((r = get_stencil(c, ps.algo_cutoff, false, &st, &nst))) return r;
+    const double lskin = reuse ? skin : 0.0;
+    if ((r = get_stencil(c, ps.algo_cutoff + lskin, false, &st, &nst))) return r;
     if (nst > 1300)
       return fail(c, PM_ERR_UNSUPPORTED, "cell stencil of %d cells (cellsize %g, cutoff %g): use a cellsize of at least cutoff/4", nst, c->cellsize, ps.algo_cutoff);
     double pcut = std::sqrt(pp.cut2);
-    if ((r = ensure_lists(c, list_capacity(c, pcut)))) return r;
+    if ((r = ensure_lists(c, list_capacity(c, pcut + lskin)))) return r;
     pair_launch.push_back(std::make_pair(PmNbLaunch{}, pp));
     PmNbLaunch& a = pair_launch.back().first;
     a.slot_pos = c->d_slot_pos;
@@ -898,7 +948,9 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.nstencil = nst;
     a.lcap = c->lcap;
     a.overflow = c->d_overflow;
-    a.bbox_filter = lmin_half > pcut + cdiag + 1e-3 ? 1 : 0;
+    a.bbox_filter = lmin_half > pcut + lskin + cdiag + 1e-3 ? 1 : 0;
+    a.rebuild = rb;
+    a.skin = lskin;
     a.max_delta = c->max_delta;
     a.xs_row = c->d_xs_row;
     a.xs_part = c->d_xs_part;
@@ -1089,6 +1141,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       pm_launch_pair(s, c->box, pl.first, pl.second, c->precision);
     }
   }
+  if (reuse) pm_launch_end_build(s, c->d_rebuild, c->d_stats + 2);   // every reader of the flag has run
   // join, then the per-slot pair forces into the per-atom forces
   if (side) HIPCHECK(c, hipStreamWaitEvent(s, c->ev_join, 0));
   if (!passes.empty()) {
@@ -1118,7 +1171,7 @@ int check_overflow(pm_context* c) {
 
 // list statistics of the last evaluation (on demand: one reduction + copy)
 int list_stats(pm_context* c) {
-  unsigned long long st[2] = {0, 0};
+  unsigned long long st[3] = {0, 0, 0};   // entries, lanes, list builds (cumulative)
   int counts[4] = {0, 0, 0, 0};
   if (c->d_stats && c->d_jcount && c->d_counts) {
     pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats, c->m);
@@ -1130,6 +1183,7 @@ int list_stats(pm_context* c) {
   c->last_stats[1] = (long long)st[1];
   c->last_stats[2] = counts[1];
   c->last_stats[3] = counts[0];
+  c->last_stats[4] = (long long)st[2];
   return PM_OK;
 }
 
@@ -1146,6 +1200,7 @@ void free_all(pm_context* c) {
   dfree(c->d_lj_f); dfree(c->d_lj_d);
   dfree(c->d_pos); dfree(c->d_vel); dfree(c->d_force); dfree(c->d_energy); dfree(c->d_scratch3);
   dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
+  dfree(c->d_xbuild); dfree(c->d_rebuild); c->list_stale = true;
   dfree(c->d_cell_rank); dfree(c->d_force_slot); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
   dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_overflow); dfree(c->d_stats); dfree(c->d_epart); dfree(c->d_ered);
@@ -1224,6 +1279,7 @@ void pm_destroy(pm_context* ctx) {
 }
 
 int pm_set_precision(pm_context* c, int precision) {
+  if (c) c->list_stale = true;
   if (!c) return PM_ERR_ARGUMENT;
   if (precision != PM_PRECISION_MIXED && precision != PM_PRECISION_DOUBLE)
     return fail(c, PM_ERR_ARGUMENT, "precision must be PM_PRECISION_MIXED (0) or PM_PRECISION_DOUBLE (1)");
@@ -1526,6 +1582,7 @@ int pm_set_boundary(pm_context* c, const double* e1, const double* e2, const dou
   c->cells_alloc = 0;
   c->pme_alpha_tables = -1;
   c->have_box = true;
+  c->list_stale = true;
   for (auto& f : c->forces) f.cached = false;
   c->md_forces_valid = false;
   c->gen++;
@@ -1596,11 +1653,12 @@ int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm
   if ((r = ensure_cells(c))) return r;
   hipStream_t s = c->stream;
   HIPCHECK(c, hipMemcpyAsync(c->d_pos, positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  c->list_stale = true;   // the binning below replaces the layout's cell lists
   pm_launch_zero_i(s, c->d_cell_count, nc);
-  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key);
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key, nullptr);
   pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
-                       c->d_cluster_cell, c->ncl_max, c->m);
-  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms);
+                       c->d_cluster_cell, c->ncl_max, c->m, nullptr);
+  pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, nullptr);
   std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);
   HIPCHECK(c, hipMemcpyAsync(cell.data(), c->d_cell_of, n * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(c, hipMemcpyAsync(cnt.data(), c->d_cell_count, nc * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1816,7 +1874,7 @@ int pm_last_stats(pm_context* c, long long* stats, int nstats) {
   int r;
   if ((r = set_device(c))) return r;
   if ((r = list_stats(c))) return r;
-  for (int k = 0; k < nstats && k < 4; k++) stats[k] = c->last_stats[k];
+  for (int k = 0; k < nstats && k < 5; k++) stats[k] = c->last_stats[k];
   return PM_OK;
 }
 
@@ -2118,6 +2176,14 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->graph_option = value;
     return PM_OK;
   }
+  if (option == PM_OPT_LIST_SKIN) {
+    if (value < -1 || value > 3000)
+      return fail(c, PM_ERR_ARGUMENT, "list skin must be -1 (automatic), 0 (off) or 1..3000 (1e-3 A; got %d)", value);
+    c->skin_milli = value;
+    c->list_stale = true;
+    c->md_forces_valid = false;
+    return PM_OK;
+  }
   return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);
 }
 
@@ -2129,6 +2195,10 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_GRAPH) {
     *value = c->graph_option;
+    return PM_OK;
+  }
+  if (option == PM_OPT_LIST_SKIN) {
+    *value = (int)std::lround(list_skin(c) * 1000.0);
     return PM_OK;
   }
   return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);

@@ -4,8 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// Fixed-point scales.  Forces: 2^32 units per kcal/mol/A in int64 (range
-// +-2^31 kcal/mol/A, resolution 2.3e-10).  PME charge grid: 2^40 units per
+// Fixed-point scales.  Forces: 2^40 units per kcal/mol/A in int64 (range
+// +-2^23 kcal/mol/A, resolution 9.1e-13).  PME charge grid: 2^40 units per
 // scaled charge (range +-2^23, resolution 9e-13).
 #define PM_FORCE_FIX 1099511627776.0
 #define PM_FORCE_UNFIX (1.0 / 1099511627776.0)
@@ -86,6 +86,10 @@ struct PmNbLaunch {
   const int* counts;         // [0] nslot, [1] nclusters
   int ncl_max;
   int dummy_slot;
+  // list reuse: the list kernel runs only when *rebuild != 0 (null: always),
+  // listing j atoms within the pair cutoff + skin of the cluster's box
+  const int* rebuild;
+  double skin;
   // cells
   const int* cell_slot;
   const int* cell_count;
@@ -126,17 +130,31 @@ struct PmNbLaunch {
 
 // ---------------------------------------------------------------- launchers
 // Binning (reference-exact) and slot layout.
+// `rebuild` (device flag, may be null = always): the kernels return at once
+// when it is 0, keeping the layout of the last build.
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
-                   int* cell_count, int* rank, float* key);
+                   int* cell_count, int* rank, float* key, const int* rebuild);
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
-                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m);
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild);
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
-                       int* cell_atoms);
+                       int* cell_atoms, const int* rebuild);
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
                           const double* q, const int* type, const int* cell_count,
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
-                          int* atom_of_slot, int dummy_type, int dummy_slot, int m);
+                          int* atom_of_slot, int dummy_type, int dummy_slot, int m, const int* rebuild,
+                          int4* x_build);
+// list reuse: *flag = 1 when an atom moved more than sqrt(lim2) A from its
+// position at the last build (x_build, 32-bit fixed point, w = 1 when valid)
+void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* pos, const int4* x_build,
+                          float lim2, int* flag);
+void pm_launch_set_flag(hipStream_t s, int* flag, int v);
+// builds[0] += (*flag != 0); *flag = 0
+void pm_launch_end_build(hipStream_t s, int* flag, unsigned long long* builds);
+void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build);
+void pm_launch_refresh_slots(hipStream_t s, const PmBox& box, int nslot_max, const int* counts,
+                             const int* atom_of_slot, const double* pos, const double* q, int4* slot_pos,
+                             longlong4* slot_pos64);
 // force_fix[atom] += force_slot[slot] for every real slot
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
                           const long long* force_slot, long long* force_fix);
@@ -231,8 +249,8 @@ int pm_reduce_scratch_records(int nparts);
 void pm_launch_zero_ll(hipStream_t s, long long* p, long long count);
 void pm_launch_zero_i(hipStream_t s, int* p, long long count);
 void pm_launch_zero_d(hipStream_t s, double* p, long long count);
-void pm_launch_zero_force_cells(hipStream_t s, long long* f, long long nf, long long* fs, long long nfs, int* cc,
-                                int nc);
+void pm_launch_zero_force_cells(hipStream_t s, const int* rebuild, long long* f, long long nf, long long* fs,
+                                long long nfs, int* cc, int nc);
 void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,

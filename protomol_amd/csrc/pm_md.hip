@@ -215,11 +215,14 @@ __global__ void k_zero_i(int* p, long long count) {
   if (i < count) p[i] = 0;
 }
 // clears the fixed-point forces and the cell counts of one evaluation in one launch
-__global__ void k_zero_force_cells(long long* f, long long nf, long long* fs, long long nfs, int* cc, int nc) {
+// force arrays, and the cell counts of the binning when it runs (*rebuild != 0
+// or rebuild null)
+__global__ void k_zero_force_cells(const int* rebuild, long long* f, long long nf, long long* fs, long long nfs,
+                                   int* cc, int nc) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nf) f[i] = 0;
   if (i < nfs) fs[i] = 0;
-  if (i < nc) cc[i] = 0;
+  if (i < nc && (!rebuild || *rebuild != 0)) cc[i] = 0;
 }
 __global__ void k_zero_d(double* p, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,11 +362,12 @@ void pm_launch_zero_ll(hipStream_t s, long long* p, long long count) {
 void pm_launch_zero_i(hipStream_t s, int* p, long long count) {
   if (count > 0) hipLaunchKernelGGL(k_zero_i, grid_for(count, 256), dim3(256), 0, s, p, count);
 }
-void pm_launch_zero_force_cells(hipStream_t s, long long* f, long long nf, long long* fs, long long nfs, int* cc,
-                                int nc) {
+void pm_launch_zero_force_cells(hipStream_t s, const int* rebuild, long long* f, long long nf, long long* fs,
+                                long long nfs, int* cc, int nc) {
   long long m = nf > nc ? nf : nc;
   if (nfs > m) m = nfs;
-  if (m > 0) hipLaunchKernelGGL(k_zero_force_cells, grid_for(m, 256), dim3(256), 0, s, f, nf, fs, nfs, cc, nc);
+  if (m > 0)
+    hipLaunchKernelGGL(k_zero_force_cells, grid_for(m, 256), dim3(256), 0, s, rebuild, f, nf, fs, nfs, cc, nc);
 }
 void pm_launch_zero_d(hipStream_t s, double* p, long long count) {
   if (count > 0) hipLaunchKernelGGL(k_zero_d, grid_for(count, 256), dim3(256), 0, s, p, count);

@@ -53,9 +53,10 @@ __device__ inline int cell_index(const PmBox& b, double px, double py, double pz
 }
 
 __global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
-                      int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key) {
+                      int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key,
+                      const int* __restrict__ rebuild) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (rebuild && *rebuild == 0)) return;
   double mz;
   int c = cell_index(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &mz);
   cell_of[i] = c;
@@ -69,7 +70,8 @@ __global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __r
 __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
                                                      int* __restrict__ start, int* __restrict__ slot,
                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
-                                                     int ncl_max, int m_cl) {
+                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) {
+  if (rebuild && *rebuild == 0) return;   // uniform over the block: before any barrier
   __shared__ int s_c[8192];
   __shared__ int s_wa[16], s_ws[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -131,9 +133,10 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __re
 }
 
 __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
-                          const int* __restrict__ cell_start, int* __restrict__ cell_atoms) {
+                          const int* __restrict__ cell_start, int* __restrict__ cell_atoms,
+                          const int* __restrict__ rebuild) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (rebuild && *rebuild == 0)) return;
   cell_atoms[cell_start[cell_of[i]] + rank[i]] = i;
 }
 
@@ -163,7 +166,9 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
                              const int* __restrict__ cell_slot, const int* __restrict__ cell_atoms,
                              const float* __restrict__ key, int4* __restrict__ slot_pos,
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
-                             int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl) {
+                             int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
+                             const int* __restrict__ rebuild, int4* __restrict__ x_build) {
+  if (rebuild && *rebuild == 0) return;
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c == 0 && lane == 0) {
@@ -208,6 +213,7 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
     }
     slot_meta[s0 + rank] = type[ai] | (ai << 8);
     atom_of_slot[s0 + rank] = ai;
+    if (x_build) x_build[ai] = make_int4(p.x, p.y, p.z, 1);   // the position the layout was built at
   }
   // padding slots: never used as j atoms, masked as i atoms, skipped by PME
   const int ps = (k + m_cl - 1) / m_cl * m_cl;
@@ -230,6 +236,66 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
     }
     slot_meta[s0 + u] = dummy_type | (0x7fffff << 8);
     atom_of_slot[s0 + u] = -1;
+  }
+}
+
+// List reuse: rebuild (flag = 1) when an atom has moved more than half the
+// list skin from where the layout was built (or the layout is invalid: w != 1).
+// Every thread that finds one writes the same value, so no atomics are needed.
+__global__ void k_check_disp(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
+                             float lim2, int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 b = x_build[i];
+  bool need = b.w != 1;
+  if (!need) {
+    const float dx = (float)(int)(to_fix(pos[3 * i], box.to_fix[0]) - (unsigned int)b.x) * (float)box.fix_to_A[0];
+    const float dy = (float)(int)(to_fix(pos[3 * i + 1], box.to_fix[1]) - (unsigned int)b.y) * (float)box.fix_to_A[1];
+    const float dz = (float)(int)(to_fix(pos[3 * i + 2], box.to_fix[2]) - (unsigned int)b.z) * (float)box.fix_to_A[2];
+    need = !(dx * dx + dy * dy + dz * dz <= lim2);   // NaN positions rebuild too
+  }
+  if (need) *flag = 1;
+}
+
+__global__ void k_set_flag(int* flag, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
+}
+
+// end of a reusing evaluation: count a build, clear the flag
+__global__ void k_end_build(int* flag, unsigned long long* builds) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (*flag) builds[0] += 1ull;
+    *flag = 0;
+  }
+}
+
+__global__ void k_invalidate_build(int n, int4* __restrict__ x_build) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x_build[i] = make_int4(0, 0, 0, 0);
+}
+
+// Current positions (and charges) into the slots of the kept layout, with the
+// arithmetic of k_fill_slots; padding slots keep their place (they are masked).
+__global__ void k_refresh_slots(PmBox box, const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+                                const double* __restrict__ pos, const double* __restrict__ q,
+                                int4* __restrict__ slot_pos, longlong4* __restrict__ slot_pos64) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= counts[0]) return;
+  const int ai = atom_of_slot[sl];
+  if (ai < 0) return;
+  int4 p;
+  p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
+  p.y = (int)to_fix(pos[3 * ai + 1], box.to_fix[1]);
+  p.z = (int)to_fix(pos[3 * ai + 2], box.to_fix[2]);
+  p.w = __float_as_int((float)q[ai]);
+  slot_pos[sl] = p;
+  if (slot_pos64) {
+    longlong4 p8;
+    p8.x = (long long)to_fix64(pos[3 * ai], box.to_fix[0]);
+    p8.y = (long long)to_fix64(pos[3 * ai + 1], box.to_fix[1]);
+    p8.z = (long long)to_fix64(pos[3 * ai + 2], box.to_fix[2]);
+    p8.w = __double_as_longlong(q[ai]);
+    slot_pos64[sl] = p8;
   }
 }
 
@@ -282,6 +348,7 @@ __device__ inline int wrap_cell(int c, int d, int n) {
 // modifies; its masks (FULL bit ii, MODIFIED bit 16 + ii) go to jmask.
 template <int M>
 __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) {
+  if (a.rebuild && *a.rebuild == 0) return;   // lists kept from the last build
   extern __shared__ int s_dyn[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nst = a.nstencil;
@@ -531,10 +598,13 @@ __device__ inline void lj_ab(const PmNbLaunch& a, const LJ4* s_lj, int ti, int m
                              T& A, T& B) {
   const bool per_atom = sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr;
   if (per_atom) {
-    const int oj = (mj >> 8) & 0x7fffff;
+    // padding and dummy slots carry the atom index 0x7fffff; their lanes are
+    // masked, so they read atom 0's parameters instead of past the array
+    const int oj0 = (mj >> 8) & 0x7fffff;
+    const int ia = oi == 0x7fffff ? 0 : oi, ja = oj0 == 0x7fffff ? 0 : oj0;
     LJ4 pi_, pj_;
-    if constexpr (sizeof(T) == 4) { pi_ = a.atom_lj_f[oi]; pj_ = a.atom_lj_f[oj]; }
-    else { pi_ = a.atom_lj_d[oi]; pj_ = a.atom_lj_d[oj]; }
+    if constexpr (sizeof(T) == 4) { pi_ = a.atom_lj_f[ia]; pj_ = a.atom_lj_f[ja]; }
+    else { pi_ = a.atom_lj_d[ia]; pj_ = a.atom_lj_d[ja]; }
     const bool m = excl == PM_EXCL_MODIFIED;
     const T rs = m ? (T)pi_.z + (T)pj_.z : (T)pi_.x + (T)pj_.x;
     const T ee = m ? (T)pi_.w * (T)pj_.w : (T)pi_.y * (T)pj_.y;
@@ -1061,8 +1131,9 @@ __global__ __launch_bounds__(256) void k_adjoint(PmBox box, PmNbLaunch a, PmPair
       if constexpr (LJSWF != PM_SWF_NONE) {
         if (per_atom && excl != PM_EXCL_MODIFIED) {
           T Rj, ej;
-          if constexpr (sizeof(T) == 4) { const float4 w = a.atom_lj_f[oj]; Rj = w.x; ej = w.y; }
-          else { const double4 w = a.atom_lj_d[oj]; Rj = w.x; ej = w.y; }
+          const int ja = oj == 0x7fffff ? 0 : oj;   // padding / dummy slot: masked lane
+          if constexpr (sizeof(T) == 4) { const float4 w = a.atom_lj_f[ja]; Rj = w.x; ej = w.y; }
+          else { const double4 w = a.atom_lj_d[ja]; Rj = w.x; ej = w.y; }
           const T rs = Ri + Rj, ee = ei * ej;
           const T rs2 = rs * rs, rs5 = rs2 * rs2 * rs, rs6 = rs5 * rs;
           const T r6 = rr2 * rr2 * rr2;
@@ -1199,33 +1270,61 @@ __global__ void k_diag(PmBox box, const double* __restrict__ pos, const int* __r
 
 // ----------------------------------------------------------------- launchers
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
-                   int* cell_count, int* rank, float* key) {
+                   int* cell_count, int* rank, float* key, const int* rebuild) {
   if (n > 0)
-    hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, cell_of, cell_count, rank, key);
+    hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, cell_of, cell_count, rank, key,
+                       rebuild);
 }
 
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
-                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m) {
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild) {
   hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
-                     counts, cluster_cell, ncl_max, m);
+                     counts, cluster_cell, ncl_max, m, rebuild);
 }
 
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
-                       int* cell_atoms) {
+                       int* cell_atoms, const int* rebuild) {
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell_of, rank, cell_start,
-                       cell_atoms);
+                       cell_atoms, rebuild);
 }
 
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
                           const double* q, const int* type, const int* cell_count,
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
-                          int* atom_of_slot, int dummy_type, int dummy_slot, int m) {
+                          int* atom_of_slot, int dummy_type, int dummy_slot, int m, const int* rebuild,
+                          int4* x_build) {
   int nb = (ncells + 3) / 4;
   hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, box, ncells, pos, q, type,
                      cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta,
-                     atom_of_slot, dummy_type, dummy_slot, m);
+                     atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build);
+}
+
+void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* pos, const int4* x_build,
+                          float lim2, int* flag) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_check_disp, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, x_build, lim2, flag);
+}
+
+void pm_launch_set_flag(hipStream_t s, int* flag, int v) {
+  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, flag, v);
+}
+
+void pm_launch_end_build(hipStream_t s, int* flag, unsigned long long* builds) {
+  hipLaunchKernelGGL(k_end_build, dim3(1), dim3(64), 0, s, flag, builds);
+}
+
+void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build) {
+  if (n > 0) hipLaunchKernelGGL(k_invalidate_build, dim3((n + 255) / 256), dim3(256), 0, s, n, x_build);
+}
+
+void pm_launch_refresh_slots(hipStream_t s, const PmBox& box, int nslot_max, const int* counts,
+                             const int* atom_of_slot, const double* pos, const double* q, int4* slot_pos,
+                             longlong4* slot_pos64) {
+  if (nslot_max > 0)
+    hipLaunchKernelGGL(k_refresh_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, box, counts, atom_of_slot,
+                       pos, q, slot_pos, slot_pos64);
 }
 
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
@@ -1240,7 +1339,7 @@ void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, u
 }
 
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
-  double cut = sqrt(p.cut2);
+  double cut = sqrt(p.cut2) + a.skin;
   float c = (float)(cut * (1.0 + 1e-5) + 1e-4);
   size_t lds = 4 * (3 * a.nstencil + 1 + 2 * a.xs_tcap) * sizeof(int);
   if (a.m == 16)

@@ -30,7 +30,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
            "pm_set_charges", "pm_lanczos"]
-OPTIONS = {"cluster_size": 0, "graph": 1}
+OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 
@@ -225,9 +225,10 @@ class Context:
         return x, v, e
 
     def stats(self):
-        s = np.zeros(4, np.int64)
-        self._check(self.lib.pm_last_stats(self.h, _lp(s), 4))
-        return dict(list_entries=int(s[0]), lanes=int(s[1]), clusters=int(s[2]), slots=int(s[3]))
+        s = np.zeros(5, np.int64)
+        self._check(self.lib.pm_last_stats(self.h, _lp(s), 5))
+        return dict(list_entries=int(s[0]), lanes=int(s[1]), clusters=int(s[2]), slots=int(s[3]),
+                    list_builds=int(s[4]))
 
     def set_timing(self, categories=()):
         mask = 0
