@@ -70,16 +70,21 @@ def pme_alpha(rc, accuracy):
     return alpha
 
 
-def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1):
+def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=0, autotune=False):
     from protomol_amd import gpu
     ctx = gpu.Context(0, precision)
     ctx.set_topology(s)
     ctx.set_boundary(s.box, s.origin, CELL)
     ctx.set_option("graph", 1 if graph else 0)
     ctx.set_option("list_skin", skin)
+    ctx.set_option("cluster_size", cluster)
     hs = make_forces(ctx, pme_alpha(RC, 1e-6))
     ctx.md_setup(hs, "leapfrog", dt=DT_FS)
     ctx.md_set_state(s.pos, s.vel)
+    tuned = None
+    if autotune:
+        # layout options chosen by timing short runs (the state is restored)
+        tuned = ctx.md_autotune(20)
     ctx.md_run(max(warmup, 1))
     ctx.synchronize()
     ctx.set_timing(timing)
@@ -96,9 +101,10 @@ def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1):
     diag = ctx.cell_diagnostics(x, RC, RC)
     pairs = int(diag["inrc"].sum())
     stats = ctx.stats()
+    cluster_used = ctx.get_option("cluster_size")
     ctx.close()
     return dict(seconds=t1 - t0, timing=tim, pairs=pairs, energies=e, x=x, v=v, stats=stats, builds=builds,
-                skin=skin_used * 1e-3)
+                skin=skin_used * 1e-3, skin_milli=skin_used, cluster=cluster_used, tuned=tuned)
 
 
 def copy_probe(mb):
@@ -247,7 +253,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: MD steps replay captured HIP graphs (default); 0: direct launches (for counter runs)")
     ap.add_argument("--list-skin", type=int, default=-1, metavar="MILLI_A",
-                    help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
+                    help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step); with --no-autotune")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default layout options instead of pm_md_autotune's choice")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run 10 device copies of MB megabytes (the calibration kernel of HBM counters) and exit")
     args = ap.parse_args()
@@ -263,12 +271,13 @@ def main():
     alpha = pme_alpha(RC, 1e-6)
 
     # the throughput pass: no instrumentation (captured graphs unless --graph 0)
-    run = run_md(s, "mixed", steps, warmup, graph=bool(args.graph), skin=args.list_skin)
+    run = run_md(s, "mixed", steps, warmup, graph=bool(args.graph), skin=args.list_skin,
+                 autotune=not args.no_autotune)
     ms_step = run["seconds"] * 1e3 / steps
     ns_day = (steps / run["seconds"]) * DT_FS * 86400 * 1e-6
     # the roofline pass: the same steps with HIP events around the pair and PME
     # kernels on the streams they run on (event timing launches directly)
-    mixed = run_md(s, "mixed", steps, warmup, timing=("pair", "pme"), skin=args.list_skin)
+    mixed = run_md(s, "mixed", steps, warmup, timing=("pair", "pme"), skin=run["skin_milli"], cluster=run["cluster"])
     ms_step_events = mixed["seconds"] * 1e3 / steps
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(10, steps // 10)
@@ -298,6 +307,8 @@ def main():
         "ms_per_step_with_kernel_events": ms_step_events,
         "graph": bool(args.graph),
         "list_skin_A": run["skin"],
+        "cluster_size": run["cluster"],
+        "autotuned": run["tuned"] is not None,
         "list_builds_per_step": run["builds"] / steps,
         "higher_is_better": True,
         "scaling": "weak",

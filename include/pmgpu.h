@@ -162,6 +162,17 @@ int pm_md_run(pm_context* ctx, int nsteps);
  * evaluation (double[PM_ENERGY_COUNT] overwritten, may be NULL). */
 int pm_md_get_state(pm_context* ctx, double* x, double* v, double* energies);
 
+/* Choose this context's layout options for the MD steps set up with
+ * pm_md_setup by timing them: cluster size 8 or 16 (PM_OPT_CLUSTER_SIZE)
+ * crossed with pair-list skins 0, 0.3, 0.6 and 1.0 A (PM_OPT_LIST_SKIN),
+ * each run for trial_steps steps after a warm-up from the current state.
+ * The state (positions, velocities, Langevin counter) is restored afterwards,
+ * so the trajectory does not advance; the fastest options stay set.  Any of
+ * the outputs may be NULL: the chosen cluster size, skin (1e-3 A) and the
+ * measured ms per step.  No reference counterpart (the reference has one
+ * fixed code path). */
+int pm_md_autotune(pm_context* ctx, int trial_steps, int* cluster_size, int* list_skin, double* ms_per_step);
+
 /* Kernel statistics of the last evaluation (for roofline reporting):
  * stats[0] pair-list entries (i-cluster x j-atom), stats[1] pairs evaluated
  * (lanes issued), stats[2] clusters, stats[3] padded slots, stats[4] pair-list

@@ -10,6 +10,7 @@
 #define PM_LIST_SKIN_AUTO 0.6
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1854,6 +1855,71 @@ int pm_md_run(pm_context* c, int nsteps) {
     if ((r = md_unit(c, 1, true))) return r;
   }
   HIPCHECK(c, hipGetLastError());
+  return PM_OK;
+}
+
+int pm_md_autotune(pm_context* c, int trial_steps, int* cluster_size, int* list_skin, double* ms_per_step) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (c->md_handles.empty()) return fail(c, PM_ERR_STATE, "pm_md_setup not called");
+  if (trial_steps < 4 || trial_steps > 100000)
+    return fail(c, PM_ERR_ARGUMENT, "pm_md_autotune: trial_steps must lie in [4, 100000] (got %d)", trial_steps);
+  if (!c->d_pos || !c->d_vel) return fail(c, PM_ERR_STATE, "pm_md_set_state not called");
+  hipStream_t s = c->stream;
+  const size_t n3 = 3 * (size_t)c->n;
+  // the state every trial starts from, restored at the end
+  std::vector<double> x(n3), v(n3);
+  PmMdDevice md{};
+  HIPCHECK(c, hipStreamSynchronize(s));
+  HIPCHECK(c, hipMemcpy(x.data(), c->d_pos, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(v.data(), c->d_vel, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  if (c->d_md) HIPCHECK(c, hipMemcpy(&md, c->d_md, sizeof(md), hipMemcpyDeviceToHost));
+  const unsigned long long counter0 = c->md_counter;
+  const int m0 = c->m_override, skin0 = c->skin_milli;
+  auto restore = [&]() -> int {
+    HIPCHECK(c, hipMemcpy(c->d_pos, x.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_vel, v.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+    if (c->d_md) HIPCHECK(c, hipMemcpy(c->d_md, &md, sizeof(md), hipMemcpyHostToDevice));
+    c->md_counter = counter0;
+    c->md_forces_valid = false;
+    return PM_OK;
+  };
+  auto configure = [&](int m, int skin) {
+    if (c->m_override != m) c->gen++;
+    c->m_override = m;
+    c->skin_milli = skin;
+    c->list_stale = true;
+    c->md_forces_valid = false;
+    c->gen++;
+  };
+  const int ms_opts[2] = {8, 16};
+  const int skin_opts[4] = {0, 300, 600, 1000};
+  double best = 1e300;
+  int best_m = 8, best_skin = 600;
+  for (int mi = 0; mi < 2; mi++)
+    for (int si = 0; si < 4; si++) {
+      configure(ms_opts[mi], skin_opts[si]);
+      if ((r = restore())) break;
+      // warm-up: allocations, graph capture, the first list build
+      if ((r = pm_md_run(c, std::max(3, trial_steps / 4)))) break;
+      HIPCHECK(c, hipStreamSynchronize(s));
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((r = pm_md_run(c, trial_steps))) break;
+      HIPCHECK(c, hipStreamSynchronize(s));
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial_steps;
+      if (ms < best) { best = ms; best_m = ms_opts[mi]; best_skin = skin_opts[si]; }
+    }
+  if (r) {
+    configure(m0, skin0);
+    restore();
+    return r;
+  }
+  configure(best_m, best_skin);
+  if ((r = restore())) return r;
+  if (cluster_size) *cluster_size = best_m;
+  if (list_skin) *list_skin = best_skin;
+  if (ms_per_step) *ms_per_step = best;
   return PM_OK;
 }
 

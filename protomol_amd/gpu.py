@@ -29,7 +29,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
-           "pm_set_charges", "pm_lanczos"]
+           "pm_set_charges", "pm_lanczos", "pm_md_autotune"]
 OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
     lib.pm_hessian_vec.argtypes = [P, I, C.c_int, D, D, D]
     lib.pm_param_gradients.argtypes = [P, I, C.c_int, D, D, D, D]
     lib.pm_set_charges.argtypes = [P, D]
+    lib.pm_md_autotune.argtypes = [P, C.c_int, I, I, D]
     lib.pm_lanczos.argtypes = [P, I, C.c_int, D, D, C.c_int, C.c_int, C.c_int, D, D, D, D]
     _lib = lib
     return lib
@@ -216,6 +217,14 @@ class Context:
 
     def md_run(self, nsteps: int):
         self._check(self.lib.pm_md_run(self.h, int(nsteps)))
+
+    def md_autotune(self, trial_steps=20):
+        """Time the MD step over cluster sizes and list skins, keep the fastest
+        options (the state is restored).  Returns (cluster_size, skin_milli_A,
+        ms_per_step)."""
+        m, sk, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.pm_md_autotune(self.h, int(trial_steps), C.byref(m), C.byref(sk), C.byref(ms)))
+        return m.value, sk.value, ms.value
 
     def md_get_state(self):
         x = np.zeros((self.n, 3))
