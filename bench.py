@@ -36,6 +36,10 @@ CELL = 5.0
 PEAK_FP32 = 157.3e12      # MI355X vector FP32, FLOP/s
 PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 FLOPS_PER_PAIR = 69       # LJ-C2 + Ewald-real per in-cutoff pair (algorithmic count)
+# HBM bytes per launch of the pair kernel at the default options: FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 (KiB per dispatch from rocprofv3 --pmc, scaled by what the
+# counters read for a 512 MB device copy), profiles/k_pair_water98k_hbm_traffic.md
+PAIR_TRAFFIC_BYTES = (25359.8 * 2.0 + 153438.6 * 1.0) * 1024
 
 
 def build_system():
@@ -253,9 +257,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: MD steps replay captured HIP graphs (default); 0: direct launches (for counter runs)")
     ap.add_argument("--list-skin", type=int, default=-1, metavar="MILLI_A",
-                    help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step); with --no-autotune")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default layout options instead of pm_md_autotune's choice")
+                    help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let pm_md_autotune choose the layout options (default: the library's defaults)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run 10 device copies of MB megabytes (the calibration kernel of HBM counters) and exit")
     args = ap.parse_args()
@@ -272,7 +276,7 @@ def main():
 
     # the throughput pass: no instrumentation (captured graphs unless --graph 0)
     run = run_md(s, "mixed", steps, warmup, graph=bool(args.graph), skin=args.list_skin,
-                 autotune=not args.no_autotune)
+                 autotune=args.autotune)
     ms_step = run["seconds"] * 1e3 / steps
     ns_day = (steps / run["seconds"]) * DT_FS * 86400 * 1e-6
     # the roofline pass: the same steps with HIP events around the pair and PME
@@ -327,7 +331,10 @@ def main():
         "pair_us_per_step": pair_avg_s * 1e6,
         "pme_hbm_frac": (pme_bytes / pme_avg_s) / PEAK_HBM,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 600 and run["cluster"] == 8 else None,
+                     "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) "
+                                     "at list skin 0.6 A, cluster 8; null for other options",
                      "kernel": "k_pair (LJ-C2 + Ewald-real, %d flops/pair)" % FLOPS_PER_PAIR,
                      "note": "compute bound on the FP32 vector ALU (the kernel issues no MFMA); "
                              "peak = MI355X FP32 157.3 TFLOP/s"},
