@@ -162,8 +162,9 @@ def replica_bench(count=64, steps=100, segments=2, seed=0):
     """Batched replicas: `count` water boxes of 100-1000 molecules at 0.0334
     molecules/A^3, LJ + Coulomb C2 7 -> 8 A (no PME), bonds and angles, Langevin
     at a geometric ladder 300 -> 450 K, an exchange attempt every `steps` steps.
-    Replica-steps/s of the batch (one captured graph for all replicas) against
-    the same steps launched replica after replica."""
+    Replica-steps/s of the batch (one launch per kernel over all replicas,
+    replayed from a captured graph) against the batch as one captured chain
+    per replica and against the same steps launched replica after replica."""
     from protomol_amd import gpu, water
     systems = water.replica_boxes(count, seed=seed)
     temps = water.replica_temperatures(count)
@@ -178,20 +179,27 @@ def replica_bench(count=64, steps=100, segments=2, seed=0):
         c.md_setup(hs, "langevin", dt=DT_FS, temperature=float(t), gamma=5.0, seed=k + 1)
         c.md_set_state(s.pos, s.vel)
         ctxs.append(c)
-    b = gpu.Batch(ctxs)
-    b.run(10)
-    for c in ctxs:
-        c.synchronize()
-    accepted = 0
-    t0 = time.perf_counter()
-    for seg in range(segments):
-        b.run(steps)
-        acc, _ = b.exchange(seg, 12345)
-        accepted += int((acc == 1).sum())
-    for c in ctxs:
-        c.synchronize()
-    t_batch = time.perf_counter() - t0
-    b.close()
+    def timed_batch(mode, first_attempt):
+        b = gpu.Batch(ctxs)
+        b.set_mode(mode)
+        b.run(10)
+        for c in ctxs:
+            c.synchronize()
+        acc_n = 0
+        t0 = time.perf_counter()
+        for seg in range(segments):
+            b.run(steps)
+            acc, _ = b.exchange(first_attempt + seg, 12345)
+            acc_n += int((acc == 1).sum())
+        for c in ctxs:
+            c.synchronize()
+        dt = time.perf_counter() - t0
+        batched, why = b.info()
+        b.close()
+        return dt, acc_n, batched, why
+
+    t_batch, accepted, batched, why = timed_batch("auto", 0)
+    t_chain, _, _, _ = timed_batch("chains", segments)
     for c in ctxs:
         c.set_option("graph", 0)
     t0 = time.perf_counter()
@@ -208,7 +216,9 @@ def replica_bench(count=64, steps=100, segments=2, seed=0):
             "atoms_min": int(min(s.n for s in systems)), "atoms_max": int(max(s.n for s in systems)),
             "steps": steps * segments, "exchange_attempts": segments, "accepted_swaps": accepted,
             "replica_steps_per_s": rs / t_batch, "sequential_replica_steps_per_s": rs / t_seq,
-            "speedup_vs_sequential": t_seq / t_batch}
+            "speedup_vs_sequential": t_seq / t_batch,
+            "batched_launches": batched, "not_batched_because": why or None,
+            "chains_replica_steps_per_s": rs / t_chain}
 
 
 def cpu_baseline(s, alpha):
@@ -258,6 +268,8 @@ def main():
                     help="1: MD steps replay captured HIP graphs (default); 0: direct launches (for counter runs)")
     ap.add_argument("--list-skin", type=int, default=-1, metavar="MILLI_A",
                     help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
+    ap.add_argument("--only-replicas", action="store_true",
+                    help="run only the batched replica-exchange measurement and print its record")
     ap.add_argument("--autotune", action="store_true",
                     help="let pm_md_autotune choose the layout options (default: the library's defaults)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -265,6 +277,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.only_replicas:
+        print(json.dumps({"replica_exchange": replica_bench()}))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {args.gpus} is not run "

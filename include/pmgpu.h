@@ -284,6 +284,19 @@ int pm_batch_exchange(pm_batch* batch, int attempt, unsigned long long seed, int
 /* ladder[k]: the replica now at ladder temperature temps[k]. */
 int pm_batch_ladder(pm_batch* batch, int* ladder, double* temps);
 
+/* How pm_batch_run advances the replicas.  PM_BATCH_AUTO (default): every
+ * kernel of an MD step runs as one launch over all replicas (each block finds
+ * its replica from a table of the replicas' arguments; the kernels are the
+ * single-context ones), replayed from a captured graph, when every replica
+ * records the same kernel sequence (same forces, precision and cluster size,
+ * no PME); otherwise, and with PM_BATCH_CHAINS, one chain of captured steps
+ * per replica on its own stream.  Both give the same trajectories.
+ * pm_batch_info: batched = 1 when the last pm_batch_run used batched launches;
+ * why (may be NULL) receives the reason when it did not. */
+enum { PM_BATCH_AUTO = 0, PM_BATCH_CHAINS = 1 };
+int pm_batch_set_mode(pm_batch* batch, int mode);
+int pm_batch_info(pm_batch* batch, int* batched, char* why, int why_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,9 +6,6 @@
 #include "pmgpu.h"
 #include "pm_internal.h"
 
-// automatic pair-list skin (A) when PM_OPT_LIST_SKIN is -1
-#define PM_LIST_SKIN_AUTO 0.6
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -20,6 +17,27 @@
 #include <sstream>
 #include <string>
 #include <vector>
+
+// automatic pair-list skin (A) when PM_OPT_LIST_SKIN is -1
+#define PM_LIST_SKIN_AUTO 0.6
+
+// launch recorder of the calling thread (replica batches)
+static thread_local std::vector<PmRecord>* g_recorder = nullptr;
+void pm_set_recorder(void* rec) { g_recorder = static_cast<std::vector<PmRecord>*>(rec); }
+bool pm_recording() { return g_recorder != nullptr; }
+void pm_record(int kind, int variant, unsigned blocks, unsigned threads, unsigned lds, const void* args,
+               unsigned argbytes) {
+  if (!g_recorder) return;
+  g_recorder->emplace_back();
+  PmRecord& r = g_recorder->back();
+  r.kind = argbytes <= PM_REC_ARGMAX ? kind : -1;
+  r.variant = variant;
+  r.blocks = blocks;
+  r.threads = threads;
+  r.lds = lds;
+  r.argbytes = argbytes <= PM_REC_ARGMAX ? argbytes : 0;
+  if (args && r.argbytes) std::memcpy(r.args, args, r.argbytes);
+}
 
 namespace {
 
@@ -869,14 +887,14 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   const int* rb = reuse ? c->d_rebuild : nullptr;
   {
   TimedScope ts(c, PM_T_BIN);
-  if (reuse) {
-    if (c->list_stale) {
-      pm_launch_invalidate_build(s, n, c->d_xbuild);
-      c->list_stale = false;
-    }
-    const float half = (float)(0.5 * skin);
-    pm_launch_check_disp(s, c->box, n, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
+  if (reuse && c->list_stale) {
+    pm_launch_invalidate_build(s, n, c->d_xbuild);
+    c->list_stale = false;
   }
+  // (n = 0 without reuse: nothing runs; a recorded replica step keeps its
+  // kernel sequence the same with and without reuse)
+  const float half = (float)(0.5 * skin);
+  pm_launch_check_disp(s, c->box, reuse ? n : 0, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
   pm_launch_zero_force_cells(s, rb, c->d_force, 3LL * n, c->d_force_slot, passes.empty() ? 0 : 3LL * c->nslot_max,
                              c->d_cell_count, nc);
   pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key, rb);
@@ -887,11 +905,9 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
                        c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
                        c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m, rb, reuse ? c->d_xbuild : nullptr);
-  if (reuse)
-    pm_launch_refresh_slots(s, c->box, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_pos, c->d_q,
-                            c->d_slot_pos, c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr);
-  else
-    c->list_stale = true;   // a layout built without the skin is not reusable
+  pm_launch_refresh_slots(s, c->box, reuse ? c->nslot_max : 0, c->d_counts, c->d_atom_of_slot, c->d_pos, c->d_q,
+                          c->d_slot_pos, c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr);
+  if (!reuse) c->list_stale = true;   // a layout built without the skin is not reusable
   }
 
   // energy records: pair passes, PME x-pass blocks, correction blocks
@@ -1142,7 +1158,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       pm_launch_pair(s, c->box, pl.first, pl.second, c->precision);
     }
   }
-  if (reuse) pm_launch_end_build(s, c->d_rebuild, c->d_stats + 2);   // every reader of the flag has run
+  // every reader of the flag has run (null flag without reuse: nothing runs)
+  pm_launch_end_build(s, reuse ? c->d_rebuild : nullptr, c->d_stats + 2);
   // join, then the per-slot pair forces into the per-atom forces
   if (side) HIPCHECK(c, hipStreamWaitEvent(s, c->ev_join, 0));
   if (!passes.empty()) {
@@ -2286,6 +2303,22 @@ int pm_synchronize(pm_context* c) {
 // together: their MD steps are captured into one HIP graph (every replica on
 // its own streams, joined on the batch's root stream) and replayed; replica
 // exchange swaps bath temperatures between neighbours of a temperature ladder.
+// One MD step of every replica as one launch per kernel: the recorded kernel
+// sequence, shared by all replicas, with device tables of their arguments.
+struct BatchPlan {
+  struct Item {
+    int kind, variant;
+    unsigned threads, lds, total, argbytes;
+    size_t args_off, pre_off;
+  };
+  std::vector<Item> items;
+  unsigned char* d_buf = nullptr;
+  std::vector<unsigned long long> gen;
+  hipGraphExec_t exec = nullptr;
+  int steps = 0;
+  bool unsupported = false;   // the replicas at generations `gen` have no batched form
+};
+
 struct pm_batch {
   std::vector<pm_context*> ctx;
   hipStream_t root = nullptr;
@@ -2297,6 +2330,11 @@ struct pm_batch {
   std::vector<int> ladder;    // ladder[k]: replica holding temperature k of the ladder
   std::vector<double> temps;  // the ladder temperatures, as given at creation
   std::string err;
+  int mode = PM_BATCH_AUTO;   // PM_BATCH_AUTO: batched launches when the replicas allow them
+  BatchPlan plan;             // steps without energies (captured, G steps per replay)
+  BatchPlan eplan;            // the step with energies
+  bool batched_last = false;  // the last pm_batch_run used batched launches
+  std::string why_chains;     // why the replicas ran as chains
 };
 
 namespace {
@@ -2311,6 +2349,141 @@ void batch_drop_graph(pm_batch* b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   b->exec = nullptr;
   b->exec_steps = 0;
+}
+
+void plan_free(BatchPlan& P);
+void batch_drop_plan(pm_batch* b) {
+  plan_free(b->plan);
+  plan_free(b->eplan);
+}
+
+// the root stream after everything queued on the replicas' streams
+int batch_join_replicas(pm_batch* b) {
+  for (size_t k = 0; k < b->ctx.size(); k++) {
+    pm_context* c = b->ctx[k];
+    if (hipEventRecord(b->ev_done[k], c->stream) != hipSuccess || hipStreamWaitEvent(b->root, b->ev_done[k], 0) != hipSuccess)
+      return bfail(b, PM_ERR_DEVICE, "stream join failed");
+  }
+  return PM_OK;
+}
+
+// launch `steps` steps of a plan's kernels directly on the root stream (or
+// into the capture under way)
+bool batch_plan_launch(pm_batch* b, const BatchPlan& P, int steps) {
+  const int R = (int)b->ctx.size();
+  for (int u = 0; u < steps; u++)
+    for (const BatchPlan::Item& it : P.items) {
+      if (it.total == 0) continue;
+      const void* t = P.d_buf + it.args_off;
+      const int* pre = reinterpret_cast<const int*>(P.d_buf + it.pre_off);
+      if (!(pm_launch_batched_md(b->root, it.kind, it.variant, t, pre, R, it.total, it.threads, it.lds) ||
+            pm_launch_batched_nb(b->root, it.kind, it.variant, t, pre, R, it.total, it.threads, it.lds) ||
+            pm_launch_batched_bonded(b->root, it.kind, it.variant, t, pre, R, it.total, it.threads, it.lds)))
+        return false;
+    }
+  return true;
+}
+
+// Record one step (with or without energies) of every replica into plan P:
+// PM_OK, or PM_ERR_UNSUPPORTED (why_chains says why) when the replicas'
+// kernel sequences differ or hold a kernel without a batched form.  With
+// graph_steps > 0 the plan also holds a captured graph of that many steps.
+int batch_make_plan(pm_batch* b, bool energy, int graph_steps, BatchPlan& P) {
+  const int R = (int)b->ctx.size();
+  std::vector<std::vector<PmRecord>> rec(R);
+  for (int k = 0; k < R; k++) {
+    pm_context* c = b->ctx[k];
+    for (int h : c->md_handles)
+      if (c->forces[h].pme) { b->why_chains = "a replica has a PME force"; return PM_ERR_UNSUPPORTED; }
+    pm_set_recorder(&rec[k]);
+    const int r = md_unit(c, 1, energy);
+    pm_set_recorder(nullptr);
+    c->md_counter -= 2;   // nothing ran: the plan's launches advance the device counter
+    if (r) return bfail(b, r, "replica step failed", c);
+  }
+  const size_t L = rec[0].size();
+  for (int k = 0; k < R; k++) {
+    if (rec[k].size() != L) { b->why_chains = "the replicas' kernel sequences differ in length"; return PM_ERR_UNSUPPORTED; }
+    for (size_t i = 0; i < L; i++) {
+      const PmRecord &x = rec[0][i], &y = rec[k][i];
+      if (y.kind <= 0) { b->why_chains = "a replica step holds a kernel without a batched form"; return PM_ERR_UNSUPPORTED; }
+      if (x.kind != y.kind || x.variant != y.variant || x.threads != y.threads || x.argbytes != y.argbytes) {
+        b->why_chains = "the replicas' kernel sequences differ (forces, precision or cluster size)";
+        return PM_ERR_UNSUPPORTED;
+      }
+      if (y.kind == PM_REC_PAIR && ((y.variant >> 8) & 1)) {
+        b->why_chains = "pair kernels of 16-atom clusters have no batched form";
+        return PM_ERR_UNSUPPORTED;
+      }
+    }
+  }
+  // argument tables and block prefixes, 256-byte aligned
+  std::vector<unsigned char> host;
+  auto align = [&]() { host.resize((host.size() + 255) / 256 * 256, 0); };
+  for (size_t i = 0; i < L; i++) {
+    BatchPlan::Item it{};
+    it.kind = rec[0][i].kind;
+    it.variant = rec[0][i].variant;
+    it.threads = rec[0][i].threads;
+    it.argbytes = rec[0][i].argbytes;
+    align();
+    it.args_off = host.size();
+    for (int k = 0; k < R; k++) host.insert(host.end(), rec[k][i].args, rec[k][i].args + it.argbytes);
+    align();
+    it.pre_off = host.size();
+    std::vector<int> pre(R + 1, 0);
+    unsigned lds = 0;
+    for (int k = 0; k < R; k++) {
+      pre[k + 1] = pre[k] + (int)rec[k][i].blocks;
+      lds = std::max(lds, rec[k][i].lds);
+    }
+    it.total = (unsigned)pre[R];
+    it.lds = lds;
+    const unsigned char* pb = reinterpret_cast<const unsigned char*>(pre.data());
+    host.insert(host.end(), pb, pb + pre.size() * sizeof(int));
+    P.items.push_back(it);
+  }
+  if (hipMalloc((void**)&P.d_buf, std::max<size_t>(host.size(), 256)) != hipSuccess ||
+      hipMemcpy(P.d_buf, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return bfail(b, PM_ERR_DEVICE, "batched plan upload failed");
+  if (graph_steps > 0) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return bfail(b, PM_ERR_DEVICE, "hipStreamBeginCapture failed");
+    const bool ok = batch_plan_launch(b, P, graph_steps);
+    hipError_t e = hipStreamEndCapture(b->root, &g);
+    if (!ok) {
+      if (g) hipGraphDestroy(g);
+      b->why_chains = "a recorded kernel has no batched launcher";
+      return PM_ERR_UNSUPPORTED;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&P.exec, g, nullptr, nullptr, 0);
+    if (g) hipGraphDestroy(g);
+    if (e != hipSuccess) { P.exec = nullptr; return bfail(b, PM_ERR_DEVICE, "batched graph failed"); }
+    P.steps = graph_steps;
+  }
+  for (pm_context* c : b->ctx) P.gen.push_back(c->gen);
+  return PM_OK;
+}
+
+void plan_free(BatchPlan& P) {
+  if (P.exec) hipGraphExecDestroy(P.exec);
+  if (P.d_buf) hipFree(P.d_buf);
+  P = BatchPlan();
+}
+
+// both plans of a batch (steps without energies replayed G at a time, the
+// energy step); PM_ERR_UNSUPPORTED leaves the batch on per-replica chains
+int batch_build_plan(pm_batch* b, int G) {
+  batch_drop_plan(b);
+  int r = batch_make_plan(b, false, G, b->plan);
+  if (r == PM_OK) r = batch_make_plan(b, true, 0, b->eplan);
+  if (r != PM_OK) {
+    const std::string why = b->why_chains;
+    batch_drop_plan(b);
+    b->why_chains = why;
+  }
+  return r;
 }
 
 // Langevin steps without energies for every replica: each replica's steps
@@ -2381,6 +2554,7 @@ void pm_batch_destroy(pm_batch* b) {
   hipStreamSynchronize(b->root);
   for (pm_context* c : b->ctx) hipStreamSynchronize(c->stream);
   batch_drop_graph(b);
+  batch_drop_plan(b);
   for (hipEvent_t e : b->ev_done) hipEventDestroy(e);
   hipEventDestroy(b->ev_root);
   hipStreamDestroy(b->root);
@@ -2401,17 +2575,56 @@ int pm_batch_run(pm_batch* b, int nsteps) {
       c->md_forces_valid = true;
     }
   }
+  if ((r = batch_join_replicas(b))) return r;
   // every step but the last from a captured graph of up to 8 steps
   const int count = nsteps - 1;
   int done = 0;
   const int G = std::min(count, 8);
-  if (count >= 2) {
+  b->batched_last = false;
+  if (count >= 2 && b->mode == PM_BATCH_AUTO) {
+    // one launch per kernel over all replicas, replayed in steps of G
+    bool same_gen = b->plan.gen.size() == b->ctx.size();
+    for (size_t k = 0; same_gen && k < b->ctx.size(); k++) same_gen = b->plan.gen[k] == b->ctx[k]->gen;
+    const bool stale = !same_gen || (!b->plan.unsupported && (!b->plan.exec || b->plan.steps != G));
+    bool usable = !stale && !b->plan.unsupported;
+    if (stale) {
+      if ((r = batch_units(b, 1))) return r;   // every buffer exists and every layout is current
+      done = 1;
+      r = batch_build_plan(b, G);
+      if (r == PM_OK) usable = true;
+      else if (r != PM_ERR_UNSUPPORTED) return r;
+      else {
+        b->plan.unsupported = true;
+        b->plan.gen.clear();
+        for (pm_context* c : b->ctx) b->plan.gen.push_back(c->gen);
+      }
+    }
+    if (usable) {
+      b->batched_last = true;
+      while (count - done >= G) {
+        if (hipGraphLaunch(b->plan.exec, b->root) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "hipGraphLaunch failed");
+        for (pm_context* c : b->ctx) c->md_counter += 2ull * G;
+        done += G;
+      }
+      // the remaining steps and the step with energies as direct batched launches
+      if (!batch_plan_launch(b, b->plan, count - done) || !batch_plan_launch(b, b->eplan, 1))
+        return bfail(b, PM_ERR_DEVICE, "batched launch failed");
+      for (pm_context* c : b->ctx) c->md_counter += 2ull * (count - done + 1);
+      if (hipGetLastError() != hipSuccess) return bfail(b, PM_ERR_DEVICE, "batched launch failed");
+      // the replicas' streams after the batch (later per-replica work)
+      if (hipEventRecord(b->ev_root, b->root) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "event record failed");
+      for (pm_context* c : b->ctx)
+        if (hipStreamWaitEvent(c->stream, b->ev_root, 0) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "stream wait failed");
+      return PM_OK;
+    }
+  }
+  if (count - done >= 2 && !b->batched_last) {
     bool stale = !b->exec || b->exec_steps != G || b->exec_gen.size() != b->ctx.size();
     for (size_t k = 0; !stale && k < b->ctx.size(); k++) stale = b->exec_gen[k] != b->ctx[k]->gen;
     if (stale) {
       batch_drop_graph(b);
       if ((r = batch_units(b, 1))) return r;   // every buffer exists before the capture
-      done = 1;
+      done += 1;
       std::vector<unsigned long long> gen0;
       for (pm_context* c : b->ctx) gen0.push_back(c->gen);
       hipGraph_t g = nullptr;
@@ -2487,6 +2700,21 @@ int pm_batch_exchange(pm_batch* b, int attempt, unsigned long long seed, int* ac
       pm_launch_scale(c->stream, 3 * c->n, std::sqrt(tnew[u] / told[u]), c->d_vel);
       if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "exchange failed");
     }
+  }
+  return PM_OK;
+}
+
+int pm_batch_set_mode(pm_batch* b, int mode) {
+  if (!b || (mode != PM_BATCH_AUTO && mode != PM_BATCH_CHAINS)) return PM_ERR_ARGUMENT;
+  b->mode = mode;
+  return PM_OK;
+}
+
+int pm_batch_info(pm_batch* b, int* batched, char* why, int why_len) {
+  if (!b) return PM_ERR_ARGUMENT;
+  if (batched) *batched = b->batched_last ? 1 : 0;
+  if (why && why_len > 0) {
+    std::snprintf(why, (size_t)why_len, "%s", b->batched_last ? "" : b->why_chains.c_str());
   }
   return PM_OK;
 }

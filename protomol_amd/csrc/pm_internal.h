@@ -285,3 +285,52 @@ struct PmBondedLaunch {
   int want_energy;      // write the energy records (skipped on steps that report none)
 };
 void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, int* nrec_bond, int* nrec_angle);
+
+// ------------------------------------------------- batched launch records
+// Replica batches run one launch per kernel over every replica.  While a
+// recorder is set on the calling thread, the launchers of the MD step append
+// a record (kernel kind, template variant, blocks, LDS bytes, the kernel's
+// arguments) instead of launching; the batch checks that every replica
+// recorded the same kernel sequence, uploads the argument tables and launches
+// each kind once: a block finds its replica from the table's prefix of block
+// counts and runs the same kernel body on that replica's arguments.
+enum PmRecKind {
+  PM_REC_KICK_DRIFT = 1, PM_REC_LANGEVIN_KICK, PM_REC_DRIFT, PM_REC_MD_ADVANCE, PM_REC_ZERO_FORCE_CELLS,
+  PM_REC_CHECK_DISP, PM_REC_BIN, PM_REC_SCAN_CELLS, PM_REC_SCATTER, PM_REC_FILL_SLOTS, PM_REC_REFRESH_SLOTS,
+  PM_REC_LIST, PM_REC_PAIR, PM_REC_END_BUILD, PM_REC_FOLD_SLOTS, PM_REC_BONDED_FORCE, PM_REC_BONDED_ENERGY,
+  PM_REC_REDUCE1, PM_REC_REDUCE2
+};
+#define PM_REC_ARGMAX 1024
+struct PmRecord {
+  int kind;
+  int variant;          // template variant (pair: potential/switching combination and precision)
+  unsigned blocks;      // blocks of this replica's launch (0: nothing to do for it)
+  unsigned threads;     // threads per block
+  unsigned lds;         // dynamic LDS bytes
+  unsigned argbytes;
+  alignas(16) unsigned char args[PM_REC_ARGMAX];
+};
+// the calling thread's recorder (null: launch normally)
+void pm_set_recorder(void* rec);
+bool pm_recording();
+void pm_record(int kind, int variant, unsigned blocks, unsigned threads, unsigned lds, const void* args,
+               unsigned argbytes);
+// One launch of `kind` over nrep replicas: args_table holds nrep argument
+// structs of argbytes each, prefix nrep + 1 block offsets (prefix[nrep] =
+// total blocks).  Returns false for a kind/variant this file does not launch.
+bool pm_launch_batched_md(hipStream_t s, int kind, int variant, const void* args_table, const int* prefix,
+                          int nrep, unsigned total_blocks, unsigned threads, unsigned lds);
+bool pm_launch_batched_nb(hipStream_t s, int kind, int variant, const void* args_table, const int* prefix,
+                          int nrep, unsigned total_blocks, unsigned threads, unsigned lds);
+bool pm_launch_batched_bonded(hipStream_t s, int kind, int variant, const void* args_table, const int* prefix,
+                              int nrep, unsigned total_blocks, unsigned threads, unsigned lds);
+
+// the replica of block `b` (prefix: nrep + 1 non-decreasing block offsets)
+__device__ inline int pm_batch_replica(const int* __restrict__ prefix, int nrep, int b) {
+  int lo = 0, hi = nrep - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}

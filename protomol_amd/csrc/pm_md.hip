@@ -162,12 +162,12 @@ __global__ __launch_bounds__(256) void k_corr_energy(PmBox box, const double* __
 // contiguous chunks (one block each), stage 2 sums the chunk totals in order
 // and adds the constant terms.
 #define PM_RED_BLOCKS 256
-__global__ __launch_bounds__(256) void k_reduce_energy1(const double* __restrict__ parts, int nparts, int chunk,
-                                                        double* __restrict__ partial) {
+__device__ inline void reduce_energy1_body(const double* __restrict__ parts, int nparts, int chunk,
+                                                        double* __restrict__ partial, int bx) {
   __shared__ double s[256];
   const int c = threadIdx.x % PM_NE;           // component
   const int r0 = threadIdx.x / PM_NE;          // 16 records in flight per component
-  const int lo = blockIdx.x * chunk, hi = min(nparts, lo + chunk);
+  const int lo = bx * chunk, hi = min(nparts, lo + chunk);
   double acc = 0;
   for (int p = lo + r0; p < hi; p += 256 / PM_NE) acc += parts[(long long)p * PM_NE + c];
   s[threadIdx.x] = acc;
@@ -175,15 +175,24 @@ __global__ __launch_bounds__(256) void k_reduce_energy1(const double* __restrict
   if (threadIdx.x < PM_NE) {
     double t = 0;
     for (int u = 0; u < 256 / PM_NE; u++) t += s[u * PM_NE + threadIdx.x];
-    partial[blockIdx.x * PM_NE + threadIdx.x] = t;
+    partial[bx * PM_NE + threadIdx.x] = t;
   }
 }
+__global__ __launch_bounds__(256) void k_reduce_energy1(const double* __restrict__ parts, int nparts, int chunk,
+                                                        double* __restrict__ partial) { reduce_energy1_body(parts, nparts, chunk, partial, blockIdx.x); }
+struct ReduceEnergy1Args { const double* parts; int nparts; int chunk; double* partial; };
+__global__ __launch_bounds__(256) void kb_reduce_energy1(const ReduceEnergy1Args* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const ReduceEnergy1Args& q = t[r];
+  reduce_energy1_body(q.parts, q.nparts, q.chunk, q.partial, blockIdx.x - pre[r]);
+}
+
 
 // stage 2: 16 record groups per component summed with independent loads,
 // then combined in a fixed order (nblocks <= PM_RED_BLOCKS)
-__global__ __launch_bounds__(256) void k_reduce_energy2(const double* __restrict__ partial, int nblocks,
+__device__ inline void reduce_energy2_body(const double* __restrict__ partial, int nblocks,
                                                         double const_coul, double const_vir_diag,
-                                                        double* __restrict__ out, int overwrite) {
+                                                        double* __restrict__ out, int overwrite, int bx) {
   __shared__ double s[256];
   const int c = threadIdx.x % PM_NE, g = threadIdx.x / PM_NE;
   double v[PM_RED_BLOCKS / 16];
@@ -205,6 +214,16 @@ __global__ __launch_bounds__(256) void k_reduce_energy2(const double* __restrict
     out[c] = overwrite ? t : out[c] + t;
   }
 }
+__global__ __launch_bounds__(256) void k_reduce_energy2(const double* __restrict__ partial, int nblocks,
+                                                        double const_coul, double const_vir_diag,
+                                                        double* __restrict__ out, int overwrite) { reduce_energy2_body(partial, nblocks, const_coul, const_vir_diag, out, overwrite, blockIdx.x); }
+struct ReduceEnergy2Args { const double* partial; int nblocks; double const_coul; double const_vir_diag; double* out; int overwrite; };
+__global__ __launch_bounds__(256) void kb_reduce_energy2(const ReduceEnergy2Args* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const ReduceEnergy2Args& q = t[r];
+  reduce_energy2_body(q.partial, q.nblocks, q.const_coul, q.const_vir_diag, q.out, q.overwrite, blockIdx.x - pre[r]);
+}
+
 
 __global__ void k_zero_ll(long long* p, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,13 +236,14 @@ __global__ void k_zero_i(int* p, long long count) {
 // clears the fixed-point forces and the cell counts of one evaluation in one launch
 // force arrays, and the cell counts of the binning when it runs (*rebuild != 0
 // or rebuild null)
-__global__ void k_zero_force_cells(const int* rebuild, long long* f, long long nf, long long* fs, long long nfs,
-                                   int* cc, int nc) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nf) f[i] = 0;
-  if (i < nfs) fs[i] = 0;
-  if (i < nc && (!rebuild || *rebuild != 0)) cc[i] = 0;
+struct ZeroFcArgs { const int* rebuild; long long* f; long long nf; long long* fs; long long nfs; int* cc; int nc; };
+__device__ inline void zero_force_cells_body(const ZeroFcArgs& a, int bx) {
+  long long i = (long long)bx * blockDim.x + threadIdx.x;
+  if (i < a.nf) a.f[i] = 0;
+  if (i < a.nfs) a.fs[i] = 0;
+  if (i < a.nc && (!a.rebuild || *a.rebuild != 0)) a.cc[i] = 0;
 }
+__global__ void k_zero_force_cells(ZeroFcArgs a) { zero_force_cells_body(a, blockIdx.x); }
 __global__ void k_zero_d(double* p, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0.0;
@@ -241,10 +261,12 @@ __global__ void k_add_fix(int n3, const long long* __restrict__ f, double* __res
 // (integrators/LeapfrogIntegrator.cpp:46-104) and StandardIntegrator::doHalfKick
 // (StandardIntegrator.cpp:83-96).  mode 0: v += F*h*0.5/m then x += v*h;
 // mode 1: v += F*h/m then x += v*h; mode 2: v += F*hh/m (hh = 0.5*dt*INV_TIMEFACTOR).
-__global__ void k_kick_drift(int n, int mode, double h, const double* __restrict__ mass,
-                             const long long* __restrict__ f, double* __restrict__ v, double* __restrict__ x) {
+struct KickArgs { int n; int mode; double h; const double* mass; const long long* f; double* v; double* x; };
+__device__ inline void kick_drift_body(int n, int mode, double h, const double* __restrict__ mass,
+                                       const long long* __restrict__ f, double* __restrict__ v,
+                                       double* __restrict__ x, int bx) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double m = mass[i];
 #pragma unroll
@@ -257,6 +279,7 @@ __global__ void k_kick_drift(int n, int mode, double h, const double* __restrict
     if (mode != 2) x[3 * i + k] += vv * h;
   }
 }
+__global__ void k_kick_drift(KickArgs a) { kick_drift_body(a.n, a.mode, a.h, a.mass, a.f, a.v, a.x, blockIdx.x); }
 
 // counter-based normal deviates (SplitMix64 hash + Box-Muller)
 __device__ inline unsigned long long mix64(unsigned long long z) {
@@ -277,10 +300,15 @@ __device__ inline double gauss(unsigned long long seed, unsigned long long count
 // The noise counter and the bath temperature come from device memory
 // (PmMdDevice), so a captured step replays with fresh deviates and a bath
 // temperature can be changed between replays.
-__global__ void k_langevin_kick(int n, double dt, double g, const double* __restrict__ mass,
-                                const long long* __restrict__ f, double* __restrict__ v,
-                                unsigned long long seed, const PmMdDevice* __restrict__ md, int offset) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+struct LangevinArgs {
+  int n; double dt; double g; const double* mass; const long long* f; double* v; unsigned long long seed;
+  const PmMdDevice* md; int offset;
+};
+__device__ inline void langevin_kick_body(int n, double dt, double g, const double* __restrict__ mass,
+                                          const long long* __restrict__ f, double* __restrict__ v,
+                                          unsigned long long seed, const PmMdDevice* __restrict__ md, int offset,
+                                          int bx) {
+  int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned long long counter = md->counter + (unsigned long long)offset;
   const double temperature = md->temperature;
@@ -296,6 +324,9 @@ __global__ void k_langevin_kick(int n, double dt, double g, const double* __rest
     v[3 * i + k] = v[3 * i + k] * vdt + F * fdt / m + xi * s * ndt;
   }
 }
+__global__ void k_langevin_kick(LangevinArgs a) {
+  langevin_kick_body(a.n, a.dt, a.g, a.mass, a.f, a.v, a.seed, a.md, a.offset, blockIdx.x);
+}
 
 // v *= factor (velocities carried to a new bath temperature)
 __global__ void k_scale(int n3, double factor, double* __restrict__ v) {
@@ -304,13 +335,42 @@ __global__ void k_scale(int n3, double factor, double* __restrict__ v) {
 }
 
 // advances the noise counter by the two half kicks of a step
-__global__ void k_md_advance(PmMdDevice* md) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) md->counter += 2;
+struct AdvanceArgs { PmMdDevice* md; };
+__device__ inline void md_advance_body(PmMdDevice* md, int bx) {
+  if (threadIdx.x == 0 && bx == 0) md->counter += 2;
 }
+__global__ void k_md_advance(AdvanceArgs a) { md_advance_body(a.md, blockIdx.x); }
 
-__global__ void k_drift(int n3, double h, const double* __restrict__ v, double* __restrict__ x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+struct DriftArgs { int n3; double h; const double* v; double* x; };
+__device__ inline void drift_body(int n3, double h, const double* __restrict__ v, double* __restrict__ x, int bx) {
+  int i = bx * blockDim.x + threadIdx.x;
   if (i < n3) x[i] += h * v[i];   // Vector3DBlock::intoWeightedAdd
+}
+__global__ void k_drift(DriftArgs a) { drift_body(a.n3, a.h, a.v, a.x, blockIdx.x); }
+
+// batched launches over replicas (argument tables in device memory)
+__global__ void kb_kick_drift(const KickArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const KickArgs& a = t[r];
+  kick_drift_body(a.n, a.mode, a.h, a.mass, a.f, a.v, a.x, blockIdx.x - pre[r]);
+}
+__global__ void kb_langevin_kick(const LangevinArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const LangevinArgs& a = t[r];
+  langevin_kick_body(a.n, a.dt, a.g, a.mass, a.f, a.v, a.seed, a.md, a.offset, blockIdx.x - pre[r]);
+}
+__global__ void kb_drift(const DriftArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const DriftArgs& a = t[r];
+  drift_body(a.n3, a.h, a.v, a.x, blockIdx.x - pre[r]);
+}
+__global__ void kb_md_advance(const AdvanceArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  md_advance_body(t[r].md, blockIdx.x - pre[r]);
+}
+__global__ void kb_zero_force_cells(const ZeroFcArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  zero_force_cells_body(t[r], blockIdx.x - pre[r]);
 }
 
 inline dim3 grid_for(long long n, int b) { return dim3((unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1)); }
@@ -351,6 +411,13 @@ void pm_launch_reduce_energy(hipStream_t s, const double* parts, int nparts, dou
   int nb = nparts < PM_RED_BLOCKS ? (nparts > 0 ? nparts : 1) : PM_RED_BLOCKS;
   int chunk = (nparts + nb - 1) / nb;
   if (chunk < 1) chunk = 1;
+  if (pm_recording()) {
+    const ReduceEnergy1Args a1{parts, nparts, chunk, scratch};
+    const ReduceEnergy2Args a2{scratch, nb, constant_coul, constant_vir_diag, out, overwrite};
+    pm_record(PM_REC_REDUCE1, 0, (unsigned)nb, 256, 0, &a1, sizeof(a1));
+    pm_record(PM_REC_REDUCE2, 0, 1, 256, 0, &a2, sizeof(a2));
+    return;
+  }
   hipLaunchKernelGGL(k_reduce_energy1, dim3(nb), dim3(256), 0, s, parts, nparts, chunk, scratch);
   hipLaunchKernelGGL(k_reduce_energy2, dim3(1), dim3(256), 0, s, scratch, nb, constant_coul, constant_vir_diag, out,
                      overwrite);
@@ -366,8 +433,12 @@ void pm_launch_zero_force_cells(hipStream_t s, const int* rebuild, long long* f,
                                 long long nfs, int* cc, int nc) {
   long long m = nf > nc ? nf : nc;
   if (nfs > m) m = nfs;
-  if (m > 0)
-    hipLaunchKernelGGL(k_zero_force_cells, grid_for(m, 256), dim3(256), 0, s, rebuild, f, nf, fs, nfs, cc, nc);
+  const ZeroFcArgs a{rebuild, f, nf, fs, nfs, cc, nc};
+  if (pm_recording()) {
+    pm_record(PM_REC_ZERO_FORCE_CELLS, 0, m > 0 ? grid_for(m, 256).x : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
+  if (m > 0) hipLaunchKernelGGL(k_zero_force_cells, grid_for(m, 256), dim3(256), 0, s, a);
 }
 void pm_launch_zero_d(hipStream_t s, double* p, long long count) {
   if (count > 0) hipLaunchKernelGGL(k_zero_d, grid_for(count, 256), dim3(256), 0, s, p, count);
@@ -380,21 +451,55 @@ void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out) {
 }
 void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
                     double* v, double* x) {
-  if (n > 0) hipLaunchKernelGGL(k_kick_drift, grid_for(n, 256), dim3(256), 0, s, n, mode, h, mass, f, v, x);
+  const KickArgs a{n, mode, h, mass, f, v, x};
+  if (pm_recording()) {
+    pm_record(PM_REC_KICK_DRIFT, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
+  if (n > 0) hipLaunchKernelGGL(k_kick_drift, grid_for(n, 256), dim3(256), 0, s, a);
 }
 void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_akma, const double* mass,
                              const long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
                              int offset) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_langevin_kick, grid_for(n, 256), dim3(256), 0, s, n, dt_akma, gamma_akma, mass, f, v,
-                       seed, md, offset);
+  const LangevinArgs a{n, dt_akma, gamma_akma, mass, f, v, seed, md, offset};
+  if (pm_recording()) {
+    pm_record(PM_REC_LANGEVIN_KICK, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
+  if (n > 0) hipLaunchKernelGGL(k_langevin_kick, grid_for(n, 256), dim3(256), 0, s, a);
 }
 void pm_launch_scale(hipStream_t s, int n3, double factor, double* v) {
   if (n3 > 0) hipLaunchKernelGGL(k_scale, grid_for(n3, 256), dim3(256), 0, s, n3, factor, v);
 }
 void pm_launch_md_advance(hipStream_t s, PmMdDevice* md) {
-  hipLaunchKernelGGL(k_md_advance, dim3(1), dim3(64), 0, s, md);
+  const AdvanceArgs a{md};
+  if (pm_recording()) {
+    pm_record(PM_REC_MD_ADVANCE, 0, 1, 64, 0, &a, sizeof(a));
+    return;
+  }
+  hipLaunchKernelGGL(k_md_advance, dim3(1), dim3(64), 0, s, a);
 }
 void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x) {
-  if (n3 > 0) hipLaunchKernelGGL(k_drift, grid_for(n3, 256), dim3(256), 0, s, n3, h, v, x);
+  const DriftArgs a{n3, h, v, x};
+  if (pm_recording()) {
+    pm_record(PM_REC_DRIFT, 0, n3 > 0 ? grid_for(n3, 256).x : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
+  if (n3 > 0) hipLaunchKernelGGL(k_drift, grid_for(n3, 256), dim3(256), 0, s, a);
+}
+
+bool pm_launch_batched_md(hipStream_t s, int kind, int variant, const void* t, const int* pre, int nrep,
+                          unsigned total, unsigned threads, unsigned lds) {
+  (void)variant; (void)lds;
+  const dim3 g(total), b(threads);
+  switch (kind) {
+    case PM_REC_KICK_DRIFT: hipLaunchKernelGGL(kb_kick_drift, g, b, 0, s, (const KickArgs*)t, pre, nrep); return true;
+    case PM_REC_LANGEVIN_KICK: hipLaunchKernelGGL(kb_langevin_kick, g, b, 0, s, (const LangevinArgs*)t, pre, nrep); return true;
+    case PM_REC_DRIFT: hipLaunchKernelGGL(kb_drift, g, b, 0, s, (const DriftArgs*)t, pre, nrep); return true;
+    case PM_REC_MD_ADVANCE: hipLaunchKernelGGL(kb_md_advance, g, b, 0, s, (const AdvanceArgs*)t, pre, nrep); return true;
+    case PM_REC_ZERO_FORCE_CELLS: hipLaunchKernelGGL(kb_zero_force_cells, g, b, 0, s, (const ZeroFcArgs*)t, pre, nrep); return true;
+    case PM_REC_REDUCE1: hipLaunchKernelGGL(kb_reduce_energy1, g, b, 0, s, (const ReduceEnergy1Args*)t, pre, nrep); return true;
+    case PM_REC_REDUCE2: hipLaunchKernelGGL(kb_reduce_energy2, g, b, 0, s, (const ReduceEnergy2Args*)t, pre, nrep); return true;
+    default: return false;
+  }
 }

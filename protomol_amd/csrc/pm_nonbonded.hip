@@ -52,10 +52,10 @@ __device__ inline int cell_index(const PmBox& b, double px, double py, double pz
   return (c[0] * b.n[1] + c[1]) * b.n[2] + c[2];
 }
 
-__global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
+__device__ inline void bin_body(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
                       int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key,
-                      const int* __restrict__ rebuild) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                      const int* __restrict__ rebuild, int bx) {
+  int i = bx * blockDim.x + threadIdx.x;
   if (i >= n || (rebuild && *rebuild == 0)) return;
   double mz;
   int c = cell_index(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &mz);
@@ -63,14 +63,24 @@ __global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __r
   key[i] = (float)mz;
   rank[i] = atomicAdd(&cell_count[c], 1);   // arrival rank; k_fill_slots orders by (z, index)
 }
+__global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
+                      int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key,
+                      const int* __restrict__ rebuild) { bin_body(box, n, pos, cell_of, cell_count, rank, key, rebuild, blockIdx.x); }
+struct BinArgs { PmBox box; int n; const double* pos; int* cell_of; int* cell_count; int* rank; float* key; const int* rebuild; };
+__global__ void kb_bin(const BinArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const BinArgs& q = t[r];
+  bin_body(q.box, q.n, q.pos, q.cell_of, q.cell_count, q.rank, q.key, q.rebuild, blockIdx.x - pre[r]);
+}
+
 
 // Exclusive scans of the cell counts (atoms and M-padded slots) in one
 // workgroup, 8192 cells per pass staged in LDS; also assigns every cluster
 // its cell.
-__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
+__device__ inline void scan_cells_body(int ncells, const int* __restrict__ cnt,
                                                      int* __restrict__ start, int* __restrict__ slot,
                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
-                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) {
+                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild, int bx) {
   if (rebuild && *rebuild == 0) return;   // uniform over the block: before any barrier
   __shared__ int s_c[8192];
   __shared__ int s_wa[16], s_ws[16];
@@ -131,14 +141,35 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __re
   }
   if (t == 0) { counts[0] = carry_s; counts[1] = carry_s / m_cl; counts[2] = carry_a; }
 }
+__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
+                                                     int* __restrict__ start, int* __restrict__ slot,
+                                                     int* __restrict__ counts, int* __restrict__ cluster_cell,
+                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) { scan_cells_body(ncells, cnt, start, slot, counts, cluster_cell, ncl_max, m_cl, rebuild, blockIdx.x); }
+struct ScanCellsArgs { int ncells; const int* cnt; int* start; int* slot; int* counts; int* cluster_cell; int ncl_max; int m_cl; const int* rebuild; };
+__global__ __launch_bounds__(1024) void kb_scan_cells(const ScanCellsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const ScanCellsArgs& q = t[r];
+  scan_cells_body(q.ncells, q.cnt, q.start, q.slot, q.counts, q.cluster_cell, q.ncl_max, q.m_cl, q.rebuild, blockIdx.x - pre[r]);
+}
 
-__global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
+
+__device__ inline void scatter_body(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
                           const int* __restrict__ cell_start, int* __restrict__ cell_atoms,
-                          const int* __restrict__ rebuild) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                          const int* __restrict__ rebuild, int bx) {
+  int i = bx * blockDim.x + threadIdx.x;
   if (i >= n || (rebuild && *rebuild == 0)) return;
   cell_atoms[cell_start[cell_of[i]] + rank[i]] = i;
 }
+__global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
+                          const int* __restrict__ cell_start, int* __restrict__ cell_atoms,
+                          const int* __restrict__ rebuild) { scatter_body(n, cell_of, rank, cell_start, cell_atoms, rebuild, blockIdx.x); }
+struct ScatterArgs { int n; const int* cell_of; const int* rank; const int* cell_start; int* cell_atoms; const int* rebuild; };
+__global__ void kb_scatter(const ScatterArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const ScatterArgs& q = t[r];
+  scatter_body(q.n, q.cell_of, q.rank, q.cell_start, q.cell_atoms, q.rebuild, blockIdx.x - pre[r]);
+}
+
 
 __device__ inline unsigned int to_fix(double x, double s) {
   // periodic 32-bit fixed point: round(x * 2^32 / L) mod 2^32
@@ -160,17 +191,17 @@ __device__ inline unsigned long long to_fix64(double x, double s) {
 // the cell (ties impossible: indices are distinct), so the layout does not
 // depend on the arrival order of k_scatter.  Cells holding more than
 // PM_SORT_MAX atoms (collapsed or non-finite coordinates) keep arrival order.
-__global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const double* __restrict__ pos,
+__device__ inline void fill_slots_body(PmBox box, int ncells, const double* __restrict__ pos,
                              const double* __restrict__ q, const int* __restrict__ type,
                              const int* __restrict__ cell_count, const int* __restrict__ cell_start,
                              const int* __restrict__ cell_slot, const int* __restrict__ cell_atoms,
                              const float* __restrict__ key, int4* __restrict__ slot_pos,
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
-                             const int* __restrict__ rebuild, int4* __restrict__ x_build) {
+                             const int* __restrict__ rebuild, int4* __restrict__ x_build, int bx) {
   if (rebuild && *rebuild == 0) return;
   const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int c = bx * 4 + (threadIdx.x >> 6);
   if (c == 0 && lane == 0) {
     // the shared padding slot of the pair lists
     slot_pos[dummy_slot] = make_int4(0, 0, 0, 0);
@@ -238,13 +269,28 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
     atom_of_slot[s0 + u] = -1;
   }
 }
+__global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const double* __restrict__ pos,
+                             const double* __restrict__ q, const int* __restrict__ type,
+                             const int* __restrict__ cell_count, const int* __restrict__ cell_start,
+                             const int* __restrict__ cell_slot, const int* __restrict__ cell_atoms,
+                             const float* __restrict__ key, int4* __restrict__ slot_pos,
+                             longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
+                             int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
+                             const int* __restrict__ rebuild, int4* __restrict__ x_build) { fill_slots_body(box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m_cl, rebuild, x_build, blockIdx.x); }
+struct FillSlotsArgs { PmBox box; int ncells; const double* pos; const double* q; const int* type; const int* cell_count; const int* cell_start; const int* cell_slot; const int* cell_atoms; const float* key; int4* slot_pos; longlong4* slot_pos64; int* slot_meta; int* atom_of_slot; int dummy_type; int dummy_slot; int m_cl; const int* rebuild; int4* x_build; };
+__global__ __launch_bounds__(256) void kb_fill_slots(const FillSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const FillSlotsArgs& q = t[r];
+  fill_slots_body(q.box, q.ncells, q.pos, q.q, q.type, q.cell_count, q.cell_start, q.cell_slot, q.cell_atoms, q.key, q.slot_pos, q.slot_pos64, q.slot_meta, q.atom_of_slot, q.dummy_type, q.dummy_slot, q.m_cl, q.rebuild, q.x_build, blockIdx.x - pre[r]);
+}
+
 
 // List reuse: rebuild (flag = 1) when an atom has moved more than half the
 // list skin from where the layout was built (or the layout is invalid: w != 1).
 // Every thread that finds one writes the same value, so no atomics are needed.
-__global__ void k_check_disp(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
-                             float lim2, int* __restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void check_disp_body(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
+                             float lim2, int* __restrict__ flag, int bx) {
+  const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int4 b = x_build[i];
   bool need = b.w != 1;
@@ -256,18 +302,35 @@ __global__ void k_check_disp(PmBox box, int n, const double* __restrict__ pos, c
   }
   if (need) *flag = 1;
 }
+__global__ void k_check_disp(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
+                             float lim2, int* __restrict__ flag) { check_disp_body(box, n, pos, x_build, lim2, flag, blockIdx.x); }
+struct CheckDispArgs { PmBox box; int n; const double* pos; const int4* x_build; float lim2; int* flag; };
+__global__ void kb_check_disp(const CheckDispArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const CheckDispArgs& q = t[r];
+  check_disp_body(q.box, q.n, q.pos, q.x_build, q.lim2, q.flag, blockIdx.x - pre[r]);
+}
+
 
 __global__ void k_set_flag(int* flag, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
 }
 
 // end of a reusing evaluation: count a build, clear the flag
-__global__ void k_end_build(int* flag, unsigned long long* builds) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+__device__ inline void end_build_body(int* flag, unsigned long long* builds, int bx) {
+  if (threadIdx.x == 0 && bx == 0) {
     if (*flag) builds[0] += 1ull;
     *flag = 0;
   }
 }
+__global__ void k_end_build(int* flag, unsigned long long* builds) { end_build_body(flag, builds, blockIdx.x); }
+struct EndBuildArgs { int* flag; unsigned long long* builds; };
+__global__ void kb_end_build(const EndBuildArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const EndBuildArgs& q = t[r];
+  end_build_body(q.flag, q.builds, blockIdx.x - pre[r]);
+}
+
 
 __global__ void k_invalidate_build(int n, int4* __restrict__ x_build) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -276,10 +339,10 @@ __global__ void k_invalidate_build(int n, int4* __restrict__ x_build) {
 
 // Current positions (and charges) into the slots of the kept layout, with the
 // arithmetic of k_fill_slots; padding slots keep their place (they are masked).
-__global__ void k_refresh_slots(PmBox box, const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+__device__ inline void refresh_slots_body(PmBox box, const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
                                 const double* __restrict__ pos, const double* __restrict__ q,
-                                int4* __restrict__ slot_pos, longlong4* __restrict__ slot_pos64) {
-  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+                                int4* __restrict__ slot_pos, longlong4* __restrict__ slot_pos64, int bx) {
+  const int sl = bx * blockDim.x + threadIdx.x;
   if (sl >= counts[0]) return;
   const int ai = atom_of_slot[sl];
   if (ai < 0) return;
@@ -298,6 +361,16 @@ __global__ void k_refresh_slots(PmBox box, const int* __restrict__ counts, const
     slot_pos64[sl] = p8;
   }
 }
+__global__ void k_refresh_slots(PmBox box, const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+                                const double* __restrict__ pos, const double* __restrict__ q,
+                                int4* __restrict__ slot_pos, longlong4* __restrict__ slot_pos64) { refresh_slots_body(box, counts, atom_of_slot, pos, q, slot_pos, slot_pos64, blockIdx.x); }
+struct RefreshSlotsArgs { PmBox box; const int* counts; const int* atom_of_slot; const double* pos; const double* q; int4* slot_pos; longlong4* slot_pos64; };
+__global__ void kb_refresh_slots(const RefreshSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const RefreshSlotsArgs& q = t[r];
+  refresh_slots_body(q.box, q.counts, q.atom_of_slot, q.pos, q.q, q.slot_pos, q.slot_pos64, blockIdx.x - pre[r]);
+}
+
 
 // ------------------------------------------------------------ wave helpers
 __device__ inline int wave_incl_scan(int v, int lane) {
@@ -347,7 +420,7 @@ __device__ inline int wrap_cell(int c, int d, int n) {
 // exactly once.  Bit 31 marks an entry that some cluster atom excludes or
 // modifies; its masks (FULL bit ii, MODIFIED bit 16 + ii) go to jmask.
 template <int M>
-__global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) {
+__device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int bx) {
   if (a.rebuild && *a.rebuild == 0) return;   // lists kept from the last build
   extern __shared__ int s_dyn[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -357,7 +430,7 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
   int* s_pref = s_cnt + nst;     // nst + 1 entries
   int* s_tri = s_pref + nst + 1; // xs_tcap entries: partner | ii << 24 | class << 28
   int* s_srt = s_tri + a.xs_tcap; // the same, sorted by partner
-  const int cl = blockIdx.x * 4 + wave;
+  const int cl = bx * 4 + wave;
   const int ncl = a.counts[1];
   const bool active = cl < ncl;  // every wave reaches every barrier below
   const int cell = active ? a.cluster_cell[cl] : 0;
@@ -493,6 +566,20 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
         const float gy = fmaxf(0.f, fmaxf(loy - dy, dy - hiy));
         const float gz = fmaxf(0.f, fmaxf(loz - dz, dz - hiz));
         pass = gx * gx + gy * gy + gz * gz < cut2_list;
+      } else {
+        // box too small for the bounding-box test: keep the entry when one
+        // of the cluster's atoms lies within the list cutoff, by the pair
+        // kernel's own minimal-image difference (the cluster's positions
+        // are wave-uniform loads)
+        pass = false;
+        for (int ii = 0; ii < M; ii++) {
+          const int4 q = a.slot_pos[cl * M + ii];
+          const bool real = ((a.slot_meta[cl * M + ii] >> 8) & 0x7fffff) != 0x7fffff;
+          const float dx = (float)(int)(p.x - q.x) * fx;
+          const float dy = (float)(int)(p.y - q.y) * fy;
+          const float dz = (float)(int)(p.z - q.z) * fz;
+          pass = pass || (real && dx * dx + dy * dy + dz * dz < cut2_list);
+        }
       }
       o = (a.slot_meta[sj] >> 8) & 0x7fffff;
     }
@@ -530,6 +617,16 @@ __global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut
     a.jcount[cl] = count;
   }
 }
+template <int M>
+__global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) { list_body<M>(box, a, cut2_list, blockIdx.x); }
+struct ListArgs { PmBox box; PmNbLaunch a; float cut2_list; };
+template <int M>
+__global__ __launch_bounds__(256) void kb_list(const ListArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const ListArgs& q = t[r];
+  list_body<M>(q.box, q.a, q.cut2_list, blockIdx.x - pre[r]);
+}
+
 
 // list statistics: total entries and issued lanes over all clusters
 __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcount, const int* __restrict__ counts,
@@ -561,9 +658,9 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
 }
 
 // slot forces of the pair kernels into the per-atom fixed-point forces
-__global__ void k_fold_slots(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix) {
-  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void fold_slots_body(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix, int bx) {
+  const int sl = bx * blockDim.x + threadIdx.x;
   if (sl >= counts[0]) return;
   const int at = atom_of_slot[sl];
   if (at < 0) return;
@@ -571,6 +668,15 @@ __global__ void k_fold_slots(const int* __restrict__ counts, const int* __restri
   force_fix[3LL * at + 1] += force_slot[3LL * sl + 1];
   force_fix[3LL * at + 2] += force_slot[3LL * sl + 2];
 }
+__global__ void k_fold_slots(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
+                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix) { fold_slots_body(counts, atom_of_slot, force_slot, force_fix, blockIdx.x); }
+struct FoldSlotsArgs { const int* counts; const int* atom_of_slot; const long long* force_slot; long long* force_fix; };
+__global__ void kb_fold_slots(const FoldSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const FoldSlotsArgs& q = t[r];
+  fold_slots_body(q.counts, q.atom_of_slot, q.force_slot, q.force_fix, blockIdx.x - pre[r]);
+}
+
 
 // ------------------------------------------------------------ pair kernel
 template <typename T> struct Prec;
@@ -703,7 +809,7 @@ __device__ inline long long force_fix_of(double v) { return __double2ll_rn(v * P
 // atomics of one instruction fall on a few contiguous cache lines; integer
 // addition is associative, so the result does not depend on their order.
 template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
-__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) {
+__device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int bx) {
   extern __shared__ unsigned char s_lj_raw[];
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
   LJ4* s_lj = reinterpret_cast<LJ4*>(s_lj_raw);
@@ -715,7 +821,7 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     __syncthreads();
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int cl = blockIdx.x * 4 + wave;
+  const int cl = bx * 4 + wave;
   const int ncl = a.counts[1];
   if (cl >= ncl) {
     if (WE && cl < a.ncl_max && lane < PM_NE) a.epart[(long long)cl * PM_NE + lane] = 0.0;
@@ -921,6 +1027,16 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
     }
   }
 }
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
+__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) { pair_body<T, LJSWF, CK, CSWF, WE, M>(box, a, pp, blockIdx.x); }
+struct PairArgs { PmBox box; PmNbLaunch a; PmPairParams pp; };
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
+__global__ __launch_bounds__(256) void kb_pair(const PairArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  const PairArgs& q = t[r];
+  pair_body<T, LJSWF, CK, CSWF, WE, M>(q.box, q.a, q.pp, blockIdx.x - pre[r]);
+}
+
 
 // ------------------------------------------------------- adjoint pair kernel
 // Force/r F(u) of one switched term and its derivative F_u = dF/du, from the
@@ -1269,8 +1385,18 @@ __global__ void k_diag(PmBox box, const double* __restrict__ pos, const int* __r
 }  // namespace
 
 // ----------------------------------------------------------------- launchers
+// pair-kernel variant of a record: precision, potentials and switching,
+// energy flag and cluster size
+static int pair_variant(bool dbl, int lj, int ck, int cs, bool we, int m) {
+  return (dbl ? 1 : 0) | (lj << 1) | (ck << 3) | (cs << 5) | ((we ? 1 : 0) << 7) | ((m == 16 ? 1 : 0) << 8);
+}
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
                    int* cell_count, int* rank, float* key, const int* rebuild) {
+  if (pm_recording()) {
+    const BinArgs a{box, n, pos, cell_of, cell_count, rank, key, rebuild};
+    pm_record(PM_REC_BIN, 0, n > 0 ? (n + 255) / 256 : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, cell_of, cell_count, rank, key,
                        rebuild);
@@ -1278,12 +1404,22 @@ void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, in
 
 void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
                           int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild) {
+  if (pm_recording()) {
+    const ScanCellsArgs a{ncells, cell_count, cell_start, cell_slot, counts, cluster_cell, ncl_max, m, rebuild};
+    pm_record(PM_REC_SCAN_CELLS, 0, 1, 1024, 0, &a, sizeof(a));
+    return;
+  }
   hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
                      counts, cluster_cell, ncl_max, m, rebuild);
 }
 
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
                        int* cell_atoms, const int* rebuild) {
+  if (pm_recording()) {
+    const ScatterArgs a{n, cell_of, rank, cell_start, cell_atoms, rebuild};
+    pm_record(PM_REC_SCATTER, 0, n > 0 ? (n + 255) / 256 : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell_of, rank, cell_start,
                        cell_atoms, rebuild);
@@ -1296,6 +1432,12 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
                           int* atom_of_slot, int dummy_type, int dummy_slot, int m, const int* rebuild,
                           int4* x_build) {
   int nb = (ncells + 3) / 4;
+  if (pm_recording()) {
+    const FillSlotsArgs a{box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos,
+                          slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build};
+    pm_record(PM_REC_FILL_SLOTS, 0, nb > 0 ? nb : 1, 256, 0, &a, sizeof(a));
+    return;
+  }
   hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, box, ncells, pos, q, type,
                      cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta,
                      atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build);
@@ -1303,25 +1445,44 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
 
 void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* pos, const int4* x_build,
                           float lim2, int* flag) {
+  if (pm_recording()) {
+    const CheckDispArgs a{box, n, pos, x_build, lim2, flag};
+    pm_record(PM_REC_CHECK_DISP, 0, n > 0 ? (n + 255) / 256 : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_check_disp, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, x_build, lim2, flag);
 }
 
 void pm_launch_set_flag(hipStream_t s, int* flag, int v) {
+  if (pm_recording()) { pm_record(-1, 0, 0, 0, 0, nullptr, 0); return; }   // not batched
   hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, flag, v);
 }
 
 void pm_launch_end_build(hipStream_t s, int* flag, unsigned long long* builds) {
-  hipLaunchKernelGGL(k_end_build, dim3(1), dim3(64), 0, s, flag, builds);
+  if (pm_recording()) {
+    // flag null: a replica without list reuse (nothing to do, the record keeps
+    // the sequences of all replicas aligned)
+    const EndBuildArgs a{flag, builds};
+    pm_record(PM_REC_END_BUILD, 0, flag ? 1 : 0, 64, 0, &a, sizeof(a));
+    return;
+  }
+  if (flag) hipLaunchKernelGGL(k_end_build, dim3(1), dim3(64), 0, s, flag, builds);
 }
 
 void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build) {
+  if (pm_recording()) { pm_record(-1, 0, 0, 0, 0, nullptr, 0); return; }   // not batched
   if (n > 0) hipLaunchKernelGGL(k_invalidate_build, dim3((n + 255) / 256), dim3(256), 0, s, n, x_build);
 }
 
 void pm_launch_refresh_slots(hipStream_t s, const PmBox& box, int nslot_max, const int* counts,
                              const int* atom_of_slot, const double* pos, const double* q, int4* slot_pos,
                              longlong4* slot_pos64) {
+  if (pm_recording()) {
+    const RefreshSlotsArgs a{box, counts, atom_of_slot, pos, q, slot_pos, slot_pos64};
+    pm_record(PM_REC_REFRESH_SLOTS, 0, nslot_max > 0 ? (nslot_max + 255) / 256 : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
   if (nslot_max > 0)
     hipLaunchKernelGGL(k_refresh_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, box, counts, atom_of_slot,
                        pos, q, slot_pos, slot_pos64);
@@ -1329,6 +1490,11 @@ void pm_launch_refresh_slots(hipStream_t s, const PmBox& box, int nslot_max, con
 
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
                           const long long* force_slot, long long* force_fix) {
+  if (pm_recording()) {
+    const FoldSlotsArgs a{counts, atom_of_slot, force_slot, force_fix};
+    pm_record(PM_REC_FOLD_SLOTS, 0, nslot_max > 0 ? (nslot_max + 255) / 256 : 0, 256, 0, &a, sizeof(a));
+    return;
+  }
   if (nslot_max > 0)
     hipLaunchKernelGGL(k_fold_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, counts, atom_of_slot,
                        force_slot, force_fix);
@@ -1342,6 +1508,11 @@ void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const 
   double cut = sqrt(p.cut2) + a.skin;
   float c = (float)(cut * (1.0 + 1e-5) + 1e-4);
   size_t lds = 4 * (3 * a.nstencil + 1 + 2 * a.xs_tcap) * sizeof(int);
+  if (pm_recording()) {
+    const ListArgs q{box, a, c * c};
+    pm_record(PM_REC_LIST, a.m, (a.ncl_max + 3) / 4, 256, (unsigned)lds, &q, sizeof(q));
+    return;
+  }
   if (a.m == 16)
     hipLaunchKernelGGL(k_list<16>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
   else
@@ -1352,6 +1523,12 @@ template <typename T, int LJ, int CK, int CS>
 static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
   const dim3 g((a.ncl_max + 3) / 4), b(256);
+  if (pm_recording()) {
+    const PairArgs q{box, a, p};
+    pm_record(PM_REC_PAIR, pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m), g.x, 256,
+              (unsigned)lds, &q, sizeof(q));
+    return;
+  }
   if (a.m == 16) {
     if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 16>), g, b, lds, s, box, a, p);
     else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 16>), g, b, lds, s, box, a, p);
@@ -1442,4 +1619,57 @@ void pm_launch_diag(hipStream_t s, const PmBox& box, int n, const double* pos, c
   hipLaunchKernelGGL(k_diag, dim3((ncells + 63) / 64), dim3(64), 0, s, box, pos, cell_start, cell_count,
                      cell_atoms, half_stencil, nhalf, molecule, ex_row, ex_partner, ex_cls, pair_cut2, ncells,
                      cand, inrc);
+}
+
+// batched pair kernels: clusters of 8, with or without energies
+template <typename T>
+static bool launch_pair_batched(hipStream_t s, int variant, const PairArgs* t, const int* pre, int nrep, dim3 g,
+                                unsigned lds) {
+  const int lj = (variant >> 1) & 3, ck = (variant >> 3) & 3, cs = (variant >> 5) & 3;
+  const bool we = (variant >> 7) & 1;
+  if ((variant >> 8) & 1) return false;   // 16-atom clusters
+#define PM_CASE(L, C, S) \
+  if (lj == L && ck == C && (C == PM_COUL_NONE || cs == S)) { \
+    if (we) hipLaunchKernelGGL((kb_pair<T, L, C, S, true, 8>), g, dim3(256), lds, s, t, pre, nrep); \
+    else hipLaunchKernelGGL((kb_pair<T, L, C, S, false, 8>), g, dim3(256), lds, s, t, pre, nrep); \
+    return true; }
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C1)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C2)
+  PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_NONE, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C1, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_CUTOFF, PM_COUL_NONE, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C1, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C1)
+  PM_CASE(PM_SWF_C2, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_CUTOFF, PM_COUL_EWALD, PM_SWF_CUTOFF)
+  PM_CASE(PM_SWF_C2, PM_COUL_PLAIN, PM_SWF_C2)
+  PM_CASE(PM_SWF_C1, PM_COUL_PLAIN, PM_SWF_C1)
+#undef PM_CASE
+  return false;
+}
+
+bool pm_launch_batched_nb(hipStream_t s, int kind, int variant, const void* t, const int* pre, int nrep,
+                          unsigned total, unsigned threads, unsigned lds) {
+  const dim3 g(total), b(threads);
+  switch (kind) {
+    case PM_REC_BIN: hipLaunchKernelGGL(kb_bin, g, b, 0, s, (const BinArgs*)t, pre, nrep); return true;
+    case PM_REC_SCAN_CELLS: hipLaunchKernelGGL(kb_scan_cells, g, b, 0, s, (const ScanCellsArgs*)t, pre, nrep); return true;
+    case PM_REC_SCATTER: hipLaunchKernelGGL(kb_scatter, g, b, 0, s, (const ScatterArgs*)t, pre, nrep); return true;
+    case PM_REC_FILL_SLOTS: hipLaunchKernelGGL(kb_fill_slots, g, b, 0, s, (const FillSlotsArgs*)t, pre, nrep); return true;
+    case PM_REC_CHECK_DISP: hipLaunchKernelGGL(kb_check_disp, g, b, 0, s, (const CheckDispArgs*)t, pre, nrep); return true;
+    case PM_REC_END_BUILD: hipLaunchKernelGGL(kb_end_build, g, b, 0, s, (const EndBuildArgs*)t, pre, nrep); return true;
+    case PM_REC_REFRESH_SLOTS: hipLaunchKernelGGL(kb_refresh_slots, g, b, 0, s, (const RefreshSlotsArgs*)t, pre, nrep); return true;
+    case PM_REC_FOLD_SLOTS: hipLaunchKernelGGL(kb_fold_slots, g, b, 0, s, (const FoldSlotsArgs*)t, pre, nrep); return true;
+    case PM_REC_LIST:
+      if (variant == 16) hipLaunchKernelGGL(kb_list<16>, g, b, lds, s, (const ListArgs*)t, pre, nrep);
+      else hipLaunchKernelGGL(kb_list<8>, g, b, lds, s, (const ListArgs*)t, pre, nrep);
+      return true;
+    case PM_REC_PAIR:
+      if (variant & 1) return launch_pair_batched<double>(s, variant, (const PairArgs*)t, pre, nrep, g, lds);
+      return launch_pair_batched<float>(s, variant, (const PairArgs*)t, pre, nrep, g, lds);
+    default: return false;
+  }
 }

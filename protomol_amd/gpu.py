@@ -29,7 +29,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
-           "pm_set_charges", "pm_lanczos", "pm_md_autotune"]
+           "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info"]
 OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
@@ -92,6 +92,8 @@ def load_library(path: str = LIB_PATH):
     lib.pm_param_gradients.argtypes = [P, I, C.c_int, D, D, D, D]
     lib.pm_set_charges.argtypes = [P, D]
     lib.pm_md_autotune.argtypes = [P, C.c_int, I, I, D]
+    lib.pm_batch_set_mode.argtypes = [P, C.c_int]
+    lib.pm_batch_info.argtypes = [P, I, C.c_char_p, C.c_int]
     lib.pm_lanczos.argtypes = [P, I, C.c_int, D, D, C.c_int, C.c_int, C.c_int, D, D, D, D]
     _lib = lib
     return lib
@@ -339,6 +341,17 @@ class Batch:
         dl = np.full(n, np.nan)
         self._check(self.lib.pm_batch_exchange(self.h, int(attempt), int(seed), _ip(acc), _dp(dl)))
         return acc, dl
+
+    def set_mode(self, mode: str):
+        """"auto" (batched launches when the replicas allow them) or "chains"."""
+        self._check(self.lib.pm_batch_set_mode(self.h, {"auto": 0, "chains": 1}[mode]))
+
+    def info(self):
+        """(batched, why): whether the last run used batched launches, and why not."""
+        b = C.c_int()
+        why = C.create_string_buffer(256)
+        self._check(self.lib.pm_batch_info(self.h, C.byref(b), why, 256))
+        return bool(b.value), why.value.decode()
 
     def ladder(self):
         n = len(self.contexts)

@@ -345,17 +345,23 @@ def test_graph_replay_matches_direct(gpu_ctx_factory, integrator):
         np.testing.assert_array_equal(a, b)
 
 
-def test_batch_matches_single_replicas(gpu_ctx_factory):
-    """Replicas of uneven size advanced as one batch (captured graph of all
-    replicas) follow the same trajectories, bit for bit, as when each is run alone."""
-    say("case: batch of 5 uneven replicas vs single runs, 23 steps")
+@pytest.mark.parametrize("mode", ["auto", "chains"])
+def test_batch_matches_single_replicas(gpu_ctx_factory, mode):
+    """Replicas of uneven size advanced as one batch (auto: one launch per
+    kernel over all replicas; chains: one captured chain per replica) follow
+    the same trajectories, bit for bit, as when each is run alone."""
+    say(f"case: batch of 5 uneven replicas vs single runs, 23 steps, mode {mode}")
     from tests.cases import replica_context
     from protomol_amd import gpu
     systems = water.replica_boxes(5, seed=3, lo=100, hi=300)
     temps = water.replica_temperatures(5)
     ctxs = [replica_context(gpu_ctx_factory, s, t, seed=k + 1) for k, (s, t) in enumerate(zip(systems, temps))]
     b = gpu.Batch(ctxs)
+    b.set_mode(mode)
     b.run(11)
+    batched, why = b.info()
+    say("batched launches:", batched, why)
+    assert batched == (mode == "auto")
     b.run(12)
     got = [c.md_get_state() for c in ctxs]
     b.close()
