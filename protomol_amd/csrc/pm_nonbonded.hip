@@ -544,6 +544,22 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   // stop scanning and report it as a list overflow
   const int scan = min(total, PM_CAND_FACTOR * a.lcap);
   if (scan < total && lane == 0) a.overflow[0] = 1;
+  // small boxes: the cluster's positions for the per-atom test, and which of
+  // its slots hold atoms (padding slots must not keep an entry: their
+  // positions follow the arrival order of the binning, so the lists, and the
+  // rounding of the sums over them, would change from run to run)
+  int qx[M], qy[M], qz[M];
+  unsigned int qreal = 0;
+  if (!a.bbox_filter) {
+#pragma unroll
+    for (int ii = 0; ii < M; ii++) {
+      const int4 q = a.slot_pos[cl * M + ii];
+      if (((a.slot_meta[cl * M + ii] >> 8) & 0x7fffff) != 0x7fffff) qreal |= 1u << ii;
+      qx[ii] = q.x;
+      qy[ii] = q.y;
+      qz[ii] = q.z;
+    }
+  }
   int scur = 0;   // stencil entry holding candidate k0 (uniform)
   for (int k0 = 0; k0 < scan; k0 += 64) {
     const int k = k0 + lane;
@@ -572,13 +588,12 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
         // kernel's own minimal-image difference (the cluster's positions
         // are wave-uniform loads)
         pass = false;
+#pragma unroll
         for (int ii = 0; ii < M; ii++) {
-          const int4 q = a.slot_pos[cl * M + ii];
-          const bool real = ((a.slot_meta[cl * M + ii] >> 8) & 0x7fffff) != 0x7fffff;
-          const float dx = (float)(int)(p.x - q.x) * fx;
-          const float dy = (float)(int)(p.y - q.y) * fy;
-          const float dz = (float)(int)(p.z - q.z) * fz;
-          pass = pass || (real && dx * dx + dy * dy + dz * dz < cut2_list);
+          const float dx = (float)(int)(p.x - qx[ii]) * fx;
+          const float dy = (float)(int)(p.y - qy[ii]) * fy;
+          const float dz = (float)(int)(p.z - qz[ii]) * fz;
+          pass = pass || (((qreal >> ii) & 1u) && dx * dx + dy * dy + dz * dz < cut2_list);
         }
       }
       o = (a.slot_meta[sj] >> 8) & 0x7fffff;
