@@ -372,6 +372,14 @@ __global__ void kb_refresh_slots(const RefreshSlotsArgs* __restrict__ t, const i
 }
 
 
+// periodic distance (fixed-point units) from a point at offset t = x - lo
+// (mod 2^32) to the interval [lo, lo + w]
+__device__ inline float cell_gap(unsigned int t, unsigned int w) {
+  if (t <= w) return 0.f;
+  const unsigned int up = t - w, down = 0u - t;
+  return (float)(up < down ? up : down);
+}
+
 // ------------------------------------------------------------ wave helpers
 __device__ inline int wave_incl_scan(int v, int lane) {
 #pragma unroll
@@ -472,6 +480,22 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   pmin = __shfl(wave_imin_m<M>(pmin), 0, 64);
   pmax = __shfl(wave_imax_m<M>(pmax), 0, 64);
 
+  // small boxes: the cluster's positions for the per-atom test, and which of
+  // its slots hold atoms (padding slots must not keep an entry: their
+  // positions follow the arrival order of the binning, so the lists, and the
+  // rounding of the sums over them, would change from run to run)
+  int qx[M], qy[M], qz[M];
+  unsigned int qreal = 0;
+  if (active && !a.bbox_filter) {
+#pragma unroll
+    for (int ii = 0; ii < M; ii++) {
+      const int4 q = a.slot_pos[cl * M + ii];
+      if (((a.slot_meta[cl * M + ii] >> 8) & 0x7fffff) != 0x7fffff) qreal |= 1u << ii;
+      qx[ii] = q.x;
+      qy[ii] = q.y;
+      qz[ii] = q.z;
+    }
+  }
   // stencil cells: half list, so only cells after this one in linear order,
   // and this cell's atoms from this cluster on (the pair kernel keeps a pair
   // only when the j slot follows the i slot); cells whose box lies beyond the
@@ -504,6 +528,25 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
         const float gy = fmaxf(0.f, fmaxf(cly - hiy, loy - chy));
         const float gz = fmaxf(0.f, fmaxf(clz - hiz, loz - chz));
         if (gx * gx + gy * gy + gz * gz >= cut2_list) ct = 0;
+      } else if (ct > 0) {
+        // small box: per axis, the periodic distance from each of the
+        // cluster's atoms to the cell's interval (padded by 1e-3 A) bounds
+        // the pair kernel's minimal-image difference from below
+        const unsigned int bx = (unsigned int)(long long)(box.cell0_fix[0] + wx * box.real_fix[0]);
+        const unsigned int by = (unsigned int)(long long)(box.cell0_fix[1] + wy * box.real_fix[1]);
+        const unsigned int bz = (unsigned int)(long long)(box.cell0_fix[2] + wz * box.real_fix[2]);
+        const unsigned int wdx = (unsigned int)(long long)box.real_fix[0] + 1u;
+        const unsigned int wdy = (unsigned int)(long long)box.real_fix[1] + 1u;
+        const unsigned int wdz = (unsigned int)(long long)box.real_fix[2] + 1u;
+        bool near = false;
+#pragma unroll
+        for (int ii = 0; ii < M; ii++) {
+          const float gx = fmaxf(cell_gap((unsigned int)qx[ii] - bx, wdx) * fx - 1e-3f, 0.f);
+          const float gy = fmaxf(cell_gap((unsigned int)qy[ii] - by, wdy) * fy - 1e-3f, 0.f);
+          const float gz = fmaxf(cell_gap((unsigned int)qz[ii] - bz, wdz) * fz - 1e-3f, 0.f);
+          near = near || (((qreal >> ii) & 1u) && gx * gx + gy * gy + gz * gz < cut2_list);
+        }
+        if (!near) ct = 0;
       }
     }
     const unsigned long long bal = __ballot(ct > 0);
@@ -544,22 +587,6 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   // stop scanning and report it as a list overflow
   const int scan = min(total, PM_CAND_FACTOR * a.lcap);
   if (scan < total && lane == 0) a.overflow[0] = 1;
-  // small boxes: the cluster's positions for the per-atom test, and which of
-  // its slots hold atoms (padding slots must not keep an entry: their
-  // positions follow the arrival order of the binning, so the lists, and the
-  // rounding of the sums over them, would change from run to run)
-  int qx[M], qy[M], qz[M];
-  unsigned int qreal = 0;
-  if (!a.bbox_filter) {
-#pragma unroll
-    for (int ii = 0; ii < M; ii++) {
-      const int4 q = a.slot_pos[cl * M + ii];
-      if (((a.slot_meta[cl * M + ii] >> 8) & 0x7fffff) != 0x7fffff) qreal |= 1u << ii;
-      qx[ii] = q.x;
-      qy[ii] = q.y;
-      qz[ii] = q.z;
-    }
-  }
   int scur = 0;   // stencil entry holding candidate k0 (uniform)
   for (int k0 = 0; k0 < scan; k0 += 64) {
     const int k = k0 + lane;
