@@ -111,6 +111,39 @@ def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=
                 skin=skin_used * 1e-3, skin_milli=skin_used, cluster=cluster_used, tuned=tuned)
 
 
+def c3_bench(steps=200, warmup=20):
+    """The million-atom workload: 333,333 waters (999,999 atoms) in a 215.4 A
+    cube, the force set of the main workload with PME 240^3 (order 4, rc 10),
+    leapfrog 1 fs, mixed precision, library defaults; ns/day over `steps`
+    steps after `warmup`."""
+    from protomol_amd import water
+    s = water.named("water1m", seed=1)
+    from protomol_amd import gpu
+    ctx = gpu.Context(0, "mixed")
+    ctx.set_topology(s)
+    ctx.set_boundary(s.box, s.origin, CELL)
+    alpha = pme_alpha(RC, 1e-6)
+    hs = [ctx.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff "
+                    "-switchingFunction C2 -switchingFunction Cutoff", [SWITCHON, RC, alpha, RC, RC]),
+          ctx.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                    [240, 240, 240, RC, ORDER, 1e-6, -1.0]),
+          ctx.force("Bond", []), ctx.force("Angle", [])]
+    ctx.md_setup(hs, "leapfrog", dt=DT_FS)
+    ctx.md_set_state(s.pos, s.vel)
+    ctx.md_run(warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.md_run(steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    builds = ctx.stats()["list_builds"]
+    ctx.close()
+    return {"workload": "water1m: 999,999-atom TIP3 water, 215.4 A cube, LJ-C2(9-10 A) + Ewald-real rc 10 A, "
+                        "PME 240^3 order 4, bonds+angles, leapfrog dt 1 fs", "atoms": int(s.n), "steps": steps,
+            "ms_per_step": dt * 1e3 / steps, "ns_per_day": steps / dt * DT_FS * 86400 * 1e-6,
+            "atom_steps_per_s": s.n * steps / dt, "list_builds": builds}
+
+
 def copy_probe(mb):
     """Ten device-to-device copies of `mb` megabytes: the kernel whose
     FETCH_SIZE / WRITE_SIZE counters calibrate the HBM-traffic runs."""
@@ -268,6 +301,7 @@ def main():
                     help="1: MD steps replay captured HIP graphs (default); 0: direct launches (for counter runs)")
     ap.add_argument("--list-skin", type=int, default=-1, metavar="MILLI_A",
                     help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the million-atom measurement")
     ap.add_argument("--only-replicas", action="store_true",
                     help="run only the batched replica-exchange measurement and print its record")
     ap.add_argument("--autotune", action="store_true",
@@ -359,6 +393,8 @@ def main():
     }
     if not args.no_replicas:
         line["replica_exchange"] = replica_bench()
+    if not args.no_c3:
+        line["c3_million_atoms"] = c3_bench()
     if not args.no_drift:
         line["nve_drift"] = {"mixed": nve_drift(s, "mixed"), "fp64": nve_drift(s, "double")}
     if not args.no_cpu_baseline:
