@@ -905,32 +905,25 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
   const bool vir = WE && a.want_virial != 0;
   long long* __restrict__ ff = a.force_slot;
   const int comp = ii < 3 ? ii : ii - 3;   // force component a lane adds (lanes ii < 6)
-  T pend_v = T(0);                        // j-force component of the previous row
-  int pend_sj = 0;
 
-  // software pipeline: the entry two iterations ahead and the j atom of the
-  // next iteration are in flight while an iteration computes
-  int e_cur = js < cnt ? lst[js] : a.dummy_slot;
-  int e_nxt = js + JW < cnt ? lst[js + JW] : a.dummy_slot;
-  P4 pj_cur = spos[e_cur & 0x7fffffff];
-  int mj_cur = a.slot_meta[e_cur & 0x7fffffff];
+  // Two iterations per trip with double-buffered prefetch: while row A
+  // computes, the j atom of row B and the entry of the next trip's row A are
+  // in flight, and the other way round, so no register copies carry values
+  // across the loop; the trip's two rows share one atomic instruction.
+  int eA = js < cnt ? lst[js] : a.dummy_slot;
+  int eB = js + JW < cnt ? lst[js + JW] : a.dummy_slot;
+  P4 pA = spos[eA & 0x7fffffff];
+  int mA = a.slot_meta[eA & 0x7fffffff];
+  P4 pB;
+  int mB;
 
-  for (int base = 0; base < cnt; base += JW) {
-    const int k = base + js;
+  // one iteration: the pairs of entry k against the cluster; returns this
+  // lane's component of the entry's j force (row sum) and the j slot
+  auto row = [&](const int k, const int e, const P4& pj, const int mj, int& sj_out) -> T {
     const bool valid = k < cnt;
-    const int e = e_cur;
-    const P4 pj = pj_cur;
-    const int mj = mj_cur;
     const int sj = e & 0x7fffffff;
     const bool flagged = e < 0;
-    {
-      const int k2 = k + 2 * JW;
-      const int e2 = k2 < cnt ? lst[k2] : a.dummy_slot;
-      pj_cur = spos[e_nxt & 0x7fffffff];
-      mj_cur = a.slot_meta[e_nxt & 0x7fffffff];
-      e_cur = e_nxt;
-      e_nxt = e2;
-    }
+    sj_out = sj;
     T dx, dy, dz;
     if constexpr (sizeof(T) == 4) {
       dx = (T)(int)(pj.x - pi.x) * sx;
@@ -1009,25 +1002,31 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
         vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
       }
     }
-    // j force of this row, added every second iteration: lanes ii = 0, 1, 2
-    // add the x, y, z component of the previous iteration's row, lanes
-    // ii = 3, 4, 5 those of this one (one atomic instruction per two rows)
     const T jx = row_sum<M>(fx_), jy = row_sum<M>(fy_), jz = row_sum<M>(fz_);
-    const T jc = comp == 0 ? jx : (comp == 1 ? jy : jz);
-    if ((base / JW) & 1) {
-      const T v = ii < 3 ? pend_v : jc;
-      const int sl = ii < 3 ? pend_sj : sj;
-      if (ii < 6 && v != T(0))
-        atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
-    } else {
-      pend_v = jc;
-      pend_sj = sj;
-    }
-  }
-  if (((cnt + JW - 1) / JW) & 1) {
-    // odd number of iterations: the last row is still pending
-    if (ii < 3 && pend_v != T(0))
-      atomicAdd((unsigned long long*)(ff + 3LL * pend_sj + comp), (unsigned long long)force_fix_of(pend_v));
+    return comp == 0 ? jx : (comp == 1 ? jy : jz);
+  };
+
+  for (int base = 0; base < cnt; base += 2 * JW) {
+    const int kA = base + js, kB = kA + JW;
+    // row A; prefetch the j atom of row B and the entry of the next row A
+    const int eA_now = eA;
+    pB = spos[eB & 0x7fffffff];
+    mB = a.slot_meta[eB & 0x7fffffff];
+    eA = kA + 2 * JW < cnt ? lst[kA + 2 * JW] : a.dummy_slot;
+    int sA;
+    const T vA = row(kA, eA_now, pA, mA, sA);
+    // row B; prefetch the j atom of the next row A and the entry of the next row B
+    const int eB_now = eB;
+    pA = spos[eA & 0x7fffffff];
+    mA = a.slot_meta[eA & 0x7fffffff];
+    eB = kB + 2 * JW < cnt ? lst[kB + 2 * JW] : a.dummy_slot;
+    int sB;
+    const T vB = row(kB, eB_now, pB, mB, sB);
+    // lanes ii = 0, 1, 2 add the x, y, z component of row A, lanes 3, 4, 5 those of row B
+    const T v = ii < 3 ? vA : vB;
+    const int sl = ii < 3 ? sA : sB;
+    if (ii < 6 && v != T(0))
+      atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
   }
   // reduce the i force over the j slots (lanes ii, ii+8, ...); lanes
   // js = 0, 1, 2 then add its x, y, z component
