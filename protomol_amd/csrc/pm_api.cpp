@@ -905,8 +905,6 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
                        c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
                        c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m, rb, reuse ? c->d_xbuild : nullptr);
-  pm_launch_refresh_slots(s, c->box, reuse ? c->nslot_max : 0, c->d_counts, c->d_atom_of_slot, c->d_pos, c->d_q,
-                          c->d_slot_pos, c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr);
   if (!reuse) c->list_stale = true;   // a layout built without the skin is not reusable
   }
 
@@ -1086,7 +1084,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       pm_launch_adjoint(s, c->box, pl.first, pl.second, c->precision, adj->mode, adj->slot_v, adj->slot);
     }
     if (!pair_launch.empty())
-      pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, adj->slot, adj->fix);
+      pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, adj->slot, adj->fix, nullptr, nullptr);
     for (auto& pl : pme_launch)
       pm_launch_pme_adjoint(s, pl.first, c->plans, c->d_slot_pos64, c->precision, adj->mode == 1 ? 1 : 0,
                             adj->slot_v, adj->mode == 1 ? adj->hv : adj->dq);
@@ -1158,13 +1156,13 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       pm_launch_pair(s, c->box, pl.first, pl.second, c->precision);
     }
   }
-  // every reader of the flag has run (null flag without reuse: nothing runs)
-  pm_launch_end_build(s, reuse ? c->d_rebuild : nullptr, c->d_stats + 2);
   // join, then the per-slot pair forces into the per-atom forces
   if (side) HIPCHECK(c, hipStreamWaitEvent(s, c->ev_join, 0));
   if (!passes.empty()) {
     TimedScope ts(c, PM_T_BIN);
-    pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_force_slot, c->d_force);
+    // with reuse also the end of the evaluation: the flag's readers all ran
+    pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_force_slot, c->d_force,
+                         reuse ? c->d_rebuild : nullptr, c->d_stats + 2);
   }
   if (want_energy) {
     TimedScope ts(c, PM_T_REDUCE);

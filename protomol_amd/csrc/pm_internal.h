@@ -149,15 +149,12 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
 void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* pos, const int4* x_build,
                           float lim2, int* flag);
 void pm_launch_set_flag(hipStream_t s, int* flag, int v);
-// builds[0] += (*flag != 0); *flag = 0
-void pm_launch_end_build(hipStream_t s, int* flag, unsigned long long* builds);
 void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build);
-void pm_launch_refresh_slots(hipStream_t s, const PmBox& box, int nslot_max, const int* counts,
-                             const int* atom_of_slot, const double* pos, const double* q, int4* slot_pos,
-                             longlong4* slot_pos64);
-// force_fix[atom] += force_slot[slot] for every real slot
+// force_fix[atom] += force_slot[slot] for every real slot; with `flag` (list
+// reuse) also the end of the evaluation: builds[0] += (*flag != 0), *flag = 0
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
-                          const long long* force_slot, long long* force_fix);
+                          const long long* force_slot, long long* force_fix, int* flag,
+                          unsigned long long* builds);
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
 // Adjoint pair kernel (clusters of 8): mode 1 H v, mode 2 (dE/dq, dE/dR, dE/de)
@@ -296,8 +293,8 @@ void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, 
 // counts and runs the same kernel body on that replica's arguments.
 enum PmRecKind {
   PM_REC_KICK_DRIFT = 1, PM_REC_LANGEVIN_KICK, PM_REC_DRIFT, PM_REC_MD_ADVANCE, PM_REC_ZERO_FORCE_CELLS,
-  PM_REC_CHECK_DISP, PM_REC_BIN, PM_REC_SCAN_CELLS, PM_REC_SCATTER, PM_REC_FILL_SLOTS, PM_REC_REFRESH_SLOTS,
-  PM_REC_LIST, PM_REC_PAIR, PM_REC_END_BUILD, PM_REC_FOLD_SLOTS, PM_REC_BONDED_FORCE, PM_REC_BONDED_ENERGY,
+  PM_REC_CHECK_DISP, PM_REC_BIN, PM_REC_SCAN_CELLS, PM_REC_SCATTER, PM_REC_FILL_SLOTS,
+  PM_REC_LIST, PM_REC_PAIR, PM_REC_FOLD_SLOTS, PM_REC_BONDED_FORCE, PM_REC_BONDED_ENERGY,
   PM_REC_REDUCE1, PM_REC_REDUCE2
 };
 #define PM_REC_ARGMAX 1024

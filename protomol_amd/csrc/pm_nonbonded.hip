@@ -199,9 +199,34 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
                              const int* __restrict__ rebuild, int4* __restrict__ x_build, int bx) {
-  if (rebuild && *rebuild == 0) return;
   const int lane = threadIdx.x & 63;
   const int c = bx * 4 + (threadIdx.x >> 6);
+  if (rebuild && *rebuild == 0) {
+    // layout kept (list reuse): current positions and charges into the
+    // cell's slots, with the arithmetic of the build below; padding slots keep
+    // their place (they are masked)
+    if (c >= ncells) return;
+    const int s0 = cell_slot[c], k = cell_count[c];
+    for (int u = lane; u < k; u += 64) {
+      const int sl = s0 + u;
+      const int ai = atom_of_slot[sl];
+      int4 p;
+      p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
+      p.y = (int)to_fix(pos[3 * ai + 1], box.to_fix[1]);
+      p.z = (int)to_fix(pos[3 * ai + 2], box.to_fix[2]);
+      p.w = __float_as_int((float)q[ai]);
+      slot_pos[sl] = p;
+      if (slot_pos64) {
+        longlong4 p8;
+        p8.x = (long long)to_fix64(pos[3 * ai], box.to_fix[0]);
+        p8.y = (long long)to_fix64(pos[3 * ai + 1], box.to_fix[1]);
+        p8.z = (long long)to_fix64(pos[3 * ai + 2], box.to_fix[2]);
+        p8.w = __double_as_longlong(q[ai]);
+        slot_pos64[sl] = p8;
+      }
+    }
+    return;
+  }
   if (c == 0 && lane == 0) {
     // the shared padding slot of the pair lists
     slot_pos[dummy_slot] = make_int4(0, 0, 0, 0);
@@ -316,61 +341,10 @@ __global__ void k_set_flag(int* flag, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
 }
 
-// end of a reusing evaluation: count a build, clear the flag
-__device__ inline void end_build_body(int* flag, unsigned long long* builds, int bx) {
-  if (threadIdx.x == 0 && bx == 0) {
-    if (*flag) builds[0] += 1ull;
-    *flag = 0;
-  }
-}
-__global__ void k_end_build(int* flag, unsigned long long* builds) { end_build_body(flag, builds, blockIdx.x); }
-struct EndBuildArgs { int* flag; unsigned long long* builds; };
-__global__ void kb_end_build(const EndBuildArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
-  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
-  const EndBuildArgs& q = t[r];
-  end_build_body(q.flag, q.builds, blockIdx.x - pre[r]);
-}
-
-
 __global__ void k_invalidate_build(int n, int4* __restrict__ x_build) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x_build[i] = make_int4(0, 0, 0, 0);
 }
-
-// Current positions (and charges) into the slots of the kept layout, with the
-// arithmetic of k_fill_slots; padding slots keep their place (they are masked).
-__device__ inline void refresh_slots_body(PmBox box, const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                                const double* __restrict__ pos, const double* __restrict__ q,
-                                int4* __restrict__ slot_pos, longlong4* __restrict__ slot_pos64, int bx) {
-  const int sl = bx * blockDim.x + threadIdx.x;
-  if (sl >= counts[0]) return;
-  const int ai = atom_of_slot[sl];
-  if (ai < 0) return;
-  int4 p;
-  p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
-  p.y = (int)to_fix(pos[3 * ai + 1], box.to_fix[1]);
-  p.z = (int)to_fix(pos[3 * ai + 2], box.to_fix[2]);
-  p.w = __float_as_int((float)q[ai]);
-  slot_pos[sl] = p;
-  if (slot_pos64) {
-    longlong4 p8;
-    p8.x = (long long)to_fix64(pos[3 * ai], box.to_fix[0]);
-    p8.y = (long long)to_fix64(pos[3 * ai + 1], box.to_fix[1]);
-    p8.z = (long long)to_fix64(pos[3 * ai + 2], box.to_fix[2]);
-    p8.w = __double_as_longlong(q[ai]);
-    slot_pos64[sl] = p8;
-  }
-}
-__global__ void k_refresh_slots(PmBox box, const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                                const double* __restrict__ pos, const double* __restrict__ q,
-                                int4* __restrict__ slot_pos, longlong4* __restrict__ slot_pos64) { refresh_slots_body(box, counts, atom_of_slot, pos, q, slot_pos, slot_pos64, blockIdx.x); }
-struct RefreshSlotsArgs { PmBox box; const int* counts; const int* atom_of_slot; const double* pos; const double* q; int4* slot_pos; longlong4* slot_pos64; };
-__global__ void kb_refresh_slots(const RefreshSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
-  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
-  const RefreshSlotsArgs& q = t[r];
-  refresh_slots_body(q.box, q.counts, q.atom_of_slot, q.pos, q.q, q.slot_pos, q.slot_pos64, blockIdx.x - pre[r]);
-}
-
 
 // periodic distance (fixed-point units) from a point at offset t = x - lo
 // (mod 2^32) to the interval [lo, lo + w]
@@ -701,7 +675,14 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
 
 // slot forces of the pair kernels into the per-atom fixed-point forces
 __device__ inline void fold_slots_body(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix, int bx) {
+                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix,
+                             int* __restrict__ flag, unsigned long long* __restrict__ builds, int bx) {
+  if (flag && bx == 0 && threadIdx.x == 0) {
+    // end of an evaluation that reuses lists (every reader of the flag ran
+    // before): count a build, clear the flag
+    if (*flag) builds[0] += 1ull;
+    *flag = 0;
+  }
   const int sl = bx * blockDim.x + threadIdx.x;
   if (sl >= counts[0]) return;
   const int at = atom_of_slot[sl];
@@ -711,12 +692,15 @@ __device__ inline void fold_slots_body(const int* __restrict__ counts, const int
   force_fix[3LL * at + 2] += force_slot[3LL * sl + 2];
 }
 __global__ void k_fold_slots(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix) { fold_slots_body(counts, atom_of_slot, force_slot, force_fix, blockIdx.x); }
-struct FoldSlotsArgs { const int* counts; const int* atom_of_slot; const long long* force_slot; long long* force_fix; };
+                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix,
+                             int* __restrict__ flag, unsigned long long* __restrict__ builds) {
+  fold_slots_body(counts, atom_of_slot, force_slot, force_fix, flag, builds, blockIdx.x);
+}
+struct FoldSlotsArgs { const int* counts; const int* atom_of_slot; const long long* force_slot; long long* force_fix; int* flag; unsigned long long* builds; };
 __global__ void kb_fold_slots(const FoldSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const FoldSlotsArgs& q = t[r];
-  fold_slots_body(q.counts, q.atom_of_slot, q.force_slot, q.force_fix, blockIdx.x - pre[r]);
+  fold_slots_body(q.counts, q.atom_of_slot, q.force_slot, q.force_fix, q.flag, q.builds, blockIdx.x - pre[r]);
 }
 
 
@@ -1500,45 +1484,22 @@ void pm_launch_set_flag(hipStream_t s, int* flag, int v) {
   hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, flag, v);
 }
 
-void pm_launch_end_build(hipStream_t s, int* flag, unsigned long long* builds) {
-  if (pm_recording()) {
-    // flag null: a replica without list reuse (nothing to do, the record keeps
-    // the sequences of all replicas aligned)
-    const EndBuildArgs a{flag, builds};
-    pm_record(PM_REC_END_BUILD, 0, flag ? 1 : 0, 64, 0, &a, sizeof(a));
-    return;
-  }
-  if (flag) hipLaunchKernelGGL(k_end_build, dim3(1), dim3(64), 0, s, flag, builds);
-}
-
 void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build) {
   if (pm_recording()) { pm_record(-1, 0, 0, 0, 0, nullptr, 0); return; }   // not batched
   if (n > 0) hipLaunchKernelGGL(k_invalidate_build, dim3((n + 255) / 256), dim3(256), 0, s, n, x_build);
 }
 
-void pm_launch_refresh_slots(hipStream_t s, const PmBox& box, int nslot_max, const int* counts,
-                             const int* atom_of_slot, const double* pos, const double* q, int4* slot_pos,
-                             longlong4* slot_pos64) {
-  if (pm_recording()) {
-    const RefreshSlotsArgs a{box, counts, atom_of_slot, pos, q, slot_pos, slot_pos64};
-    pm_record(PM_REC_REFRESH_SLOTS, 0, nslot_max > 0 ? (nslot_max + 255) / 256 : 0, 256, 0, &a, sizeof(a));
-    return;
-  }
-  if (nslot_max > 0)
-    hipLaunchKernelGGL(k_refresh_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, box, counts, atom_of_slot,
-                       pos, q, slot_pos, slot_pos64);
-}
-
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
-                          const long long* force_slot, long long* force_fix) {
+                          const long long* force_slot, long long* force_fix, int* flag,
+                          unsigned long long* builds) {
   if (pm_recording()) {
-    const FoldSlotsArgs a{counts, atom_of_slot, force_slot, force_fix};
+    const FoldSlotsArgs a{counts, atom_of_slot, force_slot, force_fix, flag, builds};
     pm_record(PM_REC_FOLD_SLOTS, 0, nslot_max > 0 ? (nslot_max + 255) / 256 : 0, 256, 0, &a, sizeof(a));
     return;
   }
   if (nslot_max > 0)
     hipLaunchKernelGGL(k_fold_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, counts, atom_of_slot,
-                       force_slot, force_fix);
+                       force_slot, force_fix, flag, builds);
 }
 
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m) {
@@ -1701,8 +1662,6 @@ bool pm_launch_batched_nb(hipStream_t s, int kind, int variant, const void* t, c
     case PM_REC_SCATTER: hipLaunchKernelGGL(kb_scatter, g, b, 0, s, (const ScatterArgs*)t, pre, nrep); return true;
     case PM_REC_FILL_SLOTS: hipLaunchKernelGGL(kb_fill_slots, g, b, 0, s, (const FillSlotsArgs*)t, pre, nrep); return true;
     case PM_REC_CHECK_DISP: hipLaunchKernelGGL(kb_check_disp, g, b, 0, s, (const CheckDispArgs*)t, pre, nrep); return true;
-    case PM_REC_END_BUILD: hipLaunchKernelGGL(kb_end_build, g, b, 0, s, (const EndBuildArgs*)t, pre, nrep); return true;
-    case PM_REC_REFRESH_SLOTS: hipLaunchKernelGGL(kb_refresh_slots, g, b, 0, s, (const RefreshSlotsArgs*)t, pre, nrep); return true;
     case PM_REC_FOLD_SLOTS: hipLaunchKernelGGL(kb_fold_slots, g, b, 0, s, (const FoldSlotsArgs*)t, pre, nrep); return true;
     case PM_REC_LIST:
       if (variant == 16) hipLaunchKernelGGL(kb_list<16>, g, b, lds, s, (const ListArgs*)t, pre, nrep);
