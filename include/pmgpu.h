@@ -252,8 +252,15 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        for one pair pass whose algorithm cutoff covers its
  *                        pair cutoff in a box wider than 2 (cutoff + skin +
  *                        cell diagonal); otherwise lists are rebuilt each time.
+ *   PM_OPT_PME_FFT       the y and x transforms of the PME grid: 0 (default)
+ *                        automatic (separate passes), 1 separate y and x
+ *                        passes over all lines, 2 one kernel per kz plane
+ *                        (y and x transforms and the influence function
+ *                        with the plane held in LDS; used where a plane of
+ *                        the grid fits, separate passes elsewhere: faster
+ *                        alone, slower beside the pair kernel).
  * pm_get_option returns the value in effect (after the automatic choice). */
-enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2 };
+enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

@@ -78,6 +78,7 @@ struct DevBuf {
 
 struct pm_context {
   int device = 0;
+  int lds_max = 65536;             // LDS bytes a workgroup may use on this device
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // PME, correction and bonded work, beside the pair work
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -191,11 +192,13 @@ struct pm_context {
 
   // captured MD steps (hipGraph), replayed while nothing they reference changes
   int graph_option = 1;              // PM_OPT_GRAPH
+  int pme_fft_option = 0;            // PM_OPT_PME_FFT
 
   // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
   // cutoff + skin and are rebuilt, with the cell layout, when an atom has
   // moved more than skin / 2 since the build (checked on the device)
   int skin_milli = -1;               // -1 automatic, 0 off, else 1e-3 A
+  bool disp_prechecked = false;      // the next evaluation's displacement test already queued
   int* d_rebuild = nullptr;          // device flag: 1 = rebuild in this evaluation
   int4* d_xbuild = nullptr;          // [n] fixed-point position at the last build, w = 1 when valid
   bool list_stale = true;            // x_build must be invalidated before the next reuse
@@ -536,6 +539,16 @@ void dftmod(int order, int n, const double* theta, double* mod) {
   }
 }
 
+// the fused kz-plane transform (PM_OPT_PME_FFT 2) where a plane fits in LDS.
+// The automatic choice is the separate passes: the plane kernel has one
+// block per kz plane (49 for a 96^3 grid), and beside the pair kernel on the
+// other stream those few blocks take several times longer than the passes'
+// hundreds.
+int pme_plane(const pm_context* c, const ForceDesc& f) {
+  if (c->pme_fft_option != 2) return 0;
+  return pm_pme_plane_fits(f.nx, f.ny, c->precision, c->lds_max);
+}
+
 bool fft_plan(int n, PmFftPlan& p) {
   p.n = n;
   p.nst = 0;
@@ -556,6 +569,8 @@ int ensure_pme(pm_context* c, ForceDesc& f) {
   const bool same_grid = c->pme_nx == nx && c->pme_ny == ny && c->pme_nz == nz && c->pme_prec == c->precision;
   if (!f.recip) return PM_OK;
   if (!same_grid) {
+    if (nx > 4096 || ny > 4096 || nz > 4096)
+      return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: at most 4096 points per axis", nx, ny, nz);
     if (!fft_plan(nx, c->plans[0]) || !fft_plan(ny, c->plans[1]) || !fft_plan(nz, c->plans[2]))
       return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: the GPU FFT handles lengths with factors 2, 3 and 5 only", nx, ny, nz);
     size_t ng = (size_t)nx * ny * nz, ngc = (size_t)nx * ny * nzc;
@@ -843,11 +858,49 @@ struct AdjointSpec {
   double* dq = nullptr;        // [n] mode 2 (PME, correction and self parts, FP64)
 };
 
+// List reuse applies to one pair pass whose algorithm cutoff covers its pair
+// cutoff (the reference's cell pairs then hold every pair within the cutoff,
+// so the pair set does not depend on the cells), in a box where the skin
+// keeps the minimal image unique; never for adjoint passes.
+bool list_reuse(const pm_context* c, const std::vector<PairPass>& passes) {
+  const double skin = list_skin(c);
+  if (!(skin > 0.0) || passes.size() != 1) return false;
+  const double pc = std::sqrt(pair_params(passes[0].terms, c->csf).cut2);
+  const double lmin_h = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
+  const double cd = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
+                              c->realcell[2] * c->realcell[2]);
+  return passes[0].algo_cutoff >= pc && lmin_h > pc + skin + cd + 1e-3;
+}
+
+// The displacement test for the MD forces' next evaluation, to be run by the
+// position update that precedes it; false (no test there) when the lists are
+// not reused or must first be invalidated.  On true the next evaluate_group
+// skips its own test.
+bool md_disp_check(pm_context* c, PmDispCheck* chk) {
+  if (!c->have_box || c->list_stale || !c->d_xbuild || !c->d_rebuild) return false;
+  if (!list_reuse(c, plan_passes(c, c->md_handles))) return false;
+  const double half = 0.5 * list_skin(c);
+  for (int k = 0; k < 3; k++) {
+    chk->to_fix[k] = c->box.to_fix[k];
+    chk->fix_to_A[k] = c->box.fix_to_A[k];
+  }
+  chk->x_build = c->d_xbuild;
+  chk->lim2 = (float)half * (float)half;
+  chk->flag = c->d_rebuild;
+  c->disp_prechecked = true;
+  return true;
+}
+
 int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bool want_energy = true,
                    const AdjointSpec* adj = nullptr) {
   hipStream_t s = c->stream;
   const int n = c->n;
   const bool vir = (flags & PM_EVAL_VIRIAL) != 0;
+  // the position update before this evaluation may have queued the
+  // displacement test already (md_disp_check); taken here, before any early
+  // return, so that it never carries over to another evaluation
+  const bool prechecked = c->disp_prechecked;
+  c->disp_prechecked = false;
   int r;
   for (int h : handles) {
     ForceDesc& f = c->forces[h];
@@ -876,14 +929,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   // the pair set does not depend on the cells), in a box where the skin
   // keeps the minimal image unique; never for adjoint passes
   const double skin = list_skin(c);
-  bool reuse = false;
-  if (skin > 0.0 && passes.size() == 1 && !adj) {
-    const double pc = std::sqrt(pair_params(passes[0].terms, c->csf).cut2);
-    const double lmin_h = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
-    const double cd = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
-                                c->realcell[2] * c->realcell[2]);
-    reuse = passes[0].algo_cutoff >= pc && lmin_h > pc + skin + cd + 1e-3;
-  }
+  const bool reuse = !adj && list_reuse(c, passes);
   const int* rb = reuse ? c->d_rebuild : nullptr;
   {
   TimedScope ts(c, PM_T_BIN);
@@ -894,7 +940,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   // (n = 0 without reuse: nothing runs; a recorded replica step keeps its
   // kernel sequence the same with and without reuse)
   const float half = (float)(0.5 * skin);
-  pm_launch_check_disp(s, c->box, reuse ? n : 0, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
+  pm_launch_check_disp(s, c->box, reuse && !prechecked ? n : 0, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
   pm_launch_zero_force_cells(s, rb, c->d_force, 3LL * n, c->d_force_slot, passes.empty() ? 0 : 3LL * c->nslot_max,
                              c->d_cell_count, nc);
   pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key, rb);
@@ -915,7 +961,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     ForceDesc& f = c->forces[h];
     if (!f.pme) continue;
     long long off = rec;
-    if (f.recip) rec += pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision);
+    if (f.recip) rec += pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision, pme_plane(c, f));
     if (f.corr) rec += ((long long)c->tab_a.size() + 255) / 256;
     pme_rec.push_back(std::make_pair(h, off));
   }
@@ -1002,6 +1048,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       PmPmeLaunch a{};
       a.n = n;
       a.nx = f.nx; a.ny = f.ny; a.nz = f.nz; a.nzc = f.nz / 2 + 1;
+      a.plane = pme_plane(c, f);
       a.order = f.order;
       for (int k = 0; k < 3; k++) {
         a.hr[k] = (k == 0 ? f.nx : k == 1 ? f.ny : f.nz) / c->box.L[k];
@@ -1110,23 +1157,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     HIPCHECK(c, hipEventRecord(c->ev_fork, s));
     HIPCHECK(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
   }
-  for (auto& pl : pme_launch) {
-    TimedScope ts(c, PM_T_PME, s2);
-    pm_launch_pme_plans(s2, pl.first, c->plans, c->d_slot_pos64, c->precision);
-  }
-  for (auto& hr : pme_rec) {
-    ForceDesc& f = c->forces[hr.first];
-    if (!f.corr) continue;
-    long long off = hr.second + (f.recip ? pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision) : 0);
-    int ntab = (int)c->tab_a.size();
-    int nblk = (ntab + 255) / 256;
-    TimedScope ts(c, PM_T_CORR, s2);
-    pm_launch_correction(s2, c->box, n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls, f.alpha,
-                         c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
-                         c->d_epart + off * PM_NE, want_energy ? nblk : 0);
-  }
+  PmBondedLaunch b{};
   if (do_bond || do_angle) {
-    PmBondedLaunch b{};
     b.n = n;
     b.pos = c->d_pos;
     b.b_row = c->d_b_row; b.b_idx = c->d_b_idx; b.a_row = c->d_a_row; b.a_idx = c->d_a_idx;
@@ -1140,9 +1172,48 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     b.epart = c->d_epart + bonded_off * PM_NE;
     b.want_virial = vir ? 1 : 0;
     b.want_energy = want_energy ? 1 : 0;
+  }
+  // with one PME reciprocal term, the gather kernel's thread for an atom
+  // also adds that atom's correction and bonded forces (two launches fewer)
+  PmAtomTail tail{};
+  const bool fuse = pme_launch.size() == 1;
+  if (fuse) {
+    const ForceDesc& f = c->forces[pme_launch[0].second];
+    tail.box = c->box;
+    if (f.corr) {
+      tail.corr = 1;
+      tail.pos = c->d_pos; tail.q = c->d_q;
+      tail.row = c->d_cx_row; tail.partner = c->d_cx_partner; tail.cls = c->d_cx_cls;
+      tail.alpha = f.alpha; tail.csf = c->csf;
+    }
+    if (do_bond || do_angle) {
+      tail.bonded = 1;
+      tail.b = b;
+    }
+  }
+  for (auto& pl : pme_launch) {
+    TimedScope ts(c, PM_T_PME, s2);
+    pm_launch_pme_plans(s2, pl.first, c->plans, c->d_slot_pos64, c->precision, fuse ? &tail : nullptr);
+  }
+  for (auto& hr : pme_rec) {
+    ForceDesc& f = c->forces[hr.first];
+    if (!f.corr) continue;
+    const bool in_tail = fuse && hr.first == pme_launch[0].second;
+    long long off = hr.second + (f.recip ? pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision, pme_plane(c, f)) : 0);
+    int ntab = (int)c->tab_a.size();
+    int nblk = (ntab + 255) / 256;
+    if (in_tail && !want_energy) continue;
+    TimedScope ts(c, PM_T_CORR, s2);
+    pm_launch_correction(s2, c->box, in_tail ? 0 : n, c->d_pos, c->d_q, c->d_cx_row, c->d_cx_partner, c->d_cx_cls,
+                         f.alpha, c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
+                         c->d_epart + off * PM_NE, want_energy ? nblk : 0);
+  }
+  if ((do_bond || do_angle) && !(fuse && !want_energy)) {
+    PmBondedLaunch bl = b;
+    if (fuse) bl.n = 0;   // the forces came with the gather: energies only
     int rb, ra;
     TimedScope ts(c, PM_T_BONDED, s2);
-    pm_launch_bonded(s2, c->box, b, &rb, &ra);
+    pm_launch_bonded(s2, c->box, bl, &rb, &ra);
   }
   if (side) HIPCHECK(c, hipEventRecord(c->ev_join, s2));
 
@@ -1259,6 +1330,11 @@ int pm_create(int device, pm_context** out) {
   if (device < 0 || device >= count) return PM_ERR_ARGUMENT;
   pm_context* c = new pm_context();
   c->device = device;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+      c->lds_max = v;
+  }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return PM_ERR_DEVICE;
@@ -1761,10 +1837,12 @@ int md_unit(pm_context* c, int kind, bool energy) {
   const int n = c->n;
   const double h = c->md_dt * kInvTimeFactor;
   int r;
+  PmDispCheck chk{};
   if (kind == 0) {
     {
       TimedScope ts(c, PM_T_INTEGRATE);
-      pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+      const bool ck = md_disp_check(c, &chk);
+      pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, ck ? &chk : nullptr);
     }
     return evaluate_group(c, c->md_handles, 0, energy);
   }
@@ -1772,7 +1850,8 @@ int md_unit(pm_context* c, int kind, bool energy) {
   {
     TimedScope ts(c, PM_T_INTEGRATE);
     pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 0);
-    pm_launch_drift(s, 3 * n, h, c->d_vel, c->d_pos);
+    const bool ck = md_disp_check(c, &chk);
+    pm_launch_drift(s, n, h, c->d_vel, c->d_pos, ck ? &chk : nullptr);
   }
   if ((r = evaluate_group(c, c->md_handles, 0, energy))) return r;
   TimedScope ts(c, PM_T_INTEGRATE);
@@ -1857,7 +1936,9 @@ int pm_md_run(pm_context* c, int nsteps) {
     // energies of the last force evaluation only
     {
       TimedScope ts(c, PM_T_INTEGRATE);
-      pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+      PmDispCheck chk{};
+      const bool ck = md_disp_check(c, &chk);
+      pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, ck ? &chk : nullptr);
     }
     if ((r = evaluate_group(c, c->md_handles, 0, nsteps == 1))) return r;
     if (nsteps >= 3 && (r = md_units(c, 0, nsteps - 2))) return r;
@@ -2265,6 +2346,13 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->md_forces_valid = false;
     return PM_OK;
   }
+  if (option == PM_OPT_PME_FFT) {
+    if (value < 0 || value > 2)
+      return fail(c, PM_ERR_ARGUMENT, "PME FFT option must be 0 (automatic), 1 (separate passes) or 2 (plane kernel; got %d)", value);
+    c->pme_fft_option = value;
+    c->md_forces_valid = false;
+    return PM_OK;
+  }
   return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);
 }
 
@@ -2280,6 +2368,12 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_LIST_SKIN) {
     *value = (int)std::lround(list_skin(c) * 1000.0);
+    return PM_OK;
+  }
+  if (option == PM_OPT_PME_FFT) {
+    *value = c->pme_fft_option == 2 ? 2 : 1;
+    if (c->pme_nx > 0 && *value == 2 && !pm_pme_plane_fits(c->pme_nx, c->pme_ny, c->precision, c->lds_max))
+      *value = 1;
     return PM_OK;
   }
   return fail(c, PM_ERR_ARGUMENT, "unknown option %d", option);

@@ -6,94 +6,17 @@
 // its own share), so no atomics are needed and the result is deterministic;
 // energies and virials are summed per term in fixed-order block reductions.
 #include "pm_internal.h"
+#include "pm_terms.h"
 
 #include <math.h>
 
 namespace {
 
-__device__ inline void min_diff(const PmBox& b, const double* c1, const double* c2, double* d) {
-#pragma clang fp contract(off)
-  double x = c2[0] - c1[0], y = c2[1] - c1[1], z = c2[2] - c1[2];
-  double s = x * x + y * y + z * z;
-  if (s > b.d) {
-    if (x < -b.h2[0]) x += b.h[0]; else if (x > b.h2[0]) x -= b.h[0];
-    if (y < -b.h2[1]) y += b.h[1]; else if (y > b.h2[1]) y -= b.h[1];
-    if (z < -b.h2[2]) z += b.h[2]; else if (z > b.h2[2]) z -= b.h[2];
-    s = x * x + y * y + z * z;
-    if (s > b.d) {
-      x -= b.L[0] * __builtin_rint(b.er[0] * x);
-      y -= b.L[1] * __builtin_rint(b.er[1] * y);
-      z -= b.L[2] * __builtin_rint(b.er[2] * z);
-    }
-  }
-  d[0] = x; d[1] = y; d[2] = z;
-}
-
-__device__ inline double norm3(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
-
-// one bond: force on atom1 (atom2 gets the opposite), energy, r12
-__device__ inline void bond_term(const PmBox& b, const PmBondedLaunch& a, int k, double* f1, double& e,
-                                 double* r12) {
-  const int a1 = a.b_a[k], a2 = a.b_b[k];
-  const double r0 = a.b_r0[k], kb = a.b_k[k];
-  min_diff(b, a.pos + 3 * a2, a.pos + 3 * a1, r12);   // vector from atom 2 to atom 1
-  const double r = norm3(r12);
-  const double dpotdr = 2.0 * kb * (r - r0);
-  const double s = -dpotdr / r;
-  f1[0] = r12[0] * s; f1[1] = r12[1] * s; f1[2] = r12[2] * s;
-  e = kb * (r - r0) * (r - r0);
-}
-
-// one angle (a2 is the vertex): forces on the three atoms, energy, r12/r32
-__device__ inline void angle_term(const PmBox& b, const PmBondedLaunch& a, int k, double* f1, double* f2,
-                                  double* f3, double& e, double* r12, double* r32) {
-  const int a1 = a.a_a[k], a2 = a.a_b[k], a3 = a.a_c[k];
-  const double t0 = a.a_t0[k], kt = a.a_k[k], kub = a.a_kub[k], rub = a.a_rub[k];
-  double r13[3];
-  min_diff(b, a.pos + 3 * a2, a.pos + 3 * a1, r12);
-  min_diff(b, a.pos + 3 * a2, a.pos + 3 * a3, r32);
-  min_diff(b, a.pos + 3 * a3, a.pos + 3 * a1, r13);
-  const double d12 = norm3(r12), d32 = norm3(r32), d13 = norm3(r13);
-  double cr[3] = {r12[1] * r32[2] - r12[2] * r32[1], r12[2] * r32[0] - r12[0] * r32[2],
-                  r12[0] * r32[1] - r12[1] * r32[0]};
-  const double theta = atan2(norm3(cr), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
-  const double st = sin(theta), ct = cos(theta);
-  const double dpot = 2.0 * kt * (theta - t0);
-  double u1[3], u3[3], ub1[3];
-  for (int c = 0; c < 3; c++) {
-    const double dr12 = r12[c] / d12, dr32 = r32[c] / d32, dr13 = r13[c] / d13;
-    const double dth1 = (dr12 * ct - dr32) / (st * d12);
-    const double dth3 = (dr32 * ct - dr12) / (st * d32);
-    ub1[c] = dr13 * (2.0 * kub * (d13 - rub));
-    u1[c] = -dth1 * dpot - ub1[c];
-    u3[c] = -dth3 * dpot + ub1[c];
-  }
-  for (int c = 0; c < 3; c++) { f1[c] = u1[c]; f3[c] = u3[c]; f2[c] = -u1[c] - u3[c]; }
-  e = kt * (theta - t0) * (theta - t0) + kub * (d13 - rub) * (d13 - rub);
-}
-
 __device__ inline void bonded_force_body(const PmBox& box, const PmBondedLaunch& a, int bx) {
   const int i = bx * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  double f[3] = {0, 0, 0};
-  if (a.do_bond) {
-    for (int u = a.b_row[i]; u < a.b_row[i + 1]; u++) {
-      const int k = a.b_idx[u];
-      double f1[3], e, r12[3];
-      bond_term(box, a, k, f1, e, r12);
-      const double sg = (a.b_a[k] == i) ? 1.0 : -1.0;
-      f[0] += sg * f1[0]; f[1] += sg * f1[1]; f[2] += sg * f1[2];
-    }
-  }
-  if (a.do_angle) {
-    for (int u = a.a_row[i]; u < a.a_row[i + 1]; u++) {
-      const int k = a.a_idx[u];
-      double f1[3], f2[3], f3[3], e, r12[3], r32[3];
-      angle_term(box, a, k, f1, f2, f3, e, r12, r32);
-      const double* g = (a.a_a[k] == i) ? f1 : (a.a_b[k] == i) ? f2 : f3;
-      f[0] += g[0]; f[1] += g[1]; f[2] += g[2];
-    }
-  }
+  double f[3];
+  bonded_atom_force(box, a, i, f);
   a.force_fix[3 * i] += __double2ll_rn(f[0] * PM_FORCE_FIX);
   a.force_fix[3 * i + 1] += __double2ll_rn(f[1] * PM_FORCE_FIX);
   a.force_fix[3 * i + 2] += __double2ll_rn(f[2] * PM_FORCE_FIX);

@@ -184,7 +184,10 @@ struct PmPmeLaunch {
   int nslot_max;
   unsigned long long* grid_fix;  // [nx*ny*nz] int64 charge grid (cleared by the z pass)
   void* grid_r;              // real grid (float or double) [nx*ny*nz]
-  void* grid_c;              // complex half grid [nx*ny*nzc] (float2 / double2)
+  void* grid_c;              // complex half grid, nx*ny*nzc points (float2 / double2):
+                             // [x][y][kz], or [kz][x][y] when plane is set
+  int plane;                 // 1: y and x transforms and the influence function in one
+                             // kernel per kz plane (the plane fits in LDS)
   const double* bmod;        // [nx + ny + nz] B(m) = 1/|b(m)|^2 (Grid::dftmod)
   const double* expf_tab;    // [nx + ny + nzc] exp(-pi^2 (m/L)^2 / alpha^2)
   double inv_pi_v;           // 1/(pi V)
@@ -192,7 +195,7 @@ struct PmPmeLaunch {
   double Lr[3];              // 1/L
   const double* tw;          // twiddles exp(-2 pi i m/n), interleaved re, im
   long long* force_fix;
-  double* epart;             // [x-pass blocks * PM_NE]
+  double* epart;             // [pm_pme_xpass_blocks() * PM_NE]
   int want_virial;
   // spread by cell bricks staged in LDS tiles
   const int* cell_slot;
@@ -212,9 +215,16 @@ struct PmFftPlan {
   int rad[PM_FFT_MAXST];
   int tw;  // offset of this length's twiddles in the table (complex entries)
 };
+struct PmAtomTail;
+// spread, transforms and gather; tail (may be null) adds the correction and
+// bonded forces of each atom in the gather kernel's thread for that atom
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
-                         const longlong4* pos64, int precision);
-int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision);
+                         const longlong4* pos64, int precision, const PmAtomTail* tail);
+// energy records written by the reciprocal term (one per x-pass block, or
+// one per kz plane)
+int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision, int plane);
+// 1 when a kz plane of an nx x ny grid fits the plane kernel's LDS
+int pm_pme_plane_fits(int nx, int ny, int precision, int lds_max);
 // PME adjoint: mode 0 out[n] += dE/dq (reciprocal potential), mode 1
 // out[3n] += (H v) of the reciprocal energy; slot_v = v in slot order
 bool pm_launch_pme_adjoint(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
@@ -250,8 +260,16 @@ void pm_launch_zero_force_cells(hipStream_t s, const int* rebuild, long long* f,
                                 long long nfs, int* cc, int nc);
 void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out);
+// the list-reuse displacement test run by a position update on the positions
+// it writes (pm_launch_check_disp's test; flag null: no test)
+struct PmDispCheck {
+  double to_fix[3], fix_to_A[3];
+  const int4* x_build;
+  float lim2;
+  int* flag;
+};
 void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
-                    double* v, double* x);
+                    double* v, double* x, const PmDispCheck* chk = nullptr);
 // MD state the integrator kernels read from device memory
 struct PmMdDevice {
   unsigned long long counter;   // Langevin noise counter (2 per step)
@@ -262,7 +280,8 @@ void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_
                              int offset);
 void pm_launch_md_advance(hipStream_t s, PmMdDevice* md);
 void pm_launch_scale(hipStream_t s, int n3, double factor, double* v);
-void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x);
+// x += h v for n atoms
+void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmDispCheck* chk = nullptr);
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation
 struct PmBondedLaunch {
@@ -282,6 +301,20 @@ struct PmBondedLaunch {
   int want_energy;      // write the energy records (skipped on steps that report none)
 };
 void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, int* nrec_bond, int* nrec_angle);
+
+// Per-atom force terms that the PME gather kernel adds for its atoms, each
+// rounded to fixed point on its own as the separate kernels round it (the
+// integer sum is then the same as theirs).
+struct PmAtomTail {
+  PmBox box;
+  int corr;                    // PME excluded-pair correction
+  const double* pos;
+  const double* q;
+  const int* row; const int* partner; const int* cls;
+  double alpha, csf;
+  int bonded;                  // bond and angle terms (b.do_bond / b.do_angle)
+  PmBondedLaunch b;
+};
 
 // ------------------------------------------------- batched launch records
 // Replica batches run one launch per kernel over every replica.  While a

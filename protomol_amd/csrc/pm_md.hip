@@ -2,67 +2,22 @@
 // correction, energy reductions, fixed-point conversions and the leapfrog /
 // Langevin position-velocity updates.
 #include "pm_internal.h"
+#include "pm_terms.h"
+#include "pm_disp.h"
 
 #include <math.h>
 
 namespace {
 
-// PeriodicBoundaryConditions::minimalDifference (topology/PeriodicBoundaryConditions.h:76-117)
-__device__ inline void min_diff(const PmBox& b, const double* c1, const double* c2, double* d, double& s) {
-#pragma clang fp contract(off)
-  double x = c2[0] - c1[0], y = c2[1] - c1[1], z = c2[2] - c1[2];
-  s = x * x + y * y + z * z;
-  if (s > b.d) {
-    if (x < -b.h2[0]) x += b.h[0]; else if (x > b.h2[0]) x -= b.h[0];
-    if (y < -b.h2[1]) y += b.h[1]; else if (y > b.h2[1]) y -= b.h[1];
-    if (z < -b.h2[2]) z += b.h[2]; else if (z > b.h2[2]) z -= b.h[2];
-    s = x * x + y * y + z * z;
-    if (s > b.d) {
-      x -= b.L[0] * __builtin_rint(b.er[0] * x);
-      y -= b.L[1] * __builtin_rint(b.er[1] * y);
-      z -= b.L[2] * __builtin_rint(b.er[2] * z);
-      s = x * x + y * y + z * z;
-    }
-  }
-  d[0] = x; d[1] = y; d[2] = z;
-}
-
-// NonbondedPMEwaldSystemForce::correctionTerm (forces/NonbondedPMEwaldSystemForce.h:942-981)
-// for one exclusion pair a1 < a2: fij (applied -= to a1, += to a2) and energy.
-__device__ inline void corr_pair(const PmBox& b, const double* pos, const double* q, int a1, int a2,
-                                 int cls, double alpha, double csf, double* rij, double* fij, double& e) {
-  double r2;
-  min_diff(b, pos + 3 * a1, pos + 3 * a2, rij, r2);
-  double qq = q[a1] * q[a2];
-  if (cls == PM_EXCL_MODIFIED) qq *= 1 - csf;
-  const double r = sqrt(r2);
-  const double rr = 1 / r;
-  const double ee = erf(alpha * r) * rr;
-  e = -qq * ee;
-  const double tapi = 2.0 * alpha / sqrt(3.14159265358979323846);
-  const double s = (qq * (tapi * exp(-alpha * alpha * r2) - ee) * rr * rr);
-  fij[0] = rij[0] * s; fij[1] = rij[1] * s; fij[2] = rij[2] * s;
-}
-
-// per-atom: sum over the atom's exclusion partners (CSR, partner encoded as
-// 2*atom + (1 if this atom is the pair's second atom))
+// per-atom: the correction force over the atom's exclusion partners
 __global__ void k_corr_force(PmBox box, int n, const double* __restrict__ pos, const double* __restrict__ q,
                              const int* __restrict__ row, const int* __restrict__ partner,
                              const int* __restrict__ cls, double alpha, double csf,
                              long long* __restrict__ force_fix) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double f[3] = {0, 0, 0};
-  for (int u = row[i]; u < row[i + 1]; u++) {
-    const int pe = partner[u];
-    const int other = pe >> 1;
-    const bool second = pe & 1;
-    const int a1 = second ? other : i, a2 = second ? i : other;
-    double rij[3], fij[3], e;
-    corr_pair(box, pos, q, a1, a2, cls[u], alpha, csf, rij, fij, e);
-    if (second) { f[0] += fij[0]; f[1] += fij[1]; f[2] += fij[2]; }
-    else { f[0] -= fij[0]; f[1] -= fij[1]; f[2] -= fij[2]; }
-  }
+  double f[3];
+  corr_atom_force(box, i, pos, q, row, partner, cls, alpha, csf, f);
   force_fix[3 * i] += __double2ll_rn(f[0] * PM_FORCE_FIX);
   force_fix[3 * i + 1] += __double2ll_rn(f[1] * PM_FORCE_FIX);
   force_fix[3 * i + 2] += __double2ll_rn(f[2] * PM_FORCE_FIX);
@@ -261,25 +216,30 @@ __global__ void k_add_fix(int n3, const long long* __restrict__ f, double* __res
 // (integrators/LeapfrogIntegrator.cpp:46-104) and StandardIntegrator::doHalfKick
 // (StandardIntegrator.cpp:83-96).  mode 0: v += F*h*0.5/m then x += v*h;
 // mode 1: v += F*h/m then x += v*h; mode 2: v += F*hh/m (hh = 0.5*dt*INV_TIMEFACTOR).
-struct KickArgs { int n; int mode; double h; const double* mass; const long long* f; double* v; double* x; };
-__device__ inline void kick_drift_body(int n, int mode, double h, const double* __restrict__ mass,
-                                       const long long* __restrict__ f, double* __restrict__ v,
-                                       double* __restrict__ x, int bx) {
+struct KickArgs { int n; int mode; double h; const double* mass; const long long* f; double* v; double* x;
+                 PmDispCheck chk; };
+__device__ inline void kick_drift_body(const KickArgs& a, int bx) {
 #pragma clang fp contract(off)
   int i = bx * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double m = mass[i];
+  if (i >= a.n) return;
+  const double m = a.mass[i];
+  double xn[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const double F = (double)f[3 * i + k] * PM_FORCE_UNFIX;
-    double vv = v[3 * i + k];
-    if (mode == 0) vv += F * h * 0.5 / m;
-    else vv += F * h / m;
-    v[3 * i + k] = vv;
-    if (mode != 2) x[3 * i + k] += vv * h;
+    const double F = (double)a.f[3 * i + k] * PM_FORCE_UNFIX;
+    double vv = a.v[3 * i + k];
+    if (a.mode == 0) vv += F * a.h * 0.5 / m;
+    else vv += F * a.h / m;
+    a.v[3 * i + k] = vv;
+    xn[k] = a.x[3 * i + k];
+    if (a.mode != 2) {
+      xn[k] += vv * a.h;
+      a.x[3 * i + k] = xn[k];
+    }
   }
+  if (a.chk.flag && pm_moved_past(a.chk.to_fix, a.chk.fix_to_A, xn, a.chk.x_build[i], a.chk.lim2)) *a.chk.flag = 1;
 }
-__global__ void k_kick_drift(KickArgs a) { kick_drift_body(a.n, a.mode, a.h, a.mass, a.f, a.v, a.x, blockIdx.x); }
+__global__ void k_kick_drift(KickArgs a) { kick_drift_body(a, blockIdx.x); }
 
 // counter-based normal deviates (SplitMix64 hash + Box-Muller)
 __device__ inline unsigned long long mix64(unsigned long long z) {
@@ -341,18 +301,25 @@ __device__ inline void md_advance_body(PmMdDevice* md, int bx) {
 }
 __global__ void k_md_advance(AdvanceArgs a) { md_advance_body(a.md, blockIdx.x); }
 
-struct DriftArgs { int n3; double h; const double* v; double* x; };
-__device__ inline void drift_body(int n3, double h, const double* __restrict__ v, double* __restrict__ x, int bx) {
+struct DriftArgs { int n; double h; const double* v; double* x; PmDispCheck chk; };
+__device__ inline void drift_body(const DriftArgs& a, int bx) {
   int i = bx * blockDim.x + threadIdx.x;
-  if (i < n3) x[i] += h * v[i];   // Vector3DBlock::intoWeightedAdd
+  if (i >= a.n) return;
+  double xn[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    xn[k] = a.x[3 * i + k] + a.h * a.v[3 * i + k];   // Vector3DBlock::intoWeightedAdd
+    a.x[3 * i + k] = xn[k];
+  }
+  if (a.chk.flag && pm_moved_past(a.chk.to_fix, a.chk.fix_to_A, xn, a.chk.x_build[i], a.chk.lim2)) *a.chk.flag = 1;
 }
-__global__ void k_drift(DriftArgs a) { drift_body(a.n3, a.h, a.v, a.x, blockIdx.x); }
+__global__ void k_drift(DriftArgs a) { drift_body(a, blockIdx.x); }
 
 // batched launches over replicas (argument tables in device memory)
 __global__ void kb_kick_drift(const KickArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const KickArgs& a = t[r];
-  kick_drift_body(a.n, a.mode, a.h, a.mass, a.f, a.v, a.x, blockIdx.x - pre[r]);
+  kick_drift_body(a, blockIdx.x - pre[r]);
 }
 __global__ void kb_langevin_kick(const LangevinArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
@@ -362,7 +329,7 @@ __global__ void kb_langevin_kick(const LangevinArgs* __restrict__ t, const int* 
 __global__ void kb_drift(const DriftArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const DriftArgs& a = t[r];
-  drift_body(a.n3, a.h, a.v, a.x, blockIdx.x - pre[r]);
+  drift_body(a, blockIdx.x - pre[r]);
 }
 __global__ void kb_md_advance(const AdvanceArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
@@ -450,8 +417,8 @@ void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out) {
   if (n3 > 0) hipLaunchKernelGGL(k_add_fix, grid_for(n3, 256), dim3(256), 0, s, n3, f, out);
 }
 void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
-                    double* v, double* x) {
-  const KickArgs a{n, mode, h, mass, f, v, x};
+                    double* v, double* x, const PmDispCheck* chk) {
+  const KickArgs a{n, mode, h, mass, f, v, x, chk ? *chk : PmDispCheck{}};
   if (pm_recording()) {
     pm_record(PM_REC_KICK_DRIFT, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
     return;
@@ -479,13 +446,13 @@ void pm_launch_md_advance(hipStream_t s, PmMdDevice* md) {
   }
   hipLaunchKernelGGL(k_md_advance, dim3(1), dim3(64), 0, s, a);
 }
-void pm_launch_drift(hipStream_t s, int n3, double h, const double* v, double* x) {
-  const DriftArgs a{n3, h, v, x};
+void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmDispCheck* chk) {
+  const DriftArgs a{n, h, v, x, chk ? *chk : PmDispCheck{}};
   if (pm_recording()) {
-    pm_record(PM_REC_DRIFT, 0, n3 > 0 ? grid_for(n3, 256).x : 0, 256, 0, &a, sizeof(a));
+    pm_record(PM_REC_DRIFT, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
     return;
   }
-  if (n3 > 0) hipLaunchKernelGGL(k_drift, grid_for(n3, 256), dim3(256), 0, s, a);
+  if (n > 0) hipLaunchKernelGGL(k_drift, grid_for(n, 256), dim3(256), 0, s, a);
 }
 
 bool pm_launch_batched_md(hipStream_t s, int kind, int variant, const void* t, const int* pre, int nrep,

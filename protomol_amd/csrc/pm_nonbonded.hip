@@ -7,6 +7,7 @@
 // every unordered pair within the switching cutoff and add both atoms' forces
 // as 64-bit fixed-point integers (order-independent, deterministic run to run).
 #include "pm_internal.h"
+#include "pm_disp.h"
 
 #include <math.h>
 #include <type_traits>
@@ -317,15 +318,7 @@ __device__ inline void check_disp_body(PmBox box, int n, const double* __restric
                              float lim2, int* __restrict__ flag, int bx) {
   const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int4 b = x_build[i];
-  bool need = b.w != 1;
-  if (!need) {
-    const float dx = (float)(int)(to_fix(pos[3 * i], box.to_fix[0]) - (unsigned int)b.x) * (float)box.fix_to_A[0];
-    const float dy = (float)(int)(to_fix(pos[3 * i + 1], box.to_fix[1]) - (unsigned int)b.y) * (float)box.fix_to_A[1];
-    const float dz = (float)(int)(to_fix(pos[3 * i + 2], box.to_fix[2]) - (unsigned int)b.z) * (float)box.fix_to_A[2];
-    need = !(dx * dx + dy * dy + dz * dz <= lim2);   // NaN positions rebuild too
-  }
-  if (need) *flag = 1;
+  if (pm_moved_past(box.to_fix, box.fix_to_A, pos + 3 * i, x_build[i], lim2)) *flag = 1;
 }
 __global__ void k_check_disp(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
                              float lim2, int* __restrict__ flag) { check_disp_body(box, n, pos, x_build, lim2, flag, blockIdx.x); }

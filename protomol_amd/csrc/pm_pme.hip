@@ -14,6 +14,7 @@
 // The 1D transforms are Stockham autosort mixed-radix (2, 3, 4, 5) passes
 // over lines staged in LDS.
 #include "pm_internal.h"
+#include "pm_terms.h"
 
 #include <math.h>
 #include <type_traits>
@@ -83,19 +84,44 @@ template <typename T, typename T2> struct Dft<T, T2, 5> {
   }
 };
 
-// one Stockham pass of radix R over nl lines of length n held in LDS
-template <typename T, typename T2, int R>
-__device__ inline void fft_stage(const T2* __restrict__ in, T2* __restrict__ out, int nl, int n, int Ns,
-                                 const T2* __restrict__ tw, int sign) {
+// Integer division by a block-uniform d >= 1 through a multiply-high:
+// floor(v / d) = umulhi(v, ceil(2^32 / d)), exact while v * d <= 2^32
+// (the host keeps FFT lengths and line counts within 4096, so every index
+// here stays far below that bound).
+struct UDiv {
+  unsigned d, m;
+};
+__device__ inline UDiv udiv_make(int d) {
+  UDiv u;
+  u.d = (unsigned)d;
+  u.m = d > 1 ? 0xFFFFFFFFu / (unsigned)d + 1u : 0u;
+  return u;
+}
+__device__ inline int udiv(int v, const UDiv& u) { return u.d > 1 ? (int)__umulhi((unsigned)v, u.m) : v; }
+
+// one Stockham pass of radix R over nl lines of length n held in LDS:
+// element m of line l sits at buf[l * ls + m * es].  LF = true gives
+// consecutive threads consecutive lines (the butterfly index outermost),
+// false consecutive butterflies of one line.
+template <typename T, typename T2, int R, bool LF>
+__device__ inline void fft_stage(const T2* __restrict__ in, T2* __restrict__ out, int nl, int n, int ls, int es,
+                                 int Ns, const T2* __restrict__ tw, int sign) {
   const int nR = n / R;
   const int tws = n / (Ns * R);
-  for (int t = threadIdx.x; t < nl * nR; t += blockDim.x) {
-    const int l = t / nR;
-    const int j = t - l * nR;
-    const int k = j % Ns;
+  const UDiv dt = udiv_make(LF ? nl : nR);
+  const UDiv dn = udiv_make(Ns);
+  const int total = nl * nR;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int q = udiv(t, dt);
+    int l, j;
+    if (LF) { j = q; l = t - q * nl; } else { l = q; j = t - q * nR; }
+    const int jq = udiv(j, dn);
+    const int k = j - jq * Ns;
+    const T2* ip = in + l * ls + j * es;
+    T2* op = out + l * ls + (jq * Ns * R + k) * es;
     T2 v[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) v[r] = in[l * n + j + r * nR];
+    for (int r = 0; r < R; r++) v[r] = ip[r * nR * es];
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; r++) {
@@ -105,18 +131,17 @@ __device__ inline void fft_stage(const T2* __restrict__ in, T2* __restrict__ out
       }
     }
     Dft<T, T2, R>::run(v, sign);
-    const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) out[l * n + idxD + r * Ns] = v[r];
+    for (int r = 0; r < R; r++) op[r * Ns * es] = v[r];
   }
 }
 
-// Stockham passes over `nl` lines of length p.n held in LDS (line l at
-// buf[l * n]); all threads of the block take part.  tw points at the
-// n twiddles exp(-2 pi i m / n) already staged in LDS.  Returns the buffer
-// that holds the result.
-template <typename T, typename T2>
-__device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const T2* tw, int sign) {
+// Stockham passes over `nl` lines of length p.n held in LDS (element m of
+// line l at buf[l * ls + m * es]); all threads of the block take part.  tw
+// points at the n twiddles exp(-2 pi i m / n) already staged in LDS.
+// Returns the buffer that holds the result.
+template <typename T, typename T2, bool LF = false>
+__device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const T2* tw, int sign, int ls, int es = 1) {
   const int n = p.n;
   T2* in = b0;
   T2* out = b1;
@@ -124,10 +149,10 @@ __device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const T2* t
   for (int st = 0; st < p.nst; st++) {
     const int R = p.rad[st];
     switch (R) {
-      case 2: fft_stage<T, T2, 2>(in, out, nl, n, Ns, tw, sign); break;
-      case 3: fft_stage<T, T2, 3>(in, out, nl, n, Ns, tw, sign); break;
-      case 4: fft_stage<T, T2, 4>(in, out, nl, n, Ns, tw, sign); break;
-      default: fft_stage<T, T2, 5>(in, out, nl, n, Ns, tw, sign); break;
+      case 2: fft_stage<T, T2, 2, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
+      case 3: fft_stage<T, T2, 3, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
+      case 4: fft_stage<T, T2, 4, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
+      default: fft_stage<T, T2, 5, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
     }
     __syncthreads();
     T2* tmp = in; in = out; out = tmp;
@@ -346,8 +371,9 @@ __global__ __launch_bounds__(256) void k_spread_tile(PmPmeLaunch a, const longlo
   }
 }
 
-template <typename T, int P>
-__global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
+// TAIL: the same thread also adds its atom's correction and bonded forces
+template <typename T, int P, bool TAIL>
+__global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* __restrict__ pos64, PmAtomTail tl) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.counts[0]) return;
   const int at = a.atom_of_slot[s];
@@ -393,9 +419,23 @@ __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* 
     }
   }
   const double q = a.q[at];
-  a.force_fix[3 * at] += __double2ll_rn((double)fx * a.hr[0] * q * PM_FORCE_FIX);
-  a.force_fix[3 * at + 1] += __double2ll_rn((double)fy * a.hr[1] * q * PM_FORCE_FIX);
-  a.force_fix[3 * at + 2] += __double2ll_rn((double)fz * a.hr[2] * q * PM_FORCE_FIX);
+  long long F[3] = {__double2ll_rn((double)fx * a.hr[0] * q * PM_FORCE_FIX),
+                    __double2ll_rn((double)fy * a.hr[1] * q * PM_FORCE_FIX),
+                    __double2ll_rn((double)fz * a.hr[2] * q * PM_FORCE_FIX)};
+  if (TAIL) {
+    double f[3];
+    if (tl.corr) {
+      corr_atom_force(tl.box, at, tl.pos, tl.q, tl.row, tl.partner, tl.cls, tl.alpha, tl.csf, f);
+      for (int k = 0; k < 3; k++) F[k] += __double2ll_rn(f[k] * PM_FORCE_FIX);
+    }
+    if (tl.bonded) {
+      bonded_atom_force(tl.box, tl.b, at, f);
+      for (int k = 0; k < 3; k++) F[k] += __double2ll_rn(f[k] * PM_FORCE_FIX);
+    }
+  }
+  a.force_fix[3 * at] += F[0];
+  a.force_fix[3 * at + 1] += F[1];
+  a.force_fix[3 * at + 2] += F[2];
 }
 
 // ---------------------------------------------------------- PME adjoint
@@ -564,17 +604,23 @@ __global__ __launch_bounds__(256) void k_gather_adj(PmPmeLaunch a, const longlon
 }
 
 // ------------------------------------------------------------- FFT passes
+// The complex half grid is stored line-major, [x][y][kz], for the separate
+// y and x passes, or plane-major, [kz][x][y], when one block transforms a
+// whole kz plane (a.plane).  The z passes pad their LDS lines to n + 1
+// points, so that the transposed plane-major reads and writes of
+// consecutive lines fall in distinct LDS banks.
+
 // z forward: fixed-point grid -> real -> complex half lines; clears the
 // fixed-point grid for the next spread.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, int TL) {
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
+  const int n = p.n, ls = n + 1, nl_tot = a.nx * a.ny, nzc = a.nzc;
   T2* b0 = reinterpret_cast<T2*>(s_raw);
-  T2* b1 = b0 + TL * p.n;
-  T2* tws = b1 + TL * p.n;
+  T2* b1 = b0 + TL * ls;
+  T2* tws = b1 + TL * ls;
   load_twiddles(tws, p, a.tw);
-  const int n = p.n, nl_tot = a.nx * a.ny;
   const int line0 = blockIdx.x * TL;
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int l = e / n, k = e - l * n;
@@ -585,113 +631,70 @@ __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, i
       v.x = (T)((double)(long long)(*gp) * PM_GRID_UNFIX);
       *gp = 0ull;
     }
-    b0[e] = v;
+    b0[l * ls + k] = v;
   }
   __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1);
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1, ls);
   T2* gc = reinterpret_cast<T2*>(a.grid_c);
-  for (int e = threadIdx.x; e < TL * a.nzc; e += blockDim.x) {
-    const int l = e / a.nzc, k = e - l * a.nzc;
-    const int line = line0 + l;
-    if (line < nl_tot) gc[(long long)line * a.nzc + k] = r[l * n + k];
-  }
-}
-
-// y pass (either direction): lines (x, kz) with TL consecutive kz per block
-template <typename T>
-__global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int TL, int sign) {
-  typedef typename Cx<T>::t T2;
-  extern __shared__ unsigned char s_raw[];
-  T2* b0 = reinterpret_cast<T2*>(s_raw);
-  T2* b1 = b0 + TL * p.n;
-  T2* tws = b1 + TL * p.n;
-  load_twiddles(tws, p, a.tw);
-  const int n = p.n;  // ny
-  const int ntile = (a.nzc + TL - 1) / TL;
-  const int x = blockIdx.x / ntile;
-  const int kz0 = (blockIdx.x - x * ntile) * TL;
-  T2* gc = reinterpret_cast<T2*>(a.grid_c);
-  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
-    const int y = e / TL, l = e - y * TL;
-    T2 v; v.x = 0; v.y = 0;
-    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
-    b0[l * n + y] = v;
-  }
-  __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, sign);
-  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
-    const int y = e / TL, l = e - y * TL;
-    if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r[l * n + y];
-  }
-}
-
-// x pass with the influence function: forward transform, multiply by
-// theta(m) (Grid::scalarSum, base/Grid.h:276-389), inverse transform.
-template <typename T>
-__global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, int TL) {
-  typedef typename Cx<T>::t T2;
-  extern __shared__ unsigned char s_raw[];
-  T2* b0 = reinterpret_cast<T2*>(s_raw);
-  T2* b1 = b0 + TL * p.n;
-  T2* tws = b1 + TL * p.n;
-  load_twiddles(tws, p, a.tw);
-  __shared__ double s_red[8][8];
-  const int n = p.n;  // nx
-  const int ntile = (a.nzc + TL - 1) / TL;
-  const int y = blockIdx.x / ntile;
-  const int kz0 = (blockIdx.x - y * ntile) * TL;
-  T2* gc = reinterpret_cast<T2*>(a.grid_c);
-  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
-    const int x = e / TL, l = e - x * TL;
-    T2 v; v.x = 0; v.y = 0;
-    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
-    b0[l * n + x] = v;
-  }
-  __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1);
-  // convolution
-  double en = 0, vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
-  const int j0 = y <= a.ny / 2 ? y : y - a.ny;
-  const double mj = j0 * a.Lr[1];
-  const double* bmx = a.bmod;
-  const double* bmy = a.bmod + a.nx;
-  const double* bmz = a.bmod + a.nx + a.ny;
-  const double* ex = a.expf_tab;
-  const double* ey = a.expf_tab + a.nx;
-  const double* ez = a.expf_tab + a.nx + a.ny;
-  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
-    const int l = e / n, kx = e - l * n;
-    const int kz = kz0 + l;
-    if (kz >= a.nzc) continue;
-    T2 v = r[e];
-    T theta = 0;
-    if (kx != 0 || y != 0 || kz != 0) {
-      const int i0 = kx <= a.nx / 2 ? kx : kx - a.nx;
-      const double mi = i0 * a.Lr[0];
-      const double mk = kz * a.Lr[2];
-      const double m2 = mi * mi + mj * mj + mk * mk;
-      const double th = bmx[kx] * bmy[y] * bmz[kz] * (ex[kx] * a.inv_pi_v) * ey[y] * ez[kz] / m2;
-      theta = (T)th;
-      const double w = (kz == 0 || ((a.nz & 1) == 0 && kz == a.nz / 2)) ? 1.0 : 2.0;
-      const double q2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-      const double eg = w * q2 * th;
-      en += eg;
-      if (a.want_virial) {
-        const double vv = 2.0 * (1.0 / m2 + a.fac);
-        vxx += eg * (1.0 - vv * mi * mi);
-        vxy -= eg * (vv * mi * mj);
-        vxz -= eg * (vv * mi * mk);
-        vyy += eg * (1.0 - vv * mj * mj);
-        vyz -= eg * (vv * mj * mk);
-        vzz += eg * (1.0 - vv * mk * mk);
-      }
+  if (a.plane) {
+    for (int e = threadIdx.x; e < TL * nzc; e += blockDim.x) {
+      const int k = e / TL, l = e - k * TL;
+      const int line = line0 + l;
+      if (line < nl_tot) gc[(long long)k * nl_tot + line] = r[l * ls + k];
     }
-    v.x *= theta;
-    v.y *= theta;
-    r[e] = v;
+  } else {
+    for (int e = threadIdx.x; e < TL * nzc; e += blockDim.x) {
+      const int l = e / nzc, k = e - l * nzc;
+      const int line = line0 + l;
+      if (line < nl_tot) gc[(long long)line * nzc + k] = r[l * ls + k];
+    }
   }
-  // block reduction of the energy/virial partials (fixed order)
-  double part[7] = {en, vxx, vxy, vxz, vyy, vyz, vzz};
+}
+
+// energy and virial partial sums of the influence-function step
+struct PmERed {
+  double en, vxx, vxy, vxz, vyy, vyz, vzz;
+};
+
+// Grid::scalarSum (base/Grid.h:276-389) at one wave vector (kx, ky, kz) of
+// the half grid: adds v's energy and virial terms (weight 2 for the kz that
+// also stand for their conjugates) and returns v * theta(m).
+template <typename T, typename T2>
+__device__ inline T2 influence(const PmPmeLaunch& a, int kx, int ky, int kz, T2 v, PmERed& s) {
+  T theta = 0;
+  if (kx != 0 || ky != 0 || kz != 0) {
+    const int i0 = kx <= a.nx / 2 ? kx : kx - a.nx;
+    const int j0 = ky <= a.ny / 2 ? ky : ky - a.ny;
+    const double mi = i0 * a.Lr[0], mj = j0 * a.Lr[1], mk = kz * a.Lr[2];
+    const double m2 = mi * mi + mj * mj + mk * mk;
+    const double* bm = a.bmod;
+    const double* ex = a.expf_tab;
+    const double th = bm[kx] * bm[a.nx + ky] * bm[a.nx + a.ny + kz] * (ex[kx] * a.inv_pi_v) * ex[a.nx + ky] *
+                      ex[a.nx + a.ny + kz] / m2;
+    theta = (T)th;
+    const double w = (kz == 0 || ((a.nz & 1) == 0 && kz == a.nz / 2)) ? 1.0 : 2.0;
+    const double q2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    const double eg = w * q2 * th;
+    s.en += eg;
+    if (a.want_virial) {
+      const double vv = 2.0 * (1.0 / m2 + a.fac);
+      s.vxx += eg * (1.0 - vv * mi * mi);
+      s.vxy -= eg * (vv * mi * mj);
+      s.vxz -= eg * (vv * mi * mk);
+      s.vyy += eg * (1.0 - vv * mj * mj);
+      s.vyz -= eg * (vv * mj * mk);
+      s.vzz += eg * (1.0 - vv * mk * mk);
+    }
+  }
+  v.x *= theta;
+  v.y *= theta;
+  return v;
+}
+
+// block sum (fixed order: lanes, then waves) of the partials into this
+// block's energy record; ends with the block's threads synchronised
+__device__ inline void store_block_energy(const PmPmeLaunch& a, const PmERed& s, double (*s_red)[8]) {
+  const double part[7] = {s.en, s.vxx, s.vxy, s.vxz, s.vyy, s.vyz, s.vzz};
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int u = 0; u < 7; u++) {
     double t = part[u];
@@ -714,12 +717,111 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
     ep[PM_E_VIR + 3] = tot[2] * 0.5; ep[PM_E_VIR + 4] = tot[4] * 0.5; ep[PM_E_VIR + 5] = tot[5] * 0.5;
     ep[PM_E_VIR + 6] = tot[3] * 0.5; ep[PM_E_VIR + 7] = tot[5] * 0.5; ep[PM_E_VIR + 8] = tot[6] * 0.5;
   }
-  // (the conv loop wrote r[] before the barrier above)
-  T2* other = (r == b0) ? b1 : b0;
-  T2* r2 = fft_lines<T, T2>(r, other, TL, p, tws, +1);
+}
+
+// y pass (either direction) on the line-major grid: lines (x, kz) with TL
+// consecutive kz per block
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int TL, int sign) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * p.n;
+  T2* tws = b1 + TL * p.n;
+  load_twiddles(tws, p, a.tw);
+  const int n = p.n;  // ny
+  const int ntile = (a.nzc + TL - 1) / TL;
+  const int x = blockIdx.x / ntile;
+  const int kz0 = (blockIdx.x - x * ntile) * TL;
+  T2* gc = reinterpret_cast<T2*>(a.grid_c);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int y = e / TL, l = e - y * TL;
+    T2 v; v.x = 0; v.y = 0;
+    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
+    b0[l * n + y] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, sign, n);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int y = e / TL, l = e - y * TL;
+    if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r[l * n + y];
+  }
+}
+
+// x pass with the influence function on the line-major grid: forward
+// transform, multiply by theta(m), inverse transform.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, int TL) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  __shared__ double s_red[16][8];
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * p.n;
+  T2* tws = b1 + TL * p.n;
+  load_twiddles(tws, p, a.tw);
+  const int n = p.n;  // nx
+  const int ntile = (a.nzc + TL - 1) / TL;
+  const int y = blockIdx.x / ntile;
+  const int kz0 = (blockIdx.x - y * ntile) * TL;
+  T2* gc = reinterpret_cast<T2*>(a.grid_c);
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int x = e / TL, l = e - x * TL;
+    T2 v; v.x = 0; v.y = 0;
+    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
+    b0[l * n + x] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1, n);
+  PmERed s = {0, 0, 0, 0, 0, 0, 0};
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int l = e / n, kx = e - l * n;
+    const int kz = kz0 + l;
+    if (kz >= a.nzc) continue;
+    r[e] = influence<T, T2>(a, kx, y, kz, r[e], s);
+  }
+  store_block_energy(a, s, s_red);   // its barrier also orders the writes to r[]
+  T2* r2 = fft_lines<T, T2>(r, r == b0 ? b1 : b0, TL, p, tws, +1, n);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int x = e / TL, l = e - x * TL;
     if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r2[l * n + x];
+  }
+}
+
+// One kz plane of the plane-major grid per block: y forward, x forward, the
+// influence function, x inverse and y inverse with the whole plane in LDS
+// (row stride ny + 1, so that lines along y and lines along x are both read
+// by consecutive threads on consecutive lines without bank conflicts).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_fft_plane(PmPmeLaunch a, PmFftPlan px, PmFftPlan py) {
+  typedef typename Cx<T>::t T2;
+  extern __shared__ unsigned char s_raw[];
+  __shared__ double s_red[16][8];
+  const int nx = a.nx, ny = a.ny, rs = ny + 1, kz = blockIdx.x;
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + nx * rs;
+  T2* twx = b1 + nx * rs;
+  T2* twy = twx + nx;
+  load_twiddles(twx, px, a.tw);
+  load_twiddles(twy, py, a.tw);
+  T2* gp = reinterpret_cast<T2*>(a.grid_c) + (long long)kz * nx * ny;
+  for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+    const int x = e / ny, y = e - x * ny;
+    b0[x * rs + y] = gp[e];
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2, true>(b0, b1, nx, py, twy, -1, rs, 1);      // y lines
+  r = fft_lines<T, T2, true>(r, r == b0 ? b1 : b0, ny, px, twx, -1, 1, rs);   // x lines
+  PmERed s = {0, 0, 0, 0, 0, 0, 0};
+  for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+    const int x = e / ny, y = e - x * ny;
+    r[x * rs + y] = influence<T, T2>(a, x, y, kz, r[x * rs + y], s);
+  }
+  store_block_energy(a, s, s_red);
+  r = fft_lines<T, T2, true>(r, r == b0 ? b1 : b0, ny, px, twx, +1, 1, rs);
+  r = fft_lines<T, T2, true>(r, r == b0 ? b1 : b0, nx, py, twy, +1, rs, 1);
+  for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+    const int x = e / ny, y = e - x * ny;
+    gp[e] = r[x * rs + y];
   }
 }
 
@@ -728,30 +830,43 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fft_z_inv(PmPmeLaunch a, PmFftPlan p, int TL) {
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
+  const int n = p.n, ls = n + 1, nl_tot = a.nx * a.ny, nzc = a.nzc;
   T2* b0 = reinterpret_cast<T2*>(s_raw);
-  T2* b1 = b0 + TL * p.n;
-  T2* tws = b1 + TL * p.n;
+  T2* b1 = b0 + TL * ls;
+  T2* tws = b1 + TL * ls;
   load_twiddles(tws, p, a.tw);
-  const int n = p.n, nl_tot = a.nx * a.ny, nzc = a.nzc;
   const int line0 = blockIdx.x * TL;
   const T2* gc = reinterpret_cast<const T2*>(a.grid_c);
-  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
-    const int l = e / n, k = e - l * n;
-    const int line = line0 + l;
-    T2 v; v.x = 0; v.y = 0;
-    if (line < nl_tot) {
-      if (k < nzc) v = gc[(long long)line * nzc + k];
-      else { v = gc[(long long)line * nzc + (n - k)]; v.y = -v.y; }
+  if (a.plane) {
+    for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+      const int k = e / TL, l = e - k * TL;
+      const int line = line0 + l;
+      T2 v; v.x = 0; v.y = 0;
+      if (line < nl_tot) {
+        if (k < nzc) v = gc[(long long)k * nl_tot + line];
+        else { v = gc[(long long)(n - k) * nl_tot + line]; v.y = -v.y; }
+      }
+      b0[l * ls + k] = v;
     }
-    b0[e] = v;
+  } else {
+    for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+      const int l = e / n, k = e - l * n;
+      const int line = line0 + l;
+      T2 v; v.x = 0; v.y = 0;
+      if (line < nl_tot) {
+        if (k < nzc) v = gc[(long long)line * nzc + k];
+        else { v = gc[(long long)line * nzc + (n - k)]; v.y = -v.y; }
+      }
+      b0[l * ls + k] = v;
+    }
   }
   __syncthreads();
-  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, +1);
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, +1, ls);
   T* g = reinterpret_cast<T*>(a.grid_r);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int l = e / n, k = e - l * n;
     const int line = line0 + l;
-    if (line < nl_tot) g[(long long)line * n + k] = r[e].x;
+    if (line < nl_tot) g[(long long)line * n + k] = r[l * ls + k].x;
   }
 }
 
@@ -759,42 +874,84 @@ __global__ __launch_bounds__(256) void k_fft_z_inv(PmPmeLaunch a, PmFftPlan p, i
 
 // ----------------------------------------------------------------- launcher
 static int lines_per_block(int n, size_t elem) {
-  int tl = (int)(32768 / ((size_t)n * elem * 2));
+  int tl = (int)(32768 / ((size_t)(n + 1) * elem * 2));
   if (tl > 16) tl = 16;
   if (tl < 1) tl = 1;
   return tl;
 }
 
-// z forward, y forward, x forward + influence function + x inverse, y
-// inverse, z inverse: the fixed-point charge grid to the real potential grid
+// LDS of the z passes (padded lines) and of the separate y and x passes
+static size_t zpass_lds(int n, int tl, size_t elem) { return ((size_t)2 * tl * (n + 1) + n) * elem; }
+static size_t line_lds(int n, int tl, size_t elem) { return ((size_t)2 * tl + 1) * n * elem; }
+// dynamic LDS of the plane kernel: two padded planes and both twiddle tables
+static size_t plane_lds(int nx, int ny, size_t elem) { return ((size_t)2 * nx * (ny + 1) + nx + ny) * elem; }
+
+static size_t fft_elem(int precision) { return precision == 1 ? sizeof(double2) : sizeof(float2); }
+
+int pm_pme_plane_fits(int nx, int ny, int precision, int lds_max) {
+  const size_t stat = sizeof(double) * 16 * 8;   // s_red of k_fft_plane
+  return plane_lds(nx, ny, fft_elem(precision)) + stat <= (size_t)lds_max ? 1 : 0;
+}
+
+// threads of the plane kernel: the block size in {1024, 768, 512} that
+// leaves the fewest idle threads over the butterfly loops of the four
+// transforms
+static int plane_threads(const PmFftPlan& px, const PmFftPlan& py) {
+  static const int cand[3] = {1024, 768, 512};
+  int best = 1024;
+  long long best_cost = -1;
+  for (int c : cand) {
+    long long cost = 0;
+    for (int t = 0; t < 2; t++) {
+      const PmFftPlan& p = t == 0 ? px : py;
+      const int nl = t == 0 ? py.n : px.n;
+      for (int st = 0; st < p.nst; st++) {
+        const long long work = (long long)nl * (p.n / p.rad[st]);
+        cost += (work + c - 1) / c * c;
+      }
+    }
+    if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = c; }
+  }
+  return best;
+}
+
+// z forward, then either y forward, x forward + influence function + x
+// inverse, y inverse as separate passes over the line-major grid or all of
+// them in one kernel per kz plane, then z inverse: the fixed-point charge
+// grid to the real potential grid
 template <typename T>
 static void launch_fft_chain(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans) {
-  typedef typename Cx<T>::t T2;
+  const size_t el = sizeof(typename Cx<T>::t);
   const int nthreads = 256;
   {
-    const int tl = lines_per_block(a.nz, sizeof(T2));
+    const int tl = lines_per_block(a.nz, el);
     const int nb = (a.nx * a.ny + tl - 1) / tl;
-    hipLaunchKernelGGL(k_fft_z_fwd<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nz * sizeof(T2), s, a, plans[2], tl);
+    hipLaunchKernelGGL(k_fft_z_fwd<T>, dim3(nb), dim3(nthreads), zpass_lds(a.nz, tl, el), s, a, plans[2], tl);
+  }
+  if (a.plane) {
+    hipLaunchKernelGGL(k_fft_plane<T>, dim3(a.nzc), dim3(plane_threads(plans[0], plans[1])),
+                       plane_lds(a.nx, a.ny, el), s, a, plans[0], plans[1]);
+  } else {
+    {
+      const int tl = lines_per_block(a.ny, el);
+      const int nb = a.nx * ((a.nzc + tl - 1) / tl);
+      hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), line_lds(a.ny, tl, el), s, a, plans[1], tl, -1);
+    }
+    {
+      const int tl = lines_per_block(a.nx, el);
+      const int nb = a.ny * ((a.nzc + tl - 1) / tl);
+      hipLaunchKernelGGL(k_fft_x_conv<T>, dim3(nb), dim3(nthreads), line_lds(a.nx, tl, el), s, a, plans[0], tl);
+    }
+    {
+      const int tl = lines_per_block(a.ny, el);
+      const int nb = a.nx * ((a.nzc + tl - 1) / tl);
+      hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), line_lds(a.ny, tl, el), s, a, plans[1], tl, +1);
+    }
   }
   {
-    const int tl = lines_per_block(a.ny, sizeof(T2));
-    const int nb = a.nx * ((a.nzc + tl - 1) / tl);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.ny * sizeof(T2), s, a, plans[1], tl, -1);
-  }
-  {
-    const int tl = lines_per_block(a.nx, sizeof(T2));
-    const int nb = a.ny * ((a.nzc + tl - 1) / tl);
-    hipLaunchKernelGGL(k_fft_x_conv<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nx * sizeof(T2), s, a, plans[0], tl);
-  }
-  {
-    const int tl = lines_per_block(a.ny, sizeof(T2));
-    const int nb = a.nx * ((a.nzc + tl - 1) / tl);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.ny * sizeof(T2), s, a, plans[1], tl, +1);
-  }
-  {
-    const int tl = lines_per_block(a.nz, sizeof(T2));
+    const int tl = lines_per_block(a.nz, el);
     const int nb = (a.nx * a.ny + tl - 1) / tl;
-    hipLaunchKernelGGL(k_fft_z_inv<T>, dim3(nb), dim3(nthreads), (2 * tl + 1) * a.nz * sizeof(T2), s, a, plans[2], tl);
+    hipLaunchKernelGGL(k_fft_z_inv<T>, dim3(nb), dim3(nthreads), zpass_lds(a.nz, tl, el), s, a, plans[2], tl);
   }
 }
 
@@ -807,10 +964,12 @@ static void launch_spread_tile(hipStream_t s, const PmPmeLaunch& a, const longlo
 
 template <typename T, int P>
 static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
-                         const longlong4* pos64) {
+                         const longlong4* pos64, const PmAtomTail* tail) {
   launch_spread_tile<T, P>(s, a, pos64);
   launch_fft_chain<T>(s, a, plans);
-  hipLaunchKernelGGL((k_gather<T, P>), dim3((a.nslot_max + 255) / 256), dim3(256), 0, s, a, pos64);
+  const dim3 g((a.nslot_max + 255) / 256), b(256);
+  if (tail) hipLaunchKernelGGL((k_gather<T, P, true>), g, b, 0, s, a, pos64, *tail);
+  else hipLaunchKernelGGL((k_gather<T, P, false>), g, b, 0, s, a, pos64, PmAtomTail{});
 }
 
 // adjoint: mode 0 the reciprocal potential at every atom (dE/dq), mode 1 the
@@ -832,21 +991,21 @@ static void launch_pme_adj_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPla
   hipLaunchKernelGGL((k_gather_adj<T, P, 2>), g, b, 0, s, a, pos64, slot_v, out);
 }
 
-int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision) {
-  (void)nz;
-  const size_t e = precision == 1 ? sizeof(double2) : sizeof(float2);
+int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision, int plane) {
   const int nzc = nz / 2 + 1;
-  const int tl = lines_per_block(nx, e);
+  if (plane) return nzc;
+  const int tl = lines_per_block(nx, fft_elem(precision));
   (void)ny;
   return ny * ((nzc + tl - 1) / tl);
 }
 
 template <typename T>
-static bool launch_pme_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64) {
+static bool launch_pme_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
+                         const PmAtomTail* tail) {
   switch (a.order) {
-    case 4: launch_pme_t<T, 4>(s, a, plans, pos64); return true;
-    case 6: launch_pme_t<T, 6>(s, a, plans, pos64); return true;
-    case 8: launch_pme_t<T, 8>(s, a, plans, pos64); return true;
+    case 4: launch_pme_t<T, 4>(s, a, plans, pos64, tail); return true;
+    case 6: launch_pme_t<T, 6>(s, a, plans, pos64, tail); return true;
+    case 8: launch_pme_t<T, 8>(s, a, plans, pos64, tail); return true;
     default: return false;
   }
 }
@@ -869,7 +1028,7 @@ bool pm_launch_pme_adjoint(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan*
 }
 
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
-                         int precision) {
-  if (precision == 1) return launch_pme_p<double>(s, a, plans, pos64);
-  return launch_pme_p<float>(s, a, plans, pos64);
+                         int precision, const PmAtomTail* tail) {
+  if (precision == 1) return launch_pme_p<double>(s, a, plans, pos64, tail);
+  return launch_pme_p<float>(s, a, plans, pos64, tail);
 }

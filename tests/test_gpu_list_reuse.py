@@ -64,16 +64,19 @@ def test_reuse_and_rebuild_match_fresh_lists(gpu_ctx_factory, precision, tol):
     assert rms_rel(f, fr) <= tol
 
 
-def test_md_with_reuse_follows_rebuild_every_step(gpu_ctx_factory):
-    """FP64 leapfrog, 50 steps of the 8,100-atom box with PME: the same
-    trajectory (to accumulation order) with and without list reuse."""
-    say("case: md 50 steps with/without list reuse, FP64")
+@pytest.mark.parametrize("integrator", ["leapfrog", "langevin"])
+def test_md_with_reuse_follows_rebuild_every_step(gpu_ctx_factory, integrator):
+    """FP64 leapfrog or Langevin, 50 steps of the 8,100-atom box with PME:
+    the same trajectory (to accumulation order) with and without list reuse.
+    With reuse the position update of each step also runs the displacement
+    test that decides the next rebuild."""
+    say(f"case: md 50 steps with/without list reuse, FP64, {integrator}")
     s = water.named("water8k", seed=3)
     out = {}
     for skin in (0, 600):
         c, hs = make(gpu_ctx_factory, s, "double", skin)
         hs = hs + [c.force("Bond", []), c.force("Angle", [])]
-        c.md_setup(hs, "leapfrog", dt=1.0)
+        c.md_setup(hs, integrator, dt=1.0, temperature=300.0, gamma=5.0, seed=11)
         c.md_set_state(s.pos, s.vel)
         b0 = c.stats()["list_builds"]
         c.md_run(50)

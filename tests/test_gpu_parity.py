@@ -113,13 +113,19 @@ def test_exclusion_rules(gpu_ctx_factory, port, precision, tol, cluster):
 
 @pytest.mark.parametrize("grid,order", [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8)])
 @pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-10)])
-def test_pme_small(gpu_ctx_factory, port, grid, order, precision, tol):
-    say(f"case: pme water648 grid {grid} order {order} {precision}")
+@pytest.mark.parametrize("fft", [1, 2])
+def test_pme_small(gpu_ctx_factory, port, grid, order, precision, tol, fft):
+    """PME reciprocal + correction against the oracle, with the y and x
+    transforms as separate passes (fft 1) and as one kernel per kz plane
+    (fft 2)."""
+    say(f"case: pme water648 grid {grid} order {order} {precision} fft option {fft}")
     s = water.named("water648", seed=3, jitter=0.4)
     c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    c.set_option("pme_fft", fft)
     h = c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
                 [grid[0], grid[1], grid[2], 8.0, order, 1e-6, -1.0])
     f, e = c.evaluate([h], s.pos, virial=True)
+    assert c.get_option("pme_fft") == fft
     o = port.pme(s, grid, order, 8.0, 4.0, flags=6, virial=True)
     say("rms rel force", rms_rel(f, o["forces"]), "E", e[0], o["energies"][0])
     assert rms_rel(f, o["forces"]) < tol
@@ -127,6 +133,35 @@ def test_pme_small(gpu_ctx_factory, port, grid, order, precision, tol):
     scale = sum(abs(o["parts"][k]) for k in ("reciprocal", "intra", "self"))
     assert abs(e[0] - o["energies"][0]) < tol * 0.2 * scale
     np.testing.assert_allclose(e[7:], o["energies"][7:], rtol=tol * 100, atol=tol * 100 * np.abs(o["energies"][7:]).max())
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_pme_gather_tail_matches_separate_terms(gpu_ctx_factory, precision):
+    """With one PME reciprocal term the gather kernel also adds each atom's
+    correction and bond/angle forces.  The same terms evaluated one handle
+    at a time (separate kernels) and with the correction as its own handle
+    give the same forces to the fixed-point granularity, and the same
+    energies."""
+    say(f"case: pme gather tail water648 {precision}")
+    s = water.named("water648", seed=3, jitter=0.4)
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    args = [24, 24, 24, 8.0, 4, 1e-6, -1.0]
+    hp = c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline", args)
+    hr = c.force("Coulomb -algorithm PMEwald -reciprocal -interpolation BSpline", args)
+    hc = c.force("Coulomb -algorithm PMEwald -correction -interpolation BSpline", args)
+    hb = c.force("Bond", [])
+    ha = c.force("Angle", [])
+    f_all, e_all = c.evaluate([hp, hb, ha], s.pos)          # correction and bonded in the gather
+    parts = [c.evaluate([h], s.pos) for h in (hr, hc, hb, ha)]   # each term on its own
+    f_sep = sum(p[0] for p in parts)
+    e_sep = sum(p[1] for p in parts)
+    f_split, e_split = c.evaluate([hr, hc, hb, ha], s.pos)  # bonded in the gather, correction apart
+    fmax = np.abs(f_all).max()
+    say("max |df| separate", np.abs(f_all - f_sep).max(), "split", np.abs(f_all - f_split).max(), "of", fmax)
+    assert np.abs(f_all - f_sep).max() <= 1e-11 * fmax
+    assert np.abs(f_all - f_split).max() <= 1e-11 * fmax
+    np.testing.assert_allclose(e_all, e_sep, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(e_all, e_split, rtol=1e-12, atol=1e-9)
 
 
 def test_pme_full_alias(gpu_ctx_factory, port):
@@ -140,12 +175,15 @@ def test_pme_full_alias(gpu_ctx_factory, port):
     np.testing.assert_allclose(e[0], o["energies"][0], rtol=1e-10)
 
 
-def test_water98k_pme_mixed_accuracy(gpu_ctx_factory, port):
+@pytest.mark.parametrize("fft", [1, 2])
+def test_water98k_pme_mixed_accuracy(gpu_ctx_factory, port, fft):
     """98,304-atom water, LJ-C2 + Ewald-real rc 10 plus PME 96^3: RMS relative
-    force error of the mixed-precision path against the FP64 oracle."""
-    say("case: water98k full force field mixed")
+    force error of the mixed-precision path against the FP64 oracle, with
+    either arrangement of the y and x transforms."""
+    say(f"case: water98k full force field mixed, fft option {fft}")
     s = water.named("water98k", seed=1, jitter=0.4)
     c = setup_ctx(gpu_ctx_factory, s, 5.0, "mixed")
+    c.set_option("pme_fft", fft)
     h1 = c.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction Cutoff",
                  [9.0, 10.0, ALPHA_RC10, 10.0, 10.0])
     h2 = c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
