@@ -144,6 +144,7 @@ struct pm_context {
   double* d_scratch3 = nullptr;   // [3n] double conversions
 
   // cells and slots
+  int* d_bin_count = nullptr;        // per-cell atom counts of the binning in progress
   int *d_cell_of = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_slot = nullptr,
       *d_cell_rank = nullptr, *d_cell_atoms = nullptr, *d_cluster_cell = nullptr, *d_counts = nullptr;
   float* d_key = nullptr;
@@ -199,6 +200,9 @@ struct pm_context {
   // moved more than skin / 2 since the build (checked on the device)
   int skin_milli = -1;               // -1 automatic, 0 off, else 1e-3 A
   bool disp_prechecked = false;      // the next evaluation's displacement test already queued
+  bool md_prebinned = false;         // the next evaluation's binning and force clearing already queued
+  int prebin_gen = -1;               // layout_gen the queued binning was made for
+  int layout_gen = 0;                // +1 whenever the cell and slot buffers are reallocated
   int* d_rebuild = nullptr;          // device flag: 1 = rebuild in this evaluation
   int4* d_xbuild = nullptr;          // [n] fixed-point position at the last build, w = 1 when valid
   bool list_stale = true;            // x_build must be invalidated before the next reuse
@@ -427,6 +431,8 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_cell_of, n))) return r;
   if ((r = dalloc(c, &c->d_key, n))) return r;
   if ((r = dalloc(c, &c->d_cell_count, nc))) return r;
+  if ((r = dalloc(c, &c->d_bin_count, nc))) return r;
+  HIPCHECK(c, hipMemset(c->d_bin_count, 0, (size_t)nc * sizeof(int)));
   if ((r = dalloc(c, &c->d_cell_start, nc))) return r;
   if ((r = dalloc(c, &c->d_cell_slot, nc))) return r;
   if ((r = dalloc(c, &c->d_cell_rank, n))) return r;
@@ -458,6 +464,7 @@ int ensure_cells(pm_context* c) {
   c->cells_alloc = nc;
   c->cells_m = m;
   c->jlist_alloc = 0;
+  c->layout_gen++;
   return PM_OK;
 }
 
@@ -872,24 +879,39 @@ bool list_reuse(const pm_context* c, const std::vector<PairPass>& passes) {
   return passes[0].algo_cutoff >= pc && lmin_h > pc + skin + cd + 1e-3;
 }
 
-// The displacement test for the MD forces' next evaluation, to be run by the
-// position update that precedes it; false (no test there) when the lists are
-// not reused or must first be invalidated.  On true the next evaluate_group
-// skips its own test.
-bool md_disp_check(pm_context* c, PmDispCheck* chk) {
-  if (!c->have_box || c->list_stale || !c->d_xbuild || !c->d_rebuild) return false;
-  if (!list_reuse(c, plan_passes(c, c->md_handles))) return false;
-  const double half = 0.5 * list_skin(c);
-  for (int k = 0; k < 3; k++) {
-    chk->to_fix[k] = c->box.to_fix[k];
-    chk->fix_to_A[k] = c->box.fix_to_A[k];
+// The start of the MD forces' next evaluation, run by the position update
+// that precedes it (PmStepWork): the binning of the new positions and, when
+// the lists are reused and need no invalidation, the displacement test.
+// False when the cell layout is not set up yet (the evaluation then does
+// both).  On true the update also clears the forces after reading them, and
+// the next evaluate_group launches neither the binning nor the clearing (nor
+// the test, when it was queued).
+bool md_step_work(pm_context* c, PmStepWork* w) {
+  *w = PmStepWork{};
+  if (!c->have_box || !c->d_bin_count || !c->d_cell_of || !c->d_cell_rank || !c->d_key ||
+      c->cells_alloc != c->ncells || c->cells_m != choose_m(c))
+    return false;
+  w->box = c->box;
+  w->cell_of = c->d_cell_of;
+  w->rank = c->d_cell_rank;
+  w->key = c->d_key;
+  w->bin_count = c->d_bin_count;
+  c->md_prebinned = true;
+  c->prebin_gen = c->layout_gen;
+  if (!c->list_stale && c->d_xbuild && c->d_rebuild && list_reuse(c, plan_passes(c, c->md_handles))) {
+    const double half = 0.5 * list_skin(c);
+    for (int k = 0; k < 3; k++) {
+      w->to_fix[k] = c->box.to_fix[k];
+      w->fix_to_A[k] = c->box.fix_to_A[k];
+    }
+    w->x_build = c->d_xbuild;
+    w->lim2 = (float)half * (float)half;
+    w->flag = c->d_rebuild;
+    c->disp_prechecked = true;
   }
-  chk->x_build = c->d_xbuild;
-  chk->lim2 = (float)half * (float)half;
-  chk->flag = c->d_rebuild;
-  c->disp_prechecked = true;
   return true;
 }
+
 
 int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bool want_energy = true,
                    const AdjointSpec* adj = nullptr) {
@@ -900,7 +922,9 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   // displacement test already (md_disp_check); taken here, before any early
   // return, so that it never carries over to another evaluation
   const bool prechecked = c->disp_prechecked;
+  const bool prebinned = c->md_prebinned;
   c->disp_prechecked = false;
+  c->md_prebinned = false;
   int r;
   for (int h : handles) {
     ForceDesc& f = c->forces[h];
@@ -941,10 +965,15 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   // kernel sequence the same with and without reuse)
   const float half = (float)(0.5 * skin);
   pm_launch_check_disp(s, c->box, reuse && !prechecked ? n : 0, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
-  pm_launch_zero_force_cells(s, rb, c->d_force, 3LL * n, c->d_force_slot, passes.empty() ? 0 : 3LL * c->nslot_max,
-                             c->d_cell_count, nc);
-  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key, rb);
-  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
+  // (the MD position update may have binned the atoms and cleared the forces
+  // already, for the layout in place; the last fold cleared the slot forces)
+  const bool binned = prebinned && c->prebin_gen == c->layout_gen;
+  if (!binned) {
+    pm_launch_zero_force_cells(s, rb, c->d_force, 3LL * n, c->d_force_slot,
+                               passes.empty() ? 0 : 3LL * c->nslot_max, nullptr, 0);
+    pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_bin_count, c->d_cell_rank, c->d_key, rb);
+  }
+  pm_launch_scan_cells(s, nc, c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
                        c->d_cluster_cell, c->ncl_max, c->m, rb);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, rb);
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
@@ -1131,7 +1160,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       pm_launch_adjoint(s, c->box, pl.first, pl.second, c->precision, adj->mode, adj->slot_v, adj->slot);
     }
     if (!pair_launch.empty())
-      pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, adj->slot, adj->fix, nullptr, nullptr);
+      pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, adj->slot, adj->fix, nullptr, nullptr, 0);
     for (auto& pl : pme_launch)
       pm_launch_pme_adjoint(s, pl.first, c->plans, c->d_slot_pos64, c->precision, adj->mode == 1 ? 1 : 0,
                             adj->slot_v, adj->mode == 1 ? adj->hv : adj->dq);
@@ -1233,7 +1262,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     TimedScope ts(c, PM_T_BIN);
     // with reuse also the end of the evaluation: the flag's readers all ran
     pm_launch_fold_slots(s, c->nslot_max, c->d_counts, c->d_atom_of_slot, c->d_force_slot, c->d_force,
-                         reuse ? c->d_rebuild : nullptr, c->d_stats + 2);
+                         reuse ? c->d_rebuild : nullptr, c->d_stats + 2, 1);
   }
   if (want_energy) {
     TimedScope ts(c, PM_T_REDUCE);
@@ -1286,7 +1315,7 @@ void free_all(pm_context* c) {
   dfree(c->d_tab_a); dfree(c->d_tab_b); dfree(c->d_tab_c);
   dfree(c->d_lj_f); dfree(c->d_lj_d);
   dfree(c->d_pos); dfree(c->d_vel); dfree(c->d_force); dfree(c->d_energy); dfree(c->d_scratch3);
-  dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
+  dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_bin_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
   dfree(c->d_xbuild); dfree(c->d_rebuild); c->list_stale = true;
   dfree(c->d_cell_rank); dfree(c->d_force_slot); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
@@ -1746,9 +1775,9 @@ int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm
   hipStream_t s = c->stream;
   HIPCHECK(c, hipMemcpyAsync(c->d_pos, positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
   c->list_stale = true;   // the binning below replaces the layout's cell lists
-  pm_launch_zero_i(s, c->d_cell_count, nc);
-  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_cell_count, c->d_cell_rank, c->d_key, nullptr);
-  pm_launch_scan_cells(s, nc, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
+  pm_launch_zero_i(s, c->d_bin_count, nc);
+  pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_bin_count, c->d_cell_rank, c->d_key, nullptr);
+  pm_launch_scan_cells(s, nc, c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
                        c->d_cluster_cell, c->ncl_max, c->m, nullptr);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, nullptr);
   std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);
@@ -1832,28 +1861,42 @@ namespace {
 // streams: kind 0 = a middle leapfrog step (LeapfrogIntegrator::doKickdoDrift,
 // then the forces), kind 1 = a whole Langevin step (StandardIntegrator::run:
 // half kick, drift, forces, half kick).
+// An evaluation after a position update with PmStepWork failed: the forces
+// were cleared and the binning counts may hold a binning that no scan
+// consumed; the next run starts from a fresh force evaluation and zero counts.
+void md_step_failed(pm_context* c) {
+  c->md_forces_valid = false;
+  c->md_prebinned = false;
+  c->disp_prechecked = false;
+  if (c->d_bin_count && c->ncells > 0) pm_launch_zero_i(c->stream, c->d_bin_count, c->ncells);
+}
+
 int md_unit(pm_context* c, int kind, bool energy) {
   hipStream_t s = c->stream;
   const int n = c->n;
   const double h = c->md_dt * kInvTimeFactor;
   int r;
-  PmDispCheck chk{};
+  PmStepWork w{};
   if (kind == 0) {
     {
       TimedScope ts(c, PM_T_INTEGRATE);
-      const bool ck = md_disp_check(c, &chk);
-      pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, ck ? &chk : nullptr);
+      const bool sw = md_step_work(c, &w);
+      pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, sw ? &w : nullptr, sw ? 1 : 0);
     }
-    return evaluate_group(c, c->md_handles, 0, energy);
+    if ((r = evaluate_group(c, c->md_handles, 0, energy))) md_step_failed(c);
+    return r;
   }
   const double g = c->md_gamma / (1000 * kInvTimeFactor);
   {
     TimedScope ts(c, PM_T_INTEGRATE);
-    pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 0);
-    const bool ck = md_disp_check(c, &chk);
-    pm_launch_drift(s, n, h, c->d_vel, c->d_pos, ck ? &chk : nullptr);
+    const bool sw = md_step_work(c, &w);
+    pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 0, sw ? 1 : 0);
+    pm_launch_drift(s, n, h, c->d_vel, c->d_pos, sw ? &w : nullptr);
   }
-  if ((r = evaluate_group(c, c->md_handles, 0, energy))) return r;
+  if ((r = evaluate_group(c, c->md_handles, 0, energy))) {
+    md_step_failed(c);
+    return r;
+  }
   TimedScope ts(c, PM_T_INTEGRATE);
   pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 1);
   pm_launch_md_advance(s, c->d_md);
@@ -1936,11 +1979,14 @@ int pm_md_run(pm_context* c, int nsteps) {
     // energies of the last force evaluation only
     {
       TimedScope ts(c, PM_T_INTEGRATE);
-      PmDispCheck chk{};
-      const bool ck = md_disp_check(c, &chk);
-      pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, ck ? &chk : nullptr);
+      PmStepWork w{};
+      const bool sw = md_step_work(c, &w);
+      pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, sw ? &w : nullptr, sw ? 1 : 0);
     }
-    if ((r = evaluate_group(c, c->md_handles, 0, nsteps == 1))) return r;
+    if ((r = evaluate_group(c, c->md_handles, 0, nsteps == 1))) {
+      md_step_failed(c);
+      return r;
+    }
     if (nsteps >= 3 && (r = md_units(c, 0, nsteps - 2))) return r;
     if (nsteps >= 2 && (r = md_unit(c, 0, true))) return r;
     pm_launch_kick(s, n, 2, 0.5 * c->md_dt * kInvTimeFactor, c->d_mass, c->d_force, c->d_vel, c->d_pos);

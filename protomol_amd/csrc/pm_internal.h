@@ -132,9 +132,12 @@ struct PmNbLaunch {
 // Binning (reference-exact) and slot layout.
 // `rebuild` (device flag, may be null = always): the kernels return at once
 // when it is 0, keeping the layout of the last build.
+// The binning counts atoms per cell into bin_count (zero before every
+// binning); the scan copies them into the layout's cell_count and clears
+// bin_count (also on steps that keep the layout).
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
-                   int* cell_count, int* rank, float* key, const int* rebuild);
-void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
+                   int* bin_count, int* rank, float* key, const int* rebuild);
+void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_count, int* cell_start,
                           int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild);
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
                        int* cell_atoms, const int* rebuild);
@@ -150,11 +153,12 @@ void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* 
                           float lim2, int* flag);
 void pm_launch_set_flag(hipStream_t s, int* flag, int v);
 void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build);
-// force_fix[atom] += force_slot[slot] for every real slot; with `flag` (list
-// reuse) also the end of the evaluation: builds[0] += (*flag != 0), *flag = 0
+// force_fix[atom] += force_slot[slot] for every real slot (then the slot
+// cleared when `clear`); with `flag` (list reuse) also the end of the
+// evaluation: builds[0] += (*flag != 0), *flag = 0
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
-                          const long long* force_slot, long long* force_fix, int* flag,
-                          unsigned long long* builds);
+                          long long* force_slot, long long* force_fix, int* flag,
+                          unsigned long long* builds, int clear);
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
 // Adjoint pair kernel (clusters of 8): mode 1 H v, mode 2 (dE/dq, dE/dR, dE/de)
@@ -260,28 +264,34 @@ void pm_launch_zero_force_cells(hipStream_t s, const int* rebuild, long long* f,
                                 long long nfs, int* cc, int nc);
 void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* out);
 void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out);
-// the list-reuse displacement test run by a position update on the positions
-// it writes (pm_launch_check_disp's test; flag null: no test)
-struct PmDispCheck {
+// The start of the next evaluation, run by the MD position update on the
+// positions it writes: the list-reuse displacement test (pm_launch_check_disp's
+// test; flag null: none) and the binning (pm_launch_bin's; bin_count null:
+// none), so that the evaluation launches neither.
+struct PmStepWork {
   double to_fix[3], fix_to_A[3];
   const int4* x_build;
   float lim2;
   int* flag;
+  PmBox box;
+  int* cell_of; int* rank; float* key; int* bin_count;
 };
-void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
-                    double* v, double* x, const PmDispCheck* chk = nullptr);
+// leapfrog kick (+ drift); zero_f: clear the forces after reading them (the
+// next evaluation then needs no clearing of its own)
+void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, long long* f,
+                    double* v, double* x, const PmStepWork* w = nullptr, int zero_f = 0);
 // MD state the integrator kernels read from device memory
 struct PmMdDevice {
   unsigned long long counter;   // Langevin noise counter (2 per step)
   double temperature;           // Langevin bath temperature (K)
 };
 void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_akma, const double* mass,
-                             const long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
-                             int offset);
+                             long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
+                             int offset, int zero_f = 0);
 void pm_launch_md_advance(hipStream_t s, PmMdDevice* md);
 void pm_launch_scale(hipStream_t s, int n3, double factor, double* v);
 // x += h v for n atoms
-void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmDispCheck* chk = nullptr);
+void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmStepWork* w = nullptr);
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation
 struct PmBondedLaunch {

@@ -3,7 +3,7 @@
 // Langevin position-velocity updates.
 #include "pm_internal.h"
 #include "pm_terms.h"
-#include "pm_disp.h"
+#include "pm_step.h"
 
 #include <math.h>
 
@@ -216,8 +216,15 @@ __global__ void k_add_fix(int n3, const long long* __restrict__ f, double* __res
 // (integrators/LeapfrogIntegrator.cpp:46-104) and StandardIntegrator::doHalfKick
 // (StandardIntegrator.cpp:83-96).  mode 0: v += F*h*0.5/m then x += v*h;
 // mode 1: v += F*h/m then x += v*h; mode 2: v += F*hh/m (hh = 0.5*dt*INV_TIMEFACTOR).
-struct KickArgs { int n; int mode; double h; const double* mass; const long long* f; double* v; double* x;
-                 PmDispCheck chk; };
+// the next evaluation's displacement test and binning of atom i at its new
+// position x (PmStepWork)
+__device__ inline void step_work(const PmStepWork& w, int i, const double* x) {
+  if (w.flag && pm_moved_past(w.to_fix, w.fix_to_A, x, w.x_build[i], w.lim2)) *w.flag = 1;
+  if (w.bin_count) pm_bin_atom(w.box, i, x, w.cell_of, w.key, w.rank, w.bin_count);
+}
+
+struct KickArgs { int n; int mode; double h; const double* mass; long long* f; double* v; double* x;
+                 PmStepWork w; int zero_f; };
 __device__ inline void kick_drift_body(const KickArgs& a, int bx) {
 #pragma clang fp contract(off)
   int i = bx * blockDim.x + threadIdx.x;
@@ -227,6 +234,7 @@ __device__ inline void kick_drift_body(const KickArgs& a, int bx) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const double F = (double)a.f[3 * i + k] * PM_FORCE_UNFIX;
+    if (a.zero_f) a.f[3 * i + k] = 0;
     double vv = a.v[3 * i + k];
     if (a.mode == 0) vv += F * a.h * 0.5 / m;
     else vv += F * a.h / m;
@@ -237,7 +245,7 @@ __device__ inline void kick_drift_body(const KickArgs& a, int bx) {
       a.x[3 * i + k] = xn[k];
     }
   }
-  if (a.chk.flag && pm_moved_past(a.chk.to_fix, a.chk.fix_to_A, xn, a.chk.x_build[i], a.chk.lim2)) *a.chk.flag = 1;
+  step_work(a.w, i, xn);
 }
 __global__ void k_kick_drift(KickArgs a) { kick_drift_body(a, blockIdx.x); }
 
@@ -261,13 +269,13 @@ __device__ inline double gauss(unsigned long long seed, unsigned long long count
 // (PmMdDevice), so a captured step replays with fresh deviates and a bath
 // temperature can be changed between replays.
 struct LangevinArgs {
-  int n; double dt; double g; const double* mass; const long long* f; double* v; unsigned long long seed;
-  const PmMdDevice* md; int offset;
+  int n; double dt; double g; const double* mass; long long* f; double* v; unsigned long long seed;
+  const PmMdDevice* md; int offset; int zero_f;
 };
 __device__ inline void langevin_kick_body(int n, double dt, double g, const double* __restrict__ mass,
-                                          const long long* __restrict__ f, double* __restrict__ v,
+                                          long long* __restrict__ f, double* __restrict__ v,
                                           unsigned long long seed, const PmMdDevice* __restrict__ md, int offset,
-                                          int bx) {
+                                          int zero_f, int bx) {
   int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned long long counter = md->counter + (unsigned long long)offset;
@@ -280,12 +288,13 @@ __device__ inline void langevin_kick_body(int n, double dt, double g, const doub
   const double s = sqrt(fc / m);
   for (int k = 0; k < 3; k++) {
     const double F = (double)f[3 * i + k] * PM_FORCE_UNFIX;
+    if (zero_f) f[3 * i + k] = 0;
     const double xi = gauss(seed, counter, 3ull * i + k);
     v[3 * i + k] = v[3 * i + k] * vdt + F * fdt / m + xi * s * ndt;
   }
 }
 __global__ void k_langevin_kick(LangevinArgs a) {
-  langevin_kick_body(a.n, a.dt, a.g, a.mass, a.f, a.v, a.seed, a.md, a.offset, blockIdx.x);
+  langevin_kick_body(a.n, a.dt, a.g, a.mass, a.f, a.v, a.seed, a.md, a.offset, a.zero_f, blockIdx.x);
 }
 
 // v *= factor (velocities carried to a new bath temperature)
@@ -301,7 +310,7 @@ __device__ inline void md_advance_body(PmMdDevice* md, int bx) {
 }
 __global__ void k_md_advance(AdvanceArgs a) { md_advance_body(a.md, blockIdx.x); }
 
-struct DriftArgs { int n; double h; const double* v; double* x; PmDispCheck chk; };
+struct DriftArgs { int n; double h; const double* v; double* x; PmStepWork w; };
 __device__ inline void drift_body(const DriftArgs& a, int bx) {
   int i = bx * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -311,7 +320,7 @@ __device__ inline void drift_body(const DriftArgs& a, int bx) {
     xn[k] = a.x[3 * i + k] + a.h * a.v[3 * i + k];   // Vector3DBlock::intoWeightedAdd
     a.x[3 * i + k] = xn[k];
   }
-  if (a.chk.flag && pm_moved_past(a.chk.to_fix, a.chk.fix_to_A, xn, a.chk.x_build[i], a.chk.lim2)) *a.chk.flag = 1;
+  step_work(a.w, i, xn);
 }
 __global__ void k_drift(DriftArgs a) { drift_body(a, blockIdx.x); }
 
@@ -324,7 +333,7 @@ __global__ void kb_kick_drift(const KickArgs* __restrict__ t, const int* __restr
 __global__ void kb_langevin_kick(const LangevinArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const LangevinArgs& a = t[r];
-  langevin_kick_body(a.n, a.dt, a.g, a.mass, a.f, a.v, a.seed, a.md, a.offset, blockIdx.x - pre[r]);
+  langevin_kick_body(a.n, a.dt, a.g, a.mass, a.f, a.v, a.seed, a.md, a.offset, a.zero_f, blockIdx.x - pre[r]);
 }
 __global__ void kb_drift(const DriftArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
@@ -416,9 +425,9 @@ void pm_launch_fix_to_double(hipStream_t s, int n3, const long long* f, double* 
 void pm_launch_add_fix(hipStream_t s, int n3, const long long* f, double* out) {
   if (n3 > 0) hipLaunchKernelGGL(k_add_fix, grid_for(n3, 256), dim3(256), 0, s, n3, f, out);
 }
-void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, const long long* f,
-                    double* v, double* x, const PmDispCheck* chk) {
-  const KickArgs a{n, mode, h, mass, f, v, x, chk ? *chk : PmDispCheck{}};
+void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass, long long* f,
+                    double* v, double* x, const PmStepWork* w, int zero_f) {
+  const KickArgs a{n, mode, h, mass, f, v, x, w ? *w : PmStepWork{}, zero_f};
   if (pm_recording()) {
     pm_record(PM_REC_KICK_DRIFT, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
     return;
@@ -426,9 +435,9 @@ void pm_launch_kick(hipStream_t s, int n, int mode, double h, const double* mass
   if (n > 0) hipLaunchKernelGGL(k_kick_drift, grid_for(n, 256), dim3(256), 0, s, a);
 }
 void pm_launch_langevin_kick(hipStream_t s, int n, double dt_akma, double gamma_akma, const double* mass,
-                             const long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
-                             int offset) {
-  const LangevinArgs a{n, dt_akma, gamma_akma, mass, f, v, seed, md, offset};
+                             long long* f, double* v, unsigned long long seed, const PmMdDevice* md,
+                             int offset, int zero_f) {
+  const LangevinArgs a{n, dt_akma, gamma_akma, mass, f, v, seed, md, offset, zero_f};
   if (pm_recording()) {
     pm_record(PM_REC_LANGEVIN_KICK, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
     return;
@@ -446,8 +455,8 @@ void pm_launch_md_advance(hipStream_t s, PmMdDevice* md) {
   }
   hipLaunchKernelGGL(k_md_advance, dim3(1), dim3(64), 0, s, a);
 }
-void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmDispCheck* chk) {
-  const DriftArgs a{n, h, v, x, chk ? *chk : PmDispCheck{}};
+void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmStepWork* w) {
+  const DriftArgs a{n, h, v, x, w ? *w : PmStepWork{}};
   if (pm_recording()) {
     pm_record(PM_REC_DRIFT, 0, n > 0 ? grid_for(n, 256).x : 0, 256, 0, &a, sizeof(a));
     return;

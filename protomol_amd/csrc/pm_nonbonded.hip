@@ -7,7 +7,7 @@
 // every unordered pair within the switching cutoff and add both atoms' forces
 // as 64-bit fixed-point integers (order-independent, deterministic run to run).
 #include "pm_internal.h"
-#include "pm_disp.h"
+#include "pm_step.h"
 
 #include <math.h>
 #include <type_traits>
@@ -15,54 +15,12 @@
 namespace {
 
 // ------------------------------------------------------------------ binning
-// PeriodicBoundaryConditions::minimalPosition (topology/PeriodicBoundaryConditions.h:120-160)
-__device__ inline void min_pos(const PmBox& b, double cx, double cy, double cz, double& ox,
-                               double& oy, double& oz) {
-#pragma clang fp contract(off)
-  double x = cx - b.origin[0], y = cy - b.origin[1], z = cz - b.origin[2];
-  if (x * x + y * y + z * z <= b.d) { ox = x; oy = y; oz = z; return; }
-  if (x < -b.h2[0]) x += b.h[0]; else if (x > b.h2[0]) x -= b.h[0];
-  if (y < -b.h2[1]) y += b.h[1]; else if (y > b.h2[1]) y -= b.h[1];
-  if (z < -b.h2[2]) z += b.h[2]; else if (z > b.h2[2]) z -= b.h[2];
-  if (x * x + y * y + z * z <= b.d) { ox = x; oy = y; oz = z; return; }
-  double tx = b.L[0] * __builtin_rint(b.er[0] * x);
-  double ty = b.L[1] * __builtin_rint(b.er[1] * y);
-  double tz = b.L[2] * __builtin_rint(b.er[2] * z);
-  ox = x - tx; oy = y - ty; oz = z - tz;
-}
-
-// CubicCellManager::findCell (topology/CubicCellManager.h:61-64); coordinates
-// outside [0, n) (the reference's x == +L/2 edge, or non-finite input) are
-// clamped into the grid so that no index leaves its buffer.
-__device__ inline int cell_index(const PmBox& b, double px, double py, double pz, double* mpz) {
-#pragma clang fp contract(off)
-  double mx, my, mz;
-  min_pos(b, px, py, pz, mx, my, mz);
-  double v[3] = {b.delta[0] + mx, b.delta[1] + my, b.delta[2] + mz};
-  int c[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    double f = __builtin_floor(v[k] * b.rreal[k]);
-    int ck;
-    if (!(f >= 0.0)) ck = 0;
-    else if (f >= (double)b.n[k]) ck = b.n[k] - 1;
-    else ck = (int)f;
-    c[k] = ck;
-  }
-  if (mpz) *mpz = mz;
-  return (c[0] * b.n[1] + c[1]) * b.n[2] + c[2];
-}
-
 __device__ inline void bin_body(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
                       int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key,
                       const int* __restrict__ rebuild, int bx) {
   int i = bx * blockDim.x + threadIdx.x;
   if (i >= n || (rebuild && *rebuild == 0)) return;
-  double mz;
-  int c = cell_index(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &mz);
-  cell_of[i] = c;
-  key[i] = (float)mz;
-  rank[i] = atomicAdd(&cell_count[c], 1);   // arrival rank; k_fill_slots orders by (z, index)
+  pm_bin_atom(box, i, pos + 3 * i, cell_of, key, rank, cell_count);
 }
 __global__ void k_bin(PmBox box, int n, const double* __restrict__ pos, int* __restrict__ cell_of,
                       int* __restrict__ cell_count, int* __restrict__ rank, float* __restrict__ key,
@@ -78,11 +36,16 @@ __global__ void kb_bin(const BinArgs* __restrict__ t, const int* __restrict__ pr
 // Exclusive scans of the cell counts (atoms and M-padded slots) in one
 // workgroup, 8192 cells per pass staged in LDS; also assigns every cluster
 // its cell.
-__device__ inline void scan_cells_body(int ncells, const int* __restrict__ cnt,
+__device__ inline void scan_cells_body(int ncells, int* __restrict__ cnt, int* __restrict__ cnt_out,
                                                      int* __restrict__ start, int* __restrict__ slot,
                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
                                                      int ncl_max, int m_cl, const int* __restrict__ rebuild, int bx) {
-  if (rebuild && *rebuild == 0) return;   // uniform over the block: before any barrier
+  if (rebuild && *rebuild == 0) {
+    // uniform over the block: before any barrier.  The binning counts start
+    // from zero at the next binning (the MD position update bins every step)
+    for (int c = threadIdx.x; c < ncells; c += blockDim.x) cnt[c] = 0;
+    return;
+  }
   __shared__ int s_c[8192];
   __shared__ int s_wa[16], s_ws[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -96,7 +59,11 @@ __device__ inline void scan_cells_body(int ncells, const int* __restrict__ cnt,
       for (int k = 0; k < 8; k++) tmp[k] = cnt[base + min(t + 1024 * k, m - 1)];
 #pragma unroll
       for (int k = 0; k < 8; k++)
-        if (t + 1024 * k < m) s_c[t + 1024 * k] = tmp[k];
+        if (t + 1024 * k < m) {
+          s_c[t + 1024 * k] = tmp[k];
+          cnt_out[base + t + 1024 * k] = tmp[k];   // the layout's cell counts
+          cnt[base + t + 1024 * k] = 0;            // binning counts cleared for the next binning
+        }
     }
     __syncthreads();
     int v[8];
@@ -142,15 +109,15 @@ __device__ inline void scan_cells_body(int ncells, const int* __restrict__ cnt,
   }
   if (t == 0) { counts[0] = carry_s; counts[1] = carry_s / m_cl; counts[2] = carry_a; }
 }
-__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, const int* __restrict__ cnt,
+__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, int* __restrict__ cnt, int* __restrict__ cnt_out,
                                                      int* __restrict__ start, int* __restrict__ slot,
                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
-                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) { scan_cells_body(ncells, cnt, start, slot, counts, cluster_cell, ncl_max, m_cl, rebuild, blockIdx.x); }
-struct ScanCellsArgs { int ncells; const int* cnt; int* start; int* slot; int* counts; int* cluster_cell; int ncl_max; int m_cl; const int* rebuild; };
+                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) { scan_cells_body(ncells, cnt, cnt_out, start, slot, counts, cluster_cell, ncl_max, m_cl, rebuild, blockIdx.x); }
+struct ScanCellsArgs { int ncells; int* cnt; int* cnt_out; int* start; int* slot; int* counts; int* cluster_cell; int ncl_max; int m_cl; const int* rebuild; };
 __global__ __launch_bounds__(1024) void kb_scan_cells(const ScanCellsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const ScanCellsArgs& q = t[r];
-  scan_cells_body(q.ncells, q.cnt, q.start, q.slot, q.counts, q.cluster_cell, q.ncl_max, q.m_cl, q.rebuild, blockIdx.x - pre[r]);
+  scan_cells_body(q.ncells, q.cnt, q.cnt_out, q.start, q.slot, q.counts, q.cluster_cell, q.ncl_max, q.m_cl, q.rebuild, blockIdx.x - pre[r]);
 }
 
 
@@ -668,8 +635,8 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
 
 // slot forces of the pair kernels into the per-atom fixed-point forces
 __device__ inline void fold_slots_body(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix,
-                             int* __restrict__ flag, unsigned long long* __restrict__ builds, int bx) {
+                             long long* __restrict__ force_slot, long long* __restrict__ force_fix,
+                             int* __restrict__ flag, unsigned long long* __restrict__ builds, int clear, int bx) {
   if (flag && bx == 0 && threadIdx.x == 0) {
     // end of an evaluation that reuses lists (every reader of the flag ran
     // before): count a build, clear the flag
@@ -683,17 +650,24 @@ __device__ inline void fold_slots_body(const int* __restrict__ counts, const int
   force_fix[3LL * at] += force_slot[3LL * sl];
   force_fix[3LL * at + 1] += force_slot[3LL * sl + 1];
   force_fix[3LL * at + 2] += force_slot[3LL * sl + 2];
+  if (clear) {
+    // the next evaluation's pair kernels start from zero (only real slots
+    // ever receive pair forces)
+    force_slot[3LL * sl] = 0;
+    force_slot[3LL * sl + 1] = 0;
+    force_slot[3LL * sl + 2] = 0;
+  }
 }
 __global__ void k_fold_slots(const int* __restrict__ counts, const int* __restrict__ atom_of_slot,
-                             const long long* __restrict__ force_slot, long long* __restrict__ force_fix,
-                             int* __restrict__ flag, unsigned long long* __restrict__ builds) {
-  fold_slots_body(counts, atom_of_slot, force_slot, force_fix, flag, builds, blockIdx.x);
+                             long long* __restrict__ force_slot, long long* __restrict__ force_fix,
+                             int* __restrict__ flag, unsigned long long* __restrict__ builds, int clear) {
+  fold_slots_body(counts, atom_of_slot, force_slot, force_fix, flag, builds, clear, blockIdx.x);
 }
-struct FoldSlotsArgs { const int* counts; const int* atom_of_slot; const long long* force_slot; long long* force_fix; int* flag; unsigned long long* builds; };
+struct FoldSlotsArgs { const int* counts; const int* atom_of_slot; long long* force_slot; long long* force_fix; int* flag; unsigned long long* builds; int clear; };
 __global__ void kb_fold_slots(const FoldSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const FoldSlotsArgs& q = t[r];
-  fold_slots_body(q.counts, q.atom_of_slot, q.force_slot, q.force_fix, q.flag, q.builds, blockIdx.x - pre[r]);
+  fold_slots_body(q.counts, q.atom_of_slot, q.force_slot, q.force_fix, q.flag, q.builds, q.clear, blockIdx.x - pre[r]);
 }
 
 
@@ -1420,14 +1394,14 @@ void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, in
                        rebuild);
 }
 
-void pm_launch_scan_cells(hipStream_t s, int ncells, const int* cell_count, int* cell_start,
+void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_count, int* cell_start,
                           int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild) {
   if (pm_recording()) {
-    const ScanCellsArgs a{ncells, cell_count, cell_start, cell_slot, counts, cluster_cell, ncl_max, m, rebuild};
+    const ScanCellsArgs a{ncells, bin_count, cell_count, cell_start, cell_slot, counts, cluster_cell, ncl_max, m, rebuild};
     pm_record(PM_REC_SCAN_CELLS, 0, 1, 1024, 0, &a, sizeof(a));
     return;
   }
-  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
+  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, bin_count, cell_count, cell_start, cell_slot,
                      counts, cluster_cell, ncl_max, m, rebuild);
 }
 
@@ -1483,16 +1457,16 @@ void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build) {
 }
 
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
-                          const long long* force_slot, long long* force_fix, int* flag,
-                          unsigned long long* builds) {
+                          long long* force_slot, long long* force_fix, int* flag,
+                          unsigned long long* builds, int clear) {
   if (pm_recording()) {
-    const FoldSlotsArgs a{counts, atom_of_slot, force_slot, force_fix, flag, builds};
+    const FoldSlotsArgs a{counts, atom_of_slot, force_slot, force_fix, flag, builds, clear};
     pm_record(PM_REC_FOLD_SLOTS, 0, nslot_max > 0 ? (nslot_max + 255) / 256 : 0, 256, 0, &a, sizeof(a));
     return;
   }
   if (nslot_max > 0)
     hipLaunchKernelGGL(k_fold_slots, dim3((nslot_max + 255) / 256), dim3(256), 0, s, counts, atom_of_slot,
-                       force_slot, force_fix, flag, builds);
+                       force_slot, force_fix, flag, builds, clear);
 }
 
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m) {
