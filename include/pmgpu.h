@@ -162,6 +162,36 @@ int pm_md_run(pm_context* ctx, int nsteps);
  * evaluation (double[PM_ENERGY_COUNT] overwritten, may be NULL). */
 int pm_md_get_state(pm_context* ctx, double* x, double* v, double* energies);
 
+/* SHAKE and RATTLE bond constraints in pm_md_run, attached as the reference
+ * application attaches them (protomol.cpp:553-578): SHAKE
+ * (ModifierShake::doExecute, integrators/ModifierShake.cpp:21-103) after
+ * every drift, RATTLE (ModifierRattle::doExecute,
+ * integrators/ModifierRattle.cpp:20-85) after every step; with RATTLE a
+ * leapfrog step is half kick, RATTLE, half kick, drift, SHAKE
+ * (LeapfrogIntegrator.cpp:72-84).  The constraints are
+ * buildRattleShakeBondConstraintList's (topology/topologyutilities.cpp:782-865):
+ * every bond of pm_set_bonded at its rest length, then for every angle whose
+ * end atoms both weigh less than 5 the end atoms' distance from the two bonds'
+ * rest lengths and the rest angle.  A modifier is on when its epsilon > 0 and
+ * max_iter > 0 (the reference's shakeEpsilon / shakeMaxIter, rattleEpsilon /
+ * rattleMaxIter; protomol's defaults are 1e-5 and 30); 0 turns it off.  Call
+ * after pm_set_bonded; SHAKE's reference positions are the positions at this
+ * call, or at pm_md_set_state.  The degrees of freedom are not reduced here
+ * (the reference subtracts the constraint count, pm_md_constraint_count, from
+ * the temperature's degrees of freedom). */
+int pm_md_set_constraints(pm_context* ctx, double shake_eps, int shake_max_iter, double rattle_eps,
+                          int rattle_max_iter);
+int pm_md_constraint_count(pm_context* ctx, int* count);
+
+/* One SHAKE application outside MD (ModifierShake::doExecute): x and v are
+ * corrected in place, x_last holds the positions after the previous
+ * application (the caller keeps the corrected x as the next x_last); dt_fs is
+ * the timestep in fs.  *sweeps (may be null) = the sweeps run.  pm_rattle:
+ * one RATTLE application (ModifierRattle::doExecute) on v at positions x. */
+int pm_shake(pm_context* ctx, double* x, double* v, const double* x_last, double dt_fs, double eps, int max_iter,
+             int* sweeps);
+int pm_rattle(pm_context* ctx, const double* x, double* v, double dt_fs, double eps, int max_iter, int* sweeps);
+
 /* Choose this context's layout options for the MD steps set up with
  * pm_md_setup by timing them: cluster size 8 or 16 (PM_OPT_CLUSTER_SIZE)
  * crossed with pair-list skins 0, 0.3, 0.6 and 1.0 A (PM_OPT_LIST_SKIN),

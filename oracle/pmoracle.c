@@ -1392,3 +1392,147 @@ double orc_rem_delta(double ti, double tj, double ei, double ej) {
   /* rem.cpp:941-946 */
   return (1 / ORC_BOLTZMANN / ti - 1 / ORC_BOLTZMANN / tj) * (ej - ei);
 }
+
+/* ------------------------------------------------------ SHAKE / RATTLE */
+/* topologyutilities.cpp:782-865 */
+int orc_constraint_list(int n, const double* mass, int nb, const int* ba, const int* bb, const double* br0,
+                        int na, const int* aa, const int* ab, const int* ac, const double* at0, int* c1, int* c2,
+                        double* cr0) {
+  int m = 0;
+  for (int i = 0; i < nb; i++) {
+    c1[m] = ba[i];
+    c2[m] = bb[i];
+    cr0[m] = br0[i];
+    m++;
+  }
+  (void)n;
+  for (int i = 0; i < na; i++) {
+    const int a1 = aa[i], a2 = ab[i], a3 = ac[i];
+    if (!(mass[a1] < 5 && mass[a3] < 5)) continue;
+    const int p1a = a1 < a2 ? a1 : a2, p1b = a1 < a2 ? a2 : a1;
+    const int p2a = a2 < a3 ? a2 : a3, p2b = a2 < a3 ? a3 : a2;
+    int b1 = -1, b2 = -1;
+    /* atoms[a].mybonds: the bonds of atom a in bond order */
+    for (int j = 0; j < nb; j++) {
+      if (ba[j] != a1 && bb[j] != a1) continue;
+      const int s1a = ba[j] < bb[j] ? ba[j] : bb[j], s1b = ba[j] < bb[j] ? bb[j] : ba[j];
+      if (s1a != p1a || s1b != p1b) continue;
+      for (int k = 0; k < nb; k++) {
+        if (ba[k] != a3 && bb[k] != a3) continue;
+        const int s2a = ba[k] < bb[k] ? ba[k] : bb[k], s2b = ba[k] < bb[k] ? bb[k] : ba[k];
+        if (s2a != p2a || s2b != p2b) continue;
+        if (!(b1 > -1 || b2 > -1)) { b1 = j; b2 = k; }
+      }
+    }
+    if (b1 > -1 && b2 > -1) {
+      const double b = br0[b1], c = br0[b2], alpha = at0[i];
+      c1[m] = a1;
+      c2[m] = a3;
+      cr0[m] = sqrt(b * b + c * c - 2 * b * c * cos(alpha));
+      m++;
+    }
+  }
+  return m;
+}
+
+/* ModifierMetaShake::calcError (ModifierMetaShake.cpp:12-31) */
+static double shake_error(int m, const int* c1, const int* c2, const double* cr0, const double* x) {
+  double error = 0;
+  for (int i = 0; i < m; i++) {
+    const int a1 = c1[i], a2 = c2[i];
+    const double r0sq = cr0[i] * cr0[i];
+    const double dx = x[3 * a2] - x[3 * a1], dy = x[3 * a2 + 1] - x[3 * a1 + 1], dz = x[3 * a2 + 2] - x[3 * a1 + 2];
+    const double err = fabs((dx * dx + dy * dy + dz * dz) - r0sq) / (2.0 * r0sq);
+    error += err;
+  }
+  return error /= m;
+}
+
+/* ModifierShake::doExecute (ModifierShake.cpp:21-103); the virial is not kept */
+int orc_shake(int m, const int* c1, const int* c2, const double* cr0, const double* mass, double dt_fs,
+              double eps, int max_iter, double* x, double* v, double* x_last) {
+  double error = shake_error(m, c1, c2, cr0, x);
+  const double dt = dt_fs / 48.88821290839616;   /* Constant::TIMEFACTOR */
+  int iter = 0;
+  while (error > eps) {
+    for (int i = 0; i < m; i++) {
+      const int a1 = c1[i], a2 = c2[i];
+      const double restLength = cr0[i];
+      double pab[3], rab[3], dp[3];
+      for (int d = 0; d < 3; d++) pab[d] = x[3 * a1 + d] - x[3 * a2 + d];
+      const double pabsq = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2];
+      const double rabsq = restLength * restLength;
+      const double diffsq = rabsq - pabsq;
+      for (int d = 0; d < 3; d++) rab[d] = x_last[3 * a1 + d] - x_last[3 * a2 + d];
+      const double rpab = rab[0] * pab[0] + rab[1] * pab[1] + rab[2] * pab[2];
+      const double rM1 = 1 / mass[a1], rM2 = 1 / mass[a2];
+      const double gab = diffsq / (2 * (rM1 + rM2) * rpab);
+      for (int d = 0; d < 3; d++) dp[d] = rab[d] * gab;
+      for (int d = 0; d < 3; d++) {
+        x[3 * a1 + d] += dp[d] * rM1;
+        x[3 * a2 + d] -= dp[d] * rM2;
+      }
+      for (int d = 0; d < 3; d++) dp[d] /= dt;
+      for (int d = 0; d < 3; d++) {
+        v[3 * a1 + d] += dp[d] * rM1;
+        v[3 * a2 + d] -= dp[d] * rM2;
+      }
+    }
+    error = shake_error(m, c1, c2, cr0, x);
+    iter++;
+    if (iter > max_iter) break;
+  }
+  /* myLastPositions = positions (every atom; only constrained ones are read) */
+  for (int i = 0; i < m; i++) {
+    for (int d = 0; d < 3; d++) {
+      x_last[3 * c1[i] + d] = x[3 * c1[i] + d];
+      x_last[3 * c2[i] + d] = x[3 * c2[i] + d];
+    }
+  }
+  return iter;
+}
+
+/* ModifierMetaRattle::calcError (ModifierMetaRattle.cpp:12-31) */
+static double rattle_error(int m, const int* c1, const int* c2, const double* x, const double* v) {
+  double error = 0;
+  for (int i = 0; i < m; i++) {
+    const int a1 = c1[i], a2 = c2[i];
+    double vab[3], rab[3];
+    for (int d = 0; d < 3; d++) {
+      vab[d] = v[3 * a1 + d] - v[3 * a2 + d];
+      rab[d] = x[3 * a1 + d] - x[3 * a2 + d];
+    }
+    const double err = fabs(rab[0] * vab[0] + rab[1] * vab[1] + rab[2] * vab[2]);
+    error += err;
+  }
+  return error /= m;
+}
+
+/* ModifierRattle::doExecute (ModifierRattle.cpp:20-85); the virial is not kept */
+int orc_rattle(int m, const int* c1, const int* c2, const double* mass, double dt_fs, double eps, int max_iter,
+               const double* x, double* v) {
+  double error = rattle_error(m, c1, c2, x, v);
+  const double dt = dt_fs / 48.88821290839616;
+  int iter = 0;
+  while (error > eps) {
+    for (int i = 0; i < m; i++) {
+      const int a1 = c1[i], a2 = c2[i];
+      const double rM1 = 1 / mass[a1], rM2 = 1 / mass[a2];
+      double rab[3], vab[3], dp[3];
+      for (int d = 0; d < 3; d++) rab[d] = x[3 * a1 + d] - x[3 * a2 + d];
+      const double rabsq = rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2];
+      for (int d = 0; d < 3; d++) vab[d] = v[3 * a1 + d] - v[3 * a2 + d];
+      const double rvab = rab[0] * vab[0] + rab[1] * vab[1] + rab[2] * vab[2];
+      const double gab = -rvab / (dt * (rM1 + rM2) * rabsq);
+      for (int d = 0; d < 3; d++) dp[d] = rab[d] * gab;
+      for (int d = 0; d < 3; d++) {
+        v[3 * a1 + d] += dp[d] * dt * rM1;
+        v[3 * a2 + d] -= dp[d] * dt * rM2;
+      }
+    }
+    error = rattle_error(m, c1, c2, x, v);
+    iter++;
+    if (iter > max_iter) break;
+  }
+  return iter;
+}

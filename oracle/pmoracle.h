@@ -24,6 +24,11 @@
  *   orc_hessian_pairs                          -> central finite differences
  *       of orc_cutoff forces; compiled reference Hessian (Hessian.cpp)
  *   orc_rem_delta                              -> closed form
+ *   orc_constraint_list                        -> law of cosines on the TIP3
+ *       geometry, bond list order (tests/test_oracle_constraints.py)
+ *   orc_shake / orc_rattle                     -> the converged SHAKE and
+ *       RATTLE solutions of one molecule solved independently (Newton on the
+ *       three multipliers; the 3x3 linear system), error bound after return
  */
 #ifndef PMORACLE_H
 #define PMORACLE_H
@@ -164,6 +169,24 @@ void orc_bonded(int n, const double* pos, const double* box, const double* origi
 
 /* Replica-exchange acceptance exponent (applications/rem-app/rem.cpp:941-946). */
 double orc_rem_delta(double ti, double tj, double ei, double ej);
+
+/* buildRattleShakeBondConstraintList (topology/topologyutilities.cpp:782-865):
+ * every bond at its rest length, then for every angle whose end atoms both
+ * weigh less than 5 the distance sqrt(b^2 + c^2 - 2 b c cos(theta0)) of the
+ * first pair of bonds joining them to the vertex.  c1, c2, cr0 hold room for
+ * nb + na entries; returns the constraint count. */
+int orc_constraint_list(int n, const double* mass, int nb, const int* ba, const int* bb, const double* br0,
+                        int na, const int* aa, const int* ab, const int* ac, const double* at0, int* c1, int* c2,
+                        double* cr0);
+/* ModifierShake::doExecute (integrators/ModifierShake.cpp:21-103) with
+ * ModifierMetaShake::calcError; x, v corrected, x_last set to x; returns the
+ * sweeps run.  dt_fs: the timestep in fs. */
+int orc_shake(int m, const int* c1, const int* c2, const double* cr0, const double* mass, double dt_fs,
+              double eps, int max_iter, double* x, double* v, double* x_last);
+/* ModifierRattle::doExecute (integrators/ModifierRattle.cpp:20-85) with
+ * ModifierMetaRattle::calcError; v corrected; returns the sweeps run. */
+int orc_rattle(int m, const int* c1, const int* c2, const double* mass, double dt_fs, double eps, int max_iter,
+               const double* x, double* v);
 
 /* OpenMP-parallel nonbonded cutoff force over cell packages with per-thread
  * force buffers (the multi-core CPU baseline; same pairs as orc_cutoff, the

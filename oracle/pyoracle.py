@@ -244,3 +244,52 @@ class Oracle:
         if r != 0:
             raise ValueError(f"{self.kind} hessian failed ({r})")
         return hv.reshape(-1, 3)
+
+    # ------------------------------------------------------ SHAKE / RATTLE
+    def constraint_list(self, sys):
+        """buildRattleShakeBondConstraintList: (atom1, atom2, rest length)."""
+        keep = []
+
+        def d(a):
+            a2, p = _d(a)
+            keep.append(a2)
+            return p
+
+        def i(a):
+            a2, p = _i(a)
+            keep.append(a2)
+            return p
+
+        nb, na = len(sys.bond_a), len(sys.angle_a)
+        c1 = np.zeros(nb + na, np.int32)
+        c2 = np.zeros(nb + na, np.int32)
+        r0 = np.zeros(nb + na)
+        m = self.lib.orc_constraint_list(sys.n, d(sys.mass), nb, i(sys.bond_a), i(sys.bond_b), d(sys.bond_r0), na,
+                                         i(sys.angle_a), i(sys.angle_b), i(sys.angle_c), d(sys.angle_t0),
+                                         c1.ctypes.data_as(_I), c2.ctypes.data_as(_I), r0.ctypes.data_as(_D))
+        return c1[:m], c2[:m], r0[:m]
+
+    def shake(self, cons, mass, x, v, x_last, dt_fs, eps, max_iter):
+        """ModifierShake::doExecute on copies: (x, v, x_last, sweeps)."""
+        c1, c2, r0 = (np.ascontiguousarray(cons[0], np.int32), np.ascontiguousarray(cons[1], np.int32),
+                      np.ascontiguousarray(cons[2], np.float64))
+        m = np.ascontiguousarray(mass, np.float64)
+        x = np.array(x, np.float64, order="C")
+        v = np.array(v, np.float64, order="C")
+        xl = np.array(x_last, np.float64, order="C")
+        self.lib.orc_shake.argtypes = [C.c_int, _I, _I, _D, _D, C.c_double, C.c_double, C.c_int, _D, _D, _D]
+        k = self.lib.orc_shake(len(c1), c1.ctypes.data_as(_I), c2.ctypes.data_as(_I), r0.ctypes.data_as(_D),
+                               m.ctypes.data_as(_D), dt_fs, eps, max_iter, x.ctypes.data_as(_D),
+                               v.ctypes.data_as(_D), xl.ctypes.data_as(_D))
+        return x, v, xl, k
+
+    def rattle(self, cons, mass, x, v, dt_fs, eps, max_iter):
+        """ModifierRattle::doExecute on a copy of v: (v, sweeps)."""
+        c1, c2 = np.ascontiguousarray(cons[0], np.int32), np.ascontiguousarray(cons[1], np.int32)
+        m = np.ascontiguousarray(mass, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.array(v, np.float64, order="C")
+        self.lib.orc_rattle.argtypes = [C.c_int, _I, _I, _D, C.c_double, C.c_double, C.c_int, _D, _D]
+        k = self.lib.orc_rattle(len(c1), c1.ctypes.data_as(_I), c2.ctypes.data_as(_I), m.ctypes.data_as(_D),
+                                dt_fs, eps, max_iter, x.ctypes.data_as(_D), v.ctypes.data_as(_D))
+        return v, k

@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmgpu.so")
-SOURCES = ["pm_nonbonded.hip", "pm_pme.hip", "pm_md.hip", "pm_bonded.hip", "pm_krylov.hip", "pm_api.cpp"]
+SOURCES = ["pm_nonbonded.hip", "pm_pme.hip", "pm_md.hip", "pm_bonded.hip", "pm_krylov.hip", "pm_shake.hip",
+           "pm_api.cpp"]
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

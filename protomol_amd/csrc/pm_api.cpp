@@ -45,6 +45,7 @@ const double kEpsilon = 1.0e-14;                 // base/pmconstants.cpp:45
 const int kMaxInt2 = 2147483647 / 2;             // base/pmconstants.cpp:41
 const double kPi = 3.14159265358979323846;
 const double kInvTimeFactor = 0.02045482828087295;
+const double kTimeFactor = 48.88821290839616;   // Constant::TIMEFACTOR (base/pmconstants.cpp:48)
 
 struct Term {
   int pot;      // 0 LJ, 1 Coulomb, 2 CoulombEwaldReal
@@ -111,6 +112,18 @@ struct pm_context {
   double *d_b_k = nullptr, *d_b_r0 = nullptr;
   int *d_a_a = nullptr, *d_a_b = nullptr, *d_a_c = nullptr, *d_a_row = nullptr, *d_a_idx = nullptr;
   double *d_a_k = nullptr, *d_a_t0 = nullptr, *d_a_kub = nullptr, *d_a_rub = nullptr;
+  std::vector<int> h_ba, h_bb, h_aa, h_ab, h_ac;   // host copies for the constraint list
+  std::vector<double> h_br0, h_at0;
+
+  // SHAKE / RATTLE (pm_md_set_constraints, pm_shake, pm_rattle)
+  bool cons_built = false;
+  int cons_n = 0, cons_ngroup = 0, cons_err_cap = 0;
+  int *d_cons_gstart = nullptr, *d_cons_a1 = nullptr, *d_cons_a2 = nullptr, *d_cons_sweeps = nullptr;
+  double *d_cons_r0 = nullptr, *d_rmass = nullptr, *d_cons_err = nullptr, *d_cons_mean = nullptr;
+  double *d_cons_scratch = nullptr, *d_shake_last = nullptr;
+  double *d_cons_x = nullptr, *d_cons_v = nullptr, *d_cons_xl = nullptr;   // pm_shake / pm_rattle staging
+  double shake_eps = 0.0, rattle_eps = 0.0;
+  int shake_maxit = 0, rattle_maxit = 0;
 
   // device topology
   double *d_q = nullptr, *d_mass = nullptr;
@@ -1303,6 +1316,109 @@ int list_stats(pm_context* c) {
   return PM_OK;
 }
 
+// The SHAKE/RATTLE constraint list of buildRattleShakeBondConstraintList
+// (topology/topologyutilities.cpp:782-865): every bond at its rest length,
+// then for every angle whose end atoms both weigh less than 5 the end-to-end
+// distance sqrt(b^2 + c^2 - 2 b c cos(theta0)) of the first pair of bonds
+// (a1's bonds in bond order, then a3's) joining the end atoms to the vertex.
+// The constraints are then ordered by connected group of atoms (groups in
+// order of their first constraint), each group's in list order, and the
+// group arrays uploaded.
+int build_constraints(pm_context* c) {
+  if (c->cons_built) return PM_OK;
+  const int n = c->n, nb = (int)c->h_ba.size(), na = (int)c->h_aa.size();
+  std::vector<int> ca1, ca2;
+  std::vector<double> cr0;
+  for (int k = 0; k < nb; k++) { ca1.push_back(c->h_ba[k]); ca2.push_back(c->h_bb[k]); cr0.push_back(c->h_br0[k]); }
+  std::vector<std::vector<int>> mybonds(n);
+  for (int k = 0; k < nb; k++) { mybonds[c->h_ba[k]].push_back(k); mybonds[c->h_bb[k]].push_back(k); }
+  auto sorted = [](int a, int b) { return a < b ? std::make_pair(a, b) : std::make_pair(b, a); };
+  for (int i = 0; i < na; i++) {
+    const int a1 = c->h_aa[i], a2 = c->h_ab[i], a3 = c->h_ac[i];
+    if (!(c->mass[a1] < 5 && c->mass[a3] < 5)) continue;
+    const auto p1 = sorted(a1, a2), p2 = sorted(a2, a3);
+    int b1 = -1, b2 = -1;
+    for (int j : mybonds[a1]) {
+      if (sorted(c->h_ba[j], c->h_bb[j]) != p1) continue;
+      for (int k : mybonds[a3]) {
+        if (sorted(c->h_ba[k], c->h_bb[k]) != p2) continue;
+        if (b1 < 0 && b2 < 0) { b1 = j; b2 = k; }
+      }
+    }
+    if (b1 > -1 && b2 > -1) {
+      const double b = c->h_br0[b1], cc = c->h_br0[b2], alpha = c->h_at0[i];
+      ca1.push_back(a1); ca2.push_back(a3); cr0.push_back(std::sqrt(b * b + cc * cc - 2 * b * cc * std::cos(alpha)));
+    }
+  }
+  const int m = (int)ca1.size();
+  // connected groups (union-find over the constraint graph)
+  std::vector<int> par(n);
+  for (int i = 0; i < n; i++) par[i] = i;
+  auto root = [&](int x) { while (par[x] != x) { par[x] = par[par[x]]; x = par[x]; } return x; };
+  for (int k = 0; k < m; k++) { const int ra = root(ca1[k]), rb = root(ca2[k]); if (ra != rb) par[ra] = rb; }
+  std::vector<int> gid(n, -1), grp(m);
+  int ng = 0;
+  for (int k = 0; k < m; k++) {
+    const int r = root(ca1[k]);
+    if (gid[r] < 0) gid[r] = ng++;
+    grp[k] = gid[r];
+  }
+  std::vector<int> gstart(ng + 1, 0), a1s(m), a2s(m);
+  std::vector<double> r0s(m);
+  for (int k = 0; k < m; k++) gstart[grp[k] + 1]++;
+  for (int g = 0; g < ng; g++) gstart[g + 1] += gstart[g];
+  {
+    std::vector<int> fill(gstart.begin(), gstart.end() - 1);
+    for (int k = 0; k < m; k++) { const int at = fill[grp[k]]++; a1s[at] = ca1[k]; a2s[at] = ca2[k]; r0s[at] = cr0[k]; }
+  }
+  std::vector<double> rm(n);
+  for (int i = 0; i < n; i++) rm[i] = 1 / c->mass[i];
+  int r;
+  if ((r = upload(c, &c->d_cons_gstart, gstart))) return r;
+  if ((r = upload(c, &c->d_cons_a1, a1s))) return r;
+  if ((r = upload(c, &c->d_cons_a2, a2s))) return r;
+  if ((r = upload(c, &c->d_cons_r0, r0s))) return r;
+  if ((r = upload(c, &c->d_rmass, rm))) return r;
+  if (!c->d_cons_sweeps && (r = dalloc(c, &c->d_cons_sweeps, 1))) return r;
+  if ((r = dalloc(c, &c->d_cons_scratch, 3 * (size_t)n))) return r;
+  if ((r = dalloc(c, &c->d_shake_last, 3 * (size_t)n))) return r;
+  c->cons_n = m;
+  c->cons_ngroup = ng;
+  c->cons_err_cap = 0;
+  c->cons_built = true;
+  return PM_OK;
+}
+
+// per-sweep error buffers for up to max_iter + 1 sweep counts
+int ensure_constraint_err(pm_context* c, int max_iter) {
+  if (max_iter + 1 <= c->cons_err_cap && c->d_cons_err) return PM_OK;
+  int r;
+  const int cap = max_iter + 1;
+  if ((r = dalloc(c, &c->d_cons_err, (size_t)cap * std::max(c->cons_ngroup, 1)))) return r;
+  if ((r = dalloc(c, &c->d_cons_mean, (size_t)cap))) return r;
+  c->cons_err_cap = cap;
+  return PM_OK;
+}
+
+PmConstraints constraint_launch(pm_context* c, double eps, int max_iter, double dt_fs) {
+  PmConstraints k{};
+  k.ngroup = c->cons_ngroup;
+  k.ncons = c->cons_n;
+  k.maxit = max_iter;
+  k.gstart = c->d_cons_gstart;
+  k.a1 = c->d_cons_a1;
+  k.a2 = c->d_cons_a2;
+  k.r0 = c->d_cons_r0;
+  k.rmass = c->d_rmass;
+  k.eps = eps;
+  k.dt = dt_fs / kTimeFactor;   // getTimestep() / Constant::TIMEFACTOR (ModifierShake.cpp:27)
+  k.err = c->d_cons_err;
+  k.mean = c->d_cons_mean;
+  k.sweeps = c->d_cons_sweeps;
+  k.scratch = c->d_cons_scratch;
+  return k;
+}
+
 void free_all(pm_context* c) {
   dfree(c->d_q); dfree(c->d_mass); dfree(c->d_type); dfree(c->d_mol);
   dfree(c->d_ex_row); dfree(c->d_ex_partner); dfree(c->d_ex_cls);
@@ -1329,6 +1445,12 @@ void free_all(pm_context* c) {
   dfree(c->d_a_a); dfree(c->d_a_b); dfree(c->d_a_c); dfree(c->d_a_row); dfree(c->d_a_idx);
   dfree(c->d_a_k); dfree(c->d_a_t0); dfree(c->d_a_kub); dfree(c->d_a_rub);
   c->nbonds = c->nangles = 0;
+  dfree(c->d_cons_gstart); dfree(c->d_cons_a1); dfree(c->d_cons_a2); dfree(c->d_cons_sweeps);
+  dfree(c->d_cons_r0); dfree(c->d_rmass); dfree(c->d_cons_err); dfree(c->d_cons_mean);
+  dfree(c->d_cons_scratch); dfree(c->d_shake_last);
+  dfree(c->d_cons_x); dfree(c->d_cons_v); dfree(c->d_cons_xl);
+  c->cons_built = false;
+  c->cons_n = c->cons_ngroup = c->cons_err_cap = 0;
   free_stencils(c);
   c->cells_alloc = 0;
   c->jlist_alloc = 0;
@@ -1643,6 +1765,9 @@ int pm_set_bonded(pm_context* c, int nb, const int* ba, const int* bb, const dou
   if ((r = upload(c, &c->d_a_rub, vd(arub, na)))) return r;
   c->nbonds = nb;
   c->nangles = na;
+  c->h_ba = vi(ba, nb); c->h_bb = vi(bb, nb); c->h_br0 = vd(br0, nb);
+  c->h_aa = vi(aa, na); c->h_ab = vi(ab, na); c->h_ac = vi(ac, na); c->h_at0 = vd(at0, na);
+  c->cons_built = false;
   c->md_forces_valid = false;
   c->gen++;
   return PM_OK;
@@ -1849,8 +1974,93 @@ int pm_md_set_state(pm_context* c, const double* x, const double* v) {
   const size_t bytes = 3 * (size_t)c->n * sizeof(double);
   if (x) HIPCHECK(c, hipMemcpy(c->d_pos, x, bytes, hipMemcpyHostToDevice));
   if (v) HIPCHECK(c, hipMemcpy(c->d_vel, v, bytes, hipMemcpyHostToDevice));
+  // SHAKE's reference positions start at the new positions
+  if (x && c->d_shake_last) HIPCHECK(c, hipMemcpy(c->d_shake_last, x, bytes, hipMemcpyHostToDevice));
   c->md_forces_valid = false;
   return PM_OK;
+}
+
+int pm_md_set_constraints(pm_context* c, double shake_eps, int shake_max_iter, double rattle_eps,
+                          int rattle_max_iter) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if (shake_max_iter < 0 || rattle_max_iter < 0 || shake_max_iter > 100000 || rattle_max_iter > 100000 ||
+      !(shake_eps == shake_eps) || !(rattle_eps == rattle_eps))
+    return fail(c, PM_ERR_ARGUMENT, "constraint settings: epsilon must be a number, max iterations 0..100000");
+  // a modifier is attached only with epsilon > 0 and maxIter > 0 (protomol.cpp:553-578)
+  const bool shake = shake_eps > 0.0 && shake_max_iter > 0, rattle = rattle_eps > 0.0 && rattle_max_iter > 0;
+  c->shake_eps = shake ? shake_eps : 0.0;
+  c->shake_maxit = shake ? shake_max_iter : 0;
+  c->rattle_eps = rattle ? rattle_eps : 0.0;
+  c->rattle_maxit = rattle ? rattle_max_iter : 0;
+  if (shake || rattle) {
+    if ((r = build_constraints(c))) return r;
+    if ((r = ensure_constraint_err(c, std::max(c->shake_maxit, c->rattle_maxit)))) return r;
+    // myLastPositions = the positions when the modifier is initialized
+    // (ModifierMetaRattleShake.cpp:17)
+    if (c->d_pos) HIPCHECK(c, hipMemcpy(c->d_shake_last, c->d_pos, 3 * (size_t)c->n * sizeof(double),
+                                        hipMemcpyDeviceToDevice));
+  }
+  c->gen++;   // captured steps are stale
+  return PM_OK;
+}
+
+int pm_md_constraint_count(pm_context* c, int* count) {
+  if (!c || !count) return PM_ERR_ARGUMENT;
+  int r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if ((r = set_device(c))) return r;
+  if ((r = build_constraints(c))) return r;
+  *count = c->cons_n;
+  return PM_OK;
+}
+
+// one SHAKE (rattle 0) or RATTLE (rattle 1) application on host arrays
+static int constraint_call(pm_context* c, int rattle, const double* x_in, double* x_out, double* v,
+                           const double* x_last, double dt_fs, double eps, int max_iter, int* sweeps) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if (!x_in || !v || (!rattle && (!x_last || !x_out)))
+    return fail(c, PM_ERR_ARGUMENT, "constraints: null positions, velocities or reference positions");
+  if (!(dt_fs > 0.0) || !(eps > 0.0) || max_iter <= 0 || max_iter > 100000)
+    return fail(c, PM_ERR_ARGUMENT, "constraints: timestep and epsilon must be > 0, max iterations 1..100000");
+  if ((r = build_constraints(c))) return r;
+  if ((r = ensure_constraint_err(c, max_iter))) return r;
+  const size_t n3 = 3 * (size_t)c->n, bytes = n3 * sizeof(double);
+  if (!c->d_cons_x) {
+    if ((r = dalloc(c, &c->d_cons_x, n3))) return r;
+    if ((r = dalloc(c, &c->d_cons_v, n3))) return r;
+    if ((r = dalloc(c, &c->d_cons_xl, n3))) return r;
+  }
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync(c->d_cons_x, x_in, bytes, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(c->d_cons_v, v, bytes, hipMemcpyHostToDevice, s));
+  if (!rattle) HIPCHECK(c, hipMemcpyAsync(c->d_cons_xl, x_last, bytes, hipMemcpyHostToDevice, s));
+  int sw = 0;
+  if (c->cons_ngroup > 0) {
+    PmConstraints k = constraint_launch(c, eps, max_iter, dt_fs);
+    pm_launch_constraints(s, k, rattle, c->d_cons_x, c->d_cons_v, c->d_cons_xl);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(&sw, c->d_cons_sweeps, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  if (!rattle) HIPCHECK(c, hipMemcpyAsync(x_out, c->d_cons_x, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(v, c->d_cons_v, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipStreamSynchronize(s));
+  if (sweeps) *sweeps = sw;
+  return PM_OK;
+}
+
+int pm_shake(pm_context* c, double* x, double* v, const double* x_last, double dt_fs, double eps, int max_iter,
+             int* sweeps) {
+  return constraint_call(c, 0, x, x, v, x_last, dt_fs, eps, max_iter, sweeps);
+}
+
+int pm_rattle(pm_context* c, const double* x, double* v, double dt_fs, double eps, int max_iter, int* sweeps) {
+  return constraint_call(c, 1, x, nullptr, v, nullptr, dt_fs, eps, max_iter, sweeps);
 }
 
 }  // extern "C"
@@ -1861,6 +2071,20 @@ namespace {
 // streams: kind 0 = a middle leapfrog step (LeapfrogIntegrator::doKickdoDrift,
 // then the forces), kind 1 = a whole Langevin step (StandardIntegrator::run:
 // half kick, drift, forces, half kick).
+// SHAKE / RATTLE in MD steps (pm_md_set_constraints)
+bool md_shake(const pm_context* c) { return c->shake_maxit > 0 && c->cons_built && c->cons_ngroup > 0; }
+bool md_rattle(const pm_context* c) { return c->rattle_maxit > 0 && c->cons_built && c->cons_ngroup > 0; }
+void md_shake_step(pm_context* c) {
+  PmConstraints k = constraint_launch(c, c->shake_eps, c->shake_maxit, c->md_dt);
+  k.sweeps = nullptr;
+  pm_launch_constraints(c->stream, k, 0, c->d_pos, c->d_vel, c->d_shake_last);
+}
+void md_rattle_step(pm_context* c) {
+  PmConstraints k = constraint_launch(c, c->rattle_eps, c->rattle_maxit, c->md_dt);
+  k.sweeps = nullptr;
+  pm_launch_constraints(c->stream, k, 1, c->d_pos, c->d_vel, c->d_shake_last);
+}
+
 // An evaluation after a position update with PmStepWork failed: the forces
 // were cleared and the binning counts may hold a binning that no scan
 // consumed; the next run starts from a fresh force evaluation and zero counts.
@@ -1877,11 +2101,26 @@ int md_unit(pm_context* c, int kind, bool energy) {
   const double h = c->md_dt * kInvTimeFactor;
   int r;
   PmStepWork w{};
+  const bool shake = md_shake(c), rattle = md_rattle(c);
   if (kind == 0) {
     {
       TimedScope ts(c, PM_T_INTEGRATE);
-      const bool sw = md_step_work(c, &w);
-      pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, sw ? &w : nullptr, sw ? 1 : 0);
+      if (rattle) {
+        // LeapfrogIntegrator::doKickdoDrift with a post-step modifier
+        // (LeapfrogIntegrator.cpp:72-84): half kick, RATTLE, half kick, drift
+        // (StandardIntegrator::doHalfKick, STSIntegrator::doDrift), SHAKE
+        const double hh = 0.5 * c->md_dt * kInvTimeFactor;
+        pm_launch_kick(s, n, 2, hh, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+        md_rattle_step(c);
+        pm_launch_kick(s, n, 2, hh, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+        pm_launch_drift(s, n, h, c->d_vel, c->d_pos);
+      } else {
+        // the displacement test and binning need the final positions: not
+        // in the update when SHAKE moves them afterwards
+        const bool sw = !shake && md_step_work(c, &w);
+        pm_launch_kick(s, n, 1, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, sw ? &w : nullptr, sw ? 1 : 0);
+      }
+      if (shake) md_shake_step(c);
     }
     if ((r = evaluate_group(c, c->md_handles, 0, energy))) md_step_failed(c);
     return r;
@@ -1889,9 +2128,10 @@ int md_unit(pm_context* c, int kind, bool energy) {
   const double g = c->md_gamma / (1000 * kInvTimeFactor);
   {
     TimedScope ts(c, PM_T_INTEGRATE);
-    const bool sw = md_step_work(c, &w);
+    const bool sw = !shake && md_step_work(c, &w);
     pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 0, sw ? 1 : 0);
     pm_launch_drift(s, n, h, c->d_vel, c->d_pos, sw ? &w : nullptr);
+    if (shake) md_shake_step(c);   // postDriftOrNextModify (STSIntegrator.cpp:42-47)
   }
   if ((r = evaluate_group(c, c->md_handles, 0, energy))) {
     md_step_failed(c);
@@ -1899,6 +2139,7 @@ int md_unit(pm_context* c, int kind, bool energy) {
   }
   TimedScope ts(c, PM_T_INTEGRATE);
   pm_launch_langevin_kick(s, n, h, g, c->d_mass, c->d_force, c->d_vel, c->md_seed, c->d_md, 1);
+  if (rattle) md_rattle_step(c);   // postStepModify (StandardIntegrator.cpp:28-34)
   pm_launch_md_advance(s, c->d_md);
   c->md_counter += 2;
   return PM_OK;
@@ -1980,8 +2221,10 @@ int pm_md_run(pm_context* c, int nsteps) {
     {
       TimedScope ts(c, PM_T_INTEGRATE);
       PmStepWork w{};
-      const bool sw = md_step_work(c, &w);
+      const bool shake = md_shake(c);
+      const bool sw = !shake && md_step_work(c, &w);
       pm_launch_kick(s, n, 0, h, c->d_mass, c->d_force, c->d_vel, c->d_pos, sw ? &w : nullptr, sw ? 1 : 0);
+      if (shake) md_shake_step(c);   // postDriftOrNextModify (LeapfrogIntegrator.cpp:66)
     }
     if ((r = evaluate_group(c, c->md_handles, 0, nsteps == 1))) {
       md_step_failed(c);
@@ -1990,6 +2233,7 @@ int pm_md_run(pm_context* c, int nsteps) {
     if (nsteps >= 3 && (r = md_units(c, 0, nsteps - 2))) return r;
     if (nsteps >= 2 && (r = md_unit(c, 0, true))) return r;
     pm_launch_kick(s, n, 2, 0.5 * c->md_dt * kInvTimeFactor, c->d_mass, c->d_force, c->d_vel, c->d_pos);
+    if (md_rattle(c)) md_rattle_step(c);   // postStepModify (LeapfrogIntegrator.cpp:119-120)
   } else {
     // StandardIntegrator::run with the Langevin half kick
     // (StandardIntegrator.cpp:24-35, LangevinLeapfrogIntegrator.cpp:50-79)

@@ -293,6 +293,28 @@ void pm_launch_scale(hipStream_t s, int n3, double factor, double* v);
 // x += h v for n atoms
 void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x, const PmStepWork* w = nullptr);
 
+// SHAKE / RATTLE bond constraints (pm_shake.hip): constraints ordered by
+// connected group (gstart), each group's in the reference's list order
+struct PmConstraints {
+  int ngroup;
+  int ncons;                 // constraints in total (the mean error's divisor)
+  int maxit;                 // maxIter: at most maxit + 1 sweeps
+  const int* gstart;         // [ngroup + 1]
+  const int* a1;
+  const int* a2;
+  const double* r0;          // rest lengths
+  const double* rmass;       // 1 / scaledMass per atom
+  double eps;
+  double dt;                 // timestep / TIMEFACTOR
+  double* err;               // [(maxit + 1) * ngroup] group errors after s sweeps
+  double* mean;              // [maxit + 1] mean error after s sweeps
+  int* sweeps;               // [1] sweeps run (may be null)
+  double* scratch;           // [3n] probe copy of positions (SHAKE) or velocities (RATTLE)
+};
+// SHAKE (rattle 0: x, v corrected, xl the reference positions, then set to
+// x) or RATTLE (rattle 1: v corrected); three launches
+bool pm_launch_constraints(hipStream_t s, const PmConstraints& c, int rattle, double* x, double* v, double* xl);
+
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation
 struct PmBondedLaunch {
   int n;

@@ -312,6 +312,7 @@ __global__ void k_md_advance(AdvanceArgs a) { md_advance_body(a.md, blockIdx.x);
 
 struct DriftArgs { int n; double h; const double* v; double* x; PmStepWork w; };
 __device__ inline void drift_body(const DriftArgs& a, int bx) {
+#pragma clang fp contract(off)
   int i = bx * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xn[3];

@@ -29,7 +29,8 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
-           "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info"]
+           "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info",
+           "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle"]
 OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
@@ -95,6 +96,10 @@ def load_library(path: str = LIB_PATH):
     lib.pm_batch_set_mode.argtypes = [P, C.c_int]
     lib.pm_batch_info.argtypes = [P, I, C.c_char_p, C.c_int]
     lib.pm_lanczos.argtypes = [P, I, C.c_int, D, D, C.c_int, C.c_int, C.c_int, D, D, D, D]
+    lib.pm_md_set_constraints.argtypes = [P, C.c_double, C.c_int, C.c_double, C.c_int]
+    lib.pm_md_constraint_count.argtypes = [P, I]
+    lib.pm_shake.argtypes = [P, D, D, D, C.c_double, C.c_double, C.c_int, I]
+    lib.pm_rattle.argtypes = [P, D, D, C.c_double, C.c_double, C.c_int, I]
     _lib = lib
     return lib
 
@@ -234,6 +239,35 @@ class Context:
         e = np.zeros(ENERGY_COUNT)
         self._check(self.lib.pm_md_get_state(self.h, _dp(x), _dp(v), _dp(e)))
         return x, v, e
+
+    def md_set_constraints(self, shake_eps=1e-5, shake_max_iter=30, rattle_eps=0.0, rattle_max_iter=0):
+        """SHAKE after every drift and RATTLE after every step of md_run (an
+        epsilon or max_iter of 0 turns a modifier off)."""
+        self._check(self.lib.pm_md_set_constraints(self.h, float(shake_eps), int(shake_max_iter),
+                                                   float(rattle_eps), int(rattle_max_iter)))
+
+    def constraint_count(self):
+        k = C.c_int()
+        self._check(self.lib.pm_md_constraint_count(self.h, C.byref(k)))
+        return k.value
+
+    def shake(self, x, v, x_last, dt_fs, eps=1e-5, max_iter=30):
+        """One SHAKE application: returns (x, v, sweeps) corrected copies."""
+        x = np.array(x, np.float64, order="C").reshape(-1, 3)
+        v = np.array(v, np.float64, order="C").reshape(-1, 3)
+        xl = np.ascontiguousarray(x_last, np.float64).reshape(-1, 3)
+        k = C.c_int()
+        self._check(self.lib.pm_shake(self.h, _dp(x), _dp(v), _dp(xl), float(dt_fs), float(eps), int(max_iter),
+                                      C.byref(k)))
+        return x, v, k.value
+
+    def rattle(self, x, v, dt_fs, eps=1e-5, max_iter=30):
+        """One RATTLE application: returns (v, sweeps), a corrected copy."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        v = np.array(v, np.float64, order="C").reshape(-1, 3)
+        k = C.c_int()
+        self._check(self.lib.pm_rattle(self.h, _dp(x), _dp(v), float(dt_fs), float(eps), int(max_iter), C.byref(k)))
+        return v, k.value
 
     def stats(self):
         s = np.zeros(5, np.int64)
