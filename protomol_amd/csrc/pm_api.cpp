@@ -119,7 +119,9 @@ struct pm_context {
   bool cons_built = false;
   int cons_n = 0, cons_ngroup = 0, cons_err_cap = 0;
   int *d_cons_gstart = nullptr, *d_cons_a1 = nullptr, *d_cons_a2 = nullptr, *d_cons_sweeps = nullptr;
-  double *d_cons_r0 = nullptr, *d_rmass = nullptr, *d_cons_err = nullptr, *d_cons_mean = nullptr;
+  int* d_cons_la = nullptr;
+  int4* d_cons_gatom = nullptr;
+  double *d_cons_r0 = nullptr, *d_rmass = nullptr, *d_cons_err = nullptr;
   double *d_cons_scratch = nullptr, *d_shake_last = nullptr;
   double *d_cons_x = nullptr, *d_cons_v = nullptr, *d_cons_xl = nullptr;   // pm_shake / pm_rattle staging
   double shake_eps = 0.0, rattle_eps = 0.0;
@@ -1371,10 +1373,37 @@ int build_constraints(pm_context* c) {
     std::vector<int> fill(gstart.begin(), gstart.end() - 1);
     for (int k = 0; k < m; k++) { const int at = fill[grp[k]]++; a1s[at] = ca1[k]; a2s[at] = ca2[k]; r0s[at] = cr0[k]; }
   }
+  // groups of at most 4 atoms and 6 constraints: their atoms (in order of
+  // appearance) and each constraint's atoms as local indices
+  std::vector<int4> gatom(std::max(ng, 1), make_int4(-1, -1, -1, -1));
+  std::vector<int> la(std::max(m, 1), 0);
+  for (int g = 0; g < ng; g++) {
+    int at[4] = {-1, -1, -1, -1}, na_ = 0;
+    bool small = true;
+    auto local = [&](int a) {
+      for (int i = 0; i < na_; i++) if (at[i] == a) return i;
+      if (na_ == 4) { small = false; return 0; }
+      at[na_] = a;
+      return na_++;
+    };
+    for (int k = gstart[g]; k < gstart[g + 1] && small; k++) {
+      const int l1 = local(a1s[k]), l2 = local(a2s[k]);
+      la[k] = l1 | (l2 << 2);
+    }
+    if (small && gstart[g + 1] - gstart[g] <= 6) {
+      gatom[g] = make_int4(at[0], at[1], at[2], at[3]);
+      const int k = gstart[g];
+      if (na_ == 3 && gstart[g + 1] - k == 3 && la[k] == (0 | 1 << 2) && la[k + 1] == (0 | 2 << 2) &&
+          la[k + 2] == (1 | 2 << 2))
+        gatom[g].w = PM_CONS_WATER;
+    }
+  }
   std::vector<double> rm(n);
   for (int i = 0; i < n; i++) rm[i] = 1 / c->mass[i];
   int r;
   if ((r = upload(c, &c->d_cons_gstart, gstart))) return r;
+  if ((r = upload(c, &c->d_cons_la, la))) return r;
+  if ((r = upload(c, &c->d_cons_gatom, gatom))) return r;
   if ((r = upload(c, &c->d_cons_a1, a1s))) return r;
   if ((r = upload(c, &c->d_cons_a2, a2s))) return r;
   if ((r = upload(c, &c->d_cons_r0, r0s))) return r;
@@ -1394,8 +1423,8 @@ int ensure_constraint_err(pm_context* c, int max_iter) {
   if (max_iter + 1 <= c->cons_err_cap && c->d_cons_err) return PM_OK;
   int r;
   const int cap = max_iter + 1;
-  if ((r = dalloc(c, &c->d_cons_err, (size_t)cap * std::max(c->cons_ngroup, 1)))) return r;
-  if ((r = dalloc(c, &c->d_cons_mean, (size_t)cap))) return r;
+  // wave sums: four waves per block of 256 groups
+  if ((r = dalloc(c, &c->d_cons_err, (size_t)cap * 4 * std::max((c->cons_ngroup + 255) / 256, 1)))) return r;
   c->cons_err_cap = cap;
   return PM_OK;
 }
@@ -1408,12 +1437,13 @@ PmConstraints constraint_launch(pm_context* c, double eps, int max_iter, double 
   k.gstart = c->d_cons_gstart;
   k.a1 = c->d_cons_a1;
   k.a2 = c->d_cons_a2;
+  k.la = c->d_cons_la;
+  k.gatom = c->d_cons_gatom;
   k.r0 = c->d_cons_r0;
   k.rmass = c->d_rmass;
   k.eps = eps;
   k.dt = dt_fs / kTimeFactor;   // getTimestep() / Constant::TIMEFACTOR (ModifierShake.cpp:27)
   k.err = c->d_cons_err;
-  k.mean = c->d_cons_mean;
   k.sweeps = c->d_cons_sweeps;
   k.scratch = c->d_cons_scratch;
   return k;
@@ -1446,7 +1476,7 @@ void free_all(pm_context* c) {
   dfree(c->d_a_k); dfree(c->d_a_t0); dfree(c->d_a_kub); dfree(c->d_a_rub);
   c->nbonds = c->nangles = 0;
   dfree(c->d_cons_gstart); dfree(c->d_cons_a1); dfree(c->d_cons_a2); dfree(c->d_cons_sweeps);
-  dfree(c->d_cons_r0); dfree(c->d_rmass); dfree(c->d_cons_err); dfree(c->d_cons_mean);
+  dfree(c->d_cons_r0); dfree(c->d_rmass); dfree(c->d_cons_err); dfree(c->d_cons_la); dfree(c->d_cons_gatom);
   dfree(c->d_cons_scratch); dfree(c->d_shake_last);
   dfree(c->d_cons_x); dfree(c->d_cons_v); dfree(c->d_cons_xl);
   c->cons_built = false;

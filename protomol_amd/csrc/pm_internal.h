@@ -295,6 +295,7 @@ void pm_launch_drift(hipStream_t s, int n, double h, const double* v, double* x,
 
 // SHAKE / RATTLE bond constraints (pm_shake.hip): constraints ordered by
 // connected group (gstart), each group's in the reference's list order
+#define PM_CONS_WATER (-2)
 struct PmConstraints {
   int ngroup;
   int ncons;                 // constraints in total (the mean error's divisor)
@@ -302,17 +303,21 @@ struct PmConstraints {
   const int* gstart;         // [ngroup + 1]
   const int* a1;
   const int* a2;
+  const int* la;             // small groups: local indices of a1, a2 (la1 | la2 << 2)
+  const int4* gatom;         // [ngroup] atoms of groups of at most 4 atoms and 6
+                             // constraints (-1 unused; w = PM_CONS_WATER: 3 atoms with
+                             // constraints (0,1), (0,2), (1,2) in that order); x = -1
+                             // for other groups
   const double* r0;          // rest lengths
   const double* rmass;       // 1 / scaledMass per atom
   double eps;
   double dt;                 // timestep / TIMEFACTOR
-  double* err;               // [(maxit + 1) * ngroup] group errors after s sweeps
-  double* mean;              // [maxit + 1] mean error after s sweeps
+  double* err;               // [(maxit + 1) * waves] wave sums of the errors after s sweeps
   int* sweeps;               // [1] sweeps run (may be null)
   double* scratch;           // [3n] probe copy of positions (SHAKE) or velocities (RATTLE)
 };
 // SHAKE (rattle 0: x, v corrected, xl the reference positions, then set to
-// x) or RATTLE (rattle 1: v corrected); three launches
+// x) or RATTLE (rattle 1: v corrected); two launches
 bool pm_launch_constraints(hipStream_t s, const PmConstraints& c, int rattle, double* x, double* v, double* xl);
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation
