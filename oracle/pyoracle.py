@@ -293,3 +293,81 @@ class Oracle:
         k = self.lib.orc_rattle(len(c1), c1.ctypes.data_as(_I), c2.ctypes.data_as(_I), m.ctypes.data_as(_D),
                                 dt_fs, eps, max_iter, x.ctypes.data_as(_D), v.ctypes.data_as(_D))
         return v, k
+
+
+# ---------------------------------------------------------------- topology
+_TOPO_INFO = ["atoms", "types", "bonds", "angles", "dihedrals", "dihedral_terms", "impropers",
+              "improper_terms", "molecules", "dof", "par_dialect", "dihedral_mult_psf"]
+
+
+def ref_topology(psf: str, par: str, exclude: str = "scaled1-4", dihedral_mult_psf: int = -1) -> dict:
+    """The reference's PSFReader + PARReader + buildTopology (compiled here)
+    on the two files: the arrays of the GenericTopology it builds."""
+    lib = Oracle("reference").lib
+    lib.ref_topo_load.restype = C.c_void_p
+    lib.ref_topo_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _I]
+    for f in ("ref_topo_free", "ref_topo_atoms", "ref_topo_lj", "ref_topo_bonds", "ref_topo_angles",
+              "ref_topo_torsions", "ref_topo_exclusion_class", "ref_topo_type_name", "ref_topo_remove_momentum"):
+        getattr(lib, f).argtypes = None
+    info_a = np.zeros(12, np.int32)
+    h = lib.ref_topo_load(psf.encode(), par.encode(), exclude.encode(), dihedral_mult_psf,
+                          info_a.ctypes.data_as(_I))
+    if not h:
+        raise RuntimeError("reference readers failed on %s / %s" % (psf, par))
+    h = C.c_void_p(h)
+    info = dict(zip(_TOPO_INFO, (int(x) for x in info_a)))
+    n, nt = info["atoms"], info["types"]
+    out = {"info": info, "handle": h, "lib": lib}
+    q, m = np.zeros(n), np.zeros(n)
+    t, mol, wat = (np.zeros(n, np.int32) for _ in range(3))
+    lib.ref_topo_atoms(h, _d(q)[1], _d(m)[1], _i(t)[1], _i(mol)[1], _i(wat)[1])
+    out.update(q=q, mass=m, type=t, molecule=mol, water=wat)
+    lj = [np.zeros((nt, nt)) for _ in range(4)]
+    lib.ref_topo_lj(h, *[x.ctypes.data_as(_D) for x in lj])
+    out.update(ljA=lj[0], ljB=lj[1], ljA14=lj[2], ljB14=lj[3])
+    nb, na = info["bonds"], info["angles"]
+    ba, bb = np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    bk, br = np.zeros(nb), np.zeros(nb)
+    lib.ref_topo_bonds(h, ba.ctypes.data_as(_I), bb.ctypes.data_as(_I), bk.ctypes.data_as(_D), br.ctypes.data_as(_D))
+    out.update(bond_a=ba, bond_b=bb, bond_k=bk, bond_r0=br)
+    aa, ab, ac = (np.zeros(na, np.int32) for _ in range(3))
+    ak, at, au, ar = (np.zeros(na) for _ in range(4))
+    lib.ref_topo_angles(h, *[x.ctypes.data_as(_I) for x in (aa, ab, ac)], *[x.ctypes.data_as(_D) for x in (ak, at, au, ar)])
+    out.update(angle_a=aa, angle_b=ab, angle_c=ac, angle_k=ak, angle_t0=at, angle_kub=au, angle_rub=ar)
+    for imp, (key, mk, uk) in enumerate([("dihedrals", "dihedrals", "dihedral_terms"),
+                                          ("impropers", "impropers", "improper_terms")]):
+        mm, uu = info[mk], info[uk]
+        atoms, row = np.zeros((mm, 4), np.int32), np.zeros(mm + 1, np.int32)
+        k, per, ph = np.zeros(uu), np.zeros(uu, np.int32), np.zeros(uu)
+        lib.ref_topo_torsions(h, imp, atoms.ctypes.data_as(_I), row.ctypes.data_as(_I), k.ctypes.data_as(_D),
+                              per.ctypes.data_as(_I), ph.ctypes.data_as(_D))
+        out[key] = dict(atoms=atoms, term_row=row, k=k, period=per, phase=ph)
+    names = []
+    buf = C.create_string_buffer(256)
+    for k in range(nt):
+        lib.ref_topo_type_name(h, k, buf, 256)
+        names.append(buf.value.decode())
+    out["type_names"] = names
+    return out
+
+
+def ref_exclusion_class(topo: dict, a, b) -> np.ndarray:
+    a, b = np.ascontiguousarray(a, np.int32), np.ascontiguousarray(b, np.int32)
+    cls = np.zeros(len(a), np.int32)
+    topo["lib"].ref_topo_exclusion_class(topo["handle"], len(a), a.ctypes.data_as(_I), b.ctypes.data_as(_I),
+                                         cls.ctypes.data_as(_I))
+    return cls
+
+
+def ref_remove_momentum(topo: dict, pos, vel, linear=True, angular=True) -> np.ndarray:
+    p = np.ascontiguousarray(pos, np.float64)
+    v = np.array(vel, np.float64, copy=True, order="C")
+    topo["lib"].ref_topo_remove_momentum(topo["handle"], p.ctypes.data_as(_D), v.ctypes.data_as(_D),
+                                         int(linear), int(angular))
+    return v
+
+
+def ref_topology_free(topo: dict) -> None:
+    if topo.get("handle"):
+        topo["lib"].ref_topo_free(topo["handle"])
+        topo["handle"] = None

@@ -35,6 +35,14 @@ using namespace ProtoMol::Report;
 #include "Hessian.h"
 #include "BondSystemForce.h"
 #include "AngleSystemForce.h"
+#include "DihedralSystemForce.h"
+#include "ImproperSystemForce.h"
+#include "PSFReader.h"
+#include "PARReader.h"
+#include "PSF.h"
+#include "PAR.h"
+#include "buildTopology.h"
+#include "topologyutilities.h"
 
 #include <vector>
 #include <map>
@@ -425,6 +433,170 @@ int ref_hessian_vec(int n, const double* pos, const double* q, const int* type, 
   }
   delete fg;
   delete t;
+  return 0;
+}
+
+
+// Dihedral / Improper system forces over topo->dihedrals / ->impropers
+// (forces/DihedralSystemForce.h, forces/ImproperSystemForce.h, both through
+// MTorsionSystemForce::calcTorsion).  Torsion t has atoms[4t..4t+3] and the
+// terms term_row[t] .. term_row[t+1]-1.
+int ref_torsions(int n, const double* pos, const double* box, const double* origin, int nt, const int* atoms,
+                 const int* term_row, const double* k, const int* period, const double* phase, int improper,
+                 int wantVirial, double* forces, double* energies) {
+  std::vector<double> q(n, 0.0), lj(1, 0.0);
+  std::vector<int> type(n, 0), mol(n, 0);
+  Vector3DBlock positions;
+  Topo* t = makeTopo(n, pos, &q[0], &type[0], 1, &lj[0], &lj[0], &lj[0], &lj[0], &mol[0], 0, 0, 0, 0, 1.0,
+                     box, origin, 5.0, positions);
+  for (int i = 0; i < nt; i++) {
+    Torsion tor;
+    tor.atom1 = atoms[4 * i]; tor.atom2 = atoms[4 * i + 1]; tor.atom3 = atoms[4 * i + 2]; tor.atom4 = atoms[4 * i + 3];
+    tor.multiplicity = term_row[i + 1] - term_row[i];
+    for (int u = term_row[i]; u < term_row[i + 1]; u++) {
+      tor.forceConstant.push_back(k[u]);
+      tor.periodicity.push_back(period[u]);
+      tor.phaseShift.push_back(phase[u]);
+    }
+    (improper ? t->impropers : t->dihedrals).push_back(tor);
+  }
+  Vector3DBlock fo;
+  fo.resize(n);
+  fo.zero();
+  ScalarStructure e;
+  e.clear();
+  e.virial(wantVirial != 0);
+  if (improper) { ImproperSystemForce<PeriodicBoundaryConditions> f; f.evaluate(t, &positions, &fo, &e); }
+  else { DihedralSystemForce<PeriodicBoundaryConditions> f; f.evaluate(t, &positions, &fo, &e); }
+  for (int i = 0; i < n; i++) {
+    forces[3 * i] += fo[i].x; forces[3 * i + 1] += fo[i].y; forces[3 * i + 2] += fo[i].z;
+  }
+  readEnergies(e, energies);
+  energies[4] = e[ScalarStructure::DIHEDRAL];
+  energies[5] = e[ScalarStructure::IMPROPER];
+  delete t;
+  return 0;
+}
+
+// The topology the reference's protomol application builds from a PSF and a
+// PAR file (applications/protomol-app/protomol.cpp:340-403): PSFReader,
+// PARReader, buildTopology with the exclusion type and the dihedral
+// multiplicity rule (-1: the application's default).  info[] receives
+// atoms, types, bonds, angles, dihedrals, dihedral terms, impropers,
+// improper terms, molecules, degrees of freedom, the detected PAR dialect
+// (19 or 28) and the multiplicity rule used.
+void* ref_topo_load(const char* psfPath, const char* parPath, const char* exclude, int dmult, int* info) {
+  PSFReader pr;
+  if (!pr.open(psfPath)) return 0;
+  PSF psf;
+  if (!(pr >> psf)) return 0;
+  PARReader ar;
+  if (!ar.open(parPath)) return 0;
+  PAR par;
+  if (!(ar >> par)) return 0;
+  const bool mult = dmult < 0 ? (ar.getCharmmTypeDetected() != PAR::CHARMM28) : (dmult != 0);
+  Topo* t = new Topo(1.0, ExclusionType(std::string(exclude)),
+                     PeriodicBoundaryConditions(Vector3D(1000, 0, 0), Vector3D(0, 1000, 0), Vector3D(0, 0, 1000),
+                                                Vector3D(0, 0, 0)),
+                     CubicCellManager(5.0));
+  t->exclude = ExclusionType(std::string(exclude));
+  buildTopology(t, psf, par, mult);
+  int dt = 0, it = 0;
+  for (size_t i = 0; i < t->dihedrals.size(); i++) dt += t->dihedrals[i].multiplicity;
+  for (size_t i = 0; i < t->impropers.size(); i++) it += t->impropers[i].multiplicity;
+  info[0] = (int)t->atoms.size(); info[1] = (int)t->atomTypes.size(); info[2] = (int)t->bonds.size();
+  info[3] = (int)t->angles.size(); info[4] = (int)t->dihedrals.size(); info[5] = dt;
+  info[6] = (int)t->impropers.size(); info[7] = it; info[8] = (int)t->molecules.size();
+  info[9] = (int)t->degreesOfFreedom; info[10] = ar.getCharmmTypeDetected() == PAR::CHARMM28 ? 28 : 19;
+  info[11] = mult ? 1 : 0;
+  return t;
+}
+
+void ref_topo_free(void* h) { delete (Topo*)h; }
+
+int ref_topo_atoms(void* h, double* q, double* mass, int* type, int* molecule, int* water) {
+  Topo* t = (Topo*)h;
+  for (size_t i = 0; i < t->atoms.size(); i++) {
+    q[i] = t->atoms[i].scaledCharge; mass[i] = t->atoms[i].scaledMass;
+    type[i] = t->atoms[i].type; molecule[i] = t->atoms[i].molecule;
+    water[i] = t->molecules[t->atoms[i].molecule].water ? 1 : 0;
+  }
+  return 0;
+}
+
+int ref_topo_type_name(void* h, int k, char* buf, int cap) {
+  Topo* t = (Topo*)h;
+  std::strncpy(buf, t->atomTypes[k].name.c_str(), cap - 1);
+  buf[cap - 1] = 0;
+  return 0;
+}
+
+int ref_topo_lj(void* h, double* A, double* B, double* A14, double* B14) {
+  Topo* t = (Topo*)h;
+  const int nt = (int)t->atomTypes.size();
+  for (int a = 0; a < nt; a++)
+    for (int b = 0; b < nt; b++) {
+      const LennardJonesParameters& p = t->lennardJonesParameters(a, b);
+      A[a * nt + b] = p.A; B[a * nt + b] = p.B; A14[a * nt + b] = p.A14; B14[a * nt + b] = p.B14;
+    }
+  return 0;
+}
+
+int ref_topo_bonds(void* h, int* a, int* b, double* k, double* r0) {
+  Topo* t = (Topo*)h;
+  for (size_t i = 0; i < t->bonds.size(); i++) {
+    a[i] = t->bonds[i].atom1; b[i] = t->bonds[i].atom2; k[i] = t->bonds[i].springConstant; r0[i] = t->bonds[i].restLength;
+  }
+  return 0;
+}
+
+int ref_topo_angles(void* h, int* a, int* b, int* c, double* k, double* t0, double* kub, double* rub) {
+  Topo* t = (Topo*)h;
+  for (size_t i = 0; i < t->angles.size(); i++) {
+    const Angle& g = t->angles[i];
+    a[i] = g.atom1; b[i] = g.atom2; c[i] = g.atom3; k[i] = g.forceConstant; t0[i] = g.restAngle;
+    kub[i] = g.ureyBradleyConstant; rub[i] = g.ureyBradleyRestLength;
+  }
+  return 0;
+}
+
+int ref_topo_torsions(void* h, int improper, int* atoms, int* term_row, double* k, int* period, double* phase) {
+  Topo* t = (Topo*)h;
+  const std::vector<Torsion>& L = improper ? t->impropers : t->dihedrals;
+  int u = 0;
+  for (size_t i = 0; i < L.size(); i++) {
+    atoms[4 * i] = L[i].atom1; atoms[4 * i + 1] = L[i].atom2; atoms[4 * i + 2] = L[i].atom3; atoms[4 * i + 3] = L[i].atom4;
+    term_row[i] = u;
+    for (int m = 0; m < L[i].multiplicity; m++, u++) {
+      k[u] = L[i].forceConstant[m]; period[u] = L[i].periodicity[m]; phase[u] = L[i].phaseShift[m];
+    }
+  }
+  term_row[L.size()] = u;
+  return 0;
+}
+
+// ExclusionTable::check for the given pairs
+int ref_topo_exclusion_class(void* h, int np, const int* a, const int* b, int* cls) {
+  Topo* t = (Topo*)h;
+  for (int k = 0; k < np; k++) cls[k] = (int)t->exclusions.check(a[k], b[k]);
+  return 0;
+}
+
+// removeLinearMomentum then removeAngularMomentum (topologyutilities.cpp),
+// as protomol.cpp:429-444 applies them at start-up
+int ref_topo_remove_momentum(void* h, const double* pos, double* vel, int linear, int angular) {
+  Topo* t = (Topo*)h;
+  const int n = (int)t->atoms.size();
+  Vector3DBlock p, v;
+  p.resize(n);
+  v.resize(n);
+  for (int i = 0; i < n; i++) {
+    p[i] = Vector3D(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    v[i] = Vector3D(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+  }
+  if (linear) removeLinearMomentum(&v, t);
+  if (angular) removeAngularMomentum(&p, &v, t);
+  for (int i = 0; i < n; i++) { vel[3 * i] = v[i].x; vel[3 * i + 1] = v[i].y; vel[3 * i + 2] = v[i].z; }
   return 0;
 }
 

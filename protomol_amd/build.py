@@ -1,4 +1,5 @@
-"""Compile the gfx950 HIP library (libpmgpu.so) in-tree with hipcc.
+"""Compile the gfx950 HIP library (libpmgpu.so) in-tree with hipcc, and the
+host front end (libpmfront.so: ProtoMol's input/output files) with g++.
 
 Usage: python -m protomol_amd.build   (or protomol_amd.build.build()).
 """
@@ -16,6 +17,10 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmgpu.so")
 SOURCES = ["pm_nonbonded.hip", "pm_pme.hip", "pm_md.hip", "pm_bonded.hip", "pm_krylov.hip", "pm_shake.hip",
            "pm_api.cpp"]
+FRONT = os.path.join(HERE, "front")
+FRONT_LIB = os.path.join(HERE, "libpmfront.so")
+FRONT_SOURCES = ["pmf_read.cpp", "pmf_build.cpp", "pmf_write.cpp", "pmf_api.cpp"]
+CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -36,7 +41,22 @@ def _compile(src: str) -> str:
     return out
 
 
+def build_front(verbose: bool = True) -> str:
+    """Host-only library: readers, topology builder and writers (no HIP)."""
+    tmp = FRONT_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", "-I" + FRONT, "-I" + os.path.join(ROOT, "include"),
+           "-o", tmp] + [os.path.join(FRONT, f) for f in FRONT_SOURCES]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"front end build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, FRONT_LIB)
+    if verbose:
+        print(f"built {FRONT_LIB}")
+    return FRONT_LIB
+
+
 def build(verbose: bool = True) -> str:
+    build_front(verbose)
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
