@@ -108,6 +108,17 @@ int pm_set_bonded(pm_context* ctx, int nbonds, const int* bond_a, const int* bon
                   const int* angle_b, const int* angle_c, const double* angle_k,
                   const double* angle_theta0, const double* angle_kub, const double* angle_rub);
 
+/* Dihedrals (kind 0) or impropers (kind 1) of the topology, as
+ * frontend/buildTopology.cpp:348-566 fills GenericTopology::dihedrals /
+ * ::impropers (Torsion records): torsion t has atoms[4t .. 4t+3] and the
+ * terms term_row[t] .. term_row[t+1]-1 (term_row[0] = 0) with force constant
+ * (kcal/mol), periodicity and phase shift (radians).  Used by the "Dihedral"
+ * and "Improper" forces (forces/DihedralSystemForce.h,
+ * forces/ImproperSystemForce.h through MTorsionSystemForce::calcTorsion,
+ * forces/MTorsionSystemForce.h:34-153).  Call after pm_set_topology. */
+int pm_set_torsions(pm_context* ctx, int kind, int ntorsions, const int* atoms, const int* term_row,
+                    const double* k, const int* period, const double* phase);
+
 /* Periodic boundary conditions and cubic cell manager: cellBasisVector1-3,
  * cellOrigin and cellsize (topology/PeriodicBoundaryConditions.cpp:34-77,
  * topology/CubicCellManager.cpp:25-42).  Only orthogonal boxes (the reference
@@ -121,7 +132,7 @@ int pm_set_boundary(pm_context* ctx, const double* e1, const double* e2, const d
  *      values = {switchon, cutoff (C2), alpha, cutoff (Cutoff), algorithm cutoff}
  *   "Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline"
  *      values = {nx, ny, nz, cutoff, order, accuracy, alpha}   (alias "CoulombPME")
- *   "Bond", "Angle"   (no values; the topology's bonded terms)
+ *   "Bond", "Angle", "Dihedral", "Improper"   (no values; the topology's bonded terms)
  * Replaces ForceFactory::make + doMake (forces/NonbondedCutoffForce.h:123-129,
  * forces/NonbondedPMEwaldSystemForce.h:1100-1139); the same checks, the same
  * messages.  *handle identifies the force in later calls. */

@@ -1388,6 +1388,101 @@ void orc_bonded(int n, const double* pos, const double* box, const double* origi
   }
 }
 
+static void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* MTorsionSystemForce::calcTorsion (forces/MTorsionSystemForce.h:34-153):
+ * phi from the unit normals a = r12 x r23, b = r23 x r34, c = r23 x a; the
+ * force through d(cos phi) when |sin phi| > 0.1, else through d(sin phi). */
+void orc_torsions(int n, const double* pos, const double* box, const double* origin, int nt, const int* atoms,
+                  const int* term_row, const double* tk, const int* period, const double* phase, int improper,
+                  int want_virial, double* forces, double* energies) {
+  orc_pbc p;
+  int t, c, u;
+  const double pi = 3.14159265358979323846;
+  (void)n;
+  pbc_set(&p, box, origin);
+  for (t = 0; t < nt; t++) {
+    const int a1 = atoms[4 * t], a2 = atoms[4 * t + 1], a3 = atoms[4 * t + 2], a4 = atoms[4 * t + 3];
+    double r12[3], r23[3], r34[3], d2, a[3], b[3], cc[3], ra, rb, rc, cosp, sinp, phi, dpot = 0.0, e = 0.0;
+    double f1[3], f2[3], f3[3];
+    pbc_min_diff(&p, pos + 3 * a2, pos + 3 * a1, r12, &d2);
+    pbc_min_diff(&p, pos + 3 * a3, pos + 3 * a2, r23, &d2);
+    pbc_min_diff(&p, pos + 3 * a4, pos + 3 * a3, r34, &d2);
+    cross3(r12, r23, a);
+    cross3(r23, r34, b);
+    cross3(r23, a, cc);
+    ra = 1.0 / nrm(a); rb = 1.0 / nrm(b); rc = 1.0 / nrm(cc);
+    for (c = 0; c < 3; c++) { a[c] *= ra; b[c] *= rb; cc[c] *= rc; }
+    cosp = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    sinp = cc[0] * b[0] + cc[1] * b[1] + cc[2] * b[2];
+    phi = -atan2(sinp, cosp);
+    for (u = term_row[t]; u < term_row[t + 1]; u++) {
+      if (period[u] > 0) {
+        dpot -= period[u] * tk[u] * sin(period[u] * phi + phase[u]);
+        e += tk[u] * (1.0 + cos(period[u] * phi + phase[u]));
+      } else {
+        double diff = phi - phase[u];
+        if (diff < -pi) diff += 2 * pi;
+        else if (diff > pi) diff -= 2 * pi;
+        dpot += 2.0 * tk[u] * diff;
+        e += tk[u] * diff * diff;
+      }
+    }
+    if (fabs(sinp) > 0.1) {
+      double dA[3], dB[3], k1 = dpot / sinp;
+      for (c = 0; c < 3; c++) { dA[c] = (a[c] * cosp - b[c]) * ra; dB[c] = (b[c] * cosp - a[c]) * rb; }
+      f1[0] = k1 * (r23[1] * dA[2] - r23[2] * dA[1]);
+      f1[1] = k1 * (r23[2] * dA[0] - r23[0] * dA[2]);
+      f1[2] = k1 * (r23[0] * dA[1] - r23[1] * dA[0]);
+      f3[0] = k1 * (r23[2] * dB[1] - r23[1] * dB[2]);
+      f3[1] = k1 * (r23[0] * dB[2] - r23[2] * dB[0]);
+      f3[2] = k1 * (r23[1] * dB[0] - r23[0] * dB[1]);
+      f2[0] = k1 * (r12[2] * dA[1] - r12[1] * dA[2] + r34[1] * dB[2] - r34[2] * dB[1]);
+      f2[1] = k1 * (r12[0] * dA[2] - r12[2] * dA[0] + r34[2] * dB[0] - r34[0] * dB[2]);
+      f2[2] = k1 * (r12[1] * dA[0] - r12[0] * dA[1] + r34[0] * dB[1] - r34[1] * dB[0]);
+    } else {
+      double dC[3], dB[3], k1 = -dpot / cosp, x[3];
+      for (c = 0; c < 3; c++) { dC[c] = (cc[c] * sinp - b[c]) * rc; dB[c] = (b[c] * sinp - cc[c]) * rb; }
+      f1[0] = k1 * ((r23[1] * r23[1] + r23[2] * r23[2]) * dC[0] - r23[0] * r23[1] * dC[1] - r23[0] * r23[2] * dC[2]);
+      f1[1] = k1 * ((r23[2] * r23[2] + r23[0] * r23[0]) * dC[1] - r23[1] * r23[2] * dC[2] - r23[1] * r23[0] * dC[0]);
+      f1[2] = k1 * ((r23[0] * r23[0] + r23[1] * r23[1]) * dC[2] - r23[2] * r23[0] * dC[0] - r23[2] * r23[1] * dC[1]);
+      cross3(dB, r23, x);
+      for (c = 0; c < 3; c++) f3[c] = x[c] * k1;
+      f2[0] = k1 * (-(r23[1] * r12[1] + r23[2] * r12[2]) * dC[0] + (2.0 * r23[0] * r12[1] - r12[0] * r23[1]) * dC[1] +
+                    (2.0 * r23[0] * r12[2] - r12[0] * r23[2]) * dC[2] + dB[2] * r34[1] - dB[1] * r34[2]);
+      f2[1] = k1 * (-(r23[2] * r12[2] + r23[0] * r12[0]) * dC[1] + (2.0 * r23[1] * r12[2] - r12[1] * r23[2]) * dC[2] +
+                    (2.0 * r23[1] * r12[0] - r12[1] * r23[0]) * dC[0] + dB[0] * r34[2] - dB[2] * r34[0]);
+      f2[2] = k1 * (-(r23[0] * r12[0] + r23[1] * r12[1]) * dC[2] + (2.0 * r23[2] * r12[0] - r12[2] * r23[0]) * dC[0] +
+                    (2.0 * r23[2] * r12[1] - r12[2] * r23[1]) * dC[1] + dB[1] * r34[0] - dB[0] * r34[1]);
+    }
+    for (c = 0; c < 3; c++) {
+      forces[3 * a1 + c] += f1[c];
+      forces[3 * a2 + c] += f2[c] - f1[c];
+      forces[3 * a3 + c] += f3[c] - f2[c];
+      forces[3 * a4 + c] -= f3[c];
+    }
+    energies[improper ? ORC_E_IMPROPER : ORC_E_DIHEDRAL] += e;
+    if (want_virial) {
+      double xy = f1[0] * r12[1] + f2[0] * r23[1] + f3[0] * r34[1];
+      double xz = f1[0] * r12[2] + f2[0] * r23[2] + f3[0] * r34[2];
+      double yz = f1[1] * r12[2] + f2[1] * r23[2] + f3[1] * r34[2];
+      energies[ORC_E_VXX] += f1[0] * r12[0] + f2[0] * r23[0] + f3[0] * r34[0];
+      energies[ORC_E_VXY] += xy;
+      energies[ORC_E_VXZ] += xz;
+      energies[ORC_E_VYX] += xy;
+      energies[ORC_E_VYY] += f1[1] * r12[1] + f2[1] * r23[1] + f3[1] * r34[1];
+      energies[ORC_E_VYZ] += yz;
+      energies[ORC_E_VZX] += xz;
+      energies[ORC_E_VZY] += yz;
+      energies[ORC_E_VZZ] += f1[2] * r12[2] + f2[2] * r23[2] + f3[2] * r34[2];
+    }
+  }
+}
+
 double orc_rem_delta(double ti, double tj, double ei, double ej) {
   /* rem.cpp:941-946 */
   return (1 / ORC_BOLTZMANN / ti - 1 / ORC_BOLTZMANN / tj) * (ej - ei);

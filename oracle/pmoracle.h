@@ -52,7 +52,7 @@ enum { ORC_SWF_CUTOFF = 0, ORC_SWF_C1 = 1, ORC_SWF_C2 = 2 };
  * ANGLE, DIHEDRAL, IMPROPER, OTHER, then the nine atomic virial entries
  * (base/ScalarStructure.h:25-45) */
 enum {
-  ORC_E_COULOMB = 0, ORC_E_LJ = 1, ORC_E_BOND = 2, ORC_E_ANGLE = 3,
+  ORC_E_COULOMB = 0, ORC_E_LJ = 1, ORC_E_BOND = 2, ORC_E_ANGLE = 3, ORC_E_DIHEDRAL = 4, ORC_E_IMPROPER = 5,
   ORC_E_VXX = 7, ORC_E_VXY, ORC_E_VXZ, ORC_E_VYX, ORC_E_VYY, ORC_E_VYZ,
   ORC_E_VZX, ORC_E_VZY, ORC_E_VZZ,
   ORC_E_COUNT
@@ -166,6 +166,17 @@ void orc_bonded(int n, const double* pos, const double* box, const double* origi
                 const int* aa, const int* ab, const int* ac, const double* ak, const double* at0,
                 const double* akub, const double* arub, int which, int want_virial,
                 double* forces, double* energies);
+
+/* Dihedral (improper = 0) or improper (improper = 1) system force
+ * (forces/DihedralSystemForce.h, forces/ImproperSystemForce.h through
+ * MTorsionSystemForce::calcTorsion, forces/MTorsionSystemForce.h:34-153):
+ * torsion t has atoms[4t..4t+3] and terms term_row[t] .. term_row[t+1]-1
+ * (force constant, periodicity, phase in radians); periodicity > 0 gives
+ * k (1 + cos(n phi + phase)), periodicity 0 the harmonic k (phi - phase)^2
+ * with the difference wrapped into [-pi, pi].  Forces and energies ACCUMULATE. */
+void orc_torsions(int n, const double* pos, const double* box, const double* origin, int nt, const int* atoms,
+                  const int* term_row, const double* k, const int* period, const double* phase, int improper,
+                  int want_virial, double* forces, double* energies);
 
 /* Replica-exchange acceptance exponent (applications/rem-app/rem.cpp:941-946). */
 double orc_rem_delta(double ti, double tj, double ei, double ej);

@@ -20,7 +20,7 @@ REF_LIB = os.path.join(HERE, "_ref", "libpmref.so")
 
 POT_LJ, POT_COULOMB, POT_EWALDREAL = 0, 1, 2
 SWF_CUTOFF, SWF_C1, SWF_C2 = 0, 1, 2
-E_COULOMB, E_LJ, E_BOND, E_ANGLE = 0, 1, 2, 3
+E_COULOMB, E_LJ, E_BOND, E_ANGLE, E_DIHEDRAL, E_IMPROPER = 0, 1, 2, 3, 4, 5
 E_VIRIAL = 7
 E_COUNT = 16
 
@@ -67,6 +67,7 @@ class Oracle:
         self._bspline = getattr(self.lib, p + "bspline")
         self._hess = getattr(self.lib, p + "hessian_vec")
         self._bonded = getattr(self.lib, p + "bonded")
+        self._torsions = getattr(self.lib, p + "torsions")
         self._cutoff_omp = getattr(self.lib, "orc_cutoff_omp", None) if kind == "port" else None
         self._delta = getattr(self.lib, "orc_delta_list", None) if kind == "port" else None
         self._fft = getattr(self.lib, "orc_fft3d", None) if kind == "port" else None
@@ -229,6 +230,19 @@ class Oracle:
                          i(sys.bond_b), d(sys.bond_k), d(sys.bond_r0), len(sys.angle_a), i(sys.angle_a),
                          i(sys.angle_b), i(sys.angle_c), d(sys.angle_k), d(sys.angle_t0), d(sys.angle_kub),
                          d(sys.angle_rub), which, int(virial), f.ctypes.data_as(_D), e.ctypes.data_as(_D))
+        return dict(forces=f, energies=e)
+
+    def torsions(self, sys, tors, improper, virial=False):
+        """Dihedral (improper=False) or improper system force over tors
+        (atoms (m, 4), term_row, k, period, phase in radians)."""
+        g = (lambda name: getattr(tors, name)) if not isinstance(tors, dict) else tors.__getitem__
+        pos, box, org = _d(sys.pos), _d(sys.box), _d(sys.origin)
+        at, row, per = _i(g("atoms")), _i(g("term_row")), _i(g("period"))
+        k, ph = _d(g("k")), _d(g("phase"))
+        f = np.zeros((sys.n, 3))
+        e = np.zeros(E_COUNT)
+        self._torsions(sys.n, pos[1], box[1], org[1], len(at[0]), at[1], row[1], k[1], per[1], ph[1],
+                       int(improper), int(virial), f.ctypes.data_as(_D), e.ctypes.data_as(_D))
         return dict(forces=f, energies=e)
 
     # -------------------------------------------------------------- Hessian

@@ -112,6 +112,12 @@ struct pm_context {
   double *d_b_k = nullptr, *d_b_r0 = nullptr;
   int *d_a_a = nullptr, *d_a_b = nullptr, *d_a_c = nullptr, *d_a_row = nullptr, *d_a_idx = nullptr;
   double *d_a_k = nullptr, *d_a_t0 = nullptr, *d_a_kub = nullptr, *d_a_rub = nullptr;
+  // torsions: kind 0 dihedrals, kind 1 impropers
+  int ntors[2] = {0, 0};
+  int *d_t_row[2] = {nullptr, nullptr}, *d_t_idx[2] = {nullptr, nullptr}, *d_t_term[2] = {nullptr, nullptr};
+  int4* d_t_atoms[2] = {nullptr, nullptr};
+  double *d_t_k[2] = {nullptr, nullptr}, *d_t_phase[2] = {nullptr, nullptr};
+  int* d_t_n[2] = {nullptr, nullptr};
   std::vector<int> h_ba, h_bb, h_aa, h_ab, h_ac;   // host copies for the constraint list
   std::vector<double> h_br0, h_at0;
 
@@ -662,9 +668,9 @@ bool ieq(const std::string& a, const std::string& b) {
 int parse_force(pm_context* c, const char* id_c, const double* v, int nv, ForceDesc& f) {
   std::string id(id_c ? id_c : "");
   std::vector<std::string> t = tokens(id);
-  if (t.size() == 1 && (ieq(t[0], "Bond") || ieq(t[0], "Angle"))) {
+  if (t.size() == 1 && (ieq(t[0], "Bond") || ieq(t[0], "Angle") || ieq(t[0], "Dihedral") || ieq(t[0], "Improper"))) {
     if (nv != 0) return fail(c, PM_ERR_ARGUMENT, "force '%s' takes no parameters", id.c_str());
-    f.bonded = ieq(t[0], "Bond") ? 1 : 2;
+    f.bonded = ieq(t[0], "Bond") ? 1 : ieq(t[0], "Angle") ? 2 : ieq(t[0], "Dihedral") ? 3 : 4;
     f.id = id;
     return PM_OK;
   }
@@ -1009,16 +1015,23 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     if (f.corr) rec += ((long long)c->tab_a.size() + 255) / 256;
     pme_rec.push_back(std::make_pair(h, off));
   }
-  bool do_bond = false, do_angle = false;
+  bool do_bond = false, do_angle = false, do_tors[2] = {false, false};
   for (int h : handles) {
     if (c->forces[h].bonded == 1) do_bond = true;
     if (c->forces[h].bonded == 2) do_angle = true;
+    if (c->forces[h].bonded == 3) do_tors[0] = true;
+    if (c->forces[h].bonded == 4) do_tors[1] = true;
   }
   if (do_bond && c->nbonds == 0) do_bond = false;
   if (do_angle && c->nangles == 0) do_angle = false;
+  for (int k = 0; k < 2; k++)
+    if (do_tors[k] && c->ntors[k] == 0) do_tors[k] = false;
+  const bool any_bonded = do_bond || do_angle || do_tors[0] || do_tors[1];
   const long long bonded_off = rec;
   if (do_bond) rec += (c->nbonds + 255) / 256;
   if (do_angle) rec += (c->nangles + 255) / 256;
+  for (int k = 0; k < 2; k++)
+    if (do_tors[k]) rec += (c->ntors[k] + 255) / 256;
   if ((r = ensure_epart(c, std::max(rec, 1LL)))) return r;
 
   // host-side set-up of every launch first (stencils, list capacity, PME
@@ -1202,7 +1215,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     HIPCHECK(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
   }
   PmBondedLaunch b{};
-  if (do_bond || do_angle) {
+  if (any_bonded) {
     b.n = n;
     b.pos = c->d_pos;
     b.b_row = c->d_b_row; b.b_idx = c->d_b_idx; b.a_row = c->d_a_row; b.a_idx = c->d_a_idx;
@@ -1212,6 +1225,12 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     b.a_kub = c->d_a_kub; b.a_rub = c->d_a_rub;
     b.do_bond = do_bond ? 1 : 0;
     b.do_angle = do_angle ? 1 : 0;
+    for (int k = 0; k < 2; k++) {
+      b.t_row[k] = c->d_t_row[k]; b.t_idx[k] = c->d_t_idx[k]; b.nt[k] = c->ntors[k];
+      b.t_atoms[k] = c->d_t_atoms[k]; b.t_term[k] = c->d_t_term[k];
+      b.t_k[k] = c->d_t_k[k]; b.t_n[k] = c->d_t_n[k]; b.t_phase[k] = c->d_t_phase[k];
+      b.do_tors[k] = do_tors[k] ? 1 : 0;
+    }
     b.force_fix = c->d_force;
     b.epart = c->d_epart + bonded_off * PM_NE;
     b.want_virial = vir ? 1 : 0;
@@ -1230,7 +1249,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       tail.row = c->d_cx_row; tail.partner = c->d_cx_partner; tail.cls = c->d_cx_cls;
       tail.alpha = f.alpha; tail.csf = c->csf;
     }
-    if (do_bond || do_angle) {
+    if (any_bonded) {
       tail.bonded = 1;
       tail.b = b;
     }
@@ -1252,12 +1271,11 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
                          f.alpha, c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
                          c->d_epart + off * PM_NE, want_energy ? nblk : 0);
   }
-  if ((do_bond || do_angle) && !(fuse && !want_energy)) {
+  if (any_bonded && !(fuse && !want_energy)) {
     PmBondedLaunch bl = b;
     if (fuse) bl.n = 0;   // the forces came with the gather: energies only
-    int rb, ra;
     TimedScope ts(c, PM_T_BONDED, s2);
-    pm_launch_bonded(s2, c->box, bl, &rb, &ra);
+    pm_launch_bonded(s2, c->box, bl);
   }
   if (side) HIPCHECK(c, hipEventRecord(c->ev_join, s2));
 
@@ -1475,6 +1493,11 @@ void free_all(pm_context* c) {
   dfree(c->d_a_a); dfree(c->d_a_b); dfree(c->d_a_c); dfree(c->d_a_row); dfree(c->d_a_idx);
   dfree(c->d_a_k); dfree(c->d_a_t0); dfree(c->d_a_kub); dfree(c->d_a_rub);
   c->nbonds = c->nangles = 0;
+  for (int k = 0; k < 2; k++) {
+    dfree(c->d_t_row[k]); dfree(c->d_t_idx[k]); dfree(c->d_t_term[k]); dfree(c->d_t_atoms[k]);
+    dfree(c->d_t_k[k]); dfree(c->d_t_phase[k]); dfree(c->d_t_n[k]);
+    c->ntors[k] = 0;
+  }
   dfree(c->d_cons_gstart); dfree(c->d_cons_a1); dfree(c->d_cons_a2); dfree(c->d_cons_sweeps);
   dfree(c->d_cons_r0); dfree(c->d_rmass); dfree(c->d_cons_err); dfree(c->d_cons_la); dfree(c->d_cons_gatom);
   dfree(c->d_cons_scratch); dfree(c->d_shake_last);
@@ -1572,7 +1595,9 @@ int pm_set_topology(pm_context* c, int n, const double* q, const int* type, int 
   if (n <= 0 || !q || !type || ntypes <= 0 || !ljA || !ljB || !ljA14 || !ljB14 || !molecule)
     return fail(c, PM_ERR_ARGUMENT, "pm_set_topology: n > 0 and non-null arrays required");
   if (n >= (1 << 23) - 1) return fail(c, PM_ERR_CAPACITY, "pm_set_topology: at most %d atoms", (1 << 23) - 2);
-  if (ntypes > 31) return fail(c, PM_ERR_UNSUPPORTED, "pm_set_topology: at most 31 atom types (got %d)", ntypes);
+  // the pair kernels stage the (types + 1)^2 parameter table in LDS: at most
+  // 44 types keep the FP64 table (45^2 x 32 bytes) within 64 KiB
+  if (ntypes > 44) return fail(c, PM_ERR_UNSUPPORTED, "pm_set_topology: at most 44 atom types (got %d)", ntypes);
   if (nexcl > 0 && (!ex_a || !ex_b || !ex_class)) return fail(c, PM_ERR_ARGUMENT, "pm_set_topology: exclusion arrays missing");
   for (int i = 0; i < n; i++)
     if (type[i] < 0 || type[i] >= ntypes) return fail(c, PM_ERR_ARGUMENT, "atom %d has type %d outside [0, %d)", i, type[i], ntypes);
@@ -1800,6 +1825,52 @@ int pm_set_bonded(pm_context* c, int nb, const int* ba, const int* bb, const dou
   c->cons_built = false;
   c->md_forces_valid = false;
   c->gen++;
+  return PM_OK;
+}
+
+int pm_set_torsions(pm_context* c, int kind, int nt, const int* atoms, const int* term_row, const double* k,
+                    const int* period, const double* phase) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if (kind != 0 && kind != 1) return fail(c, PM_ERR_ARGUMENT, "pm_set_torsions: kind is 0 (dihedrals) or 1 (impropers)");
+  if (nt < 0 || (nt > 0 && (!atoms || !term_row || !k || !period || !phase)))
+    return fail(c, PM_ERR_ARGUMENT, "pm_set_torsions: arrays missing");
+  const int n = c->n;
+  if (nt > 0 && term_row[0] != 0) return fail(c, PM_ERR_ARGUMENT, "pm_set_torsions: term_row[0] must be 0");
+  for (int t = 0; t < nt; t++) {
+    const int* a = atoms + 4 * (size_t)t;
+    for (int j = 0; j < 4; j++)
+      if (a[j] < 0 || a[j] >= n) return fail(c, PM_ERR_ARGUMENT, "torsion %d names atom %d of %d", t, a[j], n);
+    if (a[0] == a[1] || a[0] == a[2] || a[0] == a[3] || a[1] == a[2] || a[1] == a[3] || a[2] == a[3])
+      return fail(c, PM_ERR_ARGUMENT, "torsion %d (%d, %d, %d, %d) repeats an atom", t, a[0], a[1], a[2], a[3]);
+    if (term_row[t + 1] <= term_row[t])
+      return fail(c, PM_ERR_ARGUMENT, "torsion %d has no term (term_row must increase)", t);
+  }
+  const int nterm = nt > 0 ? term_row[nt] : 0;
+  std::vector<int> row(n + 1, 0), idx(4 * (size_t)nt);
+  std::vector<int4> at(nt);
+  for (int t = 0; t < nt; t++) {
+    at[t] = make_int4(atoms[4 * t], atoms[4 * t + 1], atoms[4 * t + 2], atoms[4 * t + 3]);
+    for (int j = 0; j < 4; j++) row[atoms[4 * t + j] + 1]++;
+  }
+  for (int i = 0; i < n; i++) row[i + 1] += row[i];
+  {
+    std::vector<int> fill(row.begin(), row.end() - 1);
+    for (int t = 0; t < nt; t++)
+      for (int j = 0; j < 4; j++) idx[fill[atoms[4 * t + j]]++] = t;
+  }
+  if ((r = upload(c, &c->d_t_row[kind], row))) return r;
+  if ((r = upload(c, &c->d_t_idx[kind], idx))) return r;
+  if ((r = upload(c, &c->d_t_atoms[kind], at))) return r;
+  if ((r = upload(c, &c->d_t_term[kind], std::vector<int>(term_row, term_row + (nt > 0 ? nt + 1 : 0))))) return r;
+  if ((r = upload(c, &c->d_t_k[kind], std::vector<double>(k, k + nterm)))) return r;
+  if ((r = upload(c, &c->d_t_n[kind], std::vector<int>(period, period + nterm)))) return r;
+  if ((r = upload(c, &c->d_t_phase[kind], std::vector<double>(phase, phase + nterm)))) return r;
+  c->ntors[kind] = nt;
+  c->gen++;
+  c->md_forces_valid = false;
   return PM_OK;
 }
 

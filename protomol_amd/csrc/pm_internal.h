@@ -44,6 +44,8 @@
 #define PM_E_LJ 1
 #define PM_E_BOND 2
 #define PM_E_ANGLE 3
+#define PM_E_DIHEDRAL 4
+#define PM_E_IMPROPER 5
 #define PM_E_VIR 7
 #define PM_NE 16
 
@@ -332,12 +334,22 @@ struct PmBondedLaunch {
   const int* a_a; const int* a_b; const int* a_c; const double* a_k; const double* a_t0;
   const double* a_kub; const double* a_rub;
   int do_bond, do_angle;
+  // torsions, kind 0 dihedrals and kind 1 impropers: per-atom CSR of the
+  // torsions an atom belongs to, the four atoms, and each torsion's terms
+  // term[t] .. term[t+1]-1 (force constant, periodicity, phase in radians)
+  const int* t_row[2]; const int* t_idx[2];
+  int nt[2];
+  const int4* t_atoms[2]; const int* t_term[2];
+  const double* t_k[2]; const int* t_n[2]; const double* t_phase[2];
+  int do_tors[2];
   long long* force_fix;
-  double* epart;        // records: bond blocks then angle blocks
+  double* epart;        // records: bond, angle, dihedral, improper blocks
   int want_virial;
   int want_energy;      // write the energy records (skipped on steps that report none)
 };
-void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a, int* nrec_bond, int* nrec_angle);
+// Energy-record blocks of each kind (bond, angle, dihedral, improper) for a.
+void pm_bonded_blocks(const PmBondedLaunch& a, int nblk[4]);
+void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a);
 
 // Per-atom force terms that the PME gather kernel adds for its atoms, each
 // rounded to fixed point on its own as the separate kernels round it (the
@@ -349,7 +361,7 @@ struct PmAtomTail {
   const double* q;
   const int* row; const int* partner; const int* cls;
   double alpha, csf;
-  int bonded;                  // bond and angle terms (b.do_bond / b.do_angle)
+  int bonded;                  // bonded terms (b.do_bond / do_angle / do_tors)
   PmBondedLaunch b;
 };
 

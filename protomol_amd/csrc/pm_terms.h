@@ -90,6 +90,78 @@ __device__ inline void angle_term(const PmBox& b, const PmBondedLaunch& a, int k
   e = kt * (theta - t0) * (theta - t0) + kub * (d13 - rub) * (d13 - rub);
 }
 
+// One torsion of kind k (0 dihedral, 1 improper), as
+// MTorsionSystemForce::calcTorsion (forces/MTorsionSystemForce.h:34-153):
+// the forces f1, f2, f3 (atom 1 gets f1, atom 2 f2 - f1, atom 3 f3 - f2,
+// atom 4 -f3), the energy and the three bond vectors.
+__device__ inline void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline void torsion_term(const PmBox& bx, const PmBondedLaunch& a, int kind, int t, double* f1,
+                                    double* f2, double* f3, double& e, double* r12, double* r23, double* r34) {
+  const int4 at = a.t_atoms[kind][t];
+  double d2, A[3], B[3], Cc[3];
+  min_diff(bx, a.pos + 3 * at.y, a.pos + 3 * at.x, r12, d2);
+  min_diff(bx, a.pos + 3 * at.z, a.pos + 3 * at.y, r23, d2);
+  min_diff(bx, a.pos + 3 * at.w, a.pos + 3 * at.z, r34, d2);
+  cross3(r12, r23, A);
+  cross3(r23, r34, B);
+  cross3(r23, A, Cc);
+  const double ra = 1.0 / norm3(A), rb = 1.0 / norm3(B), rc = 1.0 / norm3(Cc);
+  for (int c = 0; c < 3; c++) { A[c] *= ra; B[c] *= rb; Cc[c] *= rc; }
+  const double cosp = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+  const double sinp = Cc[0] * B[0] + Cc[1] * B[1] + Cc[2] * B[2];
+  const double phi = -atan2(sinp, cosp);
+  const double pi = 3.14159265358979323846;
+  double dpot = 0.0;
+  e = 0.0;
+  for (int u = a.t_term[kind][t]; u < a.t_term[kind][t + 1]; u++) {
+    const int n = a.t_n[kind][u];
+    const double k = a.t_k[kind][u], ph = a.t_phase[kind][u];
+    if (n > 0) {
+      dpot -= n * k * sin(n * phi + ph);
+      e += k * (1.0 + cos(n * phi + ph));
+    } else {
+      double diff = phi - ph;
+      if (diff < -pi) diff += 2 * pi;
+      else if (diff > pi) diff -= 2 * pi;
+      dpot += 2.0 * k * diff;
+      e += k * diff * diff;
+    }
+  }
+  if (fabs(sinp) > 0.1) {
+    double dA[3], dB[3];
+    const double k1 = dpot / sinp;
+    for (int c = 0; c < 3; c++) { dA[c] = (A[c] * cosp - B[c]) * ra; dB[c] = (B[c] * cosp - A[c]) * rb; }
+    f1[0] = k1 * (r23[1] * dA[2] - r23[2] * dA[1]);
+    f1[1] = k1 * (r23[2] * dA[0] - r23[0] * dA[2]);
+    f1[2] = k1 * (r23[0] * dA[1] - r23[1] * dA[0]);
+    f3[0] = k1 * (r23[2] * dB[1] - r23[1] * dB[2]);
+    f3[1] = k1 * (r23[0] * dB[2] - r23[2] * dB[0]);
+    f3[2] = k1 * (r23[1] * dB[0] - r23[0] * dB[1]);
+    f2[0] = k1 * (r12[2] * dA[1] - r12[1] * dA[2] + r34[1] * dB[2] - r34[2] * dB[1]);
+    f2[1] = k1 * (r12[0] * dA[2] - r12[2] * dA[0] + r34[2] * dB[0] - r34[0] * dB[2]);
+    f2[2] = k1 * (r12[1] * dA[0] - r12[0] * dA[1] + r34[0] * dB[1] - r34[1] * dB[0]);
+  } else {
+    double dC[3], dB[3], x[3];
+    const double k1 = -dpot / cosp;
+    for (int c = 0; c < 3; c++) { dC[c] = (Cc[c] * sinp - B[c]) * rc; dB[c] = (B[c] * sinp - Cc[c]) * rb; }
+    f1[0] = k1 * ((r23[1] * r23[1] + r23[2] * r23[2]) * dC[0] - r23[0] * r23[1] * dC[1] - r23[0] * r23[2] * dC[2]);
+    f1[1] = k1 * ((r23[2] * r23[2] + r23[0] * r23[0]) * dC[1] - r23[1] * r23[2] * dC[2] - r23[1] * r23[0] * dC[0]);
+    f1[2] = k1 * ((r23[0] * r23[0] + r23[1] * r23[1]) * dC[2] - r23[2] * r23[0] * dC[0] - r23[2] * r23[1] * dC[1]);
+    cross3(dB, r23, x);
+    for (int c = 0; c < 3; c++) f3[c] = x[c] * k1;
+    f2[0] = k1 * (-(r23[1] * r12[1] + r23[2] * r12[2]) * dC[0] + (2.0 * r23[0] * r12[1] - r12[0] * r23[1]) * dC[1] +
+                  (2.0 * r23[0] * r12[2] - r12[0] * r23[2]) * dC[2] + dB[2] * r34[1] - dB[1] * r34[2]);
+    f2[1] = k1 * (-(r23[2] * r12[2] + r23[0] * r12[0]) * dC[1] + (2.0 * r23[1] * r12[2] - r12[1] * r23[2]) * dC[2] +
+                  (2.0 * r23[1] * r12[0] - r12[1] * r23[0]) * dC[0] + dB[0] * r34[2] - dB[2] * r34[0]);
+    f2[2] = k1 * (-(r23[0] * r12[0] + r23[1] * r12[1]) * dC[2] + (2.0 * r23[2] * r12[0] - r12[2] * r23[0]) * dC[0] +
+                  (2.0 * r23[2] * r12[1] - r12[2] * r23[1]) * dC[1] + dB[1] * r34[0] - dB[0] * r34[1]);
+  }
+}
+
 // Correction force on atom i: the sum over its exclusion partners (CSR,
 // partner encoded as 2*atom + (1 if i is the pair's second atom)).
 __device__ inline void corr_atom_force(const PmBox& box, int i, const double* __restrict__ pos,
@@ -128,6 +200,17 @@ __device__ inline void bonded_atom_force(const PmBox& box, const PmBondedLaunch&
       angle_term(box, a, k, f1, f2, f3, e, r12, r32);
       const double* g = (a.a_a[k] == i) ? f1 : (a.a_b[k] == i) ? f2 : f3;
       f[0] += g[0]; f[1] += g[1]; f[2] += g[2];
+    }
+  }
+  for (int kind = 0; kind < 2; kind++) {
+    if (!a.do_tors[kind]) continue;
+    for (int u = a.t_row[kind][i]; u < a.t_row[kind][i + 1]; u++) {
+      const int t = a.t_idx[kind][u];
+      double f1[3], f2[3], f3[3], e, r12[3], r23[3], r34[3];
+      torsion_term(box, a, kind, t, f1, f2, f3, e, r12, r23, r34);
+      const int4 at = a.t_atoms[kind][t];
+      for (int c = 0; c < 3; c++)
+        f[c] += (at.x == i) ? f1[c] : (at.y == i) ? f2[c] - f1[c] : (at.z == i) ? f3[c] - f2[c] : -f3[c];
     }
   }
 }

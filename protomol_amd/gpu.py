@@ -19,12 +19,12 @@ PRECISION = {"mixed": 0, "double": 1}
 PM_EVAL_VIRIAL = 1
 INTEGRATOR = {"leapfrog": 0, "langevin": 1}
 ENERGY_COUNT = 16
-COULOMB, LENNARDJONES, BOND, ANGLE = 0, 1, 2, 3
+COULOMB, LENNARDJONES, BOND, ANGLE, DIHEDRAL, IMPROPER = 0, 1, 2, 3, 4, 5
 VIRIAL = 7
 
 # exported symbols, kept in sync with include/pmgpu.h (tested)
 EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set_precision",
-           "pm_set_topology", "pm_set_bonded", "pm_set_boundary", "pm_force_create", "pm_force_uncache",
+           "pm_set_topology", "pm_set_bonded", "pm_set_torsions", "pm_set_boundary", "pm_force_create", "pm_force_uncache",
            "pm_evaluate", "pm_cell_diagnostics", "pm_md_setup", "pm_md_set_state", "pm_md_run",
            "pm_md_get_state", "pm_last_stats", "pm_set_timing", "pm_get_timing", "pm_set_option", "pm_get_option",
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
@@ -65,6 +65,7 @@ def load_library(path: str = LIB_PATH):
     lib.pm_set_precision.argtypes = [P, C.c_int]
     lib.pm_set_topology.argtypes = [P, C.c_int, D, I, C.c_int, D, D, D, D, I, D, C.c_int, I, I, I, C.c_double]
     lib.pm_set_bonded.argtypes = [P, C.c_int, I, I, D, D, C.c_int, I, I, I, D, D, D, D]
+    lib.pm_set_torsions.argtypes = [P, C.c_int, C.c_int, I, I, D, I, D]
     lib.pm_set_boundary.argtypes = [P, D, D, D, D, C.c_double]
     lib.pm_force_create.argtypes = [P, C.c_char_p, D, C.c_int, I]
     lib.pm_force_uncache.argtypes = [P, C.c_int]
@@ -163,6 +164,10 @@ class Context:
         self.n = len(q)
         if getattr(s, "bond_a", None) is not None and (len(s.bond_a) or len(s.angle_a)):
             self.set_bonded(s)
+        for kind, name in enumerate(("dihedrals", "impropers")):
+            t = getattr(s, name, None)
+            if t is not None and len(t.atoms):
+                self.set_torsions(kind, t)
 
     def set_bonded(self, s):
         ba, bb = (np.ascontiguousarray(x, np.int32) for x in (s.bond_a, s.bond_b))
@@ -172,6 +177,13 @@ class Context:
         self._keep = [ba, bb, bk, br, aa, ab, ac, ak, at, au, ar]
         self._check(self.lib.pm_set_bonded(self.h, len(ba), _ip(ba), _ip(bb), _dp(bk), _dp(br), len(aa),
                                            _ip(aa), _ip(ab), _ip(ac), _dp(ak), _dp(at), _dp(au), _dp(ar)))
+
+    def set_torsions(self, kind: int, t):
+        """kind 0 dihedrals, 1 impropers; t has atoms (m, 4), term_row, k, period, phase."""
+        at = np.ascontiguousarray(t.atoms, np.int32).reshape(-1)
+        row, per = (np.ascontiguousarray(x, np.int32) for x in (t.term_row, t.period))
+        k, ph = (np.ascontiguousarray(x, np.float64) for x in (t.k, t.phase))
+        self._check(self.lib.pm_set_torsions(self.h, kind, len(at) // 4, _ip(at), _ip(row), _dp(k), _ip(per), _dp(ph)))
 
     def set_boundary(self, box, origin, cellsize):
         e1 = np.array([box[0], 0, 0], np.float64)
