@@ -98,6 +98,24 @@ int pmf_write_xyz(const char* path, const pmf_system* s, const double* xyz, int 
  * with the coordinates xyz. */
 int pmf_write_pdb(const char* path, const char* template_pdb, const double* xyz, int n, const char* comment);
 
+/* A force definition of a .conf Integrator block (the words after "force",
+ * e.g. "LennardJones -algorithm NonbondedCutoff -switchingFunction C2
+ * -switchon 7 -cutoff 8") as the force id and positional values that
+ * pm_force_create takes: the parameter words are matched to the force's
+ * parameter list as ForceFactoryDetails::doMake matches them
+ * (frontend/ForceFactoryDetails.cpp:48-190), defaults filled in. */
+int pmf_force_values(const char* definition, char* id, int id_cap, double* values, int cap, int* nvalues);
+
+/* Start-up momentum removal of the solute (atoms outside 3-atom waters), as
+ * removeLinearMomentum / removeAngularMomentum do it
+ * (topology/topologyutilities.cpp:463-496, 636-675); v is updated in place. */
+int pmf_remove_momentum(const pmf_system* s, const double* x, double* v, int linear, int angular);
+
+/* Velocities at a temperature as randomVelocity draws them
+ * (topology/topologyutilities.cpp:24-58, mathutilities.cpp:147-230: polar
+ * Box-Muller on drand48 seeded by srand48(seed)). */
+int pmf_random_velocity(const pmf_system* s, double temperature, unsigned int seed, double* v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -321,7 +321,8 @@ def ref_topology(psf: str, par: str, exclude: str = "scaled1-4", dihedral_mult_p
     lib.ref_topo_load.restype = C.c_void_p
     lib.ref_topo_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _I]
     for f in ("ref_topo_free", "ref_topo_atoms", "ref_topo_lj", "ref_topo_bonds", "ref_topo_angles",
-              "ref_topo_torsions", "ref_topo_exclusion_class", "ref_topo_type_name", "ref_topo_remove_momentum"):
+              "ref_topo_torsions", "ref_topo_exclusion_class", "ref_topo_type_name", "ref_topo_remove_momentum",
+              "ref_topo_random_velocity"):
         getattr(lib, f).argtypes = None
     info_a = np.zeros(12, np.int32)
     h = lib.ref_topo_load(psf.encode(), par.encode(), exclude.encode(), dihedral_mult_psf,
@@ -378,6 +379,15 @@ def ref_remove_momentum(topo: dict, pos, vel, linear=True, angular=True) -> np.n
     v = np.array(vel, np.float64, copy=True, order="C")
     topo["lib"].ref_topo_remove_momentum(topo["handle"], p.ctypes.data_as(_D), v.ctypes.data_as(_D),
                                          int(linear), int(angular))
+    return v
+
+
+def ref_random_velocity(topo: dict, temperature: float, seed: int) -> np.ndarray:
+    """randomVelocity of the reference build; its generator is seeded on the
+    first call in a process only."""
+    v = np.zeros((topo["info"]["atoms"], 3))
+    topo["lib"].ref_topo_random_velocity(topo["handle"], C.c_double(temperature), C.c_uint(seed),
+                                         v.ctypes.data_as(_D))
     return v
 
 

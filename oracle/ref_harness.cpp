@@ -600,4 +600,14 @@ int ref_topo_remove_momentum(void* h, const double* pos, double* vel, int linear
   return 0;
 }
 
+// randomVelocity (topologyutilities.cpp:24-58) on the loaded topology.  The
+// reference's generator is seeded on its first use in the process only.
+int ref_topo_random_velocity(void* h, double temperature, unsigned int seed, double* vel) {
+  Topo* t = (Topo*)h;
+  Vector3DBlock v;
+  randomVelocity(temperature, t, &v, seed);
+  for (size_t i = 0; i < t->atoms.size(); i++) { vel[3 * i] = v[i].x; vel[3 * i + 1] = v[i].y; vel[3 * i + 2] = v[i].z; }
+  return 0;
+}
+
 }  // extern "C"

@@ -19,7 +19,9 @@ SOURCES = ["pm_nonbonded.hip", "pm_pme.hip", "pm_md.hip", "pm_bonded.hip", "pm_k
            "pm_api.cpp"]
 FRONT = os.path.join(HERE, "front")
 FRONT_LIB = os.path.join(HERE, "libpmfront.so")
-FRONT_SOURCES = ["pmf_read.cpp", "pmf_build.cpp", "pmf_write.cpp", "pmf_api.cpp"]
+FRONT_SOURCES = ["pmf_read.cpp", "pmf_build.cpp", "pmf_write.cpp", "pmf_conf.cpp", "pmf_run.cpp", "pmf_api.cpp"]
+PMRUN = os.path.join(HERE, "pmrun")
+ROCM_LIB = os.environ.get("ROCM_LIB", "/opt/rocm/lib")
 CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -68,7 +70,23 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_pmrun(verbose)
     return LIB
+
+
+def build_pmrun(verbose: bool = True) -> str:
+    """The .conf driver: links the front end and the device library."""
+    tmp = PMRUN + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I" + FRONT, "-I" + os.path.join(ROOT, "include"), "-o", tmp,
+           os.path.join(FRONT, "pmrun.cpp"), "-L" + HERE, "-lpmfront", "-lpmgpu", "-L" + ROCM_LIB,
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + ROCM_LIB, "-Wl,--allow-shlib-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"pmrun link failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, PMRUN)
+    if verbose:
+        print(f"built {PMRUN}")
+    return PMRUN
 
 
 if __name__ == "__main__":

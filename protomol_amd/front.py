@@ -50,6 +50,9 @@ def load_library(path: str = LIB_PATH):
     lib.pmf_energies_line.argtypes = [_D, C.c_char_p, C.c_int]
     lib.pmf_write_xyz.argtypes = [C.c_char_p, C.c_void_p, _D, C.c_int, C.c_char_p]
     lib.pmf_write_pdb.argtypes = [C.c_char_p, C.c_char_p, _D, C.c_int, C.c_char_p]
+    lib.pmf_force_values.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _D, C.c_int, _I]
+    lib.pmf_remove_momentum.argtypes = [C.c_void_p, _D, _D, C.c_int, C.c_int]
+    lib.pmf_random_velocity.argtypes = [C.c_void_p, C.c_double, C.c_uint, _D]
     _lib = lib
     return lib
 
@@ -168,6 +171,45 @@ def load_system(psf: str, par: str, exclude: str = "scaled1-4", dihedral_mult_ps
         lib.pmf_free_system(h)
     return Topology(info, names, q, mass, typ, mol, *lj, ba, bb, bk, br, aa, ab, ac, ak, at, au, ar,
                     tors[0], tors[1], ea, eb, ec)
+
+
+def force_values(definition: str):
+    """(force id, positional values) of a .conf force definition."""
+    lib = load_library()
+    buf = C.create_string_buffer(512)
+    vals = np.zeros(32)
+    n = C.c_int()
+    _check(lib, lib.pmf_force_values(definition.encode(), buf, 512, _dp(vals), 32, C.byref(n)))
+    return buf.value.decode(), vals[:n.value].copy()
+
+
+def _with_system(psf, par, exclude, fn):
+    lib = load_library()
+    h = C.c_void_p()
+    _check(lib, lib.pmf_load_system(psf.encode(), par.encode(), exclude.encode(), -1, C.byref(h)))
+    try:
+        return fn(lib, h)
+    finally:
+        lib.pmf_free_system(h)
+
+
+def remove_momentum(psf: str, par: str, x, v, linear=True, angular=True, exclude="scaled1-4") -> np.ndarray:
+    xx = np.ascontiguousarray(x, np.float64)
+    vv = np.array(v, np.float64, copy=True, order="C")
+    _with_system(psf, par, exclude,
+                 lambda lib, h: _check(lib, lib.pmf_remove_momentum(h, _dp(xx), _dp(vv), int(linear), int(angular))))
+    return vv
+
+
+def random_velocity(psf: str, par: str, temperature: float, seed: int, exclude="scaled1-4") -> np.ndarray:
+    def go(lib, h):
+        n = C.c_int()
+        info = np.zeros(len(INFO_NAMES), np.int32)
+        _check(lib, lib.pmf_system_info(h, _ip(info)))
+        v = np.zeros((int(info[0]), 3))
+        _check(lib, lib.pmf_random_velocity(h, float(temperature), int(seed), _dp(v)))
+        return v
+    return _with_system(psf, par, exclude, go)
 
 
 def read_coords(path: str) -> np.ndarray:

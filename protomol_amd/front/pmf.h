@@ -158,16 +158,26 @@ struct Config {
   std::string dir;                                             // directory of the .conf file
   std::vector<std::pair<std::string, std::string>> keys;       // lower-cased keyword -> value, last wins
   std::vector<LevelSpec> levels;
+  std::vector<std::string> ignored;                           // words that are no keyword
   bool has(const std::string& k) const;
   std::string get(const std::string& k, const std::string& def = "") const;
   double real(const std::string& k, double def) const;
   long integer(const std::string& k, long def) const;
   bool boolean(const std::string& k, bool def) const;
-  std::string path(const std::string& k) const;               // relative to dir
+  std::string path(const std::string& k) const;               // a file name (relative to the working directory)
 };
 bool read_conf(const std::string& path, Config& cfg, std::string& err);
 // "--keyword value" overrides from a command line (parseCommandLine.cpp)
 bool apply_overrides(Config& cfg, int argc, const char* const* argv, std::string& err);
+
+// -------------------------------------------------------------- start-up
+// A force definition of the .conf (ForceSpec::id) as the device library's
+// force id and positional parameter values (pm_force_create).
+bool force_values(const ForceSpec& fs, std::string& id, std::vector<double>& values, std::string& err);
+std::vector<char> water_atoms(const System& s);
+void remove_momentum(const System& s, const std::vector<double>& x, std::vector<double>& v, bool linear,
+                     bool angular);
+std::vector<double> random_velocity(const System& s, double temperature, unsigned int seed);
 
 // ------------------------------------------------------------------ helpers
 bool to_real(const std::string& w, double& v);

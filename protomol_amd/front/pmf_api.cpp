@@ -201,4 +201,35 @@ int pmf_write_pdb(const char* path, const char* template_pdb, const double* xyz,
   return PMF_OK;
 }
 
+int pmf_force_values(const char* definition, char* id, int id_cap, double* values, int cap, int* nvalues) {
+  if (!definition || !id || id_cap <= 0 || !nvalues) return fail(PMF_ERR_ARGUMENT, "pmf_force_values: null argument");
+  pmf::ForceSpec fs;
+  fs.id = definition;
+  std::string out, err;
+  std::vector<double> v;
+  if (!pmf::force_values(fs, out, v, err)) return fail(PMF_ERR_PARSE, err);
+  if ((int)out.size() >= id_cap || (int)v.size() > cap) return fail(PMF_ERR_ARGUMENT, "pmf_force_values: buffer too small");
+  std::memcpy(id, out.c_str(), out.size() + 1);
+  if (values && !v.empty()) std::memcpy(values, v.data(), v.size() * sizeof(double));
+  *nvalues = (int)v.size();
+  return PMF_OK;
+}
+
+int pmf_remove_momentum(const pmf_system* h, const double* x, double* v, int linear, int angular) {
+  if (!h || !x || !v) return fail(PMF_ERR_ARGUMENT, "pmf_remove_momentum: null argument");
+  const size_t m = 3 * (size_t)h->s.n;
+  std::vector<double> xv(x, x + m), vv(v, v + m);
+  pmf::remove_momentum(h->s, xv, vv, linear != 0, angular != 0);
+  std::memcpy(v, vv.data(), m * sizeof(double));
+  return PMF_OK;
+}
+
+int pmf_random_velocity(const pmf_system* h, double temperature, unsigned int seed, double* v) {
+  if (!h || !v) return fail(PMF_ERR_ARGUMENT, "pmf_random_velocity: null argument");
+  if (temperature < 0) return fail(PMF_ERR_ARGUMENT, "pmf_random_velocity: temperature < 0");
+  const std::vector<double> r = pmf::random_velocity(h->s, temperature, seed);
+  std::memcpy(v, r.data(), r.size() * sizeof(double));
+  return PMF_OK;
+}
+
 }  // extern "C"
