@@ -27,3 +27,20 @@ def test_library_exports_all_symbols():
     for name in declared():
         assert hasattr(lib, name), name
     assert lib.pm_abi_version() == 1
+
+
+def declared_front():
+    text = open(os.path.join(ROOT, "include", "pmfront.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pmf_[a-z_]+)\s*\(", text)))
+
+
+def test_front_library_exports_all_symbols():
+    from protomol_amd import front
+    if not os.path.exists(front.LIB_PATH):
+        pytest.skip("libpmfront.so not built")
+    lib = ctypes.CDLL(front.LIB_PATH)
+    names = declared_front()
+    assert len(names) >= 17
+    for name in names:
+        assert hasattr(lib, name), name
