@@ -42,7 +42,9 @@ bool is_known_key(const std::string& k) {
       "angularmomentum", "shake", "shakeepsilon", "shakemaxiter", "rattle", "rattleepsilon", "rattlemaxiter",
       "dihedralmultpsf", "reducedimage", "minimalimage", "diffusionfile", "diffusionfileoutputfreq", "momentumfile",
       "momentumfileoutputfreq", "tempfile", "tempfileoutputfreq", "shadowenergy", "shadowfreq", "shadoworder",
-      "checkpointfreq", "checkpointfile", "eigfile", "eigvalfile", "screenoutputfreq", "precision"};
+      "checkpointfreq", "checkpointfile", "eigfile", "eigvalfile", "screenoutputfreq", "precision",
+      "remtemperaturefile", "numswitches", "eqswitches", "tempspace", "remexchangeratesfile",
+      "remexchangeratesfileoutputfreq", "remhistoryfile", "remhistoryfileoutputfreq"};
   for (const char* x : known)
     if (k == x) return true;
   return false;

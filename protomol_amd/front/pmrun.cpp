@@ -45,6 +45,74 @@ bool vec3(const pmf::Config& cfg, const char* key, double out[3]) {
   return std::sscanf(cfg.get(key).c_str(), "%lf %lf %lf", &out[0], &out[1], &out[2]) == 3;
 }
 
+// Replica exchange as applications/rem-app/rem.cpp:604-720 runs it: one
+// replica per temperature of the REMTemperatureFile (one per line, in
+// ladder order), NumSwitches rounds of numsteps steps, an exchange attempt
+// after every round from EqSwitches on (neighbours of the round's parity,
+// pm_batch_exchange); an accepted switch is credited to the replica of
+// the higher temperature, and REMExchangeRatesFile holds each replica's
+// credits divided by the rounds that attempted switches (the first line is
+// always 0).  All replicas run on this GPU as one batch.
+template <class MakeCtx>
+int run_rem(const pmf::Config& cfg, const pmf::System& sys, const std::vector<double>& x, const std::vector<double>& v,
+            long lin, long ang, int integrator, MakeCtx& make_ctx) {
+  std::string err;
+  if (integrator != PM_INTEGRATOR_LANGEVIN) return die("replica exchange runs LangevinLeapfrog replicas");
+  FILE* tf = std::fopen(cfg.path("remtemperaturefile").c_str(), "r");
+  if (!tf) return die("cannot open REMTemperatureFile " + cfg.get("remtemperaturefile"));
+  std::vector<double> temps;
+  double t;
+  while (std::fscanf(tf, "%lf", &t) == 1) temps.push_back(t);
+  std::fclose(tf);
+  if (temps.size() < 2) return die("REMTemperatureFile needs at least two temperatures");
+  const int nsw = (int)cfg.integer("numswitches", -1);
+  const int eq = (int)cfg.integer("eqswitches", 0);
+  const int len = (int)cfg.integer("numsteps", 0);
+  if (nsw < 1 || len < 1) return die("replica exchange needs NumSwitches >= 1 and numsteps >= 1");
+  const int R = (int)temps.size();
+  std::vector<Device> devs(R);
+  std::vector<pm_context*> ctxs(R);
+  std::vector<int> handles;
+  for (int r = 0; r < R; r++) {
+    // every replica starts from the .conf's positions with velocities at its own temperature
+    std::vector<double> vr = v;
+    if (!cfg.has("velfile")) {
+      vr = pmf::random_velocity(sys, temps[r], (unsigned)cfg.integer("seed", 1234));
+      if (lin >= 0 || ang >= 0) pmf::remove_momentum(sys, x, vr, lin >= 0, ang >= 0);
+    }
+    if (!make_ctx(devs[r], temps[r], x, vr, handles, err)) return die(err);
+    ctxs[r] = devs[r].c;
+  }
+  pm_batch* batch = nullptr;
+  if (pm_batch_create(ctxs.data(), R, &batch) != PM_OK) return die("pm_batch_create failed");
+  std::unique_ptr<pm_batch, void (*)(pm_batch*)> hold(batch, pm_batch_destroy);
+  std::vector<double> credit(R, 0.0);
+  std::vector<int> acc(R, 0);
+  const unsigned long long seed = (unsigned long long)cfg.integer("seed", 1234);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < nsw; i++) {
+    if (pm_batch_run(batch, len) != PM_OK) return die(std::string("pm_batch_run: ") + pm_batch_last_error(batch));
+    if (i < eq) continue;
+    std::fill(acc.begin(), acc.end(), 0);
+    if (pm_batch_exchange(batch, i, seed, acc.data(), nullptr) != PM_OK)
+      return die(std::string("pm_batch_exchange: ") + pm_batch_last_error(batch));
+    for (int k = (i & 1); k + 1 < R; k += 2)
+      if (acc[k]) credit[k + 1] += 1.0;
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (cfg.has("remexchangeratesfile")) {
+    FILE* f = std::fopen(cfg.path("remexchangeratesfile").c_str(), "w");
+    if (!f) return die("cannot open " + cfg.path("remexchangeratesfile"));
+    for (int r = 0; r < R; r++) std::fprintf(f, "%g\n", credit[r] / (nsw - eq));
+    std::fclose(f);
+  }
+  std::printf("Exchange counts:");
+  for (int r = 0; r < R; r++) std::printf(" %g", credit[r]);
+  std::printf("\nREM: %d replicas x %d switches x %d steps in %.3f s (%.0f replica-steps/s)\n", R, nsw, len, secs,
+              secs > 0 ? (double)R * nsw * len / secs : 0.0);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -124,72 +192,88 @@ int main(int argc, char** argv) {
   if (!(dt > 0)) return die("the integrator needs a positive timestep");
 
   // --------------------------------------------------------------- device
-  Device dev;
-  if (pm_create(0, &dev.c) != PM_OK) return die("no HIP device");
   const std::string prec = cfg.get("precision", "double");
-  if (!dev.ok(pm_set_precision(dev.c, prec == "mixed" ? PM_PRECISION_MIXED : PM_PRECISION_DOUBLE), err)) return die(err);
-  const int nt = (int)sys.type_name.size();
-  if (!dev.ok(pm_set_topology(dev.c, n, sys.q.data(), sys.type.data(), nt, sys.ljA.data(), sys.ljB.data(),
-                              sys.ljA14.data(), sys.ljB14.data(), sys.molecule.data(), sys.mass.data(),
-                              (int)sys.ex_a.size(), sys.ex_a.data(), sys.ex_b.data(), sys.ex_cls.data(), 1.0),
-              err))
-    return die(err);
-  if (!sys.bond_a.empty() || !sys.angle_a.empty())
-    if (!dev.ok(pm_set_bonded(dev.c, (int)sys.bond_a.size(), sys.bond_a.data(), sys.bond_b.data(), sys.bond_k.data(),
-                              sys.bond_r0.data(), (int)sys.angle_a.size(), sys.angle_a.data(), sys.angle_b.data(),
-                              sys.angle_c.data(), sys.angle_k.data(), sys.angle_t0.data(), sys.angle_kub.data(),
-                              sys.angle_rub.data()),
-                err))
-      return die(err);
-  for (int kind = 0; kind < 2; kind++) {
-    const auto& list = kind ? sys.impropers : sys.dihedrals;
-    if (list.empty()) continue;
-    std::vector<int> at, row{0}, per;
-    std::vector<double> k, ph;
-    for (const auto& t : list) {
-      at.insert(at.end(), t.a, t.a + 4);
-      k.insert(k.end(), t.k.begin(), t.k.end());
-      per.insert(per.end(), t.n.begin(), t.n.end());
-      ph.insert(ph.end(), t.phase.begin(), t.phase.end());
-      row.push_back((int)k.size());
+  const bool shake = cfg.boolean("shake", false) && cfg.real("shakeepsilon", 1e-5) > 0 && cfg.integer("shakemaxiter", 30) > 0;
+  const bool rattle =
+      cfg.boolean("rattle", false) && cfg.real("rattleepsilon", 1e-5) > 0 && cfg.integer("rattlemaxiter", 30) > 0;
+  int nconstraints = 0;
+  // one context with the topology, forces and integrator of the .conf at a
+  // bath temperature (the level's own for plain runs, the ladder's for REM)
+  auto make_ctx = [&](Device& dev, double temperature, const std::vector<double>& x0, const std::vector<double>& v0,
+                      std::vector<int>& handles, std::string& err) -> bool {
+    if (pm_create(0, &dev.c) != PM_OK) {
+      err = "no HIP device";
+      return false;
     }
-    if (!dev.ok(pm_set_torsions(dev.c, kind, (int)list.size(), at.data(), row.data(), k.data(), per.data(), ph.data()),
+    if (!dev.ok(pm_set_precision(dev.c, prec == "mixed" ? PM_PRECISION_MIXED : PM_PRECISION_DOUBLE), err)) return false;
+    const int nt = (int)sys.type_name.size();
+    if (!dev.ok(pm_set_topology(dev.c, n, sys.q.data(), sys.type.data(), nt, sys.ljA.data(), sys.ljB.data(),
+                                sys.ljA14.data(), sys.ljB14.data(), sys.molecule.data(), sys.mass.data(),
+                                (int)sys.ex_a.size(), sys.ex_a.data(), sys.ex_b.data(), sys.ex_cls.data(), 1.0),
                 err))
-      return die(err);
-  }
-  if (!dev.ok(pm_set_boundary(dev.c, e1, e2, e3, org, cellsize), err)) return die(err);
-  std::vector<int> handles;
+      return false;
+    if (!sys.bond_a.empty() || !sys.angle_a.empty())
+      if (!dev.ok(pm_set_bonded(dev.c, (int)sys.bond_a.size(), sys.bond_a.data(), sys.bond_b.data(),
+                                sys.bond_k.data(), sys.bond_r0.data(), (int)sys.angle_a.size(), sys.angle_a.data(),
+                                sys.angle_b.data(), sys.angle_c.data(), sys.angle_k.data(), sys.angle_t0.data(),
+                                sys.angle_kub.data(), sys.angle_rub.data()),
+                  err))
+        return false;
+    for (int kind = 0; kind < 2; kind++) {
+      const auto& list = kind ? sys.impropers : sys.dihedrals;
+      if (list.empty()) continue;
+      std::vector<int> at, row{0}, per;
+      std::vector<double> k, ph;
+      for (const auto& t : list) {
+        at.insert(at.end(), t.a, t.a + 4);
+        k.insert(k.end(), t.k.begin(), t.k.end());
+        per.insert(per.end(), t.n.begin(), t.n.end());
+        ph.insert(ph.end(), t.phase.begin(), t.phase.end());
+        row.push_back((int)k.size());
+      }
+      if (!dev.ok(pm_set_torsions(dev.c, kind, (int)list.size(), at.data(), row.data(), k.data(), per.data(),
+                                  ph.data()),
+                  err))
+        return false;
+    }
+    if (!dev.ok(pm_set_boundary(dev.c, e1, e2, e3, org, cellsize), err)) return false;
+    handles.clear();
+    for (const auto& fs : L.forces) {
+      std::string id;
+      std::vector<double> vals;
+      if (!pmf::force_values(fs, id, vals, err)) return false;
+      int h;
+      if (!dev.ok(pm_force_create(dev.c, id.c_str(), vals.data(), (int)vals.size(), &h), err)) return false;
+      handles.push_back(h);
+    }
+    if (!dev.ok(pm_md_setup(dev.c, handles.data(), (int)handles.size(), integrator, dt, temperature,
+                            lparam("gamma", 0.0), (unsigned long long)lparam("seed", 1234)),
+                err))
+      return false;
+    if (shake || rattle) {
+      if (!dev.ok(pm_md_set_constraints(dev.c, shake ? cfg.real("shakeepsilon", 1e-5) : 0.0,
+                                        shake ? (int)cfg.integer("shakemaxiter", 30) : 0,
+                                        rattle ? cfg.real("rattleepsilon", 1e-5) : 0.0,
+                                        rattle ? (int)cfg.integer("rattlemaxiter", 30) : 0),
+                  err))
+        return false;
+      pm_md_constraint_count(dev.c, &nconstraints);
+    }
+    return dev.ok(pm_md_set_state(dev.c, x0.data(), v0.data()), err);
+  };
   for (const auto& fs : L.forces) {
     std::string id;
     std::vector<double> vals;
     if (!pmf::force_values(fs, id, vals, err)) return die(err);
-    int h;
-    if (!dev.ok(pm_force_create(dev.c, id.c_str(), vals.data(), (int)vals.size(), &h), err)) return die(err);
-    handles.push_back(h);
     std::printf("Force %s", id.c_str());
     for (double d : vals) std::printf(" %g", d);
     std::printf("\n");
   }
-  if (!dev.ok(pm_md_setup(dev.c, handles.data(), (int)handles.size(), integrator, dt, lparam("temperature", 300.0),
-                          lparam("gamma", 0.0), (unsigned long long)lparam("seed", 1234)),
-              err))
-    return die(err);
-  int dof = sys.dof;
-  const bool shake = cfg.boolean("shake", false) && cfg.real("shakeepsilon", 1e-5) > 0 && cfg.integer("shakemaxiter", 30) > 0;
-  const bool rattle =
-      cfg.boolean("rattle", false) && cfg.real("rattleepsilon", 1e-5) > 0 && cfg.integer("rattlemaxiter", 30) > 0;
-  if (shake || rattle) {
-    if (!dev.ok(pm_md_set_constraints(dev.c, shake ? cfg.real("shakeepsilon", 1e-5) : 0.0,
-                                      shake ? (int)cfg.integer("shakemaxiter", 30) : 0,
-                                      rattle ? cfg.real("rattleepsilon", 1e-5) : 0.0,
-                                      rattle ? (int)cfg.integer("rattlemaxiter", 30) : 0),
-                err))
-      return die(err);
-    int nc = 0;
-    pm_md_constraint_count(dev.c, &nc);
-    dof -= nc;
-  }
-  if (!dev.ok(pm_md_set_state(dev.c, x.data(), v.data()), err)) return die(err);
+  if (cfg.has("remtemperaturefile")) return run_rem(cfg, sys, x, v, lin, ang, integrator, make_ctx);
+  Device dev;
+  std::vector<int> handles;
+  if (!make_ctx(dev, lparam("temperature", 300.0), x, v, handles, err)) return die(err);
+  const int dof = sys.dof - nconstraints;
 
   // -------------------------------------------------------------- outputs
   const int first = (int)cfg.integer("firststep", 0);

@@ -146,3 +146,88 @@ def test_pmrun_reports_unsupported(tmp_path):
     r = subprocess.run([PMRUN, "water.216.lf.conf", "--boundaryConditions", "Vacuum"], cwd=tmp,
                        capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "periodic" in r.stderr
+
+
+REM_CONF = """
+numsteps 20
+NumSwitches 7
+EqSwitches 1
+REMTemperatureFile temps.txt
+REMExchangeRatesFile rates.out
+psfFile water.216.psf
+parFile water.216.par
+posFile water.216.equil.pdb
+temperature 300
+seed 7
+boundaryConditions Periodic
+cellBasisVector1 30.8927 0 0
+cellBasisVector2 0 27.8485 0
+cellBasisVector3 0 0 30.7217
+cellorigin 0.98035 -0.08475 0.19285
+cellManager Cubic
+cellsize 7
+exclude scaled1-4
+Integrator {
+  level 0 LangevinLeapfrog {
+    timestep 1
+    temperature 300
+    gamma 5
+    seed 7
+    force Angle, Bond
+    force LennardJones -algorithm NonbondedCutoff -switchingFunction C2 -switchon 11 -cutoff 13
+    force Coulomb -algorithm NonbondedCutoff -switchingFunction C1 -cutoff 13
+  }
+}
+"""
+TEMPS = [300.0, 303.0, 306.0, 309.0, 312.0]
+
+
+def test_pmrun_replica_exchange_rates(tmp_path, gpu_ctx_factory):
+    """REM mode (rem.cpp:604-720 semantics): the rates file against the same
+    replicas driven through the Python batch binding, credits to the
+    higher-temperature partner, attempts from EqSwitches on with the round's
+    parity."""
+    from protomol_amd import gpu
+    tmp = str(tmp_path)
+    for f in FILES[1:4]:
+        shutil.copy(os.path.join(G, f), tmp)
+    open(os.path.join(tmp, "rem.conf"), "w").write(REM_CONF)
+    open(os.path.join(tmp, "temps.txt"), "w").write("\n".join(str(t) for t in TEMPS) + "\n")
+    say("case: pmrun REM water216 5 replicas 7 switches")
+    r = subprocess.run([PMRUN, "rem.conf", "--precision", "double"], cwd=tmp, capture_output=True, text=True,
+                       timeout=120)
+    say(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0, r.stderr
+    rates = np.loadtxt(os.path.join(tmp, "rates.out"))
+    # the same replicas through the binding
+    psf, par = (os.path.join(G, f) for f in FILES[1:3])
+    t = front.load_system(psf, par)
+    x = front.read_coords(os.path.join(G, FILES[3]))
+    box, org = (30.8927, 27.8485, 30.7217), (0.98035, -0.08475, 0.19285)
+    ctxs = []
+    defs = ["Angle", "Bond", "LennardJones -algorithm NonbondedCutoff -switchingFunction C2 -switchon 11 -cutoff 13",
+            "Coulomb -algorithm NonbondedCutoff -switchingFunction C1 -cutoff 13"]
+    for T in TEMPS:
+        v = front.random_velocity(psf, par, T, 7)
+        s = t.system(x, v, box=box, origin=org)
+        c = gpu_ctx_factory("double")
+        c.set_topology(s)
+        c.set_boundary(s.box, s.origin, 7.0)
+        hs = [c.force(*front.force_values(d)) for d in defs]
+        c.md_setup(hs, "langevin", dt=1.0, temperature=T, gamma=5.0, seed=7)
+        c.md_set_state(s.pos, s.vel)
+        ctxs.append(c)
+    b = gpu.Batch(ctxs)
+    credit = np.zeros(len(TEMPS))
+    for i in range(7):
+        b.run(20)
+        if i < 1:
+            continue
+        acc, _ = b.exchange(i, 7)
+        for k in range(i & 1, len(TEMPS) - 1, 2):
+            credit[k + 1] += acc[k]
+    b.close()
+    say("rates", rates, "binding", credit / 6)
+    assert rates[0] == 0
+    assert credit.sum() > 0                    # neighbours 3 K apart do exchange
+    np.testing.assert_allclose(rates, credit / 6, rtol=1e-5)      # printed with 6 significant digits
