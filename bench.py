@@ -179,6 +179,90 @@ def rigid_bench(steps=300, warmup=30, dt_fs=2.0, eps=1e-5, max_iter=30):
             "bond_error_mean_end": float(err_b.mean())}
 
 
+EXAMPLES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "examples")
+
+
+def conf_bench():
+    """The reference's own example files through the front end: the whole
+    examples/water_216/water.216.lf.conf run by pmrun (FP64, 5000 steps of
+    0.1 fs), and one evaluation of the WW-domain protein's bonded terms
+    (Bond, Angle, Dihedral, Improper; FP64) -- each beside the reference
+    build's CPU time for the same force evaluation (one core)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from protomol_amd import front, gpu
+    from protomol_amd.build import PMRUN
+    out = {}
+    w = os.path.join(EXAMPLES, "water_216")
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in os.listdir(w):
+            shutil.copy(os.path.join(w, f), tmp)
+        r = subprocess.run([PMRUN, "water.216.lf.conf"], cwd=tmp, capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError("pmrun failed: " + r.stderr[-500:])
+        m = re.search(r"Run: (\d+) steps of ([\d.]+) fs on (\d+) atoms in ([\d.]+) s \(([\d.]+) ms/step, ([\d.]+) ns/day",
+                      r.stdout)
+        lines = open(os.path.join(tmp, "water.216.energies.lf")).read().splitlines()
+        etot = [float(l.split()[3]) for l in lines]
+    out["water216_lf_conf"] = {
+        "workload": "examples/water_216/water.216.lf.conf unchanged: 648 atoms, LJ C2 11-13 A + Coulomb C1 13 A "
+                    "cutoffs, bonds + angles, leapfrog 0.1 fs, FP64",
+        "steps": int(m.group(1)), "ms_per_step": float(m.group(5)), "ns_per_day": float(m.group(6)),
+        "seconds": float(m.group(4)), "energy_lines": len(lines),
+        "total_energy_range_kcal_mol": max(etot) - min(etot)}
+    t = front.load_system(os.path.join(EXAMPLES, "wwd", "wwd_mineq.psf"),
+                          os.path.join(EXAMPLES, "wwd", "par_all27_prot_lipid.inp"))
+    x = front.read_coords(os.path.join(EXAMPLES, "wwd", "wwd.min.pdb"))
+    s = t.system(x, box=np.full(3, 80.0), origin=x.mean(0))
+    ctx = gpu.Context(0, "double")
+    ctx.set_topology(s)
+    ctx.set_boundary(s.box, s.origin, 8.0)
+    hs = [ctx.force(k, []) for k in ("Bond", "Angle", "Dihedral", "Improper")]
+    f = np.zeros((s.n, 3))
+    e = np.zeros(16)
+    for _ in range(5):
+        ctx.evaluate(hs, s.pos, forces=f, energies=e)
+    reps = 200
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.evaluate(hs, s.pos, forces=f, energies=e)
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    _, e1 = ctx.evaluate(hs, s.pos)
+    ctx.close()
+    out["wwd_bonded"] = {"workload": "WW domain (551 atoms, par_all27): Bond + Angle + Dihedral + Improper, one "
+                                     "evaluation through pm_evaluate (host round trip included), FP64",
+                         "ms_per_evaluation": gpu_ms,
+                         "energies": {"bond": float(e1[2]), "angle": float(e1[3]), "dihedral": float(e1[4]),
+                                      "improper": float(e1[5])}}
+    try:
+        from oracle import pyoracle as po
+        kind = "reference" if po.available("reference") else "port"
+        o = po.Oracle(kind)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            o.bonded(s, 3)
+            o.torsions(s, t.dihedrals, 0)
+            o.torsions(s, t.impropers, 1)
+        out["wwd_bonded"]["cpu_ms_per_evaluation"] = (time.perf_counter() - t0) * 1e3 / 20
+        ws = front.load_system(os.path.join(w, "water.216.psf"), os.path.join(w, "water.216.par"))
+        xw = front.read_coords(os.path.join(w, "water.216.equil.pdb"))
+        sw = ws.system(xw, box=(30.8927, 27.8485, 30.7217), origin=(0.98035, -0.08475, 0.19285))
+        t0 = time.perf_counter()
+        for _ in range(10):
+            o.cutoff(sw, [po.term(po.POT_LJ, po.SWF_C2, 13.0, 11.0)], 13.0, 7.0)
+            o.cutoff(sw, [po.term(po.POT_COULOMB, po.SWF_C1, 13.0)], 13.0, 7.0)
+            o.bonded(sw, 3)
+        cpu_ms = (time.perf_counter() - t0) * 1e3 / 10
+        out["water216_lf_conf"]["cpu_ms_per_step"] = cpu_ms
+        out["water216_lf_conf"]["cpu_ns_per_day"] = 0.1 * 1e-6 / (cpu_ms * 1e-3 / 86400)
+        out["cpu_kind"] = kind
+    except Exception as ex:  # informational only
+        out["cpu_error"] = str(ex)
+    return out
+
+
 def copy_probe(mb):
     """Ten device-to-device copies of `mb` megabytes: the kernel whose
     FETCH_SIZE / WRITE_SIZE counters calibrate the HBM-traffic runs."""
@@ -338,6 +422,7 @@ def main():
                     help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
     ap.add_argument("--no-c3", action="store_true", help="skip the million-atom measurement")
     ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-water (SHAKE, 2 fs) measurement")
+    ap.add_argument("--no-conf", action="store_true", help="skip the example .conf / protein measurements")
     ap.add_argument("--only-replicas", action="store_true",
                     help="run only the batched replica-exchange measurement and print its record")
     ap.add_argument("--autotune", action="store_true",
@@ -433,6 +518,8 @@ def main():
         line["c3_million_atoms"] = c3_bench()
     if not args.no_rigid:
         line["rigid_water_shake"] = rigid_bench()
+    if not args.no_conf:
+        line["reference_examples"] = conf_bench()
     if not args.no_drift:
         line["nve_drift"] = {"mixed": nve_drift(s, "mixed"), "fp64": nve_drift(s, "double")}
     if not args.no_cpu_baseline:
