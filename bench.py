@@ -182,6 +182,40 @@ def rigid_bench(steps=300, warmup=30, dt_fs=2.0, eps=1e-5, max_iter=30):
 EXAMPLES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "examples")
 
 
+def tile_system(s, t, reps, spacing):
+    """reps^3 copies of a system (front.Topology t, system s) on a cubic
+    lattice of the given spacing, in one periodic box."""
+    from protomol_amd import front
+    n = s.n
+    k = reps ** 3
+    g = np.stack(np.meshgrid(*[np.arange(reps)] * 3, indexing="ij"), -1).reshape(-1, 3) * spacing
+    pos = (s.pos[None, :, :] - s.pos.mean(0) + g[:, None, :]).reshape(-1, 3)
+    off = (np.arange(k) * n)[:, None]
+
+    def idx(a):
+        return (np.asarray(a)[None, :] + off).reshape(-1).astype(np.int32)
+
+    def tors(T):
+        m = len(T.atoms)
+        u = len(T.k)
+        row = (T.term_row[None, :-1] + (np.arange(k) * u)[:, None]).reshape(-1)
+        return front.Torsions((T.atoms[None] + off[:, :, None]).reshape(-1, 4).astype(np.int32),
+                              np.append(row, k * u).astype(np.int32), np.tile(T.k, k),
+                              np.tile(T.period, k).astype(np.int32), np.tile(T.phase, k))
+    big = t.system(pos, box=np.full(3, reps * spacing), origin=np.zeros(3))
+    big.q, big.mass, big.type = np.tile(s.q, k), np.tile(s.mass, k), np.tile(s.type, k).astype(np.int32)
+    big.molecule = (np.asarray(s.molecule)[None, :] + (np.arange(k) * (int(s.molecule.max()) + 1))[:, None]
+                    ).reshape(-1).astype(np.int32)
+    big.ex_a, big.ex_b, big.ex_cls = idx(s.ex_a), idx(s.ex_b), np.tile(s.ex_cls, k).astype(np.int32)
+    big.bond_a, big.bond_b = idx(s.bond_a), idx(s.bond_b)
+    big.bond_k, big.bond_r0 = np.tile(s.bond_k, k), np.tile(s.bond_r0, k)
+    big.angle_a, big.angle_b, big.angle_c = idx(s.angle_a), idx(s.angle_b), idx(s.angle_c)
+    for f in ("angle_k", "angle_t0", "angle_kub", "angle_rub"):
+        setattr(big, f, np.tile(getattr(s, f), k))
+    big.dihedrals, big.impropers = tors(t.dihedrals), tors(t.impropers)
+    return big
+
+
 def conf_bench():
     """The reference's own example files through the front end: the whole
     examples/water_216/water.216.lf.conf run by pmrun (FP64, 5000 steps of
@@ -236,6 +270,39 @@ def conf_bench():
                          "ms_per_evaluation": gpu_ms,
                          "energies": {"bond": float(e1[2]), "angle": float(e1[3]), "dihedral": float(e1[4]),
                                       "improper": float(e1[5])}}
+    # the bonded kernels at a size that fills the chip: 8x8x8 copies of the
+    # protein 60 A apart (282,112 atoms, 760,320 dihedrals), kernel time from
+    # HIP events around the bonded launches
+    reps = 8
+    big = tile_system(s, t, reps, 60.0)
+    ctx = gpu.Context(0, "double")
+    ctx.set_topology(big)
+    ctx.set_boundary(big.box, big.origin, 12.0)
+    hs = [ctx.force(k, []) for k in ("Bond", "Angle", "Dihedral", "Improper")]
+    ctx.set_timing(("bonded",))
+    for _ in range(3):
+        ctx.evaluate(hs, big.pos)
+    ctx.set_timing(("bonded",))
+    nev = 20
+    for _ in range(nev):
+        ctx.evaluate(hs, big.pos)
+    ms, cnt = ctx.timing()["bonded"]
+    ctx.close()
+    nt = {k: len(getattr(big, k)) for k in ("bond_a", "angle_a")}
+    nd, ni = len(big.dihedrals.atoms), len(big.impropers.atoms)
+    # algorithmic bytes of one force + energy evaluation: positions read and
+    # forces written once per atom (FP64), every term's atoms and parameters
+    # read once (bond 2 int + 2 double, angle 3 int + 4 double, torsion 4 int
+    # + term row + per term double k, int n, double phase)
+    nterm_d, nterm_i = len(big.dihedrals.k), len(big.impropers.k)
+    algo = (big.n * 48 + nt["bond_a"] * 24 + nt["angle_a"] * 44 + (nd + ni) * 20 + (nterm_d + nterm_i) * 20)
+    us = ms * 1e3 / nev
+    out["protein_bonded_tiled"] = {
+        "workload": "%d copies of the WW domain (%d atoms; %d bonds, %d angles, %d dihedrals, %d impropers), "
+                    "Bond + Angle + Dihedral + Improper force and energy kernels, FP64" %
+                    (reps ** 3, big.n, nt["bond_a"], nt["angle_a"], nd, ni),
+        "us_per_evaluation": us, "atoms_per_s": big.n / (us * 1e-6),
+        "algorithmic_bytes": algo, "achieved_GBps": algo / (us * 1e-6) / 1e9, "hbm_peak_GBps": 8000.0}
     try:
         from oracle import pyoracle as po
         kind = "reference" if po.available("reference") else "port"

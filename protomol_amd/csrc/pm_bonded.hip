@@ -5,9 +5,11 @@
 // AngleSystemForce::calcAngle (forces/AngleSystemForce.h:113-200) and
 // MTorsionSystemForce::calcTorsion (forces/MTorsionSystemForce.h:34-153) for
 // DihedralSystemForce and ImproperSystemForce.  Forces
-// are gathered per atom (each atom evaluates the terms it belongs to and keeps
-// its own share), so no atomics are needed and the result is deterministic;
-// energies and virials are summed per term in fixed-order block reductions.
+// are computed once per term and added to the atoms as 64-bit fixed-point
+// atomics (deterministic: integer sums do not depend on their order); the
+// PME gather's per-atom tail (pm_terms.h bonded_atom_force) is the other
+// route.  Energies and virials are summed per term in fixed-order block
+// reductions.
 #include "pm_internal.h"
 #include "pm_terms.h"
 
@@ -15,43 +17,65 @@
 
 namespace {
 
-__device__ inline void bonded_force_body(const PmBox& box, const PmBondedLaunch& a, int bx) {
-  const int i = bx * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  double f[3];
-  bonded_atom_force(box, a, i, f);
-  a.force_fix[3 * i] += __double2ll_rn(f[0] * PM_FORCE_FIX);
-  a.force_fix[3 * i + 1] += __double2ll_rn(f[1] * PM_FORCE_FIX);
-  a.force_fix[3 * i + 2] += __double2ll_rn(f[2] * PM_FORCE_FIX);
-}
-__global__ void k_bonded_force(PmBox box, PmBondedLaunch a) { bonded_force_body(box, a, blockIdx.x); }
-
-struct BondedArgs { PmBox box; PmBondedLaunch a; };
-__global__ void kb_bonded_force(const BondedArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
-  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
-  bonded_force_body(t[r].box, t[r].a, blockIdx.x - pre[r]);
-}
-
-// per-term energies and virials; records: bond, angle, dihedral and
-// improper blocks in that order (kind of block bx from the prefix off[])
+// One thread per term (bond, angle, dihedral, improper; blocks of each kind
+// in that order, kind of block bx from the prefix off[]): each term is
+// evaluated once, its forces go to its atoms as 64-bit fixed-point atomic
+// adds (integer addition is associative, so the sums do not depend on the
+// order of the atomics), and with want_energy the block writes one energy /
+// virial record.  a.n == 0 leaves the forces alone (the PME gather added
+// them per atom).
 struct BondedBlocks { int off[5]; };
-__device__ inline void bonded_energy_body(PmBox box, PmBondedLaunch a, BondedBlocks bl, int bx) {
+__device__ inline void add_fix(long long* f, int i, const double* v, double sg) {
+  for (int c = 0; c < 3; c++)
+    atomicAdd(reinterpret_cast<unsigned long long*>(f + 3 * (long long)i + c),
+              (unsigned long long)__double2ll_rn(sg * v[c] * PM_FORCE_FIX));
+}
+// KIND is a template parameter so that each kind's kernel holds only its
+// own registers (the torsion's FP64 trigonometry would otherwise set the
+// occupancy of all four)
+// Forces are first summed per block in LDS over the block's atom window
+// (the atoms of consecutive terms of a PSF lie close together) with 64-bit
+// LDS atomics, then added to the global fixed-point forces once per atom and
+// component; a block whose atoms span more than PM_BONDED_WINDOW atoms adds
+// straight to global memory.  Both are integer sums: the result does not
+// depend on the order.
+#define PM_BONDED_WINDOW 1024
+template <int KIND>
+__device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBlocks bl, int bx) {
   __shared__ double s_red[4][7];
-  int kind = 0;
-  while (kind < 3 && bx >= bl.off[kind + 1]) kind++;
+  __shared__ unsigned long long s_f[3 * PM_BONDED_WINDOW];
+  __shared__ int s_lo[4], s_hi[4];
+  constexpr int kind = KIND;
+  constexpr int NA = kind == 0 ? 2 : kind == 1 ? 3 : 4;
+  bx += bl.off[kind];           // the block's energy record among all kinds
   const int t = (bx - bl.off[kind]) * blockDim.x + threadIdx.x;
-  const int count = kind == 0 ? a.nb : kind == 1 ? a.na : a.nt[kind - 2];
+  int count;
+  if constexpr (kind == 0) count = a.nb;
+  else if constexpr (kind == 1) count = a.na;
+  else count = a.nt[kind - 2];
+  const bool have = t < count;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (t < count) {
+  int at[4] = {0, 0, 0, 0};
+  double g[4][3];
+  if (have) {
     double f1[3], f2[3] = {0, 0, 0}, f3[3] = {0, 0, 0}, e, r12[3], r23[3] = {0, 0, 0}, r34[3] = {0, 0, 0};
-    if (kind == 0) {
+    if constexpr (kind == 0) {
       bond_term(box, a, t, f1, e, r12);
-    } else if (kind == 1) {
-      // the angle's virial is f1 r12 + f3 r32 (AngleSystemForce.h:180-198)
+      at[0] = a.b_a[t]; at[1] = a.b_b[t];
+      for (int c = 0; c < 3; c++) { g[0][c] = f1[c]; g[1][c] = -f1[c]; }
+    } else if constexpr (kind == 1) {
       angle_term(box, a, t, f1, f2, f3, e, r12, r34);
+      at[0] = a.a_a[t]; at[1] = a.a_b[t]; at[2] = a.a_c[t];
+      for (int c = 0; c < 3; c++) { g[0][c] = f1[c]; g[1][c] = f2[c]; g[2][c] = f3[c]; }
+      // the angle's virial is f1 r12 + f3 r32 (AngleSystemForce.h:180-198)
       f2[0] = f2[1] = f2[2] = 0.0;
     } else {
       torsion_term(box, a, kind - 2, t, f1, f2, f3, e, r12, r23, r34);
+      const int4 q = a.t_atoms[kind - 2][t];
+      at[0] = q.x; at[1] = q.y; at[2] = q.z; at[3] = q.w;
+      for (int c = 0; c < 3; c++) {
+        g[0][c] = f1[c]; g[1][c] = f2[c] - f1[c]; g[2][c] = f3[c] - f2[c]; g[3][c] = -f3[c];
+      }
     }
     v[0] = e;
     if (a.want_virial) {
@@ -64,6 +88,39 @@ __device__ inline void bonded_energy_body(PmBox box, PmBondedLaunch a, BondedBlo
     }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (a.n > 0) {   // uniform over the launch
+    // the block's atom window
+    int lo = 0x7fffffff, hi = -1;
+    if (have)
+      for (int j = 0; j < NA; j++) { lo = min(lo, at[j]); hi = max(hi, at[j]); }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = min(lo, __shfl_xor(lo, off, 64));
+      hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if (lane == 0) { s_lo[wv] = lo; s_hi[wv] = hi; }
+    __syncthreads();
+    lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+    hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+    const int span = hi - lo + 1;
+    const bool local = hi >= lo && span <= PM_BONDED_WINDOW;
+    if (local) {
+      for (int k = threadIdx.x; k < 3 * span; k += blockDim.x) s_f[k] = 0ull;
+      __syncthreads();
+      if (have)
+        for (int j = 0; j < NA; j++)
+          for (int c = 0; c < 3; c++)
+            atomicAdd(&s_f[3 * (at[j] - lo) + c], (unsigned long long)__double2ll_rn(g[j][c] * PM_FORCE_FIX));
+      __syncthreads();
+      for (int k = threadIdx.x; k < 3 * span; k += blockDim.x) {
+        const unsigned long long x = s_f[k];
+        if (x) atomicAdd(reinterpret_cast<unsigned long long*>(a.force_fix + 3 * (long long)lo) + k, x);
+      }
+    } else if (have) {
+      for (int j = 0; j < NA; j++) add_fix(a.force_fix, at[j], g[j], 1.0);
+    }
+  }
+  if (!a.want_energy) return;   // uniform over the launch: no thread reaches the barrier
   for (int u = 0; u < 7; u++) {
     double x = v[u];
 #pragma unroll
@@ -83,14 +140,16 @@ __device__ inline void bonded_energy_body(PmBox box, PmBondedLaunch a, BondedBlo
     ep[PM_E_VIR + 6] = s[3]; ep[PM_E_VIR + 7] = s[5]; ep[PM_E_VIR + 8] = s[6];
   }
 }
-__global__ __launch_bounds__(256) void k_bonded_energy(PmBox box, PmBondedLaunch a, BondedBlocks bl) {
-  bonded_energy_body(box, a, bl, blockIdx.x);
+template <int KIND>
+__global__ __launch_bounds__(256) void k_bonded_terms(PmBox box, PmBondedLaunch a, BondedBlocks bl) {
+  bonded_terms_body<KIND>(box, a, bl, blockIdx.x);
 }
-struct BondedEnergyArgs { PmBox box; PmBondedLaunch a; BondedBlocks bl; };
-__global__ __launch_bounds__(256) void kb_bonded_energy(const BondedEnergyArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+struct BondedTermArgs { PmBox box; PmBondedLaunch a; BondedBlocks bl; };
+template <int KIND>
+__global__ __launch_bounds__(256) void kb_bonded_terms(const BondedTermArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
-  const BondedEnergyArgs& q = t[r];
-  bonded_energy_body(q.box, q.a, q.bl, blockIdx.x - pre[r]);
+  const BondedTermArgs& q = t[r];
+  bonded_terms_body<KIND>(q.box, q.a, q.bl, blockIdx.x - pre[r]);
 }
 
 bool any_bonded(const PmBondedLaunch& a) { return a.do_bond || a.do_angle || a.do_tors[0] || a.do_tors[1]; }
@@ -110,31 +169,31 @@ void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a) 
   BondedBlocks bl;
   bl.off[0] = 0;
   for (int k = 0; k < 4; k++) bl.off[k + 1] = bl.off[k] + nblk[k];
-  const int total = bl.off[4];
-  const bool forces = a.n > 0 && any_bonded(a);
-  if (pm_recording()) {
-    const BondedArgs b{box, a};
-    pm_record(PM_REC_BONDED_FORCE, 0, forces ? (a.n + 255) / 256 : 0, 256, 0, &b, sizeof(b));
-    if (a.want_energy) {
-      const BondedEnergyArgs e{box, a, bl};
-      pm_record(PM_REC_BONDED_ENERGY, 0, (unsigned)total, 256, 0, &e, sizeof(e));
+  // nothing to do when neither forces (a.n > 0) nor energies are wanted
+  const bool work = a.n > 0 || a.want_energy;
+  const BondedTermArgs q{box, a, bl};
+  for (int k = 0; k < 4; k++) {
+    const unsigned blocks = work ? (unsigned)nblk[k] : 0u;
+    if (pm_recording()) {
+      pm_record(PM_REC_BONDED_FORCE, k, blocks, 256, 0, &q, sizeof(q));
+      continue;
     }
-    return;
+    if (blocks == 0) continue;
+    if (k == 0) hipLaunchKernelGGL(k_bonded_terms<0>, dim3(blocks), dim3(256), 0, s, box, a, bl);
+    else if (k == 1) hipLaunchKernelGGL(k_bonded_terms<1>, dim3(blocks), dim3(256), 0, s, box, a, bl);
+    else if (k == 2) hipLaunchKernelGGL(k_bonded_terms<2>, dim3(blocks), dim3(256), 0, s, box, a, bl);
+    else hipLaunchKernelGGL(k_bonded_terms<3>, dim3(blocks), dim3(256), 0, s, box, a, bl);
   }
-  if (forces) hipLaunchKernelGGL(k_bonded_force, dim3((a.n + 255) / 256), dim3(256), 0, s, box, a);
-  if (total > 0 && a.want_energy) hipLaunchKernelGGL(k_bonded_energy, dim3(total), dim3(256), 0, s, box, a, bl);
 }
 
 bool pm_launch_batched_bonded(hipStream_t s, int kind, int variant, const void* t, const int* pre, int nrep,
                               unsigned total, unsigned threads, unsigned lds) {
-  (void)variant; (void)lds;
-  if (kind == PM_REC_BONDED_FORCE) {
-    hipLaunchKernelGGL(kb_bonded_force, dim3(total), dim3(threads), 0, s, (const BondedArgs*)t, pre, nrep);
-    return true;
-  }
-  if (kind == PM_REC_BONDED_ENERGY) {
-    hipLaunchKernelGGL(kb_bonded_energy, dim3(total), dim3(threads), 0, s, (const BondedEnergyArgs*)t, pre, nrep);
-    return true;
-  }
-  return false;
+  (void)lds;
+  if (kind != PM_REC_BONDED_FORCE) return false;
+  const BondedTermArgs* q = (const BondedTermArgs*)t;
+  if (variant == 0) hipLaunchKernelGGL(kb_bonded_terms<0>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
+  else if (variant == 1) hipLaunchKernelGGL(kb_bonded_terms<1>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
+  else if (variant == 2) hipLaunchKernelGGL(kb_bonded_terms<2>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
+  else hipLaunchKernelGGL(kb_bonded_terms<3>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
+  return true;
 }

@@ -75,7 +75,8 @@ __device__ inline void angle_term(const PmBox& b, const PmBondedLaunch& a, int k
   double cr[3] = {r12[1] * r32[2] - r12[2] * r32[1], r12[2] * r32[0] - r12[0] * r32[2],
                   r12[0] * r32[1] - r12[1] * r32[0]};
   const double theta = atan2(norm3(cr), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
-  const double st = sin(theta), ct = cos(theta);
+  double st, ct;
+  sincos(theta, &st, &ct);
   const double dpot = 2.0 * kt * (theta - t0);
   double u1[3], u3[3], ub1[3];
   for (int c = 0; c < 3; c++) {
@@ -121,8 +122,10 @@ __device__ inline void torsion_term(const PmBox& bx, const PmBondedLaunch& a, in
     const int n = a.t_n[kind][u];
     const double k = a.t_k[kind][u], ph = a.t_phase[kind][u];
     if (n > 0) {
-      dpot -= n * k * sin(n * phi + ph);
-      e += k * (1.0 + cos(n * phi + ph));
+      double sn, cs;
+      sincos(n * phi + ph, &sn, &cs);     // one argument reduction for both
+      dpot -= n * k * sn;
+      e += k * (1.0 + cs);
     } else {
       double diff = phi - ph;
       if (diff < -pi) diff += 2 * pi;
