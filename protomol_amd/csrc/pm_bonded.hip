@@ -38,14 +38,15 @@ __device__ inline void add_fix(long long* f, int i, const double* v, double sg) 
 // LDS atomics, then added to the global fixed-point forces once per atom and
 // component; a block whose atoms span more than PM_BONDED_WINDOW atoms adds
 // straight to global memory.  Both are integer sums: the result does not
-// depend on the order.
-#define PM_BONDED_WINDOW 1024
+// depend on the order.  Impropers are sparse (a protein has about one per six
+// atoms), so a block of them spans more atoms: their window is twice as wide.
 template <int KIND>
 __device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBlocks bl, int bx) {
   __shared__ double s_red[4][7];
-  __shared__ unsigned long long s_f[3 * PM_BONDED_WINDOW];
-  __shared__ int s_lo[4], s_hi[4];
   constexpr int kind = KIND;
+  constexpr int W = kind == 3 ? 2048 : 1024;   // atom window summed in LDS
+  __shared__ unsigned long long s_f[3 * W];
+  __shared__ int s_lo[4], s_hi[4];
   constexpr int NA = kind == 0 ? 2 : kind == 1 ? 3 : 4;
   bx += bl.off[kind];           // the block's energy record among all kinds
   const int t = (bx - bl.off[kind]) * blockDim.x + threadIdx.x;
@@ -103,7 +104,7 @@ __device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBloc
     lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
     hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
     const int span = hi - lo + 1;
-    const bool local = hi >= lo && span <= PM_BONDED_WINDOW;
+    const bool local = hi >= lo && span <= W;
     if (local) {
       for (int k = threadIdx.x; k < 3 * span; k += blockDim.x) s_f[k] = 0ull;
       __syncthreads();
