@@ -98,6 +98,12 @@ int pmf_write_xyz(const char* path, const pmf_system* s, const double* xyz, int 
  * with the coordinates xyz. */
 int pmf_write_pdb(const char* path, const char* template_pdb, const double* xyz, int n, const char* comment);
 
+/* DCD trajectory (io/DCDTrajectoryWriter.cpp): nframes frames of n atoms
+ * (xyz[3 * (f * n + i) + c]) with the header's first step and time step (fs)
+ * as the writer takes them (OutputDCDTrajectory uses 1 and 1.0). */
+int pmf_write_dcd(const char* path, int n, int nframes, const double* xyz, int first_step, double timestep_fs,
+                  const char* comment);
+
 /* A force definition of a .conf Integrator block (the words after "force",
  * e.g. "LennardJones -algorithm NonbondedCutoff -switchingFunction C2
  * -switchon 7 -cutoff 8") as the force id and positional values that

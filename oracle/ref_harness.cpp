@@ -42,6 +42,8 @@ using namespace ProtoMol::Report;
 #include "PSF.h"
 #include "PAR.h"
 #include "buildTopology.h"
+#include "DCDTrajectoryWriter.h"
+#include "XYZWriter.h"
 #include "topologyutilities.h"
 
 #include <vector>
@@ -608,6 +610,37 @@ int ref_topo_random_velocity(void* h, double temperature, unsigned int seed, dou
   randomVelocity(temperature, t, &v, seed);
   for (size_t i = 0; i < t->atoms.size(); i++) { vel[3 * i] = v[i].x; vel[3 * i + 1] = v[i].y; vel[3 * i + 2] = v[i].z; }
   return 0;
+}
+
+// DCDTrajectoryWriter (io/DCDTrajectoryWriter.cpp) with its defaults
+// (time step 1, first step 1), one write() per frame; XYZWriter
+// (io/XYZWriter.cpp) of one frame with the given names ('\n'-separated).
+int ref_write_dcd(const char* path, int n, int nframes, const double* xyz) {
+  DCDTrajectoryWriter w(path);
+  for (int f = 0; f < nframes; f++) {
+    Vector3DBlock c;
+    c.resize(n);
+    for (int i = 0; i < n; i++) {
+      const double* p = xyz + 3 * ((size_t)f * n + i);
+      c[i] = Vector3D(p[0], p[1], p[2]);
+    }
+    if (!w.write(c)) return -1;
+  }
+  return 0;
+}
+
+int ref_write_xyz(const char* path, int n, const double* xyz, const char* names) {
+  std::vector<std::string> nm;
+  std::string cur;
+  for (const char* p = names; *p; p++) {
+    if (*p == '\n') { nm.push_back(cur); cur.clear(); } else cur += *p;
+  }
+  if (!cur.empty()) nm.push_back(cur);
+  Vector3DBlock c;
+  c.resize(n);
+  for (int i = 0; i < n; i++) c[i] = Vector3D(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  XYZWriter w(path);
+  return w.write(c, nm) ? 0 : -1;
 }
 
 }  // extern "C"

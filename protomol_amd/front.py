@@ -50,6 +50,7 @@ def load_library(path: str = LIB_PATH):
     lib.pmf_energies_line.argtypes = [_D, C.c_char_p, C.c_int]
     lib.pmf_write_xyz.argtypes = [C.c_char_p, C.c_void_p, _D, C.c_int, C.c_char_p]
     lib.pmf_write_pdb.argtypes = [C.c_char_p, C.c_char_p, _D, C.c_int, C.c_char_p]
+    lib.pmf_write_dcd.argtypes = [C.c_char_p, C.c_int, C.c_int, _D, C.c_int, C.c_double, C.c_char_p]
     lib.pmf_force_values.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _D, C.c_int, _I]
     lib.pmf_remove_momentum.argtypes = [C.c_void_p, _D, _D, C.c_int, C.c_int]
     lib.pmf_random_velocity.argtypes = [C.c_void_p, C.c_double, C.c_uint, _D]
@@ -229,6 +230,19 @@ def energies_line(values) -> str:
     buf = C.create_string_buffer(512)
     _check(lib, lib.pmf_energies_line(_dp(v), buf, 512))
     return buf.value.decode()
+
+
+def write_dcd(path: str, frames, first_step: int = 1, timestep_fs: float = 1.0, comment: str = "") -> None:
+    lib = load_library()
+    x = np.ascontiguousarray(frames, np.float64)
+    nf, n = x.shape[0], x.shape[1]
+    _check(lib, lib.pmf_write_dcd(path.encode(), n, nf, _dp(x), first_step, timestep_fs, comment.encode()))
+
+
+def write_xyz(path: str, psf: str, par: str, xyz, comment: str = "") -> None:
+    xx = np.ascontiguousarray(xyz, np.float64)
+    _with_system(psf, par, "scaled1-4", lambda lib, h: _check(
+        lib, lib.pmf_write_xyz(path.encode(), h, _dp(xx), xx.shape[0], comment.encode())))
 
 
 def write_pdb(path: str, template_pdb: str, xyz, comment: str = "") -> None:

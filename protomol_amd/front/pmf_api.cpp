@@ -201,6 +201,19 @@ int pmf_write_pdb(const char* path, const char* template_pdb, const double* xyz,
   return PMF_OK;
 }
 
+int pmf_write_dcd(const char* path, int n, int nframes, const double* xyz, int first_step, double timestep_fs,
+                  const char* comment) {
+  if (!path || n <= 0 || nframes < 0 || (nframes > 0 && !xyz)) return fail(PMF_ERR_ARGUMENT, "pmf_write_dcd: bad argument");
+  pmf::DcdWriter w;
+  std::string err;
+  if (!w.open(path, n, first_step, 1, timestep_fs, comment ? comment : "", err)) return fail(PMF_ERR_IO, err);
+  for (int f = 0; f < nframes; f++)
+    if (!w.frame(std::vector<double>(xyz + 3 * (size_t)f * n, xyz + 3 * (size_t)(f + 1) * n), err))
+      return fail(PMF_ERR_IO, err);
+  w.close();
+  return PMF_OK;
+}
+
 int pmf_force_values(const char* definition, char* id, int id_cap, double* values, int cap, int* nvalues) {
   if (!definition || !id || id_cap <= 0 || !nvalues) return fail(PMF_ERR_ARGUMENT, "pmf_force_values: null argument");
   pmf::ForceSpec fs;

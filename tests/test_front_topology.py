@@ -169,3 +169,29 @@ def test_errors_are_reported(tmp_path):
     psf, _ = paths("wwd")
     with pytest.raises(front.FrontError, match="could not find bond"):
         front.load_system(psf, paths("water216")[1])
+
+
+def test_dcd_and_xyz_writers_match_reference_writers(ref, tmp_path):
+    """Byte for byte against DCDTrajectoryWriter and XYZWriter compiled from
+    the reference, except the free-text remark lines (build date, user)."""
+    import ctypes as C
+    psf, par = paths("water216")
+    x = front.read_coords(os.path.join(G, "water_216/water.216.equil.pdb"))
+    rng = np.random.default_rng(2)
+    frames = np.stack([x + 0.01 * k * rng.standard_normal(x.shape) for k in range(3)])
+    mine, theirs = str(tmp_path / "a.dcd"), str(tmp_path / "b.dcd")
+    front.write_dcd(mine, frames)
+    f = np.ascontiguousarray(frames)
+    assert ref.lib.ref_write_dcd(theirs.encode(), x.shape[0], 3, f.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    a, b = open(mine, "rb").read(), open(theirs, "rb").read()
+    assert len(a) == len(b)
+    title = slice(100, 260)                  # the two 80-character remarks
+    assert a[:title.start] == b[:title.start] and a[title.stop:] == b[title.stop:]
+    t = front.load_system(psf, par)
+    xm, xt = str(tmp_path / "a.xyz"), str(tmp_path / "b.xyz")
+    front.write_xyz(xm, psf, par, frames[2], "c")
+    names = "\n".join(t.type_names[k] for k in t.type).encode()
+    f2 = np.ascontiguousarray(frames[2])
+    assert ref.lib.ref_write_xyz(xt.encode(), x.shape[0], f2.ctypes.data_as(C.POINTER(C.c_double)), names) == 0
+    la, lb = open(xm).read().splitlines(), open(xt).read().splitlines()
+    assert la[0] == lb[0] and la[2:] == lb[2:]
