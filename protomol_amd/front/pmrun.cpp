@@ -158,6 +158,9 @@ int main(int argc, char** argv) {
   const long lin = cfg.has("removelinearmomentum") ? cfg.integer("removelinearmomentum", -1) : cfg.integer("commotion", -1);
   const long ang = cfg.has("removeangularmomentum") ? cfg.integer("removeangularmomentum", -1)
                                                      : cfg.integer("angularmomentum", -1);
+  if (lin > 0 || ang > 0)
+    return die("removeLinearMomentum / removeAngularMomentum > 0 (removal every n steps) is not implemented; "
+               "0 removes at start-up, -1 never");
   if (lin >= 0 || ang >= 0) pmf::remove_momentum(sys, x, v, lin >= 0, ang >= 0);
 
   // ------------------------------------------------------------- boundary
