@@ -40,13 +40,18 @@ __device__ inline void add_fix(long long* f, int i, const double* v, double sg) 
 // straight to global memory.  Both are integer sums: the result does not
 // depend on the order.  Impropers are sparse (a protein has about one per six
 // atoms), so a block of them spans more atoms: their window is twice as wide.
+struct BondedShared {
+  double red[4][7];
+  int lo[4], hi[4];
+};
 template <int KIND>
-__device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBlocks bl, int bx) {
-  __shared__ double s_red[4][7];
+__device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBlocks bl, int bx, BondedShared& sh,
+                                         unsigned long long* s_f) {
   constexpr int kind = KIND;
-  constexpr int W = kind == 3 ? 2048 : 1024;   // atom window summed in LDS
-  __shared__ unsigned long long s_f[3 * W];
-  __shared__ int s_lo[4], s_hi[4];
+  constexpr int W = kind == 3 ? 2048 : 1024;   // atom window summed in LDS (s_f holds 3 W)
+  double (*s_red)[7] = sh.red;
+  int* s_lo = sh.lo;
+  int* s_hi = sh.hi;
   constexpr int NA = kind == 0 ? 2 : kind == 1 ? 3 : 4;
   bx += bl.off[kind];           // the block's energy record among all kinds
   const int t = (bx - bl.off[kind]) * blockDim.x + threadIdx.x;
@@ -141,16 +146,30 @@ __device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBloc
     ep[PM_E_VIR + 6] = s[3]; ep[PM_E_VIR + 7] = s[5]; ep[PM_E_VIR + 8] = s[6];
   }
 }
-template <int KIND>
+// Launch GROUP 0 runs the bond and angle blocks (one launch: their
+// registers are alike), GROUP 2 the dihedrals, GROUP 3 the impropers.
+template <int GROUP>
+__device__ inline void bonded_group_body(PmBox box, const PmBondedLaunch& a, const BondedBlocks& bl, int b) {
+  __shared__ BondedShared sh;
+  __shared__ unsigned long long s_f[3 * (GROUP == 3 ? 2048 : 1024)];
+  if constexpr (GROUP == 0) {
+    const int nb0 = bl.off[1] - bl.off[0];
+    if (b < nb0) bonded_terms_body<0>(box, a, bl, b, sh, s_f);
+    else bonded_terms_body<1>(box, a, bl, b - nb0, sh, s_f);
+  } else {
+    bonded_terms_body<GROUP>(box, a, bl, b, sh, s_f);
+  }
+}
+template <int GROUP>
 __global__ __launch_bounds__(256) void k_bonded_terms(PmBox box, PmBondedLaunch a, BondedBlocks bl) {
-  bonded_terms_body<KIND>(box, a, bl, blockIdx.x);
+  bonded_group_body<GROUP>(box, a, bl, blockIdx.x);
 }
 struct BondedTermArgs { PmBox box; PmBondedLaunch a; BondedBlocks bl; };
-template <int KIND>
+template <int GROUP>
 __global__ __launch_bounds__(256) void kb_bonded_terms(const BondedTermArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const BondedTermArgs& q = t[r];
-  bonded_terms_body<KIND>(q.box, q.a, q.bl, blockIdx.x - pre[r]);
+  bonded_group_body<GROUP>(q.box, q.a, q.bl, blockIdx.x - pre[r]);
 }
 
 bool any_bonded(const PmBondedLaunch& a) { return a.do_bond || a.do_angle || a.do_tors[0] || a.do_tors[1]; }
@@ -173,16 +192,17 @@ void pm_launch_bonded(hipStream_t s, const PmBox& box, const PmBondedLaunch& a) 
   // nothing to do when neither forces (a.n > 0) nor energies are wanted
   const bool work = a.n > 0 || a.want_energy;
   const BondedTermArgs q{box, a, bl};
-  for (int k = 0; k < 4; k++) {
-    const unsigned blocks = work ? (unsigned)nblk[k] : 0u;
+  const int groups[3] = {0, 2, 3};
+  const int gblocks[3] = {nblk[0] + nblk[1], nblk[2], nblk[3]};
+  for (int g = 0; g < 3; g++) {
+    const unsigned blocks = work ? (unsigned)gblocks[g] : 0u;
     if (pm_recording()) {
-      pm_record(PM_REC_BONDED_FORCE, k, blocks, 256, 0, &q, sizeof(q));
+      pm_record(PM_REC_BONDED_FORCE, groups[g], blocks, 256, 0, &q, sizeof(q));
       continue;
     }
     if (blocks == 0) continue;
-    if (k == 0) hipLaunchKernelGGL(k_bonded_terms<0>, dim3(blocks), dim3(256), 0, s, box, a, bl);
-    else if (k == 1) hipLaunchKernelGGL(k_bonded_terms<1>, dim3(blocks), dim3(256), 0, s, box, a, bl);
-    else if (k == 2) hipLaunchKernelGGL(k_bonded_terms<2>, dim3(blocks), dim3(256), 0, s, box, a, bl);
+    if (g == 0) hipLaunchKernelGGL(k_bonded_terms<0>, dim3(blocks), dim3(256), 0, s, box, a, bl);
+    else if (g == 1) hipLaunchKernelGGL(k_bonded_terms<2>, dim3(blocks), dim3(256), 0, s, box, a, bl);
     else hipLaunchKernelGGL(k_bonded_terms<3>, dim3(blocks), dim3(256), 0, s, box, a, bl);
   }
 }
@@ -193,7 +213,6 @@ bool pm_launch_batched_bonded(hipStream_t s, int kind, int variant, const void* 
   if (kind != PM_REC_BONDED_FORCE) return false;
   const BondedTermArgs* q = (const BondedTermArgs*)t;
   if (variant == 0) hipLaunchKernelGGL(kb_bonded_terms<0>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
-  else if (variant == 1) hipLaunchKernelGGL(kb_bonded_terms<1>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
   else if (variant == 2) hipLaunchKernelGGL(kb_bonded_terms<2>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
   else hipLaunchKernelGGL(kb_bonded_terms<3>, dim3(total), dim3(threads), 0, s, q, pre, nrep);
   return true;
