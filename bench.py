@@ -216,6 +216,25 @@ def tile_system(s, t, reps, spacing):
     return big
 
 
+def bonded_probe(evaluations=10):
+    """Only the bonded evaluations of the 8x8x8 tiled WW domain (for
+    rocprofv3 runs of the bonded kernels)."""
+    from protomol_amd import front, gpu
+    t = front.load_system(os.path.join(EXAMPLES, "wwd", "wwd_mineq.psf"),
+                          os.path.join(EXAMPLES, "wwd", "par_all27_prot_lipid.inp"))
+    x = front.read_coords(os.path.join(EXAMPLES, "wwd", "wwd.min.pdb"))
+    big = tile_system(t.system(x, box=np.full(3, 80.0), origin=x.mean(0)), t, 8, 60.0)
+    ctx = gpu.Context(0, "double")
+    ctx.set_topology(big)
+    ctx.set_boundary(big.box, big.origin, 12.0)
+    hs = [ctx.force(k, []) for k in ("Bond", "Angle", "Dihedral", "Improper")]
+    for _ in range(evaluations):
+        _, e = ctx.evaluate(hs, big.pos)
+    ctx.close()
+    return {"workload": "bonded probe: 512 copies of the WW domain (%d atoms), %d evaluations" % (big.n, evaluations),
+            "energies": [float(v) for v in e[2:6]]}
+
+
 def conf_bench():
     """The reference's own example files through the front end: the whole
     examples/water_216/water.216.lf.conf run by pmrun (FP64, 5000 steps of
@@ -490,6 +509,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the million-atom measurement")
     ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-water (SHAKE, 2 fs) measurement")
     ap.add_argument("--no-conf", action="store_true", help="skip the example .conf / protein measurements")
+    ap.add_argument("--bonded-probe", action="store_true",
+                    help="only the tiled-protein bonded evaluations (for profiling the bonded kernels)")
     ap.add_argument("--only-replicas", action="store_true",
                     help="run only the batched replica-exchange measurement and print its record")
     ap.add_argument("--autotune", action="store_true",
@@ -502,6 +523,9 @@ def main():
         return
     if args.only_replicas:
         print(json.dumps({"replica_exchange": replica_bench()}))
+        return
+    if args.bonded_probe:
+        print(json.dumps(bonded_probe()))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {args.gpus} is not run "
