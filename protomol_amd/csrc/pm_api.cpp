@@ -166,6 +166,7 @@ struct pm_context {
 
   // cells and slots
   int* d_bin_count = nullptr;        // per-cell atom counts of the binning in progress
+  int* d_scan_aux = nullptr;       // [2 per 8192 cells] block totals of the multi-block cell scan
   int *d_cell_of = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_slot = nullptr,
       *d_cell_rank = nullptr, *d_cell_atoms = nullptr, *d_cluster_cell = nullptr, *d_counts = nullptr;
   float* d_key = nullptr;
@@ -452,6 +453,7 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_cell_of, n))) return r;
   if ((r = dalloc(c, &c->d_key, n))) return r;
   if ((r = dalloc(c, &c->d_cell_count, nc))) return r;
+  if ((r = dalloc(c, &c->d_scan_aux, 2 * ((size_t)nc / 8192 + 1)))) return r;
   if ((r = dalloc(c, &c->d_bin_count, nc))) return r;
   HIPCHECK(c, hipMemset(c->d_bin_count, 0, (size_t)nc * sizeof(int)));
   if ((r = dalloc(c, &c->d_cell_start, nc))) return r;
@@ -995,7 +997,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_bin_count, c->d_cell_rank, c->d_key, rb);
   }
   pm_launch_scan_cells(s, nc, c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
-                       c->d_cluster_cell, c->ncl_max, c->m, rb);
+                       c->d_cluster_cell, c->ncl_max, c->m, rb, c->d_scan_aux);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, rb);
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
@@ -1480,6 +1482,7 @@ void free_all(pm_context* c) {
   dfree(c->d_lj_f); dfree(c->d_lj_d);
   dfree(c->d_pos); dfree(c->d_vel); dfree(c->d_force); dfree(c->d_energy); dfree(c->d_scratch3);
   dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_bin_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
+  dfree(c->d_scan_aux);
   dfree(c->d_xbuild); dfree(c->d_rebuild); c->list_stale = true;
   dfree(c->d_cell_rank); dfree(c->d_force_slot); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
@@ -2004,7 +2007,7 @@ int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm
   pm_launch_zero_i(s, c->d_bin_count, nc);
   pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_bin_count, c->d_cell_rank, c->d_key, nullptr);
   pm_launch_scan_cells(s, nc, c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
-                       c->d_cluster_cell, c->ncl_max, c->m, nullptr);
+                       c->d_cluster_cell, c->ncl_max, c->m, nullptr, c->d_scan_aux);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, nullptr);
   std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);
   HIPCHECK(c, hipMemcpyAsync(cell.data(), c->d_cell_of, n * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -139,8 +139,11 @@ struct PmNbLaunch {
 // bin_count (also on steps that keep the layout).
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
                    int* bin_count, int* rank, float* key, const int* rebuild);
+// aux: 2 ints per 8192 cells; with more than two such chunks (and not
+// recording) the scan runs over many blocks, otherwise in one
 void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_count, int* cell_start,
-                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild);
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild,
+                          int* aux);
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
                        int* cell_atoms, const int* rebuild);
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,

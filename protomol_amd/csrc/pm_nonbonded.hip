@@ -121,6 +121,107 @@ __global__ __launch_bounds__(1024) void kb_scan_cells(const ScanCellsArgs* __res
 }
 
 
+// Large cell grids: the same scan split over blocks of 8192 cells.  Pass 1
+// copies each block's counts to the layout, clears the binning counts and
+// writes the block's atom and slot totals; pass 2 adds the totals of the
+// blocks before it (a fixed-order sum) and writes the block's starts,
+// slots, cluster cells and, in the last block, the totals.  Integer sums:
+// the result is the single-block scan's.
+#define PM_SCAN_CHUNK 8192
+__global__ __launch_bounds__(1024) void k_scan_chunks(int ncells, int* __restrict__ cnt, int* __restrict__ cnt_out,
+                                                      int m_cl, int* __restrict__ aux, const int* __restrict__ rebuild) {
+  const int base = blockIdx.x * PM_SCAN_CHUNK;
+  const int m = min(PM_SCAN_CHUNK, ncells - base);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (rebuild && *rebuild == 0) {
+    for (int k = t; k < m; k += blockDim.x) cnt[base + k] = 0;
+    return;
+  }
+  __shared__ int s_a[16], s_s[16];
+  int sa = 0, ss = 0;
+  for (int k = t; k < m; k += blockDim.x) {
+    const int v = cnt[base + k];
+    cnt_out[base + k] = v;
+    cnt[base + k] = 0;
+    sa += v;
+    ss += (v + m_cl - 1) / m_cl * m_cl;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sa += __shfl_down(sa, off, 64);
+    ss += __shfl_down(ss, off, 64);
+  }
+  if (lane == 0) { s_a[w] = sa; s_s[w] = ss; }
+  __syncthreads();
+  if (t == 0) {
+    int ta = 0, ts = 0;
+    for (int u = 0; u < 16; u++) { ta += s_a[u]; ts += s_s[u]; }
+    aux[2 * blockIdx.x] = ta;
+    aux[2 * blockIdx.x + 1] = ts;
+  }
+}
+__global__ __launch_bounds__(1024) void k_scan_finish(int ncells, const int* __restrict__ cnt_out,
+                                                      int* __restrict__ start, int* __restrict__ slot,
+                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
+                                                      int ncl_max, int m_cl, const int* __restrict__ aux,
+                                                      const int* __restrict__ rebuild) {
+  if (rebuild && *rebuild == 0) return;
+  __shared__ int s_c[PM_SCAN_CHUNK];
+  __shared__ int s_wa[16], s_ws[16];
+  __shared__ int s_carry[2];
+  const int base = blockIdx.x * PM_SCAN_CHUNK;
+  const int m = min(PM_SCAN_CHUNK, ncells - base);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) {
+    int ca = 0, cs = 0;
+    for (int b = 0; b < (int)blockIdx.x; b++) { ca += aux[2 * b]; cs += aux[2 * b + 1]; }
+    s_carry[0] = ca;
+    s_carry[1] = cs;
+  }
+  for (int k = t; k < m; k += blockDim.x) s_c[k] = cnt_out[base + k];
+  __syncthreads();
+  int v[8];
+  int sa = 0, ss = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int idx = 8 * t + k;
+    v[k] = idx < m ? s_c[idx] : 0;
+    sa += v[k];
+    ss += (v[k] + m_cl - 1) / m_cl * m_cl;
+  }
+  int ia = sa, is = ss;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int ta = __shfl_up(ia, off, 64), ts = __shfl_up(is, off, 64);
+    if (lane >= off) { ia += ta; is += ts; }
+  }
+  if (lane == 63) { s_wa[w] = ia; s_ws[w] = is; }
+  __syncthreads();
+  int wa = 0, ws = 0, ta = 0, ts = 0;
+  for (int u = 0; u < 16; u++) {
+    if (u < w) { wa += s_wa[u]; ws += s_ws[u]; }
+    ta += s_wa[u]; ts += s_ws[u];
+  }
+  int ra = s_carry[0] + wa + ia - sa, rs = s_carry[1] + ws + is - ss;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int idx = 8 * t + k;
+    if (idx < m) {
+      const int c = base + idx;
+      const int ps = (v[k] + m_cl - 1) / m_cl * m_cl;
+      start[c] = ra;
+      slot[c] = rs;
+      for (int u = rs / m_cl; u < (rs + ps) / m_cl && u < ncl_max; u++) cluster_cell[u] = c;
+      ra += v[k];
+      rs += ps;
+    }
+  }
+  if (t == 0 && blockIdx.x == gridDim.x - 1) {
+    const int tot_a = s_carry[0] + ta, tot_s = s_carry[1] + ts;
+    counts[0] = tot_s; counts[1] = tot_s / m_cl; counts[2] = tot_a;
+  }
+}
+
 __device__ inline void scatter_body(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
                           const int* __restrict__ cell_start, int* __restrict__ cell_atoms,
                           const int* __restrict__ rebuild, int bx) {
@@ -1395,7 +1496,15 @@ void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, in
 }
 
 void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_count, int* cell_start,
-                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild) {
+                          int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild,
+                          int* aux) {
+  const int nchunks = (ncells + PM_SCAN_CHUNK - 1) / PM_SCAN_CHUNK;
+  if (!pm_recording() && aux && nchunks > 2) {
+    hipLaunchKernelGGL(k_scan_chunks, dim3(nchunks), dim3(1024), 0, s, ncells, bin_count, cell_count, m, aux, rebuild);
+    hipLaunchKernelGGL(k_scan_finish, dim3(nchunks), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
+                       counts, cluster_cell, ncl_max, m, aux, rebuild);
+    return;
+  }
   if (pm_recording()) {
     const ScanCellsArgs a{ncells, bin_count, cell_count, cell_start, cell_slot, counts, cluster_cell, ncl_max, m, rebuild};
     pm_record(PM_REC_SCAN_CELLS, 0, 1, 1024, 0, &a, sizeof(a));

@@ -39,6 +39,48 @@ def test_cell_binning_bitexact(gpu_ctx_factory, port):
     np.testing.assert_array_equal(g["inrc"], oc["inrc"])
 
 
+def test_cell_binning_bitexact_large_grid(gpu_ctx_factory, port):
+    """A grid of more than two 8192-cell chunks takes the multi-block cell
+    scan: cells, head/next lists and per-cell counts as the reference's."""
+    say("case: binning water8k cell 1.5 (27^3 cells) cutoff 4")
+    s = water.named("water8k", seed=2)
+    c = setup_ctx(gpu_ctx_factory, s, 1.5)
+    g = c.cell_diagnostics(s.pos, 4.0, 4.0)
+    o = port.cells(s, 1.5)
+    assert g["dims"] == o["dims"]
+    assert int(np.prod(g["dims"])) > 2 * 8192
+    np.testing.assert_array_equal(g["cell"], o["cell"])
+    np.testing.assert_array_equal(g["head"], o["head"])
+    np.testing.assert_array_equal(g["next"], o["next"])
+    lj, _ = c1_terms()
+    oc = port.cutoff(s, [po_term_lj4()], 4.0, 1.5, counts=True)
+    np.testing.assert_array_equal(g["cand"], oc["cand"])
+    np.testing.assert_array_equal(g["inrc"], oc["inrc"])
+
+
+def po_term_lj4():
+    from oracle import pyoracle as po
+    return po.term(po.POT_LJ, po.SWF_C2, 4.0, 3.0)
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 2e-6), ("double", 1e-11)])
+def test_forces_large_grid(gpu_ctx_factory, port, precision, tol):
+    """Forces with the multi-block cell scan: the 98,304-atom box with
+    3.6 A cells (27^3 = 19,683 cells, three scan blocks)."""
+    say(f"case: water98k LJ+Coulomb C2 cell 3.6 {precision}")
+    s = water.named("water98k", seed=3)
+    c = setup_ctx(gpu_ctx_factory, s, 3.6, precision)
+    hs = gpu_forces_c1(c)
+    f, e = c.evaluate(hs, s.pos)
+    lj, co = c1_terms()
+    o = port.cutoff(s, lj, 8.0, 3.6)
+    o2 = port.cutoff(s, co, 8.0, 3.6)
+    fr = o["forces"] + o2["forces"]
+    err = rms_rel(f, fr)
+    say("rms rel", err)
+    assert err < tol
+
+
 @pytest.mark.parametrize("cluster", [8, 16])
 @pytest.mark.parametrize("precision,tol", [("mixed", 2e-6), ("double", 1e-11)])
 def test_cutoff_c1(gpu_ctx_factory, port, precision, tol, cluster):
