@@ -231,3 +231,63 @@ def test_pmrun_replica_exchange_rates(tmp_path, gpu_ctx_factory):
     assert rates[0] == 0
     assert credit.sum() > 0                    # neighbours 3 K apart do exchange
     np.testing.assert_allclose(rates, credit / 6, rtol=1e-5)      # printed with 6 significant digits
+
+
+PME_CONF = """
+numsteps 10
+outputfreq 10
+allEnergiesFile pme.energies
+psfFile water.216.psf
+parFile water.216.par
+posFile water.216.equil.pdb
+velFile water.216.equil.vel.xyz
+boundaryConditions Periodic
+cellBasisVector1 30.8927 0 0
+cellBasisVector2 0 27.8485 0
+cellBasisVector3 0 0 30.7217
+cellorigin 0.98035 -0.08475 0.19285
+cellManager Cubic
+cellsize 5
+exclude scaled1-4
+Integrator {
+  level 0 Leapfrog {
+    timestep 0.5
+    force Bond, Angle
+    force LennardJones CoulombEwaldReal -switchingFunction C2 -switchingFunction Cutoff -algorithm NonbondedCutoff
+      -switchon 9 -cutoff 10 -alpha ALPHA -cutoff 10 -cutoff 10
+    force Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline
+      -gridsize 32 30 32 -cutoff 10 -order 4 -accuracy 1e-6
+  }
+}
+"""
+
+
+def test_pmrun_pme_conf_first_line_matches_oracle(tmp_path, port):
+    """The main workload's force set (LJ C2 + Ewald real in one cutoff force,
+    PME reciprocal + correction, bonds, angles) written as a .conf for the
+    216-water box: pmrun's energies at step 0 against the oracle."""
+    from oracle import pyoracle as po
+    tmp = str(tmp_path)
+    for f in FILES[1:]:
+        shutil.copy(os.path.join(G, f), tmp)
+    alpha = port.pme_alpha(10.0, 1e-6)
+    open(os.path.join(tmp, "pme.conf"), "w").write(PME_CONF.replace("ALPHA", repr(alpha)))
+    say("case: pmrun PME conf water216 grid 32x30x32")
+    r = subprocess.run([PMRUN, "pme.conf", "--precision", "double"], cwd=tmp, capture_output=True, text=True,
+                       timeout=120)
+    say(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0, r.stderr
+    got = [float(w) for w in open(os.path.join(tmp, "pme.energies")).readline().split()]
+    psf, par = (os.path.join(G, f) for f in FILES[1:3])
+    t = front.load_system(psf, par)
+    x = front.read_coords(os.path.join(G, FILES[3]))
+    s = t.system(x, box=(30.8927, 27.8485, 30.7217), origin=(0.98035, -0.08475, 0.19285))
+    terms = [po.term(po.POT_LJ, po.SWF_C2, 10.0, 9.0), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 10.0, alpha=alpha)]
+    a = port.cutoff(s, terms, 10.0, 5.0)["energies"]
+    b = port.pme(s, (32, 30, 32), 4, 10.0, 5.0, flags=6)["energies"]
+    c = port.bonded(s, 3)["energies"]
+    e = a + b + c
+    say("pmrun", got[1], got[9], got[10], "oracle", e[:7].sum(), e[1], e[0])
+    np.testing.assert_allclose(got[9], e[1], rtol=1.2e-7)          # E_VdW
+    np.testing.assert_allclose(got[10], e[0], rtol=1.2e-7)         # E_coulomb: real + reciprocal + correction
+    np.testing.assert_allclose(got[1], e[:7].sum(), rtol=1.2e-7)   # E_potential
