@@ -47,11 +47,18 @@ enum {
   PM_ERR_STATE = 2,        /* call out of order (no topology, no boundary, ...) */
   PM_ERR_DEVICE = 3,       /* HIP runtime failure */
   PM_ERR_CAPACITY = 4,     /* a device buffer bound was exceeded (message says which) */
-  PM_ERR_UNSUPPORTED = 5   /* force id not implemented on the GPU */
+  PM_ERR_UNSUPPORTED = 5,  /* force id not implemented on the GPU */
+  PM_ERR_NUMERIC = 6       /* a position was not finite, or a force term left the
+                              64-bit fixed-point range (+-2^23 kcal/mol/A): the
+                              forces and energies of the call are not valid */
 };
 
 enum { PM_PRECISION_MIXED = 0, PM_PRECISION_DOUBLE = 1 };
-enum { PM_EVAL_VIRIAL = 1 };
+/* PM_EVAL_VIRIAL: accumulate the atomic virial (ScalarStructure::virial()).
+ * PM_EVAL_MOLVIRIAL: the molecular virial (ScalarStructure::molecularVirial(),
+ * OneAtomPair.h:175-183) -- not computed on the GPU: pm_evaluate returns
+ * PM_ERR_UNSUPPORTED, so an adapter never hands back a run without it. */
+enum { PM_EVAL_VIRIAL = 1, PM_EVAL_MOLVIRIAL = 2 };
 enum { PM_INTEGRATOR_LEAPFROG = 0, PM_INTEGRATOR_LANGEVIN = 1 };
 enum { PM_ENERGY_COUNT = 16 };
 enum {
@@ -145,7 +152,9 @@ int pm_force_uncache(pm_context* ctx, int handle);
 /* SystemForce::evaluate for a group of forces sharing one cell-list build
  * (ForceGroup::evaluateSystemForces).  Host buffers; positions double[3n],
  * forces double[3n] += , energies double[PM_ENERGY_COUNT] += .
- * flags: PM_EVAL_VIRIAL to accumulate the atomic virial. */
+ * flags: PM_EVAL_VIRIAL to accumulate the atomic virial; PM_EVAL_MOLVIRIAL
+ * (the molecular virial) returns PM_ERR_UNSUPPORTED; PM_ERR_NUMERIC reports
+ * a non-finite position or a force outside the fixed-point range. */
 int pm_evaluate(pm_context* ctx, const int* handles, int nhandles, const double* positions,
                 double* forces, double* energies, int flags);
 

@@ -476,10 +476,6 @@ int ensure_cells(pm_context* c) {
   HIPCHECK(c, hipMemset(c->d_rebuild, 0, sizeof(int)));
   c->list_stale = true;
   HIPCHECK(c, hipMemset(c->d_counts, 0, 4 * sizeof(int)));
-  if (!c->d_overflow) {
-    if ((r = dalloc(c, &c->d_overflow, 1))) return r;
-    HIPCHECK(c, hipMemset(c->d_overflow, 0, sizeof(int)));
-  }
   if (!c->d_stats) {
     if ((r = dalloc(c, &c->d_stats, 3))) return r;
     HIPCHECK(c, hipMemset(c->d_stats, 0, 3 * sizeof(unsigned long long)));
@@ -1126,6 +1122,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       a.inv_pi_v = 1.0 / (kPi * (c->box.L[0] * c->box.L[1] * c->box.L[2]));
       a.fac = kPi * kPi / (f.alpha * f.alpha);
       a.tw = c->d_tw;
+      a.err = c->d_overflow;
       a.force_fix = c->d_force;
       a.epart = c->d_epart + off * PM_NE;
       a.want_virial = vir ? 1 : 0;
@@ -1315,6 +1312,12 @@ int check_overflow(pm_context* c) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (ov) {
     HIPCHECK(c, hipMemset(c->d_overflow, 0, sizeof(int)));
+    if (ov & PM_FLAG_NONFINITE)
+      return fail(c, PM_ERR_NUMERIC, "a position is not finite (or lies more than 2^31 box lengths away): the "
+                                     "forces of this evaluation are not valid");
+    if (ov & PM_FLAG_FORCE_RANGE)
+      return fail(c, PM_ERR_NUMERIC, "a force term is not finite or exceeds the fixed-point range of "
+                                     "+-2^23 kcal/mol/A (overlapping atoms?): the forces of this evaluation are not valid");
     return fail(c, PM_ERR_CAPACITY, "pair list capacity (%d entries per cluster) exceeded: atoms are packed far denser than the box average", c->lcap);
   }
   return PM_OK;
@@ -1486,7 +1489,7 @@ void free_all(pm_context* c) {
   dfree(c->d_xbuild); dfree(c->d_rebuild); c->list_stale = true;
   dfree(c->d_cell_rank); dfree(c->d_force_slot); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
-  dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_overflow); dfree(c->d_stats); dfree(c->d_epart); dfree(c->d_ered);
+  dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_stats); dfree(c->d_epart); dfree(c->d_ered);
   dfree(c->d_grid_fix);
   if (c->d_grid_r) hipFree(c->d_grid_r);
   if (c->d_grid_c) hipFree(c->d_grid_c);
@@ -1556,6 +1559,19 @@ int pm_create(int device, pm_context** out) {
     delete c;
     return PM_ERR_DEVICE;
   }
+  // the error flag word (PM_FLAG_*) of every kernel of this context: lives
+  // as long as the context, so the box's pointer to it never dangles
+  if (hipMalloc((void**)&c->d_overflow, sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_overflow, 0, sizeof(int)) != hipSuccess) {
+    if (c->d_overflow) hipFree(c->d_overflow);
+    hipEventDestroy(c->ev_fork);
+    hipEventDestroy(c->ev_join);
+    hipStreamDestroy(c->stream2);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return PM_ERR_DEVICE;
+  }
+  c->box.err = c->d_overflow;
   *out = c;
   return PM_OK;
 }
@@ -1570,6 +1586,7 @@ void pm_destroy(pm_context* ctx) {
   drop_graph(ctx);
   free_all(ctx);
   dfree(ctx->d_md);
+  dfree(ctx->d_overflow);
   hipEventDestroy(ctx->ev_fork);
   hipEventDestroy(ctx->ev_join);
   hipStreamDestroy(ctx->stream2);
@@ -1974,6 +1991,11 @@ int pm_evaluate(pm_context* c, const int* handles, int nh, const double* positio
   if ((r = set_device(c))) return r;
   if ((r = check_ready(c, handles, nh))) return r;
   if (!positions || !forces || !energies) return fail(c, PM_ERR_ARGUMENT, "pm_evaluate: null buffer");
+  if (flags & PM_EVAL_MOLVIRIAL)
+    return fail(c, PM_ERR_UNSUPPORTED, "pm_evaluate: the molecular virial (ScalarStructure::molecularVirial, "
+                                       "OneAtomPair.h:175-183) is not computed on the GPU");
+  if (flags & ~(PM_EVAL_VIRIAL | PM_EVAL_MOLVIRIAL))
+    return fail(c, PM_ERR_ARGUMENT, "pm_evaluate: unknown flags 0x%x", flags);
   const int n = c->n;
   HIPCHECK(c, hipMemcpyAsync(c->d_pos, positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   std::vector<int> hv(handles, handles + nh);
@@ -3143,6 +3165,9 @@ int pm_batch_run(pm_batch* b, int nsteps) {
     if (hipStreamWaitEvent(c->stream, b->ev_root, 0) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "stream wait failed");
     if ((r = md_unit(c, 1, true))) return bfail(b, r, "replica step failed", c);
   }
+  // list capacity, non-finite positions and force range of every replica
+  for (pm_context* c : b->ctx)
+    if ((r = check_overflow(c))) return bfail(b, r, "replica step failed", c);
   return PM_OK;
 }
 

@@ -25,10 +25,10 @@ namespace {
 // virial record.  a.n == 0 leaves the forces alone (the PME gather added
 // them per atom).
 struct BondedBlocks { int off[5]; };
-__device__ inline void add_fix(long long* f, int i, const double* v, double sg) {
+__device__ inline void add_fix(long long* f, int i, const double* v, double sg, int* err) {
   for (int c = 0; c < 3; c++)
     atomicAdd(reinterpret_cast<unsigned long long*>(f + 3 * (long long)i + c),
-              (unsigned long long)__double2ll_rn(sg * v[c] * PM_FORCE_FIX));
+              (unsigned long long)pm_force_fix(sg * v[c], err));
 }
 // KIND is a template parameter so that each kind's kernel holds only its
 // own registers (the torsion's FP64 trigonometry would otherwise set the
@@ -116,14 +116,14 @@ __device__ inline void bonded_terms_body(PmBox box, PmBondedLaunch a, BondedBloc
       if (have)
         for (int j = 0; j < NA; j++)
           for (int c = 0; c < 3; c++)
-            atomicAdd(&s_f[3 * (at[j] - lo) + c], (unsigned long long)__double2ll_rn(g[j][c] * PM_FORCE_FIX));
+            atomicAdd(&s_f[3 * (at[j] - lo) + c], (unsigned long long)pm_force_fix(g[j][c], box.err));
       __syncthreads();
       for (int k = threadIdx.x; k < 3 * span; k += blockDim.x) {
         const unsigned long long x = s_f[k];
         if (x) atomicAdd(reinterpret_cast<unsigned long long*>(a.force_fix + 3 * (long long)lo) + k, x);
       }
     } else if (have) {
-      for (int j = 0; j < NA; j++) add_fix(a.force_fix, at[j], g[j], 1.0);
+      for (int j = 0; j < NA; j++) add_fix(a.force_fix, at[j], g[j], 1.0, box.err);
     }
   }
   if (!a.want_energy) return;   // uniform over the launch: no thread reaches the barrier

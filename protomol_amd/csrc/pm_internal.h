@@ -49,6 +49,28 @@
 #define PM_E_VIR 7
 #define PM_NE 16
 
+// Error flags the kernels raise in the context's flag word (atomicOr); the
+// host reads and clears the word after each evaluation or MD run and turns a
+// raised flag into an error return.
+#define PM_FLAG_CAPACITY 1      // a pair list exceeded its capacity
+#define PM_FLAG_NONFINITE 2     // a position is not finite (or beyond 2^31 box lengths)
+#define PM_FLAG_FORCE_RANGE 4   // a force term outside the fixed-point range +-2^23 kcal/mol/A
+#define PM_FORCE_LIMIT 8388608.0
+
+__device__ inline void pm_raise(int* err, int bit) {
+  if (err) atomicOr(err, bit);
+}
+
+// 64-bit fixed point of one force component; a value beyond the range (or
+// not finite) raises PM_FLAG_FORCE_RANGE and contributes 0.
+__device__ inline long long pm_force_fix(double v, int* err) {
+  if (!(__builtin_fabs(v) < PM_FORCE_LIMIT)) {
+    pm_raise(err, PM_FLAG_FORCE_RANGE);
+    return 0;
+  }
+  return __double2ll_rn(v * PM_FORCE_FIX);
+}
+
 // Box description used by binning, the pair kernels and PME.
 struct PmBox {
   double L[3];        // edge lengths (e1.x, e2.y, e3.z)
@@ -64,6 +86,7 @@ struct PmBox {
   double fix_to_A[3]; // L / 2^32
   double cell0_fix[3];  // fixed point of the cell grid's lower corner (min), in [0, 2^32)
   double real_fix[3];   // real cell size in fixed-point units
+  int* err;             // the context's error flag word (PM_FLAG_*), device memory
 };
 
 // Switched pair potentials of one (possibly fused) nonbonded force.
@@ -206,6 +229,7 @@ struct PmPmeLaunch {
   long long* force_fix;
   double* epart;             // [pm_pme_xpass_blocks() * PM_NE]
   int want_virial;
+  int* err;                  // error flag word (PM_FLAG_FORCE_RANGE)
   // spread by cell bricks staged in LDS tiles
   const int* cell_slot;
   const int* cell_count;

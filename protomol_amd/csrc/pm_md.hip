@@ -18,9 +18,9 @@ __global__ void k_corr_force(PmBox box, int n, const double* __restrict__ pos, c
   if (i >= n) return;
   double f[3];
   corr_atom_force(box, i, pos, q, row, partner, cls, alpha, csf, f);
-  force_fix[3 * i] += __double2ll_rn(f[0] * PM_FORCE_FIX);
-  force_fix[3 * i + 1] += __double2ll_rn(f[1] * PM_FORCE_FIX);
-  force_fix[3 * i + 2] += __double2ll_rn(f[2] * PM_FORCE_FIX);
+  force_fix[3 * i] += pm_force_fix(f[0], box.err);
+  force_fix[3 * i + 1] += pm_force_fix(f[1], box.err);
+  force_fix[3 * i + 2] += pm_force_fix(f[2], box.err);
 }
 
 // Adjoint of the correction term (FP64), per atom over its exclusion partners.

@@ -241,19 +241,32 @@ __global__ void kb_scatter(const ScatterArgs* __restrict__ t, const int* __restr
 
 
 __device__ inline unsigned int to_fix(double x, double s) {
-  // periodic 32-bit fixed point: round(x * 2^32 / L) mod 2^32
-  long long v = __double2ll_rn(x * s);
+  // periodic 32-bit fixed point: round(x * 2^32 / L) mod 2^32; positions
+  // that have none (check_position flags them) map to 0
+  double t = x * s;
+  if (!(__builtin_fabs(t) < 9.0e18)) t = 0.0;
+  long long v = __double2ll_rn(t);
   return (unsigned int)v;
 }
 
 __device__ inline unsigned long long to_fix64(double x, double s) {
   // round(x * 2^64 / L) mod 2^64 from round(x * 2^32 / L) split in two
   double t = x * s;                       // units of 2^-32 box
+  if (!(__builtin_fabs(t) < 9.0e18)) t = 0.0;
   double hi = __builtin_floor(t);
   double lo = (t - hi) * 4294967296.0;    // fraction in units of 2^-64 box
   unsigned long long h = (unsigned long long)(long long)hi;
   unsigned long long l = (unsigned long long)__double2ll_rn(lo);
   return (h << 32) + l;
+}
+
+// A position whose periodic fixed point cannot be formed (not finite, or more
+// than 2^31 box lengths from the origin) raises PM_FLAG_NONFINITE: such an
+// atom would otherwise drop out of every pair silently.
+__device__ inline void check_position(const PmBox& box, const double* x) {
+  const bool ok = __builtin_fabs(x[0] * box.to_fix[0]) < 9.0e18 && __builtin_fabs(x[1] * box.to_fix[1]) < 9.0e18 &&
+                  __builtin_fabs(x[2] * box.to_fix[2]) < 9.0e18;
+  if (!ok) pm_raise(box.err, PM_FLAG_NONFINITE);
 }
 
 // One wave per cell: every atom's slot is its rank in (z, index) order within
@@ -279,6 +292,7 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
     for (int u = lane; u < k; u += 64) {
       const int sl = s0 + u;
       const int ai = atom_of_slot[sl];
+      check_position(box, pos + 3 * ai);
       int4 p;
       p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
       p.y = (int)to_fix(pos[3 * ai + 1], box.to_fix[1]);
@@ -321,6 +335,7 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
         rank += (kj < ki || (kj == ki && aj < ai)) ? 1 : 0;
       }
     }
+    check_position(box, pos + 3 * ai);
     int4 p;
     p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
     p.y = (int)to_fix(pos[3 * ai + 1], box.to_fix[1]);
@@ -621,7 +636,7 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   // candidates beyond PM_CAND_FACTOR * lcap mean a degenerate cell layout:
   // stop scanning and report it as a list overflow
   const int scan = min(total, PM_CAND_FACTOR * a.lcap);
-  if (scan < total && lane == 0) a.overflow[0] = 1;
+  if (scan < total && lane == 0) atomicOr(a.overflow, PM_FLAG_CAPACITY);
   int scur = 0;   // stencil entry holding candidate k0 (uniform)
   for (int k0 = 0; k0 < scan; k0 += 64) {
     const int k = k0 + lane;
@@ -690,7 +705,7 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
     count += __popcll(bal);
   }
   if (lane == 0) {
-    if (count > a.lcap) { a.overflow[0] = 1; count = a.lcap; }
+    if (count > a.lcap) { atomicOr(a.overflow, PM_FLAG_CAPACITY); count = a.lcap; }
     a.jcount[cl] = count;
   }
 }
@@ -1058,6 +1073,7 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     return comp == 0 ? jx : (comp == 1 ? jy : jz);
   };
 
+  bool range_bad = false;   // a converted force beyond the fixed-point range (or not finite)
   for (int base = 0; base < cnt; base += 2 * JW) {
     const int kA = base + js, kB = kA + JW;
     // row A; prefetch the j atom of row B and the entry of the next row A
@@ -1077,8 +1093,10 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     // lanes ii = 0, 1, 2 add the x, y, z component of row A, lanes 3, 4, 5 those of row B
     const T v = ii < 3 ? vA : vB;
     const int sl = ii < 3 ? sA : sB;
-    if (ii < 6 && v != T(0))
+    if (ii < 6 && v != T(0)) {
+      range_bad |= !(__builtin_fabs(v) < T(PM_FORCE_LIMIT));
       atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
+    }
   }
   // reduce the i force over the j slots (lanes ii, ii+8, ...); lanes
   // js = 0, 1, 2 then add its x, y, z component
@@ -1090,8 +1108,10 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
   }
   if (js < 3 && ireal) {
     const double iv = js == 0 ? fxa : (js == 1 ? fya : fza);
+    range_bad |= !(__builtin_fabs(iv) < PM_FORCE_LIMIT);
     atomicAdd((unsigned long long*)(ff + 3LL * si + js), (unsigned long long)__double2ll_rn(iv * PM_FORCE_FIX));
   }
+  if (__ballot(range_bad) != 0ull && lane == 0) pm_raise(box.err, PM_FLAG_FORCE_RANGE);
   if constexpr (WE) {
     // energies and virial of the cluster's pairs (each pair counted once)
     double r[8] = {ireal ? (double)ec : 0.0, ireal ? (double)elj : 0.0,

@@ -419,18 +419,18 @@ __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* 
     }
   }
   const double q = a.q[at];
-  long long F[3] = {__double2ll_rn((double)fx * a.hr[0] * q * PM_FORCE_FIX),
-                    __double2ll_rn((double)fy * a.hr[1] * q * PM_FORCE_FIX),
-                    __double2ll_rn((double)fz * a.hr[2] * q * PM_FORCE_FIX)};
+  long long F[3] = {pm_force_fix((double)fx * a.hr[0] * q, a.err),
+                    pm_force_fix((double)fy * a.hr[1] * q, a.err),
+                    pm_force_fix((double)fz * a.hr[2] * q, a.err)};
   if (TAIL) {
     double f[3];
     if (tl.corr) {
       corr_atom_force(tl.box, at, tl.pos, tl.q, tl.row, tl.partner, tl.cls, tl.alpha, tl.csf, f);
-      for (int k = 0; k < 3; k++) F[k] += __double2ll_rn(f[k] * PM_FORCE_FIX);
+      for (int k = 0; k < 3; k++) F[k] += pm_force_fix(f[k], a.err);
     }
     if (tl.bonded) {
       bonded_atom_force(tl.box, tl.b, at, f);
-      for (int k = 0; k < 3; k++) F[k] += __double2ll_rn(f[k] * PM_FORCE_FIX);
+      for (int k = 0; k < 3; k++) F[k] += pm_force_fix(f[k], a.err);
     }
   }
   a.force_fix[3 * at] += F[0];

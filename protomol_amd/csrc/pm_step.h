@@ -71,7 +71,9 @@ __device__ inline void pm_bin_atom(const PmBox& box, int i, const double* x, int
   double mz;
   const int c = cell_index(box, x[0], x[1], x[2], &mz);
   cell_of[i] = c;
-  key[i] = (float)mz;
+  // a NaN key (non-finite position) would break the strict (key, index)
+  // order that makes every cell's slot ranks a permutation
+  key[i] = mz == mz ? (float)mz : 0.0f;
   // groups of lanes with the same cell: no memory access in this loop
   const int lane = threadIdx.x & 63;
   unsigned long long todo = __ballot(1);

@@ -14,9 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpmgpu.so")
 
-PM_OK, PM_ERR_ARGUMENT, PM_ERR_STATE, PM_ERR_DEVICE, PM_ERR_CAPACITY, PM_ERR_UNSUPPORTED = range(6)
+PM_OK, PM_ERR_ARGUMENT, PM_ERR_STATE, PM_ERR_DEVICE, PM_ERR_CAPACITY, PM_ERR_UNSUPPORTED, PM_ERR_NUMERIC = range(7)
 PRECISION = {"mixed": 0, "double": 1}
 PM_EVAL_VIRIAL = 1
+PM_EVAL_MOLVIRIAL = 2
 INTEGRATOR = {"leapfrog": 0, "langevin": 1}
 ENERGY_COUNT = 16
 COULOMB, LENNARDJONES, BOND, ANGLE, DIHEDRAL, IMPROPER = 0, 1, 2, 3, 4, 5
@@ -201,13 +202,13 @@ class Context:
     def uncache(self, handle: int):
         self._check(self.lib.pm_force_uncache(self.h, handle))
 
-    def evaluate(self, handles, positions, virial=False, forces=None, energies=None):
+    def evaluate(self, handles, positions, virial=False, forces=None, energies=None, molecular_virial=False):
         hs = np.ascontiguousarray(handles, np.int32)
         x = np.ascontiguousarray(positions, np.float64).reshape(-1)
         f = np.zeros(3 * self.n) if forces is None else forces.reshape(-1)
         e = np.zeros(ENERGY_COUNT) if energies is None else energies
-        self._check(self.lib.pm_evaluate(self.h, _ip(hs), len(hs), _dp(x), _dp(f), _dp(e),
-                                         PM_EVAL_VIRIAL if virial else 0))
+        flags = (PM_EVAL_VIRIAL if virial else 0) | (PM_EVAL_MOLVIRIAL if molecular_virial else 0)
+        self._check(self.lib.pm_evaluate(self.h, _ip(hs), len(hs), _dp(x), _dp(f), _dp(e), flags))
         return f.reshape(-1, 3), e
 
     def cell_diagnostics(self, positions, algorithm_cutoff, pair_cutoff):

@@ -227,6 +227,21 @@ def perturbed_box(name: str = "water8k", seed: int = 0, dq: float = 0.01, dlj: f
     return s, sigma, eps
 
 
+def with_ion(s: WaterSystem, charge_e: float = 1.0, position=(0.0, 0.0, 0.0), mass: float = 22.99) -> WaterSystem:
+    """A copy of `s` with one more atom: an ion of `charge_e` elementary
+    charges (scaled as every charge) at `position`, in a molecule of its own,
+    with no exclusion pairs, no bonded terms and the type of the water H (no
+    Lennard-Jones).  The box then carries a net charge."""
+    t = s.copy()
+    t.pos = np.vstack([t.pos, np.asarray(position, float)[None, :]])
+    t.vel = np.vstack([t.vel, np.zeros((1, 3))])
+    t.q = np.append(t.q, charge_e * SQRT_COULOMB)
+    t.type = np.append(t.type, np.int32(1)).astype(np.int32)
+    t.mass = np.append(t.mass, mass)
+    t.molecule = np.append(t.molecule, np.int32(int(t.molecule.max()) + 1)).astype(np.int32)
+    return t
+
+
 def per_atom_lj_tables(sigma, eps):
     """Type tables with one type per atom that reproduce per-atom CHARMM
     combination (for checkers on small systems)."""

@@ -114,10 +114,27 @@ VARIANTS = [
     ("LennardJones Coulomb -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction C1",
      [7.0, 8.0, 8.0, 8.0],
      [po.term(po.POT_LJ, po.SWF_C2, 8.0, 7.0), po.term(po.POT_COULOMB, po.SWF_C1, 8.0)]),
+    # the remaining registered OneAtomPairTwo combinations
+    # (factories/registerForceExemplarsCutoff.cpp:76, 91, 131)
+    ("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C1 -switchingFunction Cutoff",
+     [8.0, ALPHA_RC8, 8.0, 8.0],
+     [po.term(po.POT_LJ, po.SWF_C1, 8.0), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 8.0, alpha=ALPHA_RC8)]),
+    ("LennardJones Coulomb -algorithm NonbondedCutoff -switchingFunction C2 -switchingFunction Cutoff",
+     [7.0, 8.0, 8.0, 8.0],
+     [po.term(po.POT_LJ, po.SWF_C2, 8.0, 7.0), po.term(po.POT_COULOMB, po.SWF_CUTOFF, 8.0)]),
+    ("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction Cutoff -switchingFunction Cutoff",
+     [8.0, ALPHA_RC8, 8.0, 8.0],
+     [po.term(po.POT_LJ, po.SWF_CUTOFF, 8.0), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, 8.0, alpha=ALPHA_RC8)]),
 ]
 
 
-@pytest.mark.parametrize("fid,vals,terms", VARIANTS, ids=[v[0].split(" -")[0] + "/" + v[0].split()[-1] for v in VARIANTS])
+def _variant_id(fid):
+    w = fid.split()
+    sw = [w[i + 1] for i in range(len(w) - 1) if w[i] == "-switchingFunction"]
+    return fid.split(" -")[0].replace(" ", "+") + "/" + "/".join(sw)
+
+
+@pytest.mark.parametrize("fid,vals,terms", VARIANTS, ids=[_variant_id(v[0]) for v in VARIANTS])
 @pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-11)])
 def test_cutoff_variants(gpu_ctx_factory, port, fid, vals, terms, precision, tol):
     say(f"case: variant {fid} {precision}")
@@ -139,7 +156,7 @@ def test_exclusion_rules(gpu_ctx_factory, port, precision, tol, cluster):
     s = exclusion_mix(water.named("water648", seed=8, jitter=0.4))
     c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
     c.set_option("cluster_size", cluster)
-    for fid, vals, terms in VARIANTS[5:]:
+    for fid, vals, terms in VARIANTS[5:7]:
         f, e = c.evaluate([c.force(fid, vals)], s.pos, virial=True)
         o = port.cutoff(s, terms, vals[-1], 4.0, virial=True)
         say(fid, "rms rel force", rms_rel(f, o["forces"]))

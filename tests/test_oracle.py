@@ -142,6 +142,37 @@ def test_reference_pme(port, ref):
             np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-12, atol=1e-9)
 
 
+def test_reference_pme_charged_system(port, ref):
+    """Net-charged box (one Na+ added to 216 waters): the charged-system energy
+    -pi/(2 V alpha^2) (sum q)^2 and its virial diagonal
+    (NonbondedPMEwaldSystemForce.h:300-305, 618-628) against the compiled
+    reference, and the term itself in closed form: the ion has no exclusion
+    pairs, so the correction-only energy of the charged box exceeds the
+    neutral box's by the ion's self energy -alpha/sqrt(pi) q^2 plus the term,
+    and its virial diagonal by the term alone."""
+    base = water.named("water648", seed=25, jitter=0.3)
+    s = water.with_ion(base, 1.0, (0.3, -0.2, 0.1))
+    for flags in (4, 6, 7):
+        a = port.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=flags, virial=True)
+        b = ref.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=flags, virial=True)
+        np.testing.assert_allclose(a["forces"], b["forces"], rtol=0, atol=1e-11 * np.abs(b["forces"]).max())
+        np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-12, atol=1e-9)
+    n = port.pme(base, (24, 24, 24), 4, 8.0, 4.0, flags=4, virial=True)
+    c = port.pme(s, (24, 24, 24), 4, 8.0, 4.0, flags=4, virial=True)
+    alpha = port.pme_alpha(8.0, 1e-6)
+    vol = float(np.prod(s.box))
+    qsum = float(s.q.sum())
+    qion = float(s.q[-1])
+    term = -math.pi / (2.0 * vol * alpha * alpha) * qsum * qsum
+    assert abs(float(base.q.sum()) * 0.00268283) < 1e-5 < abs(qsum * 0.00268283)
+    np.testing.assert_allclose(c["energies"][0] - n["energies"][0], -alpha / math.sqrt(math.pi) * qion * qion + term,
+                               rtol=1e-10)
+    for k in (7, 11, 15):
+        np.testing.assert_allclose(c["energies"][k] - n["energies"][k], term, rtol=1e-9)
+    for k in (8, 9, 10, 12, 13, 14):
+        assert c["energies"][k] == n["energies"][k]
+
+
 def test_reference_bspline(port, ref):
     for order in (2, 4, 6, 8, 10):
         for w in (0.0, 0.13, 0.5, 0.999):
@@ -357,7 +388,102 @@ def test_leapfrog_constant_force(port):
     np.testing.assert_allclose(v[0], np.array([0.01, 0, 0]) + f[0] / m[0] * t, rtol=1e-12)
 
 
-def test_rem_delta_closed_form(port):
+def test_rem_delta_boltzmann_ratio(port):
+    """exp(-delta) of a swap is the ratio of the two replicas' joint Boltzmann
+    weights after and before the configurations trade temperatures,
+    exp(-E_j/kT_i - E_i/kT_j) / exp(-E_i/kT_i - E_j/kT_j), with ProtoMol's
+    Boltzmann constant (base/pmconstants.cpp:74); rem.cpp:941-946 accepts
+    when delta <= 0 or the uniform is <= exp(-delta).  A sign slip (E_i - E_j)
+    or a swapped temperature pair inverts the ratio and fails here."""
     kb = 0.001987191
-    for ti, tj, ei, ej in ((300.0, 310.0, -100.0, -95.0), (450.0, 300.0, 5.0, -7.5)):
-        assert port.rem_delta(ti, tj, ei, ej) == pytest.approx((1 / (kb * ti) - 1 / (kb * tj)) * (ej - ei), rel=1e-14)
+    cases = ((300.0, 310.0, -100.0, -95.0), (450.0, 300.0, 5.0, -7.5), (300.0, 301.5, -2.5e4, -2.49e4),
+             (350.0, 300.0, -12.0, -12.0))
+    for ti, tj, ei, ej in cases:
+        before = math.exp(-ei / (kb * ti)) * math.exp(-ej / (kb * tj)) if abs(ei) < 1e3 else None
+        d = port.rem_delta(ti, tj, ei, ej)
+        if before is not None:
+            after = math.exp(-ej / (kb * ti)) * math.exp(-ei / (kb * tj))
+            assert math.exp(-d) == pytest.approx(after / before, rel=1e-12)
+        else:
+            # large energies: the same ratio taken in logarithms
+            log_ratio = (-ej / (kb * ti) - ei / (kb * tj)) - (-ei / (kb * ti) - ej / (kb * tj))
+            assert -d == pytest.approx(log_ratio, rel=1e-9)
+    # a colder replica holding the higher energy always swaps (delta < 0)
+    assert port.rem_delta(300.0, 330.0, -90.0, -100.0) < 0.0
+    assert port.rem_delta(300.0, 330.0, -100.0, -90.0) > 0.0
+    assert port.rem_delta(300.0, 330.0, -50.0, -50.0) == 0.0
+
+
+def _langevin_kick(port, n, dt, temp, gamma, mass, f, xi, v):
+    import ctypes as C
+    D = C.POINTER(C.c_double)
+    port.lib.orc_langevin_half_kick(n, C.c_double(dt), C.c_double(temp), C.c_double(gamma), mass.ctypes.data_as(D),
+                                    f.ctypes.data_as(D), xi.ctypes.data_as(D), v.ctypes.data_as(D))
+
+
+def test_langevin_half_kick_deterministic_part(port):
+    """Without noise (T = 0) a half kick integrates dv/dt = F/m - gamma v over
+    half a step exactly: v = v0 e^(-gamma t) + F/(m gamma) (1 - e^(-gamma t)),
+    t = dt/2 in AKMA time (dt_fs * INV_TIMEFACTOR) and gamma converted from
+    1/ps to 1/AKMA-time (LangevinLeapfrogIntegrator.cpp:36, :58-79).  The
+    solution is checked against a fine RK4 integration of the ODE, and the
+    gamma -> 0 limit against the plain half kick."""
+    n = 4
+    rng = np.random.default_rng(3)
+    mass = np.array([1.008, 15.9994, 12.011, 35.45])
+    f = rng.normal(0.0, 20.0, (n, 3))
+    v0 = rng.normal(0.0, 0.02, (n, 3))
+    xi = np.zeros((n, 3))
+    for dt, gamma in ((1.0, 5.0), (2.0, 91.0), (0.5, 1000.0)):
+        v = v0.copy()
+        _langevin_kick(port, n, dt, 0.0, gamma, mass, f, xi, v)
+        t = 0.5 * dt * water.INV_TIMEFACTOR
+        g = gamma / 1000.0 / water.INV_TIMEFACTOR      # 1/ps -> 1/fs -> 1/AKMA time
+        a = f / mass[:, None]
+        # RK4 of dv/dt = a - g v with 2000 steps
+        w = v0.copy()
+        hstep = t / 2000
+        for _ in range(2000):
+            k1 = a - g * w
+            k2 = a - g * (w + 0.5 * hstep * k1)
+            k3 = a - g * (w + 0.5 * hstep * k2)
+            k4 = a - g * (w + hstep * k3)
+            w = w + hstep / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        np.testing.assert_allclose(v, w, rtol=1e-9, atol=1e-12)
+    # gamma -> 0: the Langevin half kick becomes the leapfrog half kick
+    import ctypes as C
+    D = C.POINTER(C.c_double)
+    v = v0.copy()
+    # (gamma t = 5e-7: the friction's relative effect is gamma t / 2 and the
+    # rounding of 1 - e^(-gamma t) about 1e-16 / (gamma t))
+    _langevin_kick(port, n, 1.0, 0.0, 1e-3, mass, f, xi, v)
+    w = v0.copy()
+    port.lib.orc_half_kick(n, C.c_double(1.0), mass.ctypes.data_as(D), f.ctypes.data_as(D), w.ctypes.data_as(D))
+    np.testing.assert_allclose(v, w, rtol=1e-6)
+
+
+def test_langevin_half_kick_noise_is_ornstein_uhlenbeck(port):
+    """With F = 0 the velocity is an Ornstein-Uhlenbeck process, whose exact
+    transition over t = dt/2 is v1 = v0 e^(-gamma t) + N(0, kT/m (1 - e^(-2 gamma t))):
+    from v0 ~ N(0, kT/m) the regression slope of v1 on v0 is e^(-gamma t), the
+    residual variance kT/m (1 - e^(-2 gamma t)) and the variance of v1 stays
+    kT/m (the Maxwell distribution is stationary).  A wrong friction or noise
+    amplitude (fdt, vdt, ndt, the 2 k T gamma constant) moves one of the three."""
+    n = 200_000
+    rng = np.random.default_rng(11)
+    temp, gamma, dt = 310.0, 50.0, 2.0
+    mass = np.full(n, 15.9994)
+    kt_m = water.BOLTZMANN * temp / mass[0]
+    v0 = rng.standard_normal((n, 3)) * math.sqrt(kt_m)
+    xi = rng.standard_normal((n, 3))
+    v = v0.copy()
+    _langevin_kick(port, n, dt, temp, gamma, mass, np.zeros((n, 3)), xi, v)
+    t = 0.5 * dt * water.INV_TIMEFACTOR
+    g = gamma / 1000.0 / water.INV_TIMEFACTOR
+    x, y = v0.reshape(-1), v.reshape(-1)
+    slope = (x * y).sum() / (x * x).sum()
+    resid = y - slope * x
+    m = x.size
+    assert slope == pytest.approx(math.exp(-g * t), abs=4.0 / math.sqrt(m))
+    assert resid.var() == pytest.approx(kt_m * (1 - math.exp(-2 * g * t)), rel=6.0 * math.sqrt(2.0 / m))
+    assert y.var() == pytest.approx(kt_m, rel=6.0 * math.sqrt(2.0 / m))
