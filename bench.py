@@ -111,37 +111,71 @@ def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=
                 skin=skin_used * 1e-3, skin_milli=skin_used, cluster=cluster_used, tuned=tuned)
 
 
-def c3_bench(steps=200, warmup=20):
+def c3_bench(steps=200, warmup=20, accuracy=True):
     """The million-atom workload: 333,333 waters (999,999 atoms) in a 215.4 A
     cube, the force set of the main workload with PME 240^3 (order 4, rc 10),
     leapfrog 1 fs, mixed precision, library defaults; ns/day over `steps`
-    steps after `warmup`."""
+    steps after `warmup`, then a pass with HIP events around the pair and PME
+    kernels (pair interactions/s against the FP32 VALU peak, PME compulsory
+    bytes against HBM) and the force error against the FP64 oracle."""
     from protomol_amd import water
     s = water.named("water1m", seed=1)
     from protomol_amd import gpu
-    ctx = gpu.Context(0, "mixed")
-    ctx.set_topology(s)
-    ctx.set_boundary(s.box, s.origin, CELL)
-    alpha = pme_alpha(RC, 1e-6)
-    hs = [ctx.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff "
-                    "-switchingFunction C2 -switchingFunction Cutoff", [SWITCHON, RC, alpha, RC, RC]),
-          ctx.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
-                    [240, 240, 240, RC, ORDER, 1e-6, -1.0]),
-          ctx.force("Bond", []), ctx.force("Angle", [])]
-    ctx.md_setup(hs, "leapfrog", dt=DT_FS)
-    ctx.md_set_state(s.pos, s.vel)
-    ctx.md_run(warmup)
-    ctx.synchronize()
+    grid = (240, 240, 240)
+
+    def setup(timing=()):
+        ctx = gpu.Context(0, "mixed")
+        ctx.set_topology(s)
+        ctx.set_boundary(s.box, s.origin, CELL)
+        alpha = pme_alpha(RC, 1e-6)
+        hs = [ctx.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff "
+                        "-switchingFunction C2 -switchingFunction Cutoff", [SWITCHON, RC, alpha, RC, RC]),
+              ctx.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                        [grid[0], grid[1], grid[2], RC, ORDER, 1e-6, -1.0]),
+              ctx.force("Bond", []), ctx.force("Angle", [])]
+        ctx.md_setup(hs, "leapfrog", dt=DT_FS)
+        ctx.md_set_state(s.pos, s.vel)
+        ctx.md_run(warmup)
+        ctx.synchronize()
+        ctx.set_timing(timing)
+        return ctx
+    ctx = setup()
     t0 = time.perf_counter()
     ctx.md_run(steps)
     ctx.synchronize()
     dt = time.perf_counter() - t0
     builds = ctx.stats()["list_builds"]
     ctx.close()
-    return {"workload": "water1m: 999,999-atom TIP3 water, 215.4 A cube, LJ-C2(9-10 A) + Ewald-real rc 10 A, "
-                        "PME 240^3 order 4, bonds+angles, leapfrog dt 1 fs", "atoms": int(s.n), "steps": steps,
-            "ms_per_step": dt * 1e3 / steps, "ns_per_day": steps / dt * DT_FS * 86400 * 1e-6,
-            "atom_steps_per_s": s.n * steps / dt, "list_builds": builds}
+    out = {"workload": "water1m: 999,999-atom TIP3 water, 215.4 A cube, LJ-C2(9-10 A) + Ewald-real rc 10 A, "
+                       "PME 240^3 order 4, bonds+angles, leapfrog dt 1 fs", "atoms": int(s.n), "steps": steps,
+           "ms_per_step": dt * 1e3 / steps, "ns_per_day": steps / dt * DT_FS * 86400 * 1e-6,
+           "atom_steps_per_s": s.n * steps / dt, "list_builds": builds}
+    # kernel-event pass (events launch kernels directly)
+    ev_steps = max(20, steps // 4)
+    ctx = setup(("pair", "pme"))
+    ctx.md_run(ev_steps)
+    ctx.synchronize()
+    tim = ctx.timing()
+    x, _, _ = ctx.md_get_state()
+    pairs = int(ctx.cell_diagnostics(x, RC, RC)["inrc"].sum())
+    ctx.close()
+    pair_ms, pair_n = tim["pair"]
+    pme_ms, pme_n = tim["pme"]
+    pair_s = pair_ms * 1e-3 / max(pair_n, 1)
+    pme_s = pme_ms * 1e-3 / max(pme_n, 1)
+    ng = grid[0] * grid[1] * grid[2]
+    ngc = grid[0] * grid[1] * (grid[2] // 2 + 1)
+    n = s.n
+    pme_bytes = (16 * n + 4 * ng) + (4 * ng + 8 * ngc) + (8 * ngc + 4 * ng) + (16 * n + 4 * ng + 24 * n)
+    out.update({"pairs_per_step": pairs, "pair_us_per_step": pair_s * 1e6,
+                "pair_interactions_per_s": pairs / pair_s,
+                "pair_tflops": pairs * FLOPS_PER_PAIR / pair_s / 1e12,
+                "valu_frac": pairs * FLOPS_PER_PAIR / pair_s / PEAK_FP32,
+                "pme_us_per_step": pme_s * 1e6, "pme_grid_points_per_s": ng / pme_s,
+                "pme_compulsory_bytes": pme_bytes, "pme_hbm_frac": (pme_bytes / pme_s) / PEAK_HBM})
+    if accuracy:
+        out["accuracy"] = accuracy_vs_oracle(s, x, pme_alpha(RC, 1e-6), grid, CELL)
+    return out
 
 
 def rigid_bench(steps=300, warmup=30, dt_fs=2.0, eps=1e-5, max_iter=30):
@@ -349,6 +383,182 @@ def conf_bench():
     return out
 
 
+def force_errors(f, fr, mass, e, er):
+    """Accuracy of forces f against the FP64 oracle's fr: the RMS relative
+    force error sqrt(sum |F - F_ref|^2 / sum |F_ref|^2), and the statistics
+    of the reference's CompareForce (forces/CompareForce.cpp:95-119): mass-
+    weighted average and maximum relative errors and the absolute maximum;
+    with the relative error of the potential energy (sum of the terms)."""
+    f = np.asarray(f).reshape(-1, 3)
+    fr = np.asarray(fr).reshape(-1, 3)
+    d2 = ((f - fr) ** 2).sum(1)
+    rm = 1.0 / np.asarray(mass)
+    errf = np.sqrt(d2 * rm)
+    fmag = np.sqrt((f ** 2).sum(1) * rm)
+    tot = fmag.sum()
+    return {"rms_rel_force": float(np.sqrt(d2.sum() / (fr ** 2).sum())),
+            "rel_energy": float(abs(e - er) / abs(er)),
+            "compareforce_rel_avg": float(errf.sum() / tot),
+            "compareforce_rel_max": float(len(f) * errf.max() / tot),
+            "compareforce_abs_max": float(np.sqrt(d2.max()))}
+
+
+def oracle_threads():
+    """Host threads of this run: the box's CPU share (OMP_NUM_THREADS where it
+    is set), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def accuracy_vs_oracle(s, x, alpha, grid, cell, rc=RC, switchon=SWITCHON):
+    """Forces and potential energy of the main force set (LJ-C2 + Ewald-real,
+    PME reciprocal + correction, bonds + angles) at positions x through
+    pm_evaluate in mixed and FP64 precision, against the FP64 oracle (the C
+    restatement pinned to the compiled reference), outside any timed region."""
+    from oracle import pyoracle as po
+    from protomol_amd import gpu
+    s2 = s.copy()
+    s2.pos = np.ascontiguousarray(x)
+    o = po.Oracle("port")
+    terms = [po.term(po.POT_LJ, po.SWF_C2, rc, switchon), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, rc, alpha=alpha)]
+    a = o.cutoff_omp(s2, terms, rc, cell, threads=oracle_threads())
+    b = o.pme(s2, grid, ORDER, rc, cell, flags=6)
+    c = o.bonded(s2, 3)
+    fr = a["forces"] + b["forces"] + c["forces"]
+    er = float((a["energies"] + b["energies"] + c["energies"])[:4].sum())
+    out = {}
+    for precision in ("mixed", "double"):
+        ctx = gpu.Context(0, precision)
+        ctx.set_topology(s2)
+        ctx.set_boundary(s2.box, s2.origin, cell)
+        hs = [ctx.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff "
+                        "-switchingFunction C2 -switchingFunction Cutoff", [switchon, rc, alpha, rc, rc]),
+              ctx.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                        [grid[0], grid[1], grid[2], rc, ORDER, 1e-6, -1.0]),
+              ctx.force("Bond", []), ctx.force("Angle", [])]
+        f, e = ctx.evaluate(hs, s2.pos)
+        ctx.close()
+        out["fp64" if precision == "double" else "mixed"] = force_errors(f, fr, s2.mass, float(e[:4].sum()), er)
+    out["oracle"] = "FP64 C restatement (bit-exact with the compiled reference on the cutoff path, 1e-11 on PME)"
+    return out
+
+
+def c1_bench(steps=100, warmup=10):
+    """The plain-method workload: 216 waters (648 atoms) in an 18.62 A cube,
+    LennardJones and Coulomb with C2 switching 7 -> 8 A (two NonbondedCutoff
+    forces sharing one cell build), cell size 4, bonds + angles, leapfrog
+    1 fs, FP64, 100 steps; the reference build's time for the same force
+    evaluation on one host core beside it, and the force error of both
+    precisions against the FP64 oracle."""
+    from protomol_amd import gpu, water
+    from oracle import pyoracle as po
+    s = water.named("water648", seed=1, jitter=0.3)
+
+    def forces(ctx):
+        return [ctx.force("LennardJones -algorithm NonbondedCutoff -switchingFunction C2", [7.0, 8.0, 8.0]),
+                ctx.force("Coulomb -algorithm NonbondedCutoff -switchingFunction C2", [7.0, 8.0, 8.0]),
+                ctx.force("Bond", []), ctx.force("Angle", [])]
+    out = {"workload": "water648: 648-atom TIP3 water, 18.62 A cube, LJ + Coulomb C2 7-8 A (cell 4), bonds + "
+                       "angles, leapfrog dt 1 fs", "atoms": int(s.n), "steps": steps}
+    for precision in ("double", "mixed"):
+        ctx = gpu.Context(0, precision)
+        ctx.set_topology(s)
+        ctx.set_boundary(s.box, s.origin, 4.0)
+        hs = forces(ctx)
+        ctx.md_setup(hs, "leapfrog", dt=DT_FS)
+        ctx.md_set_state(s.pos, s.vel)
+        ctx.md_run(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.md_run(steps)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        f, e = ctx.evaluate(hs[:2], s.pos)
+        pairs = int(ctx.cell_diagnostics(s.pos, 8.0, 8.0)["inrc"].sum())
+        ctx.close()
+        key = "fp64" if precision == "double" else "mixed"
+        out[key] = {"ms_per_step": dt * 1e3 / steps, "ns_per_day": steps / dt * DT_FS * 86400 * 1e-6,
+                    "steps_per_s": steps / dt, "pairs_per_step": pairs}
+        o = po.Oracle("port")
+        a = o.cutoff(s, [po.term(po.POT_LJ, po.SWF_C2, 8.0, 7.0)], 8.0, 4.0)
+        b = o.cutoff(s, [po.term(po.POT_COULOMB, po.SWF_C2, 8.0, 7.0)], 8.0, 4.0)
+        out[key]["accuracy"] = force_errors(f, a["forces"] + b["forces"], s.mass, float(e[:2].sum()),
+                                            float((a["energies"] + b["energies"])[:2].sum()))
+    try:
+        kind = "reference" if po.available("reference") else "port"
+        o = po.Oracle(kind)
+        t0 = time.perf_counter()
+        reps = 20
+        for _ in range(reps):
+            o.cutoff(s, [po.term(po.POT_LJ, po.SWF_C2, 8.0, 7.0)], 8.0, 4.0)
+            o.cutoff(s, [po.term(po.POT_COULOMB, po.SWF_C2, 8.0, 7.0)], 8.0, 4.0)
+            o.bonded(s, 3)
+        cpu = (time.perf_counter() - t0) / reps
+        out["cpu_one_core"] = {"kind": kind, "ms_per_step": cpu * 1e3,
+                               "ns_per_day": DT_FS * 86400 * 1e-6 / cpu}
+    except Exception as ex:  # informational only
+        out["cpu_error"] = str(ex)
+    return out
+
+
+def c5_bench(reps=20):
+    """The differentiation workload: 2,700 waters (8,100 atoms) in a 43.2 A
+    cube with seeded per-atom charge and LJ noise, LJ-C2 (8 -> 9 A, per-atom
+    sigma/epsilon) + Ewald-real rc 9 and PME 48^3 order 4: Hessian-vector
+    products per second (adjoint kernels, every term), the parameter
+    gradients dE/dq, dE/dsigma, dE/deps, and 30 mass-weighted Lanczos steps
+    with the 20 lowest Ritz vectors, in FP64 and mixed precision."""
+    from protomol_amd import gpu, water
+    s, sigma, eps = water.perturbed_box("water8k", seed=1)
+    alpha = pme_alpha(9.0, 1e-6)
+    out = {"workload": "water8k perturbed: 8,100 atoms, 43.2 A cube, LJ-C2 8-9 A (per-atom sigma/eps) + "
+                       "Ewald-real rc 9 + PME 48^3 order 4", "atoms": int(s.n)}
+    v = np.random.default_rng(5).standard_normal((s.n, 3))
+    v /= np.linalg.norm(v)
+    for precision in ("double", "mixed"):
+        ctx = gpu.Context(0, precision)
+        ctx.set_topology(s)
+        ctx.set_boundary(s.box, s.origin, 4.5)
+        ctx.set_lj_per_atom(sigma, eps)
+        hs = [ctx.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C2 "
+                        "-switchingFunction Cutoff", [8.0, 9.0, alpha, 9.0, 9.0]),
+              ctx.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                        [48, 48, 48, 9.0, 4, 1e-6, -1.0])]
+        hv = ctx.hessian_vec(hs, s.pos, v)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hv = ctx.hessian_vec(hs, s.pos, v)
+        t_hvp = (time.perf_counter() - t0) / reps
+        ctx.param_gradients(hs, s.pos)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.param_gradients(hs, s.pos)
+        t_grad = (time.perf_counter() - t0) / reps
+        v0 = np.random.default_rng(10).standard_normal((s.n, 3))
+        ctx.lanczos(hs, s.pos, v0, steps=30, nvec=20, mass_weighted=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        vals, _, res, _, _ = ctx.lanczos(hs, s.pos, v0, steps=30, nvec=20, mass_weighted=True)
+        t_lz = time.perf_counter() - t0
+        # central difference of the forces along v (h = 1e-4 A) as the check of H v
+        h = 1e-4
+        fp, _ = ctx.evaluate(hs, s.pos + h * v)
+        fm, _ = ctx.evaluate(hs, s.pos - h * v)
+        fd = -(fp - fm) / (2 * h)
+        ctx.close()
+        key = "fp64" if precision == "double" else "mixed"
+        out[key] = {"hvp_per_s": 1.0 / t_hvp, "hvp_ms": t_hvp * 1e3, "param_gradients_ms": t_grad * 1e3,
+                    "lanczos_30_steps_20_vectors_ms": t_lz * 1e3,
+                    "lowest_ritz_values": [float(x) for x in vals[:3]],
+                    "hvp_vs_central_difference_rms_rel": float(np.sqrt(((hv - fd) ** 2).sum() / (fd ** 2).sum()))}
+    return out
+
+
 def copy_probe(mb):
     """Ten device-to-device copies of `mb` megabytes: the kernel whose
     FETCH_SIZE / WRITE_SIZE counters calibrate the HBM-traffic runs."""
@@ -459,37 +669,71 @@ def replica_bench(count=64, steps=100, segments=2, seed=0):
             "chains_replica_steps_per_s": rs / t_chain}
 
 
-def cpu_baseline(s, alpha):
-    """The reference's own CPU path (oracle/_ref, compiled from the ProtoMol
-    sources) timed for one force evaluation of the same force set on one host
-    core; falls back to the C restatement when the reference build is absent."""
-    from oracle import pyoracle as po
-    kind = "reference" if po.available("reference") else "port"
-    o = po.Oracle(kind)
-    terms = [po.term(po.POT_LJ, po.SWF_C2, RC, SWITCHON), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, RC, alpha=alpha)]
-    t0 = time.perf_counter()
-    o.cutoff(s, terms, RC, CELL)
-    o.pme(s, GRID, ORDER, RC, CELL, flags=6)
-    o.bonded(s, 3)
-    sec = time.perf_counter() - t0
-    ns_day = (1.0 / sec) * DT_FS * 86400 * 1e-6
-    out = {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": kind,
-           "sample": "one force evaluation (LJ-C2 + Ewald-real cutoff, PME 96^3 reciprocal + correction, "
-                     "bonds + angles) of the 98,304-atom box, single thread, %.2f s" % sec}
-    # multi-core C restatement of the pair part (per-thread force buffers), for context
+def cpu_info():
+    model = ""
     try:
-        p = po.Oracle("port")
-        nthr = min(os.cpu_count() or 1, 16)
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(s, alpha):
+    """The CPU path beside the GPU (a reported baseline only), per the plan of
+    timing the reference and its restatement on the same host:
+      * t_ref: the reference itself (oracle/_ref, compiled from the ProtoMol
+        sources; single-threaded like ProtoMol) for one force evaluation of
+        the main force set (LJ-C2 + Ewald-real cutoff, PME 96^3 reciprocal +
+        correction, bonds + angles) of the 98,304-atom box;
+      * t_1, t_N: the C restatement of the same evaluation on one thread and
+        with its pair loop on all N host threads (per-thread force buffers;
+        its PME stays on one thread, as ZFFT is);
+    value = the reference's all-core equivalent t_ref * t_N / t_1 in ns/day."""
+    from oracle import pyoracle as po
+    nthr = oracle_threads()
+    terms = [po.term(po.POT_LJ, po.SWF_C2, RC, SWITCHON), po.term(po.POT_EWALDREAL, po.SWF_CUTOFF, RC, alpha=alpha)]
+
+    def ns_day(sec):
+        return (1.0 / sec) * DT_FS * 86400 * 1e-6
+    p = po.Oracle("port")
+    t0 = time.perf_counter()
+    p.cutoff(s, terms, RC, CELL)
+    p.pme(s, GRID, ORDER, RC, CELL, flags=6)
+    p.bonded(s, 3)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = p.cutoff_omp(s, terms, RC, CELL, threads=nthr)
+    p.pme(s, GRID, ORDER, RC, CELL, flags=6)
+    p.bonded(s, 3)
+    tn = time.perf_counter() - t0
+    used = int(r["threads"])
+    if po.available("reference"):
+        o = po.Oracle("reference")
         t0 = time.perf_counter()
-        r = p.cutoff_omp(s, terms, RC, CELL, threads=nthr)
-        t_pair = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        p.pme(s, GRID, ORDER, RC, CELL, flags=6)
-        t_pme = time.perf_counter() - t0
-        out["openmp_port_ns_per_day"] = (1.0 / (t_pair + t_pme)) * DT_FS * 86400 * 1e-6
-        out["openmp_port_threads"] = int(r["threads"])
-    except Exception as ex:  # informational only
-        out["openmp_port_error"] = str(ex)
+        o.cutoff(s, terms, RC, CELL)
+        o.pme(s, GRID, ORDER, RC, CELL, flags=6)
+        o.bonded(s, 3)
+        tref = time.perf_counter() - t0
+        kind = "reference"
+        sample = ("one force evaluation of the 98,304-atom box (LJ-C2 + Ewald-real cutoff, PME 96^3 reciprocal + "
+                  "correction, bonds + angles): the reference build on 1 core %.2f s, scaled by the C "
+                  "restatement's speedup on %d threads (%.2f s -> %.2f s)" % (tref, used, t1, tn))
+        value = ns_day(tref * tn / t1)
+    else:
+        tref = None
+        kind = "port"
+        sample = ("one force evaluation of the 98,304-atom box by the C restatement on %d threads, %.2f s "
+                  "(%.2f s on one thread)" % (used, tn, t1))
+        value = ns_day(tn)
+    out = {"value": value, "unit": "ns/day", "cores": used, "kind": kind, "sample": sample,
+           "reference_1core_ns_per_day": ns_day(tref) if tref else None,
+           "port_1thread_ns_per_day": ns_day(t1), "port_all_threads_ns_per_day": ns_day(tn),
+           "R_port_over_reference": (t1 / tref) if tref else None}
+    out.update(cpu_info())
     return out
 
 
@@ -507,6 +751,10 @@ def main():
     ap.add_argument("--list-skin", type=int, default=-1, metavar="MILLI_A",
                     help="pair-list skin in 1e-3 A (-1 automatic, 0 rebuild every step)")
     ap.add_argument("--no-c3", action="store_true", help="skip the million-atom measurement")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 648-atom plain-method measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the differentiation (HVP, Lanczos) measurement")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the force-error comparisons against the FP64 oracle")
     ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-water (SHAKE, 2 fs) measurement")
     ap.add_argument("--no-conf", action="store_true", help="skip the example .conf / protein measurements")
     ap.add_argument("--bonded-probe", action="store_true",
@@ -591,22 +839,31 @@ def main():
         "pme_us_per_step": pme_avg_s * 1e6,
         "pair_us_per_step": pair_avg_s * 1e6,
         "pme_hbm_frac": (pme_bytes / pme_avg_s) / PEAK_HBM,
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 600 and run["cluster"] == 8 else None,
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) "
                                      "at list skin 0.6 A, cluster 8; null for other options",
                      "kernel": "k_pair (LJ-C2 + Ewald-real, %d flops/pair)" % FLOPS_PER_PAIR,
-                     "note": "compute bound on the FP32 vector ALU (the kernel issues no MFMA); "
-                             "peak = MI355X FP32 157.3 TFLOP/s"},
+                     "note": "bound by FP32 vector ALU issue (the kernel issues no MFMA); "
+                             "peak = MI355X vector FP32 157.3 TFLOP/s (counts packed FP32 dual issue, "
+                             "which this build leaves off: the non-packed rate is 78.6)",
+                     "frac_of_nonpacked_fp32": achieved / (PEAK_FP32 / 2)},
         "energies_end": {"coulomb": float(mixed["energies"][0]), "lj": float(mixed["energies"][1]),
                          "bond": float(mixed["energies"][2]), "angle": float(mixed["energies"][3])},
         "list_lanes_per_pair": mixed["stats"]["lanes"] / max(mixed["pairs"], 1),
     }
+    if not args.no_accuracy:
+        # the main workload's force error at the end of the timed run, outside the timed region
+        line["accuracy"] = accuracy_vs_oracle(s, run["x"], alpha, GRID, CELL)
+    if not args.no_c1:
+        line["c1_water648"] = c1_bench()
     if not args.no_replicas:
         line["replica_exchange"] = replica_bench()
     if not args.no_c3:
-        line["c3_million_atoms"] = c3_bench()
+        line["c3_million_atoms"] = c3_bench(accuracy=not args.no_accuracy)
+    if not args.no_c5:
+        line["c5_differentiation"] = c5_bench()
     if not args.no_rigid:
         line["rigid_water_shake"] = rigid_bench()
     if not args.no_conf:

@@ -314,7 +314,9 @@ enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_P
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 
-/* Wait for all work queued by this context. */
+/* Wait for all work queued by this context.  Like pm_md_get_state, it
+ * reports what the kernels of the queued work flagged (PM_ERR_CAPACITY,
+ * PM_ERR_NUMERIC): pm_md_run queues its steps and returns at once. */
 int pm_synchronize(pm_context* ctx);
 
 /* Batches of independent systems (replicas), each in its own context set up

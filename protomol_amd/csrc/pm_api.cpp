@@ -435,6 +435,8 @@ void free_stencils(pm_context* c) {
 // atomics per lane but widens the clusters' bounding boxes: on the 98k water
 // box, 5 A cells, it issued 29 % more pair lanes and ran 25 % slower)
 int choose_m(const pm_context* c) {
+  // per-atom LJ parameters run in the pair kernel's 8-atom-cluster form only
+  if (c->d_atom_lj_f || c->d_atom_lj_d) return 8;
   if (c->m_override == 8 || c->m_override == 16) return c->m_override;
   return 8;
 }
@@ -833,15 +835,27 @@ struct PairPass {
 // Plan the pair passes of a force group: one pass per force, except that a
 // single-term LennardJones force and a single-term Coulomb force with the same
 // algorithm cutoff and the same switching cutoff share one pass (the sum of
-// two OneAtomPair evaluations over the same pair set).
+// two OneAtomPair evaluations over the same pair set), and that a two-term
+// force whose terms switch (OneAtomPairTwo.h:166-178) at different cutoffs
+// runs as one pass per term: each switched term is zero beyond its own
+// cutoff, so the sum of the two passes is the OneAtomPairTwo evaluation, and
+// the pair kernel never clips a term below its pass's cutoff.
 std::vector<PairPass> plan_passes(pm_context* c, const std::vector<int>& handles) {
   std::vector<PairPass> passes;
   std::vector<bool> used(handles.size(), false);
   for (size_t i = 0; i < handles.size(); i++) {
     ForceDesc& f = c->forces[handles[i]];
     if (f.pme || f.bonded || used[i]) continue;
-    PairPass p{f.terms, f.algo_cutoff};
     used[i] = true;
+    if (f.terms.size() == 2 && f.terms[0].cutoff != f.terms[1].cutoff &&
+        (f.terms[0].swf != PM_SWF_CUTOFF || f.terms[1].swf != PM_SWF_CUTOFF)) {
+      // a term with Cutoff switching is still zeroed beyond its cutoff when
+      // the other term switches (apply_switch with the Cutoff function)
+      passes.push_back(PairPass{{f.terms[0]}, f.algo_cutoff});
+      passes.push_back(PairPass{{f.terms[1]}, f.algo_cutoff});
+      continue;
+    }
+    PairPass p{f.terms, f.algo_cutoff};
     if (f.terms.size() == 1) {
       for (size_t j = i + 1; j < handles.size(); j++) {
         ForceDesc& g = c->forces[handles[j]];
@@ -2284,7 +2298,10 @@ void drop_graph(pm_context* c) {
 // fork/join of the second stream.
 int md_units(pm_context* c, int kind, int count) {
   int r;
-  const int G = std::min(count, 8);
+  // graphs always hold 8 steps: a run of fewer steps captures the graph the
+  // next runs replay and launches its own steps directly, so that runs of
+  // different lengths never capture again
+  const int G = 8;
   const bool use_graph = c->graph_option != 0 && c->timing_mask == 0 && count >= 2;
   int done = 0;
   if (use_graph) {
@@ -2801,7 +2818,7 @@ int pm_synchronize(pm_context* c) {
   if ((r = set_device(c))) return r;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream2));
-  return PM_OK;
+  return check_overflow(c);
 }
 
 }  // extern "C"
@@ -3087,7 +3104,10 @@ int pm_batch_run(pm_batch* b, int nsteps) {
   // every step but the last from a captured graph of up to 8 steps
   const int count = nsteps - 1;
   int done = 0;
-  const int G = std::min(count, 8);
+  // graphs always hold 8 steps (a shorter run captures one for the next
+  // runs and launches its steps directly), so runs of different lengths
+  // never capture again
+  const int G = 8;
   b->batched_last = false;
   if (count >= 2 && b->mode == PM_BATCH_AUTO) {
     // one launch per kernel over all replicas, replayed in steps of G

@@ -912,12 +912,26 @@ __device__ inline long long force_fix_of(double v) { return __double2ll_rn(v * P
 // ii, so a wave evaluates 64 / M list entries against the cluster per iteration.
 // A pair is kept when the j slot follows the i slot (every unordered pair is
 // listed once, by the cluster of its lower slot).  The i force accumulates
-// per lane in FP64 and is reduced over the j slots at the end; the j force of
-// each entry is summed over its row of M lanes with DPP and added with one
-// 64-bit integer atomic per component.  Forces accumulate per slot, so the
-// atomics of one instruction fall on a few contiguous cache lines; integer
-// addition is associative, so the result does not depend on their order.
-template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
+// per lane (FP32 partial sums of PM_IFLUSH trips added into FP64 totals in the
+// mixed kernel, FP64 in the FP64 kernel) and is reduced over the j slots at
+// the end; the j force of each entry is summed over its row of M lanes with
+// DPP and added with one 64-bit integer atomic per component.  Forces
+// accumulate per slot, so the atomics of one instruction fall on a few
+// contiguous cache lines; integer addition is associative, so the result does
+// not depend on their order.
+//
+// Arithmetic per pair (OneAtomPair.h:127-188, OneAtomPairTwo.h:131-199):
+// LJ E = A r^-12 - B r^-6, F/r = (12 A r^-12 - 6 B r^-6) / r^2; Coulomb
+// E = qq / r, F/r = E / r^2; Ewald real space E = qq poly5(a r) e^(-a^2 r^2) / r,
+// F/r = (E + 2a/sqrt(pi) qq e^(-a^2 r^2)) / r^2; switched F = F S - E dS,
+// E = E S.  Every term of a pass switches at the pass's cutoff (the host
+// splits two-term forces whose switched terms end at different cutoffs), so
+// no term is clipped inside the pair cutoff here.
+#define PM_IFLUSH 4
+// PA: per-atom LJ parameters (pm_set_lj_per_atom) instead of the type table,
+// a kernel of its own so that its registers do not set the table kernel's
+// occupancy.
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
 __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int bx) {
   extern __shared__ unsigned char s_lj_raw[];
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
@@ -962,13 +976,18 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
   const T c_cut2 = (T)pp.c_cut2, c_on2 = (T)pp.c_on2, c_sw1 = (T)pp.c_sw1, c_sw2 = (T)pp.c_sw2,
           c_sw3 = (T)pp.c_sw3, c_c15_1 = (T)pp.c_c15_1, c_c05_3 = (T)pp.c_c05_3, c_c15_3 = (T)pp.c_c15_3;
   const T alpha = (T)pp.alpha, tapi = (T)pp.two_alpha_pi, csf = (T)pp.csf;
+  // e^(-a^2 r^2) = 2^(r^2 * ex_k), and p a for A&S 7.1.26's t = 1/(1 + p a r)
+  const T ex_k = (T)(-(pp.alpha * pp.alpha) * 1.4426950408889634);
+  const T p_alpha = (T)(0.3275911 * pp.alpha);
   const bool modify = pp.modify != 0;
 
-  double fxa = 0, fya = 0, fza = 0;
+  double fxa = 0, fya = 0, fza = 0;       // i force totals (FP64)
+  T fxs = 0, fys = 0, fzs = 0;            // partial sums since the last flush
   T elj = 0, ec = 0;
   T vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
   const int* __restrict__ lst = a.jlist + (long long)cl * a.lcap;
   const int cnt = a.jcount[cl];
+  const int last = cnt > 0 ? cnt - 1 : 0;
   const bool vir = WE && a.want_virial != 0;
   long long* __restrict__ ff = a.force_slot;
   const int comp = ii < 3 ? ii : ii - 3;   // force component a lane adds (lanes ii < 6)
@@ -976,9 +995,10 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
   // Two iterations per trip with double-buffered prefetch: while row A
   // computes, the j atom of row B and the entry of the next trip's row A are
   // in flight, and the other way round, so no register copies carry values
-  // across the loop; the trip's two rows share one atomic instruction.
-  int eA = js < cnt ? lst[js] : a.dummy_slot;
-  int eB = js + JW < cnt ? lst[js + JW] : a.dummy_slot;
+  // across the loop; the trip's two rows share one atomic instruction.  Loads
+  // past the list's end re-read its last entry (masked), so no load branches.
+  int eA = cnt > 0 ? lst[min(js, last)] : a.dummy_slot;
+  int eB = cnt > 0 ? lst[min(js + JW, last)] : a.dummy_slot;
   P4 pA = spos[eA & 0x7fffffff];
   int mA = a.slot_meta[eA & 0x7fffffff];
   P4 pB;
@@ -1003,13 +1023,15 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     }
     const T r2 = dx * dx + dy * dy + dz * dz;
     bool mask = valid && ireal && sj > si && (r2 <= cut2);
-    int excl = PM_EXCL_NONE;
-    if (__ballot(flagged && mask) != 0ull) {
+    // exclusions: only entries the list flagged hold an excluded pair
+    bool modified = false, any_mod = false;
+    if (__ballot(flagged) != 0ull) {
       if (flagged && mask) {
         const int xm = a.jmask[(long long)cl * a.lcap + k];
-        if ((xm >> ii) & 1) { excl = PM_EXCL_FULL; mask = false; }
-        else if ((xm >> (16 + ii)) & 1) excl = PM_EXCL_MODIFIED;
+        if ((xm >> ii) & 1) mask = false;
+        else modified = ((xm >> (16 + ii)) & 1) != 0;
       }
+      any_mod = __ballot(modified) != 0ull;
     }
     T qj;
     if constexpr (sizeof(T) == 4) qj = __int_as_float(pj.w); else qj = __longlong_as_double(pj.w);
@@ -1019,48 +1041,78 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     T F, E1 = 0, E2 = 0;
     if constexpr (LJSWF != PM_SWF_NONE) {
       T A, B;
-      lj_ab<T>(a, s_lj, ti, mj, nt1, oi, excl, A, B);
+      if constexpr (PA) {
+        lj_ab<T>(a, s_lj, ti, mj, nt1, oi, modified ? PM_EXCL_MODIFIED : PM_EXCL_NONE, A, B);
+      } else {
+        const LJ4 ab = s_lj[ti * nt1 + (mj & 0xff)];
+        A = (T)ab.x;
+        B = (T)ab.y;
+        if (any_mod && modified) { A = (T)ab.z; B = (T)ab.w; }   // A14, B14 (LennardJonesForce.h:55-60)
+      }
       const T r6 = rr2 * rr2 * rr2;
-      const T r12 = r6 * r6;
-      const T r6B = B * r6;
-      const T r12A = A * r12;
-      T e1 = r12A - r6B;
-      T f1 = T(12) * r12A * rr2 - T(6) * r6B * rr2;
-      if (modify)
-        apply_switch<T, LJSWF>(r2, r2 * rinv, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3,
-                               lj_c15_3, f1, e1);
+      const T t12 = A * (r6 * r6);
+      const T t6 = B * r6;
+      T e1 = t12 - t6;
+      T f1 = (T(12) * t12 - T(6) * t6) * rr2;
+      if constexpr (LJSWF == PM_SWF_C2) {
+        // C2SwitchingFunction.h:64-82: S = sw1 c2^2 (c^2 + 2r^2 - 3s^2) for s^2 <= r^2,
+        // dS = sw3 (c2^2 - c2 (c^2 + 2r^2 - 3s^2)), with c2 = c^2 - r^2
+        const T c2 = lj_cut2 - r2;
+        const T c4 = c2 * (lj_sw2 + T(2) * r2);
+        const bool in = r2 >= lj_on2;
+        const T v = in ? lj_sw1 * (c2 * c4) : T(1);
+        const T d = in ? lj_sw3 * (c2 * c2 - c4) : T(0);
+        f1 = f1 * v - e1 * d;
+        e1 = e1 * v;
+      } else if constexpr (LJSWF == PM_SWF_C1) {
+        if (modify)
+          apply_switch<T, LJSWF>(r2, r2 * rinv, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3,
+                                 lj_c15_3, f1, e1);
+      }
       F = f1;
       E1 = e1;
     }
     if constexpr (CK != PM_COUL_NONE) {
       T e2, f2;
+      T qq = qi * qj;
+      if (any_mod && modified) qq *= csf;   // coulombScalingFactor (CoulombForce.h:49-51)
       if constexpr (CK == PM_COUL_PLAIN) {
-        e2 = qi * qj * rinv;
-        if (excl == PM_EXCL_MODIFIED) e2 *= csf;
+        e2 = qq * rinv;
         f2 = e2 * rr2;
       } else {
-        T qq = qi * qj;
-        if (excl == PM_EXCL_MODIFIED) qq *= csf;
-        const T r = r2 * rinv;
-        const T ar = alpha * r;
-        const T t = Prec<T>::rcp(T(1) + T(0.3275911) * ar);
+        // CoulombEwaldRealForce.h:45-73 with poly5 (A&S 7.1.26) for erfc
+        T t;
+        if constexpr (sizeof(T) == 4) t = Prec<T>::rcp(T(1) + p_alpha * (r2 * rinv));
+        else t = Prec<T>::rcp(T(1) + T(0.3275911) * (alpha * (r2 * rinv)));
         const T poly = t * (T(0.254829592) + t * (T(-0.284496736) + t * (T(1.421413741) +
                                                  t * (T(-1.453152027) + t * T(1.061405429)))));
-        const T ex = qq * Prec<T>::expneg(ar * ar);
+        T ex;
+        if constexpr (sizeof(T) == 4) ex = qq * __builtin_amdgcn_exp2f(r2 * ex_k);
+        else { const T ar = alpha * (r2 * rinv); ex = qq * Prec<T>::expneg(ar * ar); }
         e2 = poly * ex * rinv;
-        f2 = (e2 + tapi * ex) * rinv * rinv;
+        f2 = (e2 + tapi * ex) * rr2;
       }
-      if (modify)
-        apply_switch<T, CSWF>(r2, r2 * rinv, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3, f2, e2);
+      if constexpr (CSWF == PM_SWF_C2) {
+        const T c2 = c_cut2 - r2;
+        const T c4 = c2 * (c_sw2 + T(2) * r2);
+        const bool in = r2 >= c_on2;
+        const T v = in ? c_sw1 * (c2 * c4) : T(1);
+        const T d = in ? c_sw3 * (c2 * c2 - c4) : T(0);
+        f2 = f2 * v - e2 * d;
+        e2 = e2 * v;
+      } else if constexpr (CSWF == PM_SWF_C1) {
+        if (modify)
+          apply_switch<T, CSWF>(r2, r2 * rinv, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3, f2, e2);
+      }
       if constexpr (LJSWF != PM_SWF_NONE) F += f2; else F = f2;
       E2 = e2;
     }
     F = mask ? F : T(0);
     // fij = F * (xj - xi): -fij on i, +fij on j (OneAtomPair.h:178-181)
     const T fx_ = F * dx, fy_ = F * dy, fz_ = F * dz;
-    fxa -= (double)fx_;
-    fya -= (double)fy_;
-    fza -= (double)fz_;
+    fxs -= fx_;
+    fys -= fy_;
+    fzs -= fz_;
     if constexpr (WE) {
       elj += mask ? E1 : T(0);
       ec += mask ? E2 : T(0);
@@ -1074,20 +1126,21 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
   };
 
   bool range_bad = false;   // a converted force beyond the fixed-point range (or not finite)
+  int trip = 0;
   for (int base = 0; base < cnt; base += 2 * JW) {
     const int kA = base + js, kB = kA + JW;
     // row A; prefetch the j atom of row B and the entry of the next row A
     const int eA_now = eA;
     pB = spos[eB & 0x7fffffff];
     mB = a.slot_meta[eB & 0x7fffffff];
-    eA = kA + 2 * JW < cnt ? lst[kA + 2 * JW] : a.dummy_slot;
+    eA = lst[min(kA + 2 * JW, last)];
     int sA;
     const T vA = row(kA, eA_now, pA, mA, sA);
     // row B; prefetch the j atom of the next row A and the entry of the next row B
     const int eB_now = eB;
     pA = spos[eA & 0x7fffffff];
     mA = a.slot_meta[eA & 0x7fffffff];
-    eB = kB + 2 * JW < cnt ? lst[kB + 2 * JW] : a.dummy_slot;
+    eB = lst[min(kB + 2 * JW, last)];
     int sB;
     const T vB = row(kB, eB_now, pB, mB, sB);
     // lanes ii = 0, 1, 2 add the x, y, z component of row A, lanes 3, 4, 5 those of row B
@@ -1097,7 +1150,13 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
       range_bad |= !(__builtin_fabs(v) < T(PM_FORCE_LIMIT));
       atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
     }
+    if (sizeof(T) == 4 && ++trip == PM_IFLUSH) {
+      trip = 0;
+      fxa += (double)fxs; fya += (double)fys; fza += (double)fzs;
+      fxs = T(0); fys = T(0); fzs = T(0);
+    }
   }
+  fxa += (double)fxs; fya += (double)fys; fza += (double)fzs;
   // reduce the i force over the j slots (lanes ii, ii+8, ...); lanes
   // js = 0, 1, 2 then add its x, y, z component
 #pragma unroll
@@ -1140,14 +1199,14 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     }
   }
 }
-template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
-__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) { pair_body<T, LJSWF, CK, CSWF, WE, M>(box, a, pp, blockIdx.x); }
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
+__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) { pair_body<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, blockIdx.x); }
 struct PairArgs { PmBox box; PmNbLaunch a; PmPairParams pp; };
 template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
 __global__ __launch_bounds__(256) void kb_pair(const PairArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const PairArgs& q = t[r];
-  pair_body<T, LJSWF, CK, CSWF, WE, M>(q.box, q.a, q.pp, blockIdx.x - pre[r]);
+  pair_body<T, LJSWF, CK, CSWF, WE, M, false>(q.box, q.a, q.pp, blockIdx.x - pre[r]);
 }
 
 
@@ -1621,18 +1680,29 @@ template <typename T, int LJ, int CK, int CS>
 static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
   const dim3 g((a.ncl_max + 3) / 4), b(256);
+  const bool pa = LJ != PM_SWF_NONE && (sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr);
   if (pm_recording()) {
+    // per-atom LJ parameters have no batched kernel: the record is marked
+    // unsupported and the batch runs its replicas as chains
     const PairArgs q{box, a, p};
-    pm_record(PM_REC_PAIR, pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m), g.x, 256,
+    pm_record(pa ? -1 : PM_REC_PAIR, pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m), g.x, 256,
               (unsigned)lds, &q, sizeof(q));
     return;
   }
+  if constexpr (LJ != PM_SWF_NONE) {
+    if (pa) {
+      // clusters of 8 only (the layout uses 8 whenever per-atom parameters are set)
+      if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 8, true>), g, b, lds, s, box, a, p);
+      else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 8, true>), g, b, lds, s, box, a, p);
+      return;
+    }
+  }
   if (a.m == 16) {
-    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 16>), g, b, lds, s, box, a, p);
-    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 16>), g, b, lds, s, box, a, p);
+    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 16, false>), g, b, lds, s, box, a, p);
+    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 16, false>), g, b, lds, s, box, a, p);
   } else {
-    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 8>), g, b, lds, s, box, a, p);
-    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 8>), g, b, lds, s, box, a, p);
+    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 8, false>), g, b, lds, s, box, a, p);
+    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 8, false>), g, b, lds, s, box, a, p);
   }
 }
 

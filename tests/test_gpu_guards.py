@@ -70,12 +70,14 @@ def test_md_run_reports_nonfinite_state(gpu_ctx_factory):
     v = s.vel.copy()
     v[5, 2] = np.inf
     c.md_set_state(s.pos, v)
+    c.md_run(3)    # queues the steps; the flag is read when the run is synchronized
     with pytest.raises(gpu.PmError) as ei:
-        c.md_run(3)
+        c.synchronize()
     assert ei.value.code == gpu.PM_ERR_NUMERIC
     say("error:", ei.value.msg)
     c.md_set_state(s.pos, s.vel)
     c.md_run(3)
+    c.synchronize()
 
 
 def test_molecular_virial_request_is_refused(gpu_ctx_factory):
