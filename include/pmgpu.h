@@ -309,8 +309,12 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        with the plane held in LDS; used where a plane of
  *                        the grid fits, separate passes elsewhere: faster
  *                        alone, slower beside the pair kernel).
+ *   PM_OPT_PAIR_BLOCKS   blocks (of four waves) per compute unit of the
+ *                        pair kernel's persistent grid: 0 (default) automatic
+ *                        (6, one below what its registers allow, so that the
+ *                        PME kernels beside it find free slots), 1..8.
  * pm_get_option returns the value in effect (after the automatic choice). */
-enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3 };
+enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PAIR_BLOCKS = 4 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

@@ -216,6 +216,9 @@ struct pm_context {
   // captured MD steps (hipGraph), replayed while nothing they reference changes
   int graph_option = 1;              // PM_OPT_GRAPH
   int pme_fft_option = 0;            // PM_OPT_PME_FFT
+  int pair_blocks_per_cu = 0;        // PM_OPT_PAIR_BLOCKS (0: automatic)
+  int cu_count = 256;                // compute units of the device
+  int* d_pair_work = nullptr;        // the persistent pair kernel's work counters (2 ints, zero between launches)
 
   // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
   // cutoff + skin and are rebuilt, with the cell layout, when an atom has
@@ -439,6 +442,14 @@ int choose_m(const pm_context* c) {
   if (c->d_atom_lj_f || c->d_atom_lj_d) return 8;
   if (c->m_override == 8 || c->m_override == 16) return c->m_override;
   return 8;
+}
+
+// Blocks per CU of the persistent pair kernel (four waves each): the option,
+// or 6 -- one below the 7 that its registers allow -- so that the second
+// stream's PME kernels find free wave slots and registers while it runs.
+int pair_blocks_per_cu(const pm_context* c) {
+  if (c->pair_blocks_per_cu > 0) return c->pair_blocks_per_cu;
+  return 6;
 }
 
 // list skin in A: the option, or the automatic choice
@@ -1100,6 +1111,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
     a.want_virial = vir ? 1 : 0;
     a.want_energy = want_energy ? 1 : 0;
+    a.pair_blocks = c->cu_count * pair_blocks_per_cu(c);
+    a.work = c->d_pair_work;
   }
   // the list buffers may have grown while the passes were set up
   for (auto& pl : pair_launch) {
@@ -1586,6 +1599,16 @@ int pm_create(int device, pm_context** out) {
     return PM_ERR_DEVICE;
   }
   c->box.err = c->d_overflow;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
+      c->cu_count = v;
+  }
+  if (hipMalloc((void**)&c->d_pair_work, 2 * sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_pair_work, 0, 2 * sizeof(int)) != hipSuccess) {
+    pm_destroy(c);
+    return PM_ERR_DEVICE;
+  }
   *out = c;
   return PM_OK;
 }
@@ -1601,6 +1624,7 @@ void pm_destroy(pm_context* ctx) {
   free_all(ctx);
   dfree(ctx->d_md);
   dfree(ctx->d_overflow);
+  dfree(ctx->d_pair_work);
   hipEventDestroy(ctx->ev_fork);
   hipEventDestroy(ctx->ev_join);
   hipStreamDestroy(ctx->stream2);
@@ -2779,6 +2803,12 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->md_forces_valid = false;
     return PM_OK;
   }
+  if (option == PM_OPT_PAIR_BLOCKS) {
+    if (value < 0 || value > 8)
+      return fail(c, PM_ERR_ARGUMENT, "pair kernel blocks per CU must be 0 (automatic) or 1..8 (got %d)", value);
+    c->pair_blocks_per_cu = value;
+    return PM_OK;
+  }
   if (option == PM_OPT_PME_FFT) {
     if (value < 0 || value > 2)
       return fail(c, PM_ERR_ARGUMENT, "PME FFT option must be 0 (automatic), 1 (separate passes) or 2 (plane kernel; got %d)", value);
@@ -2801,6 +2831,10 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_LIST_SKIN) {
     *value = (int)std::lround(list_skin(c) * 1000.0);
+    return PM_OK;
+  }
+  if (option == PM_OPT_PAIR_BLOCKS) {
+    *value = pair_blocks_per_cu(c);
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {

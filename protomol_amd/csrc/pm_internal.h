@@ -151,6 +151,10 @@ struct PmNbLaunch {
   double* epart;             // [ncl_max * PM_NE]
   int want_virial;
   int want_energy;           // write the per-cluster energy records
+  // persistent pair kernel: blocks of the grid, and its work counters
+  // (next cluster, waves done; zero between launches)
+  int pair_blocks;
+  int* work;
 };
 
 // ---------------------------------------------------------------- launchers

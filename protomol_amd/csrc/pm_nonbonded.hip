@@ -931,8 +931,10 @@ __device__ inline long long force_fix_of(double v) { return __double2ll_rn(v * P
 // PA: per-atom LJ parameters (pm_set_lj_per_atom) instead of the type table,
 // a kernel of its own so that its registers do not set the table kernel's
 // occupancy.
-template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
-__device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int bx) {
+// The block's copy of the LJ type table in LDS (every thread of the block
+// must call this before any wave evaluates a cluster).
+template <typename T, int LJSWF>
+__device__ inline void pair_load_table(const PmNbLaunch& a) {
   extern __shared__ unsigned char s_lj_raw[];
   typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
   LJ4* s_lj = reinterpret_cast<LJ4*>(s_lj_raw);
@@ -943,8 +945,16 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     for (int t = threadIdx.x; t < nt1 * nt1; t += blockDim.x) s_lj[t] = src[t];
     __syncthreads();
   }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int cl = bx * 4 + wave;
+}
+
+// The pairs of i-cluster cl, evaluated by one wave.
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
+__device__ inline void pair_cluster(const PmBox& box, const PmNbLaunch& a, const PmPairParams& pp, const int cl) {
+  extern __shared__ unsigned char s_lj_raw[];
+  typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
+  const LJ4* s_lj = reinterpret_cast<const LJ4*>(s_lj_raw);
+  const int nt1 = a.ntypes1;
+  const int lane = threadIdx.x & 63;
   const int ncl = a.counts[1];
   if (cl >= ncl) {
     if (WE && cl < a.ncl_max && lane < PM_NE) a.epart[(long long)cl * PM_NE + lane] = 0.0;
@@ -1199,14 +1209,40 @@ __device__ inline void pair_body(PmBox box, PmNbLaunch a, PmPairParams pp, int b
     }
   }
 }
+// Persistent form: a fixed grid (a set number of blocks per CU, below the
+// occupancy the kernel's registers allow, so that the kernels of the second
+// stream find free wave slots and registers at once instead of waiting for
+// pair waves to retire) whose waves take clusters from a work counter.
+// work[0] is the next cluster, work[1] counts the waves that have left; the
+// last wave to leave resets both, so every launch (and graph replay) starts
+// from zero.  Clusters ncl .. ncl_max-1 still zero their energy records.
 template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
-__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) { pair_body<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) {
+  pair_load_table<T, LJSWF>(a);
+  const int lane = threadIdx.x & 63;
+  const int nwork = WE ? a.ncl_max : a.counts[1];
+  for (;;) {
+    int cl = 0;
+    if (lane == 0) cl = atomicAdd(&a.work[0], 1);
+    cl = __builtin_amdgcn_readfirstlane(cl);
+    if (cl >= nwork) break;
+    pair_cluster<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, cl);
+  }
+  if (lane == 0) {
+    const int left = atomicAdd(&a.work[1], 1);
+    if (left == (int)(gridDim.x * (blockDim.x >> 6)) - 1) {
+      atomicExch(&a.work[0], 0);
+      atomicExch(&a.work[1], 0);
+    }
+  }
+}
 struct PairArgs { PmBox box; PmNbLaunch a; PmPairParams pp; };
 template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
 __global__ __launch_bounds__(256) void kb_pair(const PairArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const PairArgs& q = t[r];
-  pair_body<T, LJSWF, CK, CSWF, WE, M, false>(q.box, q.a, q.pp, blockIdx.x - pre[r]);
+  pair_load_table<T, LJSWF>(q.a);
+  pair_cluster<T, LJSWF, CK, CSWF, WE, M, false>(q.box, q.a, q.pp, (blockIdx.x - pre[r]) * 4 + (threadIdx.x >> 6));
 }
 
 
@@ -1679,13 +1715,16 @@ void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const 
 template <typename T, int LJ, int CK, int CS>
 static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
-  const dim3 g((a.ncl_max + 3) / 4), b(256);
+  // batched launches: one block per four clusters
+  const dim3 gb((a.ncl_max + 3) / 4), b(256);
+  // single-context launches: the persistent grid (a.pair_blocks blocks at most)
+  const dim3 g(std::max(1, std::min((a.ncl_max + 3) / 4, a.pair_blocks)));
   const bool pa = LJ != PM_SWF_NONE && (sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr);
   if (pm_recording()) {
     // per-atom LJ parameters have no batched kernel: the record is marked
     // unsupported and the batch runs its replicas as chains
     const PairArgs q{box, a, p};
-    pm_record(pa ? -1 : PM_REC_PAIR, pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m), g.x, 256,
+    pm_record(pa ? -1 : PM_REC_PAIR, pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m), gb.x, 256,
               (unsigned)lds, &q, sizeof(q));
     return;
   }

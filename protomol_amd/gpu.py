@@ -32,7 +32,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
            "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info",
            "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle"]
-OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3}
+OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3, "pair_blocks": 4}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 
