@@ -74,7 +74,7 @@ def pme_alpha(rc, accuracy):
     return alpha
 
 
-def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=0, autotune=False, pair_blocks=0):
+def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=0, autotune=False):
     from protomol_amd import gpu
     ctx = gpu.Context(0, precision)
     ctx.set_topology(s)
@@ -82,7 +82,6 @@ def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=
     ctx.set_option("graph", 1 if graph else 0)
     ctx.set_option("list_skin", skin)
     ctx.set_option("cluster_size", cluster)
-    ctx.set_option("pair_blocks", pair_blocks)
     hs = make_forces(ctx, pme_alpha(RC, 1e-6))
     ctx.md_setup(hs, "leapfrog", dt=DT_FS)
     ctx.md_set_state(s.pos, s.vel)
@@ -762,8 +761,6 @@ def main():
                     help="only the tiled-protein bonded evaluations (for profiling the bonded kernels)")
     ap.add_argument("--only-replicas", action="store_true",
                     help="run only the batched replica-exchange measurement and print its record")
-    ap.add_argument("--pair-blocks", type=int, default=0,
-                    help="blocks per CU of the persistent pair kernel (0: the library's choice)")
     ap.add_argument("--autotune", action="store_true",
                     help="let pm_md_autotune choose the layout options (default: the library's defaults)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -788,13 +785,12 @@ def main():
 
     # the throughput pass: no instrumentation (captured graphs unless --graph 0)
     run = run_md(s, "mixed", steps, warmup, graph=bool(args.graph), skin=args.list_skin,
-                 autotune=args.autotune, pair_blocks=args.pair_blocks)
+                 autotune=args.autotune)
     ms_step = run["seconds"] * 1e3 / steps
     ns_day = (steps / run["seconds"]) * DT_FS * 86400 * 1e-6
     # the roofline pass: the same steps with HIP events around the pair and PME
     # kernels on the streams they run on (event timing launches directly)
-    mixed = run_md(s, "mixed", steps, warmup, timing=("pair", "pme"), skin=run["skin_milli"], cluster=run["cluster"],
-                   pair_blocks=args.pair_blocks)
+    mixed = run_md(s, "mixed", steps, warmup, timing=("pair", "pme"), skin=run["skin_milli"], cluster=run["cluster"])
     ms_step_events = mixed["seconds"] * 1e3 / steps
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(10, steps // 10)

@@ -309,12 +309,8 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        with the plane held in LDS; used where a plane of
  *                        the grid fits, separate passes elsewhere: faster
  *                        alone, slower beside the pair kernel).
- *   PM_OPT_PAIR_BLOCKS   blocks (of four waves) per compute unit of the
- *                        pair kernel's persistent grid: 0 (default) automatic
- *                        (6, one below what its registers allow, so that the
- *                        PME kernels beside it find free slots), 1..8.
  * pm_get_option returns the value in effect (after the automatic choice). */
-enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PAIR_BLOCKS = 4 };
+enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 
@@ -331,8 +327,9 @@ int pm_synchronize(pm_context* ctx);
  * with k of the attempt's parity (getNextAddress :949-957) are tested with
  * delta = (1/(kB Tk) - 1/(kB Tk+1)) (E(k+1) - E(k)) (calcDelta :941-946) and
  * accepted when delta <= 0 or u <= exp(-delta) (canDoEvenSwitch :841-880);
- * an accepted pair swaps bath temperatures and the velocities are scaled by
- * sqrt(T_new / T_old).  E is the potential energy of the last step.  No
+ * an accepted pair swaps bath temperatures, and the velocities are scaled by
+ * sqrt(T_new / T_old) or left to the caller (pm_batch_set_exchange_velocities).
+ * E is the potential energy of the last step.  No
  * reference counterpart for the batch itself: the reference runs one replica
  * per MPI process. */
 typedef struct pm_batch pm_batch;
@@ -344,6 +341,16 @@ int pm_batch_run(pm_batch* batch, int nsteps);
 /* One exchange attempt; u from (seed, attempt, k).  accepted[k] and deltas[k]
  * (nreplicas entries each, may be NULL) are written for the tested pairs. */
 int pm_batch_exchange(pm_batch* batch, int attempt, unsigned long long seed, int* accepted, double* deltas);
+/* What an accepted exchange does to the two replicas' velocities.
+ * PM_EXCHANGE_RESCALE (default): scaled by sqrt(T_new / T_old), doSwitch's
+ * rescaling branch (rem.cpp:930-935).  PM_EXCHANGE_KEEP: left as they are,
+ * for the caller to redraw at the new temperature and remove the momentum,
+ * which is what doSwitch runs (redrawVelocities = 1, rem.cpp:912, :923-928:
+ * randomVelocity, removeLinearMomentum, removeAngularMomentum); the library
+ * has no atom names to tell the solute from water, which momentum removal
+ * needs, so the host driver does it (pmrun's REM mode does). */
+enum { PM_EXCHANGE_RESCALE = 0, PM_EXCHANGE_KEEP = 1 };
+int pm_batch_set_exchange_velocities(pm_batch* batch, int mode);
 /* ladder[k]: the replica now at ladder temperature temps[k]. */
 int pm_batch_ladder(pm_batch* batch, int* ladder, double* temps);
 

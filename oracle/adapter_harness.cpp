@@ -188,7 +188,11 @@ int adp_evaluate(int n, const double* pos, const double* q, const int* type, int
   }
   ScalarStructure e;
   e.clear();
+  // as the applications set both flags from virialCalc / molVirialCalc
+  // (applications/rem-app/remMT.cpp:479-480); ScalarStructure's constructor
+  // leaves the molecular virial on, which the GPU forces refuse
   e.virial(virial != 0);
+  e.molecularVirial(false);
   group->evaluateSystemForces(t, &x, &f, &e);
   for (int i = 0; i < n; i++) {
     forces[3 * i] = f[i].x;

@@ -395,3 +395,51 @@ def ref_topology_free(topo: dict) -> None:
     if topo.get("handle"):
         topo["lib"].ref_topo_free(topo["handle"])
         topo["handle"] = None
+
+
+# ------------------------------------------------------ replica-exchange driver
+_M64 = (1 << 64) - 1
+
+
+def _splitmix(z):
+    z = (z + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def rem_uniform(seed, attempt, k):
+    """The exchange test's uniform deviate of attempt `attempt`, ladder pair
+    (k, k+1), as pmgpu.h documents it (the reference draws drand48 here,
+    rem.cpp:865)."""
+    h = _splitmix((seed ^ _splitmix((attempt * 0x100000001B3 + 2 * k + 1) & _M64)) & _M64)
+    return (h >> 11) * (1.0 / 9007199254740992.0)
+
+
+def rem_driver(oracle, temps, attempts, seed, nswitches, eqswitches):
+    """applications/rem-app/rem.cpp:646-720 over recorded attempts.
+
+    temps: the ladder temperatures; attempts: {round: energies by ladder slot
+    before that round's attempt}.  Every round i >= EqSwitches attempts the
+    pairs (k, k+1) with k of i's parity (getNextAddress :949-957); delta is
+    calcDelta (:941-946), accepted when delta <= 0 or u <= exp(-delta)
+    (canDoEvenSwitch :841-880); an accepted switch is counted by the replica
+    of the higher temperature (doEvenSwitch/doOddSwitch :783-838: the node
+    whose partner address is lower records it) and the rates are the counts
+    over NumSwitches - EqSwitches (:708-712).  Returns ({round: accepted flags
+    by k}, rates by ladder slot)."""
+    import math
+    R = len(temps)
+    credit = [0.0] * R
+    flags = {}
+    for i in range(eqswitches, nswitches):
+        e = attempts[i]
+        f = {}
+        for k in range(i & 1, R - 1, 2):
+            d = oracle.rem_delta(temps[k], temps[k + 1], e[k], e[k + 1])
+            w = 1.0 if d <= 0 else math.exp(-d)
+            f[k] = 1 if rem_uniform(seed, i, k) <= w else 0
+            if f[k]:
+                credit[k + 1] += 1.0
+        flags[i] = f
+    return flags, [c / (nswitches - eqswitches) for c in credit]

@@ -216,9 +216,6 @@ struct pm_context {
   // captured MD steps (hipGraph), replayed while nothing they reference changes
   int graph_option = 1;              // PM_OPT_GRAPH
   int pme_fft_option = 0;            // PM_OPT_PME_FFT
-  int pair_blocks_per_cu = 0;        // PM_OPT_PAIR_BLOCKS (0: automatic)
-  int cu_count = 256;                // compute units of the device
-  int* d_pair_work = nullptr;        // the persistent pair kernel's work counters (2 ints, zero between launches)
 
   // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
   // cutoff + skin and are rebuilt, with the cell layout, when an atom has
@@ -442,14 +439,6 @@ int choose_m(const pm_context* c) {
   if (c->d_atom_lj_f || c->d_atom_lj_d) return 8;
   if (c->m_override == 8 || c->m_override == 16) return c->m_override;
   return 8;
-}
-
-// Blocks per CU of the persistent pair kernel (four waves each): the option,
-// or 6 -- one below the 7 that its registers allow -- so that the second
-// stream's PME kernels find free wave slots and registers while it runs.
-int pair_blocks_per_cu(const pm_context* c) {
-  if (c->pair_blocks_per_cu > 0) return c->pair_blocks_per_cu;
-  return 6;
 }
 
 // list skin in A: the option, or the automatic choice
@@ -1111,8 +1100,6 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.epart = c->d_epart + (long long)pi * c->ncl_max * PM_NE;
     a.want_virial = vir ? 1 : 0;
     a.want_energy = want_energy ? 1 : 0;
-    a.pair_blocks = c->cu_count * pair_blocks_per_cu(c);
-    a.work = c->d_pair_work;
   }
   // the list buffers may have grown while the passes were set up
   for (auto& pl : pair_launch) {
@@ -1599,16 +1586,6 @@ int pm_create(int device, pm_context** out) {
     return PM_ERR_DEVICE;
   }
   c->box.err = c->d_overflow;
-  {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
-      c->cu_count = v;
-  }
-  if (hipMalloc((void**)&c->d_pair_work, 2 * sizeof(int)) != hipSuccess ||
-      hipMemset(c->d_pair_work, 0, 2 * sizeof(int)) != hipSuccess) {
-    pm_destroy(c);
-    return PM_ERR_DEVICE;
-  }
   *out = c;
   return PM_OK;
 }
@@ -1624,7 +1601,6 @@ void pm_destroy(pm_context* ctx) {
   free_all(ctx);
   dfree(ctx->d_md);
   dfree(ctx->d_overflow);
-  dfree(ctx->d_pair_work);
   hipEventDestroy(ctx->ev_fork);
   hipEventDestroy(ctx->ev_join);
   hipStreamDestroy(ctx->stream2);
@@ -2140,7 +2116,8 @@ int pm_md_set_state(pm_context* c, const double* x, const double* v) {
   if (v) HIPCHECK(c, hipMemcpy(c->d_vel, v, bytes, hipMemcpyHostToDevice));
   // SHAKE's reference positions start at the new positions
   if (x && c->d_shake_last) HIPCHECK(c, hipMemcpy(c->d_shake_last, x, bytes, hipMemcpyHostToDevice));
-  c->md_forces_valid = false;
+  // the forces follow the positions only
+  if (x) c->md_forces_valid = false;
   return PM_OK;
 }
 
@@ -2226,6 +2203,8 @@ int pm_shake(pm_context* c, double* x, double* v, const double* x_last, double d
 int pm_rattle(pm_context* c, const double* x, double* v, double dt_fs, double eps, int max_iter, int* sweeps) {
   return constraint_call(c, 1, x, nullptr, v, nullptr, dt_fs, eps, max_iter, sweeps);
 }
+
+
 
 }  // extern "C"
 
@@ -2803,12 +2782,6 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->md_forces_valid = false;
     return PM_OK;
   }
-  if (option == PM_OPT_PAIR_BLOCKS) {
-    if (value < 0 || value > 8)
-      return fail(c, PM_ERR_ARGUMENT, "pair kernel blocks per CU must be 0 (automatic) or 1..8 (got %d)", value);
-    c->pair_blocks_per_cu = value;
-    return PM_OK;
-  }
   if (option == PM_OPT_PME_FFT) {
     if (value < 0 || value > 2)
       return fail(c, PM_ERR_ARGUMENT, "PME FFT option must be 0 (automatic), 1 (separate passes) or 2 (plane kernel; got %d)", value);
@@ -2831,10 +2804,6 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_LIST_SKIN) {
     *value = (int)std::lround(list_skin(c) * 1000.0);
-    return PM_OK;
-  }
-  if (option == PM_OPT_PAIR_BLOCKS) {
-    *value = pair_blocks_per_cu(c);
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {
@@ -2894,6 +2863,7 @@ struct pm_batch {
   BatchPlan eplan;            // the step with energies
   bool batched_last = false;  // the last pm_batch_run used batched launches
   std::string why_chains;     // why the replicas ran as chains
+  int exchange_velocities = PM_EXCHANGE_RESCALE;   // what an accepted swap does to the velocities
 };
 
 namespace {
@@ -3242,8 +3212,10 @@ int pm_batch_exchange(pm_batch* b, int attempt, unsigned long long seed, int* ac
   // neighbour pairs (k, k+1) of the ladder with k of the attempt's parity
   // (getNextAddress, applications/rem-app/rem.cpp:949-957); the Metropolis
   // test on calcDelta (:941-946, canDoEvenSwitch :841-880) swaps the two
-  // replicas' bath temperatures, and their velocities follow by
-  // sqrt(T_new / T_old) (doSwitch's rescaling branch, :930-935)
+  // replicas' bath temperatures; with PM_EXCHANGE_RESCALE their velocities
+  // follow by sqrt(T_new / T_old) (doSwitch's rescaling branch, :930-935),
+  // with PM_EXCHANGE_KEEP they are left for the caller to redraw (doSwitch
+  // as the reference runs it, redrawVelocities = 1, :912, :923-928)
   const double kb = 0.001987191;   // Constant::BOLTZMANN (base/pmconstants.cpp:74)
   for (int k = attempt & 1; k + 1 < R; k += 2) {
     const int ra = b->ladder[k], rb = b->ladder[k + 1];
@@ -3262,10 +3234,17 @@ int pm_batch_exchange(pm_batch* b, int attempt, unsigned long long seed, int* ac
       c->md_temp = tnew[u];
       if (hipMemcpyAsync(&c->d_md->temperature, &c->md_temp, sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return bfail(b, PM_ERR_DEVICE, "temperature copy failed");
-      pm_launch_scale(c->stream, 3 * c->n, std::sqrt(tnew[u] / told[u]), c->d_vel);
+      if (b->exchange_velocities == PM_EXCHANGE_RESCALE)
+        pm_launch_scale(c->stream, 3 * c->n, std::sqrt(tnew[u] / told[u]), c->d_vel);
       if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PM_ERR_DEVICE, "exchange failed");
     }
   }
+  return PM_OK;
+}
+
+int pm_batch_set_exchange_velocities(pm_batch* b, int mode) {
+  if (!b || (mode != PM_EXCHANGE_RESCALE && mode != PM_EXCHANGE_KEEP)) return PM_ERR_ARGUMENT;
+  b->exchange_velocities = mode;
   return PM_OK;
 }
 

@@ -151,10 +151,6 @@ struct PmNbLaunch {
   double* epart;             // [ncl_max * PM_NE]
   int want_virial;
   int want_energy;           // write the per-cluster energy records
-  // persistent pair kernel: blocks of the grid, and its work counters
-  // (next cluster, waves done; zero between launches)
-  int pair_blocks;
-  int* work;
 };
 
 // ---------------------------------------------------------------- launchers
@@ -352,6 +348,7 @@ struct PmConstraints {
 // SHAKE (rattle 0: x, v corrected, xl the reference positions, then set to
 // x) or RATTLE (rattle 1: v corrected); two launches
 bool pm_launch_constraints(hipStream_t s, const PmConstraints& c, int rattle, double* x, double* v, double* xl);
+
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation
 struct PmBondedLaunch {

@@ -1209,32 +1209,11 @@ __device__ inline void pair_cluster(const PmBox& box, const PmNbLaunch& a, const
     }
   }
 }
-// Persistent form: a fixed grid (a set number of blocks per CU, below the
-// occupancy the kernel's registers allow, so that the kernels of the second
-// stream find free wave slots and registers at once instead of waiting for
-// pair waves to retire) whose waves take clusters from a work counter.
-// work[0] is the next cluster, work[1] counts the waves that have left; the
-// last wave to leave resets both, so every launch (and graph replay) starts
-// from zero.  Clusters ncl .. ncl_max-1 still zero their energy records.
+// One wave per i-cluster, four clusters per block.
 template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
 __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairParams pp) {
   pair_load_table<T, LJSWF>(a);
-  const int lane = threadIdx.x & 63;
-  const int nwork = WE ? a.ncl_max : a.counts[1];
-  for (;;) {
-    int cl = 0;
-    if (lane == 0) cl = atomicAdd(&a.work[0], 1);
-    cl = __builtin_amdgcn_readfirstlane(cl);
-    if (cl >= nwork) break;
-    pair_cluster<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, cl);
-  }
-  if (lane == 0) {
-    const int left = atomicAdd(&a.work[1], 1);
-    if (left == (int)(gridDim.x * (blockDim.x >> 6)) - 1) {
-      atomicExch(&a.work[0], 0);
-      atomicExch(&a.work[1], 0);
-    }
-  }
+  pair_cluster<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 struct PairArgs { PmBox box; PmNbLaunch a; PmPairParams pp; };
 template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
@@ -1715,10 +1694,9 @@ void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const 
 template <typename T, int LJ, int CK, int CS>
 static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   size_t lds = (LJ != PM_SWF_NONE) ? (size_t)a.ntypes1 * a.ntypes1 * (sizeof(T) == 4 ? 16 : 32) : 0;
-  // batched launches: one block per four clusters
+  // one block per four clusters (batched launches: per replica)
   const dim3 gb((a.ncl_max + 3) / 4), b(256);
-  // single-context launches: the persistent grid (a.pair_blocks blocks at most)
-  const dim3 g(std::max(1, std::min((a.ncl_max + 3) / 4, a.pair_blocks)));
+  const dim3 g = gb;
   const bool pa = LJ != PM_SWF_NONE && (sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr);
   if (pm_recording()) {
     // per-atom LJ parameters have no batched kernel: the record is marked

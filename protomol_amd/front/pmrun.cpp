@@ -52,7 +52,14 @@ bool vec3(const pmf::Config& cfg, const char* key, double out[3]) {
 // pm_batch_exchange); an accepted switch is credited to the replica of
 // the higher temperature, and REMExchangeRatesFile holds each replica's
 // credits divided by the rounds that attempted switches (the first line is
-// always 0).  All replicas run on this GPU as one batch.
+// always 0).  The two replicas of an accepted switch get new velocities at
+// their new temperatures with linear and angular momentum removed (doSwitch,
+// rem.cpp:906-936, which redraws with redrawVelocities = 1); the reference
+// seeds that draw from the clock (getTimerSeed), here the seed is
+// seed + 7919 * (round + 1) + replica, so runs repeat.  Every attempt prints
+// one "REM attempt" line: the round, then per ladder slot the potential
+// energy of the replica there before the attempt, then the accepted flag of
+// each tested pair (k, k+1) by k.  All replicas run on this GPU as one batch.
 template <class MakeCtx>
 int run_rem(const pmf::Config& cfg, const pmf::System& sys, const std::vector<double>& x, const std::vector<double>& v,
             long lin, long ang, int integrator, MakeCtx& make_ctx) {
@@ -86,18 +93,53 @@ int run_rem(const pmf::Config& cfg, const pmf::System& sys, const std::vector<do
   pm_batch* batch = nullptr;
   if (pm_batch_create(ctxs.data(), R, &batch) != PM_OK) return die("pm_batch_create failed");
   std::unique_ptr<pm_batch, void (*)(pm_batch*)> hold(batch, pm_batch_destroy);
+  if (pm_batch_set_exchange_velocities(batch, PM_EXCHANGE_KEEP) != PM_OK) return die("pm_batch_set_exchange_velocities");
   std::vector<double> credit(R, 0.0);
-  std::vector<int> acc(R, 0);
+  std::vector<int> acc(R, 0), ladder(R);
+  std::vector<double> slot_t(R), xr(3 * (size_t)sys.n), e(PM_ENERGY_COUNT);
   const unsigned long long seed = (unsigned long long)cfg.integer("seed", 1234);
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < nsw; i++) {
     if (pm_batch_run(batch, len) != PM_OK) return die(std::string("pm_batch_run: ") + pm_batch_last_error(batch));
     if (i < eq) continue;
+    // the potential energies the attempt compares, by ladder slot
+    if (pm_batch_ladder(batch, ladder.data(), slot_t.data()) != PM_OK) return die("pm_batch_ladder");
+    std::printf("REM attempt %d", i);
+    for (int k = 0; k < R; k++) {
+      Device tmp;   // not owned: only for the error text
+      tmp.c = ctxs[ladder[k]];
+      const bool ok = tmp.ok(pm_md_get_state(ctxs[ladder[k]], nullptr, nullptr, e.data()), err);
+      tmp.c = nullptr;
+      if (!ok) return die(err);
+      double ep = 0;
+      for (int u = 0; u < 7; u++) ep += e[u];   // ScalarStructure::potentialEnergy
+      std::printf(" %.17g", ep);
+    }
     std::fill(acc.begin(), acc.end(), 0);
     if (pm_batch_exchange(batch, i, seed, acc.data(), nullptr) != PM_OK)
       return die(std::string("pm_batch_exchange: ") + pm_batch_last_error(batch));
-    for (int k = (i & 1); k + 1 < R; k += 2)
-      if (acc[k]) credit[k + 1] += 1.0;
+    for (int k = (i & 1); k + 1 < R; k += 2) std::printf(" %d", acc[k]);
+    std::printf("\n");
+    if (pm_batch_ladder(batch, ladder.data(), slot_t.data()) != PM_OK) return die("pm_batch_ladder");
+    for (int k = (i & 1); k + 1 < R; k += 2) {
+      if (!acc[k]) continue;
+      credit[k + 1] += 1.0;
+      // doSwitch: randomVelocity at the slot's temperature, then
+      // removeLinearMomentum and removeAngularMomentum
+      for (int u = k; u <= k + 1; u++) {
+        const int r = ladder[u];
+        Device tmp;
+        tmp.c = ctxs[r];
+        bool ok = tmp.ok(pm_md_get_state(ctxs[r], xr.data(), nullptr, nullptr), err);
+        if (ok) {
+          std::vector<double> vr = pmf::random_velocity(sys, slot_t[u], (unsigned)(seed + 7919ull * (i + 1) + r));
+          pmf::remove_momentum(sys, xr, vr, true, true);
+          ok = tmp.ok(pm_md_set_state(ctxs[r], nullptr, vr.data()), err);
+        }
+        tmp.c = nullptr;
+        if (!ok) return die(err);
+      }
+    }
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (cfg.has("remexchangeratesfile")) {

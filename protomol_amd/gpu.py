@@ -31,8 +31,9 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_synchronize", "pm_batch_create", "pm_batch_destroy", "pm_batch_last_error", "pm_batch_run",
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
            "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info",
-           "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle"]
-OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3, "pair_blocks": 4}
+           "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle",
+           "pm_batch_set_exchange_velocities"]
+OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 
@@ -96,6 +97,7 @@ def load_library(path: str = LIB_PATH):
     lib.pm_set_charges.argtypes = [P, D]
     lib.pm_md_autotune.argtypes = [P, C.c_int, I, I, D]
     lib.pm_batch_set_mode.argtypes = [P, C.c_int]
+    lib.pm_batch_set_exchange_velocities.argtypes = [P, C.c_int]
     lib.pm_batch_info.argtypes = [P, I, C.c_char_p, C.c_int]
     lib.pm_lanczos.argtypes = [P, I, C.c_int, D, D, C.c_int, C.c_int, C.c_int, D, D, D, D]
     lib.pm_md_set_constraints.argtypes = [P, C.c_double, C.c_int, C.c_double, C.c_int]
@@ -388,6 +390,11 @@ class Batch:
         dl = np.full(n, np.nan)
         self._check(self.lib.pm_batch_exchange(self.h, int(attempt), int(seed), _ip(acc), _dp(dl)))
         return acc, dl
+
+    def set_exchange_velocities(self, mode: str):
+        """'rescale' (default): sqrt(T_new / T_old); 'keep': left for the caller
+        to redraw (rem.cpp's doSwitch redraws them)."""
+        self._check(self.lib.pm_batch_set_exchange_velocities(self.h, {"rescale": 0, "keep": 1}[mode]))
 
     def set_mode(self, mode: str):
         """"auto" (batched launches when the replicas allow them) or "chains"."""

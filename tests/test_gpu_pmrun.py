@@ -182,12 +182,16 @@ Integrator {
 TEMPS = [300.0, 303.0, 306.0, 309.0, 312.0]
 
 
-def test_pmrun_replica_exchange_rates(tmp_path, gpu_ctx_factory):
-    """REM mode (rem.cpp:604-720 semantics): the rates file against the same
-    replicas driven through the Python batch binding, credits to the
-    higher-temperature partner, attempts from EqSwitches on with the round's
-    parity."""
-    from protomol_amd import gpu
+def test_pmrun_replica_exchange_rates(tmp_path, port):
+    """REM mode against the oracle's restatement of rem.cpp's driver
+    (applications/rem-app/rem.cpp:646-720): pmrun prints, for every attempt,
+    the potential energies by ladder slot and the accepted flags; the oracle
+    recomputes each flag from those energies (calcDelta via orc_rem_delta,
+    the Metropolis test with the documented uniform), the parity of the
+    attempted pairs, the first attempted round (EqSwitches) and the rates
+    file's credits to the higher-temperature partner.  Accepted switches
+    redraw velocities (doSwitch with redrawVelocities = 1)."""
+    from oracle import pyoracle
     tmp = str(tmp_path)
     for f in FILES[1:4]:
         shutil.copy(os.path.join(G, f), tmp)
@@ -196,41 +200,28 @@ def test_pmrun_replica_exchange_rates(tmp_path, gpu_ctx_factory):
     say("case: pmrun REM water216 5 replicas 7 switches")
     r = subprocess.run([PMRUN, "rem.conf", "--precision", "double"], cwd=tmp, capture_output=True, text=True,
                        timeout=120)
-    say(r.stdout[-1500:], r.stderr[-1500:])
+    say(r.stdout[-2500:], r.stderr[-1500:])
     assert r.returncode == 0, r.stderr
     rates = np.loadtxt(os.path.join(tmp, "rates.out"))
-    # the same replicas through the binding
-    psf, par = (os.path.join(G, f) for f in FILES[1:3])
-    t = front.load_system(psf, par)
-    x = front.read_coords(os.path.join(G, FILES[3]))
-    box, org = (30.8927, 27.8485, 30.7217), (0.98035, -0.08475, 0.19285)
-    ctxs = []
-    defs = ["Angle", "Bond", "LennardJones -algorithm NonbondedCutoff -switchingFunction C2 -switchon 11 -cutoff 13",
-            "Coulomb -algorithm NonbondedCutoff -switchingFunction C1 -cutoff 13"]
-    for T in TEMPS:
-        v = front.random_velocity(psf, par, T, 7)
-        s = t.system(x, v, box=box, origin=org)
-        c = gpu_ctx_factory("double")
-        c.set_topology(s)
-        c.set_boundary(s.box, s.origin, 7.0)
-        hs = [c.force(*front.force_values(d)) for d in defs]
-        c.md_setup(hs, "langevin", dt=1.0, temperature=T, gamma=5.0, seed=7)
-        c.md_set_state(s.pos, s.vel)
-        ctxs.append(c)
-    b = gpu.Batch(ctxs)
-    credit = np.zeros(len(TEMPS))
-    for i in range(7):
-        b.run(20)
-        if i < 1:
+    R, nsw, eq = len(TEMPS), 7, 1
+    attempts, flags = {}, {}
+    for line in r.stdout.splitlines():
+        if not line.startswith("REM attempt"):
             continue
-        acc, _ = b.exchange(i, 7)
-        for k in range(i & 1, len(TEMPS) - 1, 2):
-            credit[k + 1] += acc[k]
-    b.close()
-    say("rates", rates, "binding", credit / 6)
+        w = line.split()
+        i = int(w[2])
+        attempts[i] = [float(x) for x in w[3:3 + R]]
+        flags[i] = {k: int(f) for k, f in zip(range(i & 1, R - 1, 2), w[3 + R:])}
+        assert len(w) == 3 + R + len(flags[i])
+    assert sorted(attempts) == list(range(eq, nsw))      # one attempt per round from EqSwitches on
+    want_flags, want_rates = pyoracle.rem_driver(port, TEMPS, attempts, 7, nsw, eq)
+    say("flags", flags, "oracle", want_flags, "rates", rates, "oracle", want_rates)
+    assert flags == want_flags
     assert rates[0] == 0
-    assert credit.sum() > 0                    # neighbours 3 K apart do exchange
-    np.testing.assert_allclose(rates, credit / 6, rtol=1e-5)      # printed with 6 significant digits
+    assert sum(sum(f.values()) for f in flags.values()) > 0     # neighbours 3 K apart do exchange
+    np.testing.assert_allclose(rates, want_rates, rtol=1e-5)      # printed with 6 significant digits
+    # every replica's energy stays finite and in the liquid range after redraws
+    assert all(np.isfinite(e).all() and max(e) < 0 for e in attempts.values())
 
 
 PME_CONF = """

@@ -414,6 +414,28 @@ def test_rem_delta_boltzmann_ratio(port):
     assert port.rem_delta(300.0, 330.0, -50.0, -50.0) == 0.0
 
 
+def test_rem_driver_rounds_and_credits(port):
+    """The driver restatement (rem.cpp:646-720) on hand-made attempts: rounds
+    before EqSwitches attempt nothing, round i tests the pairs (k, k+1) with k
+    of i's parity only (getNextAddress :949-957), a pair whose colder slot
+    holds the higher energy always swaps (delta < 0), one with a huge positive
+    delta never does, and each switch is credited to the hotter slot k + 1
+    (:814-815), the rates dividing by NumSwitches - EqSwitches."""
+    from oracle import pyoracle
+    temps = [300.0, 310.0, 320.0, 330.0]
+    swap, keep = (-100.0, -200.0), (-200.0, 1e6)
+    attempts = {2: [*swap, *swap], 3: [-1.0, *keep, -1.0], 4: [*keep, *swap]}
+    flags, rates = pyoracle.rem_driver(port, temps, attempts, 11, 5, 2)
+    assert sorted(flags) == [2, 3, 4]
+    assert flags[2] == {0: 1, 2: 1}
+    assert flags[3] == {1: 0}
+    assert flags[4] == {0: 0, 2: 1}
+    assert rates == [0.0, 1 / 3, 0.0, 2 / 3]
+    # the uniform is in [0, 1) and differs between attempts and pairs
+    us = {pyoracle.rem_uniform(11, i, k) for i in range(20) for k in range(8)}
+    assert len(us) == 160 and min(us) >= 0.0 and max(us) < 1.0
+
+
 def _langevin_kick(port, n, dt, temp, gamma, mass, f, xi, v):
     import ctypes as C
     D = C.POINTER(C.c_double)
