@@ -211,6 +211,21 @@ int pm_md_constraint_count(pm_context* ctx, int* count);
 int pm_shake(pm_context* ctx, double* x, double* v, const double* x_last, double dt_fs, double eps, int max_iter,
              int* sweeps);
 int pm_rattle(pm_context* ctx, const double* x, double* v, double dt_fs, double eps, int max_iter, int* sweeps);
+/* pm_shake without allocating: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as pm_shake, but its work space (device copies of x, v and x_last,
+ * the probe's copy and per-sweep error sums, the sweep count) lies in one
+ * fixed 16 MiB array per GPU that the library holds until the process ends,
+ * so it needs 4 * 24 n + 8 (max_iter + 1) * waves bytes (waves = 4 per 256
+ * constraint groups) plus alignment within 16 MiB -- about 160,000 atoms at
+ * max_iter 30 -- and returns PM_ERR_ARGUMENT beyond that.  The constraint
+ * list must exist (pm_md_set_constraints, or one pm_shake call, made it);
+ * otherwise PM_ERR_STATE.  Every call on one GPU uses that GPU's array:
+ * calls that overlap in time on one GPU (other threads, other contexts on
+ * the same device) race and must be serialised by the caller. */
+int pm_shake_noalloc(pm_context* ctx, double* x, double* v, const double* x_last, double dt_fs, double eps,
+                     int max_iter, int* sweeps);
 
 /* Choose this context's layout options for the MD steps set up with
  * pm_md_setup by timing them: cluster size 8 or 16 (PM_OPT_CLUSTER_SIZE)

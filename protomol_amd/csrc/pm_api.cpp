@@ -1586,6 +1586,12 @@ int pm_create(int device, pm_context** out) {
     return PM_ERR_DEVICE;
   }
   c->box.err = c->d_overflow;
+  // the allocation-free entry point's work space on this device (its address
+  // is looked up once per device, here, with the device current)
+  if (pm_scratch_init(device) != 0) {
+    pm_destroy(c);
+    return PM_ERR_DEVICE;
+  }
   *out = c;
   return PM_OK;
 }
@@ -2204,7 +2210,62 @@ int pm_rattle(pm_context* c, const double* x, double* v, double dt_fs, double ep
   return constraint_call(c, 1, x, nullptr, v, nullptr, dt_fs, eps, max_iter, sweeps);
 }
 
-
+int pm_shake_noalloc(pm_context* c, double* x, double* v, const double* x_last, double dt_fs, double eps,
+                     int max_iter, int* sweeps) {
+  if (!c) return PM_ERR_ARGUMENT;
+  int r;
+  if ((r = set_device(c))) return r;
+  if (!c->have_topo) return fail(c, PM_ERR_STATE, "no topology: call pm_set_topology first");
+  if (!x || !v || !x_last)
+    return fail(c, PM_ERR_ARGUMENT, "pm_shake_noalloc: null positions, velocities or reference positions");
+  if (!(dt_fs > 0.0) || !(eps > 0.0) || max_iter <= 0 || max_iter > 100000)
+    return fail(c, PM_ERR_ARGUMENT, "pm_shake_noalloc: timestep and epsilon must be > 0, max iterations 1..100000");
+  // the constraint list is set-up data (pm_md_set_constraints or an earlier
+  // pm_shake / pm_rattle built it); building it here would allocate
+  if (!c->cons_built)
+    return fail(c, PM_ERR_STATE, "pm_shake_noalloc: no constraint list: call pm_md_set_constraints (or pm_shake) "
+                                 "once first");
+  const size_t n3 = 3 * (size_t)c->n;
+  const size_t nwave = 4 * (size_t)std::max((c->cons_ngroup + 255) / 256, 1);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t vec = up(n3 * sizeof(double));
+  const size_t err = up((size_t)(max_iter + 1) * nwave * sizeof(double));
+  const size_t need = 4 * vec + err + 256;
+  if (need > PM_NOALLOC_SCRATCH_BYTES)
+    return fail(c, PM_ERR_ARGUMENT,
+                "pm_shake_noalloc: needs %zu bytes of work space (%d atoms, %d sweeps) but the library's fixed "
+                "work space holds %u bytes; use pm_shake", need, c->n, max_iter + 1, PM_NOALLOC_SCRATCH_BYTES);
+  unsigned char* base = static_cast<unsigned char*>(pm_scratch_address(c->device));
+  if (!base) return fail(c, PM_ERR_STATE, "pm_shake_noalloc: the work space of this device was not set up");
+  double* dx = reinterpret_cast<double*>(base);
+  double* dv = reinterpret_cast<double*>(base + vec);
+  double* dxl = reinterpret_cast<double*>(base + 2 * vec);
+  double* dscr = reinterpret_cast<double*>(base + 3 * vec);
+  double* derr = reinterpret_cast<double*>(base + 4 * vec);
+  int* dsweeps = reinterpret_cast<int*>(base + 4 * vec + err);
+  const size_t bytes = n3 * sizeof(double);
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(dv, v, bytes, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(dxl, x_last, bytes, hipMemcpyHostToDevice, s));
+  int sw = 0;
+  if (c->cons_ngroup > 0) {
+    // every wave of the probe writes its error part for every sweep count and
+    // the apply kernel writes the sweep count, so nothing here starts at zero
+    PmConstraints k = constraint_launch(c, eps, max_iter, dt_fs);
+    k.err = derr;
+    k.sweeps = dsweeps;
+    k.scratch = dscr;
+    pm_launch_constraints(s, k, 0, dx, dv, dxl);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(&sw, dsweeps, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHECK(c, hipMemcpyAsync(x, dx, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(v, dv, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipStreamSynchronize(s));
+  if (sweeps) *sweeps = sw;
+  return PM_OK;
+}
 
 }  // extern "C"
 

@@ -348,7 +348,14 @@ struct PmConstraints {
 // SHAKE (rattle 0: x, v corrected, xl the reference positions, then set to
 // x) or RATTLE (rattle 1: v corrected); two launches
 bool pm_launch_constraints(hipStream_t s, const PmConstraints& c, int rattle, double* x, double* v, double* xl);
-
+// The allocation-free entry point's work space: a file-scope device array of
+// PM_NOALLOC_SCRATCH_BYTES in pm_shake.hip.  pm_scratch_init (with `device`
+// current; once per device, later calls return at once) looks up its device
+// address; pm_scratch_address returns it (nullptr before the lookup).
+#define PM_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define PM_MAX_DEVICES 64
+int pm_scratch_init(int device);
+void* pm_scratch_address(int device);
 
 // Bonded terms (Bond / Angle system forces), per-atom gather formulation
 struct PmBondedLaunch {

@@ -21,6 +21,8 @@
 
 #include <math.h>
 
+#include <mutex>
+
 namespace {
 
 __device__ inline void ld3(const double* p, int a, double* o) {
@@ -501,6 +503,35 @@ __global__ __launch_bounds__(256) void k_constraint_apply(PmConstraints c, doubl
 }
 
 }  // namespace
+
+// Work space of the allocation-free SHAKE entry point (pm_shake_noalloc): one
+// copy per GPU for the life of the process.  Host code never passes `scratch`
+// itself (in host code it names a host-side stand-in, not device memory):
+// pm_scratch_init looks up each device's address once, with that device
+// current, and every use goes through that address.
+static __device__ __align__(256) unsigned char scratch[PM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[PM_MAX_DEVICES] = {};
+}  // namespace
+
+int pm_scratch_init(int device) {
+  if (device < 0 || device >= PM_MAX_DEVICES) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* ptr = nullptr;
+  // the caller has made `device` current
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
+void* pm_scratch_address(int device) {
+  if (device < 0 || device >= PM_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 bool pm_launch_constraints(hipStream_t s, const PmConstraints& c, int rattle, double* x, double* v, double* xl) {
   if (pm_recording()) {   // no batched form: replica batches fall back to chains

@@ -32,7 +32,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_batch_exchange", "pm_batch_ladder", "pm_set_lj_per_atom", "pm_hessian_vec", "pm_param_gradients",
            "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info",
            "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle",
-           "pm_batch_set_exchange_velocities"]
+           "pm_batch_set_exchange_velocities", "pm_shake_noalloc"]
 OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
@@ -104,6 +104,7 @@ def load_library(path: str = LIB_PATH):
     lib.pm_md_constraint_count.argtypes = [P, I]
     lib.pm_shake.argtypes = [P, D, D, D, C.c_double, C.c_double, C.c_int, I]
     lib.pm_rattle.argtypes = [P, D, D, C.c_double, C.c_double, C.c_int, I]
+    lib.pm_shake_noalloc.argtypes = [P, D, D, D, C.c_double, C.c_double, C.c_int, I]
     _lib = lib
     return lib
 
