@@ -74,6 +74,11 @@ def pme_alpha(rc, accuracy):
     return alpha
 
 
+# library options given on the command line (--opt name=value), applied to
+# every context of the main workload
+EXTRA_OPTIONS = {}
+
+
 def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=0, autotune=False):
     from protomol_amd import gpu
     ctx = gpu.Context(0, precision)
@@ -82,6 +87,8 @@ def run_md(s, precision, steps, warmup, timing=(), graph=True, skin=-1, cluster=
     ctx.set_option("graph", 1 if graph else 0)
     ctx.set_option("list_skin", skin)
     ctx.set_option("cluster_size", cluster)
+    for k, v in EXTRA_OPTIONS.items():
+        ctx.set_option(k, v)
     hs = make_forces(ctx, pme_alpha(RC, 1e-6))
     ctx.md_setup(hs, "leapfrog", dt=DT_FS)
     ctx.md_set_state(s.pos, s.vel)
@@ -763,9 +770,14 @@ def main():
                     help="run only the batched replica-exchange measurement and print its record")
     ap.add_argument("--autotune", action="store_true",
                     help="let pm_md_autotune choose the layout options (default: the library's defaults)")
+    ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
+                    help="a library option (pm_set_option) for the main workload's contexts, e.g. pme_priority=0")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run 10 device copies of MB megabytes (the calibration kernel of HBM counters) and exit")
     args = ap.parse_args()
+    for o in args.opt:
+        k, v = o.split("=", 1)
+        EXTRA_OPTIONS[k] = int(v)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -821,6 +833,7 @@ def main():
         "graph": bool(args.graph),
         "list_skin_A": run["skin"],
         "cluster_size": run["cluster"],
+        "options": dict(EXTRA_OPTIONS),
         "autotuned": run["tuned"] is not None,
         "list_builds_per_step": run["builds"] / steps,
         "higher_is_better": True,

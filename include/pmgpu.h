@@ -228,8 +228,8 @@ int pm_shake_noalloc(pm_context* ctx, double* x, double* v, const double* x_last
                      int max_iter, int* sweeps);
 
 /* Choose this context's layout options for the MD steps set up with
- * pm_md_setup by timing them: cluster size 8 or 16 (PM_OPT_CLUSTER_SIZE)
- * crossed with pair-list skins 0, 0.3, 0.6 and 1.0 A (PM_OPT_LIST_SKIN),
+ * pm_md_setup by timing them: cluster size 8 or 4 (PM_OPT_CLUSTER_SIZE)
+ * crossed with pair-list skins 0, 0.4, 0.8 and 1.2 A (PM_OPT_LIST_SKIN),
  * each run for trial_steps steps after a warm-up from the current state.
  * The state (positions, velocities, Langevin counter) is restored afterwards,
  * so the trajectory does not advance; the fastest options stay set.  Any of
@@ -241,7 +241,9 @@ int pm_md_autotune(pm_context* ctx, int trial_steps, int* cluster_size, int* lis
 /* Kernel statistics of the last evaluation (for roofline reporting):
  * stats[0] pair-list entries (i-cluster x j-atom), stats[1] pairs evaluated
  * (lanes issued), stats[2] clusters, stats[3] padded slots, stats[4] pair-list
- * builds of reusing evaluations since the context was created (PM_OPT_LIST_SKIN). */
+ * builds of reusing evaluations since the context was created (PM_OPT_LIST_SKIN),
+ * stats[5] prunes of the inner lists without a build (PM_OPT_PRUNE_SKIN).
+ * With inner lists, stats[0] and [1] count the inner lists. */
 int pm_last_stats(pm_context* ctx, long long* stats, int nstats);
 
 /* Kernel timing with HIP events recorded on the context's stream around the
@@ -300,14 +302,14 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
 
 /* Layout and kernel options, chosen per context (no reference counterpart:
  * they change speed, never results beyond the mixed-precision tolerance):
- *   PM_OPT_CLUSTER_SIZE  atoms per i-cluster of the pair kernels: 8, 16, or
+ *   PM_OPT_CLUSTER_SIZE  atoms per i-cluster of the pair kernels: 4, 8, or
  *                        0 for the default (8).
  *   PM_OPT_GRAPH         1 (default): pm_md_run replays captured HIP graphs
  *                        of up to 8 steps; 0: every kernel launched directly.
  *                        Steps are captured only while no kernel timing is
  *                        requested (pm_set_timing mask 0).
  *   PM_OPT_LIST_SKIN     pair-list buffer in 1e-3 A: -1 (default) automatic
- *                        (600), 0 off (cells and lists rebuilt at every
+ *                        (800), 0 off (cells and lists rebuilt at every
  *                        evaluation, as the reference rebuilds its cell list,
  *                        ForceGroup.cpp:50), 1..3000.  Lists then hold the j
  *                        atoms within the pair cutoff + skin and are rebuilt
@@ -315,8 +317,9 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        skin / 2 since the last build; every evaluation still
  *                        computes exactly the pairs within the cutoff.  Used
  *                        for one pair pass whose algorithm cutoff covers its
- *                        pair cutoff in a box wider than 2 (cutoff + skin +
- *                        cell diagonal); otherwise lists are rebuilt each time.
+ *                        pair cutoff (any box: boxes narrower than 2 (cutoff
+ *                        + skin + cell diagonal) list by the exact per-atom
+ *                        distance); otherwise lists are rebuilt each time.
  *   PM_OPT_PME_FFT       the y and x transforms of the PME grid: 0 (default)
  *                        automatic (separate passes), 1 separate y and x
  *                        passes over all lines, 2 one kernel per kz plane
@@ -324,8 +327,22 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        with the plane held in LDS; used where a plane of
  *                        the grid fits, separate passes elsewhere: faster
  *                        alone, slower beside the pair kernel).
+ *   PM_OPT_PME_PRIORITY  issue priority (s_setprio, 0..3) of the PME spread,
+ *                        transform and gather waves over the waves of the
+ *                        pair kernels that run beside them on the other
+ *                        stream: -1 (default) automatic (2), 0 equal share.
+ *   PM_OPT_PRUNE_SKIN    inner pair lists, in 1e-3 A: with list reuse, the
+ *                        pair kernel prunes the lists to the j atoms within
+ *                        the pair cutoff + this skin of some atom of the
+ *                        cluster, whenever an atom has moved more than half
+ *                        of it since the last prune (and on every build), and
+ *                        walks the pruned lists in between; every evaluation
+ *                        still computes exactly the pairs within the cutoff.
+ *                        -1 (default) automatic, 0 off, 1..3000; used when
+ *                        below the list skin.
  * pm_get_option returns the value in effect (after the automatic choice). */
-enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3 };
+enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PME_PRIORITY = 4,
+       PM_OPT_PRUNE_SKIN = 5 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

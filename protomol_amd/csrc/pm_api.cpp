@@ -18,8 +18,14 @@
 #include <string>
 #include <vector>
 
-// automatic pair-list skin (A) when PM_OPT_LIST_SKIN is -1
-#define PM_LIST_SKIN_AUTO 0.6
+// automatic pair-list skin (A) when PM_OPT_LIST_SKIN is -1 (98k water box,
+// 1 fs: 0.8 A ran 1.5-2 % faster than 0.6 A, with 0.31 builds per step)
+#define PM_LIST_SKIN_AUTO 0.8
+// automatic inner-list skin (A) when PM_OPT_PRUNE_SKIN is -1: none (on the
+// 98k water box at 1 fs the fastest hydrogens cross half of a 0.2-0.4 A
+// inner skin almost every step, and the prune pass costs more than the
+// 20-25 % of pair lanes it saves)
+#define PM_PRUNE_SKIN_AUTO 0.0
 
 // launch recorder of the calling thread (replica batches)
 static thread_local std::vector<PmRecord>* g_recorder = nullptr;
@@ -178,7 +184,7 @@ struct pm_context {
   int cells_alloc = 0;
   int m = 8;              // atoms per cluster of the current layout
   int cells_m = 0;        // cluster size the slot buffers were sized for
-  int m_override = 0;     // 8 or 16 to force the cluster size (0: from the occupancy)
+  int m_override = 0;     // 4 or 8 to force the cluster size (0: the default, 8)
 
   // stencils keyed by algorithm cutoff
   std::map<double, std::pair<int4*, int>> stencils;
@@ -216,6 +222,7 @@ struct pm_context {
   // captured MD steps (hipGraph), replayed while nothing they reference changes
   int graph_option = 1;              // PM_OPT_GRAPH
   int pme_fft_option = 0;            // PM_OPT_PME_FFT
+  int pme_prio_option = -1;          // PM_OPT_PME_PRIORITY (-1: automatic)
 
   // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
   // cutoff + skin and are rebuilt, with the cell layout, when an atom has
@@ -225,15 +232,25 @@ struct pm_context {
   bool md_prebinned = false;         // the next evaluation's binning and force clearing already queued
   int prebin_gen = -1;               // layout_gen the queued binning was made for
   int layout_gen = 0;                // +1 whenever the cell and slot buffers are reallocated
-  int* d_rebuild = nullptr;          // device flag: 1 = rebuild in this evaluation
+  int* d_rebuild = nullptr;          // device flags: [0] = rebuild in this evaluation, [1] = prune the inner lists
   int4* d_xbuild = nullptr;          // [n] fixed-point position at the last build, w = 1 when valid
+  // inner (pruned) pair lists (PM_OPT_PRUNE_SKIN): entries within the pair
+  // cutoff + the inner skin, rewritten by the pair kernel whenever an atom has
+  // moved more than half the inner skin since the last prune (or on builds)
+  int prune_milli = -1;              // -1 automatic, 0 off, else 1e-3 A
+  int4* d_xprune = nullptr;          // [n] fixed-point position at the last prune, w = 1 when valid
+  int* d_ilist = nullptr;
+  unsigned int* d_imask = nullptr;
+  int* d_icount = nullptr;
+  long long ilist_alloc = 0;
+  bool inner_last = false;           // the last evaluation's pair pass kept inner lists
   bool list_stale = true;            // x_build must be invalidated before the next reuse
   unsigned long long gen = 1;        // bumped by every allocation and every setter
   hipGraphExec_t md_exec = nullptr;
   int md_exec_steps = 0, md_exec_kind = -1;
   unsigned long long md_exec_gen = 0;
 
-  long long last_stats[5] = {0, 0, 0, 0, 0};
+  long long last_stats[6] = {0, 0, 0, 0, 0, 0};
 
   // kernel timing (HIP events on the context's stream), per category
   unsigned timing_mask = 0;
@@ -431,14 +448,25 @@ void free_stencils(pm_context* c) {
 }
 
 // ------------------------------------------------------------ allocation
-// atoms per cluster: 8 unless set (16 halves the list builds and the j-force
-// atomics per lane but widens the clusters' bounding boxes: on the 98k water
-// box, 5 A cells, it issued 29 % more pair lanes and ran 25 % slower)
+// atoms per cluster: 8 unless set to 4 (4 shrinks the clusters' bounding
+// boxes, so fewer list entries miss the cutoff, at the price of twice the
+// j-force atomics per lane; 16 issued 29 % more pair lanes than 8 on the 98k
+// water box with 5 A cells and ran 25 % slower, and is no longer built)
 int choose_m(const pm_context* c) {
   // per-atom LJ parameters run in the pair kernel's 8-atom-cluster form only
   if (c->d_atom_lj_f || c->d_atom_lj_d) return 8;
-  if (c->m_override == 8 || c->m_override == 16) return c->m_override;
+  if (c->m_override == 4 || c->m_override == 8) return c->m_override;
   return 8;
+}
+
+// wave priority of the PME kernels: the option, or the automatic choice
+int pme_priority(const pm_context* c) { return c->pme_prio_option >= 0 ? c->pme_prio_option : 2; }
+
+// inner-list skin in A (0: no inner lists): the option, or the automatic
+// choice; used only below the list skin
+double prune_skin(const pm_context* c) {
+  if (c->prune_milli >= 0) return 1e-3 * c->prune_milli;
+  return PM_PRUNE_SKIN_AUTO;
 }
 
 // list skin in A: the option, or the automatic choice
@@ -474,17 +502,19 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_atom_of_slot, (size_t)c->nslot_max + 1))) return r;
   if ((r = dalloc(c, &c->d_force_slot, 3 * ((size_t)c->nslot_max + 1)))) return r;
   if ((r = dalloc(c, &c->d_xbuild, (size_t)n))) return r;
-  if (!c->d_rebuild && (r = dalloc(c, &c->d_rebuild, 1))) return r;
-  HIPCHECK(c, hipMemset(c->d_rebuild, 0, sizeof(int)));
+  if ((r = dalloc(c, &c->d_xprune, (size_t)n))) return r;
+  if (!c->d_rebuild && (r = dalloc(c, &c->d_rebuild, 2))) return r;
+  HIPCHECK(c, hipMemset(c->d_rebuild, 0, 2 * sizeof(int)));
   c->list_stale = true;
   HIPCHECK(c, hipMemset(c->d_counts, 0, 4 * sizeof(int)));
   if (!c->d_stats) {
-    if ((r = dalloc(c, &c->d_stats, 3))) return r;
-    HIPCHECK(c, hipMemset(c->d_stats, 0, 3 * sizeof(unsigned long long)));
+    if ((r = dalloc(c, &c->d_stats, 4))) return r;
+    HIPCHECK(c, hipMemset(c->d_stats, 0, 4 * sizeof(unsigned long long)));
   }
   c->cells_alloc = nc;
   c->cells_m = m;
   c->jlist_alloc = 0;
+  c->ilist_alloc = 0;
   c->layout_gen++;
   return PM_OK;
 }
@@ -511,6 +541,18 @@ int ensure_lists(pm_context* c, int lcap) {
   if ((r = dalloc(c, &c->d_jcount, c->ncl_max))) return r;
   c->jlist_alloc = need;
   c->lcap = lcap;
+  return PM_OK;
+}
+
+// the inner lists, as large as the full ones (they hold a subset)
+int ensure_inner_lists(pm_context* c) {
+  const long long need = (long long)c->ncl_max * c->lcap;
+  if (need <= c->ilist_alloc && c->d_ilist) return PM_OK;
+  int r;
+  if ((r = dalloc(c, &c->d_ilist, (size_t)need))) return r;
+  if ((r = dalloc(c, &c->d_imask, (size_t)need))) return r;
+  if ((r = dalloc(c, &c->d_icount, c->ncl_max))) return r;
+  c->ilist_alloc = need;
   return PM_OK;
 }
 
@@ -900,16 +942,24 @@ struct AdjointSpec {
 
 // List reuse applies to one pair pass whose algorithm cutoff covers its pair
 // cutoff (the reference's cell pairs then hold every pair within the cutoff,
-// so the pair set does not depend on the cells), in a box where the skin
-// keeps the minimal image unique; never for adjoint passes.
+// so the pair set does not depend on the cells); never for adjoint passes.
+// Lists hold every j atom within the pair cutoff + skin of some atom of the
+// cluster in the minimal image at the build (the bounding-box test in boxes
+// wider than 2 (cutoff + skin + cell diagonal), the exact per-atom test in
+// smaller ones); a pair within the cutoff in its minimal image later was
+// within the cutoff + skin in that image, so in its minimal image, at the
+// build while every atom stays within half the skin of its build position.
 bool list_reuse(const pm_context* c, const std::vector<PairPass>& passes) {
   const double skin = list_skin(c);
   if (!(skin > 0.0) || passes.size() != 1) return false;
   const double pc = std::sqrt(pair_params(passes[0].terms, c->csf).cut2);
-  const double lmin_h = 0.5 * std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
-  const double cd = std::sqrt(c->realcell[0] * c->realcell[0] + c->realcell[1] * c->realcell[1] +
-                              c->realcell[2] * c->realcell[2]);
-  return passes[0].algo_cutoff >= pc && lmin_h > pc + skin + cd + 1e-3;
+  return passes[0].algo_cutoff >= pc;
+}
+
+// Inner lists: with list reuse, an inner skin below the list skin
+bool inner_lists(const pm_context* c) {
+  const double ps = prune_skin(c);
+  return ps > 0.0 && ps < list_skin(c) && c->d_xprune && c->d_rebuild;
 }
 
 // The start of the MD forces' next evaluation, run by the position update
@@ -940,6 +990,11 @@ bool md_step_work(pm_context* c, PmStepWork* w) {
     w->x_build = c->d_xbuild;
     w->lim2 = (float)half * (float)half;
     w->flag = c->d_rebuild;
+    if (inner_lists(c)) {
+      const double hin = 0.5 * prune_skin(c);
+      w->x_prune = c->d_xprune;
+      w->lim2_in = (float)hin * (float)hin;
+    }
     c->disp_prechecked = true;
   }
   return true;
@@ -988,16 +1043,21 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   const double skin = list_skin(c);
   const bool reuse = !adj && list_reuse(c, passes);
   const int* rb = reuse ? c->d_rebuild : nullptr;
+  const bool inner = reuse && inner_lists(c);
+  c->inner_last = inner;
   {
   TimedScope ts(c, PM_T_BIN);
   if (reuse && c->list_stale) {
     pm_launch_invalidate_build(s, n, c->d_xbuild);
+    pm_launch_invalidate_build(s, n, c->d_xprune);
     c->list_stale = false;
   }
   // (n = 0 without reuse: nothing runs; a recorded replica step keeps its
   // kernel sequence the same with and without reuse)
   const float half = (float)(0.5 * skin);
-  pm_launch_check_disp(s, c->box, reuse && !prechecked ? n : 0, c->d_pos, c->d_xbuild, half * half, c->d_rebuild);
+  const float hin = (float)(0.5 * prune_skin(c));
+  pm_launch_check_disp(s, c->box, reuse && !prechecked ? n : 0, c->d_pos, c->d_xbuild, half * half, c->d_rebuild,
+                       inner ? c->d_xprune : nullptr, hin * hin);
   // (the MD position update may have binned the atoms and cleared the forces
   // already, for the layout in place; the last fold cleared the slot forces)
   const bool binned = prebinned && c->prebin_gen == c->layout_gen;
@@ -1012,7 +1072,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
                        c->d_cell_slot, c->d_cell_atoms, c->d_key, c->d_slot_pos,
                        c->precision == PM_PRECISION_DOUBLE ? c->d_slot_pos64 : nullptr, c->d_slot_meta,
-                       c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m, rb, reuse ? c->d_xbuild : nullptr);
+                       c->d_atom_of_slot, c->ntypes, c->nslot_max, c->m, rb, reuse ? c->d_xbuild : nullptr,
+                       inner ? c->d_xprune : nullptr);
   if (!reuse) c->list_stale = true;   // a layout built without the skin is not reusable
   }
 
@@ -1063,6 +1124,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       return fail(c, PM_ERR_UNSUPPORTED, "cell stencil of %d cells (cellsize %g, cutoff %g): use a cellsize of at least cutoff/4", nst, c->cellsize, ps.algo_cutoff);
     double pcut = std::sqrt(pp.cut2);
     if ((r = ensure_lists(c, list_capacity(c, pcut + lskin)))) return r;
+    if (inner && (r = ensure_inner_lists(c))) return r;
     pair_launch.push_back(std::make_pair(PmNbLaunch{}, pp));
     PmNbLaunch& a = pair_launch.back().first;
     a.slot_pos = c->d_slot_pos;
@@ -1107,6 +1169,16 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     pl.first.jmask = c->d_jmask;
     pl.first.jcount = c->d_jcount;
     pl.first.lcap = c->lcap;
+    if (inner) {
+      // the pair cutoff + the inner skin, padded as the list cutoff is
+      const double cp = std::sqrt(pl.second.cut2) + prune_skin(c);
+      const float cf = (float)(cp * (1.0 + 1e-5) + 1e-4);
+      pl.first.inner = 1;
+      pl.first.cut2_prune = cf * cf;
+      pl.first.ilist = c->d_ilist;
+      pl.first.imask = c->d_imask;
+      pl.first.icount = c->d_icount;
+    }
   }
   std::vector<std::pair<PmPmeLaunch, int>> pme_launch;   // (launch, handle)
   double const_coul = 0.0, const_vir = 0.0;
@@ -1137,6 +1209,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       a.fac = kPi * kPi / (f.alpha * f.alpha);
       a.tw = c->d_tw;
       a.err = c->d_overflow;
+      a.prio = pme_priority(c);
       a.force_fix = c->d_force;
       a.epart = c->d_epart + off * PM_NE;
       a.want_virial = vir ? 1 : 0;
@@ -1296,6 +1369,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     {
       TimedScope ts(c, PM_T_LIST);
       pm_launch_list(s, c->box, pl.first, pl.second);
+      if (pl.first.inner) pm_launch_prune(s, c->box, pl.first);
     }
     {
       TimedScope ts(c, PM_T_PAIR);
@@ -1339,10 +1413,13 @@ int check_overflow(pm_context* c) {
 
 // list statistics of the last evaluation (on demand: one reduction + copy)
 int list_stats(pm_context* c) {
-  unsigned long long st[3] = {0, 0, 0};   // entries, lanes, list builds (cumulative)
+  // entries, lanes (of the inner lists when the last pass kept them), list
+  // builds and inner-list prunes without a build (both cumulative)
+  unsigned long long st[4] = {0, 0, 0, 0};
   int counts[4] = {0, 0, 0, 0};
   if (c->d_stats && c->d_jcount && c->d_counts) {
-    pm_launch_list_stats(c->stream, c->d_jcount, c->d_counts, c->d_stats, c->m);
+    const int* len = c->inner_last && c->d_icount ? c->d_icount : c->d_jcount;
+    pm_launch_list_stats(c->stream, len, c->d_counts, c->d_stats, c->m);
     HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -1352,6 +1429,7 @@ int list_stats(pm_context* c) {
   c->last_stats[2] = counts[1];
   c->last_stats[3] = counts[0];
   c->last_stats[4] = (long long)st[2];
+  c->last_stats[5] = (long long)st[3];
   return PM_OK;
 }
 
@@ -1501,6 +1579,7 @@ void free_all(pm_context* c) {
   dfree(c->d_cell_of); dfree(c->d_key); dfree(c->d_cell_count); dfree(c->d_bin_count); dfree(c->d_cell_start); dfree(c->d_cell_slot);
   dfree(c->d_scan_aux);
   dfree(c->d_xbuild); dfree(c->d_rebuild); c->list_stale = true;
+  dfree(c->d_xprune); dfree(c->d_ilist); dfree(c->d_imask); dfree(c->d_icount); c->ilist_alloc = 0;
   dfree(c->d_cell_rank); dfree(c->d_force_slot); dfree(c->d_cell_atoms); dfree(c->d_cluster_cell); dfree(c->d_counts);
   dfree(c->d_slot_pos); dfree(c->d_slot_pos64); dfree(c->d_slot_meta); dfree(c->d_atom_of_slot);
   dfree(c->d_jlist); dfree(c->d_jcount); dfree(c->d_stats); dfree(c->d_epart); dfree(c->d_ered);
@@ -2486,8 +2565,8 @@ int pm_md_autotune(pm_context* c, int trial_steps, int* cluster_size, int* list_
     c->md_forces_valid = false;
     c->gen++;
   };
-  const int ms_opts[2] = {8, 16};
-  const int skin_opts[4] = {0, 300, 600, 1000};
+  const int ms_opts[2] = {8, 4};
+  const int skin_opts[4] = {0, 400, 800, 1200};
   double best = 1e300;
   int best_m = 8, best_skin = 600;
   for (int mi = 0; mi < 2; mi++)
@@ -2533,7 +2612,7 @@ int pm_last_stats(pm_context* c, long long* stats, int nstats) {
   int r;
   if ((r = set_device(c))) return r;
   if ((r = list_stats(c))) return r;
-  for (int k = 0; k < nstats && k < 5; k++) stats[k] = c->last_stats[k];
+  for (int k = 0; k < nstats && k < 6; k++) stats[k] = c->last_stats[k];
   return PM_OK;
 }
 
@@ -2582,7 +2661,7 @@ static int adjoint_check(pm_context* c, const int* handles, int nh) {
   for (int k = 0; k < nh; k++)
     if (c->forces[handles[k]].bonded)
       return fail(c, PM_ERR_UNSUPPORTED, "Hessian-vector products and parameter gradients cover the nonbonded cutoff and PME forces, not '%s'", c->forces[handles[k]].id.c_str());
-  if (choose_m(c) != 8) return fail(c, PM_ERR_UNSUPPORTED, "the adjoint kernels use clusters of 8 atoms (PM_OPT_CLUSTER_SIZE 16 is set)");
+  if (choose_m(c) != 8) return fail(c, PM_ERR_UNSUPPORTED, "the adjoint kernels use clusters of 8 atoms (PM_OPT_CLUSTER_SIZE 4 is set)");
   if ((r = ensure_cells(c))) return r;
   return ensure_adjoint(c);
 }
@@ -2823,8 +2902,8 @@ int pm_set_option(pm_context* c, int option, int value) {
   if (!c) return PM_ERR_ARGUMENT;
   c->gen++;
   if (option == PM_OPT_CLUSTER_SIZE) {
-    if (value != 0 && value != 8 && value != 16)
-      return fail(c, PM_ERR_ARGUMENT, "cluster size must be 0 (automatic), 8 or 16 (got %d)", value);
+    if (value != 0 && value != 4 && value != 8)
+      return fail(c, PM_ERR_ARGUMENT, "cluster size must be 0 (automatic), 4 or 8 (got %d)", value);
     c->m_override = value;
     c->md_forces_valid = false;
     c->gen++;
@@ -2841,6 +2920,21 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->skin_milli = value;
     c->list_stale = true;
     c->md_forces_valid = false;
+    return PM_OK;
+  }
+  if (option == PM_OPT_PRUNE_SKIN) {
+    if (value < -1 || value > 3000)
+      return fail(c, PM_ERR_ARGUMENT, "inner-list skin must be -1 (automatic), 0 (off) or 1..3000 (1e-3 A; got %d)", value);
+    c->prune_milli = value;
+    c->list_stale = true;
+    c->md_forces_valid = false;
+    return PM_OK;
+  }
+  if (option == PM_OPT_PME_PRIORITY) {
+    if (value < -1 || value > 3)
+      return fail(c, PM_ERR_ARGUMENT, "PME priority must be -1 (automatic) or 0..3 (got %d)", value);
+    c->pme_prio_option = value;
+    c->gen++;   // captured steps hold the launch arguments
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {
@@ -2865,6 +2959,14 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_LIST_SKIN) {
     *value = (int)std::lround(list_skin(c) * 1000.0);
+    return PM_OK;
+  }
+  if (option == PM_OPT_PME_PRIORITY) {
+    *value = pme_priority(c);
+    return PM_OK;
+  }
+  if (option == PM_OPT_PRUNE_SKIN) {
+    *value = (int)std::lround(prune_skin(c) * 1000.0);
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {
@@ -3002,7 +3104,7 @@ int batch_make_plan(pm_batch* b, bool energy, int graph_steps, BatchPlan& P) {
         return PM_ERR_UNSUPPORTED;
       }
       if (y.kind == PM_REC_PAIR && ((y.variant >> 8) & 1)) {
-        b->why_chains = "pair kernels of 16-atom clusters have no batched form";
+        b->why_chains = "pair kernels of 4-atom clusters have no batched form";
         return PM_ERR_UNSUPPORTED;
       }
     }

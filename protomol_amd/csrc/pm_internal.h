@@ -151,6 +151,17 @@ struct PmNbLaunch {
   double* epart;             // [ncl_max * PM_NE]
   int want_virial;
   int want_energy;           // write the per-cluster energy records
+  // inner (pruned) lists, with list reuse: rebuild[1] is the prune flag, set
+  // when an atom has moved more than half the inner skin since the last
+  // prune.  On an evaluation with either flag set k_prune writes, per
+  // cluster, the entries of the full list with some i atom within
+  // sqrt(cut2_prune) (the pair cutoff + the inner skin) into ilist / imask /
+  // icount; the pair kernel walks only those.
+  int inner;
+  float cut2_prune;
+  int* ilist;
+  unsigned int* imask;
+  int* icount;
 };
 
 // ---------------------------------------------------------------- launchers
@@ -169,26 +180,34 @@ void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_c
                           int* aux);
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
                        int* cell_atoms, const int* rebuild);
+// x_prune (may be null): the positions of the last inner-list prune, written
+// on builds and, on kept layouts, when the prune flag rebuild[1] is set
 void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const double* pos,
                           const double* q, const int* type, const int* cell_count,
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
                           int* atom_of_slot, int dummy_type, int dummy_slot, int m, const int* rebuild,
-                          int4* x_build);
-// list reuse: *flag = 1 when an atom moved more than sqrt(lim2) A from its
-// position at the last build (x_build, 32-bit fixed point, w = 1 when valid)
+                          int4* x_build, int4* x_prune);
+// list reuse: flag[0] = 1 when an atom moved more than sqrt(lim2) A from its
+// position at the last build (x_build, 32-bit fixed point, w = 1 when valid);
+// with x_prune, flag[1] = 1 when one moved more than sqrt(lim2_in) A from its
+// position at the last prune of the inner lists
 void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* pos, const int4* x_build,
-                          float lim2, int* flag);
+                          float lim2, int* flag, const int4* x_prune = nullptr, float lim2_in = 0.f);
 void pm_launch_set_flag(hipStream_t s, int* flag, int v);
 void pm_launch_invalidate_build(hipStream_t s, int n, int4* x_build);
 // force_fix[atom] += force_slot[slot] for every real slot (then the slot
 // cleared when `clear`); with `flag` (list reuse) also the end of the
-// evaluation: builds[0] += (*flag != 0), *flag = 0
+// evaluation: builds[0] += (flag[0] != 0), builds[1] += (only flag[1] != 0:
+// an inner-list prune without a build), flag[0] = flag[1] = 0
 void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const int* atom_of_slot,
                           long long* force_slot, long long* force_fix, int* flag,
                           unsigned long long* builds, int clear);
+// entries and issued lanes of the lists whose lengths are `jcount`
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
+// inner lists from the full ones (a.inner set; returns at once unless a flag is set)
+void pm_launch_prune(hipStream_t s, const PmBox& box, const PmNbLaunch& a);
 // Adjoint pair kernel (clusters of 8): mode 1 H v, mode 2 (dE/dq, dE/dR, dE/de)
 // per slot into `out` (64-bit fixed point, 3 per slot, accumulated).
 void pm_launch_adjoint(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p, int precision,
@@ -230,6 +249,7 @@ struct PmPmeLaunch {
   double* epart;             // [pm_pme_xpass_blocks() * PM_NE]
   int want_virial;
   int* err;                  // error flag word (PM_FLAG_FORCE_RANGE)
+  int prio;                  // wave priority (s_setprio) of the PME kernels: 0 .. 3
   // spread by cell bricks staged in LDS tiles
   const int* cell_slot;
   const int* cell_count;
@@ -301,7 +321,9 @@ struct PmStepWork {
   double to_fix[3], fix_to_A[3];
   const int4* x_build;
   float lim2;
-  int* flag;
+  int* flag;                 // flag[0]: rebuild; with x_prune, flag[1]: prune the inner lists
+  const int4* x_prune;
+  float lim2_in;
   PmBox box;
   int* cell_of; int* rank; float* key; int* bin_count;
 };
@@ -412,7 +434,7 @@ enum PmRecKind {
   PM_REC_KICK_DRIFT = 1, PM_REC_LANGEVIN_KICK, PM_REC_DRIFT, PM_REC_MD_ADVANCE, PM_REC_ZERO_FORCE_CELLS,
   PM_REC_CHECK_DISP, PM_REC_BIN, PM_REC_SCAN_CELLS, PM_REC_SCATTER, PM_REC_FILL_SLOTS,
   PM_REC_LIST, PM_REC_PAIR, PM_REC_FOLD_SLOTS, PM_REC_BONDED_FORCE, PM_REC_BONDED_ENERGY,
-  PM_REC_REDUCE1, PM_REC_REDUCE2
+  PM_REC_REDUCE1, PM_REC_REDUCE2, PM_REC_PRUNE
 };
 #define PM_REC_ARGMAX 1024
 struct PmRecord {

@@ -219,7 +219,8 @@ __global__ void k_add_fix(int n3, const long long* __restrict__ f, double* __res
 // the next evaluation's displacement test and binning of atom i at its new
 // position x (PmStepWork)
 __device__ inline void step_work(const PmStepWork& w, int i, const double* x) {
-  if (w.flag && pm_moved_past(w.to_fix, w.fix_to_A, x, w.x_build[i], w.lim2)) *w.flag = 1;
+  if (w.flag && pm_moved_past(w.to_fix, w.fix_to_A, x, w.x_build[i], w.lim2)) w.flag[0] = 1;
+  if (w.flag && w.x_prune && pm_moved_past(w.to_fix, w.fix_to_A, x, w.x_prune[i], w.lim2_in)) w.flag[1] = 1;
   if (w.bin_count) pm_bin_atom(w.box, i, x, w.cell_of, w.key, w.rank, w.bin_count);
 }
 

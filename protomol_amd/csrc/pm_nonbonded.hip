@@ -280,14 +280,17 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
                              const float* __restrict__ key, int4* __restrict__ slot_pos,
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
-                             const int* __restrict__ rebuild, int4* __restrict__ x_build, int bx) {
+                             const int* __restrict__ rebuild, int4* __restrict__ x_build,
+                             int4* __restrict__ x_prune, int bx) {
   const int lane = threadIdx.x & 63;
   const int c = bx * 4 + (threadIdx.x >> 6);
   if (rebuild && *rebuild == 0) {
     // layout kept (list reuse): current positions and charges into the
     // cell's slots, with the arithmetic of the build below; padding slots keep
-    // their place (they are masked)
+    // their place (they are masked); when the inner lists are pruned in this
+    // evaluation, also the positions they are pruned at
     if (c >= ncells) return;
+    const bool prune = x_prune && rebuild[1] != 0;
     const int s0 = cell_slot[c], k = cell_count[c];
     for (int u = lane; u < k; u += 64) {
       const int sl = s0 + u;
@@ -299,6 +302,7 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
       p.z = (int)to_fix(pos[3 * ai + 2], box.to_fix[2]);
       p.w = __float_as_int((float)q[ai]);
       slot_pos[sl] = p;
+      if (prune) x_prune[ai] = make_int4(p.x, p.y, p.z, 1);
       if (slot_pos64) {
         longlong4 p8;
         p8.x = (long long)to_fix64(pos[3 * ai], box.to_fix[0]);
@@ -354,6 +358,7 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
     slot_meta[s0 + rank] = type[ai] | (ai << 8);
     atom_of_slot[s0 + rank] = ai;
     if (x_build) x_build[ai] = make_int4(p.x, p.y, p.z, 1);   // the position the layout was built at
+    if (x_prune) x_prune[ai] = make_int4(p.x, p.y, p.z, 1);   // the inner lists are pruned at it too
   }
   // padding slots: never used as j atoms, masked as i atoms, skipped by PME
   const int ps = (k + m_cl - 1) / m_cl * m_cl;
@@ -385,12 +390,13 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
                              const float* __restrict__ key, int4* __restrict__ slot_pos,
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
-                             const int* __restrict__ rebuild, int4* __restrict__ x_build) { fill_slots_body(box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m_cl, rebuild, x_build, blockIdx.x); }
-struct FillSlotsArgs { PmBox box; int ncells; const double* pos; const double* q; const int* type; const int* cell_count; const int* cell_start; const int* cell_slot; const int* cell_atoms; const float* key; int4* slot_pos; longlong4* slot_pos64; int* slot_meta; int* atom_of_slot; int dummy_type; int dummy_slot; int m_cl; const int* rebuild; int4* x_build; };
+                             const int* __restrict__ rebuild, int4* __restrict__ x_build,
+                             int4* __restrict__ x_prune) { fill_slots_body(box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m_cl, rebuild, x_build, x_prune, blockIdx.x); }
+struct FillSlotsArgs { PmBox box; int ncells; const double* pos; const double* q; const int* type; const int* cell_count; const int* cell_start; const int* cell_slot; const int* cell_atoms; const float* key; int4* slot_pos; longlong4* slot_pos64; int* slot_meta; int* atom_of_slot; int dummy_type; int dummy_slot; int m_cl; const int* rebuild; int4* x_build; int4* x_prune; };
 __global__ __launch_bounds__(256) void kb_fill_slots(const FillSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const FillSlotsArgs& q = t[r];
-  fill_slots_body(q.box, q.ncells, q.pos, q.q, q.type, q.cell_count, q.cell_start, q.cell_slot, q.cell_atoms, q.key, q.slot_pos, q.slot_pos64, q.slot_meta, q.atom_of_slot, q.dummy_type, q.dummy_slot, q.m_cl, q.rebuild, q.x_build, blockIdx.x - pre[r]);
+  fill_slots_body(q.box, q.ncells, q.pos, q.q, q.type, q.cell_count, q.cell_start, q.cell_slot, q.cell_atoms, q.key, q.slot_pos, q.slot_pos64, q.slot_meta, q.atom_of_slot, q.dummy_type, q.dummy_slot, q.m_cl, q.rebuild, q.x_build, q.x_prune, blockIdx.x - pre[r]);
 }
 
 
@@ -398,18 +404,22 @@ __global__ __launch_bounds__(256) void kb_fill_slots(const FillSlotsArgs* __rest
 // list skin from where the layout was built (or the layout is invalid: w != 1).
 // Every thread that finds one writes the same value, so no atomics are needed.
 __device__ inline void check_disp_body(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
-                             float lim2, int* __restrict__ flag, int bx) {
+                             float lim2, int* __restrict__ flag, const int4* __restrict__ x_prune, float lim2_in,
+                             int bx) {
   const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (pm_moved_past(box.to_fix, box.fix_to_A, pos + 3 * i, x_build[i], lim2)) *flag = 1;
+  if (pm_moved_past(box.to_fix, box.fix_to_A, pos + 3 * i, x_build[i], lim2)) flag[0] = 1;
+  if (x_prune && pm_moved_past(box.to_fix, box.fix_to_A, pos + 3 * i, x_prune[i], lim2_in)) flag[1] = 1;
 }
 __global__ void k_check_disp(PmBox box, int n, const double* __restrict__ pos, const int4* __restrict__ x_build,
-                             float lim2, int* __restrict__ flag) { check_disp_body(box, n, pos, x_build, lim2, flag, blockIdx.x); }
-struct CheckDispArgs { PmBox box; int n; const double* pos; const int4* x_build; float lim2; int* flag; };
+                             float lim2, int* __restrict__ flag, const int4* __restrict__ x_prune, float lim2_in) {
+  check_disp_body(box, n, pos, x_build, lim2, flag, x_prune, lim2_in, blockIdx.x);
+}
+struct CheckDispArgs { PmBox box; int n; const double* pos; const int4* x_build; float lim2; int* flag; const int4* x_prune; float lim2_in; };
 __global__ void kb_check_disp(const CheckDispArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const CheckDispArgs& q = t[r];
-  check_disp_body(q.box, q.n, q.pos, q.x_build, q.lim2, q.flag, blockIdx.x - pre[r]);
+  check_disp_body(q.box, q.n, q.pos, q.x_build, q.lim2, q.flag, q.x_prune, q.lim2_in, blockIdx.x - pre[r]);
 }
 
 
@@ -754,10 +764,12 @@ __device__ inline void fold_slots_body(const int* __restrict__ counts, const int
                              long long* __restrict__ force_slot, long long* __restrict__ force_fix,
                              int* __restrict__ flag, unsigned long long* __restrict__ builds, int clear, int bx) {
   if (flag && bx == 0 && threadIdx.x == 0) {
-    // end of an evaluation that reuses lists (every reader of the flag ran
-    // before): count a build, clear the flag
-    if (*flag) builds[0] += 1ull;
-    *flag = 0;
+    // end of an evaluation that reuses lists (every reader of the flags ran
+    // before): count a build or a prune of the inner lists, clear the flags
+    if (flag[0]) builds[0] += 1ull;
+    else if (flag[1]) builds[1] += 1ull;
+    flag[0] = 0;
+    flag[1] = 0;
   }
   const int sl = bx * blockDim.x + threadIdx.x;
   if (sl >= counts[0]) return;
@@ -786,6 +798,67 @@ __global__ void kb_fold_slots(const FoldSlotsArgs* __restrict__ t, const int* __
   fold_slots_body(q.counts, q.atom_of_slot, q.force_slot, q.force_fix, q.flag, q.builds, q.clear, blockIdx.x - pre[r]);
 }
 
+
+// ----------------------------------------------------- inner-list pruning
+// One wave per i-cluster, on evaluations that build the lists or find an
+// atom moved past half the inner skin (either flag; otherwise it returns at
+// once): the entries of the full list of which some lane is a pair the pair
+// kernel keeps (real i atom, j slot after the i slot) within the pair cutoff +
+// the inner skin at the current positions, in list order, with their mask
+// words, into the inner list.  The pair kernel then walks only these until
+// the next prune: a pair within the cutoff later was within cutoff + inner
+// skin here while every atom stays within half the inner skin of its position
+// now.  Distances from the 32-bit fixed-point slots in both precisions (the
+// cutoff is padded as the list cutoff is).
+template <int M>
+__device__ inline void prune_body(const PmBox& box, const PmNbLaunch& a, int bx) {
+  if (a.rebuild[0] == 0 && a.rebuild[1] == 0) return;
+  constexpr int JW = 64 / M;
+  const int lane = threadIdx.x & 63;
+  const int cl = bx * 4 + (threadIdx.x >> 6);
+  if (cl >= a.counts[1]) return;
+  const int ii = lane % M, js = lane / M;
+  const int si = cl * M + ii;
+  const bool ireal = ((a.slot_meta[si] >> 8) & 0x7fffff) != 0x7fffff;
+  const int4 pi = a.slot_pos[si];
+  const float sx = (float)box.fix_to_A[0], sy = (float)box.fix_to_A[1], sz = (float)box.fix_to_A[2];
+  const long long lbase = (long long)cl * a.lcap;
+  const int* __restrict__ lst = a.jlist + lbase;
+  const int cnt = a.jcount[cl];
+  int icnt = 0;   // wave-uniform
+  for (int base = 0; base < cnt; base += JW) {
+    const int k = base + js;
+    const bool valid = k < cnt;
+    const int e = valid ? lst[k] : a.dummy_slot;
+    const int sj = e & 0x7fffffff;
+    const int4 pj = a.slot_pos[sj];
+    const float dx = (float)(int)(pj.x - pi.x) * sx;
+    const float dy = (float)(int)(pj.y - pi.y) * sy;
+    const float dz = (float)(int)(pj.z - pi.z) * sz;
+    const bool kp = valid && ireal && sj > si && dx * dx + dy * dy + dz * dz <= a.cut2_prune;
+    unsigned long long t = __ballot(kp);
+    // one bit per row of M lanes, at the row's first lane (shifts of at most
+    // M - 1 in total never carry a bit into the next row's first lane)
+    if constexpr (M == 8) { t |= t >> 4; t |= t >> 2; t |= t >> 1; t &= 0x0101010101010101ull; }
+    else { t |= t >> 2; t |= t >> 1; t &= 0x1111111111111111ull; }
+    const int rowbit = js * M;
+    if (ii == 0 && ((t >> rowbit) & 1ull)) {
+      const int pos = icnt + __popcll(t & ((1ull << rowbit) - 1ull));
+      a.ilist[lbase + pos] = e;
+      if (e < 0) a.imask[lbase + pos] = a.jmask[lbase + k];
+    }
+    icnt += __popcll(t);
+  }
+  if (lane == 0) a.icount[cl] = icnt;
+}
+template <int M>
+__global__ __launch_bounds__(256) void k_prune(PmBox box, PmNbLaunch a) { prune_body<M>(box, a, blockIdx.x); }
+struct PruneArgs { PmBox box; PmNbLaunch a; };
+template <int M>
+__global__ __launch_bounds__(256) void kb_prune(const PruneArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
+  const int r = pm_batch_replica(pre, nrep, blockIdx.x);
+  prune_body<M>(t[r].box, t[r].a, blockIdx.x - pre[r]);
+}
 
 // ------------------------------------------------------------ pair kernel
 template <typename T> struct Prec;
@@ -872,7 +945,7 @@ __device__ inline void apply_switch(T r2, T dist, T cut2, T on2, T sw1, T sw2, T
 
 
 // Sum over the M lanes of a row (lanes M*r .. M*r+M-1) with DPP moves: quad
-// xor 1, quad xor 2, half-row mirror (and row mirror for M = 16).  Every lane
+// xor 1, quad xor 2 (M = 4), then half-row mirror (M = 8).  Every lane
 // of the row ends with the same value, added in the same order on every run.
 template <int CTRL>
 __device__ inline float dpp_f(float v) {
@@ -890,8 +963,7 @@ template <int M, typename T>
 __device__ inline T row_sum(T v) {
   v += dpp_f<0xB1>(v);    // quad_perm [1,0,3,2]
   v += dpp_f<0x4E>(v);    // quad_perm [2,3,0,1]
-  v += dpp_f<0x141>(v);   // row_half_mirror: sums of 8 lanes
-  if constexpr (M == 16) v += dpp_f<0x140>(v);   // row_mirror: sums of 16 lanes
+  if constexpr (M >= 8) v += dpp_f<0x141>(v);   // row_half_mirror: sums of 8 lanes
   return v;
 }
 
@@ -995,8 +1067,12 @@ __device__ inline void pair_cluster(const PmBox& box, const PmNbLaunch& a, const
   T fxs = 0, fys = 0, fzs = 0;            // partial sums since the last flush
   T elj = 0, ec = 0;
   T vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
-  const int* __restrict__ lst = a.jlist + (long long)cl * a.lcap;
-  const int cnt = a.jcount[cl];
+  // the inner (pruned) lists when kept (k_prune brought them up to date
+  // before this kernel), else the full lists
+  const long long lbase = (long long)cl * a.lcap;
+  const int* __restrict__ lst = (a.inner ? a.ilist : a.jlist) + lbase;
+  const unsigned int* __restrict__ msk = (a.inner ? a.imask : a.jmask) + lbase;
+  const int cnt = a.inner ? a.icount[cl] : a.jcount[cl];
   const int last = cnt > 0 ? cnt - 1 : 0;
   const bool vir = WE && a.want_virial != 0;
   long long* __restrict__ ff = a.force_slot;
@@ -1037,7 +1113,7 @@ __device__ inline void pair_cluster(const PmBox& box, const PmNbLaunch& a, const
     bool modified = false, any_mod = false;
     if (__ballot(flagged) != 0ull) {
       if (flagged && mask) {
-        const int xm = a.jmask[(long long)cl * a.lcap + k];
+        const int xm = (int)msk[k];
         if ((xm >> ii) & 1) mask = false;
         else modified = ((xm >> (16 + ii)) & 1) != 0;
       }
@@ -1153,12 +1229,24 @@ __device__ inline void pair_cluster(const PmBox& box, const PmNbLaunch& a, const
     eB = lst[min(kB + 2 * JW, last)];
     int sB;
     const T vB = row(kB, eB_now, pB, mB, sB);
-    // lanes ii = 0, 1, 2 add the x, y, z component of row A, lanes 3, 4, 5 those of row B
-    const T v = ii < 3 ? vA : vB;
-    const int sl = ii < 3 ? sA : sB;
-    if (ii < 6 && v != T(0)) {
-      range_bad |= !(__builtin_fabs(v) < T(PM_FORCE_LIMIT));
-      atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
+    if constexpr (M >= 8) {
+      // lanes ii = 0, 1, 2 add the x, y, z component of row A, lanes 3, 4, 5 those of row B
+      const T v = ii < 3 ? vA : vB;
+      const int sl = ii < 3 ? sA : sB;
+      if (ii < 6 && v != T(0)) {
+        range_bad |= !(__builtin_fabs(v) < T(PM_FORCE_LIMIT));
+        atomicAdd((unsigned long long*)(ff + 3LL * sl + comp), (unsigned long long)force_fix_of(v));
+      }
+    } else {
+      // rows of four lanes: lanes ii = 0, 1, 2 add row A's components, then row B's
+      if (ii < 3 && vA != T(0)) {
+        range_bad |= !(__builtin_fabs(vA) < T(PM_FORCE_LIMIT));
+        atomicAdd((unsigned long long*)(ff + 3LL * sA + comp), (unsigned long long)force_fix_of(vA));
+      }
+      if (ii < 3 && vB != T(0)) {
+        range_bad |= !(__builtin_fabs(vB) < T(PM_FORCE_LIMIT));
+        atomicAdd((unsigned long long*)(ff + 3LL * sB + comp), (unsigned long long)force_fix_of(vB));
+      }
     }
     if (sizeof(T) == 4 && ++trip == PM_IFLUSH) {
       trip = 0;
@@ -1575,7 +1663,7 @@ __global__ void k_diag(PmBox box, const double* __restrict__ pos, const int* __r
 // pair-kernel variant of a record: precision, potentials and switching,
 // energy flag and cluster size
 static int pair_variant(bool dbl, int lj, int ck, int cs, bool we, int m) {
-  return (dbl ? 1 : 0) | (lj << 1) | (ck << 3) | (cs << 5) | ((we ? 1 : 0) << 7) | ((m == 16 ? 1 : 0) << 8);
+  return (dbl ? 1 : 0) | (lj << 1) | (ck << 3) | (cs << 5) | ((we ? 1 : 0) << 7) | ((m == 4 ? 1 : 0) << 8);
 }
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
                    int* cell_count, int* rank, float* key, const int* rebuild) {
@@ -1625,28 +1713,29 @@ void pm_launch_fill_slots(hipStream_t s, const PmBox& box, int ncells, const dou
                           const int* cell_start, const int* cell_slot, int* cell_atoms,
                           const float* key, int4* slot_pos, longlong4* slot_pos64, int* slot_meta,
                           int* atom_of_slot, int dummy_type, int dummy_slot, int m, const int* rebuild,
-                          int4* x_build) {
+                          int4* x_build, int4* x_prune) {
   int nb = (ncells + 3) / 4;
   if (pm_recording()) {
     const FillSlotsArgs a{box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos,
-                          slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build};
+                          slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build, x_prune};
     pm_record(PM_REC_FILL_SLOTS, 0, nb > 0 ? nb : 1, 256, 0, &a, sizeof(a));
     return;
   }
   hipLaunchKernelGGL(k_fill_slots, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, box, ncells, pos, q, type,
                      cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta,
-                     atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build);
+                     atom_of_slot, dummy_type, dummy_slot, m, rebuild, x_build, x_prune);
 }
 
 void pm_launch_check_disp(hipStream_t s, const PmBox& box, int n, const double* pos, const int4* x_build,
-                          float lim2, int* flag) {
+                          float lim2, int* flag, const int4* x_prune, float lim2_in) {
   if (pm_recording()) {
-    const CheckDispArgs a{box, n, pos, x_build, lim2, flag};
+    const CheckDispArgs a{box, n, pos, x_build, lim2, flag, x_prune, lim2_in};
     pm_record(PM_REC_CHECK_DISP, 0, n > 0 ? (n + 255) / 256 : 0, 256, 0, &a, sizeof(a));
     return;
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_check_disp, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, x_build, lim2, flag);
+    hipLaunchKernelGGL(k_check_disp, dim3((n + 255) / 256), dim3(256), 0, s, box, n, pos, x_build, lim2, flag,
+                       x_prune, lim2_in);
 }
 
 void pm_launch_set_flag(hipStream_t s, int* flag, int v) {
@@ -1672,6 +1761,17 @@ void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const
                        force_slot, force_fix, flag, builds, clear);
 }
 
+void pm_launch_prune(hipStream_t s, const PmBox& box, const PmNbLaunch& a) {
+  const dim3 g((a.ncl_max + 3) / 4), b(256);
+  if (pm_recording()) {
+    const PruneArgs q{box, a};
+    pm_record(PM_REC_PRUNE, a.m, g.x, 256, 0, &q, sizeof(q));
+    return;
+  }
+  if (a.m == 4) hipLaunchKernelGGL(k_prune<4>, g, b, 0, s, box, a);
+  else hipLaunchKernelGGL(k_prune<8>, g, b, 0, s, box, a);
+}
+
 void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m) {
   hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out, 64 / m);
 }
@@ -1685,8 +1785,8 @@ void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const 
     pm_record(PM_REC_LIST, a.m, (a.ncl_max + 3) / 4, 256, (unsigned)lds, &q, sizeof(q));
     return;
   }
-  if (a.m == 16)
-    hipLaunchKernelGGL(k_list<16>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
+  if (a.m == 4)
+    hipLaunchKernelGGL(k_list<4>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
   else
     hipLaunchKernelGGL(k_list<8>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
 }
@@ -1714,9 +1814,9 @@ static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, 
       return;
     }
   }
-  if (a.m == 16) {
-    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 16, false>), g, b, lds, s, box, a, p);
-    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 16, false>), g, b, lds, s, box, a, p);
+  if (a.m == 4) {
+    if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 4, false>), g, b, lds, s, box, a, p);
+    else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 4, false>), g, b, lds, s, box, a, p);
   } else {
     if (a.want_energy) hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, true, 8, false>), g, b, lds, s, box, a, p);
     else hipLaunchKernelGGL((k_pair<T, LJ, CK, CS, false, 8, false>), g, b, lds, s, box, a, p);
@@ -1812,7 +1912,7 @@ static bool launch_pair_batched(hipStream_t s, int variant, const PairArgs* t, c
                                 unsigned lds) {
   const int lj = (variant >> 1) & 3, ck = (variant >> 3) & 3, cs = (variant >> 5) & 3;
   const bool we = (variant >> 7) & 1;
-  if ((variant >> 8) & 1) return false;   // 16-atom clusters
+  if ((variant >> 8) & 1) return false;   // 4-atom clusters
 #define PM_CASE(L, C, S) \
   if (lj == L && ck == C && (C == PM_COUL_NONE || cs == S)) { \
     if (we) hipLaunchKernelGGL((kb_pair<T, L, C, S, true, 8>), g, dim3(256), lds, s, t, pre, nrep); \
@@ -1847,12 +1947,16 @@ bool pm_launch_batched_nb(hipStream_t s, int kind, int variant, const void* t, c
     case PM_REC_CHECK_DISP: hipLaunchKernelGGL(kb_check_disp, g, b, 0, s, (const CheckDispArgs*)t, pre, nrep); return true;
     case PM_REC_FOLD_SLOTS: hipLaunchKernelGGL(kb_fold_slots, g, b, 0, s, (const FoldSlotsArgs*)t, pre, nrep); return true;
     case PM_REC_LIST:
-      if (variant == 16) hipLaunchKernelGGL(kb_list<16>, g, b, lds, s, (const ListArgs*)t, pre, nrep);
+      if (variant == 4) hipLaunchKernelGGL(kb_list<4>, g, b, lds, s, (const ListArgs*)t, pre, nrep);
       else hipLaunchKernelGGL(kb_list<8>, g, b, lds, s, (const ListArgs*)t, pre, nrep);
       return true;
     case PM_REC_PAIR:
       if (variant & 1) return launch_pair_batched<double>(s, variant, (const PairArgs*)t, pre, nrep, g, lds);
       return launch_pair_batched<float>(s, variant, (const PairArgs*)t, pre, nrep, g, lds);
+    case PM_REC_PRUNE:
+      if (variant == 4) hipLaunchKernelGGL(kb_prune<4>, g, b, 0, s, (const PruneArgs*)t, pre, nrep);
+      else hipLaunchKernelGGL(kb_prune<8>, g, b, 0, s, (const PruneArgs*)t, pre, nrep);
+      return true;
     default: return false;
   }
 }

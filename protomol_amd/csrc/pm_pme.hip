@@ -173,6 +173,19 @@ __device__ inline void load_twiddles(T2* dst, const PmFftPlan& p, const double* 
   }
 }
 
+// Issue priority of this wave over other kernels' waves on its CU (the pair
+// kernel runs beside the PME chain on another stream and would otherwise take
+// an equal share of every issue cycle): a wave-level setting, the streams
+// themselves keep the default priority.
+__device__ inline void pme_priority(int prio) {
+  switch (prio) {
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    default: break;
+  }
+}
+
 // ---------------------------------------------------------- B-spline (P)
 // BSpline::set (base/BSpline.cpp:64-117) for an even order P >= 4.
 template <typename T, int P>
@@ -270,6 +283,7 @@ __global__ __launch_bounds__(256) void k_spread(PmPmeLaunch a, const longlong4* 
 // grid directly.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_spread_tile(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
+  pme_priority(a.prio);
   extern __shared__ unsigned long long s_tile[];
   __shared__ int s_cs[64], s_cc[64], s_pre[65];
   const int bz = blockIdx.x % a.nbrick[2];
@@ -374,6 +388,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(PmPmeLaunch a, const longlo
 // TAIL: the same thread also adds its atom's correction and bonded forces
 template <typename T, int P, bool TAIL>
 __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* __restrict__ pos64, PmAtomTail tl) {
+  pme_priority(a.prio);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.counts[0]) return;
   const int at = a.atom_of_slot[s];
@@ -614,6 +629,7 @@ __global__ __launch_bounds__(256) void k_gather_adj(PmPmeLaunch a, const longlon
 // fixed-point grid for the next spread.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, int TL) {
+  pme_priority(a.prio);
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
   const int n = p.n, ls = n + 1, nl_tot = a.nx * a.ny, nzc = a.nzc;
@@ -723,6 +739,7 @@ __device__ inline void store_block_energy(const PmPmeLaunch& a, const PmERed& s,
 // consecutive kz per block
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int TL, int sign) {
+  pme_priority(a.prio);
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
   T2* b0 = reinterpret_cast<T2*>(s_raw);
@@ -752,6 +769,7 @@ __global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int T
 // transform, multiply by theta(m), inverse transform.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, int TL) {
+  pme_priority(a.prio);
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
   __shared__ double s_red[16][8];
@@ -793,6 +811,7 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
 // by consecutive threads on consecutive lines without bank conflicts).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_fft_plane(PmPmeLaunch a, PmFftPlan px, PmFftPlan py) {
+  pme_priority(a.prio);
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
   __shared__ double s_red[16][8];
@@ -828,6 +847,7 @@ __global__ __launch_bounds__(1024) void k_fft_plane(PmPmeLaunch a, PmFftPlan px,
 // z inverse: complex half lines -> full Hermitian lines -> real grid
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_z_inv(PmPmeLaunch a, PmFftPlan p, int TL) {
+  pme_priority(a.prio);
   typedef typename Cx<T>::t T2;
   extern __shared__ unsigned char s_raw[];
   const int n = p.n, ls = n + 1, nl_tot = a.nx * a.ny, nzc = a.nzc;

@@ -33,7 +33,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_set_charges", "pm_lanczos", "pm_md_autotune", "pm_batch_set_mode", "pm_batch_info",
            "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle",
            "pm_batch_set_exchange_velocities", "pm_shake_noalloc"]
-OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3}
+OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3, "pme_priority": 4, "prune_skin": 5}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 
@@ -286,10 +286,10 @@ class Context:
         return v, k.value
 
     def stats(self):
-        s = np.zeros(5, np.int64)
-        self._check(self.lib.pm_last_stats(self.h, _lp(s), 5))
+        s = np.zeros(6, np.int64)
+        self._check(self.lib.pm_last_stats(self.h, _lp(s), 6))
         return dict(list_entries=int(s[0]), lanes=int(s[1]), clusters=int(s[2]), slots=int(s[3]),
-                    list_builds=int(s[4]))
+                    list_builds=int(s[4]), list_prunes=int(s[5]))
 
     def set_timing(self, categories=()):
         mask = 0

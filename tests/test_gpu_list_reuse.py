@@ -87,3 +87,78 @@ def test_md_with_reuse_follows_rebuild_every_step(gpu_ctx_factory, integrator):
     dx = np.abs(out[600][0] - out[0][0]).max()
     say("max |dx|", dx, "E", out[0][2][:2], out[600][2][:2])
     assert dx < 1e-8
+
+
+def make_inner(factory, s, precision, skin, prune):
+    c, hs = make(factory, s, precision, skin)
+    c.set_option("prune_skin", prune)
+    return c, hs
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("mixed", 2e-6)])
+def test_inner_lists_match_fresh_lists(gpu_ctx_factory, precision, tol):
+    """Inner lists (PM_OPT_PRUNE_SKIN 0.2 A under a 1.0 A list skin): moves
+    below half the inner skin reuse the pruned lists, a move past it prunes
+    again without a build, a move past half the list skin builds; the forces
+    equal a context that rebuilds at every evaluation throughout, and the
+    pruned lists issue fewer lanes than the full ones."""
+    say(f"case: inner lists water8k {precision}")
+    s = water.named("water8k", seed=5)
+    c, hs = make_inner(gpu_ctx_factory, s, precision, 1000, 200)
+    full, fhs = make(gpu_ctx_factory, s, precision, 1000)
+    ref, rhs = make(gpu_ctx_factory, s, precision, 0)
+    assert c.get_option("prune_skin") == 200
+    rng = np.random.default_rng(6)
+    x = s.pos.copy()
+
+    def check(xx, what):
+        f, e = c.evaluate(hs, xx)
+        fr, er = ref.evaluate(rhs, xx)
+        ff, _ = full.evaluate(fhs, xx)
+        st, sf = c.stats(), full.stats()
+        say(what, "rms", rms_rel(f, fr), "builds", st["list_builds"], "prunes", st["list_prunes"],
+            "lanes", st["lanes"], "full lists", sf["lanes"])
+        assert rms_rel(f, fr) <= tol
+        assert rms_rel(ff, fr) <= tol
+        assert abs(e[0] - er[0]) <= tol * abs(er[0]) and abs(e[1] - er[1]) <= tol * max(abs(er[1]), 1.0)
+        assert st["lanes"] < sf["lanes"]
+        return st
+
+    st0 = check(x, "start")
+    for k in range(3):
+        x = x + 0.03 * rng.uniform(-1, 1, x.shape) / np.sqrt(3)   # 0.09 A in total, below 0.1 A
+        st = check(x, f"small move {k}")
+        assert st["list_builds"] == st0["list_builds"] and st["list_prunes"] == st0["list_prunes"]
+    x[11] += np.array([0.0, 0.15, 0.0])                           # past 0.1 A, below 0.5 A
+    st = check(x, "0.15 A move")
+    assert st["list_builds"] == st0["list_builds"] and st["list_prunes"] == st0["list_prunes"] + 1
+    x[13] += np.array([0.0, 0.0, 0.6])                            # past 0.5 A
+    st = check(x, "0.6 A move")
+    assert st["list_builds"] == st0["list_builds"] + 1
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_md_with_inner_lists_follows_rebuild_every_step(gpu_ctx_factory, precision):
+    """Leapfrog, 60 steps of the 8,100-atom box with PME, inner lists (0.3 A
+    under a 1.0 A list skin) against lists rebuilt at every step: the same
+    trajectory (FP64: to accumulation order; mixed: within the FP32 rounding
+    of pair sums grouped differently), with prunes and builds both taking
+    place."""
+    say(f"case: md 60 steps inner lists vs rebuild every step, {precision}")
+    s = water.named("water8k", seed=3)
+    out = {}
+    for skin, prune in ((0, 0), (1000, 300)):
+        c, hs = make_inner(gpu_ctx_factory, s, precision, skin, prune)
+        hs = hs + [c.force("Bond", []), c.force("Angle", [])]
+        c.md_setup(hs, "leapfrog", dt=1.0)
+        c.md_set_state(s.pos, s.vel)
+        st0 = c.stats()
+        c.md_run(60)
+        x, v, e = c.md_get_state()
+        st = c.stats()
+        out[skin] = (x, v, e, st["list_builds"] - st0["list_builds"], st["list_prunes"] - st0["list_prunes"])
+    say("builds", out[1000][3], "prunes", out[1000][4])
+    assert 0 < out[1000][3] < 30 and out[1000][4] > 0
+    dx = np.abs(out[1000][0] - out[0][0]).max()
+    say("max |dx|", dx, "E", out[0][2][:2], out[1000][2][:2])
+    assert dx < (1e-8 if precision == "double" else 1e-4)

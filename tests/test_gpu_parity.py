@@ -81,7 +81,7 @@ def test_forces_large_grid(gpu_ctx_factory, port, precision, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize("cluster", [8, 16])
+@pytest.mark.parametrize("cluster", [8, 4])
 @pytest.mark.parametrize("precision,tol", [("mixed", 2e-6), ("double", 1e-11)])
 def test_cutoff_c1(gpu_ctx_factory, port, precision, tol, cluster):
     say(f"case: water648 LJ+Coulomb C2 {precision} cluster {cluster}")
@@ -147,7 +147,7 @@ def test_cutoff_variants(gpu_ctx_factory, port, fid, vals, terms, precision, tol
     np.testing.assert_allclose(e[:2], o["energies"][:2], rtol=tol * 10, atol=1e-9)
 
 
-@pytest.mark.parametrize("cluster", [8, 16])
+@pytest.mark.parametrize("cluster", [8, 4])
 @pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-11)])
 def test_exclusion_rules(gpu_ctx_factory, port, precision, tol, cluster):
     """MODIFIED (A14/B14, csf) and NONE overrides, duplicate and cross-molecule
