@@ -309,7 +309,7 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        Steps are captured only while no kernel timing is
  *                        requested (pm_set_timing mask 0).
  *   PM_OPT_LIST_SKIN     pair-list buffer in 1e-3 A: -1 (default) automatic
- *                        (800), 0 off (cells and lists rebuilt at every
+ *                        (1400), 0 off (cells and lists rebuilt at every
  *                        evaluation, as the reference rebuilds its cell list,
  *                        ForceGroup.cpp:50), 1..3000.  Lists then hold the j
  *                        atoms within the pair cutoff + skin and are rebuilt
@@ -340,9 +340,15 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        still computes exactly the pairs within the cutoff.
  *                        -1 (default) automatic, 0 off, 1..3000; used when
  *                        below the list skin.
+ *   PM_OPT_LIST_TEST     the pair-list distance test: -1 (default) automatic
+ *                        (1), 0 the distance from the cluster's bounding box
+ *                        in boxes wider than 2 (cutoff + skin + cell
+ *                        diagonal) and the exact per-atom distance in smaller
+ *                        ones, 1 the exact per-atom distance in every box
+ *                        (fewer entries, a longer build).
  * pm_get_option returns the value in effect (after the automatic choice). */
 enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PME_PRIORITY = 4,
-       PM_OPT_PRUNE_SKIN = 5 };
+       PM_OPT_PRUNE_SKIN = 5, PM_OPT_LIST_TEST = 6 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

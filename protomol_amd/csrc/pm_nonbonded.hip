@@ -487,8 +487,11 @@ __device__ inline int wrap_cell(int c, int d, int n) {
 // cluster's list.  Every unordered pair within the cutoff is thus listed
 // exactly once.  Bit 31 marks an entry that some cluster atom excludes or
 // modifies; its masks (FULL bit ii, MODIFIED bit 16 + ii) go to jmask.
-template <int M>
+// BB: 1 the bounding-box test, 0 the exact per-atom test, 2 the one
+// a.bbox_filter names (batched launches, whose replicas may differ)
+template <int M, int BB>
 __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int bx) {
+  const bool bbox = BB == 2 ? a.bbox_filter != 0 : BB == 1;
   if (a.rebuild && *a.rebuild == 0) return;   // lists kept from the last build
   extern __shared__ int s_dyn[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -546,7 +549,7 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   // rounding of the sums over them, would change from run to run)
   int qx[M], qy[M], qz[M];
   unsigned int qreal = 0;
-  if (active && !a.bbox_filter) {
+  if (active && !bbox) {
 #pragma unroll
     for (int ii = 0; ii < M; ii++) {
       const int4 q = a.slot_pos[cl * M + ii];
@@ -573,7 +576,7 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
       ct = a.cell_count[nb];
       if (nb < cell) ct = 0;
       else if (nb == cell) { ct = st + ct - cl * M; st = cl * M; }
-      if (ct > 0 && a.bbox_filter) {
+      if (ct > 0 && bbox) {
         // gap between the cell's box (padded by 1e-3 A) and the cluster's bounding box
         const unsigned int bx = (unsigned int)(long long)(box.cell0_fix[0] + wx * box.real_fix[0]);
         const unsigned int by = (unsigned int)(long long)(box.cell0_fix[1] + wy * box.real_fix[1]);
@@ -661,7 +664,7 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
       sj = s_start[s] + (k - s_pref[s]);
       const int4 p = a.slot_pos[sj];
       pass = true;
-      if (a.bbox_filter) {
+      if (bbox) {
         const float dx = (float)(int)(p.x - anchor.x) * fx;
         const float dy = (float)(int)(p.y - anchor.y) * fy;
         const float dz = (float)(int)(p.z - anchor.z) * fz;
@@ -719,14 +722,14 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
     a.jcount[cl] = count;
   }
 }
-template <int M>
-__global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) { list_body<M>(box, a, cut2_list, blockIdx.x); }
+template <int M, int BB>
+__global__ __launch_bounds__(256) void k_list(PmBox box, PmNbLaunch a, float cut2_list) { list_body<M, BB>(box, a, cut2_list, blockIdx.x); }
 struct ListArgs { PmBox box; PmNbLaunch a; float cut2_list; };
 template <int M>
 __global__ __launch_bounds__(256) void kb_list(const ListArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const ListArgs& q = t[r];
-  list_body<M>(q.box, q.a, q.cut2_list, blockIdx.x - pre[r]);
+  list_body<M, 2>(q.box, q.a, q.cut2_list, blockIdx.x - pre[r]);
 }
 
 
@@ -1785,10 +1788,14 @@ void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const 
     pm_record(PM_REC_LIST, a.m, (a.ncl_max + 3) / 4, 256, (unsigned)lds, &q, sizeof(q));
     return;
   }
-  if (a.m == 4)
-    hipLaunchKernelGGL(k_list<4>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
-  else
-    hipLaunchKernelGGL(k_list<8>, dim3((a.ncl_max + 3) / 4), dim3(256), lds, s, box, a, c * c);
+  const dim3 g((a.ncl_max + 3) / 4), b(256);
+  if (a.m == 4) {
+    if (a.bbox_filter) hipLaunchKernelGGL((k_list<4, 1>), g, b, lds, s, box, a, c * c);
+    else hipLaunchKernelGGL((k_list<4, 0>), g, b, lds, s, box, a, c * c);
+  } else {
+    if (a.bbox_filter) hipLaunchKernelGGL((k_list<8, 1>), g, b, lds, s, box, a, c * c);
+    else hipLaunchKernelGGL((k_list<8, 0>), g, b, lds, s, box, a, c * c);
+  }
 }
 
 template <typename T, int LJ, int CK, int CS>

@@ -162,3 +162,30 @@ def test_md_with_inner_lists_follows_rebuild_every_step(gpu_ctx_factory, precisi
     dx = np.abs(out[1000][0] - out[0][0]).max()
     say("max |dx|", dx, "E", out[0][2][:2], out[1000][2][:2])
     assert dx < (1e-8 if precision == "double" else 1e-4)
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("mixed", 2e-6)])
+def test_exact_list_test_matches_bounding_box(gpu_ctx_factory, precision, tol):
+    """PM_OPT_LIST_TEST 1 (the exact per-atom distance for the list, in a box
+    where the bounding-box test applies): the same forces as the bounding-box
+    lists and as lists rebuilt at every evaluation, with fewer entries, over
+    moves that keep the lists and one that rebuilds them."""
+    say(f"case: exact list test water8k {precision}")
+    s = water.named("water8k", seed=8)
+    c, hs = make(gpu_ctx_factory, s, precision, 800)
+    c.set_option("list_test", 1)
+    assert c.get_option("list_test") == 1
+    bb, bhs = make(gpu_ctx_factory, s, precision, 800)
+    ref, rhs = make(gpu_ctx_factory, s, precision, 0)
+    rng = np.random.default_rng(9)
+    x = s.pos.copy()
+    for step, amp in enumerate([0.0, 0.1, 0.1, 0.5]):
+        x = x + amp * rng.uniform(-1, 1, x.shape) / np.sqrt(3)
+        f, e = c.evaluate(hs, x)
+        fb, _ = bb.evaluate(bhs, x)
+        fr, er = ref.evaluate(rhs, x)
+        st, sb = c.stats(), bb.stats()
+        say(step, "rms", rms_rel(f, fr), rms_rel(fb, fr), "entries", st["list_entries"], sb["list_entries"])
+        assert rms_rel(f, fr) <= tol and rms_rel(fb, fr) <= tol
+        assert abs(e[0] - er[0]) <= tol * abs(er[0])
+        assert st["list_entries"] < sb["list_entries"]
