@@ -183,6 +183,7 @@ struct pm_context {
   long long* d_force_slot = nullptr;   // [3 * (nslot_max + 1)] pair-kernel forces per slot
   int ncl_max = 0, nslot_max = 0;
   int cells_alloc = 0;
+  int cells_nz = 0;       // cells per column the slot buffers were sized for
   int m = 8;              // atoms per cluster of the current layout
   int cells_m = 0;        // cluster size the slot buffers were sized for
   int m_override = 0;     // 4 or 8 to force the cluster size (0: the default, 8)
@@ -410,19 +411,22 @@ std::vector<int4> delta_list(const pm_context* c, double cutoff) {
   return r;
 }
 
-// full stencil: self, the deltas and their negations (basis form, deduplicated)
-std::vector<int4> full_stencil(const pm_context* c, double cutoff) {
+// column stencil of the pair lists: the distinct (x, y) offsets, in basis
+// form, of the cells within the cutoff of a cell (self, the deltas and their
+// negations): the cell columns that can hold an atom within the cutoff of
+// an atom of the middle column
+std::vector<int4> column_stencil(const pm_context* c, double cutoff) {
   std::vector<int4> h = delta_list(c, cutoff);
-  std::set<std::tuple<int, int, int>> seen;
+  std::set<std::pair<int, int>> seen;
   std::vector<int4> r;
-  auto add = [&](int a, int b, int d) {
-    auto t = std::make_tuple(basis(a, c->box.n[0]), basis(b, c->box.n[1]), basis(d, c->box.n[2]));
+  auto add = [&](int a, int b) {
+    auto t = std::make_pair(basis(a, c->box.n[0]), basis(b, c->box.n[1]));
     if (seen.count(t)) return;
     seen.insert(t);
-    r.push_back(make_int4(std::get<0>(t), std::get<1>(t), std::get<2>(t), 0));
+    r.push_back(make_int4(t.first, t.second, 0, 0));
   };
-  add(0, 0, 0);
-  for (auto& d : h) { add(d.x, d.y, d.z); add(-d.x, -d.y, -d.z); }
+  add(0, 0);
+  for (auto& d : h) { add(d.x, d.y); add(-d.x, -d.y); }
   return r;
 }
 
@@ -430,7 +434,7 @@ int get_stencil(pm_context* c, double cutoff, bool half, int4** dptr, int* count
   auto& m = half ? c->half_stencils : c->stencils;
   auto it = m.find(cutoff);
   if (it == m.end()) {
-    std::vector<int4> v = half ? delta_list(c, cutoff) : full_stencil(c, cutoff);
+    std::vector<int4> v = half ? delta_list(c, cutoff) : column_stencil(c, cutoff);
     int4* d = nullptr;
     int r = dalloc(c, &d, v.size());
     if (r) return r;
@@ -485,13 +489,17 @@ double list_skin(const pm_context* c) {
 
 int ensure_cells(pm_context* c) {
   c->m = choose_m(c);
-  if (c->cells_alloc == c->ncells && c->cells_m == c->m && c->d_cell_count && c->d_slot_pos) return PM_OK;
+  if (c->cells_alloc == c->ncells && c->cells_nz == c->box.n[2] && c->cells_m == c->m && c->d_cell_count &&
+      c->d_slot_pos)
+    return PM_OK;
   int n = c->n, nc = c->ncells;
   int r;
+  if (c->box.n[2] > 8192)
+    return fail(c, PM_ERR_UNSUPPORTED, "%d cells along z (at most 8192): use a larger cellsize", c->box.n[2]);
   if ((r = dalloc(c, &c->d_cell_of, n))) return r;
   if ((r = dalloc(c, &c->d_key, n))) return r;
   if ((r = dalloc(c, &c->d_cell_count, nc))) return r;
-  if ((r = dalloc(c, &c->d_scan_aux, 2 * ((size_t)nc / 8192 + 1)))) return r;
+  if ((r = dalloc(c, &c->d_scan_aux, 2 * ((size_t)nc / 4096 + 2)))) return r;
   if ((r = dalloc(c, &c->d_bin_count, nc))) return r;
   HIPCHECK(c, hipMemset(c->d_bin_count, 0, (size_t)nc * sizeof(int)));
   if ((r = dalloc(c, &c->d_cell_start, nc))) return r;
@@ -500,7 +508,8 @@ int ensure_cells(pm_context* c) {
   if ((r = dalloc(c, &c->d_cell_atoms, n))) return r;
   if ((r = dalloc(c, &c->d_counts, 4))) return r;
   const int m = c->m;
-  long long slots = (long long)n + (long long)(m - 1) * nc;
+  // padding: fewer than m slots after each cell column
+  long long slots = (long long)n + (long long)(m - 1) * (nc / c->box.n[2]);
   c->ncl_max = (int)((slots + m - 1) / m);
   c->nslot_max = c->ncl_max * m;
   if ((r = dalloc(c, &c->d_cluster_cell, c->ncl_max))) return r;
@@ -520,6 +529,7 @@ int ensure_cells(pm_context* c) {
     HIPCHECK(c, hipMemset(c->d_stats, 0, 4 * sizeof(unsigned long long)));
   }
   c->cells_alloc = nc;
+  c->cells_nz = c->box.n[2];
   c->cells_m = m;
   c->jlist_alloc = 0;
   c->ilist_alloc = 0;
@@ -980,7 +990,7 @@ bool inner_lists(const pm_context* c) {
 bool md_step_work(pm_context* c, PmStepWork* w) {
   *w = PmStepWork{};
   if (!c->have_box || !c->d_bin_count || !c->d_cell_of || !c->d_cell_rank || !c->d_key ||
-      c->cells_alloc != c->ncells || c->cells_m != choose_m(c))
+      c->cells_alloc != c->ncells || c->cells_nz != c->box.n[2] || c->cells_m != choose_m(c))
     return false;
   w->box = c->box;
   w->cell_of = c->d_cell_of;
@@ -1074,7 +1084,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
                                passes.empty() ? 0 : 3LL * c->nslot_max, nullptr, 0);
     pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_bin_count, c->d_cell_rank, c->d_key, rb);
   }
-  pm_launch_scan_cells(s, nc, c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
+  pm_launch_scan_cells(s, nc, c->box.n[2], c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
                        c->d_cluster_cell, c->ncl_max, c->m, rb, c->d_scan_aux);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, rb);
   pm_launch_fill_slots(s, c->box, nc, c->d_pos, c->d_q, c->d_type, c->d_cell_count, c->d_cell_start,
@@ -1128,8 +1138,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     int nst;
     const double lskin = reuse ? skin : 0.0;
     if ((r = get_stencil(c, ps.algo_cutoff + lskin, false, &st, &nst))) return r;
-    if (nst > 1300)
-      return fail(c, PM_ERR_UNSUPPORTED, "cell stencil of %d cells (cellsize %g, cutoff %g): use a cellsize of at least cutoff/4", nst, c->cellsize, ps.algo_cutoff);
+    if (nst > 650)
+      return fail(c, PM_ERR_UNSUPPORTED, "cell stencil of %d columns (cellsize %g, cutoff %g): use a cellsize of at least cutoff/4", nst, c->cellsize, ps.algo_cutoff);
     double pcut = std::sqrt(pp.cut2);
     if ((r = ensure_lists(c, list_capacity(c, pcut + lskin)))) return r;
     if (inner && (r = ensure_inner_lists(c))) return r;
@@ -2135,7 +2145,7 @@ int pm_cell_diagnostics(pm_context* c, const double* positions, double algorithm
   c->list_stale = true;   // the binning below replaces the layout's cell lists
   pm_launch_zero_i(s, c->d_bin_count, nc);
   pm_launch_bin(s, c->box, n, c->d_pos, c->d_cell_of, c->d_bin_count, c->d_cell_rank, c->d_key, nullptr);
-  pm_launch_scan_cells(s, nc, c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
+  pm_launch_scan_cells(s, nc, c->box.n[2], c->d_bin_count, c->d_cell_count, c->d_cell_start, c->d_cell_slot, c->d_counts,
                        c->d_cluster_cell, c->ncl_max, c->m, nullptr, c->d_scan_aux);
   pm_launch_scatter(s, n, c->d_cell_of, c->d_cell_rank, c->d_cell_start, c->d_cell_atoms, nullptr);
   std::vector<int> cell(n), cnt(nc), start(nc), atoms(n);

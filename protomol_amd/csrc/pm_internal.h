@@ -118,7 +118,7 @@ struct PmNbLaunch {
   // cells
   const int* cell_slot;
   const int* cell_count;
-  const int4* stencil;
+  const int4* stencil;       // cell columns (x, y offsets) of the pair lists
   int nstencil;
   // lists
   int* jlist;
@@ -132,7 +132,7 @@ struct PmNbLaunch {
   const int* xs_part;
   int xs_tcap;               // LDS entries per cluster: M * largest partner count
   unsigned int* jmask;       // per list entry (flagged entries): FULL bit ii, MODIFIED bit 16 + ii
-  int m;                     // atoms per cluster (8 or 16)
+  int m;                     // atoms per cluster (4 or 8)
   // topology
   const int* molecule;
   const int* ex_row;
@@ -173,9 +173,13 @@ struct PmNbLaunch {
 // bin_count (also on steps that keep the layout).
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
                    int* bin_count, int* rank, float* key, const int* rebuild);
-// aux: 2 ints per 8192 cells; with more than two such chunks (and not
-// recording) the scan runs over many blocks, otherwise in one
-void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_count, int* cell_start,
+// Slots: the atoms of a cell column (the nz cells that share x and y) in
+// consecutive slots, cell after cell, the column padded to a multiple of m
+// after its last cell (nz <= 8192).  aux: 2 ints per chunk of whole columns
+// of at most 8192 cells (at most 2 (ncells / 4096 + 2) in all); with more
+// than two chunks (and not recording) the scan runs over many blocks,
+// otherwise in one
+void pm_launch_scan_cells(hipStream_t s, int ncells, int nz, int* bin_count, int* cell_count, int* cell_start,
                           int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild,
                           int* aux);
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,

@@ -33,25 +33,103 @@ __global__ void kb_bin(const BinArgs* __restrict__ t, const int* __restrict__ pr
 }
 
 
-// Exclusive scans of the cell counts (atoms and M-padded slots) in one
-// workgroup, 8192 cells per pass staged in LDS; also assigns every cluster
-// its cell.
-__device__ inline void scan_cells_body(int ncells, int* __restrict__ cnt, int* __restrict__ cnt_out,
-                                                     int* __restrict__ start, int* __restrict__ slot,
-                                                     int* __restrict__ counts, int* __restrict__ cluster_cell,
-                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild, int bx) {
+// Exclusive scan over a block of 1024 threads (16 waves): each thread's
+// exclusive prefix of v; the block total in `total`.
+__device__ inline int block_excl_scan(int v, int* s_w, int& total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int u = 0; u < 16; u++) {
+    const int x = s_w[u];
+    if (u < w) before += x;
+    tot += x;
+  }
+  __syncthreads();   // s_w is reused by the next scan
+  total = tot;
+  return before + inc - v;
+}
+
+// Slot layout of one chunk of whole cell columns (nz consecutive cells share
+// x and y; the chunk starts at a column's first cell) whose atom counts are
+// in s_c[0, m).  The atoms of a column take consecutive slots, cell after
+// cell, and the column's slots are padded to a multiple of m_cl after its
+// last cell, so clusters of m_cl slots never straddle two columns and only
+// the columns' ends carry padding.  Thread t owns cells 8t .. 8t+7: on
+// return v[k] holds the cell's atom count, ra[k] / rs[k] its chunk-local
+// exclusive atom / slot prefix and ps[k] its slots (atoms plus, at a
+// column's last cell, the column's padding); ta / ts the chunk's totals.
+// s_c is overwritten.
+__device__ inline void scan_chunk(int* s_c, int* s_w, int m, int nz, int m_cl, int (&v)[8], int (&ra)[8],
+                                  int (&ps)[8], int (&rs)[8], int& ta, int& ts) {
+  const int t = threadIdx.x;
+  int sa = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int idx = 8 * t + k;
+    v[k] = idx < m ? s_c[idx] : 0;
+    sa += v[k];
+  }
+  int acc = block_excl_scan(sa, s_w, ta);
+  // the atom prefixes replace the counts (each thread rewrites only its own cells)
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int idx = 8 * t + k;
+    ra[k] = acc;
+    if (idx < m) s_c[idx] = acc;
+    acc += v[k];
+  }
+  __syncthreads();
+  int ss = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int idx = 8 * t + k;
+    int p = v[k];
+    if (idx < m && idx % nz == nz - 1) {
+      const int col = ra[k] + v[k] - s_c[idx - nz + 1];   // the column's atoms
+      p += (m_cl - col % m_cl) % m_cl;
+    }
+    ps[k] = p;
+    ss += p;
+  }
+  acc = block_excl_scan(ss, s_w, ts);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    rs[k] = acc;
+    acc += ps[k];
+  }
+}
+
+// Whole columns per chunk of at most PM_SCAN_CHUNK cells (nz <= PM_SCAN_CHUNK).
+#define PM_SCAN_CHUNK 8192
+__host__ __device__ inline int scan_chunk_cells(int nz) { return PM_SCAN_CHUNK / nz * nz; }
+
+// Cell starts (atoms), cell slots and every cluster's cell, in one workgroup
+// that walks the cell grid a chunk of whole columns at a time, staged in LDS.
+// The cell a cluster belongs to is the cell of its first slot.
+__device__ inline void scan_cells_body(int ncells, int nz, int* __restrict__ cnt, int* __restrict__ cnt_out,
+                                       int* __restrict__ start, int* __restrict__ slot,
+                                       int* __restrict__ counts, int* __restrict__ cluster_cell,
+                                       int ncl_max, int m_cl, const int* __restrict__ rebuild, int bx) {
   if (rebuild && *rebuild == 0) {
     // uniform over the block: before any barrier.  The binning counts start
     // from zero at the next binning (the MD position update bins every step)
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) cnt[c] = 0;
     return;
   }
-  __shared__ int s_c[8192];
-  __shared__ int s_wa[16], s_ws[16];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  __shared__ int s_c[PM_SCAN_CHUNK];
+  __shared__ int s_w[16];
+  const int t = threadIdx.x;
+  const int ch = scan_chunk_cells(nz);
   int carry_a = 0, carry_s = 0;
-  for (int base = 0; base < ncells; base += 8192) {
-    const int m = min(8192, ncells - base);
+  for (int base = 0; base < ncells; base += ch) {
+    const int m = min(ch, ncells - base);
     {
       // eight independent loads in flight per thread (clamped, then masked)
       int tmp[8];
@@ -66,112 +144,82 @@ __device__ inline void scan_cells_body(int ncells, int* __restrict__ cnt, int* _
         }
     }
     __syncthreads();
-    int v[8];
-    int sa = 0, ss = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int idx = 8 * t + k;
-      v[k] = idx < m ? s_c[idx] : 0;
-      sa += v[k];
-      ss += (v[k] + m_cl - 1) / m_cl * m_cl;
-    }
-    // inclusive wave scans, then the 16 wave totals
-    int ia = sa, is = ss;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int ta = __shfl_up(ia, off, 64), ts = __shfl_up(is, off, 64);
-      if (lane >= off) { ia += ta; is += ts; }
-    }
-    if (lane == 63) { s_wa[w] = ia; s_ws[w] = is; }
-    __syncthreads();
-    int wa = 0, ws = 0, ta = 0, ts = 0;
-    for (int u = 0; u < 16; u++) {
-      if (u < w) { wa += s_wa[u]; ws += s_ws[u]; }
-      ta += s_wa[u]; ts += s_ws[u];
-    }
-    int ra = carry_a + wa + ia - sa, rs = carry_s + ws + is - ss;
+    int v[8], ra[8], ps[8], rs[8], ta, ts;
+    scan_chunk(s_c, s_w, m, nz, m_cl, v, ra, ps, rs, ta, ts);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const int idx = 8 * t + k;
       if (idx < m) {
-        const int c = base + idx;
-        const int ps = (v[k] + m_cl - 1) / m_cl * m_cl;
-        start[c] = ra;
-        slot[c] = rs;
-        for (int u = rs / m_cl; u < (rs + ps) / m_cl && u < ncl_max; u++) cluster_cell[u] = c;
-        ra += v[k];
-        rs += ps;
+        const int c = base + idx, s0 = carry_s + rs[k];
+        start[c] = carry_a + ra[k];
+        slot[c] = s0;
+        // clusters whose first slot is one of this cell's atoms
+        for (int u = (s0 + m_cl - 1) / m_cl; u * m_cl < s0 + v[k] && u < ncl_max; u++) cluster_cell[u] = c;
       }
     }
     carry_a += ta;
     carry_s += ts;
-    __syncthreads();
+    __syncthreads();   // s_c is refilled by the next chunk
   }
   if (t == 0) { counts[0] = carry_s; counts[1] = carry_s / m_cl; counts[2] = carry_a; }
 }
-__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, int* __restrict__ cnt, int* __restrict__ cnt_out,
+__global__ __launch_bounds__(1024) void k_scan_cells(int ncells, int nz, int* __restrict__ cnt, int* __restrict__ cnt_out,
                                                      int* __restrict__ start, int* __restrict__ slot,
                                                      int* __restrict__ counts, int* __restrict__ cluster_cell,
-                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) { scan_cells_body(ncells, cnt, cnt_out, start, slot, counts, cluster_cell, ncl_max, m_cl, rebuild, blockIdx.x); }
-struct ScanCellsArgs { int ncells; int* cnt; int* cnt_out; int* start; int* slot; int* counts; int* cluster_cell; int ncl_max; int m_cl; const int* rebuild; };
+                                                     int ncl_max, int m_cl, const int* __restrict__ rebuild) { scan_cells_body(ncells, nz, cnt, cnt_out, start, slot, counts, cluster_cell, ncl_max, m_cl, rebuild, blockIdx.x); }
+struct ScanCellsArgs { int ncells; int nz; int* cnt; int* cnt_out; int* start; int* slot; int* counts; int* cluster_cell; int ncl_max; int m_cl; const int* rebuild; };
 __global__ __launch_bounds__(1024) void kb_scan_cells(const ScanCellsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const ScanCellsArgs& q = t[r];
-  scan_cells_body(q.ncells, q.cnt, q.cnt_out, q.start, q.slot, q.counts, q.cluster_cell, q.ncl_max, q.m_cl, q.rebuild, blockIdx.x - pre[r]);
+  scan_cells_body(q.ncells, q.nz, q.cnt, q.cnt_out, q.start, q.slot, q.counts, q.cluster_cell, q.ncl_max, q.m_cl, q.rebuild, blockIdx.x - pre[r]);
 }
 
 
-// Large cell grids: the same scan split over blocks of 8192 cells.  Pass 1
-// copies each block's counts to the layout, clears the binning counts and
-// writes the block's atom and slot totals; pass 2 adds the totals of the
-// blocks before it (a fixed-order sum) and writes the block's starts,
-// slots, cluster cells and, in the last block, the totals.  Integer sums:
-// the result is the single-block scan's.
-#define PM_SCAN_CHUNK 8192
-__global__ __launch_bounds__(1024) void k_scan_chunks(int ncells, int* __restrict__ cnt, int* __restrict__ cnt_out,
+// Large cell grids: the same layout split over one block per chunk.  Pass 1
+// copies each chunk's counts to the layout, clears the binning counts and
+// writes the chunk's atom and slot totals; pass 2 adds the totals of the
+// chunks before it (a fixed-order sum) and writes the chunk's starts, slots,
+// cluster cells and, in the last block, the totals.  Integer sums over the
+// same chunks: the result is the single-block layout's.
+__global__ __launch_bounds__(1024) void k_scan_chunks(int ncells, int nz, int* __restrict__ cnt, int* __restrict__ cnt_out,
                                                       int m_cl, int* __restrict__ aux, const int* __restrict__ rebuild) {
-  const int base = blockIdx.x * PM_SCAN_CHUNK;
-  const int m = min(PM_SCAN_CHUNK, ncells - base);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int ch = scan_chunk_cells(nz);
+  const int base = blockIdx.x * ch;
+  const int m = min(ch, ncells - base);
+  const int t = threadIdx.x;
   if (rebuild && *rebuild == 0) {
     for (int k = t; k < m; k += blockDim.x) cnt[base + k] = 0;
     return;
   }
-  __shared__ int s_a[16], s_s[16];
-  int sa = 0, ss = 0;
+  __shared__ int s_c[PM_SCAN_CHUNK];
+  __shared__ int s_w[16];
   for (int k = t; k < m; k += blockDim.x) {
     const int v = cnt[base + k];
     cnt_out[base + k] = v;
     cnt[base + k] = 0;
-    sa += v;
-    ss += (v + m_cl - 1) / m_cl * m_cl;
+    s_c[k] = v;
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    sa += __shfl_down(sa, off, 64);
-    ss += __shfl_down(ss, off, 64);
-  }
-  if (lane == 0) { s_a[w] = sa; s_s[w] = ss; }
   __syncthreads();
+  int v[8], ra[8], ps[8], rs[8], ta, ts;
+  scan_chunk(s_c, s_w, m, nz, m_cl, v, ra, ps, rs, ta, ts);
   if (t == 0) {
-    int ta = 0, ts = 0;
-    for (int u = 0; u < 16; u++) { ta += s_a[u]; ts += s_s[u]; }
     aux[2 * blockIdx.x] = ta;
     aux[2 * blockIdx.x + 1] = ts;
   }
 }
-__global__ __launch_bounds__(1024) void k_scan_finish(int ncells, const int* __restrict__ cnt_out,
+__global__ __launch_bounds__(1024) void k_scan_finish(int ncells, int nz, const int* __restrict__ cnt_out,
                                                       int* __restrict__ start, int* __restrict__ slot,
                                                       int* __restrict__ counts, int* __restrict__ cluster_cell,
                                                       int ncl_max, int m_cl, const int* __restrict__ aux,
                                                       const int* __restrict__ rebuild) {
   if (rebuild && *rebuild == 0) return;
   __shared__ int s_c[PM_SCAN_CHUNK];
-  __shared__ int s_wa[16], s_ws[16];
+  __shared__ int s_w[16];
   __shared__ int s_carry[2];
-  const int base = blockIdx.x * PM_SCAN_CHUNK;
-  const int m = min(PM_SCAN_CHUNK, ncells - base);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int ch = scan_chunk_cells(nz);
+  const int base = blockIdx.x * ch;
+  const int m = min(ch, ncells - base);
+  const int t = threadIdx.x;
   if (t == 0) {
     int ca = 0, cs = 0;
     for (int b = 0; b < (int)blockIdx.x; b++) { ca += aux[2 * b]; cs += aux[2 * b + 1]; }
@@ -180,44 +228,21 @@ __global__ __launch_bounds__(1024) void k_scan_finish(int ncells, const int* __r
   }
   for (int k = t; k < m; k += blockDim.x) s_c[k] = cnt_out[base + k];
   __syncthreads();
-  int v[8];
-  int sa = 0, ss = 0;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int idx = 8 * t + k;
-    v[k] = idx < m ? s_c[idx] : 0;
-    sa += v[k];
-    ss += (v[k] + m_cl - 1) / m_cl * m_cl;
-  }
-  int ia = sa, is = ss;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int ta = __shfl_up(ia, off, 64), ts = __shfl_up(is, off, 64);
-    if (lane >= off) { ia += ta; is += ts; }
-  }
-  if (lane == 63) { s_wa[w] = ia; s_ws[w] = is; }
-  __syncthreads();
-  int wa = 0, ws = 0, ta = 0, ts = 0;
-  for (int u = 0; u < 16; u++) {
-    if (u < w) { wa += s_wa[u]; ws += s_ws[u]; }
-    ta += s_wa[u]; ts += s_ws[u];
-  }
-  int ra = s_carry[0] + wa + ia - sa, rs = s_carry[1] + ws + is - ss;
+  const int carry_a = s_carry[0], carry_s = s_carry[1];
+  int v[8], ra[8], ps[8], rs[8], ta, ts;
+  scan_chunk(s_c, s_w, m, nz, m_cl, v, ra, ps, rs, ta, ts);
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int idx = 8 * t + k;
     if (idx < m) {
-      const int c = base + idx;
-      const int ps = (v[k] + m_cl - 1) / m_cl * m_cl;
-      start[c] = ra;
-      slot[c] = rs;
-      for (int u = rs / m_cl; u < (rs + ps) / m_cl && u < ncl_max; u++) cluster_cell[u] = c;
-      ra += v[k];
-      rs += ps;
+      const int c = base + idx, s0 = carry_s + rs[k];
+      start[c] = carry_a + ra[k];
+      slot[c] = s0;
+      for (int u = (s0 + m_cl - 1) / m_cl; u * m_cl < s0 + v[k] && u < ncl_max; u++) cluster_cell[u] = c;
     }
   }
   if (t == 0 && blockIdx.x == gridDim.x - 1) {
-    const int tot_a = s_carry[0] + ta, tot_s = s_carry[1] + ts;
+    const int tot_a = carry_a + ta, tot_s = carry_s + ts;
     counts[0] = tot_s; counts[1] = tot_s / m_cl; counts[2] = tot_a;
   }
 }
@@ -323,7 +348,6 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
   }
   if (c >= ncells) return;
   const int k = cell_count[c];
-  if (k == 0) return;
   const int* a = cell_atoms + cell_start[c];
   const int s0 = cell_slot[c];
   const bool sort = k <= PM_SORT_MAX;
@@ -360,27 +384,31 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
     if (x_build) x_build[ai] = make_int4(p.x, p.y, p.z, 1);   // the position the layout was built at
     if (x_prune) x_prune[ai] = make_int4(p.x, p.y, p.z, 1);   // the inner lists are pruned at it too
   }
-  // padding slots: never used as j atoms, masked as i atoms, skipped by PME
-  const int ps = (k + m_cl - 1) / m_cl * m_cl;
-  const int u = k + lane;
-  if (u < ps) {
-    const int a0 = a[0];
+  // padding slots after a column's last cell, up to the next multiple of
+  // m_cl: never used as j atoms, masked as i atoms, skipped by PME; placed
+  // at the cell's lower corner
+  if (c % box.n[2] != box.n[2] - 1) return;
+  const int e = s0 + k, pe = (e + m_cl - 1) / m_cl * m_cl;
+  const int u = e + lane;
+  if (u < pe) {
+    const int cz = c % box.n[2], cy = (c / box.n[2]) % box.n[1], cx = c / (box.n[1] * box.n[2]);
     int4 p;
-    p.x = (int)to_fix(pos[3 * a0], box.to_fix[0]);
-    p.y = (int)to_fix(pos[3 * a0 + 1], box.to_fix[1]);
-    p.z = (int)to_fix(pos[3 * a0 + 2], box.to_fix[2]);
+    p.x = (int)(unsigned int)(long long)(box.cell0_fix[0] + cx * box.real_fix[0]);
+    p.y = (int)(unsigned int)(long long)(box.cell0_fix[1] + cy * box.real_fix[1]);
+    p.z = (int)(unsigned int)(long long)(box.cell0_fix[2] + cz * box.real_fix[2]);
     p.w = 0;
-    slot_pos[s0 + u] = p;
+    slot_pos[u] = p;
     if (slot_pos64) {
+      // the same point in 64-bit fixed point
       longlong4 p8;
-      p8.x = (long long)to_fix64(pos[3 * a0], box.to_fix[0]);
-      p8.y = (long long)to_fix64(pos[3 * a0 + 1], box.to_fix[1]);
-      p8.z = (long long)to_fix64(pos[3 * a0 + 2], box.to_fix[2]);
+      p8.x = (long long)((unsigned long long)(unsigned int)p.x << 32);
+      p8.y = (long long)((unsigned long long)(unsigned int)p.y << 32);
+      p8.z = (long long)((unsigned long long)(unsigned int)p.z << 32);
       p8.w = 0;
-      slot_pos64[s0 + u] = p8;
+      slot_pos64[u] = p8;
     }
-    slot_meta[s0 + u] = dummy_type | (0x7fffff << 8);
-    atom_of_slot[s0 + u] = -1;
+    slot_meta[u] = dummy_type | (0x7fffff << 8);
+    atom_of_slot[u] = -1;
   }
 }
 __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const double* __restrict__ pos,
@@ -481,14 +509,16 @@ __device__ inline int wrap_cell(int c, int d, int n) {
 }
 
 // ------------------------------------------------------------- pair lists
-// One wave per i-cluster: every atom of the stencil cells that follow the
-// cluster's cell (and of its own cell from the cluster on) whose distance to
-// the cluster's bounding box is below the list cutoff is appended to the
-// cluster's list.  Every unordered pair within the cutoff is thus listed
-// exactly once.  Bit 31 marks an entry that some cluster atom excludes or
-// modifies; its masks (FULL bit ii, MODIFIED bit 16 + ii) go to jmask.
-// BB: 1 the bounding-box test, 0 the exact per-atom test, 2 the one
-// a.bbox_filter names (batched launches, whose replicas may differ)
+// One wave per i-cluster (m consecutive slots of one cell column): the
+// candidates are the atoms, from the cluster's first slot on, of the cells of
+// the stencil's columns that meet the list cutoff around the cluster; those
+// that pass the list test are appended to the cluster's list.  The pair
+// kernel keeps a pair only when the j slot follows the i slot, so every
+// unordered pair within the cutoff is listed exactly once.  Bit 31 marks an
+// entry that some cluster atom excludes or modifies; its masks (FULL bit ii,
+// MODIFIED bit 16 + ii) go to jmask.  BB: 1 the bounding-box test, 0 the
+// exact per-atom test, 2 the one a.bbox_filter names (batched launches, whose
+// replicas may differ)
 template <int M, int BB>
 __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int bx) {
   const bool bbox = BB == 2 ? a.bbox_filter != 0 : BB == 1;
@@ -496,16 +526,18 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   extern __shared__ int s_dyn[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nst = a.nstencil;
-  int* s_start = s_dyn + wave * (3 * nst + 1 + 2 * a.xs_tcap);
-  int* s_cnt = s_start + nst;
-  int* s_pref = s_cnt + nst;     // nst + 1 entries
-  int* s_tri = s_pref + nst + 1; // xs_tcap entries: partner | ii << 24 | class << 28
+  // up to two slot ranges per stencil column
+  int* s_start = s_dyn + wave * (6 * nst + 1 + 2 * a.xs_tcap);
+  int* s_cnt = s_start + 2 * nst;
+  int* s_pref = s_cnt + 2 * nst;     // 2 nst + 1 entries
+  int* s_tri = s_pref + 2 * nst + 1; // xs_tcap entries: partner | ii << 24 | class << 28
   int* s_srt = s_tri + a.xs_tcap; // the same, sorted by partner
   const int cl = bx * 4 + wave;
   const int ncl = a.counts[1];
   const bool active = cl < ncl;  // every wave reaches every barrier below
   const int cell = active ? a.cluster_cell[cl] : 0;
-  const int cz = cell % box.n[2], cy = (cell / box.n[2]) % box.n[1], cx = cell / (box.n[1] * box.n[2]);
+  const int n0 = box.n[0], n1 = box.n[1], n2 = box.n[2];
+  const int cz = cell % n2, cy = (cell / n2) % n1, cx = cell / (n1 * n2);
 
   // i-cluster: anchor, bounding box and exclusion partners
   const int4 anchor = active ? a.slot_pos[cl * M] : make_int4(0, 0, 0, 0);
@@ -543,10 +575,8 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
   pmin = __shfl(wave_imin_m<M>(pmin), 0, 64);
   pmax = __shfl(wave_imax_m<M>(pmax), 0, 64);
 
-  // small boxes: the cluster's positions for the per-atom test, and which of
-  // its slots hold atoms (padding slots must not keep an entry: their
-  // positions follow the arrival order of the binning, so the lists, and the
-  // rounding of the sums over them, would change from run to run)
+  // exact test: the cluster's positions, and which of its slots hold atoms
+  // (padding slots, placed at a cell corner, must not keep an entry)
   int qx[M], qy[M], qz[M];
   unsigned int qreal = 0;
   if (active && !bbox) {
@@ -559,63 +589,93 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
       qz[ii] = q.z;
     }
   }
-  // stencil cells: half list, so only cells after this one in linear order,
-  // and this cell's atoms from this cluster on (the pair kernel keeps a pair
-  // only when the j slot follows the i slot); cells whose box lies beyond the
-  // list cutoff from the cluster's bounding box are dropped; the non-empty
-  // ones are compacted in stencil order
+  // the cells of a column lie at j * hz - oa (j = cell - cz) relative to the
+  // anchor, which the binning put in cell cz
+  const float hz = (float)(box.real_fix[2] * box.fix_to_A[2]);
+  const float rhz = 1.f / hz;
+  const float oa = (float)(((double)((unsigned int)anchor.z - (unsigned int)(long long)box.cell0_fix[2]) -
+                            cz * box.real_fix[2]) * box.fix_to_A[2]);
+  // the bounding-box test holds for clusters no taller than the cell
+  // diagonal (the box is then large enough for minimal images relative to
+  // the anchor); a taller cluster (a sparse column) keeps every candidate
+  const float rx = (float)(box.real_fix[0] * box.fix_to_A[0]), ry = (float)(box.real_fix[1] * box.fix_to_A[1]);
+  const bool loose = bbox && hiz - loz > sqrtf(rx * rx + ry * ry + hz * hz);
+  const int first = cl * M;
+  // stencil columns (one per lane): per column the z interval, relative to
+  // the anchor, within the list cutoff of a cluster atom (exact test; of the
+  // bounding box in the bounding-box test), its cells as one or two
+  // contiguous slot ranges (two when it wraps round the box), clipped to the
+  // slots from this cluster's first on (half list); the non-empty ranges are
+  // compacted in stencil order
   int nact = 0;
   for (int s0 = 0; s0 < nst; s0 += 64) {
     const int s = s0 + lane;
-    int st = 0, ct = 0;
+    int st0 = 0, ct0 = 0, st1 = 0, ct1 = 0;
     if (active && s < nst) {
       const int4 d = a.stencil[s];
-      const int wx = wrap_cell(cx, d.x, box.n[0]), wy = wrap_cell(cy, d.y, box.n[1]), wz = wrap_cell(cz, d.z, box.n[2]);
-      const int nb = (wx * box.n[1] + wy) * box.n[2] + wz;
-      st = a.cell_slot[nb];
-      ct = a.cell_count[nb];
-      if (nb < cell) ct = 0;
-      else if (nb == cell) { ct = st + ct - cl * M; st = cl * M; }
-      if (ct > 0 && bbox) {
-        // gap between the cell's box (padded by 1e-3 A) and the cluster's bounding box
-        const unsigned int bx = (unsigned int)(long long)(box.cell0_fix[0] + wx * box.real_fix[0]);
-        const unsigned int by = (unsigned int)(long long)(box.cell0_fix[1] + wy * box.real_fix[1]);
-        const unsigned int bz = (unsigned int)(long long)(box.cell0_fix[2] + wz * box.real_fix[2]);
+      const int wx = wrap_cell(cx, d.x, n0), wy = wrap_cell(cy, d.y, n1);
+      const unsigned int bx = (unsigned int)(long long)(box.cell0_fix[0] + wx * box.real_fix[0]);
+      const unsigned int by = (unsigned int)(long long)(box.cell0_fix[1] + wy * box.real_fix[1]);
+      float zl = 1e30f, zh = -1e30f;
+      if (bbox) {
+        // gap between the column's rectangle (padded by 1e-3 A) and the cluster's bounding box
         const float clx = (float)(int)(bx - (unsigned int)anchor.x) * fx - 1e-3f;
         const float cly = (float)(int)(by - (unsigned int)anchor.y) * fy - 1e-3f;
-        const float clz = (float)(int)(bz - (unsigned int)anchor.z) * fz - 1e-3f;
-        const float chx = clx + (float)(box.real_fix[0] * box.fix_to_A[0]) + 2e-3f;
-        const float chy = cly + (float)(box.real_fix[1] * box.fix_to_A[1]) + 2e-3f;
-        const float chz = clz + (float)(box.real_fix[2] * box.fix_to_A[2]) + 2e-3f;
+        const float chx = clx + rx + 2e-3f, chy = cly + ry + 2e-3f;
         const float gx = fmaxf(0.f, fmaxf(clx - hix, lox - chx));
         const float gy = fmaxf(0.f, fmaxf(cly - hiy, loy - chy));
-        const float gz = fmaxf(0.f, fmaxf(clz - hiz, loz - chz));
-        if (gx * gx + gy * gy + gz * gz >= cut2_list) ct = 0;
-      } else if (ct > 0) {
-        // small box: per axis, the periodic distance from each of the
-        // cluster's atoms to the cell's interval (padded by 1e-3 A) bounds
-        // the pair kernel's minimal-image difference from below
-        const unsigned int bx = (unsigned int)(long long)(box.cell0_fix[0] + wx * box.real_fix[0]);
-        const unsigned int by = (unsigned int)(long long)(box.cell0_fix[1] + wy * box.real_fix[1]);
-        const unsigned int bz = (unsigned int)(long long)(box.cell0_fix[2] + wz * box.real_fix[2]);
+        const float r2 = cut2_list - gx * gx - gy * gy;
+        if (r2 > 0.f) {
+          const float r = sqrtf(r2);
+          zl = loz - r;
+          zh = hiz + r;
+        }
+      } else {
+        // per axis, the periodic distance from each of the cluster's atoms
+        // to the column's interval (padded by 1e-3 A) bounds the pair
+        // kernel's minimal-image difference from below
         const unsigned int wdx = (unsigned int)(long long)box.real_fix[0] + 1u;
         const unsigned int wdy = (unsigned int)(long long)box.real_fix[1] + 1u;
-        const unsigned int wdz = (unsigned int)(long long)box.real_fix[2] + 1u;
-        bool near = false;
 #pragma unroll
         for (int ii = 0; ii < M; ii++) {
           const float gx = fmaxf(cell_gap((unsigned int)qx[ii] - bx, wdx) * fx - 1e-3f, 0.f);
           const float gy = fmaxf(cell_gap((unsigned int)qy[ii] - by, wdy) * fy - 1e-3f, 0.f);
-          const float gz = fmaxf(cell_gap((unsigned int)qz[ii] - bz, wdz) * fz - 1e-3f, 0.f);
-          near = near || (((qreal >> ii) & 1u) && gx * gx + gy * gy + gz * gz < cut2_list);
+          const float r2 = cut2_list - gx * gx - gy * gy;
+          if (((qreal >> ii) & 1u) && r2 > 0.f) {
+            const float r = sqrtf(r2);
+            const float dz = (float)(int)((unsigned int)qz[ii] - (unsigned int)anchor.z) * fz;
+            zl = fminf(zl, dz - r);
+            zh = fmaxf(zh, dz + r);
+          }
         }
-        if (!near) ct = 0;
+      }
+      if (zl <= zh) {
+        // cells j0 .. j1 (relative to cz) meet [zl, zh], each padded by 1e-3 A
+        const int j0 = (int)floorf((zl - 1e-3f + oa) * rhz), j1 = (int)floorf((zh + 1e-3f + oa) * rhz);
+        int za = 0, zb = n2 - 1, zd = -1;   // cell ranges [za, zb] and [0, zd]
+        if (j1 - j0 + 1 < n2) {
+          za = wrap_cell(cz, j0, n2);
+          zb = wrap_cell(cz, j1, n2);
+          if (za > zb) { zd = zb; zb = n2 - 1; }
+        }
+        const int col = (wx * n1 + wy) * n2;
+        int e = a.cell_slot[col + zb] + a.cell_count[col + zb];
+        st0 = max(a.cell_slot[col + za], first);
+        ct0 = max(e - st0, 0);
+        if (zd >= 0) {
+          e = a.cell_slot[col + zd] + a.cell_count[col + zd];
+          st1 = max(a.cell_slot[col], first);
+          ct1 = max(e - st1, 0);
+        }
       }
     }
-    const unsigned long long bal = __ballot(ct > 0);
-    const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-    if (ct > 0) { s_start[nact + before] = st; s_cnt[nact + before] = ct; }
-    nact += __popcll(bal);
+    const unsigned long long bal0 = __ballot(ct0 > 0), bal1 = __ballot(ct1 > 0);
+    const int b0 = __builtin_amdgcn_mbcnt_hi((unsigned)(bal0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal0, 0));
+    if (ct0 > 0) { s_start[nact + b0] = st0; s_cnt[nact + b0] = ct0; }
+    nact += __popcll(bal0);
+    const int b1 = __builtin_amdgcn_mbcnt_hi((unsigned)(bal1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal1, 0));
+    if (ct1 > 0) { s_start[nact + b1] = st1; s_cnt[nact + b1] = ct1; }
+    nact += __popcll(bal1);
   }
   __syncthreads();
   int carry = 0;
@@ -671,7 +731,7 @@ __device__ inline void list_body(PmBox box, PmNbLaunch a, float cut2_list, int b
         const float gx = fmaxf(0.f, fmaxf(lox - dx, dx - hix));
         const float gy = fmaxf(0.f, fmaxf(loy - dy, dy - hiy));
         const float gz = fmaxf(0.f, fmaxf(loz - dz, dz - hiz));
-        pass = gx * gx + gy * gy + gz * gz < cut2_list;
+        pass = loose || gx * gx + gy * gy + gz * gz < cut2_list;
       } else {
         // box too small for the bounding-box test: keep the entry when one
         // of the cluster's atoms lies within the list cutoff, by the pair
@@ -1680,23 +1740,26 @@ void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, in
                        rebuild);
 }
 
-void pm_launch_scan_cells(hipStream_t s, int ncells, int* bin_count, int* cell_count, int* cell_start,
+void pm_launch_scan_cells(hipStream_t s, int ncells, int nz, int* bin_count, int* cell_count, int* cell_start,
                           int* cell_slot, int* counts, int* cluster_cell, int ncl_max, int m, const int* rebuild,
                           int* aux) {
-  const int nchunks = (ncells + PM_SCAN_CHUNK - 1) / PM_SCAN_CHUNK;
+  const int ch = scan_chunk_cells(nz);
+  const int nchunks = (ncells + ch - 1) / ch;
   if (!pm_recording() && aux && nchunks > 2) {
-    hipLaunchKernelGGL(k_scan_chunks, dim3(nchunks), dim3(1024), 0, s, ncells, bin_count, cell_count, m, aux, rebuild);
-    hipLaunchKernelGGL(k_scan_finish, dim3(nchunks), dim3(1024), 0, s, ncells, cell_count, cell_start, cell_slot,
-                       counts, cluster_cell, ncl_max, m, aux, rebuild);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(nchunks), dim3(1024), 0, s, ncells, nz, bin_count, cell_count, m, aux,
+                       rebuild);
+    hipLaunchKernelGGL(k_scan_finish, dim3(nchunks), dim3(1024), 0, s, ncells, nz, cell_count, cell_start,
+                       cell_slot, counts, cluster_cell, ncl_max, m, aux, rebuild);
     return;
   }
   if (pm_recording()) {
-    const ScanCellsArgs a{ncells, bin_count, cell_count, cell_start, cell_slot, counts, cluster_cell, ncl_max, m, rebuild};
+    const ScanCellsArgs a{ncells, nz, bin_count, cell_count, cell_start, cell_slot, counts, cluster_cell, ncl_max, m,
+                          rebuild};
     pm_record(PM_REC_SCAN_CELLS, 0, 1, 1024, 0, &a, sizeof(a));
     return;
   }
-  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, bin_count, cell_count, cell_start, cell_slot,
-                     counts, cluster_cell, ncl_max, m, rebuild);
+  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncells, nz, bin_count, cell_count, cell_start,
+                     cell_slot, counts, cluster_cell, ncl_max, m, rebuild);
 }
 
 void pm_launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* cell_start,
@@ -1782,7 +1845,7 @@ void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, u
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
   double cut = sqrt(p.cut2) + a.skin;
   float c = (float)(cut * (1.0 + 1e-5) + 1e-4);
-  size_t lds = 4 * (3 * a.nstencil + 1 + 2 * a.xs_tcap) * sizeof(int);
+  size_t lds = 4 * (6 * a.nstencil + 1 + 2 * a.xs_tcap) * sizeof(int);
   if (pm_recording()) {
     const ListArgs q{box, a, c * c};
     pm_record(PM_REC_LIST, a.m, (a.ncl_max + 3) / 4, 256, (unsigned)lds, &q, sizeof(q));
