@@ -209,6 +209,7 @@ struct pm_context {
   unsigned long long* d_grid_fix = nullptr;
   void* d_grid_r = nullptr;
   void* d_grid_c = nullptr;
+  void* d_theta = nullptr;   // theta(m) on the line-major half grid (grid precision)
   double *d_bmod = nullptr, *d_expt = nullptr, *d_tw = nullptr;
   PmFftPlan plans[3];
   double pme_alpha_tables = -1;
@@ -668,8 +669,11 @@ int ensure_pme(pm_context* c, ForceDesc& f) {
     HIPCHECK(c, hipMemset(c->d_grid_fix, 0, ng * sizeof(unsigned long long)));
     if (c->d_grid_r) hipFree(c->d_grid_r);
     if (c->d_grid_c) hipFree(c->d_grid_c);
+    if (c->d_theta) hipFree(c->d_theta);
+    c->d_grid_r = c->d_grid_c = c->d_theta = nullptr;
     HIPCHECK(c, hipMalloc(&c->d_grid_r, ng * er));
     HIPCHECK(c, hipMalloc(&c->d_grid_c, ngc * er * 2));
+    HIPCHECK(c, hipMalloc(&c->d_theta, ngc * er));
     // twiddles exp(-2 pi i m / n) for the three lengths
     std::vector<double> tw;
     int off = 0;
@@ -704,6 +708,16 @@ int ensure_pme(pm_context* c, ForceDesc& f) {
     int r;
     if ((r = upload(c, &c->d_bmod, bm))) return r;
     if ((r = upload(c, &c->d_expt, ex))) return r;
+    // theta(m) for the x passes that write no energy records
+    PmPmeLaunch t{};
+    t.nx = nx; t.ny = ny; t.nz = nz; t.nzc = nzc;
+    for (int k = 0; k < 3; k++) t.Lr[k] = 1.0 / c->box.L[k];
+    t.bmod = c->d_bmod;
+    t.expf_tab = c->d_expt;
+    t.inv_pi_v = 1.0 / (kPi * (c->box.L[0] * c->box.L[1] * c->box.L[2]));
+    t.theta = c->d_theta;
+    pm_launch_pme_theta(c->stream, t, c->precision);
+    HIPCHECK(c, hipGetLastError());
     c->pme_alpha_tables = f.alpha;
   }
   return PM_OK;
@@ -1231,6 +1245,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       a.force_fix = c->d_force;
       a.epart = c->d_epart + off * PM_NE;
       a.want_virial = vir ? 1 : 0;
+      a.want_energy = want_energy ? 1 : 0;
+      a.theta = c->d_theta;
       // spread bricks: 2x2x2 cells per brick (fewer where the tile would not fit 48 KB of LDS)
       a.cell_slot = c->d_cell_slot;
       a.cell_count = c->d_cell_count;
@@ -1604,7 +1620,8 @@ void free_all(pm_context* c) {
   dfree(c->d_grid_fix);
   if (c->d_grid_r) hipFree(c->d_grid_r);
   if (c->d_grid_c) hipFree(c->d_grid_c);
-  c->d_grid_r = c->d_grid_c = nullptr;
+  if (c->d_theta) hipFree(c->d_theta);
+  c->d_grid_r = c->d_grid_c = c->d_theta = nullptr;
   dfree(c->d_bmod); dfree(c->d_expt); dfree(c->d_tw);
   dfree(c->d_b_a); dfree(c->d_b_b); dfree(c->d_b_row); dfree(c->d_b_idx); dfree(c->d_b_k); dfree(c->d_b_r0);
   dfree(c->d_a_a); dfree(c->d_a_b); dfree(c->d_a_c); dfree(c->d_a_row); dfree(c->d_a_idx);

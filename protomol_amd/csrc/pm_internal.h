@@ -252,6 +252,11 @@ struct PmPmeLaunch {
   long long* force_fix;
   double* epart;             // [pm_pme_xpass_blocks() * PM_NE]
   int want_virial;
+  int want_energy;           // 0: no energy records (then neither is the virial wanted)
+  // theta(m) of every point of the line-major half grid ([x][y][kz], float
+  // or double as the grid), the factor the x pass applies when it writes no
+  // energy records
+  const void* theta;
   int* err;                  // error flag word (PM_FLAG_FORCE_RANGE)
   int prio;                  // wave priority (s_setprio) of the PME kernels: 0 .. 3
   // spread by cell bricks staged in LDS tiles
@@ -277,6 +282,8 @@ struct PmAtomTail;
 // bonded forces of each atom in the gather kernel's thread for that atom
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
                          const longlong4* pos64, int precision, const PmAtomTail* tail);
+// a.theta (written through a const_cast) from bmod, expf_tab, inv_pi_v and Lr
+void pm_launch_pme_theta(hipStream_t s, const PmPmeLaunch& a, int precision);
 // energy records written by the reciprocal term (one per x-pass block, or
 // one per kz plane)
 int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision, int plane);

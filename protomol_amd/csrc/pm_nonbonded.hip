@@ -1096,6 +1096,12 @@ __device__ inline void pair_cluster(const PmBox& box, const PmNbLaunch& a, const
     return;
   }
   constexpr int JW = 64 / M;
+  // The mixed-precision force kernel needs 72 VGPRs, which lets seven of
+  // its waves share a SIMD; reserving registers up to v79 holds it to six,
+  // so that the PME kernels running beside it on the second stream find two
+  // free wave slots per SIMD instead of one (98k water: the step's PME chain
+  // and this kernel then end together, 257.6 -> 246.0 us per MD step).
+  if constexpr (sizeof(T) == 4 && !WE) asm volatile("" ::: "v79");
   const int ii = lane % M, js = lane / M;
   const int si = cl * M + ii;
   const int mi = a.slot_meta[si];

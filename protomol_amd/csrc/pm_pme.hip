@@ -675,18 +675,27 @@ struct PmERed {
 // Grid::scalarSum (base/Grid.h:276-389) at one wave vector (kx, ky, kz) of
 // the half grid: adds v's energy and virial terms (weight 2 for the kz that
 // also stand for their conjugates) and returns v * theta(m).
+// theta(m) at (kx, ky, kz) != 0 with m = (mi, mj, mk) and m2 = |m|^2
+__device__ inline double theta_at(const PmPmeLaunch& a, int kx, int ky, int kz, double& mi, double& mj, double& mk,
+                                  double& m2) {
+  const int i0 = kx <= a.nx / 2 ? kx : kx - a.nx;
+  const int j0 = ky <= a.ny / 2 ? ky : ky - a.ny;
+  mi = i0 * a.Lr[0];
+  mj = j0 * a.Lr[1];
+  mk = kz * a.Lr[2];
+  m2 = mi * mi + mj * mj + mk * mk;
+  const double* bm = a.bmod;
+  const double* ex = a.expf_tab;
+  return bm[kx] * bm[a.nx + ky] * bm[a.nx + a.ny + kz] * (ex[kx] * a.inv_pi_v) * ex[a.nx + ky] *
+         ex[a.nx + a.ny + kz] / m2;
+}
+
 template <typename T, typename T2>
 __device__ inline T2 influence(const PmPmeLaunch& a, int kx, int ky, int kz, T2 v, PmERed& s) {
   T theta = 0;
   if (kx != 0 || ky != 0 || kz != 0) {
-    const int i0 = kx <= a.nx / 2 ? kx : kx - a.nx;
-    const int j0 = ky <= a.ny / 2 ? ky : ky - a.ny;
-    const double mi = i0 * a.Lr[0], mj = j0 * a.Lr[1], mk = kz * a.Lr[2];
-    const double m2 = mi * mi + mj * mj + mk * mk;
-    const double* bm = a.bmod;
-    const double* ex = a.expf_tab;
-    const double th = bm[kx] * bm[a.nx + ky] * bm[a.nx + a.ny + kz] * (ex[kx] * a.inv_pi_v) * ex[a.nx + ky] *
-                      ex[a.nx + a.ny + kz] / m2;
+    double mi, mj, mk, m2;
+    const double th = theta_at(a, kx, ky, kz, mi, mj, mk, m2);
     theta = (T)th;
     const double w = (kz == 0 || ((a.nz & 1) == 0 && kz == a.nz / 2)) ? 1.0 : 2.0;
     const double q2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
@@ -705,6 +714,21 @@ __device__ inline T2 influence(const PmPmeLaunch& a, int kx, int ky, int kz, T2 
   v.x *= theta;
   v.y *= theta;
   return v;
+}
+
+// theta(m) over the line-major half grid, rounded as influence() rounds it
+template <typename T>
+__global__ __launch_bounds__(256) void k_theta(PmPmeLaunch a) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ng = (long long)a.nx * a.ny * a.nzc;
+  if (e >= ng) return;
+  const int kz = (int)(e % a.nzc), ky = (int)((e / a.nzc) % a.ny), kx = (int)(e / ((long long)a.nzc * a.ny));
+  T th = 0;
+  if (kx != 0 || ky != 0 || kz != 0) {
+    double mi, mj, mk, m2;
+    th = (T)theta_at(a, kx, ky, kz, mi, mj, mk, m2);
+  }
+  const_cast<T*>(reinterpret_cast<const T*>(a.theta))[e] = th;
 }
 
 // block sum (fixed order: lanes, then waves) of the partials into this
@@ -790,14 +814,28 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
   }
   __syncthreads();
   T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1, n);
-  PmERed s = {0, 0, 0, 0, 0, 0, 0};
-  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
-    const int l = e / n, kx = e - l * n;
-    const int kz = kz0 + l;
-    if (kz >= a.nzc) continue;
-    r[e] = influence<T, T2>(a, kx, y, kz, r[e], s);
+  if (a.want_energy || a.want_virial || !a.theta) {
+    PmERed s = {0, 0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+      const int l = e / n, kx = e - l * n;
+      const int kz = kz0 + l;
+      if (kz >= a.nzc) continue;
+      r[e] = influence<T, T2>(a, kx, y, kz, r[e], s);
+    }
+    store_block_energy(a, s, s_red);   // its barrier also orders the writes to r[]
+  } else {
+    // no energy records: theta(m) from the table
+    const T* th = reinterpret_cast<const T*>(a.theta);
+    for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+      const int l = e / n, kx = e - l * n;
+      const int kz = kz0 + l;
+      if (kz >= a.nzc) continue;
+      const T t = th[((long long)kx * a.ny + y) * a.nzc + kz];
+      r[e].x *= t;
+      r[e].y *= t;
+    }
+    __syncthreads();
   }
-  store_block_energy(a, s, s_red);   // its barrier also orders the writes to r[]
   T2* r2 = fft_lines<T, T2>(r, r == b0 ? b1 : b0, TL, p, tws, +1, n);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int x = e / TL, l = e - x * TL;
@@ -1045,6 +1083,13 @@ bool pm_launch_pme_adjoint(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan*
                            int precision, int mode, const double4* slot_v, double* out) {
   if (precision == 1) return launch_pme_adj_p<double>(s, a, plans, pos64, mode, slot_v, out);
   return launch_pme_adj_p<float>(s, a, plans, pos64, mode, slot_v, out);
+}
+
+void pm_launch_pme_theta(hipStream_t s, const PmPmeLaunch& a, int precision) {
+  const long long ng = (long long)a.nx * a.ny * a.nzc;
+  const dim3 g((unsigned)((ng + 255) / 256)), b(256);
+  if (precision == 1) hipLaunchKernelGGL(k_theta<double>, g, b, 0, s, a);
+  else hipLaunchKernelGGL(k_theta<float>, g, b, 0, s, a);
 }
 
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
