@@ -39,7 +39,7 @@ FLOPS_PER_PAIR = 69       # LJ-C2 + Ewald-real per in-cutoff pair (algorithmic c
 # HBM bytes per launch of the pair kernel at the default options: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (KiB per dispatch from rocprofv3 --pmc, scaled by what the
 # counters read for a 512 MB device copy), profiles/k_pair_water98k_hbm_traffic.md
-PAIR_TRAFFIC_BYTES = (25359.8 * 2.0 + 153438.6 * 1.0) * 1024
+PAIR_TRAFFIC_BYTES = (21628.5 * 2.0 + 131140.1 * 1.0) * 1024
 
 
 def build_system():
@@ -789,7 +789,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {args.gpus} is not run "
-              f"(multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
+              f"(this build measures the one-GPU share of a sharded run).", file=sys.stderr)
         sys.exit(2)
     steps, warmup = args.steps, args.warmup
     s = build_system()
@@ -854,9 +854,11 @@ def main():
         "pme_hbm_frac": (pme_bytes / pme_avg_s) / PEAK_HBM,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 600 and run["cluster"] == 8 else None,
+                     "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 1400 and run["cluster"] == 8 and not args.opt
+                                else None,
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) "
-                                     "at list skin 0.6 A, cluster 8; null for other options",
+                                     "at the default options (list skin 1.4 A, exact list test, cluster 8); "
+                                     "null for other options",
                      "kernel": "k_pair (LJ-C2 + Ewald-real, %d flops/pair)" % FLOPS_PER_PAIR,
                      "note": "bound by FP32 vector ALU issue (the kernel issues no MFMA); "
                              "peak = MI355X vector FP32 157.3 TFLOP/s (counts packed FP32 dual issue, "
