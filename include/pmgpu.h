@@ -346,9 +346,17 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        diagonal) and the exact per-atom distance in smaller
  *                        ones, 1 the exact per-atom distance in every box
  *                        (fewer entries, a longer build).
+ *   PM_OPT_PME_SPREAD    the PME charge spread: -1 (default) automatic (1),
+ *                        1 fused with the z transform (a block owns whole z
+ *                        lines and adds the stencil points of the atoms of
+ *                        the cell columns that reach them, no global
+ *                        atomics; used where a block's columns and lines fit
+ *                        its LDS), 0 by bricks of cells into LDS tiles added
+ *                        to the grid with atomics, then the z transform.
+ *                        Both give the same grid bit for bit.
  * pm_get_option returns the value in effect (after the automatic choice). */
 enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PME_PRIORITY = 4,
-       PM_OPT_PRUNE_SKIN = 5, PM_OPT_LIST_TEST = 6 };
+       PM_OPT_PRUNE_SKIN = 5, PM_OPT_LIST_TEST = 6, PM_OPT_PME_SPREAD = 7 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

@@ -225,6 +225,7 @@ struct pm_context {
   // captured MD steps (hipGraph), replayed while nothing they reference changes
   int graph_option = 1;              // PM_OPT_GRAPH
   int pme_fft_option = 0;            // PM_OPT_PME_FFT
+  int pme_spread_option = 1;         // PM_OPT_PME_SPREAD (1: fused with the z transform where it fits)
   int pme_prio_option = -1;          // PM_OPT_PME_PRIORITY (-1: automatic)
 
   // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
@@ -658,8 +659,8 @@ int ensure_pme(pm_context* c, ForceDesc& f) {
   const bool same_grid = c->pme_nx == nx && c->pme_ny == ny && c->pme_nz == nz && c->pme_prec == c->precision;
   if (!f.recip) return PM_OK;
   if (!same_grid) {
-    if (nx > 4096 || ny > 4096 || nz > 4096)
-      return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: at most 4096 points per axis", nx, ny, nz);
+    if (nx > 4096 || ny > 4096 || nz > 4096 || nx < 8 || ny < 8 || nz < 8)
+      return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: 8 to 4096 points per axis", nx, ny, nz);
     if (!fft_plan(nx, c->plans[0]) || !fft_plan(ny, c->plans[1]) || !fft_plan(nz, c->plans[2]))
       return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: the GPU FFT handles lengths with factors 2, 3 and 5 only", nx, ny, nz);
     size_t ng = (size_t)nx * ny * nz, ngc = (size_t)nx * ny * nzc;
@@ -1271,10 +1272,16 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       }
       if (a.brick[0] * a.brick[1] * a.brick[2] > 64)
         return fail(c, PM_ERR_UNSUPPORTED, "internal: spread brick of more than 64 cells");
-      if ((size_t)a.tile[0] * a.tile[1] * a.tile[2] * 8 > 64 * 1024)
-        return fail(c, PM_ERR_UNSUPPORTED, "PME grid too fine for the cell size: spread tile exceeds 64 KB of LDS (use a smaller cellsize)");
       if (f.order != 4 && f.order != 6 && f.order != 8)
         return fail(c, PM_ERR_UNSUPPORTED, "PME order %d not implemented on the GPU", f.order);
+      // spread fused with the z transform: an atom lies within half the
+      // list skin of the position it was binned at (else this evaluation
+      // rebuilds the layout), padded for the float displacement test
+      const double drift_A = (reuse ? 0.5 * skin * (1.0 + 1e-3) : 0.0) + 1e-3;
+      for (int d = 0; d < 3; d++) a.drift_u[d] = drift_A * ng[d] / c->box.L[d] + 0.01;
+      a.spread_zfwd = !adj && c->pme_spread_option != 0 && pm_pme_spread_zfwd_fits(a, c->precision) ? 1 : 0;
+      if (!a.spread_zfwd && (size_t)a.tile[0] * a.tile[1] * a.tile[2] * 8 > 64 * 1024)
+        return fail(c, PM_ERR_UNSUPPORTED, "PME grid too fine for the cell size: spread tile exceeds 64 KB of LDS (use a smaller cellsize)");
       pme_launch.push_back(std::make_pair(a, hr.first));
     }
     if (f.corr) {
@@ -2980,6 +2987,13 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->gen++;   // captured steps hold the launch arguments
     return PM_OK;
   }
+  if (option == PM_OPT_PME_SPREAD) {
+    if (value < -1 || value > 1)
+      return fail(c, PM_ERR_ARGUMENT, "PME spread option must be -1 (automatic), 0 (cell bricks) or 1 (fused with the z transform; got %d)", value);
+    c->pme_spread_option = value == 0 ? 0 : 1;
+    c->gen++;   // captured steps hold the launch arguments
+    return PM_OK;
+  }
   if (option == PM_OPT_PME_FFT) {
     if (value < 0 || value > 2)
       return fail(c, PM_ERR_ARGUMENT, "PME FFT option must be 0 (automatic), 1 (separate passes) or 2 (plane kernel; got %d)", value);
@@ -3014,6 +3028,10 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_LIST_TEST) {
     *value = list_exact(c) ? 1 : 0;
+    return PM_OK;
+  }
+  if (option == PM_OPT_PME_SPREAD) {
+    *value = c->pme_spread_option;
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {

@@ -228,6 +228,14 @@ __device__ inline void grid_coord64(unsigned long long X, int n, int& ix, T& w) 
   w = (T)lo * (T)5.421010862427522e-20;  // 2^-64
 }
 
+// v mod n for v in [-n, 2n): stencil indices ix + off + i with ix in [0, n),
+// |off + i| <= 4 (orders up to 8) and n >= 8 (the host refuses smaller grids)
+__device__ inline int wrap_grid(int v, int n) {
+  if (v < 0) v += n;
+  if (v >= n) v -= n;
+  return v;
+}
+
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_spread(PmPmeLaunch a, const longlong4* __restrict__ pos64) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,16 +264,16 @@ __global__ __launch_bounds__(256) void k_spread(PmPmeLaunch a, const longlong4* 
   const T scale = (T)PM_GRID_FIX;
 #pragma unroll
   for (int i = 0; i < P; i++) {
-    const int gx = (ix + off + i + a.nx) % a.nx;
+    const int gx = wrap_grid(ix + off + i, a.nx);
     const T qa = q * tx[i];
 #pragma unroll
     for (int j = 0; j < P; j++) {
-      const int gy = (iy + off + j + a.ny) % a.ny;
+      const int gy = wrap_grid(iy + off + j, a.ny);
       const T qab = qa * ty[j];
       unsigned long long* row = a.grid_fix + ((long long)gx * a.ny + gy) * a.nz;
 #pragma unroll
       for (int k = 0; k < P; k++) {
-        const int gz = (iz + off + k + a.nz) % a.nz;
+        const int gz = wrap_grid(iz + off + k, a.nz);
         long long v;
         if constexpr (sizeof(T) == 4) v = __float2ll_rn(qab * tz[k] * scale);
         else v = __double2ll_rn(qab * tz[k] * scale);
@@ -352,17 +360,17 @@ __global__ __launch_bounds__(256) void k_spread_tile(PmPmeLaunch a, const longlo
     const T q = (T)a.q[at];
 #pragma unroll
     for (int i = 0; i < P; i++) {
-      const int gx = (ix + off + i + a.nx) % a.nx;
+      const int gx = wrap_grid(ix + off + i, a.nx);
       int lx = gx - t0[0]; if (lx < 0) lx += a.nx;
       const T qa = q * tx[i];
 #pragma unroll
       for (int j = 0; j < P; j++) {
-        const int gy = (iy + off + j + a.ny) % a.ny;
+        const int gy = wrap_grid(iy + off + j, a.ny);
         int ly = gy - t0[1]; if (ly < 0) ly += a.ny;
         const T qab = qa * ty[j];
 #pragma unroll
         for (int k = 0; k < P; k++) {
-          const int gz = (iz + off + k + a.nz) % a.nz;
+          const int gz = wrap_grid(iz + off + k, a.nz);
           int lz = gz - t0[2]; if (lz < 0) lz += a.nz;
           long long v;
           if constexpr (sizeof(T) == 4) v = __float2ll_rn(qab * tz[k] * scale);
@@ -415,17 +423,17 @@ __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* 
   T fx = 0, fy = 0, fz = 0;
 #pragma unroll
   for (int i = 0; i < P; i++) {
-    const int gx = (ix + off + i + a.nx) % a.nx;
+    const int gx = wrap_grid(ix + off + i, a.nx);
 #pragma unroll
     for (int j = 0; j < P; j++) {
-      const int gy = (iy + off + j + a.ny) % a.ny;
+      const int gy = wrap_grid(iy + off + j, a.ny);
       const T xij = dx[i] * ty[j];
       const T yij = tx[i] * dy[j];
       const T zij = tx[i] * ty[j];
       const T* row = g + ((long long)gx * a.ny + gy) * a.nz;
 #pragma unroll
       for (int k = 0; k < P; k++) {
-        const int gz = (iz + off + k + a.nz) % a.nz;
+        const int gz = wrap_grid(iz + off + k, a.nz);
         const T term = row[gz];
         fx -= term * xij * tz[k];
         fy -= term * yij * tz[k];
@@ -535,16 +543,16 @@ __global__ __launch_bounds__(256) void k_spread_dir(PmPmeLaunch a, const longlon
   const T scale = (T)PM_GRID_FIX;
 #pragma unroll
   for (int i = 0; i < P; i++) {
-    const int gx = (ix + off + i + a.nx) % a.nx;
+    const int gx = wrap_grid(ix + off + i, a.nx);
 #pragma unroll
     for (int j = 0; j < P; j++) {
-      const int gy = (iy + off + j + a.ny) % a.ny;
+      const int gy = wrap_grid(iy + off + j, a.ny);
       const T cxy = vx * dx[i] * ty[j] + vy * tx[i] * dy[j];
       const T cz = vz * tx[i] * ty[j];
       unsigned long long* row = a.grid_fix + ((long long)gx * a.ny + gy) * a.nz;
 #pragma unroll
       for (int k = 0; k < P; k++) {
-        const int gz = (iz + off + k + a.nz) % a.nz;
+        const int gz = wrap_grid(iz + off + k, a.nz);
         const T val = cxy * tz[k] + cz * dz[k];
         long long iv;
         if constexpr (sizeof(T) == 4) iv = __float2ll_rn(val * scale);
@@ -580,14 +588,14 @@ __global__ __launch_bounds__(256) void k_gather_adj(PmPmeLaunch a, const longlon
   T r0 = 0, r1 = 0, r2 = 0;
 #pragma unroll
   for (int i = 0; i < P; i++) {
-    const int gx = (ix + off + i + a.nx) % a.nx;
+    const int gx = wrap_grid(ix + off + i, a.nx);
 #pragma unroll
     for (int j = 0; j < P; j++) {
-      const int gy = (iy + off + j + a.ny) % a.ny;
+      const int gy = wrap_grid(iy + off + j, a.ny);
       const T* row = g + ((long long)gx * a.ny + gy) * a.nz;
 #pragma unroll
       for (int k = 0; k < P; k++) {
-        const int gz = (iz + off + k + a.nz) % a.nz;
+        const int gz = wrap_grid(iz + off + k, a.nz);
         const T phi = row[gz];
         if constexpr (MODE == 0) {
           r0 += phi * tx[i] * ty[j] * tz[k];
@@ -664,6 +672,174 @@ __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, i
       const int line = line0 + l;
       if (line < nl_tot) gc[(long long)line * nzc + k] = r[l * ls + k];
     }
+  }
+}
+
+// Spread fused with the z forward transform.  A block owns PM_SZ_LX x
+// PM_SZ_LY whole z lines of the grid: it walks the slots of the cell columns
+// whose atoms can reach those lines (the cells from the last layout build,
+// widened by how far an atom may have drifted from its cell since: a.drift_u
+// grid points), adds each atom's stencil points on its own lines into an LDS
+// accumulator of 64-bit fixed-point charges (integer adds: the same sums as
+// the spread kernels make, in any order), then converts the lines to real
+// values and transforms them along z as k_fft_z_fwd does.  Every grid point
+// is owned by one block, so there are no global atomics and no charge grid
+// to clear.
+#define PM_SZ_LX 4
+#define PM_SZ_LY 4
+#define PM_SZ_MAXCOL 256
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_spread_zfwd(PmPmeLaunch a, PmFftPlan p, const longlong4* __restrict__ pos64) {
+  pme_priority(a.prio);
+  typedef typename Cx<T>::t T2;
+  constexpr int LX = PM_SZ_LX, LY = PM_SZ_LY, TL = LX * LY;
+  extern __shared__ unsigned char s_raw[];
+  __shared__ int s_start[PM_SZ_MAXCOL], s_pre[PM_SZ_MAXCOL + 1];
+  __shared__ int s_ncol;
+  const int n = p.n, ls = n + 1, nzc = a.nzc;
+  T2* b0 = reinterpret_cast<T2*>(s_raw);
+  T2* b1 = b0 + TL * ls;
+  T2* tws = b1 + TL * ls;
+  // the accumulator shares b1's space (TL * n 8-byte words <= TL * ls T2)
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(b1);
+  load_twiddles(tws, p, a.tw);
+  const int nby = (a.ny + LY - 1) / LY;
+  const int x0 = (blockIdx.x / nby) * LX, y0 = (blockIdx.x % nby) * LY;
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) acc[e] = 0ull;
+  constexpr int off = -(P - 1) / 2;
+  if (threadIdx.x < 64) {
+    // cell columns: per axis the cells that meet the grid coordinates
+    // [x0 - off - P + 1, x0 + LX - off), widened by the drift margin (an atom
+    // whose stencil reaches one of the block's lines lies there)
+    const int lane = threadIdx.x;
+    int c0[2], nc[2];
+    const int g0[2] = {x0, y0}, lw[2] = {LX, LY};
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      const double ua = g0[d] - off - P + 1 - a.drift_u[d], ub = g0[d] + lw[d] - off + a.drift_u[d];
+      const int lo = (int)floor((ua - a.cell_lo_u[d]) / a.cell_du[d]);
+      const int hi = (int)floor((ub - a.cell_lo_u[d]) / a.cell_du[d]);
+      c0[d] = lo;
+      nc[d] = hi - lo + 1;
+      if (nc[d] >= a.ncell[d]) { c0[d] = 0; nc[d] = a.ncell[d]; }
+    }
+    const int ncol = nc[0] * nc[1];   // <= PM_SZ_MAXCOL (checked by the host)
+    const int nz = a.ncell[2];
+    int carry = 0;
+    for (int b = 0; b < ncol; b += 64) {
+      const int q = b + lane;
+      int st = 0, len = 0;
+      if (q < ncol) {
+        int cx = (c0[0] + q / nc[1]) % a.ncell[0], cy = (c0[1] + q % nc[1]) % a.ncell[1];
+        if (cx < 0) cx += a.ncell[0];
+        if (cy < 0) cy += a.ncell[1];
+        const int col = (cx * a.ncell[1] + cy) * nz;
+        st = a.cell_slot[col];
+        len = a.cell_slot[col + nz - 1] + a.cell_count[col + nz - 1] - st;
+        s_start[q] = st;
+      }
+      int inc = len;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      if (q < ncol) s_pre[q] = carry + inc - len;
+      carry += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) { s_pre[ncol] = carry; s_ncol = ncol; }
+  }
+  __syncthreads();
+  const int ncol = s_ncol, total = s_pre[ncol];
+  const T scale = (T)PM_GRID_FIX;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    // the column holding candidate t: the last prefix <= t
+    int lo = 0, hi = ncol - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int sl = s_start[lo] + (t - s_pre[lo]);
+    int ix, iy, iz;
+    T wx, wy, wz;
+    if constexpr (sizeof(T) == 4) {
+      const int4 q = a.slot_pos[sl];
+      grid_coord<T>((unsigned)q.x, a.nx, ix, wx);
+      grid_coord<T>((unsigned)q.y, a.ny, iy, wy);
+      grid_coord<T>((unsigned)q.z, a.nz, iz, wz);
+    } else {
+      const longlong4 q = pos64[sl];
+      grid_coord64<T>((unsigned long long)q.x, a.nx, ix, wx);
+      grid_coord64<T>((unsigned long long)q.y, a.ny, iy, wy);
+      grid_coord64<T>((unsigned long long)q.z, a.nz, iz, wz);
+    }
+    // the stencil's x and y points on this block's lines: offsets from the
+    // block's first line, mod n
+    int dx0 = ix + off - x0, dy0 = iy + off - y0;   // in (-n - P, n)
+    if (dx0 < 0) dx0 += a.nx;
+    if (dx0 < 0) dx0 += a.nx;
+    if (dy0 < 0) dy0 += a.ny;
+    if (dy0 < 0) dy0 += a.ny;
+    int lxs[P], lys[P];
+    unsigned int mx = 0, my = 0;
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+      lxs[i] = dx0 + i >= a.nx ? dx0 + i - a.nx : dx0 + i;
+      if (lxs[i] < LX) mx |= 1u << i;
+      lys[i] = dy0 + i >= a.ny ? dy0 + i - a.ny : dy0 + i;
+      if (lys[i] < LY) my |= 1u << i;
+    }
+    if (mx == 0 || my == 0) continue;
+    const int at = a.atom_of_slot[sl];
+    T tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+    bspline<T, P>(wx, tx, dx);
+    bspline<T, P>(wy, ty, dy);
+    bspline<T, P>(wz, tz, dz);
+    const T q = (T)a.q[at];
+    int gzs[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) gzs[k] = wrap_grid(iz + off + k, a.nz);
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+      if (!((mx >> i) & 1u)) continue;
+      const T qa = q * tx[i];
+#pragma unroll
+      for (int j = 0; j < P; j++) {
+        if (!((my >> j) & 1u)) continue;
+        const T qab = qa * ty[j];
+        unsigned long long* line = acc + (lxs[i] * LY + lys[j]) * n;
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          long long v;
+          if constexpr (sizeof(T) == 4) v = __float2ll_rn(qab * tz[k] * scale);
+          else v = __double2ll_rn(qab * tz[k] * scale);
+          atomicAdd(line + gzs[k], (unsigned long long)v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // real lines into b0 (the accumulator in b1 is read first, then b1 is free)
+  for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
+    const int l = e / n, k = e - l * n;
+    T2 v;
+    v.x = (T)((double)(long long)acc[e] * PM_GRID_UNFIX);
+    v.y = 0;
+    b0[l * ls + k] = v;
+  }
+  __syncthreads();
+  T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, -1, ls);
+  T2* gc = reinterpret_cast<T2*>(a.grid_c);
+  const int nl_tot = a.nx * a.ny;
+  for (int e = threadIdx.x; e < TL * nzc; e += blockDim.x) {
+    int l, k;
+    if (a.plane) { k = e / TL; l = e - k * TL; }
+    else { l = e / nzc; k = e - l * nzc; }
+    const int gx = x0 + l / LY, gy = y0 + l % LY;
+    if (gx >= a.nx || gy >= a.ny) continue;
+    const int line = gx * a.ny + gy;
+    if (a.plane) gc[(long long)k * nl_tot + line] = r[l * ls + k];
+    else gc[(long long)line * nzc + k] = r[l * ls + k];
   }
 }
 
@@ -978,10 +1154,10 @@ static int plane_threads(const PmFftPlan& px, const PmFftPlan& py) {
 // them in one kernel per kz plane, then z inverse: the fixed-point charge
 // grid to the real potential grid
 template <typename T>
-static void launch_fft_chain(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans) {
+static void launch_fft_chain(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, bool z_done = false) {
   const size_t el = sizeof(typename Cx<T>::t);
   const int nthreads = 256;
-  {
+  if (!z_done) {
     const int tl = lines_per_block(a.nz, el);
     const int nb = (a.nx * a.ny + tl - 1) / tl;
     hipLaunchKernelGGL(k_fft_z_fwd<T>, dim3(nb), dim3(nthreads), zpass_lds(a.nz, tl, el), s, a, plans[2], tl);
@@ -1023,8 +1199,16 @@ static void launch_spread_tile(hipStream_t s, const PmPmeLaunch& a, const longlo
 template <typename T, int P>
 static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
                          const longlong4* pos64, const PmAtomTail* tail) {
-  launch_spread_tile<T, P>(s, a, pos64);
-  launch_fft_chain<T>(s, a, plans);
+  if (a.spread_zfwd) {
+    const size_t el = sizeof(typename Cx<T>::t);
+    const int nb = ((a.nx + PM_SZ_LX - 1) / PM_SZ_LX) * ((a.ny + PM_SZ_LY - 1) / PM_SZ_LY);
+    hipLaunchKernelGGL((k_spread_zfwd<T, P>), dim3(nb), dim3(256), zpass_lds(a.nz, PM_SZ_LX * PM_SZ_LY, el), s, a,
+                       plans[2], pos64);
+    launch_fft_chain<T>(s, a, plans, true);
+  } else {
+    launch_spread_tile<T, P>(s, a, pos64);
+    launch_fft_chain<T>(s, a, plans);
+  }
   const dim3 g((a.nslot_max + 255) / 256), b(256);
   if (tail) hipLaunchKernelGGL((k_gather<T, P, true>), g, b, 0, s, a, pos64, *tail);
   else hipLaunchKernelGGL((k_gather<T, P, false>), g, b, 0, s, a, pos64, PmAtomTail{});
@@ -1047,6 +1231,21 @@ static void launch_pme_adj_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPla
   hipLaunchKernelGGL((k_spread_dir<T, P>), g, b, 0, s, a, pos64, slot_v);
   launch_fft_chain<T>(s, a, plans);
   hipLaunchKernelGGL((k_gather_adj<T, P, 2>), g, b, 0, s, a, pos64, slot_v, out);
+}
+
+int pm_pme_spread_zfwd_fits(const PmPmeLaunch& a, int precision) {
+  const size_t lds = zpass_lds(a.nz, PM_SZ_LX * PM_SZ_LY, fft_elem(precision));
+  if (lds + sizeof(int) * (2 * PM_SZ_MAXCOL + 2) > 64 * 1024) return 0;
+  const int g[2] = {PM_SZ_LX, PM_SZ_LY};
+  long long cols = 1;
+  for (int d = 0; d < 2; d++) {
+    const double w = g[d] + a.order - 1 + 2.0 * a.drift_u[d];
+    // the kernel's range spans at most floor(w / du) + 2 cells; one more for rounding
+    long long c = (long long)floor(w / a.cell_du[d]) + 3;
+    if (c > a.ncell[d]) c = a.ncell[d];
+    cols *= c;
+  }
+  return cols <= PM_SZ_MAXCOL ? 1 : 0;
 }
 
 int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision, int plane) {
