@@ -935,8 +935,8 @@ __device__ inline void store_block_energy(const PmPmeLaunch& a, const PmERed& s,
   }
 }
 
-// y pass (either direction) on the line-major grid: lines (x, kz) with TL
-// consecutive kz per block
+// y pass (either direction) on the line-major grid: the lines (x, kz) in
+// x-major order, TL consecutive lines per block (a block may span two x)
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int TL, int sign) {
   pme_priority(a.prio);
@@ -947,26 +947,34 @@ __global__ __launch_bounds__(256) void k_fft_y(PmPmeLaunch a, PmFftPlan p, int T
   T2* tws = b1 + TL * p.n;
   load_twiddles(tws, p, a.tw);
   const int n = p.n;  // ny
-  const int ntile = (a.nzc + TL - 1) / TL;
-  const int x = blockIdx.x / ntile;
-  const int kz0 = (blockIdx.x - x * ntile) * TL;
+  const int nl = a.nx * a.nzc, line0 = blockIdx.x * TL;
+  __shared__ long long s_off[64];   // grid offset of each line's y = 0 point (-1: past the grid)
+  if (threadIdx.x < TL) {
+    const int line = line0 + threadIdx.x;
+    const int x = line / a.nzc, kz = line - x * a.nzc;
+    s_off[threadIdx.x] = line < nl ? (long long)x * a.ny * a.nzc + kz : -1;
+  }
+  __syncthreads();
   T2* gc = reinterpret_cast<T2*>(a.grid_c);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int y = e / TL, l = e - y * TL;
+    const long long o = s_off[l];
     T2 v; v.x = 0; v.y = 0;
-    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
+    if (o >= 0) v = gc[o + (long long)y * a.nzc];
     b0[l * n + y] = v;
   }
   __syncthreads();
   T2* r = fft_lines<T, T2>(b0, b1, TL, p, tws, sign, n);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int y = e / TL, l = e - y * TL;
-    if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r[l * n + y];
+    const long long o = s_off[l];
+    if (o >= 0) gc[o + (long long)y * a.nzc] = r[l * n + y];
   }
 }
 
 // x pass with the influence function on the line-major grid: forward
-// transform, multiply by theta(m), inverse transform.
+// transform, multiply by theta(m), inverse transform; the lines (y, kz) in
+// y-major order, TL consecutive lines per block
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, int TL) {
   pme_priority(a.prio);
@@ -978,14 +986,22 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
   T2* tws = b1 + TL * p.n;
   load_twiddles(tws, p, a.tw);
   const int n = p.n;  // nx
-  const int ntile = (a.nzc + TL - 1) / TL;
-  const int y = blockIdx.x / ntile;
-  const int kz0 = (blockIdx.x - y * ntile) * TL;
+  const int nl = a.ny * a.nzc, line0 = blockIdx.x * TL;
+  __shared__ int s_y[64], s_kz[64];   // each line's y (-1: past the grid) and kz
+  if (threadIdx.x < TL) {
+    const int line = line0 + threadIdx.x;
+    const int y = line / a.nzc;
+    s_y[threadIdx.x] = line < nl ? y : -1;
+    s_kz[threadIdx.x] = line - y * a.nzc;
+  }
+  __syncthreads();
+  const long long xs = (long long)a.ny * a.nzc;   // grid stride of x
   T2* gc = reinterpret_cast<T2*>(a.grid_c);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int x = e / TL, l = e - x * TL;
+    const int y = s_y[l];
     T2 v; v.x = 0; v.y = 0;
-    if (kz0 + l < a.nzc) v = gc[((long long)x * a.ny + y) * a.nzc + kz0 + l];
+    if (y >= 0) v = gc[x * xs + (long long)y * a.nzc + s_kz[l]];
     b0[l * n + x] = v;
   }
   __syncthreads();
@@ -994,9 +1010,9 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
     PmERed s = {0, 0, 0, 0, 0, 0, 0};
     for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
       const int l = e / n, kx = e - l * n;
-      const int kz = kz0 + l;
-      if (kz >= a.nzc) continue;
-      r[e] = influence<T, T2>(a, kx, y, kz, r[e], s);
+      const int y = s_y[l];
+      if (y < 0) continue;
+      r[e] = influence<T, T2>(a, kx, y, s_kz[l], r[e], s);
     }
     store_block_energy(a, s, s_red);   // its barrier also orders the writes to r[]
   } else {
@@ -1004,9 +1020,9 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
     const T* th = reinterpret_cast<const T*>(a.theta);
     for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
       const int l = e / n, kx = e - l * n;
-      const int kz = kz0 + l;
-      if (kz >= a.nzc) continue;
-      const T t = th[((long long)kx * a.ny + y) * a.nzc + kz];
+      const int y = s_y[l];
+      if (y < 0) continue;
+      const T t = th[kx * xs + (long long)y * a.nzc + s_kz[l]];
       r[e].x *= t;
       r[e].y *= t;
     }
@@ -1015,7 +1031,8 @@ __global__ __launch_bounds__(256) void k_fft_x_conv(PmPmeLaunch a, PmFftPlan p, 
   T2* r2 = fft_lines<T, T2>(r, r == b0 ? b1 : b0, TL, p, tws, +1, n);
   for (int e = threadIdx.x; e < TL * n; e += blockDim.x) {
     const int x = e / TL, l = e - x * TL;
-    if (kz0 + l < a.nzc) gc[((long long)x * a.ny + y) * a.nzc + kz0 + l] = r2[l * n + x];
+    const int y = s_y[l];
+    if (y >= 0) gc[x * xs + (long long)y * a.nzc + s_kz[l]] = r2[l * n + x];
   }
 }
 
@@ -1168,17 +1185,17 @@ static void launch_fft_chain(hipStream_t s, const PmPmeLaunch& a, const PmFftPla
   } else {
     {
       const int tl = lines_per_block(a.ny, el);
-      const int nb = a.nx * ((a.nzc + tl - 1) / tl);
+      const int nb = (a.nx * a.nzc + tl - 1) / tl;
       hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), line_lds(a.ny, tl, el), s, a, plans[1], tl, -1);
     }
     {
       const int tl = lines_per_block(a.nx, el);
-      const int nb = a.ny * ((a.nzc + tl - 1) / tl);
+      const int nb = (a.ny * a.nzc + tl - 1) / tl;
       hipLaunchKernelGGL(k_fft_x_conv<T>, dim3(nb), dim3(nthreads), line_lds(a.nx, tl, el), s, a, plans[0], tl);
     }
     {
       const int tl = lines_per_block(a.ny, el);
-      const int nb = a.nx * ((a.nzc + tl - 1) / tl);
+      const int nb = (a.nx * a.nzc + tl - 1) / tl;
       hipLaunchKernelGGL(k_fft_y<T>, dim3(nb), dim3(nthreads), line_lds(a.ny, tl, el), s, a, plans[1], tl, +1);
     }
   }
@@ -1252,8 +1269,7 @@ int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision, int plane) {
   const int nzc = nz / 2 + 1;
   if (plane) return nzc;
   const int tl = lines_per_block(nx, fft_elem(precision));
-  (void)ny;
-  return ny * ((nzc + tl - 1) / tl);
+  return (ny * nzc + tl - 1) / tl;
 }
 
 template <typename T>
