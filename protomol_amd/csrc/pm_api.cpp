@@ -647,6 +647,7 @@ bool fft_plan(int n, PmFftPlan& p) {
   while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; if (p.nst >= PM_FFT_MAXST) return false; }
   while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; if (p.nst >= PM_FFT_MAXST) return false; }
   while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; if (p.nst >= PM_FFT_MAXST) return false; }
+  while (m % 7 == 0) { p.rad[p.nst++] = 7; m /= 7; if (p.nst >= PM_FFT_MAXST) return false; }
   return m == 1;
 }
 
@@ -662,7 +663,7 @@ int ensure_pme(pm_context* c, ForceDesc& f) {
     if (nx > 4096 || ny > 4096 || nz > 4096 || nx < 8 || ny < 8 || nz < 8)
       return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: 8 to 4096 points per axis", nx, ny, nz);
     if (!fft_plan(nx, c->plans[0]) || !fft_plan(ny, c->plans[1]) || !fft_plan(nz, c->plans[2]))
-      return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: the GPU FFT handles lengths with factors 2, 3 and 5 only", nx, ny, nz);
+      return fail(c, PM_ERR_UNSUPPORTED, "PME grid %d x %d x %d: the GPU FFT handles lengths with factors 2, 3, 5 and 7 only", nx, ny, nz);
     size_t ng = (size_t)nx * ny * nz, ngc = (size_t)nx * ny * nzc;
     size_t er = c->precision == PM_PRECISION_DOUBLE ? sizeof(double) : sizeof(float);
     int r;

@@ -11,7 +11,7 @@
 //             theta(m), inverse) while the line sits in LDS;
 //   inverse = fftForward, exp(+i), C2C along y, C2R along z;
 //   gather  = Grid::interpolateForce (:193-228), one thread per atom.
-// The 1D transforms are Stockham autosort mixed-radix (2, 3, 4, 5) passes
+// The 1D transforms are Stockham autosort mixed-radix (2, 3, 4, 5, 7) passes
 // over lines staged in LDS.
 #include "pm_internal.h"
 #include "pm_terms.h"
@@ -84,6 +84,32 @@ template <typename T, typename T2> struct Dft<T, T2, 5> {
   }
 };
 
+// the pairs (j, 7 - j) share cos(2 pi j k / 7) and flip sin(2 pi j k / 7):
+// out_k = v0 + sum_j a_j cos + sign i sum_j b_j sin, out_(7-k) with -sin,
+// a_j = v_j + v_(7-j), b_j = v_j - v_(7-j), j = 1..3 (jk mod 7 folded to 1..3)
+template <typename T, typename T2> struct Dft<T, T2, 7> {
+  static __device__ inline void run(T2* v, int sign) {
+    const T c1 = (T)0.62348980185873353053, c2 = (T)-0.22252093395631440429, c3 = (T)-0.90096886790241912624;
+    const T s1 = (T)0.78183148246802980871, s2 = (T)0.97492791218182360702, s3 = (T)0.43388373911755812048;
+    const T2 x0 = v[0];
+    const T2 a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+    const T2 a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+    const T2 a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+    T2 p1, p2, p3, m1, m2, m3;
+    p1.x = x0.x + c1 * a1.x + c2 * a2.x + c3 * a3.x; p1.y = x0.y + c1 * a1.y + c2 * a2.y + c3 * a3.y;
+    p2.x = x0.x + c2 * a1.x + c3 * a2.x + c1 * a3.x; p2.y = x0.y + c2 * a1.y + c3 * a2.y + c1 * a3.y;
+    p3.x = x0.x + c3 * a1.x + c1 * a2.x + c2 * a3.x; p3.y = x0.y + c3 * a1.y + c1 * a2.y + c2 * a3.y;
+    m1.x = s1 * b1.x + s2 * b2.x + s3 * b3.x;       m1.y = s1 * b1.y + s2 * b2.y + s3 * b3.y;
+    m2.x = s2 * b1.x - s3 * b2.x - s1 * b3.x;       m2.y = s2 * b1.y - s3 * b2.y - s1 * b3.y;
+    m3.x = s3 * b1.x - s1 * b2.x + s2 * b3.x;       m3.y = s3 * b1.y - s1 * b2.y + s2 * b3.y;
+    const T2 q1 = mul_si(m1, sign), q2 = mul_si(m2, sign), q3 = mul_si(m3, sign);
+    v[0].x = x0.x + a1.x + a2.x + a3.x; v[0].y = x0.y + a1.y + a2.y + a3.y;
+    v[1] = cadd(p1, q1); v[6] = csub(p1, q1);
+    v[2] = cadd(p2, q2); v[5] = csub(p2, q2);
+    v[3] = cadd(p3, q3); v[4] = csub(p3, q3);
+  }
+};
+
 // Integer division by a block-uniform d >= 1 through a multiply-high:
 // floor(v / d) = umulhi(v, ceil(2^32 / d)), exact while v * d <= 2^32
 // (the host keeps FFT lengths and line counts within 4096, so every index
@@ -152,7 +178,8 @@ __device__ T2* fft_lines(T2* b0, T2* b1, int nl, const PmFftPlan& p, const T2* t
       case 2: fft_stage<T, T2, 2, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
       case 3: fft_stage<T, T2, 3, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
       case 4: fft_stage<T, T2, 4, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
-      default: fft_stage<T, T2, 5, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
+      case 5: fft_stage<T, T2, 5, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
+      default: fft_stage<T, T2, 7, LF>(in, out, nl, n, ls, es, Ns, tw, sign); break;
     }
     __syncthreads();
     T2* tmp = in; in = out; out = tmp;

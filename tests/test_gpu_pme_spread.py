@@ -36,7 +36,8 @@ def ctx(factory, s, precision, cell, spread):
 
 
 # (cell 9.31, two cells per side: the brick path's tiles fit only the order-4 24^3 grid)
-CASES = [(g, o, c) for g, o in [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8), ((18, 20, 27), 4)]
+CASES = [(g, o, c) for g, o in [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8), ((18, 20, 27), 4),
+                                ((21, 28, 14), 4)]
          for c in (2.6, 4.0)] + [((24, 24, 24), 4, 9.31)]
 
 
