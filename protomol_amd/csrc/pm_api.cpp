@@ -7,6 +7,7 @@
 #include "pm_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -210,6 +211,9 @@ struct pm_context {
   void* d_grid_r = nullptr;
   void* d_grid_c = nullptr;
   void* d_theta = nullptr;   // theta(m) on the line-major half grid (grid precision)
+  double* d_lat = nullptr;   // lattice vectors of the PME real term in small boxes
+  int nlat = 0;
+  double lat_key[4] = {0, 0, 0, 0};   // rc and box edges d_lat was built for
   double *d_bmod = nullptr, *d_expt = nullptr, *d_tw = nullptr;
   PmFftPlan plans[3];
   double pme_alpha_tables = -1;
@@ -916,6 +920,48 @@ struct PairPass {
 // runs as one pass per term: each switched term is zero beyond its own
 // cutoff, so the sum of the two passes is the OneAtomPairTwo evaluation, and
 // the pair kernel never clips a term below its pass's cutoff.
+// blocks (and energy records) of the lattice real-term kernel
+int lattice_blocks(int n) { return std::max(1, std::min(n, 512)); }
+
+// the PME real term needs lattice images beyond the minimal one when a box
+// edge is at most twice the cutoff (NonbondedPMEwaldSystemForce.h:630-631)
+bool pme_lattice(const pm_context* c, const ForceDesc& f) {
+  const double lmin = std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
+  return lmin <= 2.0 * f.rc;
+}
+
+// lattice vectors of norm <= 2 rc (PeriodicBoundaryConditions.cpp:80-111),
+// (0,0,0) first, then by norm; equal norms keep the generation order (the
+// reference's std::sort leaves their order unspecified, which matters only
+// for a MODIFIED pair whose primary image lies beyond the cutoff)
+int ensure_lattice(pm_context* c, double rc) {
+  const double key[4] = {rc, c->box.L[0], c->box.L[1], c->box.L[2]};
+  if (c->d_lat && std::equal(key, key + 4, c->lat_key)) return PM_OK;
+  const double lx = c->box.L[0], ly = c->box.L[1], lz = c->box.L[2];
+  const int bk = (int)std::floor(rc / lx + 1.0), bl = (int)std::floor(rc / ly + 1.0), bm = (int)std::floor(rc / lz + 1.0);
+  std::vector<std::array<double, 3>> v;
+  for (int k = -bk; k <= bk; k++)
+    for (int l = -bl; l <= bl; l++)
+      for (int m = -bm; m <= bm; m++) {
+        if (k == 0 && l == 0 && m == 0) continue;
+        const double x = lx * k, y = ly * l, z = lz * m;
+        if (std::sqrt(x * x + y * y + z * z) <= 2 * rc) v.push_back({x, y, z});
+      }
+  std::stable_sort(v.begin(), v.end(), [](const std::array<double, 3>& a, const std::array<double, 3>& b) {
+    return a[0] * a[0] + a[1] * a[1] + a[2] * a[2] < b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  });
+  if (v.size() + 1 > 1024)
+    return fail(c, PM_ERR_UNSUPPORTED, "PME real term: %d lattice images within 2 rc (at most 1024; the box is too small "
+                                       "for the cutoff)", (int)v.size() + 1);
+  std::vector<double> flat = {0.0, 0.0, 0.0};
+  for (auto& a : v) flat.insert(flat.end(), a.begin(), a.end());
+  int r;
+  if ((r = upload(c, &c->d_lat, flat))) return r;
+  c->nlat = (int)v.size() + 1;
+  std::copy(key, key + 4, c->lat_key);
+  return PM_OK;
+}
+
 std::vector<PairPass> plan_passes(pm_context* c, const std::vector<int>& handles) {
   std::vector<PairPass> passes;
   std::vector<bool> used(handles.size(), false);
@@ -948,10 +994,11 @@ std::vector<PairPass> plan_passes(pm_context* c, const std::vector<int>& handles
     }
     passes.push_back(p);
   }
-  // PME real-space terms run as a CoulombEwaldReal / Cutoff pair pass
+  // PME real-space terms run as a CoulombEwaldReal / Cutoff pair pass (in
+  // boxes no wider than twice the cutoff, k_ewald_lattice instead)
   for (size_t i = 0; i < handles.size(); i++) {
     ForceDesc& f = c->forces[handles[i]];
-    if (!f.pme || !f.real) continue;
+    if (!f.pme || !f.real || pme_lattice(c, f)) continue;
     Term t;
     t.pot = 2; t.swf = PM_SWF_CUTOFF; t.alpha = f.alpha; t.cutoff = f.rc;
     passes.push_back(PairPass{{t}, f.rc});
@@ -1052,12 +1099,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     ForceDesc& f = c->forces[h];
     if (f.pme) {
       if ((r = ensure_pme(c, f))) return r;
-      if (f.real) {
-        // realTerm lattice images |n| <= 2 rc (PeriodicBoundaryConditions.cpp:80-111)
-        double lmin = std::min(c->box.L[0], std::min(c->box.L[1], c->box.L[2]));
-        if (lmin <= 2.0 * f.rc)
-          return fail(c, PM_ERR_UNSUPPORTED, "PME -real with box edge %g <= 2*cutoff: lattice images beyond the minimal image are not implemented on the GPU", lmin);
-      }
+      if (f.real && pme_lattice(c, f) && (r = ensure_lattice(c, f.rc))) return r;
     }
   }
   std::vector<PairPass> passes = plan_passes(c, handles);
@@ -1120,6 +1162,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     long long off = rec;
     if (f.recip) rec += pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision, pme_plane(c, f));
     if (f.corr) rec += ((long long)c->tab_a.size() + 255) / 256;
+    if (f.real && pme_lattice(c, f)) rec += lattice_blocks(n);
     pme_rec.push_back(std::make_pair(h, off));
   }
   bool do_bond = false, do_angle = false, do_tors[2] = {false, false};
@@ -1399,6 +1442,16 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
                          f.alpha, c->csf, vir ? 1 : 0, c->d_force, ntab, c->d_tab_a, c->d_tab_b, c->d_tab_c,
                          c->d_epart + off * PM_NE, want_energy ? nblk : 0);
   }
+  for (auto& hr : pme_rec) {
+    ForceDesc& f = c->forces[hr.first];
+    if (!f.real || !pme_lattice(c, f)) continue;
+    long long off = hr.second + (f.recip ? pm_pme_xpass_blocks(f.nx, f.ny, f.nz, c->precision, pme_plane(c, f)) : 0) +
+                    (f.corr ? ((long long)c->tab_a.size() + 255) / 256 : 0);
+    TimedScope ts(c, PM_T_PME, s2);
+    pm_launch_ewald_lattice(s2, c->box, n, c->d_pos, c->d_q, c->d_xs_row, c->d_xs_part, c->d_lat, c->nlat,
+                            f.rc * f.rc, f.alpha, c->csf, vir ? 1 : 0, c->d_force, c->d_epart + off * PM_NE,
+                            lattice_blocks(n));
+  }
   if (any_bonded && !(fuse && !want_energy)) {
     PmBondedLaunch bl = b;
     if (fuse) bl.n = 0;   // the forces came with the gather: energies only
@@ -1630,6 +1683,8 @@ void free_all(pm_context* c) {
   if (c->d_grid_c) hipFree(c->d_grid_c);
   if (c->d_theta) hipFree(c->d_theta);
   c->d_grid_r = c->d_grid_c = c->d_theta = nullptr;
+  dfree(c->d_lat);
+  c->nlat = 0;
   dfree(c->d_bmod); dfree(c->d_expt); dfree(c->d_tw);
   dfree(c->d_b_a); dfree(c->d_b_b); dfree(c->d_b_row); dfree(c->d_b_idx); dfree(c->d_b_k); dfree(c->d_b_r0);
   dfree(c->d_a_a); dfree(c->d_a_b); dfree(c->d_a_c); dfree(c->d_a_row); dfree(c->d_a_idx);

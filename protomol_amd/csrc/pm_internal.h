@@ -302,6 +302,14 @@ int pm_pme_plane_fits(int nx, int ny, int precision, int lds_max);
 bool pm_launch_pme_adjoint(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
                            int precision, int mode, const double4* slot_v, double* out);
 
+// PME real-space term with lattice images, for boxes no wider than twice the
+// cutoff: lat = 3 * nlat lattice vectors ((0,0,0) first, nlat <= 1024), one
+// energy record per block (epart may be null: no records)
+void pm_launch_ewald_lattice(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
+                             const int* xs_row, const int* xs_part, const double* lat, int nlat, double rc2,
+                             double alpha, double csf, int want_virial, long long* force_fix, double* epart,
+                             int nblk);
+
 // Correction (excluded pairs), reductions, conversions and integrators
 void pm_launch_correction(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
                           const int* row, const int* partner, const int* cls, double alpha, double csf,

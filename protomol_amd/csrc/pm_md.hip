@@ -113,6 +113,101 @@ __global__ __launch_bounds__(256) void k_corr_energy(PmBox box, const double* __
   }
 }
 
+// PME real-space term in boxes no wider than twice the cutoff
+// (NonbondedPMEwaldSystemForce.h:658-768): every unordered pair i <= j (self
+// pairs included) against every lattice vector of `lat` ((0,0,0) first, then
+// the vectors of norm <= 2 rc by norm, PeriodicBoundaryConditions.cpp:80-111):
+// the first image of a FULL-excluded pair or of a self pair is skipped, a
+// MODIFIED pair takes coulombScalingFactor on its first image within the
+// cutoff, images beyond the cutoff contribute nothing (Cutoff switching), the
+// erfc by poly5 (mathutilities.cpp:36-40).  FP64 in both precisions: forces
+// into the 64-bit fixed-point atom forces, energy and virial into one record
+// per block.
+#define PM_LAT_MAX 1024
+__device__ inline double poly5_ref(double ar) {
+  const double t = 1.0 / (1.0 + 0.3275911 * ar);
+  return t * (0.254829592 + t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+}
+__global__ __launch_bounds__(256) void k_ewald_lattice(PmBox box, int n, const double* __restrict__ pos,
+                                                       const double* __restrict__ q, const int* __restrict__ xs_row,
+                                                       const int* __restrict__ xs_part, const double* __restrict__ lat,
+                                                       int nlat, double rc2, double alpha, double csf,
+                                                       int want_virial, long long* __restrict__ force_fix,
+                                                       double* __restrict__ epart) {
+  __shared__ double s_lat[3 * PM_LAT_MAX];
+  __shared__ double s_red[4][8];
+  for (int t = threadIdx.x; t < 3 * nlat; t += blockDim.x) s_lat[t] = lat[t];
+  __syncthreads();
+  const double tapi = 2.0 * alpha / sqrt(3.14159265358979323846);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double ev[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const double qi = q[i];
+    double fi[3] = {0, 0, 0};
+    for (int j = i + threadIdx.x; j < n; j += blockDim.x) {
+      double d[3], r2;
+      min_diff(box, pos + 3 * i, pos + 3 * j, d, r2);
+      int excl = PM_EXCL_NONE;
+      if (i == j) excl = PM_EXCL_FULL;
+      else
+        for (int u = xs_row[i]; u < xs_row[i + 1]; u++)
+          if ((xs_part[u] & 0xffffff) == j) { excl = xs_part[u] >> 24; break; }
+      double fj[3] = {0, 0, 0};
+      for (int t = 0; t < nlat; t++) {
+        if (excl == PM_EXCL_FULL) { excl = PM_EXCL_NONE; continue; }
+        const double rij[3] = {d[0] + s_lat[3 * t], d[1] + s_lat[3 * t + 1], d[2] + s_lat[3 * t + 2]};
+        const double rs = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+        if (rs > rc2) continue;
+        const double r = sqrt(rs);
+        double qq = qi * q[j];
+        if (excl == PM_EXCL_MODIFIED) qq *= csf;
+        const double rr = 1.0 / r, ar = alpha * r;
+        const double e = qq * exp(-ar * ar);
+        const double energy = poly5_ref(ar) * e * rr;
+        const double force = (energy + tapi * e) * rr * rr;
+        const double fij[3] = {rij[0] * force, rij[1] * force, rij[2] * force};
+        fi[0] -= fij[0]; fi[1] -= fij[1]; fi[2] -= fij[2];
+        fj[0] += fij[0]; fj[1] += fij[1]; fj[2] += fij[2];
+        ev[0] += energy;
+        if (want_virial) {
+          // ScalarStructure::addVirial (base/ScalarStructure.cpp:41-63)
+          ev[1] += fij[0] * rij[0]; ev[2] += fij[0] * rij[1]; ev[3] += fij[0] * rij[2];
+          ev[4] += fij[1] * rij[1]; ev[5] += fij[1] * rij[2]; ev[6] += fij[2] * rij[2];
+        }
+        excl = PM_EXCL_NONE;
+      }
+      for (int k = 0; k < 3; k++) {
+        const long long v = pm_force_fix(fj[k], box.err);
+        if (v != 0) atomicAdd((unsigned long long*)(force_fix + 3LL * j + k), (unsigned long long)v);
+      }
+    }
+    // the i force: fixed point per thread, integer sum over the block
+    for (int k = 0; k < 3; k++) {
+      long long v = pm_force_fix(fi[k], box.err);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0 && v != 0) atomicAdd((unsigned long long*)(force_fix + 3LL * i + k), (unsigned long long)v);
+    }
+  }
+  for (int u = 0; u < 7; u++) {
+    double x = ev[u];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) s_red[wv][u] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && epart) {
+    double s[7];
+    for (int u = 0; u < 7; u++) s[u] = s_red[0][u] + s_red[1][u] + s_red[2][u] + s_red[3][u];
+    double* ep = epart + (long long)blockIdx.x * PM_NE;
+    ep[PM_E_COUL] = s[0];
+    for (int u = 1; u < PM_E_VIR; u++) ep[u] = 0.0;
+    ep[PM_E_VIR + 0] = s[1]; ep[PM_E_VIR + 1] = s[2]; ep[PM_E_VIR + 2] = s[3];
+    ep[PM_E_VIR + 3] = s[2]; ep[PM_E_VIR + 4] = s[4]; ep[PM_E_VIR + 5] = s[5];
+    ep[PM_E_VIR + 6] = s[3]; ep[PM_E_VIR + 7] = s[5]; ep[PM_E_VIR + 8] = s[6];
+  }
+}
+
 // Deterministic two-stage sum of nparts energy records: stage 1 sums fixed
 // contiguous chunks (one block each), stage 2 sums the chunk totals in order
 // and adds the constant terms.
@@ -373,6 +468,15 @@ void pm_launch_corr_adjoint(hipStream_t s, const PmBox& box, int n, const double
   if (n > 0)
     hipLaunchKernelGGL(k_corr_adjoint, grid_for(n, 256), dim3(256), 0, s, box, n, pos, q, row, partner, cls, alpha,
                        csf, mode, v, out);
+}
+
+void pm_launch_ewald_lattice(hipStream_t s, const PmBox& box, int n, const double* pos, const double* q,
+                             const int* xs_row, const int* xs_part, const double* lat, int nlat, double rc2,
+                             double alpha, double csf, int want_virial, long long* force_fix, double* epart,
+                             int nblk) {
+  if (n > 0 && nblk > 0)
+    hipLaunchKernelGGL(k_ewald_lattice, dim3(nblk), dim3(256), 0, s, box, n, pos, q, xs_row, xs_part, lat, nlat, rc2,
+                       alpha, csf, want_virial, force_fix, epart);
 }
 
 void pm_launch_add_fix_to(hipStream_t s, int m, const long long* fixed, double* out) {

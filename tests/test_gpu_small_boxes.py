@@ -210,3 +210,34 @@ def test_small_box_list_reuse_follows_rebuild_every_step(gpu_ctx_factory, prune)
         if prune:
             assert out[600][3] > 0
         assert dx < 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-10), ("double", 1e-11)])
+@pytest.mark.parametrize("nmol,edge", [(27, 9.3), (100, 14.4)])
+def test_pme_real_term_lattice_images(gpu_ctx_factory, port, nmol, edge, precision, tol):
+    """The PME real term with a box no wider than twice the cutoff (rc 8):
+    pairs at every lattice image within the cutoff and each atom against its
+    own images (NonbondedPMEwaldSystemForce.h:658-768), through k_ewald_lattice
+    (FP64 in both precisions), every exclusion class; the real term alone and
+    the full PME force against the oracle (test_oracle.py pins the oracle's
+    lattice term against the compiled reference)."""
+    from tests.cases import exclusion_mix
+    say(f"case: pme real lattice images {nmol} waters {edge} A {precision}")
+    s = exclusion_mix(water.water_box(nmol, edge, seed=6, jitter=0.3))
+    c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)
+    args = [16, 16, 16, 8.0, 4, 1e-6, -1.0]
+    hr = c.force("Coulomb -algorithm PMEwald -real -interpolation BSpline", args)
+    f, e = c.evaluate([hr], s.pos, virial=True)
+    o = port.pme(s, (16, 16, 16), 4, 8.0, 4.0, flags=1, virial=True)
+    say("real: rms rel force", rms_rel(f, o["forces"]), "E", e[0], o["energies"][0])
+    assert rms_rel(f, o["forces"]) < tol
+    np.testing.assert_allclose(e[0], o["energies"][0], rtol=1e-10)
+    np.testing.assert_allclose(e[7:16], o["energies"][7:16], rtol=1e-9, atol=1e-9 * np.abs(o["energies"][7:16]).max())
+    hf = c.force("CoulombPME", args)
+    f, e = c.evaluate([hf], s.pos)
+    o = port.pme(s, (16, 16, 16), 4, 8.0, 4.0, flags=7)
+    full_tol = 5e-6 if precision == "mixed" else 1e-10
+    say("full: rms rel force", rms_rel(f, o["forces"]))
+    assert rms_rel(f, o["forces"]) < full_tol
+    scale = sum(abs(o["parts"][k]) for k in ("real", "reciprocal", "intra", "self"))
+    assert abs(e[0] - o["energies"][0]) < full_tol * 0.2 * scale

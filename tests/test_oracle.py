@@ -142,6 +142,21 @@ def test_reference_pme(port, ref):
             np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-12, atol=1e-9)
 
 
+def test_reference_pme_real_lattice_images(port, ref):
+    """The PME real term in boxes no wider than twice the cutoff, where pairs
+    meet lattice images beyond the minimal one (and atoms their own images):
+    the 27-water 9.3 A box and a 100-water 14.4 A box at rc 8, every exclusion
+    class, energy and virial, against the compiled reference."""
+    boxes = [exclusion_mix(water.water_box(27, 9.3, seed=4, jitter=0.2)),
+             exclusion_mix(water.water_box(100, 14.4, seed=6, jitter=0.3))]
+    for s in boxes:
+        for cellsize in (4.0, 9.3):
+            a = port.pme(s, (16, 16, 16), 4, 8.0, cellsize, flags=1, virial=True)
+            b = ref.pme(s, (16, 16, 16), 4, 8.0, cellsize, flags=1, virial=True)
+            np.testing.assert_allclose(a["forces"], b["forces"], rtol=0, atol=1e-11 * np.abs(b["forces"]).max())
+            np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-11, atol=1e-9)
+
+
 def test_reference_pme_charged_system(port, ref):
     """Net-charged box (one Na+ added to 216 waters): the charged-system energy
     -pi/(2 V alpha^2) (sum q)^2 and its virial diagonal
