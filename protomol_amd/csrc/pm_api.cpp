@@ -805,8 +805,8 @@ int parse_force(pm_context* c, const char* id_c, const double* v, int nv, ForceD
     if (f.rc <= 0.0) err += kw + " cutoff (=" + fmtnum(v[3]) + ") > 0.";
     if (f.accuracy <= 0.0) err += kw + " accuracy (=" + fmtnum(v[5]) + ") > 0.";
     if (!err.empty()) return fail(c, PM_ERR_ARGUMENT, " force %s :%s", kw.c_str(), err.c_str());
-    if (f.order != 4 && f.order != 6 && f.order != 8)
-      return fail(c, PM_ERR_UNSUPPORTED, "PME order %d: the GPU kernels implement orders 4, 6 and 8", f.order);
+    if (f.order > 12)
+      return fail(c, PM_ERR_UNSUPPORTED, "PME order %d: the GPU kernels implement the even orders 2 to 12", f.order);
     return PM_OK;
   }
   if (!ieq(algo, "NonbondedCutoff"))
@@ -1316,8 +1316,8 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       }
       if (a.brick[0] * a.brick[1] * a.brick[2] > 64)
         return fail(c, PM_ERR_UNSUPPORTED, "internal: spread brick of more than 64 cells");
-      if (f.order != 4 && f.order != 6 && f.order != 8)
-        return fail(c, PM_ERR_UNSUPPORTED, "PME order %d not implemented on the GPU", f.order);
+      if (f.order < 2 || f.order > 12 || (f.order & 1))
+        return fail(c, PM_ERR_UNSUPPORTED, "PME order %d not implemented on the GPU (even orders 2 to 12)", f.order);
       // spread fused with the z transform: an atom lies within half the
       // list skin of the position it was binned at (else this evaluation
       // rebuilds the layout), padded for the float displacement test

@@ -214,9 +214,17 @@ __device__ inline void pme_priority(int prio) {
 }
 
 // ---------------------------------------------------------- B-spline (P)
-// BSpline::set (base/BSpline.cpp:64-117) for an even order P >= 4.
+// BSpline::set (base/BSpline.cpp:64-117) for an even order P >= 2 (order 2:
+// the linear case of :74-79).
 template <typename T, int P>
 __device__ inline void bspline(T w, T* th, T* dth) {
+  if constexpr (P == 2) {
+    th[1] = w;
+    th[0] = T(1) - w;
+    dth[1] = T(1);
+    dth[0] = T(-1);
+    return;
+  }
   th[P - 1] = T(0);
   th[1] = w;
   th[0] = T(1) - w;
@@ -494,6 +502,12 @@ __global__ __launch_bounds__(256) void k_gather(PmPmeLaunch a, const longlong4* 
 // d2th[k] = m[k-2] - 2 m[k-1] + m[k] (m: order P-2, zero outside its range).
 template <typename T, int P>
 __device__ inline void bspline2(T w, T* th, T* dth, T* d2th) {
+  if constexpr (P == 2) {
+    // linear pieces: no second derivative
+    bspline<T, 2>(w, th, dth);
+    d2th[0] = d2th[1] = T(0);
+    return;
+  }
   T m[P];
 #pragma unroll
   for (int k = 0; k < P; k++) m[k] = T(0);
@@ -1303,9 +1317,12 @@ template <typename T>
 static bool launch_pme_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
                          const PmAtomTail* tail) {
   switch (a.order) {
+    case 2: launch_pme_t<T, 2>(s, a, plans, pos64, tail); return true;
     case 4: launch_pme_t<T, 4>(s, a, plans, pos64, tail); return true;
     case 6: launch_pme_t<T, 6>(s, a, plans, pos64, tail); return true;
     case 8: launch_pme_t<T, 8>(s, a, plans, pos64, tail); return true;
+    case 10: launch_pme_t<T, 10>(s, a, plans, pos64, tail); return true;
+    case 12: launch_pme_t<T, 12>(s, a, plans, pos64, tail); return true;
     default: return false;
   }
 }
@@ -1314,9 +1331,12 @@ template <typename T>
 static bool launch_pme_adj_p(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans, const longlong4* pos64,
                              int mode, const double4* sv, double* out) {
   switch (a.order) {
+    case 2: launch_pme_adj_t<T, 2>(s, a, plans, pos64, mode, sv, out); return true;
     case 4: launch_pme_adj_t<T, 4>(s, a, plans, pos64, mode, sv, out); return true;
     case 6: launch_pme_adj_t<T, 6>(s, a, plans, pos64, mode, sv, out); return true;
     case 8: launch_pme_adj_t<T, 8>(s, a, plans, pos64, mode, sv, out); return true;
+    case 10: launch_pme_adj_t<T, 10>(s, a, plans, pos64, mode, sv, out); return true;
+    case 12: launch_pme_adj_t<T, 12>(s, a, plans, pos64, mode, sv, out); return true;
     default: return false;
   }
 }

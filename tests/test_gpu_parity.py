@@ -170,14 +170,16 @@ def test_exclusion_rules(gpu_ctx_factory, port, precision, tol, cluster):
     np.testing.assert_allclose(e[0], o["energies"][0], rtol=1e-11)
 
 
-@pytest.mark.parametrize("grid,order", [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8), ((28, 21, 14), 4)])
+@pytest.mark.parametrize("grid,order", [((24, 24, 24), 4), ((32, 30, 20), 6), ((25, 24, 18), 8), ((28, 21, 14), 4),
+                                        ((24, 24, 24), 2), ((24, 24, 24), 10), ((32, 30, 20), 12)])
 @pytest.mark.parametrize("precision,tol", [("mixed", 5e-6), ("double", 1e-10)])
 @pytest.mark.parametrize("fft", [1, 2])
 def test_pme_small(gpu_ctx_factory, port, grid, order, precision, tol, fft):
     """PME reciprocal + correction against the oracle, with the y and x
     transforms as separate passes (fft 1) and as one kernel per kz plane
-    (fft 2); the last grid has a factor 7 on every side (the radix-7 pass,
-    lengths that ZFFT's fftfact_ splits by 7, base/fft.c:846-906)."""
+    (fft 2); one grid has a factor 7 on every side (the radix-7 pass,
+    lengths that ZFFT's fftfact_ splits by 7, base/fft.c:846-906), and the
+    orders run over the even orders 2 to 12 (order 2: BSpline.cpp:74-79)."""
     say(f"case: pme water648 grid {grid} order {order} {precision} fft option {fft}")
     s = water.named("water648", seed=3, jitter=0.4)
     c = setup_ctx(gpu_ctx_factory, s, 4.0, precision)

@@ -134,7 +134,8 @@ def test_reference_exclusion_rules(port, ref):
 
 def test_reference_pme(port, ref):
     s = water.named("water648", seed=23, jitter=0.3)
-    for grid, order in (((20, 24, 30), 4), ((32, 32, 32), 6), ((25, 25, 25), 8)):
+    for grid, order in (((20, 24, 30), 4), ((32, 32, 32), 6), ((25, 25, 25), 8), ((24, 24, 24), 2),
+                        ((24, 24, 24), 10), ((24, 24, 24), 12)):
         for flags in (1, 2, 4, 7):
             a = port.pme(s, grid, order, 8.0, 4.0, flags=flags, virial=True)
             b = ref.pme(s, grid, order, 8.0, 4.0, flags=flags, virial=True)
