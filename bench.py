@@ -455,6 +455,36 @@ def accuracy_vs_oracle(s, x, alpha, grid, cell, rc=RC, switchon=SWITCHON):
     return out
 
 
+def dropin_bench(s, x, alpha, n=20, warmup=3):
+    """The drop-in route: one pm_evaluate of the main force set per MD step,
+    as ProtoMol's ForceGroup::evaluateSystemForces would call it through the
+    GpuSystemForce adapter, with host position, force and energy buffers (the
+    PCIe copies in and out included) and energies every call; at the end
+    positions of the timed run, so the kept pair lists are reused between
+    calls as on most MD steps."""
+    from protomol_amd import gpu
+    out = {"workload": "water98k: pm_evaluate of LJ-C2 + Ewald-real, PME 96^3 reciprocal + correction, bonds + "
+                       "angles from host buffers, energies every call", "evaluations": n}
+    xs = np.ascontiguousarray(x)
+    for precision in ("mixed", "double"):
+        ctx = gpu.Context(0, precision)
+        ctx.set_topology(s)
+        ctx.set_boundary(s.box, s.origin, CELL)
+        hs = make_forces(ctx, alpha)
+        f = np.zeros((s.n, 3))
+        e = np.zeros(gpu.ENERGY_COUNT)
+        for _ in range(warmup):
+            ctx.evaluate(hs, xs, forces=f, energies=e)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            ctx.evaluate(hs, xs, forces=f, energies=e)
+        dt = (time.perf_counter() - t0) / n
+        ctx.close()
+        out["fp64" if precision == "double" else "mixed"] = {
+            "ms_per_evaluation": dt * 1e3, "ns_per_day_at_one_evaluation_per_step": DT_FS * 86400 * 1e-6 / dt}
+    return out
+
+
 def c1_bench(steps=100, warmup=10):
     """The plain-method workload: 216 waters (648 atoms) in an 18.62 A cube,
     LennardJones and Coulomb with C2 switching 7 -> 8 A (two NonbondedCutoff
@@ -871,6 +901,7 @@ def main():
     if not args.no_accuracy:
         # the main workload's force error at the end of the timed run, outside the timed region
         line["accuracy"] = accuracy_vs_oracle(s, run["x"], alpha, GRID, CELL)
+    line["dropin_evaluate"] = dropin_bench(s, run["x"], alpha)
     if not args.no_c1:
         line["c1_water648"] = c1_bench()
     if not args.no_replicas:
