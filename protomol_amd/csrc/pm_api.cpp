@@ -1323,7 +1323,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
       // rebuilds the layout), padded for the float displacement test
       const double drift_A = (reuse ? 0.5 * skin * (1.0 + 1e-3) : 0.0) + 1e-3;
       for (int d = 0; d < 3; d++) a.drift_u[d] = drift_A * ng[d] / c->box.L[d] + 0.01;
-      a.spread_zfwd = !adj && c->pme_spread_option != 0 && pm_pme_spread_zfwd_fits(a, c->precision) ? 1 : 0;
+      a.spread_zfwd = !adj && c->pme_spread_option != 0 ? pm_pme_spread_zfwd_fits(a, c->precision) : 0;
       if (!a.spread_zfwd && (size_t)a.tile[0] * a.tile[1] * a.tile[2] * 8 > 64 * 1024)
         return fail(c, PM_ERR_UNSUPPORTED, "PME grid too fine for the cell size: spread tile exceeds 64 KB of LDS (use a smaller cellsize)");
       pme_launch.push_back(std::make_pair(a, hr.first));

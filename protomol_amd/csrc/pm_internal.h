@@ -268,7 +268,8 @@ struct PmPmeLaunch {
   int tile[3];               // LDS tile extent (grid points) per axis
   double cell_lo_u[3];       // grid coordinate of cell 0's lower face: frac((origin - L/2)/L) * n
   double cell_du[3];         // grid points per cell
-  // spread fused with the z forward transform (k_spread_zfwd): 1 when on;
+  // spread fused with the z forward transform (k_spread_zfwd): the z lines
+  // a block owns along y (4, 2 or 1), 0 when off;
   // drift_u: how far (grid points per axis) an atom may lie outside the cell
   // it was binned in at the last layout build
   int spread_zfwd;
@@ -287,8 +288,9 @@ struct PmAtomTail;
 // bonded forces of each atom in the gather kernel's thread for that atom
 bool pm_launch_pme_plans(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* plans,
                          const longlong4* pos64, int precision, const PmAtomTail* tail);
-// 1 when k_spread_zfwd's cell columns for one block fit its LDS lists and
-// its z lines fit 64 KB of LDS (needs nz, order, ncell, cell_du, drift_u)
+// the lines along y (4, 2 or 1) of a k_spread_zfwd block whose 4 x ly z
+// lines fit 64 KB of LDS, 0 when none does or its cell columns exceed its
+// LDS lists (needs nz, order, ncell, cell_du, drift_u)
 int pm_pme_spread_zfwd_fits(const PmPmeLaunch& a, int precision);
 // a.theta (written through a const_cast) from bmod, expf_tab, inv_pi_v and Lr
 void pm_launch_pme_theta(hipStream_t s, const PmPmeLaunch& a, int precision);

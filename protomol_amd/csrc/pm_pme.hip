@@ -717,23 +717,23 @@ __global__ __launch_bounds__(256) void k_fft_z_fwd(PmPmeLaunch a, PmFftPlan p, i
 }
 
 // Spread fused with the z forward transform.  A block owns PM_SZ_LX x
-// PM_SZ_LY whole z lines of the grid: it walks the slots of the cell columns
-// whose atoms can reach those lines (the cells from the last layout build,
-// widened by how far an atom may have drifted from its cell since: a.drift_u
-// grid points), adds each atom's stencil points on its own lines into an LDS
+// a.spread_zfwd (4, 2 or 1: as many as fit LDS) whole z lines of the grid:
+// it walks the slots of the cell columns whose atoms can reach those lines
+// (the cells from the last layout build, widened by how far an atom may have
+// drifted from its cell since: a.drift_u grid points), adds each atom's stencil points on its own lines into an LDS
 // accumulator of 64-bit fixed-point charges (integer adds: the same sums as
 // the spread kernels make, in any order), then converts the lines to real
 // values and transforms them along z as k_fft_z_fwd does.  Every grid point
 // is owned by one block, so there are no global atomics and no charge grid
 // to clear.
 #define PM_SZ_LX 4
-#define PM_SZ_LY 4
 #define PM_SZ_MAXCOL 256
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_spread_zfwd(PmPmeLaunch a, PmFftPlan p, const longlong4* __restrict__ pos64) {
   pme_priority(a.prio);
   typedef typename Cx<T>::t T2;
-  constexpr int LX = PM_SZ_LX, LY = PM_SZ_LY, TL = LX * LY;
+  constexpr int LX = PM_SZ_LX;
+  const int LY = a.spread_zfwd, TL = LX * LY;
   extern __shared__ unsigned char s_raw[];
   __shared__ int s_start[PM_SZ_MAXCOL], s_pre[PM_SZ_MAXCOL + 1];
   __shared__ int s_ncol;
@@ -1259,8 +1259,9 @@ static void launch_pme_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPlan* p
                          const longlong4* pos64, const PmAtomTail* tail) {
   if (a.spread_zfwd) {
     const size_t el = sizeof(typename Cx<T>::t);
-    const int nb = ((a.nx + PM_SZ_LX - 1) / PM_SZ_LX) * ((a.ny + PM_SZ_LY - 1) / PM_SZ_LY);
-    hipLaunchKernelGGL((k_spread_zfwd<T, P>), dim3(nb), dim3(256), zpass_lds(a.nz, PM_SZ_LX * PM_SZ_LY, el), s, a,
+    const int ly = a.spread_zfwd;
+    const int nb = ((a.nx + PM_SZ_LX - 1) / PM_SZ_LX) * ((a.ny + ly - 1) / ly);
+    hipLaunchKernelGGL((k_spread_zfwd<T, P>), dim3(nb), dim3(256), zpass_lds(a.nz, PM_SZ_LX * ly, el), s, a,
                        plans[2], pos64);
     launch_fft_chain<T>(s, a, plans, true);
   } else {
@@ -1292,18 +1293,21 @@ static void launch_pme_adj_t(hipStream_t s, const PmPmeLaunch& a, const PmFftPla
 }
 
 int pm_pme_spread_zfwd_fits(const PmPmeLaunch& a, int precision) {
-  const size_t lds = zpass_lds(a.nz, PM_SZ_LX * PM_SZ_LY, fft_elem(precision));
-  if (lds + sizeof(int) * (2 * PM_SZ_MAXCOL + 2) > 64 * 1024) return 0;
-  const int g[2] = {PM_SZ_LX, PM_SZ_LY};
-  long long cols = 1;
-  for (int d = 0; d < 2; d++) {
-    const double w = g[d] + a.order - 1 + 2.0 * a.drift_u[d];
-    // the kernel's range spans at most floor(w / du) + 2 cells; one more for rounding
-    long long c = (long long)floor(w / a.cell_du[d]) + 3;
-    if (c > a.ncell[d]) c = a.ncell[d];
-    cols *= c;
+  for (int ly = 4; ly >= 1; ly /= 2) {
+    const size_t lds = zpass_lds(a.nz, PM_SZ_LX * ly, fft_elem(precision));
+    if (lds + sizeof(int) * (2 * PM_SZ_MAXCOL + 2) > 64 * 1024) continue;
+    const int g[2] = {PM_SZ_LX, ly};
+    long long cols = 1;
+    for (int d = 0; d < 2; d++) {
+      const double w = g[d] + a.order - 1 + 2.0 * a.drift_u[d];
+      // the kernel's range spans at most floor(w / du) + 2 cells; one more for rounding
+      long long c = (long long)floor(w / a.cell_du[d]) + 3;
+      if (c > a.ncell[d]) c = a.ncell[d];
+      cols *= c;
+    }
+    return cols <= PM_SZ_MAXCOL ? ly : 0;
   }
-  return cols <= PM_SZ_MAXCOL ? 1 : 0;
+  return 0;
 }
 
 int pm_pme_xpass_blocks(int nx, int ny, int nz, int precision, int plane) {

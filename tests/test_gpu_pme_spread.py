@@ -101,3 +101,17 @@ def test_fused_spread_coarse_cells(gpu_ctx_factory, port):
     from protomol_amd.gpu import PmError
     with pytest.raises(PmError, match="spread tile"):
         c.evaluate([c.force(PME, [32, 30, 20, 8.0, 6, 1e-6, -1.0])], s.pos)
+
+
+def test_fused_spread_long_z_lines(gpu_ctx_factory, port):
+    """A 240-point z axis: 4 x 4 z lines no longer fit a block's LDS, so the
+    fused spread owns 4 x 2 lines (mixed) or 4 x 1 (FP64); against the oracle."""
+    from tests.cases import rms_rel
+    say("case: pme fused spread water648 grid (20, 24, 240) order 4")
+    s = water.named("water648", seed=5, jitter=0.4)
+    for precision, tol in (("mixed", 5e-6), ("double", 1e-10)):
+        c = ctx(gpu_ctx_factory, s, precision, 4.0, 1)
+        f, e = c.evaluate([c.force(PME, [20, 24, 240, 8.0, 4, 1e-6, -1.0])], s.pos)
+        o = port.pme(s, (20, 24, 240), 4, 8.0, 4.0, flags=6)
+        say(precision, "rms rel force", rms_rel(f, o["forces"]))
+        assert rms_rel(f, o["forces"]) < tol
