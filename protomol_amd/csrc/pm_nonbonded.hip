@@ -306,20 +306,20 @@ __device__ inline void fill_slots_body(PmBox box, int ncells, const double* __re
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
                              const int* __restrict__ rebuild, int4* __restrict__ x_build,
-                             int4* __restrict__ x_prune, int bx) {
+                             int4* __restrict__ x_prune, int bx, int nb) {
   const int lane = threadIdx.x & 63;
   const int c = bx * 4 + (threadIdx.x >> 6);
   if (rebuild && *rebuild == 0) {
     // layout kept (list reuse): current positions and charges into the
-    // cell's slots, with the arithmetic of the build below; padding slots keep
-    // their place (they are masked); when the inner lists are pruned in this
-    // evaluation, also the positions they are pruned at
-    if (c >= ncells) return;
+    // slots, with the arithmetic of the build below, one thread per slot up
+    // to the last cell's last atom (padding slots keep their place: they are
+    // masked); when the inner lists are pruned in this evaluation, also the
+    // positions they are pruned at
     const bool prune = x_prune && rebuild[1] != 0;
-    const int s0 = cell_slot[c], k = cell_count[c];
-    for (int u = lane; u < k; u += 64) {
-      const int sl = s0 + u;
+    const int send = ncells > 0 ? cell_slot[ncells - 1] + cell_count[ncells - 1] : 0;
+    for (int sl = bx * blockDim.x + threadIdx.x; sl < send; sl += nb * blockDim.x) {
       const int ai = atom_of_slot[sl];
+      if (ai < 0) continue;
       check_position(box, pos + 3 * ai);
       int4 p;
       p.x = (int)to_fix(pos[3 * ai], box.to_fix[0]);
@@ -419,12 +419,12 @@ __global__ __launch_bounds__(256) void k_fill_slots(PmBox box, int ncells, const
                              longlong4* __restrict__ slot_pos64, int* __restrict__ slot_meta,
                              int* __restrict__ atom_of_slot, int dummy_type, int dummy_slot, int m_cl,
                              const int* __restrict__ rebuild, int4* __restrict__ x_build,
-                             int4* __restrict__ x_prune) { fill_slots_body(box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m_cl, rebuild, x_build, x_prune, blockIdx.x); }
+                             int4* __restrict__ x_prune) { fill_slots_body(box, ncells, pos, q, type, cell_count, cell_start, cell_slot, cell_atoms, key, slot_pos, slot_pos64, slot_meta, atom_of_slot, dummy_type, dummy_slot, m_cl, rebuild, x_build, x_prune, blockIdx.x, gridDim.x); }
 struct FillSlotsArgs { PmBox box; int ncells; const double* pos; const double* q; const int* type; const int* cell_count; const int* cell_start; const int* cell_slot; const int* cell_atoms; const float* key; int4* slot_pos; longlong4* slot_pos64; int* slot_meta; int* atom_of_slot; int dummy_type; int dummy_slot; int m_cl; const int* rebuild; int4* x_build; int4* x_prune; };
 __global__ __launch_bounds__(256) void kb_fill_slots(const FillSlotsArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const FillSlotsArgs& q = t[r];
-  fill_slots_body(q.box, q.ncells, q.pos, q.q, q.type, q.cell_count, q.cell_start, q.cell_slot, q.cell_atoms, q.key, q.slot_pos, q.slot_pos64, q.slot_meta, q.atom_of_slot, q.dummy_type, q.dummy_slot, q.m_cl, q.rebuild, q.x_build, q.x_prune, blockIdx.x - pre[r]);
+  fill_slots_body(q.box, q.ncells, q.pos, q.q, q.type, q.cell_count, q.cell_start, q.cell_slot, q.cell_atoms, q.key, q.slot_pos, q.slot_pos64, q.slot_meta, q.atom_of_slot, q.dummy_type, q.dummy_slot, q.m_cl, q.rebuild, q.x_build, q.x_prune, blockIdx.x - pre[r], pre[r + 1] - pre[r]);
 }
 
 
