@@ -354,9 +354,15 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        its LDS), 0 by bricks of cells into LDS tiles added
  *                        to the grid with atomics, then the z transform.
  *                        Both give the same grid bit for bit.
+ *   PM_OPT_PAIR_LANES    the pair kernel's lane layout: -1 (default)
+ *                        automatic (1), 1 one lane per list entry (the
+ *                        cluster's atoms in scalar registers, the entry's j
+ *                        force whole in its lane), 0 a row of one lane per
+ *                        cluster atom for each entry (j forces summed over
+ *                        the row).
  * pm_get_option returns the value in effect (after the automatic choice). */
 enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PME_PRIORITY = 4,
-       PM_OPT_PRUNE_SKIN = 5, PM_OPT_LIST_TEST = 6, PM_OPT_PME_SPREAD = 7 };
+       PM_OPT_PRUNE_SKIN = 5, PM_OPT_LIST_TEST = 6, PM_OPT_PME_SPREAD = 7, PM_OPT_PAIR_LANES = 8 };
 int pm_set_option(pm_context* ctx, int option, int value);
 int pm_get_option(pm_context* ctx, int option, int* value);
 

@@ -230,6 +230,7 @@ struct pm_context {
   int graph_option = 1;              // PM_OPT_GRAPH
   int pme_fft_option = 0;            // PM_OPT_PME_FFT
   int pme_spread_option = 1;         // PM_OPT_PME_SPREAD (1: fused with the z transform where it fits)
+  int pair_lanes_option = 1;         // PM_OPT_PAIR_LANES (1: one lane per list entry)
   int pme_prio_option = -1;          // PM_OPT_PME_PRIORITY (-1: automatic)
 
   // pair-list reuse (PM_OPT_LIST_SKIN): lists hold j atoms within the pair
@@ -1225,6 +1226,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.xs_part = c->d_xs_part;
     a.xs_tcap = c->m * c->xs_maxdeg;
     a.m = c->m;
+    a.jlanes = c->pair_lanes_option;
     a.molecule = c->d_mol;
     a.ex_row = c->d_ex_row;
     a.ex_partner = c->d_ex_partner;
@@ -3050,6 +3052,13 @@ int pm_set_option(pm_context* c, int option, int value) {
     c->gen++;   // captured steps hold the launch arguments
     return PM_OK;
   }
+  if (option == PM_OPT_PAIR_LANES) {
+    if (value < -1 || value > 1)
+      return fail(c, PM_ERR_ARGUMENT, "pair lane option must be -1 (automatic), 0 (a row per entry) or 1 (a lane per entry; got %d)", value);
+    c->pair_lanes_option = value == 0 ? 0 : 1;
+    c->gen++;   // captured steps hold the launch arguments
+    return PM_OK;
+  }
   if (option == PM_OPT_PME_FFT) {
     if (value < 0 || value > 2)
       return fail(c, PM_ERR_ARGUMENT, "PME FFT option must be 0 (automatic), 1 (separate passes) or 2 (plane kernel; got %d)", value);
@@ -3088,6 +3097,10 @@ int pm_get_option(pm_context* c, int option, int* value) {
   }
   if (option == PM_OPT_PME_SPREAD) {
     *value = c->pme_spread_option;
+    return PM_OK;
+  }
+  if (option == PM_OPT_PAIR_LANES) {
+    *value = c->pair_lanes_option;
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {

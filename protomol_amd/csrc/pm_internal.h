@@ -133,6 +133,7 @@ struct PmNbLaunch {
   int xs_tcap;               // LDS entries per cluster: M * largest partner count
   unsigned int* jmask;       // per list entry (flagged entries): FULL bit ii, MODIFIED bit 16 + ii
   int m;                     // atoms per cluster (4 or 8)
+  int jlanes;                // pair kernel: 1 one lane per list entry (k_pair_jl), 0 rows of m lanes (k_pair)
   // topology
   const int* molecule;
   const int* ex_row;

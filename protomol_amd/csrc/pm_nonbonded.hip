@@ -1372,13 +1372,347 @@ __global__ __launch_bounds__(256) void k_pair(PmBox box, PmNbLaunch a, PmPairPar
   pair_load_table<T, LJSWF>(a);
   pair_cluster<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, blockIdx.x * 4 + (threadIdx.x >> 6));
 }
+
+// orders one wave's LDS writes before its lanes read each other's values
+// (the wave's LDS operations complete in order; this keeps the compiler
+// from moving the accesses across)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// wave-uniform copies (scalar registers) of values every lane holds alike
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ inline long long uni(long long v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double uni(double v) { return __longlong_as_double(uni(__double_as_longlong(v))); }
+
+// LDS bytes per wave of pair_cluster_jl: an iteration's j forces (3 x 64
+// values and 64 slots), later the i-force reduction, PM_JL_NV(T) values of
+// the 64 lanes per pass.  2 KB, so that the kernel leaves most of a CU's LDS
+// to the PME kernels that run beside it on the other stream.
+#define PM_JL_LDS 2048
+#define PM_JL_NV(T) (PM_JL_LDS / 64 / (int)sizeof(T))
+
+// Half-list pair kernel with one lane per list entry: one wave per i-cluster
+// of M atoms, whose positions, charges and types sit in scalar registers; a
+// lane takes one entry (64 per iteration) and evaluates it against the M i
+// atoms in turn.  The j force of an entry is then whole in its lane and is
+// added with one 64-bit integer atomic per component, with no cross-lane
+// sums; the i forces accumulate per lane in FP32 (one term per iteration and
+// i atom; FP64 in the FP64 kernel) and are summed over the lanes in FP64 at
+// the end, through LDS, in lane order.  Same pair arithmetic and the same
+// half-list rule as pair_cluster; only the order in which FP32 partial sums
+// are formed differs.
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
+__device__ inline void pair_cluster_jl(const PmBox& box, const PmNbLaunch& a, const PmPairParams& pp, const int cl) {
+  extern __shared__ unsigned char s_lj_raw[];
+  typedef typename std::conditional<sizeof(T) == 4, float4, double4>::type LJ4;
+  const LJ4* s_lj = reinterpret_cast<const LJ4*>(s_lj_raw);
+  const int nt1 = a.ntypes1;
+  const int lane = threadIdx.x & 63;
+  const int ncl = a.counts[1];
+  if (cl >= ncl) {
+    if (WE && cl < a.ncl_max && lane < PM_NE) a.epart[(long long)cl * PM_NE + lane] = 0.0;
+    return;
+  }
+  typedef typename std::conditional<sizeof(T) == 4, int4, longlong4>::type P4;
+  typedef typename std::conditional<sizeof(T) == 4, int, long long>::type PI;
+  const P4* __restrict__ spos;
+  if constexpr (sizeof(T) == 4) spos = a.slot_pos; else spos = a.slot_pos64;
+  const int c0 = cl * M;
+  // the cluster's atoms, wave-uniform
+  PI ipx[M], ipy[M], ipz[M];
+  T iq[M];
+  int ity[M], ioi[M];
+  unsigned int ireal = 0;
+#pragma unroll
+  for (int ii = 0; ii < M; ii++) {
+    const P4 p = spos[c0 + ii];
+    const int m = uni(a.slot_meta[c0 + ii]);
+    ipx[ii] = uni(p.x);
+    ipy[ii] = uni(p.y);
+    ipz[ii] = uni(p.z);
+    if constexpr (sizeof(T) == 4) iq[ii] = __int_as_float(uni(p.w)); else iq[ii] = __longlong_as_double(uni(p.w));
+    ity[ii] = m & 0xff;
+    ioi[ii] = (m >> 8) & 0x7fffff;
+    if (ioi[ii] != 0x7fffff) ireal |= 1u << ii;
+  }
+  const T sx = sizeof(T) == 4 ? (T)box.fix_to_A[0] : (T)(box.fix_to_A[0] * 2.3283064365386963e-10);
+  const T sy = sizeof(T) == 4 ? (T)box.fix_to_A[1] : (T)(box.fix_to_A[1] * 2.3283064365386963e-10);
+  const T sz = sizeof(T) == 4 ? (T)box.fix_to_A[2] : (T)(box.fix_to_A[2] * 2.3283064365386963e-10);
+
+  const T cut2 = (T)pp.cut2;
+  const T lj_cut2 = (T)pp.lj_cut2, lj_on2 = (T)pp.lj_on2, lj_sw1 = (T)pp.lj_sw1, lj_sw2 = (T)pp.lj_sw2,
+          lj_sw3 = (T)pp.lj_sw3, lj_c15_1 = (T)pp.lj_c15_1, lj_c05_3 = (T)pp.lj_c05_3,
+          lj_c15_3 = (T)pp.lj_c15_3;
+  const T c_cut2 = (T)pp.c_cut2, c_on2 = (T)pp.c_on2, c_sw1 = (T)pp.c_sw1, c_sw2 = (T)pp.c_sw2,
+          c_sw3 = (T)pp.c_sw3, c_c15_1 = (T)pp.c_c15_1, c_c05_3 = (T)pp.c_c05_3, c_c15_3 = (T)pp.c_c15_3;
+  const T alpha = (T)pp.alpha, tapi = (T)pp.two_alpha_pi, csf = (T)pp.csf;
+  const T ex_k = (T)(-(pp.alpha * pp.alpha) * 1.4426950408889634);
+  const T p_alpha = (T)(0.3275911 * pp.alpha);
+  const bool modify = pp.modify != 0;
+
+  T fix[M], fiy[M], fiz[M];   // i forces of this lane's entries
+#pragma unroll
+  for (int ii = 0; ii < M; ii++) { fix[ii] = T(0); fiy[ii] = T(0); fiz[ii] = T(0); }
+  T elj = 0, ec = 0;
+  T vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+  const long long lbase = (long long)cl * a.lcap;
+  const int* __restrict__ lst = (a.inner ? a.ilist : a.jlist) + lbase;
+  const unsigned int* __restrict__ msk = (a.inner ? a.imask : a.jmask) + lbase;
+  const int cnt = a.inner ? a.icount[cl] : a.jcount[cl];
+  const int last = cnt > 0 ? cnt - 1 : 0;
+  const bool vir = WE && a.want_virial != 0;
+  long long* __restrict__ ff = a.force_slot;
+  bool range_bad = false;
+  // this wave's LDS area (after the LJ table): the j forces of an iteration
+  // (3 x 64 values and 64 slots), then the i-force reduction
+  constexpr int NV = PM_JL_NV(T);
+  const size_t tbytes = LJSWF != PM_SWF_NONE ? ((size_t)nt1 * nt1 * sizeof(LJ4) + 15) / 16 * 16 : 0;
+  T* s_red = reinterpret_cast<T*>(s_lj_raw + tbytes) + (threadIdx.x >> 6) * 64 * NV;
+  T* s_jf = s_red;
+  int* s_js = reinterpret_cast<int*>(s_red + 3 * 64);
+
+  // the entry and j atom of this iteration, and the entry of the next, in
+  // flight while the previous iteration computes; loads past the list's end
+  // re-read its last entry (masked)
+  int e = cnt > 0 ? lst[min(lane, last)] : a.dummy_slot;
+  P4 pj = spos[e & 0x7fffffff];
+  int mj = a.slot_meta[e & 0x7fffffff];
+  int en = cnt > 0 ? lst[min(lane + 64, last)] : a.dummy_slot;
+  for (int base = 0; base < cnt; base += 64) {
+    const P4 pn = spos[en & 0x7fffffff];
+    const int mn = a.slot_meta[en & 0x7fffffff];
+    const int enn = lst[min(base + 128 + lane, last)];
+    const int k = base + lane;
+    const bool valid = k < cnt;
+    const int sj = e & 0x7fffffff;
+    // exclusions: only entries the list flagged hold an excluded pair
+    const bool flagged = valid && e < 0;
+    unsigned int xm = 0;
+    const bool anyflag = __ballot(flagged) != 0ull;
+    if (anyflag && flagged) xm = msk[k];
+    const bool any_mod = anyflag && __ballot((xm >> 16) != 0u) != 0ull;
+    T qj;
+    if constexpr (sizeof(T) == 4) qj = __int_as_float(pj.w); else qj = __longlong_as_double(pj.w);
+    const int tj = mj & 0xff;
+    T fjx = 0, fjy = 0, fjz = 0;
+    // the pair of this lane's entry with cluster atom ii: MOD false for the
+    // ordinary pairs (excluded and modified ones masked; xm is 0 for entries
+    // the list did not flag), true for the modified (1-4) pairs, with the
+    // 1-4 LJ parameters and the Coulomb scaling
+    auto pair_ii = [&](const int ii, auto mod) {
+      constexpr bool MOD = decltype(mod)::value;
+      T dx, dy, dz;
+      if constexpr (sizeof(T) == 4) {
+        dx = (T)(int)(pj.x - ipx[ii]) * sx;
+        dy = (T)(int)(pj.y - ipy[ii]) * sy;
+        dz = (T)(int)(pj.z - ipz[ii]) * sz;
+      } else {
+        dx = (T)(long long)((unsigned long long)pj.x - (unsigned long long)ipx[ii]) * sx;
+        dy = (T)(long long)((unsigned long long)pj.y - (unsigned long long)ipy[ii]) * sy;
+        dz = (T)(long long)((unsigned long long)pj.z - (unsigned long long)ipz[ii]) * sz;
+      }
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      const bool mask = valid && sj > c0 + ii && (r2 <= cut2) &&
+                        (MOD ? (xm & (65536u << ii)) != 0u : (xm & (65537u << ii)) == 0u);
+      T rr2, rinv;
+      if constexpr (sizeof(T) == 4) { rinv = Prec<T>::rsq(r2); rr2 = rinv * rinv; }
+      else { rr2 = T(1) / r2; rinv = sqrt(rr2); }
+      T F, E1 = 0, E2 = 0;
+      if constexpr (LJSWF != PM_SWF_NONE) {
+        T A, B;
+        if constexpr (PA) {
+          lj_ab<T>(a, s_lj, ity[ii], mj, nt1, ioi[ii], MOD ? PM_EXCL_MODIFIED : PM_EXCL_NONE, A, B);
+        } else {
+          const LJ4 ab = s_lj[ity[ii] * nt1 + tj];
+          // A14, B14 for modified pairs (LennardJonesForce.h:55-60)
+          A = MOD ? (T)ab.z : (T)ab.x;
+          B = MOD ? (T)ab.w : (T)ab.y;
+        }
+        const T r6 = rr2 * rr2 * rr2;
+        const T t12 = A * (r6 * r6);
+        const T t6 = B * r6;
+        T e1 = t12 - t6;
+        T f1 = (T(12) * t12 - T(6) * t6) * rr2;
+        if constexpr (LJSWF == PM_SWF_C2) {
+          const T c2 = lj_cut2 - r2;
+          const T c4 = c2 * (lj_sw2 + T(2) * r2);
+          const bool in = r2 >= lj_on2;
+          const T v = in ? lj_sw1 * (c2 * c4) : T(1);
+          const T d = in ? lj_sw3 * (c2 * c2 - c4) : T(0);
+          f1 = f1 * v - e1 * d;
+          e1 = e1 * v;
+        } else if constexpr (LJSWF == PM_SWF_C1) {
+          if (modify)
+            apply_switch<T, LJSWF>(r2, r2 * rinv, lj_cut2, lj_on2, lj_sw1, lj_sw2, lj_sw3, lj_c15_1, lj_c05_3,
+                                   lj_c15_3, f1, e1);
+        }
+        F = f1;
+        E1 = e1;
+      }
+      if constexpr (CK != PM_COUL_NONE) {
+        T e2, f2;
+        T qq = iq[ii] * qj;
+        if constexpr (MOD) qq *= csf;   // coulombScalingFactor (CoulombForce.h:49-51)
+        if constexpr (CK == PM_COUL_PLAIN) {
+          e2 = qq * rinv;
+          f2 = e2 * rr2;
+        } else {
+          T t;
+          if constexpr (sizeof(T) == 4) t = Prec<T>::rcp(T(1) + p_alpha * (r2 * rinv));
+          else t = Prec<T>::rcp(T(1) + T(0.3275911) * (alpha * (r2 * rinv)));
+          const T poly = t * (T(0.254829592) + t * (T(-0.284496736) + t * (T(1.421413741) +
+                                                   t * (T(-1.453152027) + t * T(1.061405429)))));
+          T ex;
+          if constexpr (sizeof(T) == 4) ex = qq * __builtin_amdgcn_exp2f(r2 * ex_k);
+          else { const T ar = alpha * (r2 * rinv); ex = qq * Prec<T>::expneg(ar * ar); }
+          e2 = poly * ex * rinv;
+          f2 = (e2 + tapi * ex) * rr2;
+        }
+        if constexpr (CSWF == PM_SWF_C2) {
+          const T c2 = c_cut2 - r2;
+          const T c4 = c2 * (c_sw2 + T(2) * r2);
+          const bool in = r2 >= c_on2;
+          const T v = in ? c_sw1 * (c2 * c4) : T(1);
+          const T d = in ? c_sw3 * (c2 * c2 - c4) : T(0);
+          f2 = f2 * v - e2 * d;
+          e2 = e2 * v;
+        } else if constexpr (CSWF == PM_SWF_C1) {
+          if (modify)
+            apply_switch<T, CSWF>(r2, r2 * rinv, c_cut2, c_on2, c_sw1, c_sw2, c_sw3, c_c15_1, c_c05_3, c_c15_3, f2, e2);
+        }
+        if constexpr (LJSWF != PM_SWF_NONE) F += f2; else F = f2;
+        E2 = e2;
+      }
+      F = mask ? F : T(0);
+      // fij = F * (xj - xi): -fij on i, +fij on j (OneAtomPair.h:178-181)
+      const T fx_ = F * dx, fy_ = F * dy, fz_ = F * dz;
+      fix[ii] -= fx_;
+      fiy[ii] -= fy_;
+      fiz[ii] -= fz_;
+      fjx += fx_;
+      fjy += fy_;
+      fjz += fz_;
+      if constexpr (WE) {
+        elj += mask ? E1 : T(0);
+        ec += mask ? E2 : T(0);
+        if (vir) {
+          vxx += fx_ * dx; vxy += fx_ * dy; vxz += fx_ * dz;
+          vyy += fy_ * dy; vyz += fy_ * dz; vzz += fz_ * dz;
+        }
+      }
+    };
+#pragma unroll
+    for (int ii = 0; ii < M; ii++)
+      if ((ireal >> ii) & 1u) pair_ii(ii, std::false_type{});   // padding slots skipped (uniform)
+    if (any_mod) {
+#pragma unroll
+      for (int ii = 0; ii < M; ii++)
+        if ((ireal >> ii) & 1u) pair_ii(ii, std::true_type{});
+    }
+    // j forces: the 64 entries' x, y, z regrouped through LDS so that lane
+    // v of round r adds component v % 3 of entry v / 3 (v = 64 r + lane):
+    // each atomic instruction covers about 21 entries' components, which lie
+    // in a few cache lines when the entries' slots are consecutive (as along
+    // a column range), instead of 64 entries' single components
+    s_jf[3 * lane + 0] = fjx;
+    s_jf[3 * lane + 1] = fjy;
+    s_jf[3 * lane + 2] = fjz;
+    s_js[lane] = sj;
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const int v = 64 * r + lane, ent = v / 3, comp = v - 3 * ent;
+      const T val = s_jf[v];
+      range_bad |= !(__builtin_fabs(val) < T(PM_FORCE_LIMIT));
+      if (val != T(0))
+        atomicAdd((unsigned long long*)(ff + 3LL * s_js[ent] + comp), (unsigned long long)force_fix_of(val));
+    }
+    wave_lds_sync();
+    e = en;
+    pj = pn;
+    mj = mn;
+    en = enn;
+  }
+  // i forces: per pass, every lane's partial sums of NV values into LDS;
+  // lane L adds value L % NV over the G = 64 / NV lane segments' share
+  // L / NV in FP64, then the G segment sums of each value are added over the
+  // lanes (the same order on every run).  The cluster's slots are
+  // consecutive, so value c is component c % 3 of slot c0 + c / 3.
+  {
+    constexpr int G = 64 / NV, SEG = 64 / G;
+    static_assert(NV * sizeof(T) * 64 == PM_JL_LDS && 3 * 64 * sizeof(T) + 64 * sizeof(int) <= PM_JL_LDS, "LDS area");
+#pragma unroll
+    for (int pass = 0; pass < (3 * M + NV - 1) / NV; pass++) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const int c = pass * NV + v, ii = c / 3, comp = c % 3;
+        if (c < 3 * M) s_red[v * 64 + lane] = comp == 0 ? fix[ii] : (comp == 1 ? fiy[ii] : fiz[ii]);
+      }
+      wave_lds_sync();
+      const int v = lane % NV, g = lane / NV;
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < SEG; t++) acc += (double)s_red[v * 64 + g * SEG + t];
+#pragma unroll
+      for (int off = NV; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      const int c = pass * NV + v;
+      if (lane < NV && c < 3 * M && ((ireal >> (c / 3)) & 1u)) {
+        range_bad |= !(__builtin_fabs(acc) < PM_FORCE_LIMIT);
+        atomicAdd((unsigned long long*)(ff + 3LL * c0 + c), (unsigned long long)__double2ll_rn(acc * PM_FORCE_FIX));
+      }
+      wave_lds_sync();
+    }
+  }
+  if (__ballot(range_bad) != 0ull && lane == 0) pm_raise(box.err, PM_FLAG_FORCE_RANGE);
+  if constexpr (WE) {
+    // energies and virial of the cluster's pairs (each pair counted once)
+    double r[8] = {(double)ec, (double)elj, (double)vxx, (double)vxy, (double)vxz,
+                   (double)vyy, (double)vyz, (double)vzz};
+    const int nr = vir ? 8 : 2;
+    for (int u = 0; u < nr; u++) {
+      double v = r[u];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+      r[u] = v;
+    }
+    if (lane == 0) {
+      double* ep = a.epart + (long long)cl * PM_NE;
+      ep[PM_E_COUL] = r[0];
+      ep[PM_E_LJ] = r[1];
+      for (int u = 2; u < PM_E_VIR; u++) ep[u] = 0.0;
+      if (vir) {
+        ep[PM_E_VIR + 0] = r[2]; ep[PM_E_VIR + 1] = r[3]; ep[PM_E_VIR + 2] = r[4];
+        ep[PM_E_VIR + 3] = r[3]; ep[PM_E_VIR + 4] = r[5]; ep[PM_E_VIR + 5] = r[6];
+        ep[PM_E_VIR + 6] = r[4]; ep[PM_E_VIR + 7] = r[6]; ep[PM_E_VIR + 8] = r[7];
+      } else {
+        for (int u = 0; u < 9; u++) ep[PM_E_VIR + u] = 0.0;
+      }
+    }
+  }
+}
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool PA>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1) void k_pair_jl(PmBox box, PmNbLaunch a, PmPairParams pp) {
+  pair_load_table<T, LJSWF>(a);
+  pair_cluster_jl<T, LJSWF, CK, CSWF, WE, M, PA>(box, a, pp, uni((int)(blockIdx.x * 4 + (threadIdx.x >> 6))));
+}
 struct PairArgs { PmBox box; PmNbLaunch a; PmPairParams pp; };
-template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M>
+// batched launches: JL selects the kernel layout (one lane per entry or a row per entry)
+template <typename T, int LJSWF, int CK, int CSWF, bool WE, int M, bool JL>
 __global__ __launch_bounds__(256) void kb_pair(const PairArgs* __restrict__ t, const int* __restrict__ pre, int nrep) {
   const int r = pm_batch_replica(pre, nrep, blockIdx.x);
   const PairArgs& q = t[r];
   pair_load_table<T, LJSWF>(q.a);
-  pair_cluster<T, LJSWF, CK, CSWF, WE, M, false>(q.box, q.a, q.pp, (blockIdx.x - pre[r]) * 4 + (threadIdx.x >> 6));
+  const int cl = (blockIdx.x - pre[r]) * 4 + (threadIdx.x >> 6);
+  if constexpr (JL) pair_cluster_jl<T, LJSWF, CK, CSWF, WE, M, false>(q.box, q.a, q.pp, uni(cl));
+  else pair_cluster<T, LJSWF, CK, CSWF, WE, M, false>(q.box, q.a, q.pp, cl);
 }
 
 
@@ -1731,8 +2065,9 @@ __global__ void k_diag(PmBox box, const double* __restrict__ pos, const int* __r
 // ----------------------------------------------------------------- launchers
 // pair-kernel variant of a record: precision, potentials and switching,
 // energy flag and cluster size
-static int pair_variant(bool dbl, int lj, int ck, int cs, bool we, int m) {
-  return (dbl ? 1 : 0) | (lj << 1) | (ck << 3) | (cs << 5) | ((we ? 1 : 0) << 7) | ((m == 4 ? 1 : 0) << 8);
+static int pair_variant(bool dbl, int lj, int ck, int cs, bool we, int m, bool jlanes) {
+  return (dbl ? 1 : 0) | (lj << 1) | (ck << 3) | (cs << 5) | ((we ? 1 : 0) << 7) | ((m == 4 ? 1 : 0) << 8) |
+         ((jlanes ? 1 : 0) << 9);
 }
 void pm_launch_bin(hipStream_t s, const PmBox& box, int n, const double* pos, int* cell_of,
                    int* cell_count, int* rank, float* key, const int* rebuild) {
@@ -1874,12 +2209,32 @@ static void launch_pair_t(hipStream_t s, const PmBox& box, const PmNbLaunch& a, 
   const dim3 gb((a.ncl_max + 3) / 4), b(256);
   const dim3 g = gb;
   const bool pa = LJ != PM_SWF_NONE && (sizeof(T) == 4 ? a.atom_lj_f != nullptr : a.atom_lj_d != nullptr);
+  // one lane per entry: the LJ table, then each wave's i-force reduction area
+  const size_t ljl = (lds + 15) / 16 * 16 + (size_t)4 * PM_JL_LDS;
   if (pm_recording()) {
     // per-atom LJ parameters have no batched kernel: the record is marked
     // unsupported and the batch runs its replicas as chains
     const PairArgs q{box, a, p};
-    pm_record(pa ? -1 : PM_REC_PAIR, pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m), gb.x, 256,
-              (unsigned)lds, &q, sizeof(q));
+    pm_record(pa ? -1 : PM_REC_PAIR,
+              pair_variant(sizeof(T) == 8, LJ, CK, CS, a.want_energy != 0, a.m, a.jlanes != 0), gb.x, 256,
+              (unsigned)(a.jlanes ? ljl : lds), &q, sizeof(q));
+    return;
+  }
+  if (a.jlanes) {
+    if constexpr (LJ != PM_SWF_NONE) {
+      if (pa) {
+        if (a.want_energy) hipLaunchKernelGGL((k_pair_jl<T, LJ, CK, CS, true, 8, true>), g, b, ljl, s, box, a, p);
+        else hipLaunchKernelGGL((k_pair_jl<T, LJ, CK, CS, false, 8, true>), g, b, ljl, s, box, a, p);
+        return;
+      }
+    }
+    if (a.m == 4) {
+      if (a.want_energy) hipLaunchKernelGGL((k_pair_jl<T, LJ, CK, CS, true, 4, false>), g, b, ljl, s, box, a, p);
+      else hipLaunchKernelGGL((k_pair_jl<T, LJ, CK, CS, false, 4, false>), g, b, ljl, s, box, a, p);
+    } else {
+      if (a.want_energy) hipLaunchKernelGGL((k_pair_jl<T, LJ, CK, CS, true, 8, false>), g, b, ljl, s, box, a, p);
+      else hipLaunchKernelGGL((k_pair_jl<T, LJ, CK, CS, false, 8, false>), g, b, ljl, s, box, a, p);
+    }
     return;
   }
   if constexpr (LJ != PM_SWF_NONE) {
@@ -1988,11 +2343,14 @@ static bool launch_pair_batched(hipStream_t s, int variant, const PairArgs* t, c
                                 unsigned lds) {
   const int lj = (variant >> 1) & 3, ck = (variant >> 3) & 3, cs = (variant >> 5) & 3;
   const bool we = (variant >> 7) & 1;
+  const bool jl = (variant >> 9) & 1;
   if ((variant >> 8) & 1) return false;   // 4-atom clusters
 #define PM_CASE(L, C, S) \
   if (lj == L && ck == C && (C == PM_COUL_NONE || cs == S)) { \
-    if (we) hipLaunchKernelGGL((kb_pair<T, L, C, S, true, 8>), g, dim3(256), lds, s, t, pre, nrep); \
-    else hipLaunchKernelGGL((kb_pair<T, L, C, S, false, 8>), g, dim3(256), lds, s, t, pre, nrep); \
+    if (jl && we) hipLaunchKernelGGL((kb_pair<T, L, C, S, true, 8, true>), g, dim3(256), lds, s, t, pre, nrep); \
+    else if (jl) hipLaunchKernelGGL((kb_pair<T, L, C, S, false, 8, true>), g, dim3(256), lds, s, t, pre, nrep); \
+    else if (we) hipLaunchKernelGGL((kb_pair<T, L, C, S, true, 8, false>), g, dim3(256), lds, s, t, pre, nrep); \
+    else hipLaunchKernelGGL((kb_pair<T, L, C, S, false, 8, false>), g, dim3(256), lds, s, t, pre, nrep); \
     return true; }
   PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C1)
   PM_CASE(PM_SWF_NONE, PM_COUL_PLAIN, PM_SWF_C2)

@@ -34,7 +34,7 @@ EXPORTS = ["pm_abi_version", "pm_last_error", "pm_create", "pm_destroy", "pm_set
            "pm_md_set_constraints", "pm_md_constraint_count", "pm_shake", "pm_rattle",
            "pm_batch_set_exchange_velocities", "pm_shake_noalloc"]
 OPTIONS = {"cluster_size": 0, "graph": 1, "list_skin": 2, "pme_fft": 3, "pme_priority": 4, "prune_skin": 5, "list_test": 6,
-           "pme_spread": 7}
+           "pme_spread": 7, "pair_lanes": 8}
 TIMING = ["bin", "list", "pair", "pme", "correction", "bonded", "reduce", "integrate"]
 
 

@@ -1,0 +1,119 @@
+"""The pair kernel with one lane per list entry (PM_OPT_PAIR_LANES 1, the
+default) against a row of one lane per cluster atom for each entry (0): the
+same pairs with the same arithmetic, summed in another order, so forces,
+energies and virials agree to the rounding of FP32 partial sums (mixed) or of
+FP64 sums (FP64), and both against the oracle.  Covered: every cutoff force
+variant, the exclusion rules with modified (1-4) pairs, clusters of 8 and 4,
+per-atom LJ parameters, list reuse and an MD run."""
+import sys
+
+import numpy as np
+import pytest
+
+from protomol_amd import water
+from tests.cases import ALPHA_RC9, exclusion_mix, rms_rel
+from tests.test_gpu_parity import VARIANTS, _variant_id
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def ctx(factory, s, precision, lanes, cluster=0, cell=4.0):
+    c = factory(precision)
+    c.set_topology(s)
+    c.set_boundary(s.box, s.origin, cell)
+    c.set_option("pair_lanes", lanes)
+    assert c.get_option("pair_lanes") == lanes
+    if cluster:
+        c.set_option("cluster_size", cluster)
+    return c
+
+
+TOL = {"mixed": 2e-6, "double": 1e-12}
+
+
+@pytest.mark.parametrize("fid,vals,terms", VARIANTS, ids=[_variant_id(v[0]) for v in VARIANTS])
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_lane_layouts_agree_on_variants(gpu_ctx_factory, port, fid, vals, terms, precision):
+    say(f"case: pair lanes {fid} {precision}")
+    s = water.named("water648", seed=2, jitter=0.4)
+    out = []
+    for lanes in (1, 0):
+        c = ctx(gpu_ctx_factory, s, precision, lanes)
+        out.append(c.evaluate([c.force(fid, vals)], s.pos, virial=True))
+    (f1, e1), (f0, e0) = out
+    o = port.cutoff(s, terms, vals[-1], 4.0)
+    say("rms rel force lanes 1 vs 0", rms_rel(f1, f0), "vs oracle", rms_rel(f1, o["forces"]))
+    assert rms_rel(f1, f0) < TOL[precision]
+    assert rms_rel(f1, o["forces"]) < (5e-6 if precision == "mixed" else 1e-11)
+    np.testing.assert_allclose(e1[:2], e0[:2], rtol=TOL[precision] * 5, atol=1e-9)
+    np.testing.assert_allclose(e1[7:], e0[7:], rtol=TOL[precision] * 50, atol=TOL[precision] * 50 * np.abs(e0[7:]).max())
+
+
+@pytest.mark.parametrize("cluster", [8, 4])
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_lane_layouts_agree_on_exclusion_rules(gpu_ctx_factory, port, cluster, precision):
+    """Modified pairs (A14/B14, csf 0.5), NONE overrides and cross-molecule
+    pairs: the modified pairs run in a pass of their own in the lane-per-entry
+    kernel."""
+    say(f"case: pair lanes exclusion rules cluster {cluster} {precision}")
+    s = exclusion_mix(water.named("water648", seed=8, jitter=0.4))
+    for fid, vals, terms in VARIANTS[5:7]:
+        out = []
+        for lanes in (1, 0):
+            c = ctx(gpu_ctx_factory, s, precision, lanes, cluster)
+            out.append(c.evaluate([c.force(fid, vals)], s.pos, virial=True))
+        (f1, e1), (f0, e0) = out
+        o = port.cutoff(s, terms, vals[-1], 4.0, virial=True)
+        say(fid, "rms rel force lanes 1 vs 0", rms_rel(f1, f0), "vs oracle", rms_rel(f1, o["forces"]))
+        assert rms_rel(f1, f0) < TOL[precision]
+        assert rms_rel(f1, o["forces"]) < (5e-6 if precision == "mixed" else 1e-11)
+        np.testing.assert_allclose(e1[:2], o["energies"][:2], rtol=(5e-5 if precision == "mixed" else 1e-10), atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_lane_layouts_agree_per_atom_lj(gpu_ctx_factory, precision):
+    """Per-atom LJ parameters (the CHARMM combination) in both layouts."""
+    say(f"case: pair lanes per-atom LJ {precision}")
+    s = water.named("water648", seed=4, jitter=0.4)
+    rng = np.random.default_rng(3)
+    sigma = np.where(s.type == 0, water.SIGMA_O, 0.4) * (1 + 0.02 * rng.standard_normal(s.n))
+    eps = np.where(s.type == 0, water.EPS_O, 0.046) * (1 + 0.02 * rng.standard_normal(s.n))
+    fid, vals = VARIANTS[5][0], VARIANTS[5][1]
+    out = []
+    for lanes in (1, 0):
+        c = ctx(gpu_ctx_factory, s, precision, lanes)
+        c.set_lj_per_atom(sigma, eps)
+        out.append(c.evaluate([c.force(fid, vals)], s.pos, virial=True))
+    (f1, e1), (f0, e0) = out
+    say("rms rel force lanes 1 vs 0", rms_rel(f1, f0))
+    assert rms_rel(f1, f0) < TOL[precision]
+    np.testing.assert_allclose(e1[:2], e0[:2], rtol=TOL[precision] * 5, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_lane_layouts_md_with_list_reuse(gpu_ctx_factory, precision):
+    """40 leapfrog steps of the 8,100-atom box (LJ-C2 + Ewald-real rc 9, PME
+    48^3, bonds and angles) with list reuse, in both layouts: FP64 follows
+    the same trajectory to accumulation order, mixed to FP32 rounding."""
+    say(f"case: md 40 steps water8k pair lanes 1 vs 0 {precision}")
+    s = water.named("water8k", seed=3)
+    out = {}
+    for lanes in (1, 0):
+        c = ctx(gpu_ctx_factory, s, precision, lanes, cell=4.5)
+        hs = [c.force("LennardJones CoulombEwaldReal -algorithm NonbondedCutoff -switchingFunction C2 "
+                      "-switchingFunction Cutoff", [8.0, 9.0, ALPHA_RC9, 9.0, 9.0]),
+              c.force("Coulomb -algorithm PMEwald -reciprocal -correction -interpolation BSpline",
+                      [48, 48, 48, 9.0, 4, 1e-6, -1.0]),
+              c.force("Bond", []), c.force("Angle", [])]
+        c.md_setup(hs, "leapfrog", dt=1.0)
+        c.md_set_state(s.pos, s.vel)
+        c.md_run(40)
+        out[lanes] = c.md_get_state()
+    dx = np.abs(out[1][0] - out[0][0]).max()
+    say("max |dx|", dx, "E", out[1][2][:2], out[0][2][:2])
+    assert dx < (1e-8 if precision == "double" else 1e-4)
