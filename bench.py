@@ -38,8 +38,8 @@ PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 FLOPS_PER_PAIR = 69       # LJ-C2 + Ewald-real per in-cutoff pair (algorithmic count)
 # HBM bytes per launch of the pair kernel at the default options: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (KiB per dispatch from rocprofv3 --pmc, scaled by what the
-# counters read for a 512 MB device copy), profiles/k_pair_water98k_hbm_traffic.md
-PAIR_TRAFFIC_BYTES = (21628.5 * 2.0 + 131140.1 * 1.0) * 1024
+# counters read for a 512 MB device copy), profiles/k_pair_jl_water98k_hbm_traffic.md
+PAIR_TRAFFIC_BYTES = (31699.6 * 2.0 + 151784.3 * 1.0) * 1024
 
 
 def build_system():
@@ -884,13 +884,14 @@ def main():
         "pme_hbm_frac": (pme_bytes / pme_avg_s) / PEAK_HBM,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 1400 and run["cluster"] == 8 and not args.opt
+                     "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 2200 and run["cluster"] == 8 and not args.opt
                                 else None,
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) "
-                                     "at the default options (list skin 1.4 A, exact list test, cluster 8); "
+                                     "at the default options (list skin 2.2 A, exact list test, cluster 8); "
                                      "null for other options",
-                     "kernel": "k_pair (LJ-C2 + Ewald-real, %d flops/pair)" % FLOPS_PER_PAIR,
-                     "note": "bound by FP32 vector ALU issue (the kernel issues no MFMA); "
+                     "kernel": "k_pair_jl (LJ-C2 + Ewald-real, %d flops/pair)" % FLOPS_PER_PAIR,
+                     "note": "FP32 vector arithmetic (the kernel issues no MFMA); its time is set by the "
+                             "64-bit fixed-point j-force atomics more than by VALU issue; "
                              "peak = MI355X vector FP32 157.3 TFLOP/s (counts packed FP32 dual issue, "
                              "which this build leaves off: the non-packed rate is 78.6)",
                      "frac_of_nonpacked_fp32": achieved / (PEAK_FP32 / 2)},

@@ -20,9 +20,12 @@
 #include <vector>
 
 // automatic pair-list skin (A) when PM_OPT_LIST_SKIN is -1 (98k water box,
-// 1 fs, exact list test: 318.5 ns/day at 1.4 A with 0.18 builds per step,
-// against 307.3 at 1.0 A and 314.0 at 1.2 A)
-#define PM_LIST_SKIN_AUTO 1.4
+// 1 fs, exact list test, pair kernel with a lane per list entry).  Mixed
+// precision: 0.114 builds per step and 386.9 ns/day at 2.2 A, against 384.1
+// at 1.8 A, 375.6 at 1.4 A (0.18 builds per step) and 384.5 at 3.0 A.  FP64,
+// where an entry costs more: 202 ns/day at 1.4 A, 189 at 2.2 A.
+#define PM_LIST_SKIN_AUTO_MIXED 2.2
+#define PM_LIST_SKIN_AUTO_DOUBLE 1.4
 // automatic inner-list skin (A) when PM_OPT_PRUNE_SKIN is -1: none (on the
 // 98k water box at 1 fs the fastest hydrogens cross half of a 0.2-0.4 A
 // inner skin almost every step, and the prune pass costs more than the
@@ -491,7 +494,7 @@ double prune_skin(const pm_context* c) {
 // list skin in A: the option, or the automatic choice
 double list_skin(const pm_context* c) {
   if (c->skin_milli >= 0) return 1e-3 * c->skin_milli;
-  return PM_LIST_SKIN_AUTO;
+  return c->precision == PM_PRECISION_DOUBLE ? PM_LIST_SKIN_AUTO_DOUBLE : PM_LIST_SKIN_AUTO_MIXED;
 }
 
 int ensure_cells(pm_context* c) {
@@ -1516,7 +1519,7 @@ int list_stats(pm_context* c) {
   int counts[4] = {0, 0, 0, 0};
   if (c->d_stats && c->d_jcount && c->d_counts) {
     const int* len = c->inner_last && c->d_icount ? c->d_icount : c->d_jcount;
-    pm_launch_list_stats(c->stream, len, c->d_counts, c->d_stats, c->m);
+    pm_launch_list_stats(c->stream, len, c->d_counts, c->d_stats, c->m, c->pair_lanes_option);
     HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));

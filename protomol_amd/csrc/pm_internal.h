@@ -209,7 +209,8 @@ void pm_launch_fold_slots(hipStream_t s, int nslot_max, const int* counts, const
                           long long* force_slot, long long* force_fix, int* flag,
                           unsigned long long* builds, int clear);
 // entries and issued lanes of the lists whose lengths are `jcount`
-void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m);
+void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m,
+                          int jlanes);
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p);
 // inner lists from the full ones (a.inner set; returns at once unless a flag is set)
 void pm_launch_prune(hipStream_t s, const PmBox& box, const PmNbLaunch& a);

@@ -793,9 +793,12 @@ __global__ __launch_bounds__(256) void kb_list(const ListArgs* __restrict__ t, c
 }
 
 
-// list statistics: total entries and issued lanes over all clusters
+// list statistics: total entries and issued lanes (pair evaluations, masked
+// ones included) over all clusters: a pair kernel iteration takes jw entries
+// and issues 64 * mult lanes (rows of M lanes: jw = 64 / M, mult 1; a lane
+// per entry: jw = 64, mult M)
 __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcount, const int* __restrict__ counts,
-                                                    unsigned long long* __restrict__ out, int jw) {
+                                                    unsigned long long* __restrict__ out, int jw, int mult) {
   __shared__ unsigned long long s0[256], s1[256];
   const int ncl = counts[1];
   unsigned long long e = 0, l = 0;
@@ -809,7 +812,7 @@ __global__ __launch_bounds__(256) void k_list_stats(const int* __restrict__ jcou
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       e += v[k];
-      l += (unsigned long long)((v[k] + jw - 1) / jw) * 64;
+      l += (unsigned long long)((v[k] + jw - 1) / jw) * 64 * mult;
     }
   }
   s0[threadIdx.x] = e;
@@ -2179,8 +2182,10 @@ void pm_launch_prune(hipStream_t s, const PmBox& box, const PmNbLaunch& a) {
   else hipLaunchKernelGGL(k_prune<8>, g, b, 0, s, box, a);
 }
 
-void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m) {
-  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out, 64 / m);
+void pm_launch_list_stats(hipStream_t s, const int* jcount, const int* counts, unsigned long long* out, int m,
+                          int jlanes) {
+  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, jcount, counts, out, jlanes ? 64 : 64 / m,
+                     jlanes ? m : 1);
 }
 
 void pm_launch_list(hipStream_t s, const PmBox& box, const PmNbLaunch& a, const PmPairParams& p) {
