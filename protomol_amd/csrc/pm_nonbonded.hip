@@ -1525,6 +1525,7 @@ __device__ inline void pair_cluster_jl(const PmBox& box, const PmNbLaunch& a, co
       const T r2 = dx * dx + dy * dy + dz * dz;
       const bool mask = valid && sj > c0 + ii && (r2 <= cut2) &&
                         (MOD ? (xm & (65536u << ii)) != 0u : (xm & (65537u << ii)) == 0u);
+      if (__ballot(mask) == 0ull) return;   // no lane's pair with atom ii counts
       T rr2, rinv;
       if constexpr (sizeof(T) == 4) { rinv = Prec<T>::rsq(r2); rr2 = rinv * rinv; }
       else { rr2 = T(1) / r2; rinv = sqrt(rr2); }
