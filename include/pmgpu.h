@@ -359,7 +359,9 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        cluster's atoms in scalar registers, the entry's j
  *                        force whole in its lane), 0 a row of one lane per
  *                        cluster atom for each entry (j forces summed over
- *                        the row).
+ *                        the row).  The lane-per-entry kernel needs the LJ
+ *                        type table plus 8 KB of LDS per workgroup: FP64
+ *                        contexts with over 41 atom types use rows.
  * pm_get_option returns the value in effect (after the automatic choice). */
 enum { PM_OPT_CLUSTER_SIZE = 0, PM_OPT_GRAPH = 1, PM_OPT_LIST_SKIN = 2, PM_OPT_PME_FFT = 3, PM_OPT_PME_PRIORITY = 4,
        PM_OPT_PRUNE_SKIN = 5, PM_OPT_LIST_TEST = 6, PM_OPT_PME_SPREAD = 7, PM_OPT_PAIR_LANES = 8 };

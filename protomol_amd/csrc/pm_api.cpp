@@ -491,6 +491,14 @@ double prune_skin(const pm_context* c) {
   return PM_PRUNE_SKIN_AUTO;
 }
 
+// pair kernel layout in effect: the lane-per-entry kernel (1) unless the
+// option asks for rows (0) or its LDS (type table + four waves' areas) would
+// not fit a workgroup (FP64 with over 41 atom types)
+static int pair_lanes(const pm_context* c) {
+  const size_t tab = (size_t)(c->ntypes + 1) * (c->ntypes + 1) * (c->precision == PM_PRECISION_DOUBLE ? 32 : 16);
+  return c->pair_lanes_option && (tab + 15) / 16 * 16 + 4 * PM_JL_LDS <= (size_t)c->lds_max ? 1 : 0;
+}
+
 // list skin in A: the option, or the automatic choice
 double list_skin(const pm_context* c) {
   if (c->skin_milli >= 0) return 1e-3 * c->skin_milli;
@@ -1229,7 +1237,7 @@ int evaluate_group(pm_context* c, const std::vector<int>& handles, int flags, bo
     a.xs_part = c->d_xs_part;
     a.xs_tcap = c->m * c->xs_maxdeg;
     a.m = c->m;
-    a.jlanes = c->pair_lanes_option;
+    a.jlanes = pair_lanes(c);
     a.molecule = c->d_mol;
     a.ex_row = c->d_ex_row;
     a.ex_partner = c->d_ex_partner;
@@ -1519,7 +1527,7 @@ int list_stats(pm_context* c) {
   int counts[4] = {0, 0, 0, 0};
   if (c->d_stats && c->d_jcount && c->d_counts) {
     const int* len = c->inner_last && c->d_icount ? c->d_icount : c->d_jcount;
-    pm_launch_list_stats(c->stream, len, c->d_counts, c->d_stats, c->m, c->pair_lanes_option);
+    pm_launch_list_stats(c->stream, len, c->d_counts, c->d_stats, c->m, pair_lanes(c));
     HIPCHECK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -3103,7 +3111,7 @@ int pm_get_option(pm_context* c, int option, int* value) {
     return PM_OK;
   }
   if (option == PM_OPT_PAIR_LANES) {
-    *value = c->pair_lanes_option;
+    *value = pair_lanes(c);
     return PM_OK;
   }
   if (option == PM_OPT_PME_FFT) {

@@ -48,6 +48,10 @@
 #define PM_E_IMPROPER 5
 #define PM_E_VIR 7
 #define PM_NE 16
+// LDS bytes per wave of the lane-per-entry pair kernel (k_pair_jl), after
+// the block's LJ type table: small, so that the kernel leaves most of a CU's
+// LDS to the PME kernels that run beside it on the other stream
+#define PM_JL_LDS 2048
 
 // Error flags the kernels raise in the context's flag word (atomicOr); the
 // host reads and clears the word after each evaluation or MD run and turns a

@@ -1395,11 +1395,9 @@ __device__ inline long long uni(long long v) {
 }
 __device__ inline double uni(double v) { return __longlong_as_double(uni(__double_as_longlong(v))); }
 
-// LDS bytes per wave of pair_cluster_jl: an iteration's j forces (3 x 64
+// PM_JL_LDS (pm_internal.h) bytes per wave: an iteration's j forces (3 x 64
 // values and 64 slots), later the i-force reduction, PM_JL_NV(T) values of
-// the 64 lanes per pass.  2 KB, so that the kernel leaves most of a CU's LDS
-// to the PME kernels that run beside it on the other stream.
-#define PM_JL_LDS 2048
+// the 64 lanes per pass.
 #define PM_JL_NV(T) (PM_JL_LDS / 64 / (int)sizeof(T))
 
 // Half-list pair kernel with one lane per list entry: one wave per i-cluster

@@ -117,3 +117,32 @@ def test_lane_layouts_md_with_list_reuse(gpu_ctx_factory, precision):
     dx = np.abs(out[1][0] - out[0][0]).max()
     say("max |dx|", dx, "E", out[1][2][:2], out[0][2][:2])
     assert dx < (1e-8 if precision == "double" else 1e-4)
+
+
+def test_many_types_layout_choice(gpu_ctx_factory, port):
+    """44 atom types (the topology limit): the LJ type table in LDS beside the
+    lane-per-entry kernel's 8 KB (FP64: 45 x 45 entries of 32 bytes, 73 KB
+    in all, within a gfx950 workgroup's LDS; a device with 64 KB per
+    workgroup would run that case with rows).  Both precisions against the
+    oracle."""
+    say("case: 44 atom types water648, layout choice per precision")
+    s = water.named("water648", seed=6, jitter=0.3)
+    s = s.copy()
+    rng = np.random.default_rng(7)
+    se = [(water.SIGMA_O * (1 + 0.05 * rng.standard_normal()), water.EPS_O * (1 + 0.2 * rng.random()))
+          for _ in range(22)] + [(0.4 * (1 + 0.05 * rng.standard_normal()), 0.046) for _ in range(22)]
+    A, B = water._lj_tables(se)
+    s.ljA, s.ljB, s.ljA14, s.ljB14 = A, B, A.copy(), B.copy()
+    s.type = np.where(s.type == 0, rng.integers(0, 22, s.n), rng.integers(22, 44, s.n)).astype(np.int32)
+    assert s.ntypes == 44
+    fid, vals, terms = VARIANTS[5]
+    o = port.cutoff(s, terms, vals[-1], 4.0)
+    for precision, tol in (("mixed", 5e-6), ("double", 1e-11)):
+        c = gpu_ctx_factory(precision)
+        c.set_topology(s)
+        c.set_boundary(s.box, s.origin, 4.0)
+        c.set_option("pair_lanes", 1)
+        f, e = c.evaluate([c.force(fid, vals)], s.pos)
+        say(precision, "pair_lanes in effect", c.get_option("pair_lanes"), "rms rel", rms_rel(f, o["forces"]))
+        assert c.get_option("pair_lanes") in ((1,) if precision == "mixed" else (0, 1))
+        assert rms_rel(f, o["forces"]) < tol
