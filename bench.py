@@ -32,14 +32,18 @@ RC = 10.0
 SWITCHON = 9.0
 GRID = (96, 96, 96)
 ORDER = 4
-CELL = 5.0
+# cell size of the workload's NonbondedCutoff force (the reference path's
+# cellsize parameter; pairs and forces do not depend on it): 4.0 A, 24 cells
+# of 4.14 A per side of the 99.3 A box (cell 3.0 / 3.6 / 4.0 / 4.5 / 5.0 /
+# 6.5 A: 382 / 390 / 400-401 / 395 / 387 / 373 ns/day)
+CELL = 4.0
 PEAK_FP32 = 157.3e12      # MI355X vector FP32, FLOP/s
 PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 FLOPS_PER_PAIR = 69       # LJ-C2 + Ewald-real per in-cutoff pair (algorithmic count)
 # HBM bytes per launch of the pair kernel at the default options: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (KiB per dispatch from rocprofv3 --pmc, scaled by what the
 # counters read for a 512 MB device copy), profiles/k_pair_jl_water98k_hbm_traffic.md
-PAIR_TRAFFIC_BYTES = (31699.6 * 2.0 + 151784.3 * 1.0) * 1024
+PAIR_TRAFFIC_BYTES = (28791.9 * 2.0 + 137466.0 * 1.0) * 1024
 
 
 def build_system():
@@ -887,7 +891,7 @@ def main():
                      "traffic": PAIR_TRAFFIC_BYTES if run["skin_milli"] == 2200 and run["cluster"] == 8 and not args.opt
                                 else None,
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) "
-                                     "at the default options (list skin 2.2 A, exact list test, cluster 8); "
+                                     "at the default options (cell 4.0 A, list skin 2.2 A, exact list test, cluster 8); "
                                      "null for other options",
                      "kernel": "k_pair_jl (LJ-C2 + Ewald-real, %d flops/pair)" % FLOPS_PER_PAIR,
                      "note": "FP32 vector arithmetic (the kernel issues no MFMA); its time is set by the "
