@@ -309,7 +309,7 @@ int pm_lanczos(pm_context* ctx, const int* handles, int nhandles, const double* 
  *                        Steps are captured only while no kernel timing is
  *                        requested (pm_set_timing mask 0).
  *   PM_OPT_LIST_SKIN     pair-list buffer in 1e-3 A: -1 (default) automatic
- *                        (2200 mixed, 1400 FP64), 0 off (cells and lists rebuilt at every
+ *                        (2200 mixed, 1000 FP64), 0 off (cells and lists rebuilt at every
  *                        evaluation, as the reference rebuilds its cell list,
  *                        ForceGroup.cpp:50), 1..3000.  Lists then hold the j
  *                        atoms within the pair cutoff + skin and are rebuilt

@@ -23,9 +23,10 @@
 // 1 fs, exact list test, pair kernel with a lane per list entry).  Mixed
 // precision: 0.114 builds per step and 386.9 ns/day at 2.2 A, against 384.1
 // at 1.8 A, 375.6 at 1.4 A (0.18 builds per step) and 384.5 at 3.0 A.  FP64,
-// where an entry costs more: 202 ns/day at 1.4 A, 189 at 2.2 A.
+// where an entry costs more (cell 4.0 A, 200 steps): 218.6 / 220.5 / 221.5 /
+// 213.6 / 207.9 / 200.5 ns/day at 0.6 / 0.8 / 1.0 / 1.4 / 1.8 / 2.2 A.
 #define PM_LIST_SKIN_AUTO_MIXED 2.2
-#define PM_LIST_SKIN_AUTO_DOUBLE 1.4
+#define PM_LIST_SKIN_AUTO_DOUBLE 1.0
 // automatic inner-list skin (A) when PM_OPT_PRUNE_SKIN is -1: none (on the
 // 98k water box at 1 fs the fastest hydrogens cross half of a 0.2-0.4 A
 // inner skin almost every step, and the prune pass costs more than the
